@@ -1,0 +1,122 @@
+"""k-means (python_interface/python_package/aoclda/clustering.py:36-280 of the reference)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class kmeans:
+    def __init__(self, n_clusters=1, initialization_method="k-means++", C=None, n_init=10,
+                 max_iter=300, seed=-1, algorithm="lloyd", distance="euclidean",
+                 normalize_data=False, tol=1.0e-4, empty_clusters="ignore",
+                 afk_mcmc_samples=50, mixed_precision=False, low_precision_max_iter=200,
+                 low_precision_tol=1.0e-2, check_data=False):
+        self.n_clusters_opt = n_clusters
+        self.initialization_method = initialization_method
+        self.C = C
+        self.n_init, self.max_iter, self.seed = n_init, max_iter, seed
+        self.algorithm, self.distance, self.normalize_data = algorithm, distance, normalize_data
+        self.tol, self.empty_clusters = tol, empty_clusters
+        self.afk_mcmc_samples = afk_mcmc_samples
+        self.mixed_precision = mixed_precision
+        self.low_precision_max_iter, self.low_precision_tol = low_precision_max_iter, low_precision_tol
+        self.check_data = check_data
+        self.status = None
+        self._h = None
+
+    def fit(self, A):
+        A, self.order, self.dtype = L.prepare(A)
+        h = L.Handle(L.HANDLE_KMEANS, self.dtype)
+        self._h = h
+        init = self.initialization_method
+        if self.C is not None:
+            init = "supplied"
+        h.set_string("initialization method", init)
+        h.set_string("algorithm", self.algorithm)
+        h.set_string("storage order", "column-major" if self.order == L.COLUMN_MAJOR else "row-major")
+        h.set_string("empty clusters", self.empty_clusters)
+        h.set_string("distance", self.distance)
+        h.set_string("normalize data", "yes" if self.normalize_data else "no")
+        h.set_string("mixed precision", "yes" if self.mixed_precision else "no")
+        h.set_string("check data", "yes" if self.check_data else "no")
+        h.set_int("n_clusters", self.n_clusters_opt)
+        h.set_int("n_init", self.n_init)
+        h.set_int("max_iter", self.max_iter)
+        h.set_int("seed", self.seed)
+        h.set_int("afk-mc2 samples", self.afk_mcmc_samples)
+        h.set_int("low precision max_iter", self.low_precision_max_iter)
+        h.set_real("convergence tolerance", self.tol)
+        h.set_real("low precision convergence tolerance", self.low_precision_tol)
+        n, d = A.shape
+        self._A = A  # the C API keeps the caller's pointer semantics
+        ld = n if self.order == L.COLUMN_MAJOR else d
+        h.call("da_kmeans_set_data", n, d, L.ptr(A), ld, allow=(L.CODE["incompatible_options"],))
+        if self.C is not None:
+            Cm, _, _ = L.prepare(self.C, dtype=self.dtype, order=self.order)
+            self._C = Cm
+            ldc = Cm.shape[0] if self.order == L.COLUMN_MAJOR else d
+            h.call("da_kmeans_set_init_centres", L.ptr(Cm), ldc)
+        self.status = h.call("da_kmeans_compute",
+                             allow=(L.CODE["maxit"], L.CODE["incompatible_options"]))
+        self.n_samples, self.n_features = n, d
+        return self
+
+    def _rinfo(self):
+        r = np.zeros(6, dtype=self.dtype)
+        dim = L.da_int(6)
+        self._h.call("da_handle_get_result", L.RINFO, C.byref(dim), L.ptr(r))
+        return r
+
+    @property
+    def cluster_centres(self):
+        k, d = self.n_clusters, self.n_features
+        r = np.zeros(k * d, dtype=self.dtype)
+        dim = L.da_int(k * d)
+        self._h.call("da_handle_get_result", L.KMEANS_CENTRES, C.byref(dim), L.ptr(r))
+        if self.order == L.COLUMN_MAJOR:
+            return r.reshape((k, d), order="F")
+        return r.reshape((k, d))
+
+    @property
+    def labels(self):
+        n = self.n_samples
+        r = np.zeros(n, dtype=np.int32)
+        dim = L.da_int(n)
+        L.check(L.lib.da_handle_get_result_int(self._h.h, L.KMEANS_LABELS, C.byref(dim), L.ptr(r)),
+                self._h.h)
+        return r
+
+    @property
+    def inertia(self):
+        return float(self._rinfo()[4])
+
+    @property
+    def n_iter(self):
+        return int(self._rinfo()[3])
+
+    @property
+    def lp_n_iter(self):
+        return int(self._rinfo()[5])
+
+    @property
+    def n_clusters(self):
+        return int(self._rinfo()[2])
+
+    def transform(self, X):
+        X, _, _ = L.prepare(X, dtype=self.dtype, order=self.order)
+        m, d = X.shape
+        k = self.n_clusters
+        out = np.empty((m, k), dtype=self.dtype, order="F" if self.order == L.COLUMN_MAJOR else "C")
+        ldx = m if self.order == L.COLUMN_MAJOR else d
+        ldo = m if self.order == L.COLUMN_MAJOR else k
+        self._h.call("da_kmeans_transform", m, d, L.ptr(X), ldx, L.ptr(out), ldo)
+        return out
+
+    def predict(self, Y):
+        Y, _, _ = L.prepare(Y, dtype=self.dtype, order=self.order)
+        m, d = Y.shape
+        out = np.empty(m, dtype=np.int32)
+        ld = m if self.order == L.COLUMN_MAJOR else d
+        self._h.call("da_kmeans_predict", m, d, L.ptr(Y), ld, L.ptr(out))
+        return out

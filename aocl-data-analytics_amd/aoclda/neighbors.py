@@ -1,0 +1,59 @@
+"""k-nearest neighbours (python_interface/python_package/aoclda/neighbors.py:39-233 of the
+reference): brute force on the GPU with the training set resident in GPU memory."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class nearest_neighbors:
+    def __init__(self, n_neighbors=5, weights="uniform", algorithm="auto", leaf_size=30,
+                 metric="euclidean", p=2.0, radius=1.0, outlier_handling=None,
+                 check_data=False):
+        if not isinstance(n_neighbors, int) and n_neighbors is not None:
+            raise ValueError("Number of neighbors must be an integer or None. Got n_neighbors="
+                             + str(n_neighbors))
+        self.n_neighbors = n_neighbors
+        self.weights, self.algorithm, self.leaf_size = weights, algorithm, leaf_size
+        self.metric, self.p, self.radius = metric, p, radius
+        self.check_data = check_data
+        self._h = None
+        self.order = None
+        self.dtype = None
+
+    def _options(self, h):
+        h.set_int("number of neighbors", self.n_neighbors if self.n_neighbors else 1)
+        h.set_string("weights", self.weights)
+        h.set_string("algorithm", self.algorithm)
+        h.set_int("leaf size", self.leaf_size)
+        h.set_string("metric", "inner product" if self.metric == "inner_product" else self.metric)
+        h.set_real("minkowski parameter", self.p)
+        h.set_string("storage order", "column-major" if self.order == L.COLUMN_MAJOR else "row-major")
+        h.set_string("check data", "yes" if self.check_data else "no")
+
+    def fit(self, X, y=None):
+        X, self.order, self.dtype = L.prepare(X)
+        self._h = L.Handle(L.HANDLE_NN, self.dtype)
+        self._options(self._h)
+        n, d = X.shape
+        ld = n if self.order == L.COLUMN_MAJOR else d
+        self._h.call("da_nn_set_data", n, d, L.ptr(X), ld)
+        self.n_samples, self.n_features = n, d
+        return self
+
+    def kneighbors(self, X, n_neighbors=None, return_distance=True):
+        X, _, _ = L.prepare(X, dtype=self.dtype, order=self.order)
+        nq, d = X.shape
+        k = n_neighbors if n_neighbors is not None else (self.n_neighbors or 0)
+        kk = k if k > 0 else 5
+        ind = np.empty((nq, kk), dtype=np.int32, order="F" if self.order == L.COLUMN_MAJOR else "C")
+        dist = np.empty((nq, kk), dtype=self.dtype, order=ind.flags.f_contiguous and "F" or "C")
+        ld = nq if self.order == L.COLUMN_MAJOR else d
+        self._h.call("da_nn_kneighbors", nq, d, L.ptr(X), ld, L.ptr(ind),
+                     L.ptr(dist) if return_distance else None, k, 1 if return_distance else 0)
+        return (dist, ind) if return_distance else ind
+
+    @property
+    def handle(self):
+        return self._h

@@ -1,0 +1,682 @@
+// k-means kernels for gfx950: exact assignment, deterministic centroid update, inertia,
+// transform and the k-means++ sampling helpers.
+//
+// Assignment follows lloyd_iteration_kernel (source/core/clustering/kmeans/
+// kmeans_kernels.cpp:209-680 of the reference): argmin over centres with ties to the
+// lowest centre index (the scalar kernel's rule), but on exact double-precision squared
+// distances instead of the GEMM form.
+//
+// Centroid sums (kmeans_lloyd.hpp:274-361) are accumulated in 64-bit fixed point: every
+// value of feature l is scaled by 2^e[l] and split into an integer part and a remainder
+// scaled by a further 2^lshift. Integer addition is associative, so the sums are
+// identical whatever the order in which points are visited or workgroups are scheduled,
+// and the totals are exact to about 2^-100 of the data range. The exponents are chosen on
+// the host from the column maxima so that no partial or total can overflow.
+#include "launch.hpp"
+#include <climits>
+#include <hip/hip_runtime.h>
+
+namespace dagpu {
+
+namespace {
+
+constexpr int CS_ELEMS = 4096; // centres staged in LDS per chunk (doubles)
+
+template <typename T>
+__global__ void col_absmax_kernel(const T *__restrict__ X, int n, int d, double *out) {
+    const int64_t total = (int64_t)n * d;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int l = (int)(e % d);
+        double v = fabs((double)X[e]);
+        if (v == v && v > 0.0)
+            atomicMax(reinterpret_cast<unsigned long long *>(out + l),
+                      (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+template <typename T, int DMAX>
+__global__ __launch_bounds__(256) void kmeans_assign_kernel(
+    const T *__restrict__ X, int n, int d, const double *__restrict__ C, int k,
+    int *__restrict__ labels, const int *__restrict__ prev, int *__restrict__ changed,
+    double *__restrict__ mind) {
+    __shared__ double Cs[CS_ELEMS];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    double x[DMAX > 0 ? DMAX : 1];
+    if constexpr (DMAX > 0) {
+#pragma unroll
+        for (int l = 0; l < DMAX; l++)
+            x[l] = (valid && l < d) ? (double)X[(int64_t)i * d + l] : 0.0;
+    }
+    double best = __builtin_huge_val();
+    int bl = 0;
+    const int cc = CS_ELEMS / d > 0 ? CS_ELEMS / d : 1;
+    for (int c0 = 0; c0 < k; c0 += cc) {
+        const int nc = (k - c0) < cc ? (k - c0) : cc;
+        __syncthreads();
+        for (int t = threadIdx.x; t < nc * d; t += 256)
+            Cs[t] = C[(int64_t)c0 * d + t];
+        __syncthreads();
+        if (!valid)
+            continue;
+        for (int c = 0; c < nc; c++) {
+            double acc = 0.0;
+            if constexpr (DMAX > 0) {
+#pragma unroll
+                for (int l = 0; l < DMAX; l++) {
+                    if (l < d) {
+                        const double t = x[l] - Cs[c * d + l];
+                        acc = fma(t, t, acc);
+                    }
+                }
+            } else {
+                for (int l = 0; l < d; l++) {
+                    const double t = (double)X[(int64_t)i * d + l] - Cs[c * d + l];
+                    acc = fma(t, t, acc);
+                }
+            }
+            if (acc < best) {
+                best = acc;
+                bl = c0 + c;
+            }
+        }
+    }
+    if (valid) {
+        labels[i] = bl;
+        if (mind)
+            mind[i] = best;
+        if (prev && prev[i] != bl)
+            atomicAdd(changed, 1);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kmeans_partial_kernel(
+    const T *__restrict__ X, int n, int d, const int *__restrict__ labels, int c0, int kc,
+    const int *__restrict__ escale, int ppb, long long *__restrict__ part,
+    int *__restrict__ cnt) {
+    extern __shared__ long long acc[];
+    const int words = kc * d * 2;
+    int *lcnt = reinterpret_cast<int *>(acc + words);
+    for (int t = threadIdx.x; t < words; t += 256)
+        acc[t] = 0;
+    for (int t = threadIdx.x; t < kc; t += 256)
+        lcnt[t] = 0;
+    __syncthreads();
+    const int b = blockIdx.x;
+    const int64_t start = (int64_t)b * ppb;
+    int64_t end = start + ppb;
+    if (end > n)
+        end = n;
+    const int lshift = escale[d];
+    for (int64_t e = start * d + threadIdx.x; e < end * d; e += 256) {
+        const int64_t i = e / d;
+        const int l = (int)(e - i * d);
+        const int lab = labels[i] - c0;
+        if (lab < 0 || lab >= kc)
+            continue;
+        const double v = (double)X[e];
+        if (v == v && v - v == 0.0) {
+            const double sv = ldexp(v, escale[l]);
+            const double h = rint(sv);
+            const long long H = (long long)h;
+            const long long L = (long long)rint(ldexp(sv - h, lshift));
+            atomicAdd(reinterpret_cast<unsigned long long *>(&acc[(lab * d + l) * 2]),
+                      (unsigned long long)H);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&acc[(lab * d + l) * 2 + 1]),
+                      (unsigned long long)L);
+        }
+        if (l == 0)
+            atomicAdd(&lcnt[lab], 1);
+    }
+    __syncthreads();
+    long long *pb = part + (int64_t)b * words;
+    for (int t = threadIdx.x; t < words; t += 256)
+        pb[t] = acc[t];
+    for (int t = threadIdx.x; t < kc; t += 256)
+        cnt[(int64_t)b * kc + t] = lcnt[t];
+}
+
+__global__ void kmeans_reduce_kernel(const long long *__restrict__ part,
+                                     const int *__restrict__ cnt, int nblocks, int c0, int kc,
+                                     int d, const int *__restrict__ escale,
+                                     double *__restrict__ C, int *__restrict__ counts) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kc * d)
+        return;
+    const int c = e / d, l = e - c * d;
+    const int words = kc * d * 2;
+    long long H = 0, L = 0;
+    int count = 0;
+    for (int b = 0; b < nblocks; b++) {
+        H += part[(int64_t)b * words + 2 * e];
+        L += part[(int64_t)b * words + 2 * e + 1];
+        count += cnt[(int64_t)b * kc + c];
+    }
+    const int ex = escale[l], lshift = escale[d];
+    const double sum = ldexp((double)H, -ex) + ldexp((double)L, -(ex + lshift));
+    // An empty cluster keeps count 1 and a zero sum, i.e. the origin
+    // (scale_current_cluster_centres, kmeans_lloyd.hpp:363-373).
+    C[(int64_t)(c0 + c) * d + l] = sum / (double)(count > 0 ? count : 1);
+    if (l == 0)
+        counts[c0 + c] = count;
+}
+
+__global__ __launch_bounds__(1024) void kmeans_shift_kernel(const double *__restrict__ a,
+                                                            const double *__restrict__ b,
+                                                            int len, double *out) {
+    __shared__ double s0[1024], s1[1024];
+    double x0 = 0.0, x1 = 0.0;
+    for (int t = threadIdx.x; t < len; t += 1024) {
+        const double va = a[t];
+        const double dv = va - b[t];
+        x0 = fma(va, va, x0);
+        x1 = fma(dv, dv, x1);
+    }
+    s0[threadIdx.x] = x0;
+    s1[threadIdx.x] = x1;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s0[0];
+        out[1] = s1[0];
+    }
+}
+
+// Block sums over a fixed partition: block b sums elements b, b+G, b+2G... per thread.
+__global__ __launch_bounds__(256) void sum_partial_kernel(const double *__restrict__ v, int n,
+                                                          double *tmp) {
+    __shared__ double s[256];
+    double x = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+        x += v[t];
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        tmp[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void sum_final_kernel(const double *__restrict__ tmp, int nb,
+                                                        double *out) {
+    __shared__ double s[256];
+    double x = 0.0;
+    for (int t = threadIdx.x; t < nb; t += 256)
+        x += tmp[t];
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[0] = s[0];
+}
+
+template <typename T>
+__global__ void convert_kernel(const T *__restrict__ src, int64_t n, double *__restrict__ dst) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = (double)src[e];
+}
+template <typename T>
+__global__ void convert_back_kernel(const double *__restrict__ src, int64_t n, T *__restrict__ dst) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x)
+        dst[e] = (T)src[e];
+}
+template <typename T>
+__global__ void gather_rows_kernel(const T *__restrict__ X, int d, const int *__restrict__ idx,
+                                   int m, double *__restrict__ C) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)m * d)
+        return;
+    const int64_t r = e / d, l = e - r * d;
+    C[e] = (double)X[(int64_t)idx[r] * d + l];
+}
+
+template <typename T>
+__device__ __forceinline__ double sqdist_rows(const T *__restrict__ X, int d, int64_t i,
+                                              const double *__restrict__ cs) {
+    double acc = 0.0;
+    for (int l = 0; l < d; l++) {
+        const double t = (double)X[i * d + l] - cs[l];
+        acc = fma(t, t, acc);
+    }
+    return acc;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void dist_to_row_kernel(const T *__restrict__ X, int n, int d,
+                                                          int ci, double *__restrict__ w,
+                                                          int mode) {
+    extern __shared__ double cs[];
+    for (int t = threadIdx.x; t < d; t += 256)
+        cs[t] = (double)X[(int64_t)ci * d + t];
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    double v = sqdist_rows(X, d, i, cs);
+    if (mode == 1) {
+        // min update: max(0, min(w, v))
+        const double o = w[i];
+        v = o < v ? o : v;
+        v = v > 0.0 ? v : 0.0;
+    } else {
+        if (!(v - v == 0.0) || v < 0.0)
+            v = 0.0;
+    }
+    if (i == ci)
+        v = 0.0;
+    w[i] = v;
+}
+
+constexpr int SCAN_BLOCKS = 1024;
+
+// Phase 1: per-block sums of w/total over a contiguous chunk.
+__global__ __launch_bounds__(256) void scan_blocksum_kernel(const double *__restrict__ w, int n,
+                                                            const double *__restrict__ total,
+                                                            double *__restrict__ bsum, int chunk) {
+    __shared__ double s[256];
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const double tot = total[0];
+    double x = 0.0;
+    // Thread t handles the contiguous sub-chunk [b0 + t*sub, b0 + (t+1)*sub).
+    const int sub = (chunk + 255) / 256;
+    for (int j = 0; j < sub; j++) {
+        const int64_t e = b0 + (int64_t)threadIdx.x * sub + j;
+        if (e < n && e < b0 + chunk)
+            x += w[e] / tot;
+    }
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m)
+            s[threadIdx.x] += s[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        bsum[blockIdx.x] = s[0];
+}
+
+// Phase 2: exclusive scan of the block sums (sequential, fixed order).
+__global__ void scan_blockoffsets_kernel(double *bsum, int nb) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double run = 0.0;
+        for (int b = 0; b < nb; b++) {
+            const double v = bsum[b];
+            bsum[b] = run;
+            run += v;
+        }
+    }
+}
+
+// Phase 3: each block scans its chunk with its offset.
+__global__ __launch_bounds__(256) void scan_final_kernel(const double *__restrict__ w, int n,
+                                                         const double *__restrict__ total,
+                                                         const double *__restrict__ boff,
+                                                         double *__restrict__ cp, int chunk) {
+    __shared__ double s[256];
+    const int64_t b0 = (int64_t)blockIdx.x * chunk;
+    const double tot = total[0];
+    const int sub = (chunk + 255) / 256;
+    double x = 0.0;
+    for (int j = 0; j < sub; j++) {
+        const int64_t e = b0 + (int64_t)threadIdx.x * sub + j;
+        if (e < n && e < b0 + chunk)
+            x += w[e] / tot;
+    }
+    s[threadIdx.x] = x;
+    __syncthreads();
+    // Exclusive scan of the thread sums (Hillis-Steele on a copy, fixed order).
+    for (int off = 1; off < 256; off <<= 1) {
+        double v = (int)threadIdx.x >= off ? s[threadIdx.x - off] : 0.0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    double run = boff[blockIdx.x] + ((int)threadIdx.x > 0 ? s[threadIdx.x - 1] : 0.0);
+    for (int j = 0; j < sub; j++) {
+        const int64_t e = b0 + (int64_t)threadIdx.x * sub + j;
+        if (e < n && e < b0 + chunk) {
+            run += w[e] / tot;
+            cp[e] = (e == n - 1) ? 1.0 : run;
+        }
+    }
+}
+
+__global__ void lower_bound_kernel(const double *__restrict__ cp, int n, const double *__restrict__ u,
+                                   int nt, int *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt)
+        return;
+    const double p = u[t];
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (cp[mid] < p)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    out[t] = lo < n ? lo : n - 1;
+}
+
+constexpr int NT_MAX = 32;
+
+template <typename T>
+__global__ __launch_bounds__(256) void candidate_cost_kernel(const T *__restrict__ X, int n, int d,
+                                                             const double *__restrict__ w,
+                                                             const int *__restrict__ cand, int nt,
+                                                             double *__restrict__ part, int nb) {
+    extern __shared__ double cs[]; // nt x d
+    __shared__ double red[256];
+    for (int t = threadIdx.x; t < nt * d; t += 256) {
+        const int c = t / d, l = t - c * d;
+        cs[t] = (double)X[(int64_t)cand[c] * d + l];
+    }
+    __syncthreads();
+    double acc[NT_MAX];
+#pragma unroll
+    for (int c = 0; c < NT_MAX; c++)
+        acc[c] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const double wi = w[i];
+#pragma unroll
+        for (int c = 0; c < NT_MAX; c++) {
+            if (c < nt) {
+                double v = sqdist_rows(X, d, i, cs + c * d);
+                v = wi < v ? wi : v;
+                acc[c] += v > 0.0 ? v : 0.0;
+            }
+        }
+    }
+    for (int c = 0; c < nt; c++) {
+        red[threadIdx.x] = acc[c];
+        __syncthreads();
+        for (int m = 128; m > 0; m >>= 1) {
+            if ((int)threadIdx.x < m)
+                red[threadIdx.x] += red[threadIdx.x + m];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            part[(int64_t)c * nb + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void reduce_rows_kernel(const double *__restrict__ part, int nt,
+                                                          int nb, double *__restrict__ out) {
+    __shared__ double s[256];
+    const int t = blockIdx.x;
+    double x = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256)
+        x += part[(int64_t)t * nb + b];
+    s[threadIdx.x] = x;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m)
+            s[threadIdx.x] += s[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[t] = s[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kmeans_transform_kernel(const T *__restrict__ X, int n, int d,
+                                                               const double *__restrict__ C, int k,
+                                                               T *__restrict__ D, int64_t ldd) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)n * k)
+        return;
+    const int64_t i = e / k;
+    const int c = (int)(e - i * k);
+    double acc = 0.0;
+    for (int l = 0; l < d; l++) {
+        const double t = (double)X[i * d + l] - C[(int64_t)c * d + l];
+        acc = fma(t, t, acc);
+    }
+    D[i * ldd + c] = (T)sqrt(acc);
+}
+
+__host__ inline unsigned grid_for(int64_t n, int block, int64_t cap = 65536) {
+    int64_t g = (n + block - 1) / block;
+    if (g > cap)
+        g = cap;
+    if (g < 1)
+        g = 1;
+    return (unsigned)g;
+}
+
+} // namespace
+
+template <typename T>
+hipError_t launch_col_absmax(const T *X, int n, int d, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(col_absmax_kernel<T>, dim3(grid_for((int64_t)n * d, 256, 4096)), dim3(256),
+                       0, s, X, n, d, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k, int *labels,
+                                const int *prev, int *changed, double *mind, hipStream_t s) {
+    dim3 grid((n + 255) / 256), block(256);
+    if (d <= 8)
+        hipLaunchKernelGGL((kmeans_assign_kernel<T, 8>), grid, block, 0, s, X, n, d, C, k, labels,
+                           prev, changed, mind);
+    else if (d <= 16)
+        hipLaunchKernelGGL((kmeans_assign_kernel<T, 16>), grid, block, 0, s, X, n, d, C, k,
+                           labels, prev, changed, mind);
+    else if (d <= 32)
+        hipLaunchKernelGGL((kmeans_assign_kernel<T, 32>), grid, block, 0, s, X, n, d, C, k,
+                           labels, prev, changed, mind);
+    else if (d <= 64)
+        hipLaunchKernelGGL((kmeans_assign_kernel<T, 64>), grid, block, 0, s, X, n, d, C, k,
+                           labels, prev, changed, mind);
+    else
+        hipLaunchKernelGGL((kmeans_assign_kernel<T, 0>), grid, block, 0, s, X, n, d, C, k, labels,
+                           prev, changed, mind);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *labels, int c0,
+                                      int kc, const int *escale, int ppb, int nblocks,
+                                      long long *part, int *cnt, hipStream_t s) {
+    const size_t lds = (size_t)kc * d * 2 * sizeof(long long) + (size_t)kc * sizeof(int);
+    hipLaunchKernelGGL(kmeans_partial_kernel<T>, dim3(nblocks), dim3(256), lds, s, X, n, d, labels,
+                       c0, kc, escale, ppb, part, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_kmeans_reduce(const long long *part, const int *cnt, int nblocks, int c0,
+                                int kc, int d, const int *escale, double *C, int *counts,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(kmeans_reduce_kernel, dim3((kc * d + 255) / 256), dim3(256), 0, s, part,
+                       cnt, nblocks, c0, kc, d, escale, C, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_kmeans_shift(const double *Cprev, const double *Cnew, int len, double *out,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(kmeans_shift_kernel, dim3(1), dim3(1024), 0, s, Cprev, Cnew, len, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_double(const double *v, int n, double *tmp, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_partial_kernel, dim3(256), dim3(256), 0, s, v, n, tmp);
+    hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(256), 0, s, tmp, 256, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_convert(const T *src, int64_t n, double *dst, hipStream_t s) {
+    hipLaunchKernelGGL(convert_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, dst);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_convert_back(const double *src, int64_t n, T *dst, hipStream_t s) {
+    hipLaunchKernelGGL(convert_back_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n,
+                       dst);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_gather_rows(const T *X, int d, const int *idx, int m, double *C,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(gather_rows_kernel<T>, dim3(grid_for((int64_t)m * d, 256, 1 << 30)),
+                       dim3(256), 0, s, X, d, idx, m, C);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_dist_to_row(const T *X, int n, int d, int ci, double *w, hipStream_t s) {
+    hipLaunchKernelGGL(dist_to_row_kernel<T>, dim3((n + 255) / 256), dim3(256),
+                       (size_t)d * sizeof(double), s, X, n, d, ci, w, 0);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_min_update(const T *X, int n, int d, int ci, double *w, hipStream_t s) {
+    hipLaunchKernelGGL(dist_to_row_kernel<T>, dim3((n + 255) / 256), dim3(256),
+                       (size_t)d * sizeof(double), s, X, n, d, ci, w, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalised_scan(const double *w, int n, const double *total, double *cp,
+                                  double *tmp, hipStream_t s) {
+    const int nb = SCAN_BLOCKS;
+    const int chunk = (n + nb - 1) / nb;
+    hipLaunchKernelGGL(scan_blocksum_kernel, dim3(nb), dim3(256), 0, s, w, n, total, tmp, chunk);
+    hipLaunchKernelGGL(scan_blockoffsets_kernel, dim3(1), dim3(64), 0, s, tmp, nb);
+    hipLaunchKernelGGL(scan_final_kernel, dim3(nb), dim3(256), 0, s, w, n, total, tmp, cp, chunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_lower_bound(const double *cp, int n, const double *u, int nt, int *out,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(lower_bound_kernel, dim3((nt + 63) / 64), dim3(64), 0, s, cp, n, u, nt, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_candidate_cost(const T *X, int n, int d, const double *w, const int *cand,
+                                 int nt, double *part, int nb, hipStream_t s) {
+    if (nt > NT_MAX)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(candidate_cost_kernel<T>, dim3(nb), dim3(256),
+                       (size_t)nt * d * sizeof(double), s, X, n, d, w, cand, nt, part, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_rows(const double *part, int nt, int nb, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(nt), dim3(256), 0, s, part, nt, nb, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_kmeans_transform(const T *X, int n, int d, const double *C, int k, T *D,
+                                   int64_t ldd, hipStream_t s) {
+    const int64_t total = (int64_t)n * k;
+    hipLaunchKernelGGL(kmeans_transform_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256),
+                       0, s, X, n, d, C, k, D, ldd);
+    return hipGetLastError();
+}
+
+#define DA_KM_INST(T)                                                                           \
+    template hipError_t launch_col_absmax<T>(const T *, int, int, double *, hipStream_t);       \
+    template hipError_t launch_kmeans_assign<T>(const T *, int, int, const double *, int, int *, \
+                                                const int *, int *, double *, hipStream_t);     \
+    template hipError_t launch_kmeans_partial_sums<T>(const T *, int, int, const int *, int, int, \
+                                                      const int *, int, int, long long *, int *, \
+                                                      hipStream_t);                              \
+    template hipError_t launch_convert<T>(const T *, int64_t, double *, hipStream_t);           \
+    template hipError_t launch_convert_back<T>(const double *, int64_t, T *, hipStream_t);      \
+    template hipError_t launch_gather_rows<T>(const T *, int, const int *, int, double *,       \
+                                              hipStream_t);                                      \
+    template hipError_t launch_dist_to_row<T>(const T *, int, int, int, double *, hipStream_t); \
+    template hipError_t launch_min_update<T>(const T *, int, int, int, double *, hipStream_t);  \
+    template hipError_t launch_candidate_cost<T>(const T *, int, int, const double *,           \
+                                                 const int *, int, double *, int, hipStream_t); \
+    template hipError_t launch_kmeans_transform<T>(const T *, int, int, const double *, int, T *, \
+                                                   int64_t, hipStream_t);
+DA_KM_INST(float)
+DA_KM_INST(double)
+#undef DA_KM_INST
+
+} // namespace dagpu
+
+namespace dagpu {
+namespace {
+template <typename T>
+__global__ void label_sqdist_kernel(const T *__restrict__ X, int n, int d,
+                                    const double *__restrict__ C, const int *__restrict__ labels,
+                                    double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double *c = C + (int64_t)labels[i] * d;
+    double acc = 0.0;
+    for (int l = 0; l < d; l++) {
+        const double t = (double)X[i * d + l] - c[l];
+        acc = fma(t, t, acc);
+    }
+    out[i] = acc;
+}
+} // namespace
+template <typename T>
+hipError_t launch_label_sqdist(const T *X, int n, int d, const double *C, const int *labels,
+                               double *out, hipStream_t s) {
+    hipLaunchKernelGGL(label_sqdist_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, s, X, n, d, C,
+                       labels, out);
+    return hipGetLastError();
+}
+template hipError_t launch_label_sqdist<float>(const float *, int, int, const double *,
+                                               const int *, double *, hipStream_t);
+template hipError_t launch_label_sqdist<double>(const double *, int, int, const double *,
+                                                const int *, double *, hipStream_t);
+} // namespace dagpu
+
+namespace dagpu {
+namespace {
+__global__ void round_through_kernel(double *v, int64_t n, int mode) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const double x = v[e];
+        v[e] = mode == 2 ? (double)(float)(_Float16)(float)x : (double)(float)x;
+    }
+}
+template <typename T>
+__global__ void round_data_kernel(const T *__restrict__ src, int64_t n, int mode, T *__restrict__ dst) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const T x = src[e];
+        dst[e] = mode == 2 ? (T)(float)(_Float16)(float)x : (T)(float)x;
+    }
+}
+} // namespace
+hipError_t launch_round_through(double *v, int64_t n, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(round_through_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, v, n, mode);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_round_data(const T *src, int64_t n, int mode, T *dst, hipStream_t s) {
+    hipLaunchKernelGGL(round_data_kernel<T>, dim3(grid_for(n, 256)), dim3(256), 0, s, src, n, mode,
+                       dst);
+    return hipGetLastError();
+}
+template hipError_t launch_round_data<float>(const float *, int64_t, int, float *, hipStream_t);
+template hipError_t launch_round_data<double>(const double *, int64_t, int, double *, hipStream_t);
+} // namespace dagpu

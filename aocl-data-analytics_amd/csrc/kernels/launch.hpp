@@ -1,0 +1,167 @@
+// Host-callable launchers of the HIP kernels. Every launcher enqueues on the given stream
+// and returns the launch status; none of them synchronizes.
+#pragma once
+#include <cstdint>
+#include <hip/hip_runtime_api.h>
+
+namespace dagpu {
+
+// Metric codes used by the distance kernels (values are internal).
+enum KMetric : int {
+    KM_SQEUCLIDEAN = 0, // sum (x-y)^2
+    KM_EUCLIDEAN = 1,   // sqrt(sum (x-y)^2)
+    KM_MANHATTAN = 2,   // sum |x-y|
+    KM_MINKOWSKI = 3,   // (sum |x-y|^p)^(1/p)
+    KM_COSINE = 4,      // 1 - x.y / (|x||y|), norms <= eps replaced by 1
+    KM_INNER = 5,       // x.y
+    KM_SQEUCLIDEAN_GEMM = 6, // max(0, |x|^2 + |y|^2 - 2 x.y)
+    KM_EUCLIDEAN_GEMM = 7,   // sqrt of the above
+    KM_MINKOWSKI_KEY = 8,    // sum |x-y|^p (monotone key of minkowski)
+};
+
+// Layout conversion: column-major (ld) -> row-major compact, and row-major (ld) -> compact.
+template <typename T>
+hipError_t launch_colmajor_to_rowmajor(const T *src, int64_t ld, int rows, int cols, T *dst,
+                                       hipStream_t s);
+// Row-major rows -> row-major with padded leading dimension ldd, zero padding on the right
+// and zero rows up to rows_pad; output element type U.
+template <typename T, typename U>
+hipError_t launch_pack_rows(const T *src, int64_t lds, int rows, int cols, U *dst, int64_t ldd,
+                            int rows_pad, hipStream_t s);
+
+// Squared row norms in double: out[i] = sum_j x[i*ld+j]^2.
+template <typename T>
+hipError_t launch_row_sqnorms(const T *X, int64_t ld, int rows, int cols, double *out,
+                              hipStream_t s);
+
+// Cosine norms: sqrt of the squared norms, replaced by 1 when <= eps (in place).
+hipError_t launch_cosine_norms(double *sq, int rows, double eps, hipStream_t s);
+
+// Distance tile: D[i*ldd + j] = metric(X_i, Y_j) for i<m, j<n, X and Y row-major compact
+// (ld = k), accumulated in double and stored as U. nx/ny: cosine norms or squared norms for
+// the GEMM-form metrics (may be null otherwise). zero_diag forces D[i][j] = 0 where
+// i + diag_offset == j.
+template <typename T, typename U>
+hipError_t launch_pairwise_direct(int metric, const T *X, const T *Y, int m, int n, int k,
+                                  U *D, int64_t ldd, const double *nx, const double *ny,
+                                  double p, int zero_diag, int diag_offset, hipStream_t s);
+
+// Per-row top-k of a row-major key matrix (rows x n, leading dimension ld) by
+// (key, column index), ascending (descending when `largest`). Writes the k best keys and
+// indices per row, sorted, to out_key[row*k + j] / out_idx[row*k + j]. k <= 3840.
+template <typename K>
+hipError_t launch_row_topk(const K *keys, int64_t ld, int rows, int n, int k, int largest,
+                           K *out_key, int *out_idx, hipStream_t s);
+
+} // namespace dagpu
+
+namespace dagpu {
+// Converts selected keys into reported distances and scatters them into the output
+// arrays: out_dist[q*ldo + j] (post: 0 key, 1 sqrt(max(key,0)), 2 key^(1/p)).
+template <typename T>
+hipError_t launch_knn_finalize(const double *keys, const int *idx, int rows, int k, int post,
+                               double p, T *out_dist, int *out_idx, int64_t ldo,
+                               hipStream_t s);
+} // namespace dagpu
+
+namespace dagpu {
+// ---- k-means ----
+// Per-feature maximum absolute value (as double), out[j] = max_i |X[i*d+j]|; out must be
+// zeroed beforehand.
+template <typename T>
+hipError_t launch_col_absmax(const T *X, int n, int d, double *out, hipStream_t s);
+
+// Exact assignment: labels[i] = argmin_c sum_l (X[i,l]-C[c,l])^2 in double, ties to the
+// lowest c. mind[i] receives the minimum squared distance (may be null). When prev is
+// non-null, *changed is incremented for every i with labels[i] != prev[i].
+template <typename T>
+hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k, int *labels,
+                                const int *prev, int *changed, double *mind, hipStream_t s);
+
+// Fixed-point partial sums of the points of each cluster in [c0, c0+kc): each workgroup
+// b handles points [b*ppb, (b+1)*ppb) and writes part[(b*kc + c)*d*W + l*W + w] (W = 2
+// words: the value scaled by 2^e[l] rounded to int64 and the remainder scaled by
+// 2^(e[l]+40)) and cnt[b*kc + c].
+template <typename T>
+hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *labels, int c0,
+                                      int kc, const int *escale, int ppb, int nblocks,
+                                      long long *part, int *cnt, hipStream_t s);
+// Reduces the partials over blocks (exactly, in 128-bit integers) and writes
+// C[(c0+c)*d + l] = sum / max(count, 1), counts[c0+c] = count.
+hipError_t launch_kmeans_reduce(const long long *part, const int *cnt, int nblocks, int c0,
+                                int kc, int d, const int *escale, double *C, int *counts,
+                                hipStream_t s);
+// out[0] = ||Cprev||_F^2, out[1] = ||Cprev - Cnew||_F^2 (single workgroup, fixed order).
+hipError_t launch_kmeans_shift(const double *Cprev, const double *Cnew, int len, double *out,
+                               hipStream_t s);
+// Deterministic sum of v[0..n): partial sums per block into tmp[0..256), then out[0].
+hipError_t launch_sum_double(const double *v, int n, double *tmp, double *out, hipStream_t s);
+// Converts centres between double and T.
+template <typename T>
+hipError_t launch_convert(const T *src, int64_t n, double *dst, hipStream_t s);
+template <typename T>
+hipError_t launch_convert_back(const double *src, int64_t n, T *dst, hipStream_t s);
+// Gathers rows idx[0..m) of X (n x d, type T) into C (m x d, double).
+template <typename T>
+hipError_t launch_gather_rows(const T *X, int d, const int *idx, int m, double *C,
+                              hipStream_t s);
+
+// k-means++ helpers. w[i] = squared distance of X_i to row `ci` of X (exact, double), with
+// negative or non-finite values set to 0 and w[ci] = 0.
+template <typename T>
+hipError_t launch_dist_to_row(const T *X, int n, int d, int ci, double *w, hipStream_t s);
+// cp[i] = inclusive prefix sum of w[i]/sum (sum = *total), cp[n-1] = 1. tmp: >= 2*1024 doubles.
+hipError_t launch_normalised_scan(const double *w, int n, const double *total, double *cp,
+                                  double *tmp, hipStream_t s);
+// out[t] = first i with cp[i] >= u[t] (lower_bound), t < nt.
+hipError_t launch_lower_bound(const double *cp, int n, const double *u, int nt, int *out,
+                              hipStream_t s);
+// For each candidate row cand[t]: part[t*nb + b] = partial of sum_i max(0, min(w[i], d_i(t))).
+template <typename T>
+hipError_t launch_candidate_cost(const T *X, int n, int d, const double *w, const int *cand,
+                                 int nt, double *part, int nb, hipStream_t s);
+// w[i] = max(0, min(w[i], d_i(ci))), w[ci] = 0.
+template <typename T>
+hipError_t launch_min_update(const T *X, int n, int d, int ci, double *w, hipStream_t s);
+// Reduces part[t*nb + b] over b for each t into out[t] (fixed order).
+hipError_t launch_reduce_rows(const double *part, int nt, int nb, double *out, hipStream_t s);
+// Direct distances of rows of X to centres (double), sqrt applied: D[i*ldd + c].
+template <typename T>
+hipError_t launch_kmeans_transform(const T *X, int n, int d, const double *C, int k, T *D,
+                                   int64_t ldd, hipStream_t s);
+} // namespace dagpu
+
+namespace dagpu {
+// out[i] = ||X_i - C[labels[i]]||^2 in double.
+template <typename T>
+hipError_t launch_label_sqdist(const T *X, int n, int d, const double *C, const int *labels,
+                               double *out, hipStream_t s);
+} // namespace dagpu
+
+namespace dagpu {
+// Rounds doubles through a lower precision and back: mode 1 float, mode 2 half.
+hipError_t launch_round_through(double *v, int64_t n, int mode, hipStream_t s);
+// dst[i] = (T)(lower(src[i])) for a data matrix: mode 1 float, mode 2 half.
+template <typename T>
+hipError_t launch_round_data(const T *src, int64_t n, int mode, T *dst, hipStream_t s);
+} // namespace dagpu
+
+namespace dagpu {
+// ---- fused kNN (fp32 MFMA) ----
+// Candidates kept per query and split for a request of k neighbours.
+int knn_tile_kc(int k);
+// Q: nq x Kp, Y: n x Kp (row-major, zero padded, Kp a multiple of 16, Kp <= 128), ny: fp32
+// squared norms of Y. Writes part_key/part_idx[(q*nsplit + s)*kc + i] and
+// part_tau[q*nsplit + s].
+hipError_t launch_knn_tile_f32(const float *Q, int nq, const float *Y, const float *ny, int n,
+                               int Kp, int nsplit, int kc, float *part_key, int *part_idx,
+                               float *part_tau, hipStream_t s);
+// Q: nq x d and Y: n x d unpadded rows (type T); unit: roundoff of the key arithmetic;
+// ymax: max row norm of Y. Certified queries get out_dist/out_idx (row-major nq x k);
+// the others are appended to fb_list (fb_count must be zeroed before).
+template <typename T>
+hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const float *part_tau,
+                            int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,
+                            double unit, double ymax, int post, T *out_dist, int *out_idx,
+                            int *fb_count, int *fb_list, hipStream_t s);
+} // namespace dagpu

@@ -1,0 +1,861 @@
+#include "kmeans.hpp"
+#include "../core/factories.hpp"
+#include "../kernels/launch.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+
+namespace dagpu {
+
+// kmeans_types.hpp:39-41 of the reference.
+enum { KM_LLOYD = 0, KM_ELKAN, KM_HARTIGAN_WONG, KM_MACQUEEN };
+enum { INIT_RANDOM = 0, INIT_KMEANSPP, INIT_SUPPLIED, INIT_RANDOM_PARTITIONS, INIT_AFK_MCMC };
+enum { EMPTY_IGNORE = 0, EMPTY_ERROR, EMPTY_SPLIT };
+
+template <typename T> KMeansHandle<T>::KMeansHandle(ErrorStack &e, GpuContext &g) : AlgHandle(e, g) {
+    // Options of source/core/clustering/kmeans/kmeans_options.hpp:46-137.
+    const da_int imax = std::numeric_limits<da_int>::max();
+    opts.add_int("n_clusters", "Number of clusters required.", 1, LowerBound::greater_equal, imax,
+                 UpperBound::p_inf, 1);
+    opts.add_int("n_init",
+                 "Number of runs with different random seeds (ignored if you have specified "
+                 "initial cluster centres).",
+                 1, LowerBound::greater_equal, imax, UpperBound::p_inf, 10);
+    opts.add_int("max_iter", "Maximum number of iterations.", 1, LowerBound::greater_equal, imax,
+                 UpperBound::p_inf, 300);
+    opts.add_int("low precision max_iter",
+                 "If mixed precision iterative refinement is enabled, maximum number of "
+                 "iterations for the low precision phase.",
+                 1, LowerBound::greater_equal, imax, UpperBound::p_inf, 200);
+    opts.add_int("seed", "Seed for random number generation; set to -1 for non-deterministic results.",
+                 -1, LowerBound::greater_equal, imax, UpperBound::p_inf, 0);
+    opts.add_int("afk-mc2 samples", "Number of samples to take for the AFK-MC2 initialization method.",
+                 1, LowerBound::greater_equal, imax, UpperBound::p_inf, 50);
+    opts.add_string("initialization method", "How to determine the initial cluster centres.",
+                    {{"random", INIT_RANDOM},
+                     {"k-means++", INIT_KMEANSPP},
+                     {"supplied", INIT_SUPPLIED},
+                     {"random partitions", INIT_RANDOM_PARTITIONS},
+                     {"afk-mc2", INIT_AFK_MCMC}},
+                    "k-means++");
+    opts.add_string("algorithm", "Choice of underlying k-means algorithm.",
+                    {{"lloyd", KM_LLOYD},
+                     {"elkan", KM_ELKAN},
+                     {"hartigan-wong", KM_HARTIGAN_WONG},
+                     {"macqueen", KM_MACQUEEN}},
+                    "lloyd");
+    opts.add_string("mixed precision",
+                    "Whether to use mixed precision iterative refinement, in which lower precision "
+                    "arithmetic is used before switching to the working precision for the final "
+                    "iterations.",
+                    {{"yes", 1}, {"no", 0}}, "no");
+    opts.add_string("empty clusters",
+                    "How to deal with empty clusters at the end of a k-means iteration.",
+                    {{"ignore", EMPTY_IGNORE}, {"error", EMPTY_ERROR}, {"split", EMPTY_SPLIT}},
+                    "ignore");
+    opts.add_string("distance",
+                    "Distance metric used for clustering. Use 'euclidean' for standard k-means or "
+                    "'cosine' for spherical k-means.",
+                    {{"euclidean", 0}, {"cosine", 1}}, "euclidean");
+    opts.add_string("normalize data",
+                    "Whether to normalize the input data before clustering. This option is only "
+                    "used if distance is set to cosine.",
+                    {{"yes", 1}, {"no", 0}}, "yes");
+    opts.add_real("convergence tolerance", "Convergence tolerance.", 0.0,
+                  LowerBound::greater_equal, 0.0, UpperBound::p_inf, (double)(T)1.0e-4);
+    opts.add_real("low precision convergence tolerance",
+                  "If mixed precision iterative refinement is enabled, convergence tolerance for "
+                  "the low precision phase.",
+                  0.0, LowerBound::greater_equal, 0.0, UpperBound::p_inf, (double)(T)1.0e-2);
+}
+
+template <typename T> void KMeansHandle<T>::refresh() { model_trained = false; }
+
+template <typename T>
+da_status KMeansHandle<T>::upload_data(const T *src, da_int ld, DeviceBuffer &dst) {
+    hipStream_t st = gpu->stream();
+    const size_t bytes = (size_t)n_samples_ * n_features_ * sizeof(T);
+    DA_HIP(err, dst.reserve(bytes));
+    if (data_order_ == row_major) {
+        DA_HIP(err, hipMemcpy2DAsync(dst.ptr(), n_features_ * sizeof(T), src, (size_t)ld * sizeof(T),
+                                     n_features_ * sizeof(T), n_samples_, hipMemcpyHostToDevice, st));
+    } else {
+        DeviceBuffer stage;
+        DA_HIP(err, stage.reserve(bytes));
+        DA_HIP(err, hipMemcpy2DAsync(stage.ptr(), n_samples_ * sizeof(T), src, (size_t)ld * sizeof(T),
+                                     n_samples_ * sizeof(T), n_features_, hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(stage.as<T>(), n_samples_, n_samples_, n_features_,
+                                                   dst.as<T>(), st));
+        DA_HIP(err, hipStreamSynchronize(st));
+    }
+    return da_status_success;
+}
+
+// Chooses the fixed-point exponents of the centroid sums from the column maxima so that
+// no partial or total of n values can exceed 2^60 (see kmeans.hip).
+template <typename T>
+template <typename U>
+da_status KMeansHandle<T>::prepare_scales_for(const U *dX) {
+    hipStream_t st = gpu->stream();
+    const int d = n_features_;
+    DA_HIP(err, tmp_.reserve((size_t)std::max(d, 2048) * sizeof(double)));
+    DA_HIP(err, hipMemsetAsync(tmp_.ptr(), 0, (size_t)d * sizeof(double), st));
+    DA_HIP(err, launch_col_absmax<U>(dX, n_samples_, d, tmp_.as<double>(), st));
+    std::vector<double> amax(d);
+    DA_HIP(err, hipMemcpyAsync(amax.data(), tmp_.ptr(), d * sizeof(double), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    int l2n = 0;
+    while (((int64_t)1 << l2n) < (int64_t)n_samples_ + 1)
+        l2n++;
+    escale_host_.assign(d + 1, 0);
+    for (int l = 0; l < d; l++) {
+        if (amax[l] > 0 && std::isfinite(amax[l])) {
+            int E = 0;
+            std::frexp(amax[l], &E);
+            escale_host_[l] = 60 - l2n - E;
+        }
+    }
+    escale_host_[d] = 61 - l2n;
+    DA_HIP(err, escale_.reserve((d + 1) * sizeof(int)));
+    DA_HIP(err, hipMemcpyAsync(escale_.ptr(), escale_host_.data(), (d + 1) * sizeof(int),
+                               hipMemcpyHostToDevice, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    return da_status_success;
+}
+
+template <typename T>
+da_status KMeansHandle<T>::set_data(da_int n_samples, da_int n_features, const T *A, da_int lda) {
+    refresh();
+    initdone_ = false;
+    data_order_ = order();
+    da_status s = check_2D_array<T>(data_order_, n_samples, n_features, A, lda, "n_samples",
+                                    "n_features", "A", "lda");
+    if (s != da_status_success)
+        return s;
+    s = gpu->init(err);
+    if (s != da_status_success)
+        return s;
+    A_usr_ = A;
+    lda_usr_ = lda;
+    n_samples_ = n_samples;
+    n_features_ = n_features;
+    s = upload_data(A, lda, X_);
+    if (s != da_status_success)
+        return s;
+    s = prepare_scales_for<T>(X_.as<T>());
+    if (s != da_status_success)
+        return s;
+    initdone_ = true;
+    model_trained = false;
+    // n_clusters is bounded by n_samples once the data is known (kmeans.cpp:283-297).
+    da_int temp = opts.ival("n_clusters");
+    opts.rebound_int("n_clusters", n_samples, UpperBound::less_equal);
+    opts.set_int("n_clusters", std::min(temp, n_samples));
+    if (temp > n_samples)
+        return err->warn(da_status_incompatible_options,
+                         "The requested number of clusters has been decreased from " +
+                             std::to_string(temp) + " to " + std::to_string(n_samples) +
+                             " due to the size of the data array.");
+    return da_status_success;
+}
+
+template <typename T> da_status KMeansHandle<T>::set_init_centres(const T *C, da_int ldc) {
+    if (!initdone_)
+        return err->error(da_status_no_data, "No data has been passed to the handle. Please call "
+                                             "da_kmeans_set_data_s or da_kmeans_set_data_d.");
+    if (order() != data_order_)
+        return err->error(da_status_incompatible_options,
+                          "The storage order option was changed after data was set.");
+    da_int k = opts.ival("n_clusters");
+    da_status s = check_2D_array<T>(data_order_, k, n_features_, C, ldc, "n_clusters", "n_features",
+                                    "C", "ldc");
+    if (s != da_status_success)
+        return s;
+    // Row-major centres are copied now, column-major ones read at compute time, as the
+    // reference does (kmeans.cpp:300-348).
+    if (data_order_ == row_major) {
+        C_rowmajor_copy_.resize((size_t)k * n_features_);
+        for (da_int i = 0; i < k; i++)
+            for (da_int j = 0; j < n_features_; j++)
+                C_rowmajor_copy_[(size_t)i * n_features_ + j] = C[(size_t)i * ldc + j];
+        C_usr_ = C_rowmajor_copy_.data();
+        ldc_usr_ = n_features_;
+    } else {
+        C_usr_ = C;
+        ldc_usr_ = ldc;
+    }
+    n_clusters_ = k;
+    centres_supplied_ = true;
+    return da_status_success;
+}
+
+template <typename T>
+da_status KMeansHandle<T>::upload_rows_double(const T *src, da_int rows, da_int ld, da_order ord,
+                                              DeviceBuffer &dst) {
+    std::vector<double> h((size_t)rows * n_features_);
+    for (da_int i = 0; i < rows; i++)
+        for (da_int j = 0; j < n_features_; j++)
+            h[(size_t)i * n_features_ + j] =
+                (double)(ord == row_major ? src[(size_t)i * ld + j] : src[i + (size_t)j * ld]);
+    DA_HIP(err, dst.reserve(h.size() * sizeof(double)));
+    DA_HIP(err, hipMemcpyAsync(dst.ptr(), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                               gpu->stream()));
+    DA_HIP(err, hipStreamSynchronize(gpu->stream()));
+    return da_status_success;
+}
+
+template <typename T>
+template <typename U>
+da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Cout, int *lab_new,
+                                      const int *lab_old, bool update, int &changed,
+                                      double &normc2, double &shift2) {
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    int *dchanged = reinterpret_cast<int *>(small_.as<double>() + 4);
+    DA_HIP(err, hipMemsetAsync(dchanged, 0, sizeof(int), st));
+    {
+        KernelScope ks(*gpu, "kmeans_assign");
+        DA_HIP(err, launch_kmeans_assign<U>(dX, n, d, Cin, k, lab_new, lab_old, dchanged,
+                                            mind_.as<double>(), st));
+    }
+    if (update) {
+        KernelScope ks(*gpu, "kmeans_update");
+        for (int c0 = 0; c0 < k; c0 += kc_) {
+            const int kcc = std::min<int>(kc_, k - c0);
+            DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n, d, lab_new, c0, kcc, escale_.as<int>(),
+                                                      ppb_, nblocks_, part_.as<long long>(),
+                                                      pcnt_.as<int>(), st));
+            DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), nblocks_, c0,
+                                             kcc, d, escale_.as<int>(), Cout, counts_.as<int>(), st));
+        }
+        DA_HIP(err, launch_kmeans_shift(Cin, Cout, k * d, small_.as<double>(), st));
+    }
+    double h[5];
+    DA_HIP(err, hipMemcpyAsync(h, small_.ptr(), 5 * sizeof(double), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    std::memcpy(&changed, &h[4], sizeof(int));
+    normc2 = h[0];
+    shift2 = h[1];
+    return da_status_success;
+}
+
+template <typename T>
+template <typename U>
+da_status KMeansHandle<T>::centres_from_labels(const U *dX, const int *labels, double *C) {
+    hipStream_t st = gpu->stream();
+    for (int c0 = 0; c0 < n_clusters_; c0 += kc_) {
+        const int kcc = std::min<int>(kc_, n_clusters_ - c0);
+        DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n_samples_, n_features_, labels, c0, kcc,
+                                                  escale_.as<int>(), ppb_, nblocks_,
+                                                  part_.as<long long>(), pcnt_.as<int>(), st));
+        DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), nblocks_, c0, kcc,
+                                         n_features_, escale_.as<int>(), C, counts_.as<int>(), st));
+    }
+    return da_status_success;
+}
+
+// Empty-cluster handling after an iteration (kmeans.cpp:1127-1330): "error" flags the run,
+// "split" moves the furthest point of a non-singleton cluster into each empty cluster. The
+// split bookkeeping is sequential and only runs when a cluster has emptied, so it is done
+// on the host on copies of the labels, centres and data.
+template <typename T>
+template <typename U>
+da_status KMeansHandle<T>::handle_empty_clusters(const U *dX, double *C, int *labels, bool &split,
+                                                 bool &empty_error) {
+    split = false;
+    empty_error = false;
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    std::vector<int> lab(n);
+    DA_HIP(err, hipMemcpyAsync(lab.data(), labels, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    std::vector<int> count(k, 0);
+    for (int i = 0; i < n; i++)
+        count[lab[i]]++;
+    int n_empty = 0;
+    for (int c = 0; c < k; c++)
+        n_empty += count[c] == 0;
+    if (n_empty == 0)
+        return da_status_success;
+    if (empty_mode_ == EMPTY_ERROR) {
+        empty_error = true;
+        return da_status_success;
+    }
+    std::vector<U> A((size_t)n * d);
+    std::vector<double> cen((size_t)k * d);
+    DA_HIP(err, hipMemcpyAsync(A.data(), dX, A.size() * sizeof(U), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipMemcpyAsync(cen.data(), C, cen.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    auto sqd = [&](int i, int c) {
+        double s = 0;
+        for (int j = 0; j < d; j++) {
+            double t = (double)A[(size_t)i * d + j] - cen[(size_t)c * d + j];
+            s += t * t;
+        }
+        return s;
+    };
+    std::vector<double> dist(n);
+    for (int i = 0; i < n; i++)
+        dist[i] = sqd(i, lab[i]);
+    for (int c = 0; c < k; c++) {
+        if (count[c] != 0)
+            continue;
+        int best = -1;
+        double bd = -1.0;
+        for (int i = 0; i < n; i++)
+            if (count[lab[i]] > 1 && std::isfinite(dist[i]) && dist[i] > bd) {
+                bd = dist[i];
+                best = i;
+            }
+        if (best < 0)
+            break;
+        const int old = lab[best];
+        const int n_old = count[old];
+        for (int j = 0; j < d; j++) {
+            double &cv = cen[(size_t)old * d + j];
+            cv = (cv * n_old - (double)A[(size_t)best * d + j]) / (n_old - 1);
+        }
+        for (int j = 0; j < d; j++)
+            cen[(size_t)c * d + j] = (double)A[(size_t)best * d + j];
+        count[old] -= 1;
+        count[c] = 1;
+        lab[best] = c;
+        dist[best] = 0.0;
+        if (n_empty > 1)
+            for (int i = 0; i < n; i++)
+                if (lab[i] == old)
+                    dist[i] = sqd(i, old);
+    }
+    DA_HIP(err, hipMemcpyAsync(C, cen.data(), cen.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    DA_HIP(err, hipMemcpyAsync(labels, lab.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    split = true;
+    return da_status_success;
+}
+
+// One k-means run from the centres in Ccur_ (perform_kmeans, kmeans.cpp:996-1058).
+template <typename T>
+template <typename U>
+da_status KMeansHandle<T>::run_lloyd(const U *dX, double tol, da_int max_iter, RunResult &res) {
+    hipStream_t st = gpu->stream();
+    int conv = 0;
+    da_int it = 0;
+    int changed = 0;
+    double normc2 = 0, shift2 = 0;
+    res = RunResult{};
+    for (it = 0; it < max_iter; it++) {
+        std::swap(Cprev_, Ccur_);
+        std::swap(lab_prev_, lab_cur_);
+        da_status s = lloyd_step<U>(dX, Cprev_, Ccur_, lab_cur_, lab_prev_, true, changed, normc2, shift2);
+        if (s != da_status_success)
+            return s;
+        if (empty_mode_ != EMPTY_IGNORE) {
+            bool split = false, eerr = false;
+            s = handle_empty_clusters<U>(dX, Ccur_, lab_cur_, split, eerr);
+            if (s != da_status_success)
+                return s;
+            if (eerr) {
+                res.empty_found = true;
+                return da_status_success;
+            }
+            if (split)
+                continue;
+        }
+        // convergence_test, kmeans.cpp:1359-1397.
+        conv = 0;
+        if (it > 1 && changed == 0)
+            conv = 2;
+        else if (std::sqrt(shift2) < tol * std::sqrt(normc2))
+            conv = 1;
+        if (conv > 0)
+            break;
+    }
+    res.n_iter = it;
+    res.converged = conv;
+    if (conv == 1 || it == max_iter) {
+        std::swap(lab_prev_, lab_cur_);
+        std::swap(Cprev_, Ccur_);
+        da_status s = lloyd_step<U>(dX, Cprev_, nullptr, lab_cur_, nullptr, false, changed, normc2, shift2);
+        if (s != da_status_success)
+            return s;
+        std::swap(Cprev_, Ccur_);
+    }
+    // compute_current_inertia (kmeans.cpp:1060-1124), exact squared distances summed in a
+    // fixed order.
+    DA_HIP(err, launch_label_sqdist<U>(dX, n_samples_, n_features_, Ccur_, lab_cur_, mind_.as<double>(), st));
+    DA_HIP(err, launch_sum_double(mind_.as<double>(), n_samples_, tmp_.as<double>(), small_.as<double>() + 2, st));
+    double inertia = 0;
+    DA_HIP(err, hipMemcpyAsync(&inertia, small_.as<double>() + 2, sizeof(double), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    res.inertia = inertia;
+    return da_status_success;
+}
+
+// k-means++ (kmeans.cpp:1524-1664): the random draws are made on the host with the same
+// libstdc++ engine and distributions in the same order; the distance passes, the weight
+// normalisation and the inverse-CDF lookup of std::discrete_distribution run on the GPU.
+template <typename T> da_status KMeansHandle<T>::kmeans_plusplus() {
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    const T *dX = X_.as<T>();
+    std::vector<int> chosen(k, 0);
+    const da_int n_trials = 2 + (da_int)std::log((double)k);
+    std::uniform_int_distribution<> dis_int(0, n - 1);
+    const int first = dis_int(mt_);
+    chosen[0] = first;
+    double *w = w_.as<double>();
+    DA_HIP(err, launch_dist_to_row<T>(dX, n, d, first, w, st));
+    double *total = small_.as<double>() + 3;
+    DA_HIP(err, launch_sum_double(w, n, tmp_.as<double>(), total, st));
+    double htot = 0;
+    DA_HIP(err, hipMemcpyAsync(&htot, total, sizeof(double), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    if (!(htot > 0.0)) {
+        // All points coincide with the first centre: take the first k points.
+        for (int c = 0; c < k; c++)
+            chosen[c] = c;
+    } else {
+        const int nb = 256;
+        std::vector<double> u(n_trials), cost(n_trials);
+        std::vector<int> cand(n_trials);
+        double *du = tmp_.as<double>() + 1024;
+        int *dcand = idx_.as<int>();
+        double *dcost = tmp_.as<double>() + 1024 + 64;
+        double *part = cp_.as<double>(); // reused after the lookup
+        for (int c = 1; c < k; c++) {
+            if (n >= 2) {
+                DA_HIP(err, launch_sum_double(w, n, tmp_.as<double>(), total, st));
+                DA_HIP(err, launch_normalised_scan(w, n, total, cp_.as<double>(), tmp_.as<double>(), st));
+                for (int t = 0; t < n_trials; t++)
+                    u[t] = std::generate_canonical<double, std::numeric_limits<double>::digits>(mt_);
+                DA_HIP(err, hipMemcpyAsync(du, u.data(), n_trials * sizeof(double), hipMemcpyHostToDevice, st));
+                DA_HIP(err, launch_lower_bound(cp_.as<double>(), n, du, n_trials, dcand, st));
+                DA_HIP(err, hipMemcpyAsync(cand.data(), dcand, n_trials * sizeof(int), hipMemcpyDeviceToHost, st));
+                DA_HIP(err, hipStreamSynchronize(st));
+            } else {
+                std::fill(cand.begin(), cand.end(), 0);
+            }
+            DA_HIP(err, launch_candidate_cost<T>(dX, n, d, w, dcand, n_trials, part, nb, st));
+            DA_HIP(err, launch_reduce_rows(part, n_trials, nb, dcost, st));
+            DA_HIP(err, hipMemcpyAsync(cost.data(), dcost, n_trials * sizeof(double), hipMemcpyDeviceToHost, st));
+            DA_HIP(err, hipStreamSynchronize(st));
+            int best = 0;
+            double best_cost = std::numeric_limits<double>::infinity();
+            for (int t = 0; t < n_trials; t++) {
+                bool repeat = false;
+                for (int j = 0; j < t; j++)
+                    repeat |= cand[j] == cand[t];
+                if (repeat)
+                    continue;
+                if (std::isfinite(cost[t]) && cost[t] < best_cost) {
+                    best_cost = cost[t];
+                    best = cand[t];
+                }
+            }
+            chosen[c] = best;
+            DA_HIP(err, launch_min_update<T>(dX, n, d, best, w, st));
+        }
+    }
+    DA_HIP(err, hipMemcpyAsync(idx_.ptr(), chosen.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
+    DA_HIP(err, launch_gather_rows<T>(dX, d, idx_.as<int>(), k, Ccur_, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    return da_status_success;
+}
+
+// initialize_centres, kmeans.cpp:1400-1486.
+template <typename T> da_status KMeansHandle<T>::initialise_centres(da_int method) {
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    switch (method) {
+    case INIT_SUPPLIED: {
+        std::vector<double> h((size_t)k * d);
+        for (int i = 0; i < k; i++)
+            for (int j = 0; j < d; j++)
+                h[(size_t)i * d + j] = (double)(data_order_ == row_major
+                                                    ? C_usr_[(size_t)i * ldc_usr_ + j]
+                                                    : C_usr_[i + (size_t)j * ldc_usr_]);
+        DA_HIP(err, hipMemcpyAsync(Ccur_, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        return da_status_success;
+    }
+    case INIT_RANDOM: {
+        std::vector<da_int> all(n), pick(k);
+        std::iota(all.begin(), all.end(), 0);
+        std::sample(all.begin(), all.end(), pick.begin(), k, mt_);
+        std::vector<int> p32(pick.begin(), pick.end());
+        DA_HIP(err, hipMemcpyAsync(idx_.ptr(), p32.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_gather_rows<T>(X_.as<T>(), d, idx_.as<int>(), k, Ccur_, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        return da_status_success;
+    }
+    case INIT_RANDOM_PARTITIONS: {
+        std::uniform_int_distribution<> dis(0, k - 1);
+        std::vector<int> lab(n);
+        for (int i = 0; i < n; i++)
+            lab[i] = dis(mt_);
+        DA_HIP(err, hipMemcpyAsync(lab_cur_, lab.data(), n * sizeof(int), hipMemcpyHostToDevice, st));
+        da_status s = centres_from_labels<T>(X_.as<T>(), lab_cur_, Ccur_);
+        if (s != da_status_success)
+            return s;
+        // The reference divides by the count without a guard, so an empty partition gives
+        // a NaN centre.
+        std::vector<int> cnt(k);
+        DA_HIP(err, hipMemcpyAsync(cnt.data(), counts_.ptr(), k * sizeof(int), hipMemcpyDeviceToHost, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        std::vector<double> nanrow(d, std::numeric_limits<double>::quiet_NaN());
+        for (int c = 0; c < k; c++)
+            if (cnt[c] == 0)
+                DA_HIP(err, hipMemcpy(Ccur_ + (size_t)c * d, nanrow.data(), d * sizeof(double),
+                                      hipMemcpyHostToDevice));
+        return da_status_success;
+    }
+    case INIT_KMEANSPP:
+        return kmeans_plusplus();
+    default:
+        return err->error(da_status_not_implemented,
+                          "The AFK-MC2 initialization method is not implemented by the MI355X build.");
+    }
+}
+
+template <typename T> da_status KMeansHandle<T>::compute() {
+    if (!initdone_)
+        return err->error(da_status_no_data, "No data has been passed to the handle. Please call "
+                                             "da_kmeans_set_data_s or da_kmeans_set_data_d.");
+    if (order() != data_order_)
+        return err->error(da_status_incompatible_options,
+                          "The storage order option was changed after data was set.");
+    da_int k = opts.ival("n_clusters");
+    const da_int init = opts.skey("initialization method");
+    da_int n_init = opts.ival("n_init");
+    const da_int max_iter = opts.ival("max_iter");
+    const double tol = opts.rval("convergence tolerance");
+    da_int seed = opts.ival("seed");
+    const da_int algorithm = opts.skey("algorithm");
+    const bool mixed = opts.skey("mixed precision") == 1;
+    empty_mode_ = opts.skey("empty clusters");
+    const bool spherical = opts.skey("distance") == 1;
+    opts.rebound_int("n_clusters", std::numeric_limits<da_int>::max(), UpperBound::p_inf);
+
+    da_status status = da_status_success;
+    if (n_init > 1 && init == INIT_SUPPLIED) {
+        n_init = 1;
+        err->warn(da_status_incompatible_options,
+                  "n_init was set to " + std::to_string(opts.ival("n_init")) +
+                      " but the initialization method was set to 'supplied'. The k-means "
+                      "algorithm will only be run once.");
+    }
+    if (algorithm == KM_HARTIGAN_WONG && (k == 1 || k >= n_samples_))
+        return err->error(da_status_incompatible_options,
+                          "The Hartigan-Wong algorithm requires 1 < k < n_samples.");
+    if (algorithm == KM_HARTIGAN_WONG || algorithm == KM_MACQUEEN)
+        return err->error(da_status_not_implemented,
+                          "Only the Lloyd and Elkan k-means algorithms are implemented by the "
+                          "MI355X build.");
+    if (spherical)
+        return err->error(da_status_not_implemented,
+                          "Spherical (cosine) k-means is not implemented by the MI355X build.");
+    if (k > n_samples_)
+        return err->error(da_status_incompatible_options,
+                          "n_clusters = " + std::to_string(k) + ", and n_samples = " +
+                              std::to_string(n_samples_) + ". Constraint: n_clusters <= n_samples.");
+    if (init == INIT_SUPPLIED && !centres_supplied_)
+        return err->error(da_status_no_data, "The initialization method was set to 'supplied' but "
+                                             "no initial centres have been provided.");
+    if (init == INIT_SUPPLIED && k != n_clusters_)
+        return err->error(da_status_incompatible_options,
+                          "n_clusters was changed after the initial centres were supplied.");
+    n_clusters_ = k;
+    const int n = n_samples_, d = n_features_;
+    hipStream_t st = gpu->stream();
+
+    // Work buffers.
+    const size_t cb = (size_t)k * d * sizeof(double);
+    DA_HIP(err, Ca_.reserve(cb));
+    DA_HIP(err, Cb_.reserve(cb));
+    DA_HIP(err, lab_a_.reserve((size_t)n * sizeof(int)));
+    DA_HIP(err, lab_b_.reserve((size_t)n * sizeof(int)));
+    DA_HIP(err, hipMemsetAsync(lab_a_.ptr(), 0, (size_t)n * sizeof(int), st));
+    DA_HIP(err, hipMemsetAsync(lab_b_.ptr(), 0, (size_t)n * sizeof(int), st));
+    DA_HIP(err, counts_.reserve((size_t)k * sizeof(int)));
+    DA_HIP(err, mind_.reserve((size_t)n * sizeof(double)));
+    DA_HIP(err, small_.reserve(16 * sizeof(double)));
+    DA_HIP(err, tmp_.reserve(4096 * sizeof(double)));
+    DA_HIP(err, idx_.reserve((size_t)std::max(k, 64) * sizeof(int)));
+    DA_HIP(err, w_.reserve((size_t)n * sizeof(double)));
+    DA_HIP(err, cp_.reserve((size_t)std::max(n, 256 * 64) * sizeof(double)));
+    kc_ = std::max<da_int>(1, std::min<da_int>(k, (65536 - 64) / (16 * d + 4)));
+    nblocks_ = std::min<da_int>(512, (n + 255) / 256);
+    ppb_ = (n + nblocks_ - 1) / nblocks_;
+    nblocks_ = (n + ppb_ - 1) / ppb_;
+    DA_HIP(err, part_.reserve((size_t)nblocks_ * kc_ * d * 2 * sizeof(long long)));
+    DA_HIP(err, pcnt_.reserve((size_t)nblocks_ * kc_ * sizeof(int)));
+    Ccur_ = Ca_.as<double>();
+    Cprev_ = Cb_.as<double>();
+    lab_cur_ = lab_a_.as<int>();
+    lab_prev_ = lab_b_.as<int>();
+
+    // initialize_rng, kmeans.cpp:1814-1822.
+    if (init != INIT_SUPPLIED) {
+        if (seed == -1) {
+            std::random_device r;
+            seed = std::abs((da_int)r());
+        }
+        mt_.seed(seed);
+    }
+
+    bool valid = false, warn_maxit = false;
+    best_inertia_ = std::numeric_limits<double>::infinity();
+    best_centres_.clear();
+    for (da_int run = 0; run < n_init; run++) {
+        da_status s = initialise_centres(init);
+        if (s != da_status_success)
+            return s;
+        da_int lp_iter = 0;
+        if (mixed) {
+            // lower_precision_init, kmeans.cpp:1824-1876: a full k-means on the data rounded
+            // to the lower precision (float for double, half for float) from the current
+            // centres rounded likewise; its centres, rounded, seed the working-precision run.
+            const double lp_tol = opts.rval("low precision convergence tolerance");
+            const da_int lp_max = opts.ival("low precision max_iter");
+            if (lp_tol <= tol)
+                return err->error(da_status_incompatible_options,
+                                  "Low precision convergence tolerance must be greater than "
+                                  "convergence tolerance.");
+            const int mode = std::is_same_v<T, double> ? 1 : 2;
+            DA_HIP(err, Xlp_.reserve((size_t)n * d * sizeof(T)));
+            DA_HIP(err, launch_round_data<T>(X_.as<T>(), (int64_t)n * d, mode, Xlp_.as<T>(), st));
+            DA_HIP(err, launch_round_through(Ccur_, (int64_t)k * d, mode, st));
+            RunResult lp;
+            s = run_lloyd<T>(Xlp_.as<T>(), lp_tol, lp_max, lp);
+            if (s != da_status_success)
+                return s;
+            if (lp.empty_found)
+                return err->error(da_status_empty_clusters,
+                                  "All 1 run(s) of the k-means algorithm resulted in at least one "
+                                  "empty cluster during the k-means iterations. A better set of "
+                                  "initial cluster centers is needed.");
+            lp_iter = lp.n_iter;
+            DA_HIP(err, launch_round_through(Ccur_, (int64_t)k * d, mode, st));
+        }
+        RunResult r;
+        s = run_lloyd<T>(X_.as<T>(), tol, max_iter, r);
+        if (s != da_status_success)
+            return s;
+        if (r.empty_found)
+            continue;
+        valid = true;
+        if (r.inertia < best_inertia_ || best_centres_.empty()) {
+            best_inertia_ = r.inertia;
+            best_n_iter_ = r.n_iter;
+            best_lp_n_iter_ = lp_iter;
+            warn_maxit = (r.converged == 0);
+            best_centres_.resize((size_t)k * d);
+            best_labels_.resize(n);
+            std::vector<int> lab(n);
+            DA_HIP(err, hipMemcpyAsync(best_centres_.data(), Ccur_, cb, hipMemcpyDeviceToHost, st));
+            DA_HIP(err, hipMemcpyAsync(lab.data(), lab_cur_, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, st));
+            DA_HIP(err, hipStreamSynchronize(st));
+            std::copy(lab.begin(), lab.end(), best_labels_.begin());
+        }
+    }
+    if (!valid)
+        return err->error(da_status_empty_clusters,
+                          "All " + std::to_string(n_init) +
+                              " run(s) of the k-means algorithm resulted in at least one empty "
+                              "cluster during the k-means iterations. A better set of initial "
+                              "cluster centers is needed.");
+    // The reported centres are in the working precision.
+    for (auto &v : best_centres_)
+        v = (double)(T)v;
+    DA_HIP(err, Cbest_.reserve(cb));
+    DA_HIP(err, hipMemcpyAsync(Cbest_.ptr(), best_centres_.data(), cb, hipMemcpyHostToDevice, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    model_trained = true;
+    if (warn_maxit)
+        return err->warn(da_status_maxit, "The maximum number of iterations was reached.");
+    return status;
+}
+
+template <typename T>
+da_status KMeansHandle<T>::predict(da_int k_samples, da_int k_features, const T *Y, da_int ldy,
+                                   da_int *Y_labels) {
+    if (!model_trained)
+        return err->warn(da_status_no_data, "The k-means has not been computed. Please call "
+                                            "da_kmeans_compute_s or da_kmeans_compute_d.");
+    if (k_features != n_features_)
+        return err->error(da_status_invalid_input,
+                          "The function was called with k_features = " + std::to_string(k_features) +
+                              " but the k-means has been computed with " +
+                              std::to_string(n_features_) + " features.");
+    da_status s = check_2D_array<T>(order(), k_samples, k_features, Y, ldy, "k_samples",
+                                    "k_features", "Y", "ldy");
+    if (s != da_status_success)
+        return s;
+    if (Y_labels == nullptr)
+        return err->error(da_status_invalid_pointer, "The array Y_labels is null.");
+    hipStream_t st = gpu->stream();
+    const size_t bytes = (size_t)k_samples * k_features * sizeof(T);
+    DA_HIP(err, q_.reserve(bytes));
+    if (order() == row_major) {
+        DA_HIP(err, hipMemcpy2DAsync(q_.ptr(), k_features * sizeof(T), Y, (size_t)ldy * sizeof(T),
+                                     k_features * sizeof(T), k_samples, hipMemcpyHostToDevice, st));
+    } else {
+        DA_HIP(err, q_stage_.reserve(bytes));
+        DA_HIP(err, hipMemcpy2DAsync(q_stage_.ptr(), k_samples * sizeof(T), Y, (size_t)ldy * sizeof(T),
+                                     k_samples * sizeof(T), k_features, hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), k_samples, k_samples,
+                                                   k_features, q_.as<T>(), st));
+    }
+    DA_HIP(err, out_.reserve((size_t)k_samples * sizeof(int)));
+    DA_HIP(err, launch_kmeans_assign<T>(q_.as<T>(), k_samples, k_features, Cbest_.as<double>(),
+                                        n_clusters_, out_.as<int>(), nullptr, nullptr, nullptr, st));
+    std::vector<int> lab(k_samples);
+    DA_HIP(err, hipMemcpyAsync(lab.data(), out_.ptr(), k_samples * sizeof(int), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    for (da_int i = 0; i < k_samples; i++)
+        Y_labels[i] = lab[i];
+    return da_status_success;
+}
+
+template <typename T>
+da_status KMeansHandle<T>::transform(da_int m_samples, da_int m_features, const T *X, da_int ldx,
+                                     T *X_transform, da_int ldx_transform) {
+    if (!model_trained)
+        return err->warn(da_status_no_data, "The k-means has not been computed. Please call "
+                                            "da_kmeans_compute_s or da_kmeans_compute_d.");
+    if (m_features != n_features_)
+        return err->error(da_status_invalid_input,
+                          "The function was called with m_features = " + std::to_string(m_features) +
+                              " but the k-means has been computed with " +
+                              std::to_string(n_features_) + " features.");
+    da_status s = check_2D_array<T>(order(), m_samples, m_features, X, ldx, "m_samples",
+                                    "m_features", "X", "ldx");
+    if (s != da_status_success)
+        return s;
+    s = check_2D_array<T>(order(), m_samples, n_clusters_, X_transform, ldx_transform, "m_samples",
+                          "n_clusters", "X_transform", "ldx_transform");
+    if (s != da_status_success)
+        return s;
+    hipStream_t st = gpu->stream();
+    const size_t bytes = (size_t)m_samples * m_features * sizeof(T);
+    DA_HIP(err, q_.reserve(bytes));
+    if (order() == row_major) {
+        DA_HIP(err, hipMemcpy2DAsync(q_.ptr(), m_features * sizeof(T), X, (size_t)ldx * sizeof(T),
+                                     m_features * sizeof(T), m_samples, hipMemcpyHostToDevice, st));
+    } else {
+        DA_HIP(err, q_stage_.reserve(bytes));
+        DA_HIP(err, hipMemcpy2DAsync(q_stage_.ptr(), m_samples * sizeof(T), X, (size_t)ldx * sizeof(T),
+                                     m_samples * sizeof(T), m_features, hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), m_samples, m_samples,
+                                                   m_features, q_.as<T>(), st));
+    }
+    const int k = n_clusters_;
+    const size_t ob = (size_t)m_samples * k * sizeof(T);
+    DA_HIP(err, out_.reserve(2 * ob));
+    DA_HIP(err, launch_kmeans_transform<T>(q_.as<T>(), m_samples, m_features, Cbest_.as<double>(),
+                                           k, out_.as<T>(), k, st));
+    if (order() == row_major) {
+        DA_HIP(err, hipMemcpy2DAsync(X_transform, (size_t)ldx_transform * sizeof(T), out_.ptr(),
+                                     k * sizeof(T), k * sizeof(T), m_samples, hipMemcpyDeviceToHost, st));
+    } else {
+        T *tr = out_.as<T>() + (size_t)m_samples * k;
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(out_.as<T>(), k, k, m_samples, tr, st));
+        DA_HIP(err, hipMemcpy2DAsync(X_transform, (size_t)ldx_transform * sizeof(T), tr,
+                                     m_samples * sizeof(T), m_samples * sizeof(T), k,
+                                     hipMemcpyDeviceToHost, st));
+    }
+    DA_HIP(err, hipStreamSynchronize(st));
+    return da_status_success;
+}
+
+template <typename T>
+da_status KMeansHandle<T>::get_result_typed(da_result query, da_int *dim, T *result) {
+    if (!model_trained)
+        return err->warn(da_status_no_data, "k-means clustering has not yet been computed. Please "
+                                            "call da_kmeans_compute_s or da_kmeans_compute_d "
+                                            "before extracting results.");
+    const da_int k = n_clusters_, d = n_features_;
+    switch (query) {
+    case da_rinfo:
+        if (*dim < 6) {
+            *dim = 6;
+            return err->warn(da_status_invalid_array_dimension,
+                             "The array is too small. Please provide an array of at least size: 6.");
+        }
+        result[0] = (T)n_samples_;
+        result[1] = (T)n_features_;
+        result[2] = (T)k;
+        result[3] = (T)best_n_iter_;
+        result[4] = (T)best_inertia_;
+        result[5] = (T)best_lp_n_iter_;
+        return da_status_success;
+    case da_kmeans_cluster_centres:
+        if (*dim < k * d) {
+            *dim = k * d;
+            return err->warn(da_status_invalid_array_dimension,
+                             "The array is too small. Please provide an array of at least size: " +
+                                 std::to_string(k * d) + ".");
+        }
+        for (da_int i = 0; i < k; i++)
+            for (da_int j = 0; j < d; j++) {
+                const T v = (T)best_centres_[(size_t)i * d + j];
+                if (order() == column_major)
+                    result[i + (size_t)j * k] = v;
+                else
+                    result[(size_t)i * d + j] = v;
+            }
+        return da_status_success;
+    default:
+        return err->warn(da_status_unknown_query, "The requested result could not be found.");
+    }
+}
+
+template <typename T>
+da_status KMeansHandle<T>::get_result(da_result query, da_int *dim, double *result) {
+    if constexpr (std::is_same_v<T, double>)
+        return get_result_typed(query, dim, reinterpret_cast<T *>(result));
+    else
+        return err->error(da_status_wrong_type, "Wrong precision type.");
+}
+template <typename T>
+da_status KMeansHandle<T>::get_result(da_result query, da_int *dim, float *result) {
+    if constexpr (std::is_same_v<T, float>)
+        return get_result_typed(query, dim, reinterpret_cast<T *>(result));
+    else
+        return err->error(da_status_wrong_type, "Wrong precision type.");
+}
+
+template <typename T>
+da_status KMeansHandle<T>::get_result(da_result query, da_int *dim, da_int *result) {
+    if (query == da_trained)
+        return AlgHandle::get_result(query, dim, result);
+    if (!model_trained)
+        return err->warn(da_status_no_data, "k-means clustering has not yet been computed. Please "
+                                            "call da_kmeans_compute_s or da_kmeans_compute_d "
+                                            "before extracting results.");
+    if (query == da_kmeans_labels) {
+        if (*dim < n_samples_) {
+            *dim = n_samples_;
+            return err->warn(da_status_invalid_array_dimension,
+                             "The array is too small. Please provide an array of at least size: " +
+                                 std::to_string(n_samples_) + ".");
+        }
+        for (da_int i = 0; i < n_samples_; i++)
+            result[i] = best_labels_[i];
+        return da_status_success;
+    }
+    return err->warn(da_status_unknown_query, "The requested result could not be found.");
+}
+
+template class KMeansHandle<float>;
+template class KMeansHandle<double>;
+
+std::unique_ptr<AlgHandle> make_kmeans_handle(da_precision p, ErrorStack &err, GpuContext &gpu) {
+    if (p == da_double)
+        return std::make_unique<KMeansHandle<double>>(err, gpu);
+    return std::make_unique<KMeansHandle<float>>(err, gpu);
+}
+
+} // namespace dagpu

@@ -1,0 +1,76 @@
+// k-means handle (the role of da_kmeans::kmeans<T>, source/core/clustering/kmeans/
+// kmeans.hpp:51-312 of the reference) with the data matrix resident in GPU memory.
+#pragma once
+#include "../core/handle.hpp"
+#include <random>
+#include <vector>
+
+namespace dagpu {
+
+template <typename T> class KMeansHandle : public AlgHandle {
+  public:
+    KMeansHandle(ErrorStack &err, GpuContext &gpu);
+    da_status set_data(da_int n_samples, da_int n_features, const T *A, da_int lda);
+    da_status set_init_centres(const T *C, da_int ldc);
+    da_status compute();
+    da_status transform(da_int m_samples, da_int m_features, const T *X, da_int ldx,
+                        T *X_transform, da_int ldx_transform);
+    da_status predict(da_int k_samples, da_int k_features, const T *Y, da_int ldy,
+                      da_int *Y_labels);
+    da_status get_result_typed(da_result query, da_int *dim, T *result);
+    da_status get_result(da_result query, da_int *dim, double *result) override;
+    da_status get_result(da_result query, da_int *dim, float *result) override;
+    da_status get_result(da_result query, da_int *dim, da_int *result) override;
+    void refresh() override;
+
+  private:
+    struct RunResult {
+        da_int n_iter = 0;
+        int converged = 0;
+        bool empty_found = false;
+        double inertia = 0;
+    };
+    da_status upload_data(const T *src, da_int ld, DeviceBuffer &dst);
+    da_status initialise_centres(da_int method);
+    da_status kmeans_plusplus();
+    template <typename U>
+    da_status run_lloyd(const U *dX, double tol, da_int max_iter, RunResult &res);
+    template <typename U>
+    da_status lloyd_step(const U *dX, const double *Cin, double *Cout, int *lab_new,
+                         const int *lab_old, bool update, int &changed, double &normc2,
+                         double &shift2);
+    template <typename U> da_status centres_from_labels(const U *dX, const int *labels, double *C);
+    template <typename U>
+    da_status handle_empty_clusters(const U *dX, double *C, int *labels, bool &split,
+                                    bool &empty_error);
+    template <typename U> da_status prepare_scales_for(const U *dX);
+    da_status upload_rows_double(const T *src, da_int rows, da_int ld, da_order ord,
+                                 DeviceBuffer &dst);
+
+    bool initdone_ = false, centres_supplied_ = false;
+    da_int n_samples_ = 0, n_features_ = 0, n_clusters_ = 1;
+    da_order data_order_ = column_major;
+    const T *A_usr_ = nullptr;
+    da_int lda_usr_ = 0;
+    const T *C_usr_ = nullptr;
+    da_int ldc_usr_ = 0;
+    std::vector<T> C_rowmajor_copy_;
+
+    std::mt19937_64 mt_;
+    // Device state.
+    DeviceBuffer X_, Xlp_, escale_, Ca_, Cb_, lab_a_, lab_b_, counts_, part_, pcnt_, mind_,
+        small_, w_, cp_, tmp_, idx_, Cbest_, q_, q_stage_, out_;
+    da_int ppb_ = 0, nblocks_ = 0, kc_ = 0;
+    std::vector<int> escale_host_;
+    da_int empty_mode_ = 0;
+    double *Ccur_ = nullptr, *Cprev_ = nullptr;
+    int *lab_cur_ = nullptr, *lab_prev_ = nullptr;
+
+    // Results.
+    std::vector<double> best_centres_; // row-major k x d
+    std::vector<da_int> best_labels_;
+    double best_inertia_ = 0;
+    da_int best_n_iter_ = 0, best_lp_n_iter_ = 0;
+};
+
+} // namespace dagpu

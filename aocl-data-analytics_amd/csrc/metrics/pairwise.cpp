@@ -1,0 +1,203 @@
+// da_pairwise_distances_{d,s}: argument checks of
+// source/core/metrics/pairwise_distances.cpp:79-114 and the metric switch of :36-75 of
+// the reference, with the distance tiles computed on the GPU.
+#include "../core/gpu.hpp"
+#include "../kernels/launch.hpp"
+#include "aoclda_metrics.h"
+#include <algorithm>
+#include <cfloat>
+#include <limits>
+#include <mutex>
+
+using namespace dagpu;
+
+namespace {
+
+// Process-wide state of the stateless distance entry points: one stream and cached
+// device buffers, serialised by a mutex.
+struct PairwiseState {
+    std::mutex mu;
+    GpuContext gpu;
+    ErrorStack err;
+    DeviceBuffer stage, xbuf, ybuf, dbuf, nx, ny;
+};
+PairwiseState &state() {
+    static PairwiseState *s = new PairwiseState;
+    return *s;
+}
+
+// Copies an m x k array (given order and leading dimension) to device row-major compact.
+template <typename T>
+da_status upload_rows(ErrorStack *err, hipStream_t st, da_order order, da_int m, da_int k,
+                      const T *A, da_int lda, DeviceBuffer &dst, DeviceBuffer &stage) {
+    const size_t bytes = (size_t)m * k * sizeof(T);
+    DA_HIP(err, dst.reserve(bytes));
+    if (order == row_major) {
+        DA_HIP(err, hipMemcpy2DAsync(dst.ptr(), k * sizeof(T), A, (size_t)lda * sizeof(T),
+                                     k * sizeof(T), m, hipMemcpyHostToDevice, st));
+    } else {
+        DA_HIP(err, stage.reserve(bytes));
+        DA_HIP(err, hipMemcpy2DAsync(stage.ptr(), m * sizeof(T), A, (size_t)lda * sizeof(T),
+                                     m * sizeof(T), k, hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(stage.as<T>(), m, m, k, dst.as<T>(), st));
+    }
+    return da_status_success;
+}
+
+template <typename T>
+da_status pairwise(da_order order, da_int m, da_int n, da_int k, const T *X, da_int ldx,
+                   const T *Y, da_int ldy, T *D, da_int ldd, T p, da_metric metric) {
+    // Argument checks in the order of pairwise_distances.cpp:85-111.
+    if (m < 1 || k < 1)
+        return da_status_invalid_array_dimension;
+    if (order == column_major && ldx < m)
+        return da_status_invalid_leading_dimension;
+    if (order == row_major && ldx < k)
+        return da_status_invalid_leading_dimension;
+    if (X == nullptr || D == nullptr)
+        return da_status_invalid_pointer;
+    if (Y != nullptr) {
+        if (n < 1)
+            return da_status_invalid_array_dimension;
+        if (order == column_major) {
+            if (ldy < n || ldd < m)
+                return da_status_invalid_leading_dimension;
+        } else {
+            if (ldy < k || ldd < n)
+                return da_status_invalid_leading_dimension;
+        }
+    } else {
+        if (ldd < m)
+            return da_status_invalid_leading_dimension;
+    }
+    if (metric == da_minkowski && p <= 0)
+        return da_status_invalid_input;
+
+    int km;
+    switch (metric) {
+    case da_euclidean:
+        km = KM_EUCLIDEAN;
+        break;
+    case da_sqeuclidean:
+        km = KM_SQEUCLIDEAN;
+        break;
+    case da_manhattan:
+        km = KM_MANHATTAN;
+        break;
+    case da_cosine:
+        km = KM_COSINE;
+        break;
+    case da_minkowski:
+        km = (p == (T)2.0) ? KM_EUCLIDEAN : (p == (T)1.0) ? KM_MANHATTAN : KM_MINKOWSKI;
+        break;
+    case da_euclidean_gemm:
+        km = KM_EUCLIDEAN_GEMM;
+        break;
+    case da_sqeuclidean_gemm:
+        km = KM_SQEUCLIDEAN_GEMM;
+        break;
+    case da_inner_product:
+        km = KM_INNER;
+        break;
+    default:
+        return da_status_not_implemented;
+    }
+
+    PairwiseState &S = state();
+    std::lock_guard<std::mutex> lock(S.mu);
+    ErrorStack *err = &S.err;
+    err->clear();
+    da_status st = S.gpu.init(err);
+    if (st != da_status_success)
+        return st;
+    hipStream_t s = S.gpu.stream();
+
+    const bool x_is_y = (Y == nullptr);
+    if (x_is_y) {
+        n = m;
+        ldy = ldx;
+    }
+    st = upload_rows<T>(err, s, order, m, k, X, ldx, S.xbuf, S.stage);
+    if (st != da_status_success)
+        return st;
+    const T *dX = S.xbuf.as<T>();
+    const T *dY = dX;
+    if (!x_is_y) {
+        st = upload_rows<T>(err, s, order, n, k, Y, ldy, S.ybuf, S.stage);
+        if (st != da_status_success)
+            return st;
+        dY = S.ybuf.as<T>();
+    }
+
+    double *dnx = nullptr, *dny = nullptr;
+    if (km == KM_COSINE || km == KM_EUCLIDEAN_GEMM || km == KM_SQEUCLIDEAN_GEMM) {
+        DA_HIP(err, S.nx.reserve((size_t)m * sizeof(double)));
+        DA_HIP(err, launch_row_sqnorms<T>(dX, k, m, k, S.nx.as<double>(), s));
+        dnx = S.nx.as<double>();
+        if (x_is_y) {
+            dny = dnx;
+        } else {
+            DA_HIP(err, S.ny.reserve((size_t)n * sizeof(double)));
+            DA_HIP(err, launch_row_sqnorms<T>(dY, k, n, k, S.ny.as<double>(), s));
+            dny = S.ny.as<double>();
+        }
+        if (km == KM_COSINE) {
+            const double eps = std::numeric_limits<T>::epsilon();
+            DA_HIP(err, launch_cosine_norms(dnx, m, eps, s));
+            if (!x_is_y)
+                DA_HIP(err, launch_cosine_norms(dny, n, eps, s));
+        }
+    }
+    // The reference forces a zero diagonal for cosine (cosine_distance.cpp:57-61) and for
+    // the GEMM form of X against itself (euclidean_gemm_distance.cpp:196-226); for the
+    // direct forms the diagonal is exactly zero already.
+    const int zero_diag = x_is_y && (km == KM_COSINE || km == KM_EUCLIDEAN_GEMM ||
+                                     km == KM_SQEUCLIDEAN_GEMM);
+
+    // Row-major output is computed as rows of X against Y. Column-major output is the
+    // transposed problem (rows of Y against X, written row by row), which keeps the device
+    // stores contiguous; every metric here is symmetric in its two arguments.
+    const bool colmaj = (order == column_major);
+    const T *A = colmaj ? dY : dX;
+    const T *B = colmaj ? dX : dY;
+    const double *na = colmaj ? dny : dnx;
+    const double *nb = colmaj ? dnx : dny;
+    const da_int ma = colmaj ? n : m;
+    const da_int nb_cols = colmaj ? m : n;
+
+    // Output is produced in row chunks that fit a bounded device buffer.
+    const size_t row_bytes = (size_t)nb_cols * sizeof(T);
+    const size_t max_bytes = (size_t)1 << 30;
+    da_int chunk = (da_int)std::max<size_t>(1, std::min<size_t>(ma, max_bytes / row_bytes));
+    DA_HIP(err, S.dbuf.reserve((size_t)chunk * row_bytes));
+    for (da_int r0 = 0; r0 < ma; r0 += chunk) {
+        const da_int rows = std::min(chunk, ma - r0);
+        DA_HIP(err, (launch_pairwise_direct<T, T>(km, A + (size_t)r0 * k, B, rows, nb_cols, k,
+                                                  S.dbuf.as<T>(), nb_cols,
+                                                  na ? na + r0 : nullptr, nb, (double)p,
+                                                  zero_diag, (int)r0, s)));
+        DA_HIP(err, hipMemcpy2DAsync(D + (size_t)r0 * ldd, (size_t)ldd * sizeof(T),
+                                     S.dbuf.ptr(), row_bytes, row_bytes, rows,
+                                     hipMemcpyDeviceToHost, s));
+    }
+    DA_HIP(err, hipStreamSynchronize(s));
+    return da_status_success;
+}
+
+} // namespace
+
+extern "C" {
+
+da_status da_pairwise_distances_d(da_order order, da_int m, da_int n, da_int k,
+                                  const double *X, da_int ldx, const double *Y, da_int ldy,
+                                  double *D, da_int ldd, double p, da_metric metric) {
+    return pairwise<double>(order, m, n, k, X, ldx, Y, ldy, D, ldd, p, metric);
+}
+
+da_status da_pairwise_distances_s(da_order order, da_int m, da_int n, da_int k, const float *X,
+                                  da_int ldx, const float *Y, da_int ldy, float *D, da_int ldd,
+                                  float p, da_metric metric) {
+    return pairwise<float>(order, m, n, k, X, ldx, Y, ldy, D, ldd, p, metric);
+}
+
+} // extern "C"

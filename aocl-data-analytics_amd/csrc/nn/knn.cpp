@@ -1,0 +1,501 @@
+#include "knn.hpp"
+#include "../core/factories.hpp"
+#include "../kernels/launch.hpp"
+#include "aoclda_metrics.h"
+#include <algorithm>
+#include <cmath>
+#include <vector>
+#include <limits>
+
+namespace dagpu {
+
+// Internal algorithm codes, nearest_neighbors_types.hpp:29 of the reference.
+enum { NN_BRUTE = 0, NN_KD_TREE = 1, NN_BALL_TREE = 2, NN_AUTO = 3 };
+
+template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgHandle(e, g) {
+    // Options of source/core/nearest_neighbors/nearest_neighbors_options.hpp:52-124.
+    const da_int imax = std::numeric_limits<da_int>::max();
+    const da_int imin = std::numeric_limits<da_int>::min();
+    const double fmax = (double)std::numeric_limits<T>::max();
+    const double fmin = (double)std::numeric_limits<T>::lowest();
+    opts.add_int("number of neighbors", "Number of neighbors considered for k-nearest neighbors.",
+                 1, LowerBound::greater_equal, imax, UpperBound::p_inf, 5);
+    opts.add_int("leaf size", "Leaf size for k-d tree.", 1, LowerBound::greater_equal, imax,
+                 UpperBound::p_inf, 30);
+    opts.add_int("outlier label",
+                 "Classification label for queries with no neighbors within the specified radius.",
+                 imin, LowerBound::m_inf, imax, UpperBound::p_inf, 0);
+    opts.add_real("minkowski parameter",
+                  "Minkowski parameter for metric used for the computation of nearest neighbors.",
+                  0.0, LowerBound::greater_than, fmax, UpperBound::p_inf, 2.0);
+    opts.add_real("radius", "Maximum distance for the radius neighbors computation.", 0.0,
+                  LowerBound::greater_equal, fmax, UpperBound::p_inf, 1.0);
+    opts.add_real("outlier target",
+                  "Regression target for queries with no neighbors within the specified radius.",
+                  fmin, LowerBound::m_inf, fmax, UpperBound::p_inf, 0.0);
+    opts.add_string("algorithm", "Algorithm used to compute the k-nearest neighbors.",
+                    {{"auto", NN_AUTO}, {"brute", NN_BRUTE}, {"kd tree", NN_KD_TREE},
+                     {"ball tree", NN_BALL_TREE}},
+                    "auto");
+    opts.add_string("metric", "Metric used to compute the pairwise distance matrix.",
+                    {{"euclidean", da_euclidean},
+                     {"l2", da_l2},
+                     {"sqeuclidean", da_sqeuclidean},
+                     {"manhattan", da_manhattan},
+                     {"l1", da_l1},
+                     {"cityblock", da_cityblock},
+                     {"cosine", da_cosine},
+                     {"minkowski", da_minkowski},
+                     {"sqeuclidean_gemm", da_sqeuclidean_gemm},
+                     {"euclidean_gemm", da_euclidean_gemm},
+                     {"inner product", da_inner_product}},
+                    "euclidean");
+    opts.add_string("weights", "Weight function used to compute the k-nearest neighbors.",
+                    {{"uniform", 0}, {"distance", 1}}, "uniform");
+    opts.add_string("outlier handling",
+                    "How to handle samples with no neighbors within the specified radius.",
+                    {{"none", 0}, {"manual", 1}, {"most frequent", 2}}, "none");
+}
+
+template <typename T> da_status NNHandle<T>::set_params() {
+    n_neighbors_ = opts.ival("number of neighbors");
+    algo_ = opts.skey("algorithm");
+    metric_ = opts.skey("metric");
+    p_ = opts.rval("minkowski parameter");
+    leaf_size_ = opts.ival("leaf size");
+    // Tree algorithms are incompatible with these metrics (nearest_neighbors.cpp:313-329).
+    if (algo_ == NN_KD_TREE || algo_ == NN_BALL_TREE) {
+        if (metric_ == da_cosine || metric_ == da_sqeuclidean || metric_ == da_sqeuclidean_gemm ||
+            metric_ == da_inner_product)
+            return err->error(da_status_incompatible_options,
+                              "The tree algorithms are not compatible with the cosine, "
+                              "squared Euclidean, or inner product distances.");
+        if (metric_ == da_minkowski && p_ < 1.0)
+            return err->error(da_status_incompatible_options,
+                              "Tree algorithms are not compatible with the Minkowski metric "
+                              "when 0 < p < 1.");
+    }
+    return da_status_success;
+}
+
+// nearest_neighbors.cpp:394-466: options that shaped set_data may not change afterwards.
+template <typename T> da_status NNHandle<T>::check_options_update() {
+    const da_int a = opts.skey("algorithm");
+    if (a == NN_AUTO || a == NN_KD_TREE || a == NN_BALL_TREE) {
+        if (a != algo_ || opts.ival("leaf size") != leaf_size_)
+            return err->error(da_status_option_locked,
+                              "Options need to be set before calling set_training_data().");
+        if (a == NN_AUTO &&
+            (opts.skey("metric") != metric_ || opts.rval("minkowski parameter") != p_))
+            return err->error(da_status_option_locked,
+                              "Options need to be set before calling set_training_data().");
+    }
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::set_data(da_int n_samples, da_int n_features, const T *X, da_int ldx) {
+    da_status s = check_2D_array<T>(order(), n_samples, n_features, X, ldx, "n_samples",
+                                    "n_features", "X_train", "ldx_train");
+    if (s != da_status_success)
+        return s;
+    s = gpu->init(err);
+    if (s != da_status_success)
+        return s;
+    s = set_params();
+    if (s != da_status_success)
+        return s;
+    hipStream_t st = gpu->stream();
+    const size_t bytes = (size_t)n_samples * n_features * sizeof(T);
+    DA_HIP(err, train_.reserve(bytes));
+    if (order() == row_major) {
+        DA_HIP(err, hipMemcpy2DAsync(train_.ptr(), n_features * sizeof(T), X,
+                                     (size_t)ldx * sizeof(T), n_features * sizeof(T), n_samples,
+                                     hipMemcpyHostToDevice, st));
+    } else {
+        DeviceBuffer stage;
+        DA_HIP(err, stage.reserve(bytes));
+        DA_HIP(err, hipMemcpy2DAsync(stage.ptr(), n_samples * sizeof(T), X,
+                                     (size_t)ldx * sizeof(T), n_samples * sizeof(T), n_features,
+                                     hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(stage.as<T>(), n_samples, n_samples,
+                                                   n_features, train_.as<T>(), st));
+        DA_HIP(err, hipStreamSynchronize(st));
+    }
+    DA_HIP(err, train_sq_.reserve((size_t)n_samples * sizeof(double)));
+    DA_HIP(err, launch_row_sqnorms<T>(train_.as<T>(), n_features, n_samples, n_features,
+                                      train_sq_.as<double>(), st));
+    DA_HIP(err, train_cos_.reserve((size_t)n_samples * sizeof(double)));
+    DA_HIP(err, hipMemcpyAsync(train_cos_.ptr(), train_sq_.ptr(), n_samples * sizeof(double),
+                               hipMemcpyDeviceToDevice, st));
+    DA_HIP(err, launch_cosine_norms(train_cos_.as<double>(), n_samples,
+                                    std::numeric_limits<T>::epsilon(), st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    n_samples_ = n_samples;
+    n_features_ = n_features;
+    trained_ = true;
+    fused_ready_ = false;
+    return prepare_fused();
+}
+
+// Squared-euclidean metrics are searched by the fused MFMA kernel (fp32 data, up to 128
+// features, k up to 16); everything else uses the key-tile + selection path.
+template <typename T> bool NNHandle<T>::fused_eligible(da_int k) const {
+    if (!std::is_same_v<T, float> || !fused_ready_)
+        return false;
+    const bool sq = metric_ == da_euclidean || metric_ == da_sqeuclidean ||
+                    metric_ == da_euclidean_gemm || metric_ == da_sqeuclidean_gemm ||
+                    (metric_ == da_minkowski && p_ == 2.0);
+    return sq && k <= 16 && n_features_ <= 128 && n_samples_ >= 16;
+}
+
+template <typename T> da_status NNHandle<T>::prepare_fused() {
+    if constexpr (std::is_same_v<T, float>) {
+        if (n_features_ > 128)
+            return da_status_success;
+        hipStream_t st = gpu->stream();
+        kp_ = (n_features_ + 15) / 16 * 16;
+        DA_HIP(err, ypad_.reserve((size_t)n_samples_ * kp_ * sizeof(float)));
+        DA_HIP(err, (launch_pack_rows<float, float>(train_.as<float>(), n_features_, n_samples_,
+                                                    n_features_, ypad_.as<float>(), kp_,
+                                                    n_samples_, st)));
+        std::vector<double> sq(n_samples_);
+        DA_HIP(err, hipMemcpyAsync(sq.data(), train_sq_.ptr(), n_samples_ * sizeof(double),
+                                   hipMemcpyDeviceToHost, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        std::vector<float> sq32(n_samples_);
+        double mx = 0;
+        for (da_int i = 0; i < n_samples_; i++) {
+            sq32[i] = (float)sq[i];
+            mx = std::max(mx, sq[i]);
+        }
+        ymax_ = std::sqrt(mx);
+        DA_HIP(err, ny32_.reserve(n_samples_ * sizeof(float)));
+        DA_HIP(err, hipMemcpyAsync(ny32_.ptr(), sq32.data(), n_samples_ * sizeof(float),
+                                   hipMemcpyHostToDevice, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        fused_ready_ = true;
+    }
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::search_fused(const T *dQ, da_int nq, da_int k, bool return_distance,
+                                    da_int *d_ind, T *d_dist) {
+    if constexpr (!std::is_same_v<T, float>) {
+        return da_status_internal_error;
+    } else {
+        hipStream_t st = gpu->stream();
+        const int n = n_samples_, d = n_features_;
+        const int kc = knn_tile_kc(k);
+        // Split the training rows so that the launch has about four workgroups per CU.
+        const int qblocks = (nq + 127) / 128;
+        int nsplit = (1024 + qblocks - 1) / qblocks;
+        nsplit = std::max(1, std::min({nsplit, 256 / kc, (n + 1023) / 1024}));
+        DA_HIP(err, qpad_.reserve((size_t)nq * kp_ * sizeof(float)));
+        DA_HIP(err, (launch_pack_rows<float, float>(dQ, d, nq, d, qpad_.as<float>(), kp_, nq, st)));
+        const size_t np = (size_t)nq * nsplit;
+        DA_HIP(err, part_key_.reserve(np * kc * sizeof(float)));
+        DA_HIP(err, part_idx_.reserve(np * kc * sizeof(int)));
+        DA_HIP(err, part_tau_.reserve(np * sizeof(float)));
+        DA_HIP(err, fb_.reserve((size_t)(nq + 1) * sizeof(int)));
+        {
+            KernelScope ks(*gpu, "knn_tile");
+            DA_HIP(err, launch_knn_tile_f32(qpad_.as<float>(), nq, ypad_.as<float>(),
+                                            ny32_.as<float>(), n, kp_, nsplit, kc,
+                                            part_key_.as<float>(), part_idx_.as<int>(),
+                                            part_tau_.as<float>(), st));
+        }
+        int *fbc = fb_.as<int>();
+        DA_HIP(err, hipMemsetAsync(fbc, 0, sizeof(int), st));
+        const int post = (metric_ == da_sqeuclidean || metric_ == da_sqeuclidean_gemm) ? 0 : 1;
+        {
+            KernelScope ks(*gpu, "knn_merge");
+            DA_HIP(err, launch_knn_merge<float>(part_key_.as<float>(), part_idx_.as<int>(),
+                                                part_tau_.as<float>(), nsplit, kc, dQ,
+                                                train_.as<float>(), nq, d, k, std::ldexp(1.0, -24),
+                                                ymax_, post, return_distance ? d_dist : nullptr,
+                                                d_ind, fbc, fbc + 1, st));
+        }
+        int nfb = 0;
+        DA_HIP(err, hipMemcpyAsync(&nfb, fbc, sizeof(int), hipMemcpyDeviceToHost, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        last_candidates = nsplit * kc;
+        last_fallback = nfb;
+        if (nfb > 0) {
+            // Exact fallback for the uncertified queries: gather, key tiles + selection,
+            // scatter back.
+            std::vector<int> list(nfb);
+            DA_HIP(err, hipMemcpyAsync(list.data(), fbc + 1, nfb * sizeof(int),
+                                       hipMemcpyDeviceToHost, st));
+            DA_HIP(err, hipStreamSynchronize(st));
+            DA_HIP(err, fb_q_.reserve((size_t)nfb * d * sizeof(T)));
+            for (int i = 0; i < nfb; i++)
+                DA_HIP(err, hipMemcpyAsync(fb_q_.as<T>() + (size_t)i * d, dQ + (size_t)list[i] * d,
+                                           d * sizeof(T), hipMemcpyDeviceToDevice, st));
+            DA_HIP(err, fb_ind_.reserve((size_t)nfb * k * sizeof(da_int)));
+            DA_HIP(err, fb_dist_.reserve((size_t)nfb * k * sizeof(T)));
+            da_status s = search_tiles(fb_q_.as<T>(), nfb, k, true, fb_ind_.as<da_int>(),
+                                       fb_dist_.as<T>());
+            if (s != da_status_success)
+                return s;
+            for (int i = 0; i < nfb; i++) {
+                DA_HIP(err, hipMemcpyAsync(d_ind + (size_t)list[i] * k,
+                                           fb_ind_.as<da_int>() + (size_t)i * k,
+                                           k * sizeof(da_int), hipMemcpyDeviceToDevice, st));
+                if (return_distance)
+                    DA_HIP(err, hipMemcpyAsync(d_dist + (size_t)list[i] * k,
+                                               fb_dist_.as<T>() + (size_t)i * k, k * sizeof(T),
+                                               hipMemcpyDeviceToDevice, st));
+            }
+        }
+        return da_status_success;
+    }
+}
+
+template <typename T> da_status NNHandle<T>::resolve_k(da_int &k) {
+    if (k <= 0 && n_neighbors_ <= 0)
+        return err->error(da_status_invalid_input,
+                          "Number of requested neighbors must be positive.");
+    if (k <= 0)
+        k = opts.ival("number of neighbors");
+    if (k > n_samples_)
+        return err->error(da_status_invalid_input,
+                          "Number of requested neighbors must be at least as big as the "
+                          "number of samples.");
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::kneighbors(da_int n_queries, da_int n_features, const T *X_test,
+                                  da_int ldx_test, da_int *n_ind, T *n_dist, da_int k,
+                                  da_int return_distance) {
+    // Checks in the order of nearest_neighbors.cpp:939-995 and :61-106.
+    if (!trained_)
+        return err->error(da_status_no_data, "No training data have been set. Please call "
+                                             "da_nn_set_data_s or da_nn_set_data_d.");
+    da_status s = check_options_update();
+    if (s != da_status_success)
+        return s;
+    if (n_ind == nullptr)
+        return err->error(da_status_invalid_pointer, "n_ind is not a valid pointer.");
+    if (X_test == nullptr)
+        return err->error(da_status_invalid_pointer, "X_test is not a valid pointer.");
+    if (n_features != n_features_)
+        return err->error(da_status_invalid_array_dimension,
+                          "n_features = " + std::to_string(n_features) +
+                              " doesn't match the expected value " +
+                              std::to_string(n_features_) + ".");
+    s = check_2D_array<T>(order(), n_queries, n_features, X_test, ldx_test, "n_queries",
+                          "n_features", "X_test", "ldx_test");
+    if (s != da_status_success)
+        return s;
+    s = resolve_k(k);
+    if (s != da_status_success)
+        return s;
+    if (return_distance && n_dist == nullptr)
+        return err->error(da_status_invalid_pointer, "n_dist is not a valid pointer.");
+
+    hipStream_t st = gpu->stream();
+    const size_t qbytes = (size_t)n_queries * n_features * sizeof(T);
+    DA_HIP(err, q_.reserve(qbytes));
+    if (order() == row_major) {
+        DA_HIP(err, hipMemcpy2DAsync(q_.ptr(), n_features * sizeof(T), X_test,
+                                     (size_t)ldx_test * sizeof(T), n_features * sizeof(T),
+                                     n_queries, hipMemcpyHostToDevice, st));
+    } else {
+        DA_HIP(err, q_stage_.reserve(qbytes));
+        DA_HIP(err, hipMemcpy2DAsync(q_stage_.ptr(), n_queries * sizeof(T), X_test,
+                                     (size_t)ldx_test * sizeof(T), n_queries * sizeof(T),
+                                     n_features, hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), n_queries, n_queries,
+                                                   n_features, q_.as<T>(), st));
+    }
+    const size_t ob = (size_t)n_queries * k;
+    DA_HIP(err, out_ind_.reserve(ob * sizeof(da_int)));
+    DA_HIP(err, out_dist_.reserve(ob * sizeof(T)));
+    s = search(q_.as<T>(), n_queries, k, return_distance != 0, out_ind_.as<da_int>(),
+               return_distance ? out_dist_.as<T>() : nullptr);
+    if (s != da_status_success)
+        return s;
+    // Output is n_queries x k in the handle's storage order (nearest_neighbors.cpp:997-1009).
+    if (order() == row_major) {
+        DA_HIP(err, hipMemcpyAsync(n_ind, out_ind_.ptr(), ob * sizeof(da_int),
+                                   hipMemcpyDeviceToHost, st));
+        if (return_distance)
+            DA_HIP(err, hipMemcpyAsync(n_dist, out_dist_.ptr(), ob * sizeof(T),
+                                       hipMemcpyDeviceToHost, st));
+    } else {
+        // Transpose on the device (k x n_queries row-major == n_queries x k column-major).
+        DA_HIP(err, q_stage_.reserve(ob * std::max(sizeof(T), sizeof(da_int))));
+        DA_HIP(err, launch_colmajor_to_rowmajor<float>(
+                        reinterpret_cast<const float *>(out_ind_.ptr()), k, k, n_queries,
+                        q_stage_.as<float>(), st));
+        DA_HIP(err, hipMemcpyAsync(n_ind, q_stage_.ptr(), ob * sizeof(da_int),
+                                   hipMemcpyDeviceToHost, st));
+        if (return_distance) {
+            DA_HIP(err, hipStreamSynchronize(st));
+            DA_HIP(err, launch_colmajor_to_rowmajor<T>(out_dist_.as<T>(), k, k, n_queries,
+                                                       q_stage_.as<T>(), st));
+            DA_HIP(err, hipMemcpyAsync(n_dist, q_stage_.ptr(), ob * sizeof(T),
+                                       hipMemcpyDeviceToHost, st));
+        }
+    }
+    DA_HIP(err, hipStreamSynchronize(st));
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::kneighbors_device(da_int n_queries, da_int n_features, const T *d_X_test,
+                                         da_int ldx_test, da_int *d_ind, T *d_dist, da_int k,
+                                         da_int return_distance) {
+    if (!trained_)
+        return err->error(da_status_no_data, "No training data have been set.");
+    da_status s = check_options_update();
+    if (s != da_status_success)
+        return s;
+    if (d_ind == nullptr || d_X_test == nullptr || (return_distance && d_dist == nullptr))
+        return err->error(da_status_invalid_pointer, "A device pointer is null.");
+    if (n_features != n_features_)
+        return err->error(da_status_invalid_array_dimension,
+                          "n_features doesn't match the training data.");
+    if (n_queries < 1)
+        return err->error(da_status_invalid_array_dimension, "n_queries must be >= 1.");
+    if (ldx_test < n_features)
+        return err->error(da_status_invalid_leading_dimension, "ldx_test < n_features.");
+    s = resolve_k(k);
+    if (s != da_status_success)
+        return s;
+    const T *dq = d_X_test;
+    if (ldx_test != n_features) {
+        DA_HIP(err, q_.reserve((size_t)n_queries * n_features * sizeof(T)));
+        DA_HIP(err, hipMemcpy2DAsync(q_.ptr(), n_features * sizeof(T), d_X_test,
+                                     (size_t)ldx_test * sizeof(T), n_features * sizeof(T),
+                                     n_queries, hipMemcpyDeviceToDevice, gpu->stream()));
+        dq = q_.as<T>();
+    }
+    return search(dq, n_queries, k, return_distance != 0, d_ind,
+                  return_distance ? d_dist : nullptr);
+}
+
+template <typename T>
+da_status NNHandle<T>::search(const T *dQ, da_int nq, da_int k, bool return_distance,
+                              da_int *d_ind, T *d_dist) {
+    last_fallback = 0;
+    last_candidates = 0;
+    if (fused_eligible(k))
+        return search_fused(dQ, nq, k, return_distance, d_ind, d_dist);
+    return search_tiles(dQ, nq, k, return_distance, d_ind, d_dist);
+}
+
+// Generic path for every metric: key tiles (queries x training points, in double) followed
+// by the per-row selection, the structure of kneighbors_compute_brute_force
+// (nearest_neighbors.cpp:567-816) with the blocks sized for HBM instead of caches.
+template <typename T>
+da_status NNHandle<T>::search_tiles(const T *dQ, da_int nq, da_int k, bool return_distance,
+                                    da_int *d_ind, T *d_dist) {
+    hipStream_t st = gpu->stream();
+    const da_int d = n_features_, n = n_samples_;
+    if (k > 3840)
+        return err->error(da_status_not_implemented,
+                          "k > 3840 neighbours is not supported by the GPU selection kernel.");
+    int km, post = 0, largest = 0;
+    const double *nq_norm = nullptr, *nt_norm = nullptr;
+    double p = p_;
+    bool need_qcos = false, need_qsq = false;
+    switch (metric_) {
+    case da_euclidean:
+        km = KM_SQEUCLIDEAN;
+        post = 1;
+        break;
+    case da_sqeuclidean:
+        km = KM_SQEUCLIDEAN;
+        break;
+    case da_euclidean_gemm:
+        km = KM_SQEUCLIDEAN_GEMM;
+        post = 1;
+        need_qsq = true;
+        break;
+    case da_sqeuclidean_gemm:
+        km = KM_SQEUCLIDEAN_GEMM;
+        need_qsq = true;
+        break;
+    case da_manhattan:
+        km = KM_MANHATTAN;
+        break;
+    case da_cosine:
+        km = KM_COSINE;
+        need_qcos = true;
+        break;
+    case da_inner_product:
+        km = KM_INNER;
+        largest = 1;
+        break;
+    case da_minkowski:
+        if (p == 2.0) {
+            km = KM_SQEUCLIDEAN;
+            post = 1;
+        } else if (p == 1.0) {
+            km = KM_MANHATTAN;
+        } else {
+            km = KM_MINKOWSKI_KEY;
+            post = 2;
+        }
+        break;
+    default:
+        return err->error(da_status_not_implemented, "Unknown metric.");
+    }
+    if (need_qsq || need_qcos) {
+        DA_HIP(err, q_norm_.reserve((size_t)nq * sizeof(double)));
+        DA_HIP(err, launch_row_sqnorms<T>(dQ, d, nq, d, q_norm_.as<double>(), st));
+        if (need_qcos) {
+            DA_HIP(err, launch_cosine_norms(q_norm_.as<double>(), nq,
+                                            std::numeric_limits<T>::epsilon(), st));
+            nt_norm = train_cos_.as<double>();
+        } else {
+            nt_norm = train_sq_.as<double>();
+        }
+        nq_norm = q_norm_.as<double>();
+    }
+    const size_t max_keys = (size_t)64 << 20; // 512 MB of double keys per chunk
+    da_int chunk = (da_int)std::max<size_t>(1, std::min<size_t>(nq, max_keys / (size_t)n));
+    DA_HIP(err, keys_.reserve((size_t)chunk * n * sizeof(double)));
+    DA_HIP(err, sel_key_.reserve((size_t)chunk * k * sizeof(double)));
+    DA_HIP(err, sel_idx_.reserve((size_t)chunk * k * sizeof(int)));
+    for (da_int q0 = 0; q0 < nq; q0 += chunk) {
+        const da_int rows = std::min(chunk, nq - q0);
+        {
+            KernelScope ks(*gpu, "pairwise_direct");
+            DA_HIP(err, (launch_pairwise_direct<T, double>(
+                            km, dQ + (size_t)q0 * d, train_.as<T>(), rows, n, d,
+                            keys_.as<double>(), n, nq_norm ? nq_norm + q0 : nullptr, nt_norm, p,
+                            0, 0, st)));
+        }
+        {
+            KernelScope ks(*gpu, "row_topk");
+            DA_HIP(err, launch_row_topk<double>(keys_.as<double>(), n, rows, n, k, largest,
+                                                sel_key_.as<double>(), sel_idx_.as<int>(), st));
+        }
+        DA_HIP(err, launch_knn_finalize<T>(sel_key_.as<double>(), sel_idx_.as<int>(), rows, k,
+                                           post, p,
+                                           return_distance ? d_dist + (size_t)q0 * k : nullptr,
+                                           d_ind + (size_t)q0 * k, k, st));
+    }
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::get_result(da_result query, da_int *dim, da_int *result) {
+    return AlgHandle::get_result(query, dim, result);
+}
+
+template class NNHandle<float>;
+template class NNHandle<double>;
+
+std::unique_ptr<AlgHandle> make_nn_handle(da_precision p, ErrorStack &err, GpuContext &gpu) {
+    if (p == da_double)
+        return std::make_unique<NNHandle<double>>(err, gpu);
+    return std::make_unique<NNHandle<float>>(err, gpu);
+}
+
+} // namespace dagpu

@@ -1,0 +1,84 @@
+"""ctypes loader of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker. The product library never uses it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+METRICS = {"euclidean": 0, "sqeuclidean": 1, "minkowski": 2, "manhattan": 3, "cosine": 4,
+           "euclidean_gemm": 5, "sqeuclidean_gemm": 6, "inner_product": 7}
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+if not os.path.exists(LIB):
+    build()
+_lib = C.CDLL(LIB)
+_d = C.POINTER(C.c_double)
+_i = C.POINTER(C.c_int)
+_lib.oracle_pairwise.restype = C.c_int
+_lib.oracle_pairwise.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _d, C.c_int, _d, C.c_int,
+                                 _d, C.c_int, C.c_double, C.c_int, C.c_double]
+_lib.oracle_knn.restype = C.c_int
+_lib.oracle_knn.argtypes = [C.c_int, C.c_int, _d, C.c_int, _d, C.c_int, C.c_int, C.c_double,
+                            _i, _d, C.c_double]
+_lib.oracle_kmeans.restype = C.c_int
+_lib.oracle_kmeans.argtypes = [C.c_int, C.c_int, _d, C.c_int, C.c_int, _d, C.c_int, C.c_int,
+                               C.c_double, C.c_longlong, C.c_int, _d, _i, _d]
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def pairwise(X, Y=None, metric="euclidean", p=2.0, eps=np.finfo(np.float64).eps):
+    """Distance matrix (row-major), restating da_pairwise_distances in double."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    m, k = X.shape
+    Yc = None if Y is None else np.ascontiguousarray(Y, dtype=np.float64)
+    n = m if Yc is None else Yc.shape[0]
+    D = np.empty((m, n))
+    _lib.oracle_pairwise(0, m, n, k, _p(X), k, _p(Yc), k, _p(D), n, float(p), METRICS[metric],
+                         float(eps))
+    return D
+
+
+def knn(Xtrain, Xq, k, metric="euclidean", p=2.0, eps=np.finfo(np.float64).eps):
+    """(indices, distances) of the k nearest training rows per query, ordered by
+    (distance, index)."""
+    Xt = np.ascontiguousarray(Xtrain, dtype=np.float64)
+    Q = np.ascontiguousarray(Xq, dtype=np.float64)
+    ind = np.empty((Q.shape[0], k), dtype=np.int32)
+    dist = np.empty((Q.shape[0], k))
+    _lib.oracle_knn(Xt.shape[0], Xt.shape[1], _p(Xt), Q.shape[0], _p(Q), k, METRICS[metric],
+                    float(p), _p(ind, _i), _p(dist), float(eps))
+    return ind, dist
+
+
+INIT = {"random": 0, "k-means++": 1, "supplied": 2, "random partitions": 3}
+EMPTY = {"ignore": 0, "error": 1, "split": 2}
+
+
+def kmeans(X, n_clusters, init="k-means++", C=None, n_init=1, max_iter=300, tol=1e-4, seed=0,
+           empty_clusters="ignore"):
+    """Returns dict(status, centres, labels, n_iter, inertia) of the restated k-means."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, d = X.shape
+    Ci = None if C is None else np.ascontiguousarray(C, dtype=np.float64)
+    Co = np.zeros((n_clusters, d))
+    lab = np.zeros(n, dtype=np.int32)
+    info = np.zeros(2)
+    st = _lib.oracle_kmeans(n, d, _p(X), n_clusters, INIT[init], _p(Ci), n_init, max_iter,
+                            float(tol), int(seed), EMPTY[empty_clusters], _p(Co), _p(lab, _i),
+                            _p(info))
+    return {"status": st, "centres": Co, "labels": lab, "n_iter": int(info[0]),
+            "inertia": float(info[1])}

@@ -1,0 +1,202 @@
+"""Writes the golden fixtures used by the tests from the reference's own test data.
+
+Run once in the build container (the reference checkout is not available where the tests
+run):  python tests/golden/make_fixtures.py /root/reference
+
+Outputs (data only: inputs and expected outputs):
+  nn_csv_2092x5.npz, nn_csv_2125x20.npz  - tests/data/nn_data/{X_train,X_test,k_ind,k_dist}
+                                           CSV files of the reference, k = 5, 0-based indices.
+  known_answers.json                     - small known-answer cases transcribed from the
+                                           reference's unit tests (file:line in each entry).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def csv(name, dtype=np.float64):
+    return np.loadtxt(os.path.join(REF, "tests", "data", "nn_data", name), delimiter=",",
+                      dtype=dtype, ndmin=2)
+
+
+for ntr, nte, d in ((2092, 2071, 5), (2125, 2068, 20)):
+    np.savez_compressed(
+        os.path.join(HERE, f"nn_csv_{ntr}x{d}.npz"),
+        X_train=csv(f"X_train_{ntr}x{d}.csv"),
+        X_test=csv(f"X_test_{nte}x{d}.csv"),
+        k_ind=csv(f"k_ind_{ntr}x{nte}x{d}_5.csv", np.int64).astype(np.int32),
+        k_dist=csv(f"k_dist_{ntr}x{nte}x{d}_5.csv"),
+        source=np.array(f"tests/data/nn_data/*_{ntr}x{nte}x{d}_5.csv (k=5)"))
+
+known = {
+    "pairwise_fixed_xy": {
+        "source": "tests/unit_tests/metrics/metric_public.cpp:297-364",
+        "X_row": [[1.0, 2.0, -3.0], [1.0, -1.0, 2.0], [0.0, -2.0, 4.0]],
+        "Y_row": [[4.0, -2.0, 3.0], [3.0, -1.0, 2.0]],
+        "p": 4.5,
+        "expected": {
+            "sqeuclidean": [[61., 38.], [11., 4.], [17., 14.]],
+            "manhattan": [[13., 10.], [5., 2.], [5., 6.]],
+            "cosine": [[1.4466625002869187, 1.3571428571428572],
+                       [0.0902823477053158, 0.1271284390560302],
+                       [0.3356361611700803, 0.4023856953328032]],
+            "minkowski": [[6.254413691636758, 5.124034794569624],
+                          [3.009451461921273, 2.0],
+                          [4.001734793974489, 3.1055785043917186]],
+            "inner_product": [[-9., -5.], [12., 8.], [16., 10.]],
+        },
+    },
+    "pairwise_fixed_xx": {
+        "source": "tests/unit_tests/metrics/metric_public.cpp:366-459",
+        "X_row": [[1.0, 4.0, -3.0], [2.0, -1.0, 3.0], [1.0, -2.0, 5.0], [-3.0, 1.0, 3.0]],
+        "p": 1.5,
+        "expected": {
+            "sqeuclidean": [[0., 62., 100., 61.], [62., 0., 6., 29.], [100., 6., 0., 29.],
+                            [61., 29., 29., 0.]],
+            "manhattan": [[0., 12., 14., 13.], [12., 0., 4., 7.], [14., 4., 0., 9.],
+                          [13., 7., 9., 0.]],
+            "cosine": [[0., 1.576556660197055, 1.7877263614433763, 1.359937016532678],
+                       [1.576556660197055, 0., 0.0728949306988934, 0.8773721321030069],
+                       [1.7877263614433763, 0.0728949306988934, 0., 0.5811460917083046],
+                       [1.359937016532678, 0.8773721321030069, 0.5811460917083046, 0.]],
+            "minkowski": [[0., 8.972707819822075, 11.168500752960059, 9.19738630386655],
+                          [8.972707819822075, 0., 2.8567382778502783, 5.811210513661822],
+                          [11.168500752960059, 2.8567382778502783, 0., 6.356105477264618],
+                          [9.19738630386655, 5.811210513661822, 6.356105477264618, 0.]],
+            "inner_product": [[26., -11., -22., -8.], [-11., 14., 19., 2.],
+                              [-22., 19., 30., 10.], [-8., 2., 10., 19.]],
+        },
+    },
+    "knn_example": {
+        "source": "tests/unit_tests/nearest_neighbors/nearest_neighbors_tests.hpp:97-144,443-470",
+        "order": "column-major",
+        "X_train_colmajor": [-1., -2., -3., 1., 2., 3., -1., -1., -2., 3., 5., -1.,
+                             2., 3., -1., 1., 1., 2.],
+        "n_samples": 6, "n_features": 3, "n_queries": 3,
+        "X_test_colmajor": [-2., -1., 2., 2., -2., 1., 3., -1., -3.],
+        "k3_ind_colmajor": [1, 2, 3, 0, 0, 5, 3, 1, 4],
+        "k4_ind_colmajor": [1, 2, 3, 0, 0, 5, 3, 1, 4, 4, 5, 2],
+        "k4_dist_colmajor": [3., 2., 4.58257569495584, 3.3166247903554, 3.1622776601683795,
+                             5.477225575051661, 3.7416573867739413, 4.242640687119285,
+                             5.656854249492381, 5.385164807134504, 5.0990195135927845,
+                             6.164414002968976],
+    },
+    "kmeans": [
+        {
+            "name": "three clusters, lloyd",
+            "source": "tests/unit_tests/kmeans/kmeans_test_data.hpp:250-308,586-595",
+            "n_samples": 10, "n_features": 2, "n_clusters": 3, "max_iter": 50,
+            "A_colmajor": [1.0, 1.1, 0.5, 0.49, -2.0, -2.0, 0.53, 0.9, 1.2, -1.8,
+                           1.0, 1.2, -2.0, -1.9, 0.5, 0.51, -2.1, 0.95, 0.8, 0.6],
+            "C_colmajor": [0.5, 0.7, -1.3, 0.5, -1.7, 0.2],
+            "tol": 1e-4, "empty_clusters": "ignore",
+            "rinfo": [10.0, 2.0, 3.0, 1.0, 0.185475, 0.0],
+            "centres_colmajor": [1.05, 0.506667, -1.93333333333, 0.9875, -2.0, 0.53666666666],
+            "labels": [0, 0, 1, 1, 2, 2, 1, 0, 0, 2],
+            "X_colmajor": [0.0, 0.0, 1.0, 0.0, 0.0, 1.0, 0.0, -1.0], "m_samples": 4,
+            "X_transform_colmajor": [1.4414077320453085, 1.0500744021258683, 0.9887650125282548,
+                                     2.2478114355968564, 2.0631798542810347, 3.042484364973978,
+                                     2.0599460618612757, 1.1210312712458608, 2.0064368639179446,
+                                     1.9880783574989047, 2.9820220581939925, 2.469640099735629],
+            "Y_colmajor": [0.5, 0.5, -1.0, 0.5, -1.0, 0.0], "k_samples": 3,
+            "Y_labels": [0, 1, 2],
+            "tol_check": "100*sqrt(eps)",
+        },
+        {
+            "name": "larger data matrix, lloyd",
+            "source": "tests/unit_tests/kmeans/kmeans_test_data.hpp:444-528",
+            "n_samples": 80, "n_features": 2, "n_clusters": 16, "max_iter": 30,
+            "A_colmajor": [
+                2.1, -1.0, 3.0, 2.0, -3.0, -2.2, -2.4, 0.9, 1.0, -2.0, 2.0, 1.0, -2.0,
+                -1.0, -3.0, 2.3, 2.1, -1.5, 3.0, 2.0, -3.0, -2.8, -2.8, 1.0, 1.2, -2.0,
+                2.1, 3.7, -2.0, -1.0, -3.8, 2.0, 2.0, -3.9, 3.0, 2.0, -3.9, -2.7, -2.0,
+                1.0, 1.0, -2.4, 2.1, 3.0, -2.6, -1.5, -3.4, 2.0, 2.0, -1.0, 3.0, 2.6,
+                -3.0, -2.6, -2.0, 1.0, 1.3, -2.5, 2.8, 3.5, -2.0, -1.0, -3.0, 2.2, 2.0,
+                -1.0, 3.4, 2.0, -3.8, -2.0, -2.0, 1.0, 1.9, -2.1, 2.1, 3.0, -2.0, -1.0,
+                -3.4, 2.0, 2.0, -8.0, 3.7, 2.0, -3.2, -2.6, -2.2, 1.0, 1.0, -2.5, 6.0,
+                -5.5, -2.0, -1.4, -3.7, 2.0, 2.0, -1.2, 3.4, 2.0, -4.0, -2.0, -2.0, 1.0,
+                1.0, -2.0, 2.0, 3.3, -2.0, -1.4, -3.0, 2.3, 2.0, -9.0, 3.0, 2.0, -3.0,
+                -2.0, -2.0, 1.7, 1.3, -2.0, 2.3, 3.3, -2.0, -6.0, -3.0, 2.7, 2.0, -1.5,
+                3.0, 2.6, -3.3, -2.0, -2.0, 1.0, 1.7, -2.0, 2.0, 8.5, -2.0, -1.7, -3.0,
+                2.0, 2.2, -1.2, 3.6, 2.9, -6.0, -2.0, -2.1, 1.5, 9.0, -2.0, 2.0, 3.8,
+                -2.0, -1.0, -8.1, 2.9],
+            "C_colmajor": [1.9, -5.1, 1.0, -3.0, 1.0, -3.2, 6.0, -3.7, 8.5, -3.0, 1.7, -3.0,
+                           1.0, -7.0, 1.0, -3.4, 4.8, -3.0, 1.0, -3.4, 1.9, -3.0, 1.3, -3.8,
+                           5.4, -3.2, -8.0, -3.0, -6.1, -3.3, 1.6, -3.0],
+            "tol": 1e-4, "empty_clusters": "ignore",
+            "rinfo": [80.0, 2.0, 16.0, 4.0, 45.78715, 0.0],
+            "centres_colmajor": [2.4666666667, 0.0, 1.02000, -3.0, 2.42593, 0.0, 0.0, -3.7,
+                                 0.0, -3.0, -1.0, -1.94815, -0.25, 0.0, 1.075, -3.7,
+                                 7.833333333333, 0.0, 1.0, -3.85, 2.55555556, 0.0, 0.0, -7.7,
+                                 0.0, -3.166666667, -8.0, -1.88148, -5.75, 0.0, 1.55, -3.0],
+            "labels": [4, 10, 4, 4, 9, 11, 11, 2, 2, 11, 0, 12, 11, 11, 3, 4, 4, 11, 4, 4,
+                       3, 11, 11, 2, 2, 11, 4, 4, 11, 11, 15, 4, 4, 7, 4, 4, 15, 11, 11, 14,
+                       14, 11, 4, 4, 11, 12, 15, 4, 4, 11, 4, 4, 9, 11, 11, 2, 14, 11, 4, 0,
+                       11, 11, 9, 4, 4, 11, 4, 4, 7, 11, 11, 14, 0, 11, 4, 4, 11, 11, 7, 4],
+            "X_colmajor": [-3.700001, -7.7], "m_samples": 1,
+            "X_transform_colmajor": [16.712636602948745, 8.5428332536694178, 9.8978987669100764,
+                                     3.9131189606246322, 11.945852343094563, 8.5428332536694178,
+                                     8.5428332536694178, 0.0, 8.5428332536694178,
+                                     4.5870590917396212, 2.7166155414412292, 6.0765238962074113,
+                                     3.9629534440868719, 8.5428332536694178, 10.409761044327579,
+                                     4.7000000000000002],
+            "Y_colmajor": [-3.7, -7.7], "k_samples": 1, "Y_labels": [7],
+            "tol_check": "10000*sqrt(eps)",
+        },
+        {
+            "name": "empty cluster, ignore",
+            "source": "tests/unit_tests/kmeans/kmeans_test_data.hpp:141-193",
+            "n_samples": 5, "n_features": 2, "n_clusters": 2, "max_iter": 30,
+            "A_colmajor": [2.13, 2.11, 2.12, 2.13, 2.14, 2.11, 2.14, 2.13, 2.12, 2.13],
+            "C_colmajor": [2.12, 100.0, 2.13, 100.0],
+            "tol": 1e-4, "empty_clusters": "ignore",
+            "rinfo": [5.0, 2.0, 2.0, 1.0, 0.00104, 0.0],
+            "centres_colmajor": [2.126, 0.0, 2.126, 0.0],
+            "labels": [0, 0, 0, 0, 0],
+            "X_colmajor": [3.3, 3.3], "m_samples": 1,
+            "X_transform_colmajor": [1.6602867222260123, 4.6669047558312133],
+            "Y_colmajor": [2.3, 2.4], "k_samples": 1, "Y_labels": [0],
+            "tol_check": "10*eps",
+        },
+        {
+            "name": "empty cluster, split",
+            "source": "tests/unit_tests/kmeans/kmeans_test_data.hpp:784-797",
+            "n_samples": 5, "n_features": 2, "n_clusters": 2, "max_iter": 30,
+            "A_colmajor": [2.13, 2.11, 2.12, 2.13, 2.14, 2.11, 2.14, 2.13, 2.12, 2.13],
+            "C_colmajor": [2.12, 100.0, 2.13, 100.0],
+            "tol": 1e-4, "empty_clusters": "split",
+            "rinfo": [5.0, 2.0, 2.0, 1.0, 0.000475, 0.0],
+            "centres_colmajor": [2.13, 2.11, 2.1225, 2.14],
+            "labels": [0, 1, 0, 0, 0],
+            "X_colmajor": [3.3, 3.3], "m_samples": 1,
+            "X_transform_colmajor": [1.6599416405404137, 1.6618363336983584],
+            "Y_colmajor": [2.3, 2.4], "k_samples": 1, "Y_labels": [1],
+            "tol_check": "10*eps",
+        },
+        {
+            "name": "1 by 1 data matrix, random initialization, lloyd",
+            "source": "tests/unit_tests/kmeans/kmeans_test_data.hpp:88-139,558-567",
+            "init": "random",
+            "n_samples": 1, "n_features": 1, "n_clusters": 1, "max_iter": 30,
+            "A_colmajor": [2.1], "C_colmajor": [2.3],
+            "tol": 1e-4, "empty_clusters": "ignore",
+            "rinfo": [1.0, 1.0, 1.0, 0.0, 0.0, 0.0],
+            "centres_colmajor": [2.1], "labels": [0],
+            "X_colmajor": [3.3], "m_samples": 1, "X_transform_colmajor": [1.2],
+            "Y_colmajor": [1.3], "k_samples": 1, "Y_labels": [0],
+            "tol_check": "10*eps",
+        },
+    ],
+    "kmeans_error_empty": {
+        "source": "tests/unit_tests/kmeans/kmeans_test_data.hpp:799-805",
+        "comment": "the empty-cluster case with 'empty clusters'='error' returns da_status_empty_clusters",
+    },
+}
+with open(os.path.join(HERE, "known_answers.json"), "w") as f:
+    json.dump(known, f, indent=1)
+print("fixtures written to", HERE)

@@ -1,0 +1,174 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function the public
+headers declare, and returns the reference's status codes for argument and option errors
+(those are decided before any device work)."""
+import ctypes as C
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from aoclda import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"^(?:da_status|void)\s+(da_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_exports_every_declared_symbol():
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L.lib, n)]
+    assert not missing, missing
+
+
+def st(name):
+    return L.CODE[name]
+
+
+def test_pairwise_argument_checks():
+    # pairwise_distances.cpp:85-111 order of checks
+    X = np.ones((3, 2))
+    D = np.zeros((3, 3))
+    f = L.lib.da_pairwise_distances_d
+    p = L.ptr
+    assert f(L.ROW_MAJOR, 0, 3, 2, p(X), 2, None, 0, p(D), 3, 2.0, 0) == st("invalid_array_dimension")
+    assert f(L.ROW_MAJOR, 3, 3, 0, p(X), 2, None, 0, p(D), 3, 2.0, 0) == st("invalid_array_dimension")
+    assert f(L.ROW_MAJOR, 3, 3, 2, p(X), 1, None, 0, p(D), 3, 2.0, 0) == st("invalid_leading_dimension")
+    assert f(L.COLUMN_MAJOR, 3, 3, 2, p(X), 2, None, 0, p(D), 3, 2.0, 0) == st("invalid_leading_dimension")
+    assert f(L.ROW_MAJOR, 3, 3, 2, None, 2, None, 0, p(D), 3, 2.0, 0) == st("invalid_pointer")
+    assert f(L.ROW_MAJOR, 3, 3, 2, p(X), 2, None, 0, None, 3, 2.0, 0) == st("invalid_pointer")
+    assert f(L.ROW_MAJOR, 3, 0, 2, p(X), 2, p(X), 2, p(D), 3, 2.0, 0) == st("invalid_array_dimension")
+    assert f(L.ROW_MAJOR, 3, 3, 2, p(X), 2, p(X), 1, p(D), 3, 2.0, 0) == st("invalid_leading_dimension")
+    assert f(L.ROW_MAJOR, 3, 3, 2, p(X), 2, p(X), 2, p(D), 2, 2.0, 0) == st("invalid_leading_dimension")
+    assert f(L.ROW_MAJOR, 3, 3, 2, p(X), 2, None, 0, p(D), 2, 2.0, 0) == st("invalid_leading_dimension")
+    assert f(L.ROW_MAJOR, 3, 3, 2, p(X), 2, None, 0, p(D), 3, 0.0, 2) == st("invalid_input")
+    assert f(L.ROW_MAJOR, 3, 3, 2, p(X), 2, None, 0, p(D), 3, -1.0, 2) == st("invalid_input")
+
+
+def test_handle_types():
+    h = L.da_handle()
+    assert L.lib.da_handle_init_d(C.byref(h), 1) == st("not_implemented")  # linmod
+    L.lib.da_handle_destroy(C.byref(h))
+    assert not h
+    L.lib.da_handle_destroy(None)
+    for t in (L.HANDLE_NN, L.HANDLE_KMEANS):
+        assert L.lib.da_handle_init_s(C.byref(h), t) == 0
+        L.lib.da_handle_destroy(C.byref(h))
+
+
+def test_options_registry():
+    h = L.Handle(L.HANDLE_NN, np.float64)
+    lib = L.lib
+    v = L.da_int()
+    assert lib.da_options_get_int(h.h, b"number of neighbors", C.byref(v)) == 0 and v.value == 5
+    # names are trimmed, blank-squeezed and lower-cased (options.hpp:192-201)
+    assert lib.da_options_set_int(h.h, b"  Number   of NEIGHBORS ", 7) == 0
+    assert lib.da_options_get_int(h.h, b"number of neighbors", C.byref(v)) == 0 and v.value == 7
+    assert lib.da_options_set_int(h.h, b"number of neighbors", 0) == st("option_invalid_bounds")
+    assert lib.da_options_set_int(h.h, b"no such option", 1) == st("option_not_found")
+    assert lib.da_options_set_int(h.h, b"metric", 1) == st("option_wrong_type")
+    assert lib.da_options_set_string(h.h, b"metric", b"chebyshev") == st("option_invalid_value")
+    assert lib.da_options_set_string(h.h, b"metric", b"  CoSine ") == 0
+    buf = C.create_string_buffer(64)
+    n = L.da_int(64)
+    assert lib.da_options_get_string(h.h, b"metric", buf, C.byref(n)) == 0
+    assert buf.value == b"cosine"
+    n = L.da_int(3)
+    assert lib.da_options_get_string(h.h, b"metric", buf, C.byref(n)) == st("invalid_input")
+    assert n.value == 7
+    # real options are typed by the handle precision
+    assert lib.da_options_set_real_s(h.h, b"minkowski parameter", 3.0) == st("wrong_type")
+    assert lib.da_options_set_real_d(h.h, b"minkowski parameter", 3.0) == 0
+    assert lib.da_options_set_real_d(h.h, b"minkowski parameter", 0.0) == st("option_invalid_bounds")
+    d = C.c_double()
+    assert lib.da_options_get_real_d(h.h, b"minkowski parameter", C.byref(d)) == 0 and d.value == 3.0
+    sev = C.c_int()
+    assert lib.da_handle_get_error_severity(h.h, C.byref(sev)) == 0
+
+
+def test_kmeans_options_and_states():
+    h = L.Handle(L.HANDLE_KMEANS, np.float32)
+    lib = L.lib
+    v = L.da_int()
+    for name, default in ((b"n_clusters", 1), (b"n_init", 10), (b"max_iter", 300), (b"seed", 0)):
+        assert lib.da_options_get_int(h.h, name, C.byref(v)) == 0 and v.value == default
+    f = C.c_float()
+    assert lib.da_options_get_real_s(h.h, b"convergence tolerance", C.byref(f)) == 0
+    assert abs(f.value - 1e-4) < 1e-9
+    buf = C.create_string_buffer(64)
+    n = L.da_int(64)
+    assert lib.da_options_get_string(h.h, b"initialization method", buf, C.byref(n)) == 0
+    assert buf.value == b"k-means++"
+    assert lib.da_options_set_int(h.h, b"seed", -2) == st("option_invalid_bounds")
+    # compute / set_init_centres before set_data
+    assert lib.da_kmeans_compute_s(h.h) == st("no_data")
+    Cm = np.ones((1, 1), dtype=np.float32)
+    assert lib.da_kmeans_set_init_centres_s(h.h, L.ptr(Cm), 1) == st("no_data")
+    # wrong precision and wrong handle type
+    assert lib.da_kmeans_compute_d(h.h) == st("wrong_type")
+    assert lib.da_nn_set_data_s(h.h, 1, 1, L.ptr(Cm), 1) == st("invalid_handle_type")
+    # results before compute
+    tr = L.da_int(-1)
+    dim = L.da_int(1)
+    assert lib.da_handle_get_result_int(h.h, L.TRAINED, C.byref(dim), C.byref(tr)) == 0
+    assert tr.value == 0
+    r = np.zeros(6, dtype=np.float32)
+    dim = L.da_int(6)
+    assert lib.da_handle_get_result_s(h.h, L.RINFO, C.byref(dim), L.ptr(r)) == st("no_data")
+    assert lib.da_handle_get_result_d(h.h, L.RINFO, C.byref(dim), L.ptr(r)) == st("wrong_type")
+    assert lib.da_handle_get_result_s(h.h, L.RINFO, None, L.ptr(r)) == st("invalid_input")
+
+
+def test_kmeans_set_data_checks():
+    h = L.Handle(L.HANDLE_KMEANS, np.float64)
+    lib = L.lib
+    A = np.ones((4, 2))
+    assert lib.da_kmeans_set_data_d(h.h, 0, 2, L.ptr(A), 4) == st("invalid_array_dimension")
+    assert lib.da_kmeans_set_data_d(h.h, 4, 0, L.ptr(A), 4) == st("invalid_array_dimension")
+    assert lib.da_kmeans_set_data_d(h.h, 4, 2, None, 4) == st("invalid_pointer")
+    assert lib.da_kmeans_set_data_d(h.h, 4, 2, L.ptr(A), 3) == st("invalid_leading_dimension")
+    # "check data" rejects NaN before any device work
+    assert lib.da_options_set_string(h.h, b"check data", b"yes") == 0
+    An = A.copy(order="F")
+    An[1, 1] = np.nan
+    assert lib.da_kmeans_set_data_d(h.h, 4, 2, L.ptr(An), 4) == st("invalid_input")
+
+
+def test_nn_checks_without_data():
+    h = L.Handle(L.HANDLE_NN, np.float64)
+    lib = L.lib
+    X = np.ones((3, 2))
+    ind = np.zeros((3, 2), dtype=np.int32)
+    dist = np.zeros((3, 2))
+    assert lib.da_nn_kneighbors_d(h.h, 3, 2, L.ptr(X), 3, L.ptr(ind), L.ptr(dist), 2, 1) == st("no_data")
+    assert lib.da_nn_set_data_d(h.h, 0, 2, L.ptr(X), 3) == st("invalid_array_dimension")
+    assert lib.da_nn_set_data_d(h.h, 3, 2, None, 3) == st("invalid_pointer")
+    assert lib.da_nn_set_data_d(h.h, 3, 2, L.ptr(X), 2) == st("invalid_leading_dimension")
+    assert lib.da_nn_set_data_s(h.h, 3, 2, L.ptr(X), 3) == st("wrong_type")
+    # tree algorithms reject the metrics they cannot prune with (nearest_neighbors.cpp:313-329)
+    assert lib.da_options_set_string(h.h, b"algorithm", b"kd tree") == 0
+    assert lib.da_options_set_string(h.h, b"metric", b"cosine") == 0
+    assert lib.da_nn_set_data_d(h.h, 3, 2, L.ptr(X), 3) in (st("incompatible_options"),
+                                                            st("arch_not_supported"))
+
+
+def test_no_gpu_is_an_error_not_a_fallback():
+    """Without a GPU the library reports da_status_arch_not_supported; there is no CPU path."""
+    import subprocess, sys
+    code = ("import sys; sys.path.insert(0, %r); import numpy as np; from aoclda import _lib as L;"
+            "X=np.ones((3,2)); D=np.zeros((3,3));"
+            "print(L.lib.da_pairwise_distances_d(0,3,3,2,L.ptr(X),2,None,0,L.ptr(D),3,2.0,0))"
+            % os.path.join(ROOT, "aocl-data-analytics_amd"))
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="-1", ROCR_VISIBLE_DEVICES="-1")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
+                         timeout=120)
+    assert out.stdout.strip() == str(st("arch_not_supported")), out.stderr

@@ -1,0 +1,127 @@
+"""GPU parity of k-means (da_kmeans_*) against the reference's known answers and the oracle.
+Labels must be identical (no ties in these data); centres and inertia within 1e-12 relative
+(fp64) or 1e-5 relative (fp32)."""
+import numpy as np
+import pytest
+
+import aoclda
+from aoclda import _lib as L
+
+pytestmark = pytest.mark.gpu
+EPS64 = np.finfo(np.float64).eps
+
+
+def _colmajor(v, rows, cols, dtype=np.float64):
+    return np.array(v, dtype=dtype).reshape((cols, rows)).T.copy(order="F")
+
+
+@pytest.mark.parametrize("idx", range(5))
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("algorithm", ["lloyd", "elkan"])
+def test_known_answers(known, idx, dtype, algorithm):
+    c = known["kmeans"][idx]
+    n, d, k = c["n_samples"], c["n_features"], c["n_clusters"]
+    A = _colmajor(c["A_colmajor"], n, d, dtype)
+    init = c.get("init", "supplied")
+    C0 = _colmajor(c["C_colmajor"], k, d, dtype) if init == "supplied" else None
+    km = aoclda.clustering.kmeans(n_clusters=k, initialization_method=init, C=C0, n_init=1,
+                                  max_iter=c["max_iter"], tol=c["tol"], seed=78,
+                                  algorithm=algorithm, empty_clusters=c["empty_clusters"])
+    km.fit(A)
+    eps = np.finfo(dtype).eps
+    tol = {"100*sqrt(eps)": 100 * np.sqrt(eps), "10000*sqrt(eps)": 1e4 * np.sqrt(eps),
+           "10*eps": 10 * eps}[c["tol_check"]]
+    tol = max(tol, 1e-6 if dtype == np.float64 else 2e-5)
+    assert km.n_iter == c["rinfo"][3]
+    assert abs(km.inertia - c["rinfo"][4]) <= tol
+    np.testing.assert_allclose(km.cluster_centres, _colmajor(c["centres_colmajor"], k, d), rtol=0,
+                               atol=tol)
+    np.testing.assert_array_equal(km.labels, c["labels"])
+    X = _colmajor(c["X_colmajor"], c["m_samples"], d, dtype)
+    np.testing.assert_allclose(km.transform(X), _colmajor(c["X_transform_colmajor"], c["m_samples"], k),
+                               rtol=0, atol=tol)
+    Y = _colmajor(c["Y_colmajor"], c["k_samples"], d, dtype)
+    np.testing.assert_array_equal(km.predict(Y), c["Y_labels"])
+
+
+def test_empty_cluster_error(known):
+    c = known["kmeans"][2]
+    A = _colmajor(c["A_colmajor"], 5, 2)
+    C0 = _colmajor(c["C_colmajor"], 2, 2)
+    km = aoclda.clustering.kmeans(n_clusters=2, C=C0, max_iter=30, empty_clusters="error")
+    with pytest.raises(aoclda.DAError) as e:
+        km.fit(A)
+    assert e.value.status == L.CODE["empty_clusters"]
+
+
+def blobs(rng, n_per, centres, sigma=1.0):
+    return np.concatenate([c + sigma * rng.standard_normal((n_per, len(c))) for c in centres])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("init", ["supplied", "random", "k-means++", "random partitions"])
+def test_vs_oracle(oracle, dtype, init):
+    rng = np.random.default_rng(9)
+    centres = rng.uniform(-10, 10, (12, 6))
+    X = blobs(rng, 150, centres).astype(dtype)
+    rng.shuffle(X)
+    C0 = X[rng.choice(len(X), 12, replace=False)] if init == "supplied" else None
+    km = aoclda.clustering.kmeans(n_clusters=12, initialization_method=init, C=C0, n_init=3,
+                                  max_iter=100, tol=1e-6, seed=123)
+    km.fit(X)
+    r = oracle.kmeans(X.astype(np.float64), 12, init, None if C0 is None else C0.astype(np.float64),
+                      n_init=1 if init == "supplied" else 3, max_iter=100, tol=1e-6, seed=123)
+    np.testing.assert_array_equal(km.labels, r["labels"])
+    assert km.n_iter == r["n_iter"]
+    rtol = 1e-12 if dtype == np.float64 else 1e-5
+    np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=rtol, atol=rtol)
+    np.testing.assert_allclose(km.inertia, r["inertia"], rtol=rtol)
+
+
+def test_row_major_and_column_major_agree():
+    rng = np.random.default_rng(10)
+    X = blobs(rng, 100, rng.uniform(-5, 5, (5, 3)))
+    a = aoclda.clustering.kmeans(n_clusters=5, seed=7, n_init=2).fit(np.ascontiguousarray(X))
+    b = aoclda.clustering.kmeans(n_clusters=5, seed=7, n_init=2).fit(np.asfortranarray(X))
+    np.testing.assert_array_equal(a.labels, b.labels)
+    np.testing.assert_array_equal(a.cluster_centres, b.cluster_centres)
+
+
+def test_maxit_warning(oracle):
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((500, 4))
+    km = aoclda.clustering.kmeans(n_clusters=8, max_iter=1, seed=3, n_init=1)
+    km.fit(X)
+    assert km.status == L.CODE["maxit"]
+    r = oracle.kmeans(X, 8, "k-means++", max_iter=1, seed=3)
+    assert r["status"] == 21
+    np.testing.assert_array_equal(km.labels, r["labels"])
+
+
+def test_many_clusters():
+    """kmeans_test_data.hpp:809-858: 2000 points in pairs, 1000 clusters from supplied
+    centres; converges in 2 iterations with inertia <= 0.01."""
+    n, d, k = 2000, 2, 1000
+    A = np.zeros((n, d))
+    for i in range(0, n, 2):
+        for j in range(d):
+            A[i, j] = i + j * 0.00001
+            A[i + 1, j] = i + 0.00001 + j * 0.00001
+    C0 = np.array([[2 * i + j * 0.00001 for j in range(d)] for i in range(k)])
+    km = aoclda.clustering.kmeans(n_clusters=k, C=C0, max_iter=50).fit(np.asfortranarray(A))
+    assert km.n_iter == 2
+    assert km.inertia <= 0.01
+
+
+def test_mixed_precision(known):
+    """kmeans_test_data.hpp:944-969: iterative refinement reports the low precision
+    iteration count in rinfo[5]."""
+    c = known["kmeans"][0]
+    A = _colmajor(c["A_colmajor"], 10, 2)
+    C0 = _colmajor(c["C_colmajor"], 3, 2)
+    km = aoclda.clustering.kmeans(n_clusters=3, C=C0, max_iter=50, mixed_precision=True,
+                                  low_precision_max_iter=1)
+    km.fit(A)
+    assert km.lp_n_iter == 1
+    assert km.n_iter == 0
+    np.testing.assert_array_equal(km.labels, c["labels"])

@@ -1,0 +1,144 @@
+"""GPU parity of brute-force kNN (da_nn_set_data / da_nn_kneighbors) against the reference's
+fixtures and the oracle. Distances: fp64 1e-12 relative, fp32 1e-5 relative; indices must
+be identical except where two candidates are at the same distance (ties)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import aoclda
+from aoclda import _lib as L
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_knn(ind, dist, exp_ind, exp_dist, Xt, Xq, rtol, metric="euclidean"):
+    scale = np.maximum(np.abs(exp_dist), 1.0)
+    assert np.all(np.abs(dist - exp_dist) <= rtol * scale), np.abs(dist - exp_dist).max()
+    for q, j in np.argwhere(ind != exp_ind):
+        a = np.linalg.norm(Xt[ind[q, j]] - Xq[q]) if metric == "euclidean" else dist[q, j]
+        b = np.linalg.norm(Xt[exp_ind[q, j]] - Xq[q]) if metric == "euclidean" else exp_dist[q, j]
+        assert abs(a - b) <= rtol * max(1.0, abs(b)), (q, j, ind[q, j], exp_ind[q, j])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_example(known, dtype):
+    c = known["knn_example"]
+    Xt = np.array(c["X_train_colmajor"], dtype=dtype).reshape((3, 6)).T.copy(order="F")
+    Xq = np.array(c["X_test_colmajor"], dtype=dtype).reshape((3, 3)).T.copy(order="F")
+    nn = aoclda.neighbors.nearest_neighbors(n_neighbors=3).fit(Xt)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=4)
+    exp_i = np.array(c["k4_ind_colmajor"]).reshape((4, 3)).T
+    exp_d = np.array(c["k4_dist_colmajor"]).reshape((4, 3)).T
+    assert_knn(ind, dist, exp_i, exp_d, Xt.astype(np.float64), Xq.astype(np.float64),
+               1e-12 if dtype == np.float64 else 1e-5)
+    # the default number of neighbours comes from the option
+    d3, i3 = nn.kneighbors(Xq)
+    assert i3.shape == (3, 3)
+
+
+@pytest.mark.parametrize("name", ["2092x5", "2125x20"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("order", ["C", "F"])
+def test_csv_fixtures(nn_csv, name, dtype, order):
+    z = nn_csv[name]
+    Xt = np.asarray(z["X_train"].astype(dtype), order=order)
+    Xq = np.asarray(z["X_test"].astype(dtype), order=order)
+    nn = aoclda.neighbors.nearest_neighbors(n_neighbors=5).fit(Xt)
+    dist, ind = nn.kneighbors(Xq)
+    assert_knn(ind, dist, z["k_ind"], z["k_dist"], z["X_train"], z["X_test"],
+               1e-12 if dtype == np.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("metric", ["euclidean", "sqeuclidean", "manhattan", "cosine",
+                                    "minkowski", "inner_product", "euclidean_gemm",
+                                    "sqeuclidean_gemm"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_vs_oracle_metrics(oracle, metric, dtype):
+    rng = np.random.default_rng(3)
+    Xt = rng.standard_normal((1500, 24)).astype(dtype)
+    Xq = rng.standard_normal((300, 24)).astype(dtype)
+    nn = aoclda.neighbors.nearest_neighbors(metric=metric, p=3.0).fit(Xt)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=7)
+    om = {"sqeuclidean_gemm": "sqeuclidean", "euclidean_gemm": "euclidean"}.get(metric, metric)
+    ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), 7, om, p=3.0,
+                        eps=np.finfo(dtype).eps)
+    rtol = 1e-12 if dtype == np.float64 else 1e-5
+    np.testing.assert_allclose(dist, ed, rtol=rtol, atol=rtol)
+    np.testing.assert_array_equal(ind, ei)
+
+
+@pytest.mark.parametrize("k", [1, 2, 64, 300])
+def test_k_edges(oracle, k):
+    rng = np.random.default_rng(4)
+    Xt = rng.standard_normal((300, 5))
+    Xq = rng.standard_normal((17, 5))
+    nn = aoclda.neighbors.nearest_neighbors().fit(Xt)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=k)
+    ei, ed = oracle.knn(Xt, Xq, k)
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-12)
+
+
+def test_k_errors():
+    X = np.random.default_rng(0).standard_normal((10, 3))
+    nn = aoclda.neighbors.nearest_neighbors().fit(X)
+    with pytest.raises(aoclda.DAError) as e:
+        nn.kneighbors(X, n_neighbors=11)
+    assert e.value.status == L.CODE["invalid_input"]
+    with pytest.raises(aoclda.DAError) as e:
+        nn.kneighbors(np.ones((2, 4)))
+    assert e.value.status == L.CODE["invalid_array_dimension"]
+
+
+def test_duplicates_and_ties(oracle):
+    """Many identical training rows: ties are resolved by index, deterministically."""
+    base = np.random.default_rng(5).standard_normal((50, 4))
+    Xt = np.concatenate([base, base, base])
+    nn = aoclda.neighbors.nearest_neighbors().fit(Xt)
+    d1, i1 = nn.kneighbors(base, n_neighbors=6)
+    d2, i2 = nn.kneighbors(base, n_neighbors=6)
+    np.testing.assert_array_equal(i1, i2)
+    ei, ed = oracle.knn(Xt, base, 6)
+    np.testing.assert_array_equal(i1, ei)
+    assert np.all(d1[:, :3] == 0)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_device_api(oracle, dtype):
+    """da_nn_kneighbors_device_?: queries and results in device memory."""
+    rng = np.random.default_rng(6)
+    Xt = rng.standard_normal((2000, 16)).astype(dtype)
+    Xq = rng.standard_normal((100, 16)).astype(dtype)
+    nn = aoclda.neighbors.nearest_neighbors().fit(Xt)
+    h = nn.handle
+    k = 10
+    dq, di, dd = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    L.check(L.lib.da_amd_malloc(Xq.nbytes, C.byref(dq)))
+    L.check(L.lib.da_amd_malloc(100 * k * 4, C.byref(di)))
+    L.check(L.lib.da_amd_malloc(100 * k * Xq.itemsize, C.byref(dd)))
+    try:
+        L.check(L.lib.da_amd_memcpy_htod(dq, L.ptr(Xq), Xq.nbytes))
+        h.call("da_nn_kneighbors_device", 100, 16, dq, 16, di, dd, k, 1)
+        L.check(L.lib.da_amd_synchronize(h.h))
+        ind = np.empty((100, k), dtype=np.int32)
+        dist = np.empty((100, k), dtype=dtype)
+        L.check(L.lib.da_amd_memcpy_dtoh(L.ptr(ind), di, ind.nbytes))
+        L.check(L.lib.da_amd_memcpy_dtoh(L.ptr(dist), dd, dist.nbytes))
+    finally:
+        for p in (dq, di, dd):
+            L.lib.da_amd_free(p)
+    ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), k)
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-12 if dtype == np.float64 else 1e-5)
+
+
+def test_config_plain_small_knn(oracle):
+    """X 2000x64 as training and Y 1500x64 as queries, fp64 U[0,1) seed 0, k = 5."""
+    rng = np.random.default_rng(0)
+    X = rng.random((2000, 64))
+    Y = rng.random((1500, 64))
+    dist, ind = aoclda.neighbors.nearest_neighbors(n_neighbors=5).fit(X).kneighbors(Y)
+    ei, ed = oracle.knn(X, Y, 5)
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-12)

@@ -1,0 +1,157 @@
+"""Pins the CPU oracle against the reference's own known answers and fixtures, and
+against independent implementations (scipy, scikit-learn) and invariants. CPU only."""
+import numpy as np
+import pytest
+from scipy.spatial.distance import cdist
+
+EPS = np.finfo(np.float64).eps
+
+
+@pytest.mark.parametrize("case", ["pairwise_fixed_xy", "pairwise_fixed_xx"])
+@pytest.mark.parametrize("metric", ["euclidean", "sqeuclidean", "manhattan", "cosine",
+                                    "minkowski", "inner_product", "euclidean_gemm",
+                                    "sqeuclidean_gemm"])
+def test_pairwise_known_answers(known, oracle, case, metric):
+    c = known[case]
+    X = np.array(c["X_row"])
+    Y = np.array(c["Y_row"]) if "Y_row" in c else None
+    key = {"euclidean": "sqeuclidean", "euclidean_gemm": "sqeuclidean",
+           "sqeuclidean_gemm": "sqeuclidean"}.get(metric, metric)
+    exp = np.array(c["expected"][key])
+    if metric in ("euclidean", "euclidean_gemm"):
+        exp = np.sqrt(exp)
+    D = oracle.pairwise(X, Y, metric, p=c["p"])
+    np.testing.assert_allclose(D, exp, rtol=0, atol=100 * EPS * max(1, np.abs(exp).max()))
+
+
+@pytest.mark.parametrize("metric,sp", [("euclidean", "euclidean"), ("sqeuclidean", "sqeuclidean"),
+                                       ("manhattan", "cityblock"), ("cosine", "cosine"),
+                                       ("minkowski", "minkowski")])
+def test_pairwise_vs_scipy(oracle, metric, sp):
+    rng = np.random.default_rng(7)
+    X = rng.uniform(-10, 10, (37, 13))
+    Y = rng.uniform(-10, 10, (29, 13))
+    kw = {"p": 3.5} if sp == "minkowski" else {}
+    exp = cdist(X, Y, sp, **kw)
+    D = oracle.pairwise(X, Y, metric, p=3.5)
+    np.testing.assert_allclose(D, exp, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(oracle.pairwise(X, None, metric, p=3.5),
+                               cdist(X, X, sp, **kw), rtol=1e-12, atol=1e-12)
+
+
+def test_pairwise_invariants(oracle):
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((20, 6))
+    for metric in ("euclidean", "manhattan", "cosine", "sqeuclidean_gemm"):
+        D = oracle.pairwise(X, None, metric)
+        assert np.all(np.diag(D) == 0)
+        np.testing.assert_allclose(D, D.T, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(oracle.pairwise(X, X, "inner_product"), X @ X.T, rtol=1e-13)
+
+
+def test_knn_example(known, oracle):
+    c = known["knn_example"]
+    Xt = np.array(c["X_train_colmajor"]).reshape((c["n_features"], c["n_samples"])).T
+    Xq = np.array(c["X_test_colmajor"]).reshape((c["n_features"], c["n_queries"])).T
+    exp_d = np.array(c["k4_dist_colmajor"]).reshape((4, 3)).T
+    for k, key in ((3, "k3_ind_colmajor"), (4, "k4_ind_colmajor")):
+        ind, dist = oracle.knn(Xt, Xq, k)
+        exp = np.array(c[key]).reshape((k, c["n_queries"])).T
+        # Query 2 has training rows 0 and 2 tied at squared distance 38 in slot 4: the
+        # reference keeps the later index, the oracle orders ties by index.
+        _assert_knn_match(ind, dist, exp, exp_d[:, :k], 500 * EPS, Xt, Xq)
+    assert ind[2, 3] == 0 and exp[2, 3] == 2
+
+
+def _assert_knn_match(ind, dist, exp_ind, exp_dist, atol, Xt, Xq):
+    """Distances must match; an index may differ only where the two candidates are at the
+    same distance from the query (a tie, whose order the reference leaves to its schedule)."""
+    np.testing.assert_allclose(dist, exp_dist, rtol=0, atol=atol)
+    for q, j in np.argwhere(ind != exp_ind):
+        da = np.linalg.norm(Xt[ind[q, j]] - Xq[q])
+        db = np.linalg.norm(Xt[exp_ind[q, j]] - Xq[q])
+        assert abs(da - db) <= atol, f"query {q} slot {j}: index {ind[q, j]} vs {exp_ind[q, j]}"
+
+
+@pytest.mark.parametrize("name", ["2092x5", "2125x20"])
+def test_knn_csv_fixtures(nn_csv, oracle, name):
+    z = nn_csv[name]
+    ind, dist = oracle.knn(z["X_train"], z["X_test"], 5)
+    _assert_knn_match(ind, dist, z["k_ind"], z["k_dist"], 500 * EPS * 10, z["X_train"],
+                      z["X_test"])
+
+
+def test_knn_vs_sklearn(oracle):
+    from sklearn.neighbors import NearestNeighbors
+    rng = np.random.default_rng(11)
+    Xt = rng.standard_normal((500, 9))
+    Xq = rng.standard_normal((60, 9))
+    for metric, skm in (("euclidean", "euclidean"), ("manhattan", "manhattan"),
+                        ("cosine", "cosine")):
+        nn = NearestNeighbors(n_neighbors=7, algorithm="brute", metric=skm).fit(Xt)
+        sd, si = nn.kneighbors(Xq)
+        ind, dist = oracle.knn(Xt, Xq, 7, metric)
+        np.testing.assert_array_equal(ind, si)
+        np.testing.assert_allclose(dist, sd, rtol=1e-10, atol=1e-12)
+
+
+def _colmajor(v, rows, cols):
+    return np.array(v).reshape((cols, rows)).T
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_kmeans_known_answers(known, oracle, idx):
+    c = known["kmeans"][idx]
+    n, d, k = c["n_samples"], c["n_features"], c["n_clusters"]
+    A = _colmajor(c["A_colmajor"], n, d)
+    C0 = _colmajor(c["C_colmajor"], k, d)
+    init = c.get("init", "supplied")
+    r = oracle.kmeans(A, k, init, C0 if init == "supplied" else None, max_iter=c["max_iter"],
+                      tol=c["tol"], empty_clusters=c["empty_clusters"], seed=78)
+    assert r["status"] == 0
+    tol = {"100*sqrt(eps)": 100 * np.sqrt(EPS), "10000*sqrt(eps)": 1e4 * np.sqrt(EPS),
+           "10*eps": 10 * EPS}[c["tol_check"]]
+    tol = max(tol, 1e-6)  # the expected values are printed to ~7 significant digits
+    assert r["n_iter"] == c["rinfo"][3]
+    assert abs(r["inertia"] - c["rinfo"][4]) <= tol
+    np.testing.assert_allclose(r["centres"], _colmajor(c["centres_colmajor"], k, d), rtol=0,
+                               atol=tol)
+    np.testing.assert_array_equal(r["labels"], c["labels"])
+    X = _colmajor(c["X_colmajor"], c["m_samples"], d)
+    np.testing.assert_allclose(oracle.pairwise(X, r["centres"], "euclidean"),
+                               _colmajor(c["X_transform_colmajor"], c["m_samples"], k), rtol=0,
+                               atol=tol)
+    Y = _colmajor(c["Y_colmajor"], c["k_samples"], d)
+    pred = np.argmin(oracle.pairwise(Y, r["centres"], "sqeuclidean"), axis=1)
+    np.testing.assert_array_equal(pred, c["Y_labels"])
+
+
+def test_kmeans_error_mode(known, oracle):
+    c = known["kmeans"][2]
+    A = _colmajor(c["A_colmajor"], 5, 2)
+    C0 = _colmajor(c["C_colmajor"], 2, 2)
+    assert oracle.kmeans(A, 2, "supplied", C0, max_iter=30, empty_clusters="error")["status"] == 900
+
+
+def test_kmeans_vs_sklearn(oracle):
+    from sklearn.cluster import KMeans
+    rng = np.random.default_rng(5)
+    centres = rng.uniform(-10, 10, (6, 4))
+    X = np.concatenate([c + rng.standard_normal((200, 4)) for c in centres])
+    C0 = X[rng.choice(len(X), 6, replace=False)]
+    r = oracle.kmeans(X, 6, "supplied", C0, max_iter=100, tol=0.0)
+    sk = KMeans(n_clusters=6, init=C0, n_init=1, algorithm="lloyd", max_iter=100, tol=0.0).fit(X)
+    np.testing.assert_array_equal(r["labels"], sk.labels_)
+    np.testing.assert_allclose(r["centres"], sk.cluster_centers_, rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(r["inertia"], sk.inertia_, rtol=1e-10)
+
+
+def test_kmeanspp_quality_and_determinism(oracle):
+    rng = np.random.default_rng(1)
+    centres = rng.uniform(-20, 20, (8, 3))
+    X = np.concatenate([c + 0.3 * rng.standard_normal((100, 3)) for c in centres])
+    a = oracle.kmeans(X, 8, "k-means++", seed=42, max_iter=50)
+    b = oracle.kmeans(X, 8, "k-means++", seed=42, max_iter=50)
+    np.testing.assert_array_equal(a["centres"], b["centres"])
+    # well separated blobs: k-means++ finds every blob, so the inertia is close to n*d*0.09
+    assert a["inertia"] < 1.5 * 800 * 3 * 0.09
