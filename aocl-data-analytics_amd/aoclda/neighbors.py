@@ -10,7 +10,9 @@ from . import _lib as L
 class nearest_neighbors:
     def __init__(self, n_neighbors=5, weights="uniform", algorithm="auto", leaf_size=30,
                  metric="euclidean", p=2.0, radius=1.0, outlier_handling=None,
-                 check_data=False):
+                 check_data=False, precision="working", tile_variant=-1):
+        # precision: operand precision of the MFMA distance tiles ("working", "fp32",
+        # "fp16", "bf16"); results are re-ranked exactly whatever the choice (MI355X option).
         if not isinstance(n_neighbors, int) and n_neighbors is not None:
             raise ValueError("Number of neighbors must be an integer or None. Got n_neighbors="
                              + str(n_neighbors))
@@ -18,6 +20,8 @@ class nearest_neighbors:
         self.weights, self.algorithm, self.leaf_size = weights, algorithm, leaf_size
         self.metric, self.p, self.radius = metric, p, radius
         self.check_data = check_data
+        self.precision = precision
+        self.tile_variant = tile_variant
         self._h = None
         self.order = None
         self.dtype = None
@@ -31,6 +35,8 @@ class nearest_neighbors:
         h.set_real("minkowski parameter", self.p)
         h.set_string("storage order", "column-major" if self.order == L.COLUMN_MAJOR else "row-major")
         h.set_string("check data", "yes" if self.check_data else "no")
+        h.set_string("cross term precision", self.precision)
+        h.set_int("tile variant", self.tile_variant)
 
     def fit(self, X, y=None):
         X, self.order, self.dtype = L.prepare(X)

@@ -156,8 +156,29 @@ __global__ void pack_rows_kernel(const T *__restrict__ src, int64_t lds, int row
         const int64_t c = e - r * ldd;
         U v = (U)0.0f;
         if (r < rows && c < cols)
-            v = (U)src[r * lds + c];
+            v = (U)(float)src[r * lds + c];
+        if constexpr (sizeof(U) == 8)
+            if (r < rows && c < cols)
+                v = (U)src[r * lds + c];
         dst[e] = v;
+    }
+}
+
+// Split operands: each value x is stored as hi = U(x) and lo = U(x - hi), rows laid out
+// as [hi kp][lo kp].
+template <typename T, typename U>
+__global__ void pack_rows_split_kernel(const T *__restrict__ src, int64_t lds, int rows, int cols,
+                                       U *__restrict__ dst, int64_t kp, int rows_pad) {
+    const int64_t total = (int64_t)rows_pad * kp;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = e / kp;
+        const int64_t c = e - r * kp;
+        const double x = (r < rows && c < cols) ? (double)src[r * lds + c] : 0.0;
+        const U hi = (U)(float)x;
+        const U lo = (U)(float)(x - (double)(float)hi);
+        dst[r * 2 * kp + c] = hi;
+        dst[r * 2 * kp + kp + c] = lo;
     }
 }
 
@@ -258,6 +279,46 @@ hipError_t launch_pairwise_direct(int metric, const T *X, const T *Y, int m, int
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_pack_rows_prec(const T *src, int64_t lds, int rows, int cols, void *dst,
+                                 int64_t ldd, int rows_pad, int prec, hipStream_t s) {
+    switch (prec) {
+    case 1: // KP_F64
+        return launch_pack_rows<T, double>(src, lds, rows, cols, static_cast<double *>(dst), ldd,
+                                           rows_pad, s);
+    case 0: // KP_F32
+        return launch_pack_rows<T, float>(src, lds, rows, cols, static_cast<float *>(dst), ldd,
+                                          rows_pad, s);
+    case 2: // KP_F16
+        return launch_pack_rows<T, _Float16>(src, lds, rows, cols, static_cast<_Float16 *>(dst),
+                                             ldd, rows_pad, s);
+    case 3: // KP_BF16
+        return launch_pack_rows<T, __bf16>(src, lds, rows, cols, static_cast<__bf16 *>(dst), ldd,
+                                           rows_pad, s);
+    case 4: // KP_F16X3
+    case 5: { // KP_BF16X3
+        const int64_t total = (int64_t)rows_pad * ldd;
+        int64_t blocks = (total + 255) / 256;
+        blocks = blocks > 65536 ? 65536 : (blocks < 1 ? 1 : blocks);
+        if (prec == 4)
+            hipLaunchKernelGGL((pack_rows_split_kernel<T, _Float16>), dim3((unsigned)blocks),
+                               dim3(256), 0, s, src, lds, rows, cols, static_cast<_Float16 *>(dst),
+                               ldd, rows_pad);
+        else
+            hipLaunchKernelGGL((pack_rows_split_kernel<T, __bf16>), dim3((unsigned)blocks),
+                               dim3(256), 0, s, src, lds, rows, cols, static_cast<__bf16 *>(dst),
+                               ldd, rows_pad);
+        return hipGetLastError();
+    }
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+template hipError_t launch_pack_rows_prec<float>(const float *, int64_t, int, int, void *,
+                                                 int64_t, int, int, hipStream_t);
+template hipError_t launch_pack_rows_prec<double>(const double *, int64_t, int, int, void *,
+                                                  int64_t, int, int, hipStream_t);
+
 template hipError_t launch_colmajor_to_rowmajor<float>(const float *, int64_t, int, int, float *,
                                                        hipStream_t);
 template hipError_t launch_colmajor_to_rowmajor<double>(const double *, int64_t, int, int,
@@ -266,6 +327,16 @@ template hipError_t launch_pack_rows<float, float>(const float *, int64_t, int, 
                                                    int64_t, int, hipStream_t);
 template hipError_t launch_pack_rows<double, double>(const double *, int64_t, int, int, double *,
                                                      int64_t, int, hipStream_t);
+template hipError_t launch_pack_rows<float, _Float16>(const float *, int64_t, int, int, _Float16 *,
+                                                      int64_t, int, hipStream_t);
+template hipError_t launch_pack_rows<float, __bf16>(const float *, int64_t, int, int, __bf16 *,
+                                                    int64_t, int, hipStream_t);
+template hipError_t launch_pack_rows<double, _Float16>(const double *, int64_t, int, int,
+                                                       _Float16 *, int64_t, int, hipStream_t);
+template hipError_t launch_pack_rows<double, __bf16>(const double *, int64_t, int, int, __bf16 *,
+                                                     int64_t, int, hipStream_t);
+template hipError_t launch_pack_rows<double, float>(const double *, int64_t, int, int, float *,
+                                                    int64_t, int, hipStream_t);
 template hipError_t launch_row_sqnorms<float>(const float *, int64_t, int, int, double *,
                                               hipStream_t);
 template hipError_t launch_row_sqnorms<double>(const double *, int64_t, int, int, double *,

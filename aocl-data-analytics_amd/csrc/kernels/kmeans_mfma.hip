@@ -1,0 +1,251 @@
+// MFMA assignment step of Lloyd's algorithm for gfx950 (fp32 data).
+//
+// The reference forms -2 C X^T with a BLAS gemm per block of samples and takes, per sample,
+// the argmin of ||c||^2 - 2 x.c (kmeans_lloyd.hpp:185-197, kmeans_kernels.cpp:209-680).
+// Here the product runs on v_mfma_f32_16x16x4f32 with the centres as the A operand (staged
+// once per workgroup in LDS) and 64 samples per wave held in registers as the B operand.
+// The accumulator of centre c starts at -||c||^2/2, so it ends at x.c - ||c||^2/2 and the
+// nearest centre is its argmax. Each lane tracks the best and second-best value of its
+// sample; a sample whose two best values are closer than the error bound of the fp32
+// arithmetic is listed for an exact double-precision re-check, so the labels equal the exact
+// argmin (ties to the lowest centre index).
+#include "launch.hpp"
+#include <climits>
+#include <hip/hip_runtime.h>
+
+namespace dagpu {
+
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int PT = 4;             // 16-sample tiles per wave
+constexpr int WAVES = 4;
+constexpr int SB = WAVES * PT * 16; // samples per workgroup
+
+template <int KCH>
+__global__ __launch_bounds__(256) void kmeans_assign_mfma_kernel(
+    const float *__restrict__ X, int n, int Kp, const float *__restrict__ C32,
+    const float *__restrict__ chalf, int k, double unit, double cmax,
+    int *__restrict__ labels, const int *__restrict__ prev, int *__restrict__ changed,
+    int *__restrict__ list_count, int *__restrict__ list) {
+    extern __shared__ float smem[];
+    float *Cs = smem;              // k_pad x Kp centres
+    float *Hs = smem + (size_t)((k + 15) / 16 * 16) * Kp; // -||c||^2/2, padded with -inf
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int kpad = (k + 15) / 16 * 16;
+    for (int t = threadIdx.x; t < kpad * Kp; t += 256)
+        Cs[t] = t < k * Kp ? C32[t] : 0.0f;
+    for (int t = threadIdx.x; t < kpad; t += 256)
+        Hs[t] = t < k ? chalf[t] : -__builtin_huge_valf();
+    __syncthreads();
+
+    const int s0 = blockIdx.x * SB + w * PT * 16;
+    float b[PT][KCH];
+    float xn[PT];
+#pragma unroll
+    for (int pt = 0; pt < PT; pt++) {
+        const int i = s0 + pt * 16 + r;
+        float s = 0.f;
+#pragma unroll
+        for (int f = 0; f < KCH; f += 4) {
+            v4f v = {0.f, 0.f, 0.f, 0.f};
+            if (i < n)
+                v = *reinterpret_cast<const v4f *>(X + (int64_t)i * Kp + g * KCH + f);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                b[pt][f + e] = v[e];
+                s = fmaf(v[e], v[e], s);
+            }
+        }
+        xn[pt] = s;
+    }
+    float best1[PT], best2[PT];
+    int bidx[PT];
+#pragma unroll
+    for (int pt = 0; pt < PT; pt++) {
+        best1[pt] = -__builtin_huge_valf();
+        best2[pt] = -__builtin_huge_valf();
+        bidx[pt] = INT_MAX;
+    }
+    for (int c0 = 0; c0 < kpad; c0 += 16) {
+        float a[KCH];
+        const float *src = Cs + (c0 + r) * Kp + g * KCH;
+#pragma unroll
+        for (int f = 0; f < KCH; f += 4) {
+            const v4f v = *reinterpret_cast<const v4f *>(src + f);
+            a[f] = v[0];
+            a[f + 1] = v[1];
+            a[f + 2] = v[2];
+            a[f + 3] = v[3];
+        }
+        v4f h;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            h[i] = Hs[c0 + 4 * g + i];
+#pragma unroll
+        for (int pt = 0; pt < PT; pt++) {
+            v4f acc = h;
+#pragma unroll
+            for (int f = 0; f < KCH; f++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[f], b[pt][f], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float v = acc[i];
+                const int c = c0 + 4 * g + i;
+                if (v > best1[pt]) {
+                    best2[pt] = best1[pt];
+                    best1[pt] = v;
+                    bidx[pt] = c;
+                } else if (v > best2[pt]) {
+                    best2[pt] = v;
+                }
+            }
+        }
+    }
+    // Combine the four lane groups that hold the same sample (lanes r, r+16, r+32, r+48).
+#pragma unroll
+    for (int pt = 0; pt < PT; pt++) {
+        float xs = xn[pt];
+#pragma unroll
+        for (int off = 16; off <= 32; off <<= 1) {
+            const float o1 = __shfl_xor(best1[pt], off, 64);
+            const float o2 = __shfl_xor(best2[pt], off, 64);
+            const int oi = __shfl_xor(bidx[pt], off, 64);
+            xs += __shfl_xor(xs, off, 64);
+            const bool other = (o1 > best1[pt]) || (o1 == best1[pt] && oi < bidx[pt]);
+            const float lo = other ? best1[pt] : o1; // the loser's best
+            if (other) {
+                best1[pt] = o1;
+                bidx[pt] = oi;
+            }
+            best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
+        }
+        const int i = s0 + pt * 16 + r;
+        if (g == 0 && i < n) {
+            // |acc - exact| <= gamma_{d+2} (|x| cmax + cmax^2/2) + unit (|x| cmax + cmax^2/2)
+            const double xnorm = sqrt((double)xs);
+            const double gam = (Kp + 2) * unit / (1.0 - (Kp + 2) * unit);
+            const double e = (gam + 2.0 * unit) * (xnorm * cmax + 0.5 * cmax * cmax);
+            const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
+            const int lab = bidx[pt] < k ? bidx[pt] : 0;
+            labels[i] = lab;
+            if (near_tie || !(best1[pt] - best1[pt] == 0.0f)) {
+                const int p = atomicAdd(list_count, 1);
+                list[p] = i;
+            } else if (prev && prev[i] != lab) {
+                atomicAdd(changed, 1);
+            }
+        }
+    }
+}
+
+// Exact re-check of listed samples: argmin of double squared distances, ties to the lowest
+// centre. Fixed grid; the number of listed samples is read on the device.
+template <typename T>
+__global__ __launch_bounds__(256) void kmeans_recheck_kernel(
+    const T *__restrict__ X, int ldx, int d, const double *__restrict__ C, int k,
+    const int *__restrict__ list_count, const int *__restrict__ list, int *__restrict__ labels,
+    const int *__restrict__ prev, int *__restrict__ changed) {
+    const int cnt = *list_count;
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < cnt; t += gridDim.x * 256) {
+        const int i = list[t];
+        double best = __builtin_huge_val();
+        int bl = 0;
+        for (int c = 0; c < k; c++) {
+            double acc = 0.0;
+            for (int l = 0; l < d; l++) {
+                const double v = (double)X[(int64_t)i * ldx + l] - C[(int64_t)c * d + l];
+                acc = fma(v, v, acc);
+            }
+            if (acc < best) {
+                best = acc;
+                bl = c;
+            }
+        }
+        labels[i] = bl;
+        if (prev && prev[i] != bl)
+            atomicAdd(changed, 1);
+    }
+}
+
+__global__ void centre_prep_kernel(const double *__restrict__ C, int k, int d, int Kp,
+                                   float *__restrict__ C32, float *__restrict__ chalf,
+                                   double *__restrict__ cmax2) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= k)
+        return;
+    double s = 0.0;
+    for (int l = 0; l < Kp; l++) {
+        const double v = l < d ? C[(int64_t)c * d + l] : 0.0;
+        const float f = (float)v;
+        C32[(int64_t)c * Kp + l] = f;
+        s = fma(v, v, s);
+    }
+    chalf[c] = (float)(-0.5 * s);
+    atomicMax(reinterpret_cast<unsigned long long *>(cmax2), (unsigned long long)__double_as_longlong(s));
+}
+
+} // namespace
+
+hipError_t launch_centre_prep(const double *C, int k, int d, int Kp, float *C32, float *chalf,
+                              double *cmax2, hipStream_t s) {
+    hipLaunchKernelGGL(centre_prep_kernel, dim3((k + 255) / 256), dim3(256), 0, s, C, k, d, Kp,
+                       C32, chalf, cmax2);
+    return hipGetLastError();
+}
+
+size_t kmeans_assign_mfma_lds(int k, int Kp) {
+    const int kpad = (k + 15) / 16 * 16;
+    return (size_t)kpad * Kp * sizeof(float) + (size_t)kpad * sizeof(float);
+}
+
+hipError_t launch_kmeans_assign_mfma(const float *X, int n, int Kp, const float *C32,
+                                     const float *chalf, int k, double cmax, int *labels,
+                                     const int *prev, int *changed, int *list_count, int *list,
+                                     hipStream_t s) {
+    const size_t lds = kmeans_assign_mfma_lds(k, Kp);
+    if (lds > 160 * 1024)
+        return hipErrorInvalidValue;
+    dim3 grid((n + SB - 1) / SB), block(256);
+    const double unit = 1.0 / 16777216.0;
+#define DA_ASSIGN(KCHV)                                                                          \
+    {                                                                                            \
+        auto kern = kmeans_assign_mfma_kernel<KCHV>;                                             \
+        if (lds > 65536)                                                                         \
+            (void)hipFuncSetAttribute((const void *)kern,                                        \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
+        hipLaunchKernelGGL(kern, grid, block, lds, s, X, n, Kp, C32, chalf, k, unit, cmax,       \
+                           labels, prev, changed, list_count, list);                             \
+    }
+    switch (Kp / 4) {
+    case 4: DA_ASSIGN(4) break;
+    case 8: DA_ASSIGN(8) break;
+    case 12: DA_ASSIGN(12) break;
+    case 16: DA_ASSIGN(16) break;
+    case 24: DA_ASSIGN(24) break;
+    case 32: DA_ASSIGN(32) break;
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef DA_ASSIGN
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_kmeans_recheck(const T *X, int ldx, int d, const double *C, int k,
+                                 const int *list_count, const int *list, int *labels,
+                                 const int *prev, int *changed, hipStream_t s) {
+    hipLaunchKernelGGL(kmeans_recheck_kernel<T>, dim3(256), dim3(256), 0, s, X, ldx, d, C, k,
+                       list_count, list, labels, prev, changed);
+    return hipGetLastError();
+}
+template hipError_t launch_kmeans_recheck<float>(const float *, int, int, const double *, int,
+                                                 const int *, const int *, int *, const int *,
+                                                 int *, hipStream_t);
+template hipError_t launch_kmeans_recheck<double>(const double *, int, int, const double *, int,
+                                                  const int *, const int *, int *, const int *,
+                                                  int *, hipStream_t);
+
+} // namespace dagpu

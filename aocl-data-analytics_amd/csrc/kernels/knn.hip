@@ -30,6 +30,10 @@ namespace dagpu {
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int WAVES = 4;
 constexpr int ROWT = 2;                   // 16-row query tiles per wave
@@ -76,11 +80,103 @@ __device__ __forceinline__ void compact_query(float *bkey, int *bidx, int *cnt, 
     }
 }
 
-template <int KCH, int KC>
-__global__ __launch_bounds__(256) void knn_tile_f32_kernel(
-    const float *__restrict__ Q, int nq, const float *__restrict__ Y, const float *__restrict__ ny,
-    int n, int Kp, int split_len, int nsplit, float *__restrict__ part_key,
-    int *__restrict__ part_idx, float *__restrict__ part_tau) {
+// MFMA operand traits. E = elements per lane per instruction; rows_strided: the fp64
+// instruction returns rows lane/16 + 4i, the others 4*(lane/16) + i.
+struct OpF32 {
+    using in_t = float;
+    using frag_t = float;
+    using acc_t = v4f;
+    using acc_e = float;
+    static constexpr int E = 1;
+    static constexpr bool rows_strided = false;
+    __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    template <int NF> __device__ static void load(const in_t *p, frag_t *f) {
+#pragma unroll
+        for (int v = 0; v < NF; v += 4) {
+            const v4f x = *reinterpret_cast<const v4f *>(p + v);
+            f[v] = x[0];
+            f[v + 1] = x[1];
+            f[v + 2] = x[2];
+            f[v + 3] = x[3];
+        }
+    }
+};
+struct OpF64 {
+    using in_t = double;
+    using frag_t = double;
+    using acc_t = v4d;
+    using acc_e = double;
+    static constexpr int E = 1;
+    static constexpr bool rows_strided = true;
+    __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    template <int NF> __device__ static void load(const in_t *p, frag_t *f) {
+#pragma unroll
+        for (int v = 0; v < NF; v += 2) {
+            const v2d x = *reinterpret_cast<const v2d *>(p + v);
+            f[v] = x[0];
+            f[v + 1] = x[1];
+        }
+    }
+};
+struct OpF16 {
+    using in_t = _Float16;
+    using frag_t = v8h;
+    using acc_t = v4f;
+    using acc_e = float;
+    static constexpr int E = 8;
+    static constexpr bool rows_strided = false;
+    __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+    template <int NF> __device__ static void load(const in_t *p, frag_t *f) {
+#pragma unroll
+        for (int v = 0; v < NF; v++)
+            f[v] = *reinterpret_cast<const v8h *>(p + 8 * v);
+    }
+};
+struct OpBF16 {
+    using in_t = __bf16;
+    using frag_t = v8bf;
+    using acc_t = v4f;
+    using acc_e = float;
+    static constexpr int E = 8;
+    static constexpr bool rows_strided = false;
+    __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+    template <int NF> __device__ static void load(const in_t *p, frag_t *f) {
+#pragma unroll
+        for (int v = 0; v < NF; v++)
+            f[v] = *reinterpret_cast<const v8bf *>(p + 8 * v);
+    }
+};
+
+__device__ __forceinline__ void wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Sweep of one training split for 128 queries. Scores are x.y - |y|^2/2 (the accumulator
+// starts at -|y|^2/2), so a larger score is a nearer row; keys -2*score are what the
+// candidate buffers, thresholds and the merge use. Each wave owns its 32 queries' buffers,
+// so no workgroup barrier is needed inside the sweep. SPLIT operands carry each value as
+// hi + lo in the 16-bit type (rows stored as [hi Kp][lo Kp]); the product is
+// hi*hi + hi*lo + lo*hi. The next step's training operands are loaded while the current
+// step's MFMAs run (two register buffers).
+template <class Op, int KCH, int KC, bool SPLIT, bool PREFETCH>
+__global__ __launch_bounds__(256) void knn_tile_kernel(
+    const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
+    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
+    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
+    using frag_t = typename Op::frag_t;
+    using acc_t = typename Op::acc_t;
+    using ae = typename Op::acc_e;
+    constexpr int NF = KCH / Op::E;
+    constexpr int NS = SPLIT ? 2 : 1;
     __shared__ float bkey[QB * 64];
     __shared__ int bidx[QB * 64];
     __shared__ int cnt[QB];
@@ -96,91 +192,144 @@ __global__ __launch_bounds__(256) void knn_tile_f32_kernel(
     int t_end = t_begin + split_len;
     if (t_end > n)
         t_end = n;
-
-    for (int t = threadIdx.x; t < QB; t += 256) {
-        cnt[t] = 0;
-        thr[t] = __builtin_huge_valf();
+    const int qbase = (w * ROWT) * 16;
+    const int64_t rowlen = (int64_t)Kp * NS;
+    if (lane < ROWT * 16) {
+        cnt[qbase + lane] = 0;
+        thr[qbase + lane] = __builtin_huge_valf();
     }
 
-    // A operands: this lane's slice of its query rows, resident for the whole sweep.
-    float a[ROWT][KCH];
+    frag_t a[NS][ROWT][NF];
 #pragma unroll
     for (int rt = 0; rt < ROWT; rt++) {
-        const int qr = q0 + (w * ROWT + rt) * 16 + r;
-        const float *src = Q + (int64_t)qr * Kp + g * KCH;
+        const int qr = q0 + qbase + rt * 16 + r;
 #pragma unroll
-        for (int f = 0; f < KCH; f += 4) {
-            v4f v = {0.f, 0.f, 0.f, 0.f};
-            if (qr < nq)
-                v = *reinterpret_cast<const v4f *>(src + f);
-            a[rt][f] = v[0];
-            a[rt][f + 1] = v[1];
-            a[rt][f + 2] = v[2];
-            a[rt][f + 3] = v[3];
-        }
-    }
-    __syncthreads();
-
-    // Query slots of this lane's accumulator rows: 4*g + i of each row tile.
-    const int qbase = (w * ROWT) * 16;
-    for (int tb = t_begin; tb < t_end; tb += STEP) {
-        v4f acc[ROWT][COLT];
-        float nyv[COLT];
-        int col[COLT];
+        for (int h = 0; h < NS; h++) {
+            Op::template load<NF>(Q + (int64_t)(qr < nq ? qr : 0) * rowlen + h * Kp + g * KCH,
+                                  a[h][rt]);
+            if (qr >= nq) {
 #pragma unroll
-        for (int ct = 0; ct < COLT; ct++) {
-#pragma unroll
-            for (int rt = 0; rt < ROWT; rt++)
-                acc[rt][ct] = (v4f){0.f, 0.f, 0.f, 0.f};
-            const int j = tb + ct * 16 + r;
-            col[ct] = j;
-            nyv[ct] = j < t_end ? ny[j] : __builtin_huge_valf();
-            const float *src = Y + (int64_t)(j < t_end ? j : t_begin) * Kp + g * KCH;
-#pragma unroll
-            for (int f = 0; f < KCH; f += 4) {
-                const v4f b = *reinterpret_cast<const v4f *>(src + f);
-#pragma unroll
-                for (int e = 0; e < 4; e++)
-#pragma unroll
-                    for (int rt = 0; rt < ROWT; rt++)
-                        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt][f + e], b[e],
-                                                                           acc[rt][ct], 0, 0, 0);
+                for (int f = 0; f < NF; f++)
+                    a[h][rt][f] = frag_t{};
             }
         }
-        // Epilogue: keys against the running thresholds.
+    }
+    ae ts[ROWT][4];
+    auto row_of = [&](int i) { return Op::rows_strided ? g + 4 * i : 4 * g + i; };
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            ts[rt][i] = -__builtin_huge_val();
+    wave_fence();
+
+    auto load_step = [&](int tb, frag_t (&b)[COLT][NS][NF], ae (&hv)[COLT]) {
+#pragma unroll
+        for (int ct = 0; ct < COLT; ct++) {
+            const int j = tb + ct * 16 + r;
+            const bool ok = j < t_end;
+            hv[ct] = ok ? nyh[j] : (ae)(-__builtin_huge_val());
+#pragma unroll
+            for (int h = 0; h < NS; h++)
+                Op::template load<NF>(Y + (int64_t)(ok ? j : t_begin) * rowlen + h * Kp + g * KCH,
+                                      b[ct][h]);
+        }
+    };
+
+    auto compute_step = [&](int tb, const frag_t (&b)[COLT][NS][NF], const ae (&hv)[COLT]) {
+        acc_t acc[ROWT][COLT];
+#pragma unroll
+        for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+            for (int rt = 0; rt < ROWT; rt++)
+                acc[rt][ct] = acc_t{hv[ct], hv[ct], hv[ct], hv[ct]};
+        // All ROWT x COLT accumulators advance together: independent MFMA chains keep the
+        // matrix pipeline busy while each result is still in flight.
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+                for (int rt = 0; rt < ROWT; rt++) {
+                    acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][0][f], acc[rt][ct]);
+                    if constexpr (SPLIT) {
+                        acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][NS - 1][f], acc[rt][ct]);
+                        acc[rt][ct] = Op::mfma(a[NS - 1][rt][f], b[ct][0][f], acc[rt][ct]);
+                    }
+                }
+        }
+        bool hit = false;
+#pragma unroll
+        for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int ct = 0; ct < COLT; ct++)
+                    hit |= acc[rt][ct][i] > ts[rt][i];
+        if (__ballot(hit) == 0)
+            return;
 #pragma unroll
         for (int rt = 0; rt < ROWT; rt++) {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const int q = qbase + rt * 16 + 4 * g + i;
-                const float th = thr[q];
+                const int q = qbase + rt * 16 + row_of(i);
 #pragma unroll
                 for (int ct = 0; ct < COLT; ct++) {
-                    const float key = fmaf(-2.0f, acc[rt][ct][i], nyv[ct]);
-                    if (key < th) {
+                    if (acc[rt][ct][i] > ts[rt][i]) {
                         const int pos = atomicAdd(&cnt[q], 1);
-                        bkey[q * 64 + pos] = key;
-                        bidx[q * 64 + pos] = col[ct];
+                        bkey[q * 64 + pos] = (float)(-2.0 * (double)acc[rt][ct][i]);
+                        bidx[q * 64 + pos] = tb + ct * 16 + r;
                     }
                 }
             }
         }
-        __syncthreads();
-        // Compact the buffers that could overflow in the next step.
-        const int myq = qbase + (lane & 31);
-        const bool need = (lane < ROWT * 16) && (cnt[myq] > 64 - STEP);
+        wave_fence();
+        const bool need = (lane < ROWT * 16) && (cnt[qbase + (lane & 31)] > 64 - STEP);
         unsigned long long mask = __ballot(need);
-        while (mask) {
-            const int b = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            compact_query<KC>(bkey, bidx, cnt, thr, qbase + b);
+        if (mask) {
+            while (mask) {
+                const int bq = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                compact_query<KC>(bkey, bidx, cnt, thr, qbase + bq);
+            }
+            wave_fence();
+#pragma unroll
+            for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    ts[rt][i] = (ae)(-0.5 * (double)thr[qbase + rt * 16 + row_of(i)]);
         }
-        __syncthreads();
+    };
+
+    if (!PREFETCH) {
+        for (int tb = t_begin; tb < t_end; tb += STEP) {
+            frag_t b0[COLT][NS][NF];
+            ae h0[COLT];
+            load_step(tb, b0, h0);
+            compute_step(tb, b0, h0);
+        }
+    } else if (t_begin < t_end) {
+        frag_t b0[COLT][NS][NF], b1[COLT][NS][NF];
+        ae h0[COLT], h1[COLT];
+        load_step(t_begin, b0, h0);
+        for (int tb = t_begin;; tb += 2 * STEP) {
+            const bool more1 = tb + STEP < t_end;
+            if (more1)
+                load_step(tb + STEP, b1, h1);
+            compute_step(tb, b0, h0);
+            if (!more1)
+                break;
+            const bool more2 = tb + 2 * STEP < t_end;
+            if (more2)
+                load_step(tb + 2 * STEP, b0, h0);
+            compute_step(tb + STEP, b1, h1);
+            if (!more2)
+                break;
+        }
     }
-    // Final compaction and output of every query of this wave.
     for (int b = 0; b < ROWT * 16; b++)
         compact_query<KC>(bkey, bidx, cnt, thr, qbase + b);
-    __syncthreads();
+    wave_fence();
     for (int b = 0; b < ROWT * 16; b++) {
         const int q = q0 + qbase + b;
         if (q >= nq)
@@ -201,7 +350,8 @@ template <typename T>
 __global__ __launch_bounds__(64) void knn_merge_kernel(
     const float *__restrict__ part_key, const int *__restrict__ part_idx,
     const float *__restrict__ part_tau, int nsplit, int kc, const T *__restrict__ Q,
-    const T *__restrict__ Yrows, int d, int k, double unit, double ymax, int post,
+    const T *__restrict__ Yrows, int d, int k, double u_in, double abs_in, double u_acc,
+    double ymax, int post,
     T *__restrict__ out_dist, int *__restrict__ out_idx, int *__restrict__ fb_count,
     int *__restrict__ fb_list) {
     __shared__ float skey[256];
@@ -252,11 +402,19 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
             __syncthreads();
         }
     }
-    // Error bound of the fp32 key ||y||^2 - 2 q.y: products and a d-term accumulation in
-    // fp32 (gamma_{d+1}), the rounded norm and the final fma.
+    // Error bound of the stored key |y|^2 - 2 q.y. With S = sum|q_l y_l| <= |q| ymax: the
+    // operands rounded to the input precision (2u_in + u_in^2) S; the d+1-term accumulation
+    // from -|y|^2/2 (gamma_{d+2}); the rounded -|y|^2/2; the float storage of the key.
     const double qnorm = sqrt(qn);
-    const double gam = (d + 1) * unit / (1.0 - (d + 1) * unit);
-    const double E = (2.0 * gam + 2.5 * unit) * qnorm * ymax + 2.5 * unit * ymax * ymax;
+    const double S = qnorm * ymax;
+    const double gam = (d + 2) * u_acc / (1.0 - (d + 2) * u_acc);
+    const double u32 = 1.0 / 16777216.0;
+    const double E = 2.0 * ((2.0 * u_in + u_in * u_in) * S +
+                            gam * (S * (1.0 + u_in) * (1.0 + u_in) + 0.5 * ymax * ymax) +
+                            u_acc * 0.5 * ymax * ymax) +
+                     u32 * (ymax * ymax + 2.01 * S) +
+                     // operands below the normal range of fp16 carry an absolute error
+                     2.0 * abs_in * sqrt((double)d) * (qnorm + ymax) * 1.01;
     const float ak = skey[k - 1];
     const double lim = (double)ak + 2.0 * E;
     // Candidates within 2E of the k-th key, re-ranked with exact distances.
@@ -331,55 +489,100 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
 
 int knn_tile_kc(int k) { return k <= 16 ? 16 : 32; }
 
-hipError_t launch_knn_tile_f32(const float *Q, int nq, const float *Y, const float *ny, int n,
-                               int Kp, int nsplit, int kc, float *part_key, int *part_idx,
-                               float *part_tau, hipStream_t s) {
+
+template <class Op, int KC, bool SPLIT, bool PREFETCH>
+static hipError_t launch_tile_pf(const void *Q, int nq, const void *Y, const void *nyh, int n,
+                                 int Kp, int nsplit, float *part_key, int *part_idx,
+                                 float *part_tau, hipStream_t s) {
+    using in_t = typename Op::in_t;
+    using ae = typename Op::acc_e;
     int split_len = (n + nsplit - 1) / nsplit;
     split_len = (split_len + STEP - 1) / STEP * STEP;
     const int qblocks = (nq + QB - 1) / QB;
     dim3 grid(qblocks * nsplit), block(256);
-    const int kch = Kp / 4;
-#define DA_TILE(KCHV, KCV)                                                                     \
-    hipLaunchKernelGGL((knn_tile_f32_kernel<KCHV, KCV>), grid, block, 0, s, Q, nq, Y, ny, n, Kp, \
-                       split_len, nsplit, part_key, part_idx, part_tau)
-#define DA_TILE_K(KCHV)                                                                        \
-    if (kc == 16)                                                                              \
-        DA_TILE(KCHV, 16);                                                                     \
-    else                                                                                       \
-        DA_TILE(KCHV, 32);
-    switch (kch) {
-    case 4: DA_TILE_K(4) break;
-    case 8: DA_TILE_K(8) break;
-    case 12: DA_TILE_K(12) break;
-    case 16: DA_TILE_K(16) break;
-    case 20: DA_TILE_K(20) break;
-    case 24: DA_TILE_K(24) break;
-    case 28: DA_TILE_K(28) break;
-    case 32: DA_TILE_K(32) break;
+    const in_t *q = static_cast<const in_t *>(Q);
+    const in_t *y = static_cast<const in_t *>(Y);
+    const ae *h = static_cast<const ae *>(nyh);
+#define DA_TILE(KCHV)                                                                           \
+    case KCHV:                                                                                  \
+        hipLaunchKernelGGL((knn_tile_kernel<Op, KCHV, KC, SPLIT, PREFETCH>), grid, block, 0, s, \
+                           q, nq, y,                                                           \
+                           h, n,                                                               \
+                           Kp, split_len, nsplit, part_key, part_idx, part_tau);               \
+        break;
+    switch (Kp / 4) {
+        DA_TILE(8)
+        DA_TILE(16)
+        DA_TILE(24)
+        DA_TILE(32)
     default:
         return hipErrorInvalidValue;
     }
-#undef DA_TILE_K
 #undef DA_TILE
     return hipGetLastError();
+}
+
+template <class Op, int KC, bool SPLIT>
+static hipError_t launch_tile_op(const void *Q, int nq, const void *Y, const void *nyh, int n,
+                                 int Kp, int nsplit, float *part_key, int *part_idx,
+                                 float *part_tau, int variant, hipStream_t s) {
+    if (variant & 1)
+        return launch_tile_pf<Op, KC, SPLIT, true>(Q, nq, Y, nyh, n, Kp, nsplit, part_key,
+                                                   part_idx, part_tau, s);
+    return launch_tile_pf<Op, KC, SPLIT, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx,
+                                                part_tau, s);
+}
+
+int knn_tile_kp(int d) {
+    // Padded feature count: a multiple of 32 so that every operand type gets whole 16-byte
+    // slices per lane group.
+    return (d + 31) / 32 * 32;
+}
+
+hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
+                           int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
+                           float *part_tau, int variant, hipStream_t s) {
+    if (kc != 16)
+        return hipErrorInvalidValue;
+    switch (prec) {
+    case KP_F32:
+        return launch_tile_op<OpF32, 16, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
+    case KP_F64:
+        return launch_tile_op<OpF64, 16, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
+    case KP_F16:
+        return launch_tile_op<OpF16, 16, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
+    case KP_BF16:
+        return launch_tile_op<OpBF16, 16, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
+    case KP_F16X3:
+        return launch_tile_op<OpF16, 16, true>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
+    case KP_BF16X3:
+        return launch_tile_op<OpBF16, 16, true>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
+    default:
+        return hipErrorInvalidValue;
+    }
 }
 
 template <typename T>
 hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const float *part_tau,
                             int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,
-                            double unit, double ymax, int post, T *out_dist, int *out_idx,
-                            int *fb_count, int *fb_list, hipStream_t s) {
+                            double u_in, double abs_in, double u_acc, double ymax, int post,
+                            T *out_dist, int *out_idx, int *fb_count, int *fb_list, hipStream_t s) {
     if (nsplit * kc > 256 || d > 1024 || k > 128)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(knn_merge_kernel<T>, dim3(nq), dim3(64), 0, s, part_key, part_idx,
-                       part_tau, nsplit, kc, Q, Y, d, k, unit, ymax, post, out_dist, out_idx,
+                       part_tau, nsplit, kc, Q, Y, d, k, u_in, abs_in, u_acc, ymax, post, out_dist,
+                       out_idx,
                        fb_count, fb_list);
     return hipGetLastError();
 }
 
 template hipError_t launch_knn_merge<float>(const float *, const int *, const float *, int, int,
                                             const float *, const float *, int, int, int, double,
-                                            double, int, float *, int *, int *, int *,
-                                            hipStream_t);
+                                            double, double, double, int, float *, int *, int *,
+                                            int *, hipStream_t);
+template hipError_t launch_knn_merge<double>(const float *, const int *, const float *, int, int,
+                                             const double *, const double *, int, int, int,
+                                             double, double, double, double, int, double *, int *,
+                                             int *, int *, hipStream_t);
 
 } // namespace dagpu

@@ -29,6 +29,12 @@ template <typename T, typename U>
 hipError_t launch_pack_rows(const T *src, int64_t lds, int rows, int cols, U *dst, int64_t ldd,
                             int rows_pad, hipStream_t s);
 
+// launch_pack_rows into the operand type of a fused-tile precision code (KP_* below):
+// double, float, _Float16 or __bf16.
+template <typename T>
+hipError_t launch_pack_rows_prec(const T *src, int64_t lds, int rows, int cols, void *dst,
+                                 int64_t ldd, int rows_pad, int prec, hipStream_t s);
+
 // Squared row norms in double: out[i] = sum_j x[i*ld+j]^2.
 template <typename T>
 hipError_t launch_row_sqnorms(const T *X, int64_t ld, int rows, int cols, double *out,
@@ -150,18 +156,56 @@ namespace dagpu {
 // ---- fused kNN (fp32 MFMA) ----
 // Candidates kept per query and split for a request of k neighbours.
 int knn_tile_kc(int k);
-// Q: nq x Kp, Y: n x Kp (row-major, zero padded, Kp a multiple of 16, Kp <= 128), ny: fp32
-// squared norms of Y. Writes part_key/part_idx[(q*nsplit + s)*kc + i] and
-// part_tau[q*nsplit + s].
-hipError_t launch_knn_tile_f32(const float *Q, int nq, const float *Y, const float *ny, int n,
-                               int Kp, int nsplit, int kc, float *part_key, int *part_idx,
-                               float *part_tau, hipStream_t s);
-// Q: nq x d and Y: n x d unpadded rows (type T); unit: roundoff of the key arithmetic;
+// Operand precision of the fused tile.
+enum KnnPrec : int { KP_F32 = 0, KP_F64 = 1, KP_F16 = 2, KP_BF16 = 3, KP_F16X3 = 4, KP_BF16X3 = 5 };
+// Bytes per padded operand row element group: split forms store [hi Kp][lo Kp] per row.
+inline int knn_prec_row_factor(int prec) { return prec >= KP_F16X3 ? 2 : 1; }
+inline int knn_prec_elem_size(int prec) { return prec == KP_F64 ? 8 : prec == KP_F32 ? 4 : 2; }
+// Padded feature count used by the fused tile for d features (a multiple of 32, <= 128).
+int knn_tile_kp(int d);
+// Q: nq x Kp, Y: n x Kp row-major zero-padded rows in the operand type of prec (float,
+// double, _Float16 or __bf16); nyh: -|y|^2/2 per row (double for KP_F64, float otherwise).
+// Writes part_key/part_idx[(q*nsplit + s)*kc + i] (keys |y|^2 - 2 q.y as float) and
+// part_tau[q*nsplit + s]. kc must be 16. variant bit 0: prefetch the next training tile
+// into a second register buffer.
+hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
+                           int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
+                           float *part_tau, int variant, hipStream_t s);
+// Q: nq x d and Y: n x d unpadded rows (type T); u_in: relative representation error of the
+// tile operands (0 when they are the data themselves); abs_in: absolute representation error
+// per element (fp16 subnormals); u_acc: unit roundoff of the tile accumulation;
 // ymax: max row norm of Y. Certified queries get out_dist/out_idx (row-major nq x k);
 // the others are appended to fb_list (fb_count must be zeroed before).
 template <typename T>
 hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const float *part_tau,
                             int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,
-                            double unit, double ymax, int post, T *out_dist, int *out_idx,
-                            int *fb_count, int *fb_list, hipStream_t s);
+                            double u_in, double abs_in, double u_acc, double ymax, int post,
+                            T *out_dist, int *out_idx, int *fb_count, int *fb_list, hipStream_t s);
+} // namespace dagpu
+
+namespace dagpu {
+// ---- MFMA k-means assignment (fp32 data) ----
+// C32[c*Kp + l] = (float)C[c*d + l] (zero padded), chalf[c] = -||C_c||^2/2, *cmax2 =
+// max ||C_c||^2 (must be zeroed before).
+hipError_t launch_centre_prep(const double *C, int k, int d, int Kp, float *C32, float *chalf,
+                              double *cmax2, hipStream_t s);
+size_t kmeans_assign_mfma_lds(int k, int Kp);
+// X: n x Kp zero-padded fp32 rows, Kp in {16, 32, 48, 64, 96, 128}. Writes labels; samples
+// whose best two centres are within the fp32 error bound are appended to list (count in
+// *list_count, zeroed before) for launch_kmeans_recheck; the others update *changed.
+hipError_t launch_kmeans_assign_mfma(const float *X, int n, int Kp, const float *C32,
+                                     const float *chalf, int k, double cmax, int *labels,
+                                     const int *prev, int *changed, int *list_count, int *list,
+                                     hipStream_t s);
+template <typename T>
+hipError_t launch_kmeans_recheck(const T *X, int ldx, int d, const double *C, int k,
+                                 const int *list_count, const int *list, int *labels,
+                                 const int *prev, int *changed, hipStream_t s);
+} // namespace dagpu
+
+namespace dagpu {
+// Row gather/scatter for the kNN fallback: dst[i*ld + c] = src[idx[i]*ld + c] (gather) and
+// dst[idx[i]*ld + c] = src[i*ld + c] (scatter), c < cols, i < m. elem: 4 or 8 bytes.
+hipError_t launch_gather_scatter(const void *src, void *dst, const int *idx, int m, int cols,
+                                 int elem, int scatter, hipStream_t s);
 } // namespace dagpu

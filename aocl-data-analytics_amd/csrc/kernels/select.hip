@@ -178,3 +178,35 @@ template hipError_t launch_knn_finalize<float>(const double *, const int *, int,
 template hipError_t launch_knn_finalize<double>(const double *, const int *, int, int, int,
                                                 double, double *, int *, int64_t, hipStream_t);
 } // namespace dagpu
+
+namespace dagpu {
+namespace {
+template <typename E>
+__global__ void gather_scatter_kernel(const E *__restrict__ src, E *__restrict__ dst,
+                                      const int *__restrict__ idx, int m, int cols, int scatter) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)m * cols)
+        return;
+    const int64_t i = e / cols, c = e - i * cols;
+    const int64_t row = idx[i];
+    if (scatter)
+        dst[row * cols + c] = src[i * cols + c];
+    else
+        dst[i * cols + c] = src[row * cols + c];
+}
+} // namespace
+hipError_t launch_gather_scatter(const void *src, void *dst, const int *idx, int m, int cols,
+                                 int elem, int scatter, hipStream_t s) {
+    const int64_t total = (int64_t)m * cols;
+    dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    if (elem == 8)
+        hipLaunchKernelGGL(gather_scatter_kernel<long long>, grid, block, 0, s,
+                           static_cast<const long long *>(src), static_cast<long long *>(dst), idx,
+                           m, cols, scatter);
+    else
+        hipLaunchKernelGGL(gather_scatter_kernel<int>, grid, block, 0, s,
+                           static_cast<const int *>(src), static_cast<int *>(dst), idx, m, cols,
+                           scatter);
+    return hipGetLastError();
+}
+} // namespace dagpu

@@ -147,6 +147,21 @@ da_status KMeansHandle<T>::set_data(da_int n_samples, da_int n_features, const T
     s = prepare_scales_for<T>(X_.as<T>());
     if (s != da_status_success)
         return s;
+    mfma_ok_ = false;
+    if constexpr (std::is_same_v<T, float>) {
+        // Padded copy for the MFMA assignment (feature count rounded up to 16, at most 128).
+        if (n_features <= 128) {
+            kp_ = (n_features + 15) / 16 * 16;
+            if (kp_ == 80 || kp_ == 112)
+                kp_ = kp_ == 80 ? 96 : 128;
+            DA_HIP(err, xpad_.reserve((size_t)n_samples * kp_ * sizeof(float)));
+            DA_HIP(err, (launch_pack_rows<float, float>(X_.as<float>(), n_features, n_samples,
+                                                        n_features, xpad_.as<float>(), kp_,
+                                                        n_samples, gpu->stream())));
+            DA_HIP(err, hipStreamSynchronize(gpu->stream()));
+            mfma_ok_ = true;
+        }
+    }
     initdone_ = true;
     model_trained = false;
     // n_clusters is bounded by n_samples once the data is known (kmeans.cpp:283-297).
@@ -214,8 +229,31 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
     hipStream_t st = gpu->stream();
     const int n = n_samples_, d = n_features_, k = n_clusters_;
     int *dchanged = reinterpret_cast<int *>(small_.as<double>() + 4);
-    DA_HIP(err, hipMemsetAsync(dchanged, 0, sizeof(int), st));
-    {
+    DA_HIP(err, hipMemsetAsync(dchanged, 0, 2 * sizeof(int), st));
+    const bool use_mfma = std::is_same_v<U, float> && mfma_ok_ &&
+                          (const void *)dX == (const void *)X_.ptr() &&
+                          kmeans_assign_mfma_lds(k, kp_) <= 160 * 1024;
+    if (use_mfma) {
+        // fp32 cross term on MFMA, exact re-check of near ties (kmeans_mfma.hip).
+        int *dlist_count = dchanged + 1;
+        double *cmax2 = small_.as<double>() + 6;
+        DA_HIP(err, hipMemsetAsync(cmax2, 0, sizeof(double), st));
+        DA_HIP(err, launch_centre_prep(Cin, k, d, kp_, c32_.as<float>(), chalf_.as<float>(), cmax2, st));
+        double hc = 0;
+        DA_HIP(err, hipMemcpyAsync(&hc, cmax2, sizeof(double), hipMemcpyDeviceToHost, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        {
+            KernelScope ks(*gpu, "kmeans_assign");
+            DA_HIP(err, launch_kmeans_assign_mfma(xpad_.as<float>(), n, kp_, c32_.as<float>(),
+                                                  chalf_.as<float>(), k, std::sqrt(hc), lab_new,
+                                                  lab_old, dchanged, dlist_count, list_.as<int>(), st));
+        }
+        {
+            KernelScope ks(*gpu, "kmeans_recheck");
+            DA_HIP(err, launch_kmeans_recheck<U>(dX, d, d, Cin, k, dlist_count, list_.as<int>(),
+                                                 lab_new, lab_old, dchanged, st));
+        }
+    } else {
         KernelScope ks(*gpu, "kmeans_assign");
         DA_HIP(err, launch_kmeans_assign<U>(dX, n, d, Cin, k, lab_new, lab_old, dchanged,
                                             mind_.as<double>(), st));
@@ -590,6 +628,11 @@ template <typename T> da_status KMeansHandle<T>::compute() {
     ppb_ = (n + nblocks_ - 1) / nblocks_;
     nblocks_ = (n + ppb_ - 1) / ppb_;
     DA_HIP(err, part_.reserve((size_t)nblocks_ * kc_ * d * 2 * sizeof(long long)));
+    if (mfma_ok_) {
+        DA_HIP(err, c32_.reserve((size_t)(k + 16) * kp_ * sizeof(float)));
+        DA_HIP(err, chalf_.reserve((size_t)(k + 16) * sizeof(float)));
+        DA_HIP(err, list_.reserve((size_t)n * sizeof(int)));
+    }
     DA_HIP(err, pcnt_.reserve((size_t)nblocks_ * kc_ * sizeof(int)));
     Ccur_ = Ca_.as<double>();
     Cprev_ = Cb_.as<double>();

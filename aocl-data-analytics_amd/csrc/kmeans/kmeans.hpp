@@ -62,6 +62,11 @@ template <typename T> class KMeansHandle : public AlgHandle {
         small_, w_, cp_, tmp_, idx_, Cbest_, q_, q_stage_, out_;
     da_int ppb_ = 0, nblocks_ = 0, kc_ = 0;
     std::vector<int> escale_host_;
+    // fp32 MFMA assignment: zero-padded copy of the data (n x kp_), fp32 centres, -|c|^2/2,
+    // the re-check list.
+    DeviceBuffer xpad_, c32_, chalf_, list_;
+    int kp_ = 0;
+    bool mfma_ok_ = false;
     da_int empty_mode_ = 0;
     double *Ccur_ = nullptr, *Cprev_ = nullptr;
     int *lab_cur_ = nullptr, *lab_prev_ = nullptr;

@@ -55,6 +55,48 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
     opts.add_string("outlier handling",
                     "How to handle samples with no neighbors within the specified radius.",
                     {{"none", 0}, {"manual", 1}, {"most frequent", 2}}, "none");
+    // MI355X extension: operand precision of the distance tiles of the squared-euclidean
+    // search. The candidates are always re-ranked with exact double-precision distances and
+    // the result is certified against the error bound of the chosen precision.
+    opts.add_string("cross term precision",
+                    "Operand precision of the MFMA distance tiles: working, fp32, fp16, bf16, "
+                    "fp16x3 or bf16x3 (hi+lo split operands, three products).",
+                    {{"working", 0}, {"fp32", 1}, {"fp16", 2}, {"bf16", 3}, {"fp16x3", 4},
+                     {"bf16x3", 5}},
+                    "working");
+    // MI355X extension: kernel variant of the fused tile (-1 = chosen by the tuning table,
+    // 0 = single register buffer, 1 = prefetch the next training tile).
+    opts.add_int("tile variant", "Kernel variant of the fused kNN distance tile (-1 automatic).",
+                 -1, LowerBound::greater_equal, 1, UpperBound::less_equal, -1);
+}
+
+// Tuning table of the fused tile variant per operand precision and padded width.
+static int knn_tile_auto_variant(int prec, int kp) {
+    (void)kp;
+    switch (prec) {
+    case KP_F16:
+    case KP_BF16:
+        return 1;
+    default:
+        return 1;
+    }
+}
+
+template <typename T> int NNHandle<T>::tile_prec() const {
+    switch (opts.skey("cross term precision")) {
+    case 1:
+        return KP_F32;
+    case 2:
+        return KP_F16;
+    case 3:
+        return KP_BF16;
+    case 4:
+        return KP_F16X3;
+    case 5:
+        return KP_BF16X3;
+    default:
+        return std::is_same_v<T, double> ? KP_F64 : KP_F32;
+    }
 }
 
 template <typename T> da_status NNHandle<T>::set_params() {
@@ -135,13 +177,14 @@ da_status NNHandle<T>::set_data(da_int n_samples, da_int n_features, const T *X,
     n_features_ = n_features;
     trained_ = true;
     fused_ready_ = false;
-    return prepare_fused();
+    ypad_prec_ = -1;
+    return prepare_fused(tile_prec());
 }
 
 // Squared-euclidean metrics are searched by the fused MFMA kernel (fp32 data, up to 128
 // features, k up to 16); everything else uses the key-tile + selection path.
 template <typename T> bool NNHandle<T>::fused_eligible(da_int k) const {
-    if (!std::is_same_v<T, float> || !fused_ready_)
+    if (!fused_ready_)
         return false;
     const bool sq = metric_ == da_euclidean || metric_ == da_sqeuclidean ||
                     metric_ == da_euclidean_gemm || metric_ == da_sqeuclidean_gemm ||
@@ -149,108 +192,143 @@ template <typename T> bool NNHandle<T>::fused_eligible(da_int k) const {
     return sq && k <= 16 && n_features_ <= 128 && n_samples_ >= 16;
 }
 
-template <typename T> da_status NNHandle<T>::prepare_fused() {
-    if constexpr (std::is_same_v<T, float>) {
-        if (n_features_ > 128)
-            return da_status_success;
-        hipStream_t st = gpu->stream();
-        kp_ = (n_features_ + 15) / 16 * 16;
-        DA_HIP(err, ypad_.reserve((size_t)n_samples_ * kp_ * sizeof(float)));
-        DA_HIP(err, (launch_pack_rows<float, float>(train_.as<float>(), n_features_, n_samples_,
-                                                    n_features_, ypad_.as<float>(), kp_,
-                                                    n_samples_, st)));
-        std::vector<double> sq(n_samples_);
-        DA_HIP(err, hipMemcpyAsync(sq.data(), train_sq_.ptr(), n_samples_ * sizeof(double),
-                                   hipMemcpyDeviceToHost, st));
+// Builds the zero-padded copy of the training rows in the operand precision of the fused
+// tile, -|y|^2/2 per row and the largest row norm.
+template <typename T> da_status NNHandle<T>::prepare_fused(int prec) {
+    if (n_features_ > 128)
+        return da_status_success;
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_;
+    kp_ = knn_tile_kp(d);
+    const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
+    DA_HIP(err, ypad_.reserve((size_t)n * kp_ * esz));
+    const T *src = train_.as<T>();
+    DA_HIP(err, launch_pack_rows_prec<T>(src, d, n, d, ypad_.ptr(), kp_, n, prec, st));
+    std::vector<double> sq(n);
+    DA_HIP(err, hipMemcpyAsync(sq.data(), train_sq_.ptr(), n * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    double mx = 0;
+    for (int i = 0; i < n; i++)
+        mx = std::max(mx, sq[i]);
+    ymax_ = std::sqrt(mx);
+    if (prec == KP_F64) {
+        std::vector<double> h(n);
+        for (int i = 0; i < n; i++)
+            h[i] = -0.5 * sq[i];
+        DA_HIP(err, nyh_.reserve(n * sizeof(double)));
+        DA_HIP(err, hipMemcpyAsync(nyh_.ptr(), h.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
         DA_HIP(err, hipStreamSynchronize(st));
-        std::vector<float> sq32(n_samples_);
-        double mx = 0;
-        for (da_int i = 0; i < n_samples_; i++) {
-            sq32[i] = (float)sq[i];
-            mx = std::max(mx, sq[i]);
-        }
-        ymax_ = std::sqrt(mx);
-        DA_HIP(err, ny32_.reserve(n_samples_ * sizeof(float)));
-        DA_HIP(err, hipMemcpyAsync(ny32_.ptr(), sq32.data(), n_samples_ * sizeof(float),
-                                   hipMemcpyHostToDevice, st));
+    } else {
+        std::vector<float> h(n);
+        for (int i = 0; i < n; i++)
+            h[i] = (float)(-0.5 * sq[i]);
+        DA_HIP(err, nyh_.reserve(n * sizeof(float)));
+        DA_HIP(err, hipMemcpyAsync(nyh_.ptr(), h.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
         DA_HIP(err, hipStreamSynchronize(st));
-        fused_ready_ = true;
     }
+    ypad_prec_ = prec;
+    fused_ready_ = true;
     return da_status_success;
 }
 
 template <typename T>
 da_status NNHandle<T>::search_fused(const T *dQ, da_int nq, da_int k, bool return_distance,
                                     da_int *d_ind, T *d_dist) {
-    if constexpr (!std::is_same_v<T, float>) {
-        return da_status_internal_error;
-    } else {
-        hipStream_t st = gpu->stream();
-        const int n = n_samples_, d = n_features_;
-        const int kc = knn_tile_kc(k);
-        // Split the training rows so that the launch has about four workgroups per CU.
-        const int qblocks = (nq + 127) / 128;
-        int nsplit = (1024 + qblocks - 1) / qblocks;
-        nsplit = std::max(1, std::min({nsplit, 256 / kc, (n + 1023) / 1024}));
-        DA_HIP(err, qpad_.reserve((size_t)nq * kp_ * sizeof(float)));
-        DA_HIP(err, (launch_pack_rows<float, float>(dQ, d, nq, d, qpad_.as<float>(), kp_, nq, st)));
-        const size_t np = (size_t)nq * nsplit;
-        DA_HIP(err, part_key_.reserve(np * kc * sizeof(float)));
-        DA_HIP(err, part_idx_.reserve(np * kc * sizeof(int)));
-        DA_HIP(err, part_tau_.reserve(np * sizeof(float)));
-        DA_HIP(err, fb_.reserve((size_t)(nq + 1) * sizeof(int)));
-        {
-            KernelScope ks(*gpu, "knn_tile");
-            DA_HIP(err, launch_knn_tile_f32(qpad_.as<float>(), nq, ypad_.as<float>(),
-                                            ny32_.as<float>(), n, kp_, nsplit, kc,
-                                            part_key_.as<float>(), part_idx_.as<int>(),
-                                            part_tau_.as<float>(), st));
-        }
-        int *fbc = fb_.as<int>();
-        DA_HIP(err, hipMemsetAsync(fbc, 0, sizeof(int), st));
-        const int post = (metric_ == da_sqeuclidean || metric_ == da_sqeuclidean_gemm) ? 0 : 1;
-        {
-            KernelScope ks(*gpu, "knn_merge");
-            DA_HIP(err, launch_knn_merge<float>(part_key_.as<float>(), part_idx_.as<int>(),
-                                                part_tau_.as<float>(), nsplit, kc, dQ,
-                                                train_.as<float>(), nq, d, k, std::ldexp(1.0, -24),
-                                                ymax_, post, return_distance ? d_dist : nullptr,
-                                                d_ind, fbc, fbc + 1, st));
-        }
-        int nfb = 0;
-        DA_HIP(err, hipMemcpyAsync(&nfb, fbc, sizeof(int), hipMemcpyDeviceToHost, st));
-        DA_HIP(err, hipStreamSynchronize(st));
-        last_candidates = nsplit * kc;
-        last_fallback = nfb;
-        if (nfb > 0) {
-            // Exact fallback for the uncertified queries: gather, key tiles + selection,
-            // scatter back.
-            std::vector<int> list(nfb);
-            DA_HIP(err, hipMemcpyAsync(list.data(), fbc + 1, nfb * sizeof(int),
-                                       hipMemcpyDeviceToHost, st));
-            DA_HIP(err, hipStreamSynchronize(st));
-            DA_HIP(err, fb_q_.reserve((size_t)nfb * d * sizeof(T)));
-            for (int i = 0; i < nfb; i++)
-                DA_HIP(err, hipMemcpyAsync(fb_q_.as<T>() + (size_t)i * d, dQ + (size_t)list[i] * d,
-                                           d * sizeof(T), hipMemcpyDeviceToDevice, st));
-            DA_HIP(err, fb_ind_.reserve((size_t)nfb * k * sizeof(da_int)));
-            DA_HIP(err, fb_dist_.reserve((size_t)nfb * k * sizeof(T)));
-            da_status s = search_tiles(fb_q_.as<T>(), nfb, k, true, fb_ind_.as<da_int>(),
-                                       fb_dist_.as<T>());
-            if (s != da_status_success)
-                return s;
-            for (int i = 0; i < nfb; i++) {
-                DA_HIP(err, hipMemcpyAsync(d_ind + (size_t)list[i] * k,
-                                           fb_ind_.as<da_int>() + (size_t)i * k,
-                                           k * sizeof(da_int), hipMemcpyDeviceToDevice, st));
-                if (return_distance)
-                    DA_HIP(err, hipMemcpyAsync(d_dist + (size_t)list[i] * k,
-                                               fb_dist_.as<T>() + (size_t)i * k, k * sizeof(T),
-                                               hipMemcpyDeviceToDevice, st));
-            }
-        }
-        return da_status_success;
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_;
+    const int prec = tile_prec();
+    if (prec != ypad_prec_) {
+        da_status s = prepare_fused(prec);
+        if (s != da_status_success)
+            return s;
     }
+    const int kc = knn_tile_kc(k);
+    // Split the training rows so that the launch has about four workgroups per CU.
+    const int qblocks = (nq + 127) / 128;
+    int nsplit = (1024 + qblocks - 1) / qblocks;
+    nsplit = std::max(1, std::min({nsplit, 256 / kc, (n + 1023) / 1024}));
+    const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
+    DA_HIP(err, qpad_.reserve((size_t)nq * kp_ * esz));
+    DA_HIP(err, launch_pack_rows_prec<T>(dQ, d, nq, d, qpad_.ptr(), kp_, nq, prec, st));
+    const size_t np = (size_t)nq * nsplit;
+    DA_HIP(err, part_key_.reserve(np * kc * sizeof(float)));
+    DA_HIP(err, part_idx_.reserve(np * kc * sizeof(int)));
+    DA_HIP(err, part_tau_.reserve(np * sizeof(float)));
+    DA_HIP(err, fb_.reserve((size_t)(nq + 1) * sizeof(int)));
+    {
+        KernelScope ks(*gpu, "knn_tile");
+        int variant = opts.ival("tile variant");
+        if (variant < 0)
+            variant = knn_tile_auto_variant(prec, kp_);
+        DA_HIP(err, launch_knn_tile(prec, qpad_.ptr(), nq, ypad_.ptr(), nyh_.ptr(), n, kp_, nsplit,
+                                    kc, part_key_.as<float>(), part_idx_.as<int>(),
+                                    part_tau_.as<float>(), variant, st));
+    }
+    int *fbc = fb_.as<int>();
+    DA_HIP(err, hipMemsetAsync(fbc, 0, sizeof(int), st));
+    const int post = (metric_ == da_sqeuclidean || metric_ == da_sqeuclidean_gemm) ? 0 : 1;
+    // Unit roundoffs: operands (0 when the tile runs in the data precision), accumulation.
+    const bool same = (prec == KP_F64 && std::is_same_v<T, double>) ||
+                      (prec == KP_F32 && std::is_same_v<T, float>);
+    // Representation error of an operand element: relative u_in, absolute abs_in (fp16
+    // subnormals). Split forms keep hi + lo, so only the remainder of lo and the dropped
+    // lo*lo product (folded into u_in) remain.
+    double u_in = 0.0, abs_in = 0.0;
+    const double u16 = std::ldexp(1.0, -11), ub16 = std::ldexp(1.0, -8), u32 = std::ldexp(1.0, -24);
+    if (!same) {
+        switch (prec) {
+        case KP_F32:
+            u_in = u32;
+            break;
+        case KP_F16:
+            u_in = u16 + u32;
+            abs_in = std::ldexp(1.0, -25);
+            break;
+        case KP_BF16:
+            u_in = ub16 + u32;
+            break;
+        case KP_F16X3:
+            u_in = 1.5 * u16 * u16 * (1 + u16) + 2 * u32 * u16 + u32 * u32;
+            abs_in = std::ldexp(1.0, -24);
+            break;
+        case KP_BF16X3:
+            u_in = 1.5 * ub16 * ub16 * (1 + ub16) + 2 * u32 * ub16 + u32 * u32;
+            break;
+        default:
+            break;
+        }
+    }
+    const double u_acc = prec == KP_F64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+    {
+        KernelScope ks(*gpu, "knn_merge");
+        DA_HIP(err, launch_knn_merge<T>(part_key_.as<float>(), part_idx_.as<int>(),
+                                        part_tau_.as<float>(), nsplit, kc, dQ, train_.as<T>(), nq,
+                                        d, k, u_in, abs_in, u_acc, ymax_, post,
+                                        return_distance ? d_dist : nullptr, d_ind, fbc, fbc + 1,
+                                        st));
+    }
+    int nfb = 0;
+    DA_HIP(err, hipMemcpyAsync(&nfb, fbc, sizeof(int), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    last_candidates = nsplit * kc;
+    last_fallback = nfb;
+    if (nfb > 0) {
+        // Exact fallback for the uncertified queries (listed by the merge kernel): gather
+        // their rows, key tiles + selection, scatter the results back.
+        const int *list = fbc + 1;
+        DA_HIP(err, fb_q_.reserve((size_t)nfb * d * sizeof(T)));
+        DA_HIP(err, launch_gather_scatter(dQ, fb_q_.ptr(), list, nfb, d, sizeof(T), 0, st));
+        DA_HIP(err, fb_ind_.reserve((size_t)nfb * k * sizeof(da_int)));
+        DA_HIP(err, fb_dist_.reserve((size_t)nfb * k * sizeof(T)));
+        da_status s = search_tiles(fb_q_.as<T>(), nfb, k, true, fb_ind_.as<da_int>(), fb_dist_.as<T>());
+        if (s != da_status_success)
+            return s;
+        DA_HIP(err, launch_gather_scatter(fb_ind_.ptr(), d_ind, list, nfb, k, sizeof(da_int), 1, st));
+        if (return_distance)
+            DA_HIP(err, launch_gather_scatter(fb_dist_.ptr(), d_dist, list, nfb, k, sizeof(T), 1, st));
+    }
+    return da_status_success;
 }
 
 template <typename T> da_status NNHandle<T>::resolve_k(da_int &k) {

@@ -35,7 +35,7 @@ template <typename T> class NNHandle : public AlgHandle {
     bool fused_eligible(da_int k) const;
     da_status search_fused(const T *dQ, da_int nq, da_int k, bool return_distance,
                            da_int *d_ind, T *d_dist);
-    da_status prepare_fused();
+    da_status prepare_fused(int prec);
 
     bool trained_ = false;
     da_int n_samples_ = 0, n_features_ = 0;
@@ -49,11 +49,13 @@ template <typename T> class NNHandle : public AlgHandle {
     DeviceBuffer q_, q_stage_, q_norm_, keys_, sel_key_, sel_idx_, out_ind_, out_dist_;
     // Fused-path state: zero-padded training rows (n x Kp, fp32), fp32 squared norms,
     // the largest row norm, and per-call candidate buffers.
-    DeviceBuffer ypad_, ny32_, qpad_, part_key_, part_idx_, part_tau_, fb_, fb_q_, fb_ind_,
+    DeviceBuffer ypad_, nyh_, qpad_, part_key_, part_idx_, part_tau_, fb_, fb_q_, fb_ind_,
         fb_dist_;
     int kp_ = 0;
+    int ypad_prec_ = -1; // operand precision the padded copy was built for
     double ymax_ = 0;
     bool fused_ready_ = false;
+    int tile_prec() const;
 };
 
 } // namespace dagpu

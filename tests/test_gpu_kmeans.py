@@ -98,9 +98,9 @@ def test_maxit_warning(oracle):
     np.testing.assert_array_equal(km.labels, r["labels"])
 
 
-def test_many_clusters():
+def test_many_clusters(oracle):
     """kmeans_test_data.hpp:809-858: 2000 points in pairs, 1000 clusters from supplied
-    centres; converges in 2 iterations with inertia <= 0.01."""
+    centres (the reference bounds only the inertia, <= 0.01, for this case)."""
     n, d, k = 2000, 2, 1000
     A = np.zeros((n, d))
     for i in range(0, n, 2):
@@ -108,9 +108,14 @@ def test_many_clusters():
             A[i, j] = i + j * 0.00001
             A[i + 1, j] = i + 0.00001 + j * 0.00001
     C0 = np.array([[2 * i + j * 0.00001 for j in range(d)] for i in range(k)])
-    km = aoclda.clustering.kmeans(n_clusters=k, C=C0, max_iter=50).fit(np.asfortranarray(A))
-    assert km.n_iter == 2
-    assert km.inertia <= 0.01
+    for dtype in (np.float64, np.float32):
+        km = aoclda.clustering.kmeans(n_clusters=k, C=C0.astype(dtype), max_iter=50).fit(
+            np.asfortranarray(A.astype(dtype)))
+        assert km.inertia <= 0.01
+        r = oracle.kmeans(A.astype(dtype).astype(np.float64), k, "supplied",
+                          C0.astype(dtype).astype(np.float64), max_iter=50)
+        assert km.n_iter == r["n_iter"]
+        np.testing.assert_array_equal(km.labels, r["labels"])
 
 
 def test_mixed_precision(known):
@@ -125,3 +130,35 @@ def test_mixed_precision(known):
     assert km.lp_n_iter == 1
     assert km.n_iter == 0
     np.testing.assert_array_equal(km.labels, c["labels"])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_assignment_ties_go_to_lowest_centre(dtype):
+    """Points exactly halfway between two centres take the lower centre index (the scalar
+    kernel's rule, kmeans_kernels.cpp:209-260); the fp32 MFMA path must re-check them."""
+    X = np.array([[0.0, 0.0], [0.0, 1.0], [0.0, -1.0], [-1.0, 0.0], [1.0, 0.0]], dtype=dtype)
+    C0 = np.array([[-1.0, 0.0], [1.0, 0.0]], dtype=dtype)
+    km = aoclda.clustering.kmeans(n_clusters=2, C=C0, max_iter=1).fit(X)
+    # the first three samples are tied in the first assignment and go to centre 0, which
+    # becomes (-0.25, 0); centre 1 stays (1, 0)
+    np.testing.assert_array_equal(km.labels, [0, 0, 0, 0, 1])
+    np.testing.assert_allclose(km.cluster_centres, [[-0.25, 0.0], [1.0, 0.0]])
+    # (0.375, 0) is exactly halfway between the final centres
+    Y = np.array([[0.0, 5.0], [0.0, -3.0], [0.375, 0.0], [0.5, 0.0]], dtype=dtype)
+    np.testing.assert_array_equal(km.predict(Y), [0, 0, 0, 1])
+
+
+def test_larger_mfma_vs_oracle(oracle):
+    """fp32 data large enough for several MFMA workgroups, many clusters, blobs close
+    enough that some samples sit near decision boundaries."""
+    rng = np.random.default_rng(21)
+    centres = rng.uniform(-3, 3, (40, 32))
+    X = blobs(rng, 400, centres, sigma=1.0).astype(np.float32)
+    rng.shuffle(X)
+    C0 = X[:40].copy()
+    km = aoclda.clustering.kmeans(n_clusters=40, C=C0, max_iter=30, tol=0.0).fit(X)
+    r = oracle.kmeans(X.astype(np.float64), 40, "supplied", C0.astype(np.float64), max_iter=30,
+                      tol=0.0)
+    np.testing.assert_array_equal(km.labels, r["labels"])
+    assert km.n_iter == r["n_iter"]
+    np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-5, atol=1e-5)

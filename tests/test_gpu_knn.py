@@ -142,3 +142,53 @@ def test_config_plain_small_knn(oracle):
     ei, ed = oracle.knn(X, Y, 5)
     np.testing.assert_array_equal(ind, ei)
     np.testing.assert_allclose(dist, ed, rtol=1e-12)
+
+
+def gaussian_mixture(rng, n, means, sigma=1.0):
+    comp = rng.integers(0, len(means), n)
+    return means[comp] + sigma * rng.standard_normal((n, means.shape[1]))
+
+
+@pytest.mark.parametrize("precision", ["working", "fp32", "fp16", "bf16", "fp16x3", "bf16x3"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("d", [20, 96, 128])
+def test_precision_forms_exact(oracle, precision, dtype, d):
+    """Every operand precision of the distance tiles must return the exact neighbours: the
+    candidates are re-ranked in double and certified, uncertified queries are recomputed."""
+    rng = np.random.default_rng(d)
+    means = rng.uniform(-10, 10, (16, d))
+    Xt = gaussian_mixture(rng, 6000, means).astype(dtype)
+    Xq = gaussian_mixture(rng, 500, means).astype(dtype)
+    nn = aoclda.neighbors.nearest_neighbors(precision=precision).fit(Xt)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=10)
+    ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), 10)
+    np.testing.assert_array_equal(ind, ei)
+    rtol = 1e-12 if dtype == np.float64 else 1e-5
+    np.testing.assert_allclose(dist, ed, rtol=rtol)
+    nfb, ncand = L.da_int(), L.da_int()
+    L.check(L.lib.da_nn_last_query_stats(nn.handle.h, C.byref(nfb), C.byref(ncand)))
+    assert ncand.value > 0  # the fused tile path ran
+
+
+def test_fused_split_edges(oracle):
+    """Training sizes that leave partial steps and empty splits; queries not a multiple of
+    the 128-query block."""
+    rng = np.random.default_rng(12)
+    for n in (16, 17, 33, 1000, 1025, 4097):
+        Xt = rng.standard_normal((n, 40)).astype(np.float32)
+        Xq = rng.standard_normal((130, 40)).astype(np.float32)
+        nn = aoclda.neighbors.nearest_neighbors().fit(Xt)
+        dist, ind = nn.kneighbors(Xq, n_neighbors=min(16, n))
+        ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), min(16, n))
+        np.testing.assert_array_equal(ind, ei)
+        np.testing.assert_allclose(dist, ed, rtol=1e-5)
+
+
+def test_sqeuclidean_fused(oracle):
+    rng = np.random.default_rng(13)
+    Xt = rng.standard_normal((3000, 64)).astype(np.float32)
+    Xq = rng.standard_normal((200, 64)).astype(np.float32)
+    dist, ind = aoclda.neighbors.nearest_neighbors(metric="sqeuclidean").fit(Xt).kneighbors(Xq, 8)
+    ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), 8, "sqeuclidean")
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-5)
