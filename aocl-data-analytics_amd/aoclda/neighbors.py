@@ -63,3 +63,53 @@ class nearest_neighbors:
     @property
     def handle(self):
         return self._h
+
+
+class nn_server:
+    """Serving loop over a fitted nearest_neighbors object (MI355X extension,
+    da_nn_server_*): submit() queues a request and returns its id without blocking,
+    wait() returns (distances, indices, latency_ms)."""
+
+    def __init__(self, nn, max_batch_queries=8192, max_device_bytes=1 << 30):
+        self.nn = nn
+        self.dtype = nn.dtype
+        self.h = C.c_void_p()
+        L.check(L.lib.da_nn_server_create(nn.handle.h, max_batch_queries, max_device_bytes,
+                                          C.byref(self.h)))
+        self._k = {}
+
+    def submit(self, X, k):
+        X = np.ascontiguousarray(X, dtype=self.dtype)
+        rid = L.da_int()
+        f = getattr(L.lib, f"da_nn_server_submit_{L.suffix(self.dtype)}")
+        L.check(f(self.h, X.shape[0], X.shape[1], L.ptr(X), X.shape[1], k, C.byref(rid)))
+        self._k[rid.value] = (X.shape[0], k)
+        return rid.value
+
+    def wait(self, rid):
+        nq, k = self._k.pop(rid)
+        ind = np.empty((nq, k), dtype=np.int32)
+        dist = np.empty((nq, k), dtype=self.dtype)
+        lat = C.c_double()
+        f = getattr(L.lib, f"da_nn_server_wait_{L.suffix(self.dtype)}")
+        L.check(f(self.h, rid, L.ptr(ind), L.ptr(dist), C.byref(lat)))
+        return dist, ind, lat.value
+
+    def stats(self):
+        out = np.zeros(8)
+        n = L.da_int(8)
+        L.check(L.lib.da_nn_server_stats(self.h, L.ptr(out), C.byref(n)))
+        keys = ("requests", "queries", "batches", "p50_ms", "p99_ms", "queries_per_batch",
+                "elapsed_s", "fallback_queries")
+        return dict(zip(keys, out.tolist()))
+
+    def close(self):
+        if self.h:
+            L.lib.da_nn_server_destroy(C.byref(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

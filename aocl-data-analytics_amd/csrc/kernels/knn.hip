@@ -22,6 +22,7 @@
 // K-permutation: for MFMA step s, lane group g = lane/16 supplies feature g*(Kp/4) + s, so a
 // lane's operands are one contiguous slice of its row and load as 16-byte vectors.
 #include "launch.hpp"
+#include <algorithm>
 #include <climits>
 #include <hip/hip_runtime.h>
 
@@ -35,11 +36,17 @@ typedef double v2d __attribute__((ext_vector_type(2)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-constexpr int WAVES = 4;
-constexpr int ROWT = 2;                   // 16-row query tiles per wave
 constexpr int COLT = 2;                   // 16-row training tiles per step
-constexpr int QB = WAVES * ROWT * 16;     // queries per workgroup
 constexpr int STEP = COLT * 16;           // training rows per step
+
+// Tile shapes per candidate-list length KC: buffer capacity CAP per query, waves per
+// workgroup WV and 16-query row tiles per wave RT. The LDS buffers (WV*RT*16*CAP*8 bytes)
+// stay at or below 64 KB so that two workgroups fit a CU.
+template <int KC> struct TileShape;
+template <> struct TileShape<16> { static constexpr int CAP = 64, WV = 4, RT = 2; };
+template <> struct TileShape<32> { static constexpr int CAP = 128, WV = 4, RT = 1; };
+template <> struct TileShape<64> { static constexpr int CAP = 128, WV = 4, RT = 1; };
+template <> struct TileShape<128> { static constexpr int CAP = 256, WV = 2, RT = 1; };
 
 __device__ __forceinline__ void wave_sort64(float &key, int &idx) {
     const int lane = threadIdx.x & 63;
@@ -61,27 +68,66 @@ __device__ __forceinline__ void wave_sort64(float &key, int &idx) {
     }
 }
 
-// Compacts query q's buffer (count c <= CAP = 64) to its KC best entries.
-template <int KC>
+// Compacts query q's buffer (count c <= CAP) to its KC best entries, sorted by (key,
+// index). CAP = 64: one entry per lane, sorted in registers; larger buffers are sorted in
+// place in LDS by the owning wave.
+template <int KC, int CAP>
 __device__ __forceinline__ void compact_query(float *bkey, int *bidx, int *cnt, float *thr, int q) {
     const int lane = threadIdx.x & 63;
     const int c = cnt[q];
-    float key = lane < c ? bkey[q * 64 + lane] : __builtin_huge_valf();
-    int idx = lane < c ? bidx[q * 64 + lane] : INT_MAX;
-    wave_sort64(key, idx);
-    if (lane < KC) {
-        bkey[q * 64 + lane] = key;
-        bidx[q * 64 + lane] = idx;
-    }
-    const float kth = __shfl(key, KC - 1, 64);
-    if (lane == 0) {
-        cnt[q] = c < KC ? c : KC;
-        thr[q] = c >= KC ? kth : __builtin_huge_valf();
+    float *kk = bkey + q * CAP;
+    int *ii = bidx + q * CAP;
+    if constexpr (CAP == 64) {
+        float key = lane < c ? kk[lane] : __builtin_huge_valf();
+        int idx = lane < c ? ii[lane] : INT_MAX;
+        wave_sort64(key, idx);
+        if (lane < KC) {
+            kk[lane] = key;
+            ii[lane] = idx;
+        }
+        const float kth = __shfl(key, KC - 1, 64);
+        if (lane == 0) {
+            cnt[q] = c < KC ? c : KC;
+            thr[q] = c >= KC ? kth : __builtin_huge_valf();
+        }
+    } else {
+        for (int t = c + lane; t < CAP; t += 64) {
+            kk[t] = __builtin_huge_valf();
+            ii[t] = INT_MAX;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        for (int size = 2; size <= CAP; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int t = lane; t < CAP / 2; t += 64) {
+                    const int i = 2 * t - (t & (stride - 1));
+                    const int j = i + stride;
+                    const bool asc = (i & size) == 0;
+                    const float ki = kk[i], kj = kk[j];
+                    const int xi = ii[i], xj = ii[j];
+                    const bool gt = (kj < ki) || (kj == ki && xj < xi);
+                    if (gt == asc) {
+                        kk[i] = kj;
+                        kk[j] = ki;
+                        ii[i] = xj;
+                        ii[j] = xi;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        if (lane == 0) {
+            cnt[q] = c < KC ? c : KC;
+            thr[q] = c >= KC ? kk[KC - 1] : __builtin_huge_valf();
+        }
     }
 }
 
 // MFMA operand traits. E = elements per lane per instruction; rows_strided: the fp64
-// instruction returns rows lane/16 + 4i, the others 4*(lane/16) + i.
+// instruction returns rows lane/16 + 4i, the others 4*(lane/16) + i. load<NF, CS> reads a
+// lane's NF fragments as 16-byte chunks CS elements apart (CS = 16 bytes: contiguous row
+// slice; CS = 1024 bytes: the tiled training layout).
 struct OpF32 {
     using in_t = float;
     using frag_t = float;
@@ -92,10 +138,10 @@ struct OpF32 {
     __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
-    template <int NF> __device__ static void load(const in_t *p, frag_t *f) {
+    template <int NF, int CS> __device__ static void load(const in_t *p, frag_t *f) {
 #pragma unroll
         for (int v = 0; v < NF; v += 4) {
-            const v4f x = *reinterpret_cast<const v4f *>(p + v);
+            const v4f x = *reinterpret_cast<const v4f *>(p + (v / 4) * CS);
             f[v] = x[0];
             f[v + 1] = x[1];
             f[v + 2] = x[2];
@@ -113,10 +159,10 @@ struct OpF64 {
     __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
-    template <int NF> __device__ static void load(const in_t *p, frag_t *f) {
+    template <int NF, int CS> __device__ static void load(const in_t *p, frag_t *f) {
 #pragma unroll
         for (int v = 0; v < NF; v += 2) {
-            const v2d x = *reinterpret_cast<const v2d *>(p + v);
+            const v2d x = *reinterpret_cast<const v2d *>(p + (v / 2) * CS);
             f[v] = x[0];
             f[v + 1] = x[1];
         }
@@ -132,10 +178,10 @@ struct OpF16 {
     __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
-    template <int NF> __device__ static void load(const in_t *p, frag_t *f) {
+    template <int NF, int CS> __device__ static void load(const in_t *p, frag_t *f) {
 #pragma unroll
         for (int v = 0; v < NF; v++)
-            f[v] = *reinterpret_cast<const v8h *>(p + 8 * v);
+            f[v] = *reinterpret_cast<const v8h *>(p + v * CS);
     }
 };
 struct OpBF16 {
@@ -148,10 +194,10 @@ struct OpBF16 {
     __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
-    template <int NF> __device__ static void load(const in_t *p, frag_t *f) {
+    template <int NF, int CS> __device__ static void load(const in_t *p, frag_t *f) {
 #pragma unroll
         for (int v = 0; v < NF; v++)
-            f[v] = *reinterpret_cast<const v8bf *>(p + 8 * v);
+            f[v] = *reinterpret_cast<const v8bf *>(p + v * CS);
     }
 };
 
@@ -164,21 +210,27 @@ __device__ __forceinline__ void wave_fence() {
 // starts at -|y|^2/2), so a larger score is a nearer row; keys -2*score are what the
 // candidate buffers, thresholds and the merge use. Each wave owns its 32 queries' buffers,
 // so no workgroup barrier is needed inside the sweep. SPLIT operands carry each value as
-// hi + lo in the 16-bit type (rows stored as [hi Kp][lo Kp]); the product is
+// hi + lo in the 16-bit type (query rows stored as [hi Kp][lo Kp]); the product is
 // hi*hi + hi*lo + lo*hi. The next step's training operands are loaded while the current
 // step's MFMAs run (two register buffers).
 template <class Op, int KCH, int KC, bool SPLIT, bool PREFETCH>
-__global__ __launch_bounds__(256) void knn_tile_kernel(
+__global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
     float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
     using frag_t = typename Op::frag_t;
     using acc_t = typename Op::acc_t;
     using ae = typename Op::acc_e;
+    using in_t = typename Op::in_t;
     constexpr int NF = KCH / Op::E;
     constexpr int NS = SPLIT ? 2 : 1;
-    __shared__ float bkey[QB * 64];
-    __shared__ int bidx[QB * 64];
+    constexpr int CSR = 16 / (int)sizeof(in_t);   // chunk stride, row-major query rows
+    constexpr int CST = 1024 / (int)sizeof(in_t); // chunk stride, tiled training rows
+    constexpr int CAP = TileShape<KC>::CAP;
+    constexpr int ROWT = TileShape<KC>::RT;
+    constexpr int QB = TileShape<KC>::WV * ROWT * 16;
+    __shared__ float bkey[QB * CAP];
+    __shared__ int bidx[QB * CAP];
     __shared__ int cnt[QB];
     __shared__ float thr[QB];
 
@@ -205,8 +257,8 @@ __global__ __launch_bounds__(256) void knn_tile_kernel(
         const int qr = q0 + qbase + rt * 16 + r;
 #pragma unroll
         for (int h = 0; h < NS; h++) {
-            Op::template load<NF>(Q + (int64_t)(qr < nq ? qr : 0) * rowlen + h * Kp + g * KCH,
-                                  a[h][rt]);
+            Op::template load<NF, CSR>(Q + (int64_t)(qr < nq ? qr : 0) * rowlen + h * Kp + g * KCH,
+                                       a[h][rt]);
             if (qr >= nq) {
 #pragma unroll
                 for (int f = 0; f < NF; f++)
@@ -223,16 +275,18 @@ __global__ __launch_bounds__(256) void knn_tile_kernel(
             ts[rt][i] = -__builtin_huge_val();
     wave_fence();
 
+    // Training tile t (rows 16t .. 16t+15) is stored as [half][chunk][lane][16 bytes]: the
+    // 16-byte chunk c of lane l = 16g + r is row 16t + r, features g*KCH + c*16/sizeof(in_t)
+    // onward, so each load instruction of the wave reads 1 KB of contiguous memory.
     auto load_step = [&](int tb, frag_t (&b)[COLT][NS][NF], ae (&hv)[COLT]) {
 #pragma unroll
         for (int ct = 0; ct < COLT; ct++) {
             const int j = tb + ct * 16 + r;
-            const bool ok = j < t_end;
-            hv[ct] = ok ? nyh[j] : (ae)(-__builtin_huge_val());
+            hv[ct] = j < t_end ? nyh[j] : (ae)(-__builtin_huge_val());
+            const in_t *tile = Y + (int64_t)(tb + ct * 16) * rowlen + lane * CSR;
 #pragma unroll
             for (int h = 0; h < NS; h++)
-                Op::template load<NF>(Y + (int64_t)(ok ? j : t_begin) * rowlen + h * Kp + g * KCH,
-                                      b[ct][h]);
+                Op::template load<NF, CST>(tile + h * 16 * Kp, b[ct][h]);
         }
     };
 
@@ -277,20 +331,20 @@ __global__ __launch_bounds__(256) void knn_tile_kernel(
                 for (int ct = 0; ct < COLT; ct++) {
                     if (acc[rt][ct][i] > ts[rt][i]) {
                         const int pos = atomicAdd(&cnt[q], 1);
-                        bkey[q * 64 + pos] = (float)(-2.0 * (double)acc[rt][ct][i]);
-                        bidx[q * 64 + pos] = tb + ct * 16 + r;
+                        bkey[q * CAP + pos] = (float)(-2.0 * (double)acc[rt][ct][i]);
+                        bidx[q * CAP + pos] = tb + ct * 16 + r;
                     }
                 }
             }
         }
         wave_fence();
-        const bool need = (lane < ROWT * 16) && (cnt[qbase + (lane & 31)] > 64 - STEP);
+        const bool need = (lane < ROWT * 16) && (cnt[qbase + (lane & 31)] > CAP - STEP);
         unsigned long long mask = __ballot(need);
         if (mask) {
             while (mask) {
                 const int bq = __ffsll((long long)mask) - 1;
                 mask &= mask - 1;
-                compact_query<KC>(bkey, bidx, cnt, thr, qbase + bq);
+                compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + bq);
             }
             wave_fence();
 #pragma unroll
@@ -328,7 +382,7 @@ __global__ __launch_bounds__(256) void knn_tile_kernel(
         }
     }
     for (int b = 0; b < ROWT * 16; b++)
-        compact_query<KC>(bkey, bidx, cnt, thr, qbase + b);
+        compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + b);
     wave_fence();
     for (int b = 0; b < ROWT * 16; b++) {
         const int q = q0 + qbase + b;
@@ -336,17 +390,21 @@ __global__ __launch_bounds__(256) void knn_tile_kernel(
             continue;
         const int ql = qbase + b;
         const int64_t o = ((int64_t)q * nsplit + split) * KC;
-        if (lane < KC) {
-            const bool valid = lane < cnt[ql];
-            part_key[o + lane] = valid ? bkey[ql * 64 + lane] : __builtin_huge_valf();
-            part_idx[o + lane] = valid ? bidx[ql * 64 + lane] : -1;
+        for (int t = lane; t < KC; t += 64) {
+            const bool valid = t < cnt[ql];
+            part_key[o + t] = valid ? bkey[ql * CAP + t] : __builtin_huge_valf();
+            part_idx[o + t] = valid ? bidx[ql * CAP + t] : -1;
         }
         if (lane == 0)
             part_tau[(int64_t)q * nsplit + split] = thr[ql];
     }
 }
 
-template <typename T>
+constexpr int MMAX = 2048; // candidates merged per query (nsplit * kc)
+constexpr int RMAX = 256; // candidates re-ranked exactly per query
+
+// MM: candidate capacity of this instance (a power of two >= nsplit * kc).
+template <typename T, int MM>
 __global__ __launch_bounds__(64) void knn_merge_kernel(
     const float *__restrict__ part_key, const int *__restrict__ part_idx,
     const float *__restrict__ part_tau, int nsplit, int kc, const T *__restrict__ Q,
@@ -354,14 +412,14 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
     double ymax, int post,
     T *__restrict__ out_dist, int *__restrict__ out_idx, int *__restrict__ fb_count,
     int *__restrict__ fb_list) {
-    __shared__ float skey[256];
-    __shared__ int sidx[256];
+    __shared__ float skey[MM];
+    __shared__ int sidx[MM];
     __shared__ double qrow[1024];
     const int q = blockIdx.x;
     const int lane = threadIdx.x;
-    const int m = nsplit * kc; // <= 256
+    const int m = nsplit * kc; // <= MM
     const int64_t base = (int64_t)q * m;
-    for (int t = lane; t < 256; t += 64) {
+    for (int t = lane; t < MM; t += 64) {
         skey[t] = t < m ? part_key[base + t] : __builtin_huge_valf();
         sidx[t] = t < m ? part_idx[base + t] : INT_MAX;
         if (skey[t] != skey[t]) {
@@ -382,10 +440,10 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
     for (int off = 32; off > 0; off >>= 1)
         qn += __shfl_xor(qn, off, 64);
     __syncthreads();
-    // Bitonic sort of the 256 (key, index) candidates, ascending.
-    for (int size = 2; size <= 256; size <<= 1) {
+    // Bitonic sort of the (key, index) candidates, ascending.
+    for (int size = 2; size <= MM; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = lane; t < 128; t += 64) {
+            for (int t = lane; t < MM / 2; t += 64) {
                 const int i = 2 * t - (t & (stride - 1));
                 const int j = i + stride;
                 const bool asc = (i & size) == 0;
@@ -419,9 +477,9 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
     const double lim = (double)ak + 2.0 * E;
     // Candidates within 2E of the k-th key, re-ranked with exact distances.
     int nr = 0;
-    while (nr < 256 && (double)skey[nr] <= lim)
+    while (nr < MM && (double)skey[nr] <= lim)
         nr++;
-    const bool certified = (double)tau > lim && nr <= 128;
+    const bool certified = (double)tau > lim && nr <= RMAX;
     if (!certified) {
         if (lane == 0) {
             const int p = atomicAdd(fb_count, 1);
@@ -429,10 +487,10 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
         }
         return;
     }
-    double ex[2];
-    int ei[2];
+    double ex[RMAX / 64];
+    int ei[RMAX / 64];
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < RMAX / 64; h++) {
         const int t = lane + 64 * h;
         ex[h] = __builtin_huge_val();
         ei[h] = INT_MAX;
@@ -449,28 +507,29 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
         }
     }
     __syncthreads();
-    // Sort the <= 128 exact candidates by (distance, index) through LDS (reusing qrow).
+    // Sort the <= RMAX exact candidates by (distance, index) through LDS (reusing qrow).
     double *xk = qrow;
     int *xi = sidx;
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < RMAX / 64; h++) {
         xk[lane + 64 * h] = ex[h];
         xi[lane + 64 * h] = ei[h];
     }
     __syncthreads();
-    for (int size = 2; size <= 128; size <<= 1) {
+    for (int size = 2; size <= RMAX; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const int t = lane;
-            const int i = 2 * t - (t & (stride - 1));
-            const int j = i + stride;
-            const bool asc = (i & size) == 0;
-            const double ki = xk[i], kj = xk[j];
-            const int ii = xi[i], ij = xi[j];
-            const bool gt = (kj < ki) || (kj == ki && ij < ii);
-            if (gt == asc) {
-                xk[i] = kj;
-                xk[j] = ki;
-                xi[i] = ij;
-                xi[j] = ii;
+            for (int t = lane; t < RMAX / 2; t += 64) {
+                const int i = 2 * t - (t & (stride - 1));
+                const int j = i + stride;
+                const bool asc = (i & size) == 0;
+                const double ki = xk[i], kj = xk[j];
+                const int ii = xi[i], ij = xi[j];
+                const bool gt = (kj < ki) || (kj == ki && ij < ii);
+                if (gt == asc) {
+                    xk[i] = kj;
+                    xk[j] = ki;
+                    xi[i] = ij;
+                    xi[j] = ii;
+                }
             }
             __syncthreads();
         }
@@ -487,50 +546,109 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
 
 } // namespace
 
-int knn_tile_kc(int k) { return k <= 16 ? 16 : 32; }
+namespace {
+
+// Rearranges row-major operand rows (row_bytes each, ns halves of four lane-group slices)
+// into the tiled training layout: tile t, half h, chunk c, lane l = 16g + r holds the 16
+// bytes at row 16t + r, byte h*row_bytes/ns + g*row_bytes/(4ns) + 16c. Rows at or beyond
+// `rows` are written as zeros. One thread per destination chunk.
+__global__ void tile_operand_kernel(const uint4 *__restrict__ src, int rows, int row_bytes,
+                                    int ns, uint4 *__restrict__ dst, int64_t total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total)
+        return;
+    const int ru = row_bytes / 16;       // chunks per row = chunks per lane per tile * 64 / 16
+    const int64_t t = i / (16 * ru);     // tile
+    const int within = (int)(i - t * 16 * ru);
+    const int l = within & 63, rest = within >> 6;
+    const int nc = ru / (4 * ns);        // chunks per lane-group slice
+    const int h = rest / nc, c = rest - h * nc;
+    const int g = l >> 4, r = l & 15;
+    const int64_t row = t * 16 + r;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (row < rows)
+        v = src[row * ru + h * (ru / ns) + g * nc + c];
+    dst[i] = v;
+}
+
+} // namespace
+
+hipError_t launch_knn_tile_operand(const void *src, int rows, int row_bytes, int ns, void *dst,
+                                   int rows_pad, hipStream_t s) {
+    if (rows_pad % 16 != 0 || row_bytes % (64 * ns) != 0)
+        return hipErrorInvalidValue;
+    const int64_t total = (int64_t)rows_pad * row_bytes / 16;
+    if (total == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(tile_operand_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                       static_cast<const uint4 *>(src), rows, row_bytes, ns,
+                       static_cast<uint4 *>(dst), total);
+    return hipGetLastError();
+}
+
+int knn_tile_kc(int k) { return k <= 16 ? 16 : k <= 32 ? 32 : k <= 64 ? 64 : 128; }
+
+int knn_tile_qb(int kc) {
+    switch (kc) {
+    case 16: return TileShape<16>::WV * TileShape<16>::RT * 16;
+    case 32: return TileShape<32>::WV * TileShape<32>::RT * 16;
+    case 64: return TileShape<64>::WV * TileShape<64>::RT * 16;
+    default: return TileShape<128>::WV * TileShape<128>::RT * 16;
+    }
+}
+
+int knn_merge_max_candidates() { return MMAX; }
 
 
+// Launches the fused tile for one operand type, candidate-list length and variant or, with
+// `slots` set, reports how many of its workgroups the device runs at once instead.
 template <class Op, int KC, bool SPLIT, bool PREFETCH>
 static hipError_t launch_tile_pf(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                  int Kp, int nsplit, float *part_key, int *part_idx,
-                                 float *part_tau, hipStream_t s) {
+                                 float *part_tau, int *slots, hipStream_t s) {
     using in_t = typename Op::in_t;
     using ae = typename Op::acc_e;
+    using kernel_t = void (*)(const in_t *, int, const in_t *, const ae *, int, int, int, int,
+                              float *, int *, float *);
+    constexpr int QB = TileShape<KC>::WV * TileShape<KC>::RT * 16;
+    kernel_t kern;
+    switch (Kp / 4) {
+    case 8: kern = knn_tile_kernel<Op, 8, KC, SPLIT, PREFETCH>; break;
+    case 16: kern = knn_tile_kernel<Op, 16, KC, SPLIT, PREFETCH>; break;
+    case 24: kern = knn_tile_kernel<Op, 24, KC, SPLIT, PREFETCH>; break;
+    case 32: kern = knn_tile_kernel<Op, 32, KC, SPLIT, PREFETCH>; break;
+    default: return hipErrorInvalidValue;
+    }
+    const dim3 block(64 * TileShape<KC>::WV);
+    if (slots) {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess)
+            e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, (int)block.x, 0);
+        *slots = e == hipSuccess ? std::max(1, cus * per_cu) : 0;
+        return e;
+    }
     int split_len = (n + nsplit - 1) / nsplit;
     split_len = (split_len + STEP - 1) / STEP * STEP;
     const int qblocks = (nq + QB - 1) / QB;
-    dim3 grid(qblocks * nsplit), block(256);
-    const in_t *q = static_cast<const in_t *>(Q);
-    const in_t *y = static_cast<const in_t *>(Y);
-    const ae *h = static_cast<const ae *>(nyh);
-#define DA_TILE(KCHV)                                                                           \
-    case KCHV:                                                                                  \
-        hipLaunchKernelGGL((knn_tile_kernel<Op, KCHV, KC, SPLIT, PREFETCH>), grid, block, 0, s, \
-                           q, nq, y,                                                           \
-                           h, n,                                                               \
-                           Kp, split_len, nsplit, part_key, part_idx, part_tau);               \
-        break;
-    switch (Kp / 4) {
-        DA_TILE(8)
-        DA_TILE(16)
-        DA_TILE(24)
-        DA_TILE(32)
-    default:
-        return hipErrorInvalidValue;
-    }
-#undef DA_TILE
+    const dim3 grid(qblocks * nsplit);
+    hipLaunchKernelGGL(kern, grid, block, 0, s, static_cast<const in_t *>(Q), nq,
+                       static_cast<const in_t *>(Y), static_cast<const ae *>(nyh), n, Kp,
+                       split_len, nsplit, part_key, part_idx, part_tau);
     return hipGetLastError();
 }
 
 template <class Op, int KC, bool SPLIT>
 static hipError_t launch_tile_op(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                  int Kp, int nsplit, float *part_key, int *part_idx,
-                                 float *part_tau, int variant, hipStream_t s) {
+                                 float *part_tau, int variant, int *slots, hipStream_t s) {
     if (variant & 1)
         return launch_tile_pf<Op, KC, SPLIT, true>(Q, nq, Y, nyh, n, Kp, nsplit, part_key,
-                                                   part_idx, part_tau, s);
+                                                   part_idx, part_tau, slots, s);
     return launch_tile_pf<Op, KC, SPLIT, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx,
-                                                part_tau, s);
+                                                part_tau, slots, s);
 }
 
 int knn_tile_kp(int d) {
@@ -539,27 +657,52 @@ int knn_tile_kp(int d) {
     return (d + 31) / 32 * 32;
 }
 
-hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
-                           int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
-                           float *part_tau, int variant, hipStream_t s) {
-    if (kc != 16)
-        return hipErrorInvalidValue;
+template <int KC>
+static hipError_t launch_tile_kc(int prec, const void *Q, int nq, const void *Y, const void *nyh,
+                                 int n, int Kp, int nsplit, float *part_key, int *part_idx,
+                                 float *part_tau, int variant, int *slots, hipStream_t s) {
+#define DA_OP(OP, SPLIT)                                                                          \
+    launch_tile_op<OP, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau,     \
+                                  variant, slots, s)
     switch (prec) {
-    case KP_F32:
-        return launch_tile_op<OpF32, 16, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
-    case KP_F64:
-        return launch_tile_op<OpF64, 16, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
-    case KP_F16:
-        return launch_tile_op<OpF16, 16, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
-    case KP_BF16:
-        return launch_tile_op<OpBF16, 16, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
-    case KP_F16X3:
-        return launch_tile_op<OpF16, 16, true>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
-    case KP_BF16X3:
-        return launch_tile_op<OpBF16, 16, true>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, s);
+    case KP_F32: return DA_OP(OpF32, false);
+    case KP_F64: return DA_OP(OpF64, false);
+    case KP_F16: return DA_OP(OpF16, false);
+    case KP_BF16: return DA_OP(OpBF16, false);
+    case KP_F16X3: return DA_OP(OpF16, true);
+    case KP_BF16X3: return DA_OP(OpBF16, true);
+    default: return hipErrorInvalidValue;
+    }
+#undef DA_OP
+}
+
+static hipError_t tile_dispatch(int prec, const void *Q, int nq, const void *Y, const void *nyh,
+                                int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
+                                float *part_tau, int variant, int *slots, hipStream_t s) {
+    switch (kc) {
+    case 16:
+        return launch_tile_kc<16>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+    case 32:
+        return launch_tile_kc<32>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+    case 64:
+        return launch_tile_kc<64>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+    case 128:
+        return launch_tile_kc<128>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
     default:
         return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
+                           int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
+                           float *part_tau, int variant, hipStream_t s) {
+    return tile_dispatch(prec, Q, nq, Y, nyh, n, Kp, nsplit, kc, part_key, part_idx, part_tau,
+                         variant, nullptr, s);
+}
+
+hipError_t knn_tile_slots(int prec, int Kp, int kc, int variant, int *slots) {
+    return tile_dispatch(prec, nullptr, 0, nullptr, nullptr, 0, Kp, 1, kc, nullptr, nullptr,
+                         nullptr, variant, slots, nullptr);
 }
 
 template <typename T>
@@ -567,12 +710,18 @@ hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const fl
                             int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,
                             double u_in, double abs_in, double u_acc, double ymax, int post,
                             T *out_dist, int *out_idx, int *fb_count, int *fb_list, hipStream_t s) {
-    if (nsplit * kc > 256 || d > 1024 || k > 128)
+    const int m = nsplit * kc;
+    if (m > MMAX || d > 1024 || k > 128 || k > kc)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(knn_merge_kernel<T>, dim3(nq), dim3(64), 0, s, part_key, part_idx,
-                       part_tau, nsplit, kc, Q, Y, d, k, u_in, abs_in, u_acc, ymax, post, out_dist,
-                       out_idx,
-                       fb_count, fb_list);
+    if (nq == 0)
+        return hipSuccess;
+    auto kern = m <= 256 ? knn_merge_kernel<T, 256>
+              : m <= 512 ? knn_merge_kernel<T, 512>
+              : m <= 1024 ? knn_merge_kernel<T, 1024>
+                          : knn_merge_kernel<T, 2048>;
+    hipLaunchKernelGGL(kern, dim3(nq), dim3(64), 0, s, part_key, part_idx, part_tau, nsplit, kc,
+                       Q, Y, d, k, u_in, abs_in, u_acc, ymax, post, out_dist, out_idx, fb_count,
+                       fb_list);
     return hipGetLastError();
 }
 

@@ -154,8 +154,20 @@ hipError_t launch_round_data(const T *src, int64_t n, int mode, T *dst, hipStrea
 
 namespace dagpu {
 // ---- fused kNN (fp32 MFMA) ----
-// Candidates kept per query and split for a request of k neighbours.
+// Rearranges packed operand rows (row_bytes each, ns halves) into the tiled training layout
+// the fused tile reads: per 16-row tile, [half][16-byte chunk][lane]. rows_pad (a multiple
+// of 32) rows are written; rows at or beyond `rows` are zero.
+hipError_t launch_knn_tile_operand(const void *src, int rows, int row_bytes, int ns, void *dst,
+                                   int rows_pad, hipStream_t s);
+// Workgroups of the fused tile (operand form prec, padded width Kp, list length kc, variant)
+// that the current device runs at once.
+hipError_t knn_tile_slots(int prec, int Kp, int kc, int variant, int *slots);
+// Candidates kept per query and split for a request of k neighbours (16, 32, 64 or 128).
 int knn_tile_kc(int k);
+// Queries per workgroup of the fused tile for a candidate-list length kc.
+int knn_tile_qb(int kc);
+// Largest nsplit * kc the merge accepts.
+int knn_merge_max_candidates();
 // Operand precision of the fused tile.
 enum KnnPrec : int { KP_F32 = 0, KP_F64 = 1, KP_F16 = 2, KP_BF16 = 3, KP_F16X3 = 4, KP_BF16X3 = 5 };
 // Bytes per padded operand row element group: split forms store [hi Kp][lo Kp] per row.
@@ -163,10 +175,12 @@ inline int knn_prec_row_factor(int prec) { return prec >= KP_F16X3 ? 2 : 1; }
 inline int knn_prec_elem_size(int prec) { return prec == KP_F64 ? 8 : prec == KP_F32 ? 4 : 2; }
 // Padded feature count used by the fused tile for d features (a multiple of 32, <= 128).
 int knn_tile_kp(int d);
-// Q: nq x Kp, Y: n x Kp row-major zero-padded rows in the operand type of prec (float,
-// double, _Float16 or __bf16); nyh: -|y|^2/2 per row (double for KP_F64, float otherwise).
+// Q: nq x Kp row-major zero-padded rows in the operand type of prec (float, double,
+// _Float16 or __bf16; split forms [hi Kp][lo Kp]); Y: the n training rows in the tiled
+// layout of launch_knn_tile_operand, padded to a multiple of 32 rows; nyh: -|y|^2/2 per row
+// (double for KP_F64, float otherwise).
 // Writes part_key/part_idx[(q*nsplit + s)*kc + i] (keys |y|^2 - 2 q.y as float) and
-// part_tau[q*nsplit + s]. kc must be 16. variant bit 0: prefetch the next training tile
+// part_tau[q*nsplit + s]. kc in {16, 32, 64, 128}. variant bit 0: prefetch the next training tile
 // into a second register buffer.
 hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                            int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
@@ -174,8 +188,9 @@ hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const
 // Q: nq x d and Y: n x d unpadded rows (type T); u_in: relative representation error of the
 // tile operands (0 when they are the data themselves); abs_in: absolute representation error
 // per element (fp16 subnormals); u_acc: unit roundoff of the tile accumulation;
-// ymax: max row norm of Y. Certified queries get out_dist/out_idx (row-major nq x k);
-// the others are appended to fb_list (fb_count must be zeroed before).
+// ymax: max row norm of Y. Requires nsplit * kc <= knn_merge_max_candidates() and k <= kc.
+// Certified queries get out_dist/out_idx (row-major nq x k); the others are appended to
+// fb_list (fb_count must be zeroed before).
 template <typename T>
 hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const float *part_tau,
                             int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,

@@ -70,16 +70,13 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
                  -1, LowerBound::greater_equal, 1, UpperBound::less_equal, -1);
 }
 
-// Tuning table of the fused tile variant per operand precision and padded width.
+// Tuning table of the fused tile variant per operand precision and padded width: the
+// single-buffered sweep runs two waves per SIMD where the prefetching one holds one, and
+// measured faster for every form at 128 features.
 static int knn_tile_auto_variant(int prec, int kp) {
+    (void)prec;
     (void)kp;
-    switch (prec) {
-    case KP_F16:
-    case KP_BF16:
-        return 1;
-    default:
-        return 1;
-    }
+    return 0;
 }
 
 template <typename T> int NNHandle<T>::tile_prec() const {
@@ -181,15 +178,15 @@ da_status NNHandle<T>::set_data(da_int n_samples, da_int n_features, const T *X,
     return prepare_fused(tile_prec());
 }
 
-// Squared-euclidean metrics are searched by the fused MFMA kernel (fp32 data, up to 128
-// features, k up to 16); everything else uses the key-tile + selection path.
+// Squared-euclidean metrics are searched by the fused MFMA kernel (up to 128 features, k up
+// to 128); everything else uses the key-tile + selection path.
 template <typename T> bool NNHandle<T>::fused_eligible(da_int k) const {
     if (!fused_ready_)
         return false;
     const bool sq = metric_ == da_euclidean || metric_ == da_sqeuclidean ||
                     metric_ == da_euclidean_gemm || metric_ == da_sqeuclidean_gemm ||
                     (metric_ == da_minkowski && p_ == 2.0);
-    return sq && k <= 16 && n_features_ <= 128 && n_samples_ >= 16;
+    return sq && k <= 128 && n_features_ <= 128 && n_samples_ >= 16;
 }
 
 // Builds the zero-padded copy of the training rows in the operand precision of the fused
@@ -201,9 +198,19 @@ template <typename T> da_status NNHandle<T>::prepare_fused(int prec) {
     const int n = n_samples_, d = n_features_;
     kp_ = knn_tile_kp(d);
     const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
-    DA_HIP(err, ypad_.reserve((size_t)n * kp_ * esz));
-    const T *src = train_.as<T>();
-    DA_HIP(err, launch_pack_rows_prec<T>(src, d, n, d, ypad_.ptr(), kp_, n, prec, st));
+    // Pack the rows in the operand type, then rearrange them into 16-row tiles whose
+    // per-lane operand slices are contiguous across the wave.
+    const int rows_pad = (n + 31) / 32 * 32;
+    {
+        DeviceBuffer packed;
+        DA_HIP(err, packed.reserve((size_t)n * kp_ * esz));
+        DA_HIP(err, launch_pack_rows_prec<T>(train_.as<T>(), d, n, d, packed.ptr(), kp_, n, prec,
+                                             st));
+        DA_HIP(err, ypad_.reserve((size_t)rows_pad * kp_ * esz));
+        DA_HIP(err, launch_knn_tile_operand(packed.ptr(), n, (int)(kp_ * esz),
+                                            knn_prec_row_factor(prec), ypad_.ptr(), rows_pad, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+    }
     std::vector<double> sq(n);
     DA_HIP(err, hipMemcpyAsync(sq.data(), train_sq_.ptr(), n * sizeof(double),
                                hipMemcpyDeviceToHost, st));
@@ -232,48 +239,105 @@ template <typename T> da_status NNHandle<T>::prepare_fused(int prec) {
     return da_status_success;
 }
 
+template <typename T> SearchParams NNHandle<T>::params() const {
+    SearchParams p;
+    p.prec = tile_prec();
+    p.variant = opts.ival("tile variant");
+    return p;
+}
+
+template <typename T> da_status NNHandle<T>::prepare_search(const SearchParams &p) {
+    if (fused_ready_ && p.prec != ypad_prec_)
+        return prepare_fused(p.prec);
+    return da_status_success;
+}
+
+SearchContext::~SearchContext() {
+    if (h_fb)
+        (void)hipHostFree(h_fb);
+}
+
+// Fused path, launch phase: pack the queries, distance tile, merge/certify. The fallback
+// count is copied to pinned host memory; search_finish reads it after the stream completes.
+// Number of training splits for a batch of nq queries. The grid has qblocks * nsplit
+// workgroups and the device runs `slots` of them at once, so a sweep takes
+// ceil(qblocks * nsplit / slots) rounds of one split's steps each; every split also pays
+// a warm-up while its thresholds are still open. The count minimising that estimate is
+// taken, within the merge's capacity (nsplit * kc candidates per query) and keeping each
+// split at 1024 rows or more.
 template <typename T>
-da_status NNHandle<T>::search_fused(const T *dQ, da_int nq, da_int k, bool return_distance,
-                                    da_int *d_ind, T *d_dist) {
-    hipStream_t st = gpu->stream();
+int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq) {
+    const int key = (prec * 8 + variant) * 256 + kc;
+    int slots = 0;
+    auto it = ctx.slots.find(key);
+    if (it != ctx.slots.end()) {
+        slots = it->second;
+    } else {
+        if (knn_tile_slots(prec, kp_, kc, variant, &slots) != hipSuccess || slots <= 0)
+            slots = 512;
+        ctx.slots[key] = slots;
+    }
+    const int64_t n = n_samples_;
+    const int64_t qb = knn_tile_qb(kc);
+    const int64_t qblocks = (nq + qb - 1) / qb;
+    const int64_t hi = std::max<int64_t>(
+        1, std::min<int64_t>(knn_merge_max_candidates() / kc, (n + 1023) / 1024));
+    const int64_t warm = 8;
+    int64_t best = 1;
+    double best_cost = 0;
+    for (int64_t s = 1; s <= hi; s++) {
+        const int64_t steps = ((n + s - 1) / s + 31) / 32;
+        const int64_t rounds = (qblocks * s + slots - 1) / slots;
+        const double cost = (double)rounds * (double)(steps + warm);
+        if (s == 1 || cost < best_cost * 0.995) {
+            best = s;
+            best_cost = cost;
+        }
+    }
+    return (int)best;
+}
+
+template <typename T>
+da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp, const T *dQ,
+                                     da_int nq, da_int k, bool return_distance, da_int *d_ind,
+                                     T *d_dist, bool &fused) {
+    fused = fused_eligible(k) && sp.prec == ypad_prec_;
+    ctx.last_fallback = 0;
+    ctx.last_candidates = 0;
+    if (!fused)
+        return search_tiles(ctx, dQ, nq, k, return_distance, d_ind, d_dist);
+    hipStream_t st = ctx.gpu->stream();
     const int n = n_samples_, d = n_features_;
-    const int prec = tile_prec();
-    if (prec != ypad_prec_) {
-        da_status s = prepare_fused(prec);
-        if (s != da_status_success)
-            return s;
-    }
+    const int prec = sp.prec;
     const int kc = knn_tile_kc(k);
-    // Split the training rows so that the launch has about four workgroups per CU.
-    const int qblocks = (nq + 127) / 128;
-    int nsplit = (1024 + qblocks - 1) / qblocks;
-    nsplit = std::max(1, std::min({nsplit, 256 / kc, (n + 1023) / 1024}));
+    if (!ctx.h_fb)
+        DA_HIP(err, hipHostMalloc((void **)&ctx.h_fb, sizeof(int)));
+    int variant = sp.variant;
+    if (variant < 0)
+        variant = knn_tile_auto_variant(prec, kp_);
+    const int nsplit = tile_splits(ctx, prec, kc, variant, nq);
     const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
-    DA_HIP(err, qpad_.reserve((size_t)nq * kp_ * esz));
-    DA_HIP(err, launch_pack_rows_prec<T>(dQ, d, nq, d, qpad_.ptr(), kp_, nq, prec, st));
+    DA_HIP(err, ctx.qpad.reserve((size_t)nq * kp_ * esz));
+    DA_HIP(err, launch_pack_rows_prec<T>(dQ, d, nq, d, ctx.qpad.ptr(), kp_, nq, prec, st));
     const size_t np = (size_t)nq * nsplit;
-    DA_HIP(err, part_key_.reserve(np * kc * sizeof(float)));
-    DA_HIP(err, part_idx_.reserve(np * kc * sizeof(int)));
-    DA_HIP(err, part_tau_.reserve(np * sizeof(float)));
-    DA_HIP(err, fb_.reserve((size_t)(nq + 1) * sizeof(int)));
+    DA_HIP(err, ctx.part_key.reserve(np * kc * sizeof(float)));
+    DA_HIP(err, ctx.part_idx.reserve(np * kc * sizeof(int)));
+    DA_HIP(err, ctx.part_tau.reserve(np * sizeof(float)));
+    DA_HIP(err, ctx.fb.reserve((size_t)(nq + 1) * sizeof(int)));
     {
-        KernelScope ks(*gpu, "knn_tile");
-        int variant = opts.ival("tile variant");
-        if (variant < 0)
-            variant = knn_tile_auto_variant(prec, kp_);
-        DA_HIP(err, launch_knn_tile(prec, qpad_.ptr(), nq, ypad_.ptr(), nyh_.ptr(), n, kp_, nsplit,
-                                    kc, part_key_.as<float>(), part_idx_.as<int>(),
-                                    part_tau_.as<float>(), variant, st));
+        KernelScope ks(*ctx.gpu, "knn_tile");
+        DA_HIP(err, launch_knn_tile(prec, ctx.qpad.ptr(), nq, ypad_.ptr(), nyh_.ptr(), n, kp_,
+                                    nsplit, kc, ctx.part_key.as<float>(), ctx.part_idx.as<int>(),
+                                    ctx.part_tau.as<float>(), variant, st));
     }
-    int *fbc = fb_.as<int>();
+    int *fbc = ctx.fb.as<int>();
     DA_HIP(err, hipMemsetAsync(fbc, 0, sizeof(int), st));
     const int post = (metric_ == da_sqeuclidean || metric_ == da_sqeuclidean_gemm) ? 0 : 1;
-    // Unit roundoffs: operands (0 when the tile runs in the data precision), accumulation.
-    const bool same = (prec == KP_F64 && std::is_same_v<T, double>) ||
-                      (prec == KP_F32 && std::is_same_v<T, float>);
     // Representation error of an operand element: relative u_in, absolute abs_in (fp16
     // subnormals). Split forms keep hi + lo, so only the remainder of lo and the dropped
     // lo*lo product (folded into u_in) remain.
+    const bool same = (prec == KP_F64 && std::is_same_v<T, double>) ||
+                      (prec == KP_F32 && std::is_same_v<T, float>);
     double u_in = 0.0, abs_in = 0.0;
     const double u16 = std::ldexp(1.0, -11), ub16 = std::ldexp(1.0, -8), u32 = std::ldexp(1.0, -24);
     if (!same) {
@@ -301,32 +365,45 @@ da_status NNHandle<T>::search_fused(const T *dQ, da_int nq, da_int k, bool retur
     }
     const double u_acc = prec == KP_F64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
     {
-        KernelScope ks(*gpu, "knn_merge");
-        DA_HIP(err, launch_knn_merge<T>(part_key_.as<float>(), part_idx_.as<int>(),
-                                        part_tau_.as<float>(), nsplit, kc, dQ, train_.as<T>(), nq,
-                                        d, k, u_in, abs_in, u_acc, ymax_, post,
+        KernelScope ks(*ctx.gpu, "knn_merge");
+        DA_HIP(err, launch_knn_merge<T>(ctx.part_key.as<float>(), ctx.part_idx.as<int>(),
+                                        ctx.part_tau.as<float>(), nsplit, kc, dQ, train_.as<T>(),
+                                        nq, d, k, u_in, abs_in, u_acc, ymax_, post,
                                         return_distance ? d_dist : nullptr, d_ind, fbc, fbc + 1,
                                         st));
     }
-    int nfb = 0;
-    DA_HIP(err, hipMemcpyAsync(&nfb, fbc, sizeof(int), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipMemcpyAsync(ctx.h_fb, fbc, sizeof(int), hipMemcpyDeviceToHost, st));
+    ctx.last_candidates = nsplit * kc;
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::search_finish(SearchContext &ctx, const T *dQ, da_int nq, da_int k,
+                                     bool return_distance, da_int *d_ind, T *d_dist,
+                                     bool fused) {
+    (void)nq;
+    if (!fused)
+        return da_status_success;
+    hipStream_t st = ctx.gpu->stream();
     DA_HIP(err, hipStreamSynchronize(st));
-    last_candidates = nsplit * kc;
-    last_fallback = nfb;
+    const int nfb = *ctx.h_fb;
+    ctx.last_fallback = nfb;
     if (nfb > 0) {
         // Exact fallback for the uncertified queries (listed by the merge kernel): gather
         // their rows, key tiles + selection, scatter the results back.
-        const int *list = fbc + 1;
-        DA_HIP(err, fb_q_.reserve((size_t)nfb * d * sizeof(T)));
-        DA_HIP(err, launch_gather_scatter(dQ, fb_q_.ptr(), list, nfb, d, sizeof(T), 0, st));
-        DA_HIP(err, fb_ind_.reserve((size_t)nfb * k * sizeof(da_int)));
-        DA_HIP(err, fb_dist_.reserve((size_t)nfb * k * sizeof(T)));
-        da_status s = search_tiles(fb_q_.as<T>(), nfb, k, true, fb_ind_.as<da_int>(), fb_dist_.as<T>());
+        const int d = n_features_;
+        const int *list = ctx.fb.as<int>() + 1;
+        DA_HIP(err, ctx.fb_q.reserve((size_t)nfb * d * sizeof(T)));
+        DA_HIP(err, launch_gather_scatter(dQ, ctx.fb_q.ptr(), list, nfb, d, sizeof(T), 0, st));
+        DA_HIP(err, ctx.fb_ind.reserve((size_t)nfb * k * sizeof(da_int)));
+        DA_HIP(err, ctx.fb_dist.reserve((size_t)nfb * k * sizeof(T)));
+        da_status s = search_tiles(ctx, ctx.fb_q.as<T>(), nfb, k, true, ctx.fb_ind.as<da_int>(),
+                                   ctx.fb_dist.as<T>());
         if (s != da_status_success)
             return s;
-        DA_HIP(err, launch_gather_scatter(fb_ind_.ptr(), d_ind, list, nfb, k, sizeof(da_int), 1, st));
+        DA_HIP(err, launch_gather_scatter(ctx.fb_ind.ptr(), d_ind, list, nfb, k, sizeof(da_int), 1, st));
         if (return_distance)
-            DA_HIP(err, launch_gather_scatter(fb_dist_.ptr(), d_dist, list, nfb, k, sizeof(T), 1, st));
+            DA_HIP(err, launch_gather_scatter(ctx.fb_dist.ptr(), d_dist, list, nfb, k, sizeof(T), 1, st));
     }
     return da_status_success;
 }
@@ -459,20 +536,28 @@ da_status NNHandle<T>::kneighbors_device(da_int n_queries, da_int n_features, co
 template <typename T>
 da_status NNHandle<T>::search(const T *dQ, da_int nq, da_int k, bool return_distance,
                               da_int *d_ind, T *d_dist) {
-    last_fallback = 0;
-    last_candidates = 0;
-    if (fused_eligible(k))
-        return search_fused(dQ, nq, k, return_distance, d_ind, d_dist);
-    return search_tiles(dQ, nq, k, return_distance, d_ind, d_dist);
+    main_ctx.gpu = gpu;
+    const SearchParams sp = params();
+    da_status s = prepare_search(sp);
+    if (s != da_status_success)
+        return s;
+    bool fused = false;
+    s = search_launch(main_ctx, sp, dQ, nq, k, return_distance, d_ind, d_dist, fused);
+    if (s != da_status_success)
+        return s;
+    s = search_finish(main_ctx, dQ, nq, k, return_distance, d_ind, d_dist, fused);
+    last_fallback = main_ctx.last_fallback;
+    last_candidates = main_ctx.last_candidates;
+    return s;
 }
 
 // Generic path for every metric: key tiles (queries x training points, in double) followed
 // by the per-row selection, the structure of kneighbors_compute_brute_force
 // (nearest_neighbors.cpp:567-816) with the blocks sized for HBM instead of caches.
 template <typename T>
-da_status NNHandle<T>::search_tiles(const T *dQ, da_int nq, da_int k, bool return_distance,
-                                    da_int *d_ind, T *d_dist) {
-    hipStream_t st = gpu->stream();
+da_status NNHandle<T>::search_tiles(SearchContext &ctx, const T *dQ, da_int nq, da_int k,
+                                    bool return_distance, da_int *d_ind, T *d_dist) {
+    hipStream_t st = ctx.gpu->stream();
     const da_int d = n_features_, n = n_samples_;
     if (k > 3840)
         return err->error(da_status_not_implemented,
@@ -524,37 +609,37 @@ da_status NNHandle<T>::search_tiles(const T *dQ, da_int nq, da_int k, bool retur
         return err->error(da_status_not_implemented, "Unknown metric.");
     }
     if (need_qsq || need_qcos) {
-        DA_HIP(err, q_norm_.reserve((size_t)nq * sizeof(double)));
-        DA_HIP(err, launch_row_sqnorms<T>(dQ, d, nq, d, q_norm_.as<double>(), st));
+        DA_HIP(err, ctx.q_norm.reserve((size_t)nq * sizeof(double)));
+        DA_HIP(err, launch_row_sqnorms<T>(dQ, d, nq, d, ctx.q_norm.as<double>(), st));
         if (need_qcos) {
-            DA_HIP(err, launch_cosine_norms(q_norm_.as<double>(), nq,
+            DA_HIP(err, launch_cosine_norms(ctx.q_norm.as<double>(), nq,
                                             std::numeric_limits<T>::epsilon(), st));
             nt_norm = train_cos_.as<double>();
         } else {
             nt_norm = train_sq_.as<double>();
         }
-        nq_norm = q_norm_.as<double>();
+        nq_norm = ctx.q_norm.as<double>();
     }
     const size_t max_keys = (size_t)64 << 20; // 512 MB of double keys per chunk
     da_int chunk = (da_int)std::max<size_t>(1, std::min<size_t>(nq, max_keys / (size_t)n));
-    DA_HIP(err, keys_.reserve((size_t)chunk * n * sizeof(double)));
-    DA_HIP(err, sel_key_.reserve((size_t)chunk * k * sizeof(double)));
-    DA_HIP(err, sel_idx_.reserve((size_t)chunk * k * sizeof(int)));
+    DA_HIP(err, ctx.keys.reserve((size_t)chunk * n * sizeof(double)));
+    DA_HIP(err, ctx.sel_key.reserve((size_t)chunk * k * sizeof(double)));
+    DA_HIP(err, ctx.sel_idx.reserve((size_t)chunk * k * sizeof(int)));
     for (da_int q0 = 0; q0 < nq; q0 += chunk) {
         const da_int rows = std::min(chunk, nq - q0);
         {
-            KernelScope ks(*gpu, "pairwise_direct");
+            KernelScope ks(*ctx.gpu, "pairwise_direct");
             DA_HIP(err, (launch_pairwise_direct<T, double>(
                             km, dQ + (size_t)q0 * d, train_.as<T>(), rows, n, d,
-                            keys_.as<double>(), n, nq_norm ? nq_norm + q0 : nullptr, nt_norm, p,
+                            ctx.keys.as<double>(), n, nq_norm ? nq_norm + q0 : nullptr, nt_norm, p,
                             0, 0, st)));
         }
         {
-            KernelScope ks(*gpu, "row_topk");
-            DA_HIP(err, launch_row_topk<double>(keys_.as<double>(), n, rows, n, k, largest,
-                                                sel_key_.as<double>(), sel_idx_.as<int>(), st));
+            KernelScope ks(*ctx.gpu, "row_topk");
+            DA_HIP(err, launch_row_topk<double>(ctx.keys.as<double>(), n, rows, n, k, largest,
+                                                ctx.sel_key.as<double>(), ctx.sel_idx.as<int>(), st));
         }
-        DA_HIP(err, launch_knn_finalize<T>(sel_key_.as<double>(), sel_idx_.as<int>(), rows, k,
+        DA_HIP(err, launch_knn_finalize<T>(ctx.sel_key.as<double>(), ctx.sel_idx.as<int>(), rows, k,
                                            post, p,
                                            return_distance ? d_dist + (size_t)q0 * k : nullptr,
                                            d_ind + (size_t)q0 * k, k, st));

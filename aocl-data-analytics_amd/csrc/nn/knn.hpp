@@ -3,8 +3,29 @@
 // training set resident in GPU memory.
 #pragma once
 #include "../core/handle.hpp"
+#include <map>
 
 namespace dagpu {
+
+// Scratch state of one in-flight search: its stream (through a GpuContext) and the device
+// buffers of the fused and tile paths. A handle owns one; a serving loop owns several so
+// that batches overlap on the device.
+struct SearchContext {
+    GpuContext *gpu = nullptr;
+    std::map<int, int> slots; // concurrent workgroups per fused-tile configuration
+    DeviceBuffer q_norm, keys, sel_key, sel_idx, qpad, part_key, part_idx, part_tau, fb, fb_q,
+        fb_ind, fb_dist;
+    // Pinned host word receiving the fallback count of the last fused launch.
+    int *h_fb = nullptr;
+    da_int last_fallback = 0, last_candidates = 0;
+    ~SearchContext();
+};
+
+// Parameters of a search fixed by the caller (read from the options once).
+struct SearchParams {
+    int prec = 0;     // operand precision of the fused tile (KP_*)
+    int variant = -1; // fused tile kernel variant (-1: tuning table)
+};
 
 template <typename T> class NNHandle : public AlgHandle {
   public:
@@ -19,22 +40,35 @@ template <typename T> class NNHandle : public AlgHandle {
     da_status get_result(da_result query, da_int *dim, da_int *result) override;
     void refresh() override {}
 
+    // Search entry points shared with the serving loop. dQ: row-major n_features columns;
+    // outputs row-major nq x k. search_launch enqueues everything that needs no host
+    // decision; search_finish waits, runs the exact fallback if the certification listed
+    // queries, and leaves the results in d_ind/d_dist (enqueued on ctx's stream).
+    SearchParams params() const;
+    da_status prepare_search(const SearchParams &p);
+    da_status search_launch(SearchContext &ctx, const SearchParams &p, const T *dQ, da_int nq,
+                            da_int k, bool return_distance, da_int *d_ind, T *d_dist,
+                            bool &fused);
+    da_status search_finish(SearchContext &ctx, const T *dQ, da_int nq, da_int k,
+                            bool return_distance, da_int *d_ind, T *d_dist, bool fused);
+    bool trained() const { return trained_; }
+    da_int n_samples() const { return n_samples_; }
+    da_int n_features() const { return n_features_; }
+    ErrorStack *errors() { return err; }
+
+    SearchContext main_ctx;
     da_int last_fallback = 0, last_candidates = 0;
 
   private:
     da_status set_params();
     da_status check_options_update();
     da_status resolve_k(da_int &k);
-    // Runs the search for device queries (row-major, ld = n_features) into device outputs
-    // (row-major, leading dimension k).
     da_status search(const T *dQ, da_int nq, da_int k, bool return_distance, da_int *d_ind,
                      T *d_dist);
-    da_status search_tiles(const T *dQ, da_int nq, da_int k, bool return_distance,
-                           da_int *d_ind, T *d_dist);
-    // Fused MFMA path for the squared-euclidean family (fp32 data).
+    da_status search_tiles(SearchContext &ctx, const T *dQ, da_int nq, da_int k,
+                           bool return_distance, da_int *d_ind, T *d_dist);
     bool fused_eligible(da_int k) const;
-    da_status search_fused(const T *dQ, da_int nq, da_int k, bool return_distance,
-                           da_int *d_ind, T *d_dist);
+    int tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq);
     da_status prepare_fused(int prec);
 
     bool trained_ = false;
@@ -46,11 +80,10 @@ template <typename T> class NNHandle : public AlgHandle {
     DeviceBuffer train_;     // row-major n_samples x n_features, type T
     DeviceBuffer train_sq_;  // squared row norms (double)
     DeviceBuffer train_cos_; // cosine norms (double)
-    DeviceBuffer q_, q_stage_, q_norm_, keys_, sel_key_, sel_idx_, out_ind_, out_dist_;
-    // Fused-path state: zero-padded training rows (n x Kp, fp32), fp32 squared norms,
-    // the largest row norm, and per-call candidate buffers.
-    DeviceBuffer ypad_, nyh_, qpad_, part_key_, part_idx_, part_tau_, fb_, fb_q_, fb_ind_,
-        fb_dist_;
+    DeviceBuffer q_, q_stage_, out_ind_, out_dist_;
+    // Fused-path state: zero-padded training rows in the tile's operand precision, -|y|^2/2
+    // per row, the largest row norm.
+    DeviceBuffer ypad_, nyh_;
     int kp_ = 0;
     int ypad_prec_ = -1; // operand precision the padded copy was built for
     double ymax_ = 0;

@@ -96,6 +96,121 @@ def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1):
             "candidates_per_query": ncand.value}
 
 
+SERVE_METRIC = ("kNN serving: sustained queries/s with p50/p99 request latency, resident "
+                "2,000,000x96 index, 1 GPU")
+
+
+def serving(args, L, aoclda):
+    """Continuous-batching kNN service over a resident index of unit-normalised rows from a
+    1,024-cluster Gaussian mixture (means N(0,1)^96, sigma 0.25, then each row scaled to unit
+    length). Requests carry a log-uniform number of queries in [1, 4096] and k drawn from
+    {1, 10, 100}. A closed-loop pass (all requests queued at once) measures the capacity;
+    open-loop passes with Poisson arrivals at fractions of it give the latency figures."""
+    import threading
+    rng = np.random.default_rng(7)
+    n, d = args.serve_rows, 96
+    means = rng.standard_normal((1024, d))
+    X = np.empty((n, d), dtype=np.float32)
+    for a in range(0, n, 250_000):
+        b = min(n, a + 250_000)
+        blk = means[rng.integers(0, 1024, b - a)] + 0.25 * rng.standard_normal((b - a, d))
+        X[a:b] = blk / np.linalg.norm(blk, axis=1, keepdims=True)
+    pool = means[rng.integers(0, 1024, 8192)] + 0.25 * rng.standard_normal((8192, d))
+    pool = (pool / np.linalg.norm(pool, axis=1, keepdims=True)).astype(np.float32)
+    log(f"serving: index {n}x{d} ready")
+
+    def requests(count, seed):
+        r = np.random.default_rng(seed)
+        nqs = np.minimum(4096, np.floor(np.exp(r.uniform(0, np.log(4097), count)))).astype(int)
+        ks = r.choice([1, 10, 100], count)
+        offs = [int(r.integers(0, 8192 - q + 1)) for q in nqs]
+        return nqs, ks, offs
+
+    out = {"metric": SERVE_METRIC, "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic: unit-normalised rows of a 1,024-cluster Gaussian mixture "
+                   "(means N(0,1)^96, sigma 0.25), numpy default_rng(7)",
+           "config": {"workload": f"kNN serving, resident index {n}x{d} fp32, euclidean, "
+                                  "queries per request log-uniform [1,4096], k in {1,10,100}, "
+                                  "Poisson arrivals",
+                      "max_batch_queries": args.serve_batch,
+                      "max_device_bytes": args.serve_bytes}, "forms": {}}
+    best = None
+    for form in args.serve_forms.split(","):
+        nn = aoclda.neighbors.nearest_neighbors(precision=form).fit(X)
+        srv = aoclda.neighbors.nn_server(nn, max_batch_queries=args.serve_batch,
+                                         max_device_bytes=args.serve_bytes)
+        # Warm-up and result check against direct calls on the same handle.
+        nqs, ks, offs = requests(6, 11)
+        ids = [srv.submit(pool[o:o + q], int(k)) for q, k, o in zip(nqs, ks, offs)]
+        for rid, q, k, o in zip(ids, nqs, ks, offs):
+            dist, ind, _ = srv.wait(rid)
+            d2, i2 = nn.kneighbors(pool[o:o + q], n_neighbors=int(k))
+            if not (np.array_equal(ind, i2) and np.array_equal(dist, d2)):
+                raise RuntimeError("served result differs from the direct search")
+        # Closed loop: capacity.
+        nqs, ks, offs = requests(args.serve_closed, 12)
+        t0 = time.perf_counter()
+        ids = [srv.submit(pool[o:o + q], int(k)) for q, k, o in zip(nqs, ks, offs)]
+        for rid in ids:
+            srv.wait(rid)
+        cap = float(nqs.sum()) / (time.perf_counter() - t0)
+        mean_nq = float(nqs.mean())
+        log(f"serving {form}: capacity {cap:.0f} queries/s (closed loop, "
+            f"{args.serve_closed} requests)")
+        points = []
+        for load in [float(x) for x in args.serve_loads.split(",")]:
+            rate = load * cap / mean_nq  # requests per second
+            count = max(20, int(rate * args.serve_seconds))
+            nqs, ks, offs = requests(count, 13)
+            gaps = np.random.default_rng(14).exponential(1.0 / rate, count)
+            lat = [0.0] * count
+            ids = [None] * count
+            ready = threading.Semaphore(0)
+
+            def collect():
+                for i in range(count):
+                    ready.acquire()
+                    lat[i] = srv.wait(ids[i])[2]
+
+            th = threading.Thread(target=collect)
+            th.start()
+            t0 = time.perf_counter()
+            t_next = t0
+            for i in range(count):
+                t_next += gaps[i]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(pool[offs[i]:offs[i] + nqs[i]], int(ks[i]))
+                ready.release()
+            th.join()
+            el = time.perf_counter() - t0
+            lat_a = np.array(lat)
+            pt = {"load": load, "offered_requests_per_s": rate, "requests": count,
+                  "queries": int(nqs.sum()), "qps": float(nqs.sum()) / el,
+                  "p50_ms": float(np.percentile(lat_a, 50)),
+                  "p99_ms": float(np.percentile(lat_a, 99))}
+            points.append(pt)
+            log(f"serving {form}: load {load:.2f}: {pt['qps']:.0f} q/s, p50 {pt['p50_ms']:.2f} "
+                f"ms, p99 {pt['p99_ms']:.2f} ms")
+        st = srv.stats()
+        srv.close()
+        del nn
+        out["forms"][form] = {"capacity_qps": cap, "points": points,
+                              "fallback_queries": st["fallback_queries"],
+                              "mean_queries_per_batch": st["queries_per_batch"]}
+        for pt in points:
+            if pt["p99_ms"] < 10.0 and (best is None or pt["qps"] > best[1]["qps"]):
+                best = (form, pt)
+    if best is None:
+        form = next(iter(out["forms"]))
+        best = (form, min(out["forms"][form]["points"], key=lambda p: p["p99_ms"]))
+    out.update({"value": best[1]["qps"], "p50_ms": best[1]["p50_ms"],
+                "p99_ms": best[1]["p99_ms"], "precision_form": best[0],
+                "p99_under_10ms": best[1]["p99_ms"] < 10.0})
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,11 +220,23 @@ def main():
     ap.add_argument("--n-query", type=int, default=20_000)
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--forms", default="working,fp16x3,bf16x3,fp16,bf16")
+    ap.add_argument("--forms", default="working,fp16x3,bf16x3")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="fused tile kernel variant (-1: tuning table)")
-    ap.add_argument("--cpu-sample", type=int, default=48,
+    ap.add_argument("--cpu-sample", type=int, default=1000,
                     help="queries checked against (and timed with) the host oracle")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (two launches), the "
+                         "reference point that scales memory counters to bytes")
+    ap.add_argument("--serving", action="store_true",
+                    help="run the continuous-batching serving benchmark instead")
+    ap.add_argument("--serve-rows", type=int, default=2_000_000)
+    ap.add_argument("--serve-forms", default="fp16x3")
+    ap.add_argument("--serve-batch", type=int, default=2048)
+    ap.add_argument("--serve-bytes", type=int, default=1 << 30)
+    ap.add_argument("--serve-closed", type=int, default=200)
+    ap.add_argument("--serve-loads", default="0.3,0.5,0.7")
+    ap.add_argument("--serve-seconds", type=float, default=6.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -118,6 +245,21 @@ def main():
 
     from aoclda import _lib as L
     import aoclda
+
+    if args.serving:
+        serving(args, L, aoclda)
+        return
+
+    if args.copy_calibration_mib > 0:
+        import torch
+        n = args.copy_calibration_mib << 18  # float32 elements
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 2 copies of {args.copy_calibration_mib} MiB")
+        del a, b
 
     rng = np.random.default_rng(0)
     means = rng.uniform(-10, 10, (64, args.dim))

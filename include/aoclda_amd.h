@@ -61,6 +61,36 @@ da_status da_amd_profile_names(da_handle handle, char *names, da_int *lnames);
  */
 da_status da_nn_last_query_stats(da_handle handle, da_int *n_fallback, da_int *n_candidates);
 
+/*
+ * k-NN serving loop over the resident training set of a kNN handle (da_nn_set_data_? done;
+ * the handle must outlive the server and not be modified while it runs). Requests are
+ * queued by da_nn_server_submit_? (the queries are copied, the call does not block) and
+ * served by a worker thread that coalesces queued requests into batches of at most
+ * max_batch_queries queries whose device buffers fit in max_device_bytes (admission by
+ * memory; a request larger than a batch is served in slices). Two batches are in flight on
+ * two HIP streams, so a batch is prepared and admitted while the previous one runs.
+ * Every query's result is the exact k nearest neighbours, identical whether the request is
+ * served alone or together with others. da_nn_server_wait_? blocks until a request is
+ * complete and returns its n_queries x k results (row-major) and its latency (submission to
+ * completion, milliseconds).
+ */
+typedef struct _da_nn_server *da_nn_server;
+da_status da_nn_server_create(da_handle handle, da_int max_batch_queries,
+                              size_t max_device_bytes, da_nn_server *server);
+da_status da_nn_server_submit_s(da_nn_server server, da_int n_queries, da_int n_features,
+                                const float *X, da_int ldx, da_int k, da_int *request_id);
+da_status da_nn_server_submit_d(da_nn_server server, da_int n_queries, da_int n_features,
+                                const double *X, da_int ldx, da_int k, da_int *request_id);
+da_status da_nn_server_wait_s(da_nn_server server, da_int request_id, da_int *n_ind,
+                              float *n_dist, double *latency_ms);
+da_status da_nn_server_wait_d(da_nn_server server, da_int request_id, da_int *n_ind,
+                              double *n_dist, double *latency_ms);
+/* stats[0..7] = completed requests, completed queries, batches, latency p50 (ms), latency
+ * p99 (ms), mean queries per batch, seconds since creation, fallback queries. *n_stats is
+ * the array size on entry (>= 8) and the number written on exit. */
+da_status da_nn_server_stats(da_nn_server server, double *stats, da_int *n_stats);
+void da_nn_server_destroy(da_nn_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -192,3 +192,21 @@ def test_sqeuclidean_fused(oracle):
     ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), 8, "sqeuclidean")
     np.testing.assert_array_equal(ind, ei)
     np.testing.assert_allclose(dist, ed, rtol=1e-5)
+
+
+@pytest.mark.parametrize("k", [17, 32, 33, 64, 100, 128])
+@pytest.mark.parametrize("precision", ["working", "fp16x3", "bf16"])
+def test_fused_large_k(oracle, k, precision):
+    """Candidate lists of 32, 64 and 128 entries per split (k up to 128) stay exact."""
+    rng = np.random.default_rng(100 + k)
+    means = rng.uniform(-10, 10, (8, 48))
+    Xt = gaussian_mixture(rng, 5000, means).astype(np.float32)
+    Xq = gaussian_mixture(rng, 150, means).astype(np.float32)
+    nn = aoclda.neighbors.nearest_neighbors(precision=precision).fit(Xt)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=k)
+    ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), k)
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-5)
+    nfb, ncand = L.da_int(), L.da_int()
+    L.check(L.lib.da_nn_last_query_stats(nn.handle.h, C.byref(nfb), C.byref(ncand)))
+    assert ncand.value >= k  # the fused tile path ran
