@@ -11,7 +11,10 @@ class kmeans:
                  max_iter=300, seed=-1, algorithm="lloyd", distance="euclidean",
                  normalize_data=False, tol=1.0e-4, empty_clusters="ignore",
                  afk_mcmc_samples=50, mixed_precision=False, low_precision_max_iter=200,
-                 low_precision_tol=1.0e-2, check_data=False):
+                 low_precision_tol=1.0e-2, check_data=False, precision="working"):
+        # precision: operand precision of the MFMA cross term for fp32 data ("working" =
+        # fp32, "bf16"); labels are re-checked exactly either way (MI355X extension option
+        # "cross term precision").
         self.n_clusters_opt = n_clusters
         self.initialization_method = initialization_method
         self.C = C
@@ -22,6 +25,7 @@ class kmeans:
         self.mixed_precision = mixed_precision
         self.low_precision_max_iter, self.low_precision_tol = low_precision_max_iter, low_precision_tol
         self.check_data = check_data
+        self.precision = precision
         self.status = None
         self._h = None
 
@@ -29,6 +33,8 @@ class kmeans:
         A, self.order, self.dtype = L.prepare(A)
         h = L.Handle(L.HANDLE_KMEANS, self.dtype)
         self._h = h
+        if getattr(self, "profile", False):
+            L.check(L.lib.da_amd_profile_enable(h.h, 1))
         init = self.initialization_method
         if self.C is not None:
             init = "supplied"
@@ -40,6 +46,7 @@ class kmeans:
         h.set_string("normalize data", "yes" if self.normalize_data else "no")
         h.set_string("mixed precision", "yes" if self.mixed_precision else "no")
         h.set_string("check data", "yes" if self.check_data else "no")
+        h.set_string("cross term precision", self.precision)
         h.set_int("n_clusters", self.n_clusters_opt)
         h.set_int("n_init", self.n_init)
         h.set_int("max_iter", self.max_iter)
@@ -111,6 +118,17 @@ class kmeans:
         ldx = m if self.order == L.COLUMN_MAJOR else d
         ldo = m if self.order == L.COLUMN_MAJOR else k
         self._h.call("da_kmeans_transform", m, d, L.ptr(X), ldx, L.ptr(out), ldo)
+        return out
+
+    def kernel_times(self, names=("kmeans_assign", "kmeans_recheck", "kmeans_update")):
+        """Per-kernel GPU time of the fit (set ``profile = True`` before fit): name ->
+        (total ms, launches), from HIP events on the handle's stream."""
+        out = {}
+        for name in names:
+            ms, cnt = C.c_double(), L.da_int()
+            L.check(L.lib.da_amd_profile_get(self._h.h, name.encode(), C.byref(ms), C.byref(cnt)))
+            if cnt.value:
+                out[name] = (ms.value, cnt.value)
         return out
 
     def predict(self, Y):

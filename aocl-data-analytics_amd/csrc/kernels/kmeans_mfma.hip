@@ -141,6 +141,137 @@ __global__ __launch_bounds__(256) void kmeans_assign_mfma_kernel(
     }
 }
 
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// The assignment with a bf16 cross term (v_mfma_f32_16x16x32_bf16, fp32 accumulation). The
+// data are a bf16 copy padded to Kp (a multiple of 32) features; lane group g holds features
+// g*KCH .. g*KCH + KCH - 1 of its sample (the k-permutation, so that a lane's operands are
+// contiguous), KCH / 8 MFMA steps per 16 x 16 tile. The near-tie bound adds the bf16
+// rounding of both operands to the fp32 bound.
+template <int KCH>
+__global__ __launch_bounds__(256) void kmeans_assign_bf16_kernel(
+    const __bf16 *__restrict__ X, int n, int Kp, const __bf16 *__restrict__ C16,
+    const float *__restrict__ chalf, int k, double cmax, int *__restrict__ labels,
+    const int *__restrict__ prev, int *__restrict__ changed, int *__restrict__ list_count,
+    int *__restrict__ list) {
+    constexpr int NF = KCH / 8;
+    extern __shared__ float smem[];
+    const int kpad = (k + 15) / 16 * 16;
+    __bf16 *Cs = reinterpret_cast<__bf16 *>(smem); // kpad x Kp centres
+    float *Hs = smem + (size_t)kpad * Kp / 2;       // -||c||^2/2, padded with -inf
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(C16);
+        uint4 *dst = reinterpret_cast<uint4 *>(Cs);
+        const int n16 = kpad * Kp / 8;
+        for (int t = threadIdx.x; t < n16; t += 256)
+            dst[t] = src[t];
+    }
+    for (int t = threadIdx.x; t < kpad; t += 256)
+        Hs[t] = t < k ? chalf[t] : -__builtin_huge_valf();
+    __syncthreads();
+
+    const int s0 = blockIdx.x * SB + w * PT * 16;
+    v8bf b[PT][NF];
+    float xn[PT];
+#pragma unroll
+    for (int pt = 0; pt < PT; pt++) {
+        const int i = s0 + pt * 16 + r;
+        float s = 0.f;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+            v8bf v = {};
+            if (i < n)
+                v = *reinterpret_cast<const v8bf *>(X + (int64_t)i * Kp + g * KCH + f * 8);
+            b[pt][f] = v;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const float x = (float)v[e];
+                s = fmaf(x, x, s);
+            }
+        }
+        xn[pt] = s;
+    }
+    float best1[PT], best2[PT];
+    int bidx[PT];
+#pragma unroll
+    for (int pt = 0; pt < PT; pt++) {
+        best1[pt] = -__builtin_huge_valf();
+        best2[pt] = -__builtin_huge_valf();
+        bidx[pt] = INT_MAX;
+    }
+    for (int c0 = 0; c0 < kpad; c0 += 16) {
+        v8bf a[NF];
+        const __bf16 *src = Cs + (c0 + r) * Kp + g * KCH;
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+            a[f] = *reinterpret_cast<const v8bf *>(src + f * 8);
+        v4f h;
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            h[i] = Hs[c0 + 4 * g + i];
+#pragma unroll
+        for (int pt = 0; pt < PT; pt++) {
+            v4f acc = h;
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[f], b[pt][f], acc, 0, 0, 0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float v = acc[i];
+                const int c = c0 + 4 * g + i;
+                if (v > best1[pt]) {
+                    best2[pt] = best1[pt];
+                    best1[pt] = v;
+                    bidx[pt] = c;
+                } else if (v > best2[pt]) {
+                    best2[pt] = v;
+                }
+            }
+        }
+    }
+    const double u = 1.0 / 256.0, u32 = 1.0 / 16777216.0;
+#pragma unroll
+    for (int pt = 0; pt < PT; pt++) {
+        float xs = xn[pt];
+#pragma unroll
+        for (int off = 16; off <= 32; off <<= 1) {
+            const float o1 = __shfl_xor(best1[pt], off, 64);
+            const float o2 = __shfl_xor(best2[pt], off, 64);
+            const int oi = __shfl_xor(bidx[pt], off, 64);
+            xs += __shfl_xor(xs, off, 64);
+            const bool other = (o1 > best1[pt]) || (o1 == best1[pt] && oi < bidx[pt]);
+            const float lo = other ? best1[pt] : o1;
+            if (other) {
+                best1[pt] = o1;
+                bidx[pt] = oi;
+            }
+            best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
+        }
+        const int i = s0 + pt * 16 + r;
+        if (g == 0 && i < n) {
+            // |x| of the data from the norm of its bf16 copy (relative error <= u), with a
+            // margin for the fp32 sum. Score error: operand rounding (2u + u^2)|x| cmax, the
+            // fp32 accumulation gamma_{Kp+2} over |x~| |c~| + cmax^2/2, the rounded -|c|^2/2.
+            const double xnorm = sqrt((double)xs) / (1.0 - u) * (1.0 + 1e-6);
+            const double gam = (Kp + 2) * u32 / (1.0 - (Kp + 2) * u32);
+            const double S = xnorm * cmax;
+            const double e = (2.0 * u + u * u) * S + gam * (S * (1.0 + u) * (1.0 + u) + 0.5 * cmax * cmax) +
+                             u32 * 0.5 * cmax * cmax;
+            const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
+            const int lab = bidx[pt] < k ? bidx[pt] : 0;
+            labels[i] = lab;
+            if (near_tie || !(best1[pt] - best1[pt] == 0.0f)) {
+                const int p = atomicAdd(list_count, 1);
+                list[p] = i;
+            } else if (prev && prev[i] != lab) {
+                atomicAdd(changed, 1);
+            }
+        }
+    }
+}
+
 // Exact re-check of listed samples: argmin of double squared distances, ties to the lowest
 // centre. Fixed grid; the number of listed samples is read on the device.
 template <typename T>
@@ -187,7 +318,92 @@ __global__ void centre_prep_kernel(const double *__restrict__ C, int k, int d, i
     atomicMax(reinterpret_cast<unsigned long long *>(cmax2), (unsigned long long)__double_as_longlong(s));
 }
 
+// bf16 centres padded to Kp features (rows up to kpad are zero), -||c||^2/2 in fp32 and
+// the largest ||c||^2 (atomic max on the bit pattern of a non-negative double).
+__global__ void centre_prep_bf16_kernel(const double *__restrict__ C, int k, int kpad, int d,
+                                        int Kp, __bf16 *__restrict__ C16,
+                                        float *__restrict__ chalf, double *__restrict__ cmax2) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= kpad)
+        return;
+    double s = 0.0;
+    for (int l = 0; l < Kp; l++) {
+        const double v = (c < k && l < d) ? C[(int64_t)c * d + l] : 0.0;
+        C16[(int64_t)c * Kp + l] = (__bf16)(float)v;
+        s = fma(v, v, s);
+    }
+    if (c < k) {
+        chalf[c] = (float)(-0.5 * s);
+        atomicMax(reinterpret_cast<unsigned long long *>(cmax2),
+                  (unsigned long long)__double_as_longlong(s));
+    }
+}
+
+__global__ void to_bf16_rows_kernel(const float *__restrict__ X, int n, int d, int ldx,
+                                    __bf16 *__restrict__ Y, int Kp) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * Kp)
+        return;
+    const int64_t i = t / Kp;
+    const int l = (int)(t - i * Kp);
+    Y[t] = l < d ? (__bf16)X[i * ldx + l] : (__bf16)0.0f;
+}
+
 } // namespace
+
+hipError_t launch_centre_prep_bf16(const double *C, int k, int d, int Kp, void *C16,
+                                   float *chalf, double *cmax2, hipStream_t s) {
+    const int kpad = (k + 15) / 16 * 16;
+    hipLaunchKernelGGL(centre_prep_bf16_kernel, dim3((kpad + 255) / 256), dim3(256), 0, s, C, k,
+                       kpad, d, Kp, static_cast<__bf16 *>(C16), chalf, cmax2);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_to_bf16(const float *X, int n, int d, int ldx, void *Y, int Kp,
+                               hipStream_t s) {
+    const int64_t total = (int64_t)n * Kp;
+    if (total == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(to_bf16_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       s, X, n, d, ldx, static_cast<__bf16 *>(Y), Kp);
+    return hipGetLastError();
+}
+
+size_t kmeans_assign_bf16_lds(int k, int Kp) {
+    const int kpad = (k + 15) / 16 * 16;
+    return (size_t)kpad * Kp * 2 + (size_t)kpad * sizeof(float);
+}
+
+hipError_t launch_kmeans_assign_bf16(const void *X, int n, int Kp, const void *C16,
+                                     const float *chalf, int k, double cmax, int *labels,
+                                     const int *prev, int *changed, int *list_count, int *list,
+                                     hipStream_t s) {
+    const size_t lds = kmeans_assign_bf16_lds(k, Kp);
+    if (lds > 160 * 1024 || Kp % 32 != 0)
+        return hipErrorInvalidValue;
+    dim3 grid((n + SB - 1) / SB), block(256);
+    const __bf16 *x = static_cast<const __bf16 *>(X);
+    const __bf16 *c = static_cast<const __bf16 *>(C16);
+#define DA_ASSIGN(KCHV)                                                                          \
+    {                                                                                            \
+        auto kern = kmeans_assign_bf16_kernel<KCHV>;                                             \
+        if (lds > 65536)                                                                         \
+            (void)hipFuncSetAttribute((const void *)kern,                                        \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);     \
+        hipLaunchKernelGGL(kern, grid, block, lds, s, x, n, Kp, c, chalf, k, cmax, labels, prev, \
+                           changed, list_count, list);                                           \
+    }
+    switch (Kp / 4) {
+    case 8: DA_ASSIGN(8) break;
+    case 16: DA_ASSIGN(16) break;
+    case 24: DA_ASSIGN(24) break;
+    case 32: DA_ASSIGN(32) break;
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef DA_ASSIGN
+    return hipGetLastError();
+}
 
 hipError_t launch_centre_prep(const double *C, int k, int d, int Kp, float *C32, float *chalf,
                               double *cmax2, hipStream_t s) {

@@ -24,6 +24,7 @@
 #include "launch.hpp"
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 namespace dagpu {
@@ -211,10 +212,11 @@ __device__ __forceinline__ void wave_fence() {
 // candidate buffers, thresholds and the merge use. Each wave owns its 32 queries' buffers,
 // so no workgroup barrier is needed inside the sweep. SPLIT operands carry each value as
 // hi + lo in the 16-bit type (query rows stored as [hi Kp][lo Kp]); the product is
-// hi*hi + hi*lo + lo*hi. The next step's training operands are loaded while the current
-// step's MFMAs run (two register buffers).
-template <class Op, int KCH, int KC, bool SPLIT, bool PREFETCH>
-__global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
+// hi*hi + hi*lo + lo*hi. Training operands are loaded from the tiled layout straight into
+// registers, one step at a time (PF: the next step's loads are issued before the current
+// step's MFMAs, into a second register buffer).
+template <class Op, int KCH, int KC, bool SPLIT, bool PF>
+__device__ __forceinline__ void knn_tile_body(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
     float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
@@ -355,14 +357,17 @@ __global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
         }
     };
 
-    if (!PREFETCH) {
+    if constexpr (!PF) {
         for (int tb = t_begin; tb < t_end; tb += STEP) {
             frag_t b0[COLT][NS][NF];
             ae h0[COLT];
             load_step(tb, b0, h0);
+            // Keep every load of the step ahead of its first MFMA.
+            __builtin_amdgcn_sched_barrier(0);
             compute_step(tb, b0, h0);
         }
     } else if (t_begin < t_end) {
+        // Two register buffers: the loads of step i+1 are issued before the MFMAs of step i.
         frag_t b0[COLT][NS][NF], b1[COLT][NS][NF];
         ae h0[COLT], h1[COLT];
         load_step(t_begin, b0, h0);
@@ -370,16 +375,287 @@ __global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
             const bool more1 = tb + STEP < t_end;
             if (more1)
                 load_step(tb + STEP, b1, h1);
+            __builtin_amdgcn_sched_barrier(0);
             compute_step(tb, b0, h0);
             if (!more1)
                 break;
             const bool more2 = tb + 2 * STEP < t_end;
             if (more2)
                 load_step(tb + 2 * STEP, b0, h0);
+            __builtin_amdgcn_sched_barrier(0);
             compute_step(tb + STEP, b1, h1);
             if (!more2)
                 break;
         }
+    }
+    for (int b = 0; b < ROWT * 16; b++)
+        compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + b);
+    wave_fence();
+    for (int b = 0; b < ROWT * 16; b++) {
+        const int q = q0 + qbase + b;
+        if (q >= nq)
+            continue;
+        const int ql = qbase + b;
+        const int64_t o = ((int64_t)q * nsplit + split) * KC;
+        for (int t = lane; t < KC; t += 64) {
+            const bool valid = t < cnt[ql];
+            part_key[o + t] = valid ? bkey[ql * CAP + t] : __builtin_huge_valf();
+            part_idx[o + t] = valid ? bidx[ql * CAP + t] : -1;
+        }
+        if (lane == 0)
+            part_tau[(int64_t)q * nsplit + split] = thr[ql];
+    }
+}
+
+// The sweep as a kernel: K2 asks the compiler for at most 256 registers per lane, two waves
+// per SIMD (the LDS of two workgroups fits a CU); the fp64 operands need more registers
+// and run one wave per SIMD without spilling.
+template <class Op, int KCH, int KC, bool SPLIT, bool PF>
+__global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
+    const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
+    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
+    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
+    knn_tile_body<Op, KCH, KC, SPLIT, PF>(Q, nq, Y, nyh, n, Kp, split_len, nsplit, part_key,
+                                          part_idx, part_tau);
+}
+template <class Op, int KCH, int KC, bool SPLIT, bool PF>
+__global__ __launch_bounds__(64 * TileShape<KC>::WV)
+__attribute__((amdgpu_waves_per_eu(2))) void knn_tile_kernel_k2(
+    const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
+    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
+    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
+    knn_tile_body<Op, KCH, KC, SPLIT, PF>(Q, nq, Y, nyh, n, Kp, split_len, nsplit, part_key,
+                                          part_idx, part_tau);
+}
+
+// Wave-uniform wait for this wave's outstanding vector-memory operations to drop to N and
+// for its LDS operations to finish, then a workgroup barrier. Inline assembly keeps the
+// compiler from moving LDS accesses across the barrier.
+template <int N> __device__ __forceinline__ void wait_vm_barrier() {
+    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
+__device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Direct global-to-LDS copy of 16 bytes per lane (4 bytes per lane for the dword form): lane
+// l's bytes land at LDS offset lds + 16 l (4 l). lds must be wave-uniform. Issued as inline
+// assembly so that the compiler, which does not see the LDS write, adds no waits of its own
+// for it: every read of the destination follows a wait_vm_barrier that covers the copy.
+__device__ __forceinline__ void lds_copy16(const void *src, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
+                 ::"s"(lds), "v"(src) : "memory", "m0");
+}
+__device__ __forceinline__ void lds_copy4(const void *src, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off"
+                 ::"s"(lds), "v"(src) : "memory", "m0");
+}
+template <typename P> __device__ __forceinline__ uint32_t lds_offset(P *p) {
+    return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) char *)(p);
+}
+
+// LDS-staged sweep. The same work as knn_tile_kernel, but the training rows stream through
+// an NSTAGE-deep ring of LDS stages (32 rows each) filled by direct global-to-LDS loads
+// issued NSTAGE-1 steps ahead and shared by all waves of the workgroup. Each wave copies
+// CPW 1 KB chunks of every stage (one 16-byte load per lane and chunk: the tiled layout
+// makes a chunk contiguous in memory and in LDS) and its share of the stage's -|y|^2/2
+// values. A step starts with a wait for this wave's copies of the stage and a barrier; the
+// slot refilled after that barrier is the one every wave finished reading in the previous
+// step.
+template <class Op, int KCH, int KC, bool SPLIT, int NSTAGE>
+__global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_staged_kernel(
+    const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
+    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
+    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
+    using in_t = typename Op::in_t;
+    using frag_t = typename Op::frag_t;
+    using acc_t = typename Op::acc_t;
+    using ae = typename Op::acc_e;
+    constexpr int WV = TileShape<KC>::WV;
+    constexpr int NF = KCH / Op::E;
+    constexpr int NS = SPLIT ? 2 : 1;
+    constexpr int KPC = 4 * KCH;                                 // padded features
+    constexpr int RB = KPC * (int)sizeof(in_t) * NS;             // bytes per row
+    constexpr int SBY = STEP * RB;                               // bytes per stage
+    constexpr int CPW = SBY / 1024 / WV;                         // 1 KB chunks per wave
+    constexpr int NYL = STEP * (int)sizeof(ae) / 4 / WV;         // nyh dwords per wave
+    constexpr int CSR = 16 / (int)sizeof(in_t);
+    constexpr int CST = 1024 / (int)sizeof(in_t);
+    constexpr int CAP = TileShape<KC>::CAP;
+    constexpr int ROWT = TileShape<KC>::RT;
+    constexpr int QB = WV * ROWT * 16;
+    constexpr int VMW = (NSTAGE - 2) * (CPW + 1);
+    static_assert(CPW >= 1 && CPW * WV * 1024 == SBY, "stage must split into whole chunks");
+    static_assert(NYL >= 1 && NYL <= 64, "nyh share");
+    static_assert((NSTAGE - 1) * (CPW + 1) <= 63, "vmcnt range");
+    __shared__ float bkey[QB * CAP];
+    __shared__ int bidx[QB * CAP];
+    __shared__ int cnt[QB];
+    __shared__ float thr[QB];
+    __shared__ __attribute__((aligned(1024))) char ring[NSTAGE * SBY];
+    __shared__ __attribute__((aligned(16))) ae nh[NSTAGE * STEP];
+    (void)Kp;
+
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int qblock = blockIdx.x / nsplit;
+    const int split = blockIdx.x - qblock * nsplit;
+    const int q0 = qblock * QB;
+    const int t_begin = split * split_len;
+    int t_end = t_begin + split_len;
+    if (t_end > n)
+        t_end = n;
+    const int nsteps = t_end > t_begin ? (t_end - t_begin + STEP - 1) / STEP : 0;
+    const int qbase = (w * ROWT) * 16;
+    const int64_t rowlen = (int64_t)KPC * NS;
+    if (lane < ROWT * 16) {
+        cnt[qbase + lane] = 0;
+        thr[qbase + lane] = __builtin_huge_valf();
+    }
+
+    frag_t a[NS][ROWT][NF];
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++) {
+        const int qr = q0 + qbase + rt * 16 + r;
+#pragma unroll
+        for (int h = 0; h < NS; h++) {
+            Op::template load<NF, CSR>(Q + (int64_t)(qr < nq ? qr : 0) * rowlen + h * KPC + g * KCH,
+                                       a[h][rt]);
+            if (qr >= nq) {
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+                    a[h][rt][f] = frag_t{};
+            }
+        }
+    }
+    // Consume the query operands here, so that the compiler's waits for these loads come
+    // before the sweep and not inside it, where they would also wait for the stage copies.
+#pragma unroll
+    for (int h = 0; h < NS; h++)
+#pragma unroll
+        for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+            for (int f = 0; f < NF; f++)
+                asm volatile("" ::"v"(a[h][rt][f]));
+    ae ts[ROWT][4];
+    auto row_of = [&](int i) { return Op::rows_strided ? g + 4 * i : 4 * g + i; };
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            ts[rt][i] = -__builtin_huge_val();
+    wave_fence();
+    const int t_last = t_begin + (nsteps > 0 ? nsteps - 1 : 0) * STEP;
+    const uint32_t ring_lds = lds_offset(ring);
+    const uint32_t nh_lds = lds_offset(nh);
+
+    // Copies stage s of the split (rows t_begin + 32 s ..) into ring slot s % NSTAGE.
+    // Stages past the end re-read the last stage, so every wave issues the same number of
+    // copies per step.
+    auto issue = [&](int s) {
+        int tb = t_begin + s * STEP;
+        if (tb > t_last)
+            tb = t_last;
+        const int slot = s % NSTAGE;
+        const char *src = reinterpret_cast<const char *>(Y) + (int64_t)tb * RB + lane * 16;
+#pragma unroll
+        for (int c = 0; c < CPW; c++) {
+            const int chunk = w * CPW + c;
+            lds_copy16(src + chunk * 1024,
+                       __builtin_amdgcn_readfirstlane(ring_lds + slot * SBY + chunk * 1024));
+        }
+        const uint32_t ndst =
+            __builtin_amdgcn_readfirstlane(nh_lds + slot * STEP * (int)sizeof(ae) + w * NYL * 4);
+        const char *nsrc = reinterpret_cast<const char *>(nyh + tb) + (w * NYL + (lane < NYL ? lane : 0)) * 4;
+        if (lane < NYL)
+            lds_copy4(nsrc, ndst);
+    };
+
+    auto compute = [&](int tb, int slot) {
+        const char *stage = ring + slot * SBY;
+        const ae *nhs = nh + slot * STEP;
+        acc_t acc[ROWT][COLT];
+#pragma unroll
+        for (int ct = 0; ct < COLT; ct++) {
+            const ae hv = nhs[ct * 16 + r];
+#pragma unroll
+            for (int rt = 0; rt < ROWT; rt++)
+                acc[rt][ct] = acc_t{hv, hv, hv, hv};
+        }
+        frag_t b[COLT][NS][NF];
+#pragma unroll
+        for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+            for (int h = 0; h < NS; h++)
+                Op::template load<NF, CST>(
+                    reinterpret_cast<const in_t *>(stage + ct * 16 * RB) + h * 16 * KPC + lane * CSR,
+                    b[ct][h]);
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+#pragma unroll
+            for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+                for (int rt = 0; rt < ROWT; rt++) {
+                    acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][0][f], acc[rt][ct]);
+                    if constexpr (SPLIT) {
+                        acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][NS - 1][f], acc[rt][ct]);
+                        acc[rt][ct] = Op::mfma(a[NS - 1][rt][f], b[ct][0][f], acc[rt][ct]);
+                    }
+                }
+        }
+        bool hit = false;
+#pragma unroll
+        for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int ct = 0; ct < COLT; ct++)
+                    hit |= acc[rt][ct][i] > ts[rt][i];
+        if (__ballot(hit) == 0)
+            return;
+#pragma unroll
+        for (int rt = 0; rt < ROWT; rt++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int q = qbase + rt * 16 + row_of(i);
+#pragma unroll
+                for (int ct = 0; ct < COLT; ct++) {
+                    if (acc[rt][ct][i] > ts[rt][i]) {
+                        const int pos = atomicAdd(&cnt[q], 1);
+                        bkey[q * CAP + pos] = (float)(-2.0 * (double)acc[rt][ct][i]);
+                        bidx[q * CAP + pos] = tb + ct * 16 + r;
+                    }
+                }
+            }
+        }
+        wave_fence();
+        const bool need = (lane < ROWT * 16) && (cnt[qbase + (lane & 31)] > CAP - STEP);
+        unsigned long long mask = __ballot(need);
+        if (mask) {
+            while (mask) {
+                const int bq = __ffsll((long long)mask) - 1;
+                mask &= mask - 1;
+                compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + bq);
+            }
+            wave_fence();
+#pragma unroll
+            for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    ts[rt][i] = (ae)(-0.5 * (double)thr[qbase + rt * 16 + row_of(i)]);
+        }
+    };
+
+    if (nsteps > 0) {
+        for (int s = 0; s < NSTAGE - 1; s++)
+            issue(s);
+        for (int i = 0; i < nsteps; i++) {
+            wait_vm_barrier<VMW>();
+            issue(i + NSTAGE - 1);
+            compute(t_begin + i * STEP, i % NSTAGE);
+        }
+        // Copies of stages past the end may still be in flight into LDS.
+        wait_vm_all();
     }
     for (int b = 0; b < ROWT * 16; b++)
         compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + b);
@@ -600,24 +876,39 @@ int knn_tile_qb(int kc) {
 int knn_merge_max_candidates() { return MMAX; }
 
 
-// Launches the fused tile for one operand type, candidate-list length and variant or, with
-// `slots` set, reports how many of its workgroups the device runs at once instead.
-template <class Op, int KC, bool SPLIT, bool PREFETCH>
-static hipError_t launch_tile_pf(const void *Q, int nq, const void *Y, const void *nyh, int n,
-                                 int Kp, int nsplit, float *part_key, int *part_idx,
-                                 float *part_tau, int *slots, hipStream_t s) {
+// Geometry of the LDS-staged kernel: stage bytes, and whether a stage splits into whole
+// 1 KB chunks per wave.
+template <class Op, int KCH, int KC, bool SPLIT> struct StagedGeom {
+    static constexpr int RB = 4 * KCH * (int)sizeof(typename Op::in_t) * (SPLIT ? 2 : 1);
+    static constexpr int SBY = STEP * RB;
+    static constexpr bool ok = SBY % (1024 * TileShape<KC>::WV) == 0;
+    static constexpr int NSTAGE = SBY * 4 <= 65536 ? 4 : SBY * 3 <= 65536 ? 3 : 2;
+};
+
+// Launches the fused tile for one operand type, candidate-list length, width and variant
+// (0: register operands, 1: LDS-staged where the geometry allows, else 0; 2: register
+// operands with the next step's loads in flight) or, with `slots`
+// set, reports how many of its workgroups the device runs at once instead.
+template <class Op, int KCH, int KC, bool SPLIT>
+static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void *nyh, int n,
+                                int Kp, int nsplit, float *part_key, int *part_idx,
+                                float *part_tau, int variant, int *slots, hipStream_t s) {
     using in_t = typename Op::in_t;
     using ae = typename Op::acc_e;
+    using G = StagedGeom<Op, KCH, KC, SPLIT>;
     using kernel_t = void (*)(const in_t *, int, const in_t *, const ae *, int, int, int, int,
                               float *, int *, float *);
     constexpr int QB = TileShape<KC>::WV * TileShape<KC>::RT * 16;
     kernel_t kern;
-    switch (Kp / 4) {
-    case 8: kern = knn_tile_kernel<Op, 8, KC, SPLIT, PREFETCH>; break;
-    case 16: kern = knn_tile_kernel<Op, 16, KC, SPLIT, PREFETCH>; break;
-    case 24: kern = knn_tile_kernel<Op, 24, KC, SPLIT, PREFETCH>; break;
-    case 32: kern = knn_tile_kernel<Op, 32, KC, SPLIT, PREFETCH>; break;
-    default: return hipErrorInvalidValue;
+    if constexpr (std::is_same_v<Op, OpF64>)
+        kern = variant == 2 ? knn_tile_kernel<Op, KCH, KC, SPLIT, true>
+                            : knn_tile_kernel<Op, KCH, KC, SPLIT, false>;
+    else
+        kern = variant == 2 ? knn_tile_kernel_k2<Op, KCH, KC, SPLIT, true>
+                            : knn_tile_kernel_k2<Op, KCH, KC, SPLIT, false>;
+    if constexpr (G::ok) {
+        if (variant == 1)
+            kern = knn_tile_staged_kernel<Op, KCH, KC, SPLIT, G::NSTAGE>;
     }
     const dim3 block(64 * TileShape<KC>::WV);
     if (slots) {
@@ -644,11 +935,18 @@ template <class Op, int KC, bool SPLIT>
 static hipError_t launch_tile_op(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                  int Kp, int nsplit, float *part_key, int *part_idx,
                                  float *part_tau, int variant, int *slots, hipStream_t s) {
-    if (variant & 1)
-        return launch_tile_pf<Op, KC, SPLIT, true>(Q, nq, Y, nyh, n, Kp, nsplit, part_key,
-                                                   part_idx, part_tau, slots, s);
-    return launch_tile_pf<Op, KC, SPLIT, false>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx,
-                                                part_tau, slots, s);
+    switch (Kp / 4) {
+    case 8:
+        return launch_tile_w<Op, 8, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+    case 16:
+        return launch_tile_w<Op, 16, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+    case 24:
+        return launch_tile_w<Op, 24, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+    case 32:
+        return launch_tile_w<Op, 32, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+    default:
+        return hipErrorInvalidValue;
+    }
 }
 
 int knn_tile_kp(int d) {

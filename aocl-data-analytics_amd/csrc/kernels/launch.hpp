@@ -180,8 +180,11 @@ int knn_tile_kp(int d);
 // layout of launch_knn_tile_operand, padded to a multiple of 32 rows; nyh: -|y|^2/2 per row
 // (double for KP_F64, float otherwise).
 // Writes part_key/part_idx[(q*nsplit + s)*kc + i] (keys |y|^2 - 2 q.y as float) and
-// part_tau[q*nsplit + s]. kc in {16, 32, 64, 128}. variant bit 0: prefetch the next training tile
-// into a second register buffer.
+// part_tau[q*nsplit + s]. kc in {16, 32, 64, 128}. variant 0: training operands loaded into
+// registers step by step; 1: streamed through a ring of LDS stages by direct global-to-LDS
+// loads (used where a 32-row stage splits into whole 1 KB chunks per wave, else variant 0);
+// 2: as 0 with the next step's loads issued before the current step's MFMAs.
+// nyh must hold a multiple of 32 entries (the padded rows at -inf).
 hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                            int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
                            float *part_tau, int variant, hipStream_t s);
@@ -212,6 +215,17 @@ hipError_t launch_kmeans_assign_mfma(const float *X, int n, int Kp, const float 
                                      const float *chalf, int k, double cmax, int *labels,
                                      const int *prev, int *changed, int *list_count, int *list,
                                      hipStream_t s);
+// bf16 cross-term form of the same assignment: X and C16 are bf16 rows padded to Kp (a
+// multiple of 32, at most 128) features; C16 has (k+15)/16*16 rows.
+size_t kmeans_assign_bf16_lds(int k, int Kp);
+hipError_t launch_kmeans_assign_bf16(const void *X, int n, int Kp, const void *C16,
+                                     const float *chalf, int k, double cmax, int *labels,
+                                     const int *prev, int *changed, int *list_count, int *list,
+                                     hipStream_t s);
+hipError_t launch_centre_prep_bf16(const double *C, int k, int d, int Kp, void *C16,
+                                   float *chalf, double *cmax2, hipStream_t s);
+hipError_t launch_rows_to_bf16(const float *X, int n, int d, int ldx, void *Y, int Kp,
+                               hipStream_t s);
 template <typename T>
 hipError_t launch_kmeans_recheck(const T *X, int ldx, int d, const double *C, int k,
                                  const int *list_count, const int *list, int *labels,

@@ -63,6 +63,14 @@ template <typename T> KMeansHandle<T>::KMeansHandle(ErrorStack &e, GpuContext &g
                     "Whether to normalize the input data before clustering. This option is only "
                     "used if distance is set to cosine.",
                     {{"yes", 1}, {"no", 0}}, "yes");
+    // MI355X extension: operand precision of the MFMA cross term of the assignment (fp32
+    // data). "working": fp32 MFMA; "bf16": bf16 MFMA with fp32 accumulation. Both list the
+    // samples whose two nearest centres are within the error bound and re-check them with
+    // exact double distances, so the labels do not depend on the choice.
+    opts.add_string("cross term precision",
+                    "Operand precision of the MFMA cross term in the assignment step (fp32 data): "
+                    "working or bf16.",
+                    {{"working", 0}, {"bf16", 1}}, "working");
     opts.add_real("convergence tolerance", "Convergence tolerance.", 0.0,
                   LowerBound::greater_equal, 0.0, UpperBound::p_inf, (double)(T)1.0e-4);
     opts.add_real("low precision convergence tolerance",
@@ -148,6 +156,8 @@ da_status KMeansHandle<T>::set_data(da_int n_samples, da_int n_features, const T
     if (s != da_status_success)
         return s;
     mfma_ok_ = false;
+    bf16_ok_ = false;
+    xb_ready_ = false;
     if constexpr (std::is_same_v<T, float>) {
         // Padded copy for the MFMA assignment (feature count rounded up to 16, at most 128).
         if (n_features <= 128) {
@@ -160,6 +170,9 @@ da_status KMeansHandle<T>::set_data(da_int n_samples, da_int n_features, const T
                                                         n_samples, gpu->stream())));
             DA_HIP(err, hipStreamSynchronize(gpu->stream()));
             mfma_ok_ = true;
+            kp16_ = (n_features + 31) / 32 * 32;
+            bf16_ok_ = true;
+            xb_ready_ = false;
         }
     }
     initdone_ = true;
@@ -230,10 +243,42 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
     const int n = n_samples_, d = n_features_, k = n_clusters_;
     int *dchanged = reinterpret_cast<int *>(small_.as<double>() + 4);
     DA_HIP(err, hipMemsetAsync(dchanged, 0, 2 * sizeof(int), st));
-    const bool use_mfma = std::is_same_v<U, float> && mfma_ok_ &&
-                          (const void *)dX == (const void *)X_.ptr() &&
-                          kmeans_assign_mfma_lds(k, kp_) <= 160 * 1024;
-    if (use_mfma) {
+    const bool on_data = std::is_same_v<U, float> && (const void *)dX == (const void *)X_.ptr();
+    const bool use_bf16 = on_data && bf16_ok_ && opts.skey("cross term precision") == 1 &&
+                          kmeans_assign_bf16_lds(k, kp16_) <= 160 * 1024;
+    const bool use_mfma = on_data && mfma_ok_ && kmeans_assign_mfma_lds(k, kp_) <= 160 * 1024;
+    if (use_bf16) {
+        // bf16 cross term on MFMA, exact re-check of near ties (kmeans_mfma.hip).
+        int *dlist_count = dchanged + 1;
+        double *cmax2 = small_.as<double>() + 6;
+        DA_HIP(err, xb_.reserve((size_t)n * kp16_ * 2));
+        if (!xb_ready_) {
+            DA_HIP(err, launch_rows_to_bf16(reinterpret_cast<const float *>(X_.ptr()), n, d, d,
+                                            xb_.ptr(), kp16_, st));
+            xb_ready_ = true;
+        }
+        DA_HIP(err, c16_.reserve((size_t)(k + 16) * kp16_ * 2));
+        DA_HIP(err, chalf_.reserve((size_t)(k + 16) * sizeof(float)));
+        DA_HIP(err, list_.reserve((size_t)n * sizeof(int)));
+        DA_HIP(err, hipMemsetAsync(cmax2, 0, sizeof(double), st));
+        DA_HIP(err, launch_centre_prep_bf16(Cin, k, d, kp16_, c16_.ptr(), chalf_.as<float>(),
+                                            cmax2, st));
+        double hc = 0;
+        DA_HIP(err, hipMemcpyAsync(&hc, cmax2, sizeof(double), hipMemcpyDeviceToHost, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        {
+            KernelScope ks(*gpu, "kmeans_assign");
+            DA_HIP(err, launch_kmeans_assign_bf16(xb_.ptr(), n, kp16_, c16_.ptr(),
+                                                  chalf_.as<float>(), k, std::sqrt(hc), lab_new,
+                                                  lab_old, dchanged, dlist_count, list_.as<int>(),
+                                                  st));
+        }
+        {
+            KernelScope ks(*gpu, "kmeans_recheck");
+            DA_HIP(err, launch_kmeans_recheck<U>(dX, d, d, Cin, k, dlist_count, list_.as<int>(),
+                                                 lab_new, lab_old, dchanged, st));
+        }
+    } else if (use_mfma) {
         // fp32 cross term on MFMA, exact re-check of near ties (kmeans_mfma.hip).
         int *dlist_count = dchanged + 1;
         double *cmax2 = small_.as<double>() + 6;

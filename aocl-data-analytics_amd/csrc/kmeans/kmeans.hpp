@@ -67,6 +67,11 @@ template <typename T> class KMeansHandle : public AlgHandle {
     DeviceBuffer xpad_, c32_, chalf_, list_;
     int kp_ = 0;
     bool mfma_ok_ = false;
+    // bf16 cross term: bf16 copy of the data padded to kp16_ (a multiple of 32) features,
+    // made at the first bf16 assignment; bf16 centres.
+    DeviceBuffer xb_, c16_;
+    int kp16_ = 0;
+    bool bf16_ok_ = false, xb_ready_ = false;
     da_int empty_mode_ = 0;
     double *Ccur_ = nullptr, *Cprev_ = nullptr;
     int *lab_cur_ = nullptr, *lab_prev_ = nullptr;

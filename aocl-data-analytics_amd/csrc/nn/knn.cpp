@@ -65,14 +65,15 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
                      {"bf16x3", 5}},
                     "working");
     // MI355X extension: kernel variant of the fused tile (-1 = chosen by the tuning table,
-    // 0 = single register buffer, 1 = prefetch the next training tile).
+    // 0 = training operands loaded into registers, 1 = streamed through LDS stages,
+    // 2 = registers with the next step's loads in flight).
     opts.add_int("tile variant", "Kernel variant of the fused kNN distance tile (-1 automatic).",
-                 -1, LowerBound::greater_equal, 1, UpperBound::less_equal, -1);
+                 -1, LowerBound::greater_equal, 2, UpperBound::less_equal, -1);
 }
 
-// Tuning table of the fused tile variant per operand precision and padded width: the
-// single-buffered sweep runs two waves per SIMD where the prefetching one holds one, and
-// measured faster for every form at 128 features.
+// Tuning table of the fused tile variant per operand precision and padded width. At 128
+// features the register sweep (two workgroups per CU) measured faster than the LDS-staged
+// one (one workgroup per CU, a barrier per step) for fp32 and fp16x3 operands.
 static int knn_tile_auto_variant(int prec, int kp) {
     (void)prec;
     (void)kp;
@@ -219,19 +220,22 @@ template <typename T> da_status NNHandle<T>::prepare_fused(int prec) {
     for (int i = 0; i < n; i++)
         mx = std::max(mx, sq[i]);
     ymax_ = std::sqrt(mx);
+    // -|y|^2/2 per row, padded to rows_pad entries with -inf (padded rows never score).
     if (prec == KP_F64) {
-        std::vector<double> h(n);
+        std::vector<double> h(rows_pad, -std::numeric_limits<double>::infinity());
         for (int i = 0; i < n; i++)
             h[i] = -0.5 * sq[i];
-        DA_HIP(err, nyh_.reserve(n * sizeof(double)));
-        DA_HIP(err, hipMemcpyAsync(nyh_.ptr(), h.data(), n * sizeof(double), hipMemcpyHostToDevice, st));
+        DA_HIP(err, nyh_.reserve(rows_pad * sizeof(double)));
+        DA_HIP(err, hipMemcpyAsync(nyh_.ptr(), h.data(), rows_pad * sizeof(double),
+                                   hipMemcpyHostToDevice, st));
         DA_HIP(err, hipStreamSynchronize(st));
     } else {
-        std::vector<float> h(n);
+        std::vector<float> h(rows_pad, -std::numeric_limits<float>::infinity());
         for (int i = 0; i < n; i++)
             h[i] = (float)(-0.5 * sq[i]);
-        DA_HIP(err, nyh_.reserve(n * sizeof(float)));
-        DA_HIP(err, hipMemcpyAsync(nyh_.ptr(), h.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
+        DA_HIP(err, nyh_.reserve(rows_pad * sizeof(float)));
+        DA_HIP(err, hipMemcpyAsync(nyh_.ptr(), h.data(), rows_pad * sizeof(float),
+                                   hipMemcpyHostToDevice, st));
         DA_HIP(err, hipStreamSynchronize(st));
     }
     ypad_prec_ = prec;

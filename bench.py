@@ -211,6 +211,73 @@ def serving(args, L, aoclda):
     print(json.dumps(out))
 
 
+KMEANS_METRIC = ("k-means Lloyd step time with the bf16 precision form (exact re-check of near "
+                 "ties), n=2,000,000 d=32 k=256, 1 GPU")
+
+
+def kmeans_bench(args, L, aoclda):
+    """Lloyd on 2,000,000 x 32 fp32 points from 256 Gaussian blobs (means U[-20,20]^32, sigma
+    U[0.5,2]), k = 256, k-means++ with seed 42, 25 iterations at tolerance 0. Each cross-term
+    form is timed; labels and inertia are compared with the same run on the data in fp64
+    (exact double arithmetic throughout)."""
+    rng = np.random.default_rng(3)
+    n, d, k = args.km_n, 32, 256
+    means = rng.uniform(-20, 20, (256, d))
+    sig = rng.uniform(0.5, 2.0, 256)
+    comp = rng.integers(0, 256, n)
+    X = (means[comp] + sig[comp, None] * rng.standard_normal((n, d))).astype(np.float32)
+    log(f"kmeans: data {n}x{d} ready")
+
+    def run(data, form):
+        km = aoclda.clustering.kmeans(n_clusters=k, initialization_method="k-means++", n_init=1,
+                                      max_iter=args.km_iter, seed=42, tol=0.0, precision=form)
+        km.profile = True
+        t0 = time.perf_counter()
+        km.fit(data)
+        el = time.perf_counter() - t0
+        return km, el
+
+    ref, ref_t = run(X.astype(np.float64), "working")
+    ref_lab = ref.labels.copy()
+    ref_inertia = float(ref.inertia)
+    log(f"kmeans: fp64 reference {ref_t:.2f}s, {ref.n_iter} iterations")
+    forms = {}
+    for form in args.km_forms.split(","):
+        km, el = run(X, form)
+        kt = km.kernel_times()
+        it = max(1, int(km.n_iter))
+        step_ms = sum(v[0] for v in kt.values()) / it
+        forms[form] = {
+            "fit_s": el, "n_iter": int(km.n_iter),
+            "lloyd_step_ms": step_ms,
+            "kernels_ms": {nm: {"total_ms": v[0], "launches": v[1]} for nm, v in kt.items()},
+            "label_agreement": float(np.mean(km.labels == ref_lab)),
+            "inertia_rel_err": abs(float(km.inertia) - ref_inertia) / ref_inertia,
+        }
+        log(f"kmeans {form}: fit {el:.2f}s, step {step_ms:.3f} ms, agreement "
+            f"{forms[form]['label_agreement']}")
+    head = min(forms, key=lambda f: forms[f]["lloyd_step_ms"])
+    H = forms[head]
+    a_ms, a_cnt = H["kernels_ms"]["kmeans_assign"]["total_ms"], H["kernels_ms"]["kmeans_assign"]["launches"]
+    per = a_ms / a_cnt * 1e-3
+    if head == "bf16":
+        kp = (d + 31) // 32 * 32
+        byts = n * kp * 2 + 8 * n  # bf16 rows, labels written and previous labels read
+        roof = {"kernel": "kmeans_assign", "bound": "hbm", "achieved": byts / per / 1e9,
+                "peak": 8000.0, "unit": "GB/s", "frac": byts / per / 8e12, "traffic": None}
+    else:
+        fl = 2.0 * n * k * d
+        roof = {"kernel": "kmeans_assign", "bound": "mfma", "achieved": fl / per / 1e12,
+                "peak": 157.3, "unit": "TFLOP/s", "frac": fl / per / 157.3e12, "traffic": None}
+    print(json.dumps({
+        "metric": KMEANS_METRIC, "value": H["lloyd_step_ms"], "unit": "ms per Lloyd step",
+        "n_gpus": 1, "higher_is_better": False, "dtype": "bf16" if head == "bf16" else "f32",
+        "data": "synthetic: 256 Gaussian blobs, numpy default_rng(3)",
+        "config": {"workload": f"k-means Lloyd, {n}x{d} fp32, k={k}, k-means++ seed 42, "
+                               f"{args.km_iter} iterations, tol 0", "precision_form": head},
+        "fp64_reference_fit_s": ref_t, "forms": forms, "roofline": roof}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +295,11 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (two launches), the "
                          "reference point that scales memory counters to bytes")
+    ap.add_argument("--kmeans", action="store_true",
+                    help="run the k-means precision-form benchmark instead")
+    ap.add_argument("--km-n", type=int, default=2_000_000)
+    ap.add_argument("--km-iter", type=int, default=25)
+    ap.add_argument("--km-forms", default="working,bf16")
     ap.add_argument("--serving", action="store_true",
                     help="run the continuous-batching serving benchmark instead")
     ap.add_argument("--serve-rows", type=int, default=2_000_000)
@@ -248,6 +320,9 @@ def main():
 
     if args.serving:
         serving(args, L, aoclda)
+        return
+    if args.kmeans:
+        kmeans_bench(args, L, aoclda)
         return
 
     if args.copy_calibration_mib > 0:

@@ -162,3 +162,40 @@ def test_larger_mfma_vs_oracle(oracle):
     np.testing.assert_array_equal(km.labels, r["labels"])
     assert km.n_iter == r["n_iter"]
     np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("d", [8, 32, 40, 96])
+def test_bf16_cross_term_matches_exact(oracle, d):
+    """The bf16 cross term with exact re-check of near ties gives the labels, iteration count
+    and centres of the exact computation."""
+    rng = np.random.default_rng(30 + d)
+    centres = rng.uniform(-20, 20, (24, d))
+    X = blobs(rng, 300, centres, sigma=1.5).astype(np.float32)
+    rng.shuffle(X)
+    C0 = X[:24].copy()
+    km = aoclda.clustering.kmeans(n_clusters=24, C=C0, max_iter=20, tol=0.0,
+                                  precision="bf16").fit(X)
+    r = oracle.kmeans(X.astype(np.float64), 24, "supplied", C0.astype(np.float64), max_iter=20,
+                      tol=0.0)
+    np.testing.assert_array_equal(km.labels, r["labels"])
+    assert km.n_iter == r["n_iter"]
+    np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-5, atol=1e-5)
+    ref = aoclda.clustering.kmeans(n_clusters=24, C=C0, max_iter=20, tol=0.0).fit(X)
+    np.testing.assert_array_equal(km.labels, ref.labels)
+    np.testing.assert_array_equal(km.cluster_centres, ref.cluster_centres)
+
+
+def test_bf16_cross_term_near_ties(oracle):
+    """Blobs that overlap, so that many samples are within the bf16 error of a decision
+    boundary and go through the exact re-check."""
+    rng = np.random.default_rng(41)
+    centres = rng.uniform(-1, 1, (30, 16))
+    X = blobs(rng, 200, centres, sigma=0.8).astype(np.float32)
+    rng.shuffle(X)
+    C0 = X[:30].copy()
+    km = aoclda.clustering.kmeans(n_clusters=30, C=C0, max_iter=10, tol=0.0,
+                                  precision="bf16").fit(X)
+    r = oracle.kmeans(X.astype(np.float64), 30, "supplied", C0.astype(np.float64), max_iter=10,
+                      tol=0.0)
+    np.testing.assert_array_equal(km.labels, r["labels"])
+    np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-5, atol=1e-5)

@@ -210,3 +210,25 @@ def test_fused_large_k(oracle, k, precision):
     nfb, ncand = L.da_int(), L.da_int()
     L.check(L.lib.da_nn_last_query_stats(nn.handle.h, C.byref(nfb), C.byref(ncand)))
     assert ncand.value >= k  # the fused tile path ran
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("precision", ["working", "fp64", "fp16", "fp16x3"])
+@pytest.mark.parametrize("d", [20, 64, 128])
+def test_tile_variants(oracle, variant, precision, d):
+    """Every sweep of the fused tile (register operands, LDS-staged, register prefetch) gives
+    the exact result, including training sizes that end inside a stage and splits of unequal
+    length."""
+    rng = np.random.default_rng(200 + d)
+    means = rng.uniform(-5, 5, (8, d))
+    for n in (1000, 4133):
+        Xt = gaussian_mixture(rng, n, means)
+        Xq = gaussian_mixture(rng, 140, means)
+        dtype = np.float64 if precision == "fp64" else np.float32
+        prec = "working" if precision == "fp64" else precision
+        nn = aoclda.neighbors.nearest_neighbors(precision=prec, tile_variant=variant).fit(
+            Xt.astype(dtype))
+        dist, ind = nn.kneighbors(Xq.astype(dtype), n_neighbors=12)
+        ei, ed = oracle.knn(Xt, Xq, 12)
+        np.testing.assert_array_equal(ind, ei)
+        np.testing.assert_allclose(dist, ed, rtol=1e-12 if dtype == np.float64 else 1e-5)
