@@ -131,6 +131,14 @@ class kmeans:
                 out[name] = (ms.value, cnt.value)
         return out
 
+    @property
+    def n_rechecked(self):
+        """Samples re-assigned with exact double distances during the last fit (summed over
+        iterations)."""
+        v = L.da_int()
+        L.check(L.lib.da_kmeans_last_compute_stats(self._h.h, C.byref(v)))
+        return v.value
+
     def predict(self, Y):
         Y, _, _ = L.prepare(Y, dtype=self.dtype, order=self.order)
         m, d = Y.shape
@@ -138,3 +146,46 @@ class kmeans:
         ld = m if self.order == L.COLUMN_MAJOR else d
         self._h.call("da_kmeans_predict", m, d, L.ptr(Y), ld, L.ptr(out))
         return out
+
+
+def kmeans_batch(datasets, init_centres, max_iter=300, tol=1.0e-4):
+    """Batched Lloyd k-means (MI355X extension da_kmeans_batch_device_s): instance i clusters
+    the fp32 rows datasets[i] (n_i x d_i, d_i <= 64) from the initial centres init_centres[i]
+    (k_i x d_i, k_i <= 64). Returns (centres, labels, n_iter, inertia) as lists / arrays, one
+    entry per instance."""
+    X = [np.ascontiguousarray(a, dtype=np.float32) for a in datasets]
+    C0 = [np.ascontiguousarray(c, dtype=np.float32) for c in init_centres]
+    if len(X) != len(C0) or not X:
+        raise ValueError("one initial centre array per dataset is required")
+    n = np.array([a.shape[0] for a in X], dtype=np.int32)
+    d = np.array([a.shape[1] for a in X], dtype=np.int32)
+    k = np.array([c.shape[0] for c in C0], dtype=np.int32)
+    xs = np.concatenate([a.ravel() for a in X])
+    cs = np.concatenate([c.ravel() for c in C0])
+    dx, dc, dl = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    L.check(L.lib.da_amd_malloc(xs.nbytes, C.byref(dx)))
+    L.check(L.lib.da_amd_malloc(cs.nbytes, C.byref(dc)))
+    L.check(L.lib.da_amd_malloc(int(n.sum()) * 4, C.byref(dl)))
+    try:
+        L.check(L.lib.da_amd_memcpy_htod(dx, L.ptr(xs), xs.nbytes))
+        L.check(L.lib.da_amd_memcpy_htod(dc, L.ptr(cs), cs.nbytes))
+        n_iter = np.zeros(len(X), dtype=np.int32)
+        inertia = np.zeros(len(X), dtype=np.float32)
+        L.check(L.lib.da_kmeans_batch_device_s(len(X), L.ptr(n), L.ptr(d), L.ptr(k), dx, dc, dl,
+                                               max_iter, tol, L.ptr(n_iter), L.ptr(inertia)))
+        cout = np.empty_like(cs)
+        lab = np.empty(int(n.sum()), dtype=np.int32)
+        L.check(L.lib.da_amd_memcpy_dtoh(L.ptr(cout), dc, cout.nbytes))
+        L.check(L.lib.da_amd_memcpy_dtoh(L.ptr(lab), dl, lab.nbytes))
+    finally:
+        for p in (dx, dc, dl):
+            L.lib.da_amd_free(p)
+        L.check(L.lib.da_kmeans_batch_release())
+    centres, labels = [], []
+    co = lo = 0
+    for i in range(len(X)):
+        centres.append(cout[co:co + k[i] * d[i]].reshape(k[i], d[i]))
+        labels.append(lab[lo:lo + n[i]])
+        co += k[i] * d[i]
+        lo += n[i]
+    return centres, labels, n_iter, inertia

@@ -92,16 +92,16 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void kmeans_partial_kernel(
+__global__ __launch_bounds__(1024) void kmeans_partial_kernel(
     const T *__restrict__ X, int n, int d, const int *__restrict__ labels, int c0, int kc,
     const int *__restrict__ escale, int ppb, long long *__restrict__ part,
     int *__restrict__ cnt) {
     extern __shared__ long long acc[];
     const int words = kc * d * 2;
     int *lcnt = reinterpret_cast<int *>(acc + words);
-    for (int t = threadIdx.x; t < words; t += 256)
+    for (int t = threadIdx.x; t < words; t += 1024)
         acc[t] = 0;
-    for (int t = threadIdx.x; t < kc; t += 256)
+    for (int t = threadIdx.x; t < kc; t += 1024)
         lcnt[t] = 0;
     __syncthreads();
     const int b = blockIdx.x;
@@ -110,13 +110,11 @@ __global__ __launch_bounds__(256) void kmeans_partial_kernel(
     if (end > n)
         end = n;
     const int lshift = escale[d];
-    for (int64_t e = start * d + threadIdx.x; e < end * d; e += 256) {
-        const int64_t i = e / d;
-        const int l = (int)(e - i * d);
-        const int lab = labels[i] - c0;
-        if (lab < 0 || lab >= kc)
-            continue;
-        const double v = (double)X[e];
+    // Thread t takes feature t % d of samples start + t / d, start + t / d + spp, ... (spp
+    // samples per pass of the 1024-thread block), or, for d > 1024, features t, t + 1024, ...
+    // of every sample: no per-element division.
+    auto add = [&](int64_t i, int l, int lab) {
+        const double v = (double)X[i * d + l];
         if (v == v && v - v == 0.0) {
             const double sv = ldexp(v, escale[l]);
             const double h = rint(sv);
@@ -129,12 +127,31 @@ __global__ __launch_bounds__(256) void kmeans_partial_kernel(
         }
         if (l == 0)
             atomicAdd(&lcnt[lab], 1);
+    };
+    if (d <= 1024) {
+        const int spp = 1024 / d;
+        const int sub = (int)threadIdx.x / d, l = (int)threadIdx.x - sub * d;
+        if (sub < spp) {
+            for (int64_t i = start + sub; i < end; i += spp) {
+                const int lab = labels[i] - c0;
+                if (lab >= 0 && lab < kc)
+                    add(i, l, lab);
+            }
+        }
+    } else {
+        for (int64_t i = start; i < end; i++) {
+            const int lab = labels[i] - c0;
+            if (lab < 0 || lab >= kc)
+                continue;
+            for (int l = threadIdx.x; l < d; l += 1024)
+                add(i, l, lab);
+        }
     }
     __syncthreads();
     long long *pb = part + (int64_t)b * words;
-    for (int t = threadIdx.x; t < words; t += 256)
+    for (int t = threadIdx.x; t < words; t += 1024)
         pb[t] = acc[t];
-    for (int t = threadIdx.x; t < kc; t += 256)
+    for (int t = threadIdx.x; t < kc; t += 1024)
         cnt[(int64_t)b * kc + t] = lcnt[t];
 }
 
@@ -498,7 +515,12 @@ hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *label
                                       int kc, const int *escale, int ppb, int nblocks,
                                       long long *part, int *cnt, hipStream_t s) {
     const size_t lds = (size_t)kc * d * 2 * sizeof(long long) + (size_t)kc * sizeof(int);
-    hipLaunchKernelGGL(kmeans_partial_kernel<T>, dim3(nblocks), dim3(256), lds, s, X, n, d, labels,
+    if (lds > 160 * 1024)
+        return hipErrorInvalidValue;
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)kmeans_partial_kernel<T>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(kmeans_partial_kernel<T>, dim3(nblocks), dim3(1024), lds, s, X, n, d, labels,
                        c0, kc, escale, ppb, part, cnt);
     return hipGetLastError();
 }

@@ -273,21 +273,27 @@ __global__ __launch_bounds__(256) void kmeans_assign_bf16_kernel(
 }
 
 // Exact re-check of listed samples: argmin of double squared distances, ties to the lowest
-// centre. Fixed grid; the number of listed samples is read on the device.
+// centre. One wave per listed sample (fixed grid, the count is read on the device): lane c
+// takes centres c, c + 64, ..., then a wave argmin by (distance, index).
 template <typename T>
 __global__ __launch_bounds__(256) void kmeans_recheck_kernel(
     const T *__restrict__ X, int ldx, int d, const double *__restrict__ C, int k,
     const int *__restrict__ list_count, const int *__restrict__ list, int *__restrict__ labels,
     const int *__restrict__ prev, int *__restrict__ changed) {
     const int cnt = *list_count;
-    for (int t = blockIdx.x * 256 + threadIdx.x; t < cnt; t += gridDim.x * 256) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (int t = wave; t < cnt; t += nwaves) {
         const int i = list[t];
+        const T *x = X + (int64_t)i * ldx;
         double best = __builtin_huge_val();
-        int bl = 0;
-        for (int c = 0; c < k; c++) {
+        int bl = INT_MAX;
+        for (int c = lane; c < k; c += 64) {
+            const double *cc = C + (int64_t)c * d;
             double acc = 0.0;
             for (int l = 0; l < d; l++) {
-                const double v = (double)X[(int64_t)i * ldx + l] - C[(int64_t)c * d + l];
+                const double v = (double)x[l] - cc[l];
                 acc = fma(v, v, acc);
             }
             if (acc < best) {
@@ -295,9 +301,19 @@ __global__ __launch_bounds__(256) void kmeans_recheck_kernel(
                 bl = c;
             }
         }
-        labels[i] = bl;
-        if (prev && prev[i] != bl)
-            atomicAdd(changed, 1);
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ob = __shfl_xor(best, off, 64);
+            const int oi = __shfl_xor(bl, off, 64);
+            if (ob < best || (ob == best && oi < bl)) {
+                best = ob;
+                bl = oi;
+            }
+        }
+        if (lane == 0) {
+            labels[i] = bl;
+            if (prev && prev[i] != bl)
+                atomicAdd(changed, 1);
+        }
     }
 }
 
@@ -453,7 +469,7 @@ template <typename T>
 hipError_t launch_kmeans_recheck(const T *X, int ldx, int d, const double *C, int k,
                                  const int *list_count, const int *list, int *labels,
                                  const int *prev, int *changed, hipStream_t s) {
-    hipLaunchKernelGGL(kmeans_recheck_kernel<T>, dim3(256), dim3(256), 0, s, X, ldx, d, C, k,
+    hipLaunchKernelGGL(kmeans_recheck_kernel<T>, dim3(1024), dim3(256), 0, s, X, ldx, d, C, k,
                        list_count, list, labels, prev, changed);
     return hipGetLastError();
 }

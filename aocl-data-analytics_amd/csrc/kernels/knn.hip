@@ -49,33 +49,69 @@ template <> struct TileShape<32> { static constexpr int CAP = 128, WV = 4, RT = 
 template <> struct TileShape<64> { static constexpr int CAP = 128, WV = 4, RT = 1; };
 template <> struct TileShape<128> { static constexpr int CAP = 256, WV = 2, RT = 1; };
 
-__device__ __forceinline__ void wave_sort64(float &key, int &idx) {
+// Value of lane (lane ^ S) for S in {1, 2, 4, 8, 16, 32}, by register-to-register lane
+// moves (DPP within rows of 16, the gfx950 permlane swaps across rows) instead of the LDS
+// crossbar: quad_perm for 1 and 2; the two row shifts by 4, selected by lane bit 2, for 4;
+// a row rotation by 8 for 8; v_permlane16_swap / v_permlane32_swap for 16 and 32.
+template <int S> __device__ __forceinline__ int lane_xor(int v) {
     const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int size = 2; size <= 64; size <<= 1) {
-#pragma unroll
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            const float ok = __shfl_xor(key, stride, 64);
-            const int oi = __shfl_xor(idx, stride, 64);
-            const bool up = (lane & size) == 0;
-            const bool lower = (lane & stride) == 0;
-            const bool other_less = (ok < key) || (ok == key && oi < idx);
-            const bool take = lower ? (up ? other_less : !other_less) : (up ? !other_less : other_less);
-            if (take) {
-                key = ok;
-                idx = oi;
-            }
-        }
+    if constexpr (S == 1) {
+        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false); // quad_perm [1,0,3,2]
+    } else if constexpr (S == 2) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false); // quad_perm [2,3,0,1]
+    } else if constexpr (S == 4) {
+        const int up = __builtin_amdgcn_update_dpp(0, v, 0x104, 0xF, 0xF, false);   // row_shl:4
+        const int down = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false); // row_shr:4
+        return (lane & 4) ? down : up;
+    } else if constexpr (S == 8) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false); // row_ror:8
+    } else if constexpr (S == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16) ? (int)p[0] : (int)p[1];
+    } else {
+        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane & 32) ? (int)p[0] : (int)p[1];
     }
 }
 
-// Compacts query q's buffer (count c <= CAP) to its KC best entries, sorted by (key,
-// index). CAP = 64: one entry per lane, sorted in registers; larger buffers are sorted in
-// place in LDS by the owning wave.
-template <int KC, int CAP>
-__device__ __forceinline__ void compact_query(float *bkey, int *bidx, int *cnt, float *thr, int q) {
+template <int SIZE, int S>
+__device__ __forceinline__ void sort_stage(float &key, int &idx, int lane) {
+    const float ok = __int_as_float(lane_xor<S>(__float_as_int(key)));
+    const int oi = lane_xor<S>(idx);
+    const bool up = (lane & SIZE) == 0;
+    const bool lower = (lane & S) == 0;
+    const bool other_less = (ok < key) || (ok == key && oi < idx);
+    const bool take = lower ? (up ? other_less : !other_less) : (up ? !other_less : other_less);
+    if (take) {
+        key = ok;
+        idx = oi;
+    }
+}
+
+template <int SIZE, int S> __device__ __forceinline__ void sort_merge(float &key, int &idx, int lane) {
+    sort_stage<SIZE, S>(key, idx, lane);
+    if constexpr (S > 1)
+        sort_merge<SIZE, S / 2>(key, idx, lane);
+}
+
+// Bitonic sort of one (key, index) pair per lane across the wave, ascending by (key, index).
+__device__ __forceinline__ void wave_sort64(float &key, int &idx) {
     const int lane = threadIdx.x & 63;
-    const int c = cnt[q];
+    sort_merge<2, 1>(key, idx, lane);
+    sort_merge<4, 2>(key, idx, lane);
+    sort_merge<8, 4>(key, idx, lane);
+    sort_merge<16, 8>(key, idx, lane);
+    sort_merge<32, 16>(key, idx, lane);
+    sort_merge<64, 32>(key, idx, lane);
+}
+
+// Compacts the buffer of query q (c entries, c <= CAP) to its KC best entries sorted by
+// (key, index) and returns the new threshold: the KC-th key, or +inf while fewer than KC
+// entries have been seen. Wave-uniform; not inlined, so that the sweep's many call sites
+// share one copy.
+template <int KC, int CAP>
+__device__ __attribute__((noinline)) float compact_count(float *bkey, int *bidx, int c, int q) {
+    const int lane = threadIdx.x & 63;
     float *kk = bkey + q * CAP;
     int *ii = bidx + q * CAP;
     if constexpr (CAP == 64) {
@@ -86,11 +122,8 @@ __device__ __forceinline__ void compact_query(float *bkey, int *bidx, int *cnt, 
             kk[lane] = key;
             ii[lane] = idx;
         }
-        const float kth = __shfl(key, KC - 1, 64);
-        if (lane == 0) {
-            cnt[q] = c < KC ? c : KC;
-            thr[q] = c >= KC ? kth : __builtin_huge_valf();
-        }
+        const float kth = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key), KC - 1));
+        return c >= KC ? kth : __builtin_huge_valf();
     } else {
         for (int t = c + lane; t < CAP; t += 64) {
             kk[t] = __builtin_huge_valf();
@@ -118,10 +151,7 @@ __device__ __forceinline__ void compact_query(float *bkey, int *bidx, int *cnt, 
                 __builtin_amdgcn_wave_barrier();
             }
         }
-        if (lane == 0) {
-            cnt[q] = c < KC ? c : KC;
-            thr[q] = c >= KC ? kk[KC - 1] : __builtin_huge_valf();
-        }
+        return c >= KC ? kk[KC - 1] : __builtin_huge_valf();
     }
 }
 
@@ -207,14 +237,117 @@ __device__ __forceinline__ void wave_fence() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Candidate bookkeeping of one wave after a step. Lane (r, g) holds, for row tile rt and
+// accumulator element i, the scores of query rt*16 + row(g, i) against training rows
+// tb + ct*16 + r. Scores above the query's threshold are appended to its LDS buffer at
+// positions from a ballot: the 16 lanes of a lane group serve one query per (rt, i), so a
+// hit's slot is the group's running count (cq, held in registers by the group's lanes)
+// plus the number of hits of the lower lanes of the group. A buffer with more than
+// CAP - STEP entries is compacted to its KC best, which sets the new threshold.
+template <class Op, int KC, int CAP, int ROWT>
+__device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][COLT],
+                                          typename Op::acc_e (&ts)[ROWT][4],
+                                          int (&cq)[ROWT][4], float *bkey, int *bidx,
+                                          int qbase, int tb) {
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    auto row_of = [&](int i, int gg) { return Op::rows_strided ? gg + 4 * i : 4 * gg + i; };
+    const unsigned long long below = (1ull << lane) - 1;
+    const unsigned long long gmask = 0xFFFFull << (16 * g);
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int q = qbase + rt * 16 + row_of(i, g);
+#pragma unroll
+            for (int ct = 0; ct < COLT; ct++) {
+                const bool h = acc[rt][ct][i] > ts[rt][i];
+                const unsigned long long m = __ballot(h);
+                if (m == 0)
+                    continue;
+                const unsigned long long mg = m & gmask;
+                if (h) {
+                    const int pos = cq[rt][i] + __popcll(mg & below);
+                    bkey[q * CAP + pos] = (float)(-2.0 * (double)acc[rt][ct][i]);
+                    bidx[q * CAP + pos] = tb + ct * 16 + r;
+                }
+                cq[rt][i] += __popcll(mg);
+            }
+        }
+    }
+    bool need = false;
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            need |= cq[rt][i] > CAP - STEP;
+    if (__ballot(need) == 0)
+        return;
+    wave_fence();
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            // one representative lane (r = 0) per group
+            unsigned long long m2 = __ballot(cq[rt][i] > CAP - STEP) & 0x0001000100010001ull;
+            while (m2) {
+                const int l0 = __ffsll((long long)m2) - 1;
+                m2 &= m2 - 1;
+                const int gg = l0 >> 4;
+                const int c = __builtin_amdgcn_readlane(cq[rt][i], l0);
+                const float t = compact_count<KC, CAP>(bkey, bidx, c, qbase + rt * 16 + row_of(i, gg));
+                if (g == gg) {
+                    cq[rt][i] = c < KC ? c : KC;
+                    ts[rt][i] = (typename Op::acc_e)(-0.5 * (double)t);
+                }
+            }
+        }
+    }
+    wave_fence();
+}
+
+// Final compaction of a wave's queries and the per-(query, split) output: the KC best keys
+// and indices and the split threshold tau (every key of the split not kept is >= tau).
+template <class Op, int KC, int CAP, int ROWT>
+__device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bkey, int *bidx,
+                                            int qbase, int q0, int nq, int nsplit, int split,
+                                            float *__restrict__ part_key,
+                                            int *__restrict__ part_idx,
+                                            float *__restrict__ part_tau) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            for (int gg = 0; gg < 4; gg++) {
+                const int ql = qbase + rt * 16 + (Op::rows_strided ? gg + 4 * i : 4 * gg + i);
+                const int c = __builtin_amdgcn_readlane(cq[rt][i], 16 * gg);
+                const float t = compact_count<KC, CAP>(bkey, bidx, c, ql);
+                wave_fence();
+                const int q = q0 + ql;
+                if (q >= nq)
+                    continue;
+                const int64_t o = ((int64_t)q * nsplit + split) * KC;
+                const int valid = c < KC ? c : KC;
+                for (int u = lane; u < KC; u += 64) {
+                    part_key[o + u] = u < valid ? bkey[ql * CAP + u] : __builtin_huge_valf();
+                    part_idx[o + u] = u < valid ? bidx[ql * CAP + u] : -1;
+                }
+                if (lane == 0)
+                    part_tau[(int64_t)q * nsplit + split] = t;
+            }
+        }
+    }
+}
+
 // Sweep of one training split for 128 queries. Scores are x.y - |y|^2/2 (the accumulator
 // starts at -|y|^2/2), so a larger score is a nearer row; keys -2*score are what the
 // candidate buffers, thresholds and the merge use. Each wave owns its 32 queries' buffers,
 // so no workgroup barrier is needed inside the sweep. SPLIT operands carry each value as
 // hi + lo in the 16-bit type (query rows stored as [hi Kp][lo Kp]); the product is
 // hi*hi + hi*lo + lo*hi. Training operands are loaded from the tiled layout straight into
-// registers, one step at a time (PF: the next step's loads are issued before the current
-// step's MFMAs, into a second register buffer).
+// registers (PF: the next step's loads are issued right after the current step's MFMAs and
+// overlap its candidate bookkeeping; otherwise each step loads, then computes).
 template <class Op, int KCH, int KC, bool SPLIT, bool PF>
 __device__ __forceinline__ void knn_tile_body(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
@@ -233,8 +366,6 @@ __device__ __forceinline__ void knn_tile_body(
     constexpr int QB = TileShape<KC>::WV * ROWT * 16;
     __shared__ float bkey[QB * CAP];
     __shared__ int bidx[QB * CAP];
-    __shared__ int cnt[QB];
-    __shared__ float thr[QB];
 
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -248,10 +379,7 @@ __device__ __forceinline__ void knn_tile_body(
         t_end = n;
     const int qbase = (w * ROWT) * 16;
     const int64_t rowlen = (int64_t)Kp * NS;
-    if (lane < ROWT * 16) {
-        cnt[qbase + lane] = 0;
-        thr[qbase + lane] = __builtin_huge_valf();
-    }
+    int cq[ROWT][4] = {};
 
     frag_t a[NS][ROWT][NF];
 #pragma unroll
@@ -269,7 +397,6 @@ __device__ __forceinline__ void knn_tile_body(
         }
     }
     ae ts[ROWT][4];
-    auto row_of = [&](int i) { return Op::rows_strided ? g + 4 * i : 4 * g + i; };
 #pragma unroll
     for (int rt = 0; rt < ROWT; rt++)
 #pragma unroll
@@ -283,8 +410,9 @@ __device__ __forceinline__ void knn_tile_body(
     auto load_step = [&](int tb, frag_t (&b)[COLT][NS][NF], ae (&hv)[COLT]) {
 #pragma unroll
         for (int ct = 0; ct < COLT; ct++) {
-            const int j = tb + ct * 16 + r;
-            hv[ct] = j < t_end ? nyh[j] : (ae)(-__builtin_huge_val());
+            // Unconditional: a step never crosses the end of a split except at the end of
+            // the data, where the padded rows have -|y|^2/2 = -inf.
+            hv[ct] = nyh[tb + ct * 16 + r];
             const in_t *tile = Y + (int64_t)(tb + ct * 16) * rowlen + lane * CSR;
 #pragma unroll
             for (int h = 0; h < NS; h++)
@@ -292,8 +420,8 @@ __device__ __forceinline__ void knn_tile_body(
         }
     };
 
-    auto compute_step = [&](int tb, const frag_t (&b)[COLT][NS][NF], const ae (&hv)[COLT]) {
-        acc_t acc[ROWT][COLT];
+    auto mfma_step = [&](const frag_t (&b)[COLT][NS][NF], const ae (&hv)[COLT],
+                         acc_t (&acc)[ROWT][COLT]) {
 #pragma unroll
         for (int ct = 0; ct < COLT; ct++)
 #pragma unroll
@@ -314,6 +442,8 @@ __device__ __forceinline__ void knn_tile_body(
                     }
                 }
         }
+    };
+    auto check_step = [&](int tb, const acc_t (&acc)[ROWT][COLT]) {
         bool hit = false;
 #pragma unroll
         for (int rt = 0; rt < ROWT; rt++)
@@ -324,87 +454,39 @@ __device__ __forceinline__ void knn_tile_body(
                     hit |= acc[rt][ct][i] > ts[rt][i];
         if (__ballot(hit) == 0)
             return;
-#pragma unroll
-        for (int rt = 0; rt < ROWT; rt++) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int q = qbase + rt * 16 + row_of(i);
-#pragma unroll
-                for (int ct = 0; ct < COLT; ct++) {
-                    if (acc[rt][ct][i] > ts[rt][i]) {
-                        const int pos = atomicAdd(&cnt[q], 1);
-                        bkey[q * CAP + pos] = (float)(-2.0 * (double)acc[rt][ct][i]);
-                        bidx[q * CAP + pos] = tb + ct * 16 + r;
-                    }
-                }
-            }
-        }
-        wave_fence();
-        const bool need = (lane < ROWT * 16) && (cnt[qbase + (lane & 31)] > CAP - STEP);
-        unsigned long long mask = __ballot(need);
-        if (mask) {
-            while (mask) {
-                const int bq = __ffsll((long long)mask) - 1;
-                mask &= mask - 1;
-                compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + bq);
-            }
-            wave_fence();
-#pragma unroll
-            for (int rt = 0; rt < ROWT; rt++)
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    ts[rt][i] = (ae)(-0.5 * (double)thr[qbase + rt * 16 + row_of(i)]);
-        }
+        take_hits<Op, KC, CAP, ROWT>(acc, ts, cq, bkey, bidx, qbase, tb);
     };
 
-    if constexpr (!PF) {
+    if constexpr (PF) {
+        // The next step's operands are loaded once this step's MFMAs have consumed the
+        // registers, so the loads run under this step's candidate bookkeeping.
+        if (t_begin < t_end) {
+            frag_t b[COLT][NS][NF];
+            ae hv[COLT];
+            load_step(t_begin, b, hv);
+            for (int tb = t_begin; tb < t_end; tb += STEP) {
+                acc_t acc[ROWT][COLT];
+                mfma_step(b, hv, acc);
+                if (tb + STEP < t_end)
+                    load_step(tb + STEP, b, hv);
+                __builtin_amdgcn_sched_barrier(0);
+                check_step(tb, acc);
+            }
+        }
+    } else {
         for (int tb = t_begin; tb < t_end; tb += STEP) {
-            frag_t b0[COLT][NS][NF];
-            ae h0[COLT];
-            load_step(tb, b0, h0);
+            frag_t b[COLT][NS][NF];
+            ae hv[COLT];
+            load_step(tb, b, hv);
             // Keep every load of the step ahead of its first MFMA.
             __builtin_amdgcn_sched_barrier(0);
-            compute_step(tb, b0, h0);
-        }
-    } else if (t_begin < t_end) {
-        // Two register buffers: the loads of step i+1 are issued before the MFMAs of step i.
-        frag_t b0[COLT][NS][NF], b1[COLT][NS][NF];
-        ae h0[COLT], h1[COLT];
-        load_step(t_begin, b0, h0);
-        for (int tb = t_begin;; tb += 2 * STEP) {
-            const bool more1 = tb + STEP < t_end;
-            if (more1)
-                load_step(tb + STEP, b1, h1);
-            __builtin_amdgcn_sched_barrier(0);
-            compute_step(tb, b0, h0);
-            if (!more1)
-                break;
-            const bool more2 = tb + 2 * STEP < t_end;
-            if (more2)
-                load_step(tb + 2 * STEP, b0, h0);
-            __builtin_amdgcn_sched_barrier(0);
-            compute_step(tb + STEP, b1, h1);
-            if (!more2)
-                break;
+            acc_t acc[ROWT][COLT];
+            mfma_step(b, hv, acc);
+            check_step(tb, acc);
         }
     }
-    for (int b = 0; b < ROWT * 16; b++)
-        compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + b);
-    wave_fence();
-    for (int b = 0; b < ROWT * 16; b++) {
-        const int q = q0 + qbase + b;
-        if (q >= nq)
-            continue;
-        const int ql = qbase + b;
-        const int64_t o = ((int64_t)q * nsplit + split) * KC;
-        for (int t = lane; t < KC; t += 64) {
-            const bool valid = t < cnt[ql];
-            part_key[o + t] = valid ? bkey[ql * CAP + t] : __builtin_huge_valf();
-            part_idx[o + t] = valid ? bidx[ql * CAP + t] : -1;
-        }
-        if (lane == 0)
-            part_tau[(int64_t)q * nsplit + split] = thr[ql];
-    }
+    write_split<Op, KC, CAP, ROWT>(cq, bkey, bidx, qbase, q0, nq, nsplit, split, part_key,
+                                   part_idx, part_tau);
 }
 
 // The sweep as a kernel: K2 asks the compiler for at most 256 registers per lane, two waves
@@ -426,254 +508,6 @@ __attribute__((amdgpu_waves_per_eu(2))) void knn_tile_kernel_k2(
     float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
     knn_tile_body<Op, KCH, KC, SPLIT, PF>(Q, nq, Y, nyh, n, Kp, split_len, nsplit, part_key,
                                           part_idx, part_tau);
-}
-
-// Wave-uniform wait for this wave's outstanding vector-memory operations to drop to N and
-// for its LDS operations to finish, then a workgroup barrier. Inline assembly keeps the
-// compiler from moving LDS accesses across the barrier.
-template <int N> __device__ __forceinline__ void wait_vm_barrier() {
-    asm volatile("s_waitcnt vmcnt(%0) lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
-}
-
-__device__ __forceinline__ void wait_vm_all() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-
-// Direct global-to-LDS copy of 16 bytes per lane (4 bytes per lane for the dword form): lane
-// l's bytes land at LDS offset lds + 16 l (4 l). lds must be wave-uniform. Issued as inline
-// assembly so that the compiler, which does not see the LDS write, adds no waits of its own
-// for it: every read of the destination follows a wait_vm_barrier that covers the copy.
-__device__ __forceinline__ void lds_copy16(const void *src, uint32_t lds) {
-    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off"
-                 ::"s"(lds), "v"(src) : "memory", "m0");
-}
-__device__ __forceinline__ void lds_copy4(const void *src, uint32_t lds) {
-    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off"
-                 ::"s"(lds), "v"(src) : "memory", "m0");
-}
-template <typename P> __device__ __forceinline__ uint32_t lds_offset(P *p) {
-    return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) char *)(p);
-}
-
-// LDS-staged sweep. The same work as knn_tile_kernel, but the training rows stream through
-// an NSTAGE-deep ring of LDS stages (32 rows each) filled by direct global-to-LDS loads
-// issued NSTAGE-1 steps ahead and shared by all waves of the workgroup. Each wave copies
-// CPW 1 KB chunks of every stage (one 16-byte load per lane and chunk: the tiled layout
-// makes a chunk contiguous in memory and in LDS) and its share of the stage's -|y|^2/2
-// values. A step starts with a wait for this wave's copies of the stage and a barrier; the
-// slot refilled after that barrier is the one every wave finished reading in the previous
-// step.
-template <class Op, int KCH, int KC, bool SPLIT, int NSTAGE>
-__global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_staged_kernel(
-    const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
-    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
-    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
-    using in_t = typename Op::in_t;
-    using frag_t = typename Op::frag_t;
-    using acc_t = typename Op::acc_t;
-    using ae = typename Op::acc_e;
-    constexpr int WV = TileShape<KC>::WV;
-    constexpr int NF = KCH / Op::E;
-    constexpr int NS = SPLIT ? 2 : 1;
-    constexpr int KPC = 4 * KCH;                                 // padded features
-    constexpr int RB = KPC * (int)sizeof(in_t) * NS;             // bytes per row
-    constexpr int SBY = STEP * RB;                               // bytes per stage
-    constexpr int CPW = SBY / 1024 / WV;                         // 1 KB chunks per wave
-    constexpr int NYL = STEP * (int)sizeof(ae) / 4 / WV;         // nyh dwords per wave
-    constexpr int CSR = 16 / (int)sizeof(in_t);
-    constexpr int CST = 1024 / (int)sizeof(in_t);
-    constexpr int CAP = TileShape<KC>::CAP;
-    constexpr int ROWT = TileShape<KC>::RT;
-    constexpr int QB = WV * ROWT * 16;
-    constexpr int VMW = (NSTAGE - 2) * (CPW + 1);
-    static_assert(CPW >= 1 && CPW * WV * 1024 == SBY, "stage must split into whole chunks");
-    static_assert(NYL >= 1 && NYL <= 64, "nyh share");
-    static_assert((NSTAGE - 1) * (CPW + 1) <= 63, "vmcnt range");
-    __shared__ float bkey[QB * CAP];
-    __shared__ int bidx[QB * CAP];
-    __shared__ int cnt[QB];
-    __shared__ float thr[QB];
-    __shared__ __attribute__((aligned(1024))) char ring[NSTAGE * SBY];
-    __shared__ __attribute__((aligned(16))) ae nh[NSTAGE * STEP];
-    (void)Kp;
-
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int r = lane & 15, g = lane >> 4;
-    const int qblock = blockIdx.x / nsplit;
-    const int split = blockIdx.x - qblock * nsplit;
-    const int q0 = qblock * QB;
-    const int t_begin = split * split_len;
-    int t_end = t_begin + split_len;
-    if (t_end > n)
-        t_end = n;
-    const int nsteps = t_end > t_begin ? (t_end - t_begin + STEP - 1) / STEP : 0;
-    const int qbase = (w * ROWT) * 16;
-    const int64_t rowlen = (int64_t)KPC * NS;
-    if (lane < ROWT * 16) {
-        cnt[qbase + lane] = 0;
-        thr[qbase + lane] = __builtin_huge_valf();
-    }
-
-    frag_t a[NS][ROWT][NF];
-#pragma unroll
-    for (int rt = 0; rt < ROWT; rt++) {
-        const int qr = q0 + qbase + rt * 16 + r;
-#pragma unroll
-        for (int h = 0; h < NS; h++) {
-            Op::template load<NF, CSR>(Q + (int64_t)(qr < nq ? qr : 0) * rowlen + h * KPC + g * KCH,
-                                       a[h][rt]);
-            if (qr >= nq) {
-#pragma unroll
-                for (int f = 0; f < NF; f++)
-                    a[h][rt][f] = frag_t{};
-            }
-        }
-    }
-    // Consume the query operands here, so that the compiler's waits for these loads come
-    // before the sweep and not inside it, where they would also wait for the stage copies.
-#pragma unroll
-    for (int h = 0; h < NS; h++)
-#pragma unroll
-        for (int rt = 0; rt < ROWT; rt++)
-#pragma unroll
-            for (int f = 0; f < NF; f++)
-                asm volatile("" ::"v"(a[h][rt][f]));
-    ae ts[ROWT][4];
-    auto row_of = [&](int i) { return Op::rows_strided ? g + 4 * i : 4 * g + i; };
-#pragma unroll
-    for (int rt = 0; rt < ROWT; rt++)
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            ts[rt][i] = -__builtin_huge_val();
-    wave_fence();
-    const int t_last = t_begin + (nsteps > 0 ? nsteps - 1 : 0) * STEP;
-    const uint32_t ring_lds = lds_offset(ring);
-    const uint32_t nh_lds = lds_offset(nh);
-
-    // Copies stage s of the split (rows t_begin + 32 s ..) into ring slot s % NSTAGE.
-    // Stages past the end re-read the last stage, so every wave issues the same number of
-    // copies per step.
-    auto issue = [&](int s) {
-        int tb = t_begin + s * STEP;
-        if (tb > t_last)
-            tb = t_last;
-        const int slot = s % NSTAGE;
-        const char *src = reinterpret_cast<const char *>(Y) + (int64_t)tb * RB + lane * 16;
-#pragma unroll
-        for (int c = 0; c < CPW; c++) {
-            const int chunk = w * CPW + c;
-            lds_copy16(src + chunk * 1024,
-                       __builtin_amdgcn_readfirstlane(ring_lds + slot * SBY + chunk * 1024));
-        }
-        const uint32_t ndst =
-            __builtin_amdgcn_readfirstlane(nh_lds + slot * STEP * (int)sizeof(ae) + w * NYL * 4);
-        const char *nsrc = reinterpret_cast<const char *>(nyh + tb) + (w * NYL + (lane < NYL ? lane : 0)) * 4;
-        if (lane < NYL)
-            lds_copy4(nsrc, ndst);
-    };
-
-    auto compute = [&](int tb, int slot) {
-        const char *stage = ring + slot * SBY;
-        const ae *nhs = nh + slot * STEP;
-        acc_t acc[ROWT][COLT];
-#pragma unroll
-        for (int ct = 0; ct < COLT; ct++) {
-            const ae hv = nhs[ct * 16 + r];
-#pragma unroll
-            for (int rt = 0; rt < ROWT; rt++)
-                acc[rt][ct] = acc_t{hv, hv, hv, hv};
-        }
-        frag_t b[COLT][NS][NF];
-#pragma unroll
-        for (int ct = 0; ct < COLT; ct++)
-#pragma unroll
-            for (int h = 0; h < NS; h++)
-                Op::template load<NF, CST>(
-                    reinterpret_cast<const in_t *>(stage + ct * 16 * RB) + h * 16 * KPC + lane * CSR,
-                    b[ct][h]);
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-#pragma unroll
-            for (int ct = 0; ct < COLT; ct++)
-#pragma unroll
-                for (int rt = 0; rt < ROWT; rt++) {
-                    acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][0][f], acc[rt][ct]);
-                    if constexpr (SPLIT) {
-                        acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][NS - 1][f], acc[rt][ct]);
-                        acc[rt][ct] = Op::mfma(a[NS - 1][rt][f], b[ct][0][f], acc[rt][ct]);
-                    }
-                }
-        }
-        bool hit = false;
-#pragma unroll
-        for (int rt = 0; rt < ROWT; rt++)
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int ct = 0; ct < COLT; ct++)
-                    hit |= acc[rt][ct][i] > ts[rt][i];
-        if (__ballot(hit) == 0)
-            return;
-#pragma unroll
-        for (int rt = 0; rt < ROWT; rt++) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int q = qbase + rt * 16 + row_of(i);
-#pragma unroll
-                for (int ct = 0; ct < COLT; ct++) {
-                    if (acc[rt][ct][i] > ts[rt][i]) {
-                        const int pos = atomicAdd(&cnt[q], 1);
-                        bkey[q * CAP + pos] = (float)(-2.0 * (double)acc[rt][ct][i]);
-                        bidx[q * CAP + pos] = tb + ct * 16 + r;
-                    }
-                }
-            }
-        }
-        wave_fence();
-        const bool need = (lane < ROWT * 16) && (cnt[qbase + (lane & 31)] > CAP - STEP);
-        unsigned long long mask = __ballot(need);
-        if (mask) {
-            while (mask) {
-                const int bq = __ffsll((long long)mask) - 1;
-                mask &= mask - 1;
-                compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + bq);
-            }
-            wave_fence();
-#pragma unroll
-            for (int rt = 0; rt < ROWT; rt++)
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    ts[rt][i] = (ae)(-0.5 * (double)thr[qbase + rt * 16 + row_of(i)]);
-        }
-    };
-
-    if (nsteps > 0) {
-        for (int s = 0; s < NSTAGE - 1; s++)
-            issue(s);
-        for (int i = 0; i < nsteps; i++) {
-            wait_vm_barrier<VMW>();
-            issue(i + NSTAGE - 1);
-            compute(t_begin + i * STEP, i % NSTAGE);
-        }
-        // Copies of stages past the end may still be in flight into LDS.
-        wait_vm_all();
-    }
-    for (int b = 0; b < ROWT * 16; b++)
-        compact_query<KC, CAP>(bkey, bidx, cnt, thr, qbase + b);
-    wave_fence();
-    for (int b = 0; b < ROWT * 16; b++) {
-        const int q = q0 + qbase + b;
-        if (q >= nq)
-            continue;
-        const int ql = qbase + b;
-        const int64_t o = ((int64_t)q * nsplit + split) * KC;
-        for (int t = lane; t < KC; t += 64) {
-            const bool valid = t < cnt[ql];
-            part_key[o + t] = valid ? bkey[ql * CAP + t] : __builtin_huge_valf();
-            part_idx[o + t] = valid ? bidx[ql * CAP + t] : -1;
-        }
-        if (lane == 0)
-            part_tau[(int64_t)q * nsplit + split] = thr[ql];
-    }
 }
 
 constexpr int MMAX = 2048; // candidates merged per query (nsplit * kc)
@@ -876,40 +710,26 @@ int knn_tile_qb(int kc) {
 int knn_merge_max_candidates() { return MMAX; }
 
 
-// Geometry of the LDS-staged kernel: stage bytes, and whether a stage splits into whole
-// 1 KB chunks per wave.
-template <class Op, int KCH, int KC, bool SPLIT> struct StagedGeom {
-    static constexpr int RB = 4 * KCH * (int)sizeof(typename Op::in_t) * (SPLIT ? 2 : 1);
-    static constexpr int SBY = STEP * RB;
-    static constexpr bool ok = SBY % (1024 * TileShape<KC>::WV) == 0;
-    static constexpr int NSTAGE = SBY * 4 <= 65536 ? 4 : SBY * 3 <= 65536 ? 3 : 2;
-};
-
 // Launches the fused tile for one operand type, candidate-list length, width and variant
-// (0: register operands, 1: LDS-staged where the geometry allows, else 0; 2: register
-// operands with the next step's loads in flight) or, with `slots`
-// set, reports how many of its workgroups the device runs at once instead.
+// (0: loads of the next step under the current step's bookkeeping, 1: load then compute
+// step by step) or, with `slots` set, reports how many of its workgroups the device runs at
+// once instead.
 template <class Op, int KCH, int KC, bool SPLIT>
 static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                 int Kp, int nsplit, float *part_key, int *part_idx,
                                 float *part_tau, int variant, int *slots, hipStream_t s) {
     using in_t = typename Op::in_t;
     using ae = typename Op::acc_e;
-    using G = StagedGeom<Op, KCH, KC, SPLIT>;
     using kernel_t = void (*)(const in_t *, int, const in_t *, const ae *, int, int, int, int,
                               float *, int *, float *);
     constexpr int QB = TileShape<KC>::WV * TileShape<KC>::RT * 16;
     kernel_t kern;
     if constexpr (std::is_same_v<Op, OpF64>)
-        kern = variant == 2 ? knn_tile_kernel<Op, KCH, KC, SPLIT, true>
-                            : knn_tile_kernel<Op, KCH, KC, SPLIT, false>;
+        kern = variant == 1 ? knn_tile_kernel<Op, KCH, KC, SPLIT, false>
+                            : knn_tile_kernel<Op, KCH, KC, SPLIT, true>;
     else
-        kern = variant == 2 ? knn_tile_kernel_k2<Op, KCH, KC, SPLIT, true>
-                            : knn_tile_kernel_k2<Op, KCH, KC, SPLIT, false>;
-    if constexpr (G::ok) {
-        if (variant == 1)
-            kern = knn_tile_staged_kernel<Op, KCH, KC, SPLIT, G::NSTAGE>;
-    }
+        kern = variant == 1 ? knn_tile_kernel_k2<Op, KCH, KC, SPLIT, false>
+                            : knn_tile_kernel_k2<Op, KCH, KC, SPLIT, true>;
     const dim3 block(64 * TileShape<KC>::WV);
     if (slots) {
         int dev = 0, cus = 0, per_cu = 0;

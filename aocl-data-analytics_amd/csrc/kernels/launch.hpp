@@ -180,10 +180,8 @@ int knn_tile_kp(int d);
 // layout of launch_knn_tile_operand, padded to a multiple of 32 rows; nyh: -|y|^2/2 per row
 // (double for KP_F64, float otherwise).
 // Writes part_key/part_idx[(q*nsplit + s)*kc + i] (keys |y|^2 - 2 q.y as float) and
-// part_tau[q*nsplit + s]. kc in {16, 32, 64, 128}. variant 0: training operands loaded into
-// registers step by step; 1: streamed through a ring of LDS stages by direct global-to-LDS
-// loads (used where a 32-row stage splits into whole 1 KB chunks per wave, else variant 0);
-// 2: as 0 with the next step's loads issued before the current step's MFMAs.
+// part_tau[q*nsplit + s]. kc in {16, 32, 64, 128}. variant 0: the next step's loads are
+// issued under the current step's candidate bookkeeping; 1: each step loads, then computes.
 // nyh must hold a multiple of 32 entries (the padded rows at -inf).
 hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                            int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
@@ -237,4 +235,62 @@ namespace dagpu {
 // dst[idx[i]*ld + c] = src[i*ld + c] (scatter), c < cols, i < m. elem: 4 or 8 bytes.
 hipError_t launch_gather_scatter(const void *src, void *dst, const int *idx, int m, int cols,
                                  int elem, int scatter, hipStream_t s);
+// ---- Batched k-means (kmeans_batch.hip) ----
+// Instance states.
+enum { KB_ACTIVE = 0, KB_CONV_LABELS = 1, KB_CONV_SHIFT = 2, KB_MAXIT = 3 };
+// Per-instance device arrays: sizes, padded feature count (32 or 64) and element offsets of
+// the instance's rows (X), bf16 rows (Xb), centres (k x d), bf16 centres (kpad x kp) and
+// labels.
+struct KBatchDesc {
+    const int *n, *d, *k, *kp;
+    const int64_t *xoff, *xboff, *coff, *cboff, *loff;
+};
+// An assignment tile or re-check entry (instance, sample) and a sample chunk.
+struct KTile {
+    int inst, start;
+};
+struct KChunk {
+    int inst, start, count, pad;
+};
+// Device work buffers of a batch.
+struct KBatchBuffers {
+    const float *X;
+    uint16_t *Xb;                     // bf16 rows
+    double *C0, *C1;                  // centres by parity
+    int *L0, *L1;                     // labels by parity
+    uint16_t *Cb;                     // bf16 centres
+    float *chalf;                     // -|c|^2/2, 64 per instance
+    double *cmax2;                    // max |c|^2 per instance
+    const KTile *tiles;               // assignment tiles
+    int ntiles;
+    const KChunk *chunks;             // sample chunks
+    int nchunks;
+    const int64_t *poff;              // partial-sum offset of each chunk
+    const int *cbeg, *cend;           // chunk range of each instance
+    int *list_count, *changed;
+    KTile *list;                      // re-check list (instance, sample)
+    int *escale;                      // 65 per instance
+    long long *part;
+    int *pcnt;                        // 64 per chunk
+    int *state, *fpar, *n_iter;
+    double *cin;                      // inertia per chunk
+};
+hipError_t kb_launch_prepare(const float *X, const float *Cinit, const KBatchDesc &b, int n_inst,
+                             const KChunk *chunks, int nchunks, int64_t ctotal,
+                             unsigned long long *amax, int *escale, uint16_t *Xb, double *C0,
+                             int *state, int *fpar, int *n_iter, hipStream_t s);
+// One assignment pass (mode 0: iteration with centre parity par; mode 1: final relabel) with
+// pt 16-sample tiles per wave on `grid` persistent workgroups, plus the exact re-check.
+hipError_t kb_launch_assign(const KBatchBuffers &w, const KBatchDesc &b, int n_inst, int mode,
+                            int par, int pt, int grid, hipStream_t s);
+// Centroid update, shift and convergence test of iteration `iter` (parity par).
+hipError_t kb_launch_update(const KBatchBuffers &w, const KBatchDesc &b, int n_inst, int par,
+                            int iter, double tol, size_t lds, hipStream_t s);
+// Final relabel, inertia and outputs.
+hipError_t kb_launch_finish(const KBatchBuffers &w, const KBatchDesc &b, int n_inst, int pt,
+                            int grid, float *Cout, int *labels, double *inertia, hipStream_t s);
+int kb_assign_blocks_per_cu(int pt);
+// Allows the partial-sum kernel `lds` bytes of dynamic LDS (up to 160 KB).
+hipError_t kb_set_partial_lds(size_t lds);
+
 } // namespace dagpu

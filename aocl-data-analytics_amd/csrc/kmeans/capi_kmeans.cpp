@@ -1,6 +1,8 @@
 // C entry points of the k-means chapter (aoclda_kmeans.h), wrapped as in
 // source/core/clustering/kmeans/kmeans_public.cpp of the reference: null handle, log
 // clearing, precision check, handle-type check.
+#include <algorithm>
+#include <climits>
 #include "aoclda_kmeans.h"
 #include "kmeans.hpp"
 
@@ -76,6 +78,20 @@ da_status da_kmeans_predict_s(da_handle handle, da_int k_samples, da_int k_featu
                               const float *Y, da_int ldy, da_int *Y_labels) {
     DA_KM_PROLOGUE(float)
     return km->predict(k_samples, k_features, Y, ldy, Y_labels);
+}
+
+da_status da_kmeans_last_compute_stats(da_handle handle, da_int *n_rechecked) {
+    if (!handle || !n_rechecked)
+        return da_status_invalid_pointer;
+    if (auto *a = dynamic_cast<KMeansHandle<float> *>(handle->alg.get())) {
+        *n_rechecked = (da_int)std::min<long long>(a->rechecked_, INT32_MAX);
+        return da_status_success;
+    }
+    if (auto *a = dynamic_cast<KMeansHandle<double> *>(handle->alg.get())) {
+        *n_rechecked = (da_int)std::min<long long>(a->rechecked_, INT32_MAX);
+        return da_status_success;
+    }
+    return da_status_invalid_handle_type;
 }
 
 } // extern "C"

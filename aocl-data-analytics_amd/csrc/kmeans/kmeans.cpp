@@ -319,6 +319,9 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
     DA_HIP(err, hipMemcpyAsync(h, small_.ptr(), 5 * sizeof(double), hipMemcpyDeviceToHost, st));
     DA_HIP(err, hipStreamSynchronize(st));
     std::memcpy(&changed, &h[4], sizeof(int));
+    int listed = 0;
+    std::memcpy(&listed, reinterpret_cast<const char *>(&h[4]) + sizeof(int), sizeof(int));
+    rechecked_ += listed;
     normc2 = h[0];
     shift2 = h[1];
     return da_status_success;
@@ -603,6 +606,7 @@ template <typename T> da_status KMeansHandle<T>::initialise_centres(da_int metho
 }
 
 template <typename T> da_status KMeansHandle<T>::compute() {
+    rechecked_ = 0;
     if (!initdone_)
         return err->error(da_status_no_data, "No data has been passed to the handle. Please call "
                                              "da_kmeans_set_data_s or da_kmeans_set_data_d.");
@@ -668,7 +672,9 @@ template <typename T> da_status KMeansHandle<T>::compute() {
     DA_HIP(err, idx_.reserve((size_t)std::max(k, 64) * sizeof(int)));
     DA_HIP(err, w_.reserve((size_t)n * sizeof(double)));
     DA_HIP(err, cp_.reserve((size_t)std::max(n, 256 * 64) * sizeof(double)));
-    kc_ = std::max<da_int>(1, std::min<da_int>(k, (65536 - 64) / (16 * d + 4)));
+    // Clusters per partial-sum pass: as many as fit the 160 KB of LDS of a CU (all of them
+    // for k = 256 at d = 32).
+    kc_ = std::max<da_int>(1, std::min<da_int>(k, (160 * 1024 - 64) / (16 * d + 4)));
     nblocks_ = std::min<da_int>(512, (n + 255) / 256);
     ppb_ = (n + nblocks_ - 1) / nblocks_;
     nblocks_ = (n + ppb_ - 1) / ppb_;

@@ -72,6 +72,12 @@ template <typename T> class KMeansHandle : public AlgHandle {
     DeviceBuffer xb_, c16_;
     int kp16_ = 0;
     bool bf16_ok_ = false, xb_ready_ = false;
+
+  public:
+    // Samples re-checked in double by the MFMA assignments since the last compute started.
+    long long rechecked_ = 0;
+
+  private:
     da_int empty_mode_ = 0;
     double *Ccur_ = nullptr, *Cprev_ = nullptr;
     int *lab_cur_ = nullptr, *lab_prev_ = nullptr;

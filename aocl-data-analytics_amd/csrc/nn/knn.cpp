@@ -65,15 +65,13 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
                      {"bf16x3", 5}},
                     "working");
     // MI355X extension: kernel variant of the fused tile (-1 = chosen by the tuning table,
-    // 0 = training operands loaded into registers, 1 = streamed through LDS stages,
-    // 2 = registers with the next step's loads in flight).
+    // 0 = next step's loads under the current step's bookkeeping, 1 = load then compute).
     opts.add_int("tile variant", "Kernel variant of the fused kNN distance tile (-1 automatic).",
-                 -1, LowerBound::greater_equal, 2, UpperBound::less_equal, -1);
+                 -1, LowerBound::greater_equal, 1, UpperBound::less_equal, -1);
 }
 
-// Tuning table of the fused tile variant per operand precision and padded width. At 128
-// features the register sweep (two workgroups per CU) measured faster than the LDS-staged
-// one (one workgroup per CU, a barrier per step) for fp32 and fp16x3 operands.
+// Tuning table of the fused tile variant per operand precision and padded width: the
+// pipelined sweep (variant 0) for every form.
 static int knn_tile_auto_variant(int prec, int kp) {
     (void)prec;
     (void)kp;

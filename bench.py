@@ -251,6 +251,7 @@ def kmeans_bench(args, L, aoclda):
             "fit_s": el, "n_iter": int(km.n_iter),
             "lloyd_step_ms": step_ms,
             "kernels_ms": {nm: {"total_ms": v[0], "launches": v[1]} for nm, v in kt.items()},
+            "rechecked_per_iteration": km.n_rechecked / (it + 1),
             "label_agreement": float(np.mean(km.labels == ref_lab)),
             "inertia_rel_err": abs(float(km.inertia) - ref_inertia) / ref_inertia,
         }
@@ -278,6 +279,80 @@ def kmeans_bench(args, L, aoclda):
         "fp64_reference_fit_s": ref_t, "forms": forms, "roofline": roof}))
 
 
+BATCH_METRIC = ("batched uneven k-means: Lloyd iterations per second over 1,024 instances in "
+                "one persistent launch per phase, graph-captured, 1 GPU")
+
+
+def kmeans_batch_bench(args, L, aoclda):
+    """1,024 independent k-means instances: n log-uniform in [1e3, 2e5], d in {8,16,32,64},
+    k uniform in [4, 64], Gaussian blobs (means U[-10,10]^d, sigma 1) from numpy
+    default_rng(5); initial centres are the first k rows; 15 iterations at tolerance 0.
+    The first call tunes the tile shape and captures the graph; the timed calls replay it."""
+    rng = np.random.default_rng(5)
+    ni = args.kb_instances
+    n = np.floor(np.exp(rng.uniform(np.log(1e3), np.log(2e5), ni))).astype(np.int32)
+    d = rng.choice(np.array([8, 16, 32, 64], dtype=np.int32), ni)
+    k = rng.integers(4, 65, ni).astype(np.int32)
+    xs, cs = [], []
+    for i in range(ni):
+        means = rng.uniform(-10, 10, (int(k[i]), int(d[i]))).astype(np.float32)
+        lab = rng.integers(0, k[i], n[i])
+        Xi = means[lab] + rng.standard_normal((int(n[i]), int(d[i])), dtype=np.float32)
+        xs.append(Xi.ravel())
+        cs.append(Xi[:k[i]].ravel().copy())
+    X = np.concatenate(xs)
+    C0 = np.concatenate(cs)
+    del xs
+    log(f"kmeans batch: {ni} instances, {int(n.sum())} samples, {X.nbytes / 2**30:.2f} GiB")
+    dx, dc, dl = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    L.check(L.lib.da_amd_malloc(X.nbytes, C.byref(dx)))
+    L.check(L.lib.da_amd_malloc(C0.nbytes, C.byref(dc)))
+    L.check(L.lib.da_amd_malloc(int(n.sum()) * 4, C.byref(dl)))
+    L.check(L.lib.da_amd_memcpy_htod(dx, L.ptr(X), X.nbytes))
+    it = np.zeros(ni, dtype=np.int32)
+    inert = np.zeros(ni, dtype=np.float32)
+
+    def run():
+        L.check(L.lib.da_amd_memcpy_htod(dc, L.ptr(C0), C0.nbytes))
+        L.check(L.lib.da_amd_device_synchronize())
+        t0 = time.perf_counter()
+        L.check(L.lib.da_kmeans_batch_device_s(ni, L.ptr(n), L.ptr(d), L.ptr(k), dx, dc, dl,
+                                               args.kb_iter, 0.0, L.ptr(it), L.ptr(inert)))
+        return time.perf_counter() - t0
+
+    first = run()
+    times = [run() for _ in range(args.kb_reps)]
+    tile = L.da_int()
+    L.check(L.lib.da_kmeans_batch_tile_shape(C.byref(tile)))
+    L.check(L.lib.da_kmeans_batch_release())
+    for p in (dx, dc, dl):
+        L.lib.da_amd_free(p)
+    t = float(np.median(times))
+    iters = int(it.sum())
+    kp = np.where(d <= 32, 32, 64)
+    # Algorithmic bytes of one iteration: bf16 rows read by the assignment, fp32 rows read by
+    # the centroid sums, labels written once and read twice.
+    byts = float((n * kp * 2).sum() + (n * d * 4).sum() + 12 * n.sum())
+    flops = float((2 * n.astype(np.int64) * k * d).sum())
+    mean_it = iters / ni
+    print(json.dumps({
+        "metric": BATCH_METRIC, "value": iters / t, "unit": "instance-iterations/s",
+        "n_gpus": 1, "higher_is_better": True, "ms_per_batch": t * 1e3,
+        "first_call_ms": first * 1e3, "reps": args.kb_reps, "dtype": "bf16",
+        "data": "synthetic: Gaussian blobs per instance, numpy default_rng(5)",
+        "config": {"workload": f"{ni} k-means instances, n log-uniform [1e3,2e5], d in "
+                               f"{{8,16,32,64}}, k in [4,64], {args.kb_iter} Lloyd iterations, "
+                               "tol 0", "total_samples": int(n.sum())},
+        "samples_per_tile": tile.value, "mean_iterations": mean_it,
+        "per_iteration_ms": t * 1e3 / max(1.0, mean_it),
+        "roofline": {"kernel": "whole batch iteration", "bound": "hbm",
+                     "achieved": byts / (t / max(1.0, mean_it)) / 1e9, "peak": 8000.0,
+                     "unit": "GB/s", "frac": byts / (t / max(1.0, mean_it)) / 8e12,
+                     "traffic": None, "algorithmic_bytes_per_iteration": byts,
+                     "flops_per_iteration": flops},
+        "inertia_sum": float(inert.astype(np.float64).sum())}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +375,11 @@ def main():
     ap.add_argument("--km-n", type=int, default=2_000_000)
     ap.add_argument("--km-iter", type=int, default=25)
     ap.add_argument("--km-forms", default="working,bf16")
+    ap.add_argument("--kmeans-batch", action="store_true",
+                    help="run the batched uneven k-means benchmark instead")
+    ap.add_argument("--kb-instances", type=int, default=1024)
+    ap.add_argument("--kb-iter", type=int, default=15)
+    ap.add_argument("--kb-reps", type=int, default=3)
     ap.add_argument("--serving", action="store_true",
                     help="run the continuous-batching serving benchmark instead")
     ap.add_argument("--serve-rows", type=int, default=2_000_000)
@@ -323,6 +403,9 @@ def main():
         return
     if args.kmeans:
         kmeans_bench(args, L, aoclda)
+        return
+    if args.kmeans_batch:
+        kmeans_batch_bench(args, L, aoclda)
         return
 
     if args.copy_calibration_mib > 0:

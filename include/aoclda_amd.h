@@ -62,6 +62,14 @@ da_status da_amd_profile_names(da_handle handle, char *names, da_int *lnames);
 da_status da_nn_last_query_stats(da_handle handle, da_int *n_fallback, da_int *n_candidates);
 
 /*
+ * Diagnostics of the last da_kmeans_compute_? of a k-means handle: number of samples whose
+ * two nearest centres were within the error bound of the MFMA cross term and were therefore
+ * re-assigned with exact double distances (summed over the iterations). No counterpart in
+ * the reference.
+ */
+da_status da_kmeans_last_compute_stats(da_handle handle, da_int *n_rechecked);
+
+/*
  * k-NN serving loop over the resident training set of a kNN handle (da_nn_set_data_? done;
  * the handle must outlive the server and not be modified while it runs). Requests are
  * queued by da_nn_server_submit_? (the queries are copied, the call does not block) and
@@ -90,6 +98,30 @@ da_status da_nn_server_wait_d(da_nn_server server, da_int request_id, da_int *n_
  * the array size on entry (>= 8) and the number written on exit. */
 da_status da_nn_server_stats(da_nn_server server, double *stats, da_int *n_stats);
 void da_nn_server_destroy(da_nn_server *server);
+
+/*
+ * Batched Lloyd k-means over many independent instances whose data are resident in device
+ * memory. No counterpart in the reference (its k-means handles one instance per handle,
+ * kmeans.cpp:351-783); each instance runs the reference's Lloyd iteration with supplied
+ * initial centres (convergence test, final relabel and inertia of kmeans.cpp:996-1124;
+ * empty clusters go to the origin as with "empty clusters" = "ignore").
+ *
+ * Instance i has n[i] samples of d[i] <= 64 features and k[i] <= min(64, n[i]) clusters.
+ * Device arrays, instances packed one after another: X holds the rows (row-major, d[i]
+ * columns), C holds the initial centres (k[i] x d[i], row-major) on entry and the final
+ * centres on return, labels receives n[i] labels per instance. Host arrays n_iter and
+ * inertia receive the iteration count and inertia of each instance. The cross term runs in
+ * bf16 on MFMA with an exact double re-check of near ties, so the labels are exact.
+ * The first call for a batch layout tunes the tile shape and captures the run in a HIP graph;
+ * later calls with the same sizes, pointers, max_iter and tol replay it.
+ */
+da_status da_kmeans_batch_device_s(da_int n_inst, const da_int *n, const da_int *d,
+                                   const da_int *k, const float *X, float *C, da_int *labels,
+                                   da_int max_iter, float tol, da_int *n_iter, float *inertia);
+/* Samples per assignment tile chosen by the tuner for the last batch run (0 if none). */
+da_status da_kmeans_batch_tile_shape(da_int *samples_per_tile);
+/* Releases the device buffers and graphs kept for batch layouts. */
+da_status da_kmeans_batch_release(void);
 
 #ifdef __cplusplus
 }

@@ -199,3 +199,37 @@ def test_bf16_cross_term_near_ties(oracle):
                       tol=0.0)
     np.testing.assert_array_equal(km.labels, r["labels"])
     np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-5, atol=1e-5)
+
+
+def test_batch_matches_single_instances(oracle):
+    """Batched k-means over uneven instances equals each instance run alone (oracle with
+    supplied centres): labels, iteration counts, centres and inertia."""
+    rng = np.random.default_rng(50)
+    datasets, inits, cfg = [], [], []
+    for i in range(14):
+        d = int(rng.choice([3, 8, 16, 32, 40, 64]))
+        k = int(rng.integers(2, 65))
+        n = int(rng.integers(max(k, 50), 3000))
+        centres = rng.uniform(-10, 10, (k, d))
+        X = (centres[rng.integers(0, k, n)] + rng.standard_normal((n, d))).astype(np.float32)
+        datasets.append(X)
+        inits.append(X[:k].copy())
+        cfg.append((n, d, k))
+    for max_iter, tol in ((12, 0.0), (40, 1e-4)):
+        cs, labs, n_iter, inertia = aoclda.clustering.kmeans_batch(datasets, inits, max_iter, tol)
+        for i, X in enumerate(datasets):
+            r = oracle.kmeans(X.astype(np.float64), cfg[i][2], "supplied",
+                              inits[i].astype(np.float64), max_iter=max_iter, tol=tol)
+            np.testing.assert_array_equal(labs[i], r["labels"], err_msg=f"instance {i} {cfg[i]}")
+            assert n_iter[i] == r["n_iter"], (i, cfg[i], n_iter[i], r["n_iter"])
+            np.testing.assert_allclose(cs[i], r["centres"], rtol=1e-5, atol=1e-5)
+            np.testing.assert_allclose(inertia[i], r["inertia"], rtol=1e-5)
+
+
+def test_batch_argument_errors():
+    X = np.zeros((10, 4), dtype=np.float32)
+    with pytest.raises(aoclda.DAError):
+        aoclda.clustering.kmeans_batch([X], [np.zeros((65, 4), dtype=np.float32)])
+    with pytest.raises(aoclda.DAError):
+        aoclda.clustering.kmeans_batch([np.zeros((10, 65), dtype=np.float32)],
+                                       [np.zeros((2, 65), dtype=np.float32)])

@@ -212,12 +212,12 @@ def test_fused_large_k(oracle, k, precision):
     assert ncand.value >= k  # the fused tile path ran
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("precision", ["working", "fp64", "fp16", "fp16x3"])
 @pytest.mark.parametrize("d", [20, 64, 128])
 def test_tile_variants(oracle, variant, precision, d):
-    """Every sweep of the fused tile (register operands, LDS-staged, register prefetch) gives
-    the exact result, including training sizes that end inside a stage and splits of unequal
+    """Both sweeps of the fused tile (loads under the bookkeeping, load-then-compute) give the
+    exact result, including training sizes that end inside a step and splits of unequal
     length."""
     rng = np.random.default_rng(200 + d)
     means = rng.uniform(-5, 5, (8, d))
