@@ -261,42 +261,87 @@ SearchContext::~SearchContext() {
 
 // Fused path, launch phase: pack the queries, distance tile, merge/certify. The fallback
 // count is copied to pinned host memory; search_finish reads it after the stream completes.
-// Number of training splits for a batch of nq queries. The grid has qblocks * nsplit
-// workgroups and the device runs `slots` of them at once, so a sweep takes
-// ceil(qblocks * nsplit / slots) rounds of one split's steps each; every split also pays
-// a warm-up while its thresholds are still open. The count minimising that estimate is
-// taken, within the merge's capacity (nsplit * kc candidates per query) and keeping each
-// split at 1024 rows or more.
+// Number of training splits for a batch of nq queries, picked by measurement. With qb
+// queries per workgroup the grid has qblocks * nsplit workgroups and the device runs `slots`
+// of them at once; s1 = the fewest splits that fill about 90% of the slots. More splits
+// shorten each workgroup's sweep but restart more candidate lists (each start costs a burst
+// of threshold updates), so the best multiple of s1 depends on the operand form and the
+// data. The first batch of each size class (log2 of qblocks) times the multiples 1, 2 and 4
+// of s1 on its own queries and keeps the fastest; later batches reuse the choice. Splits are
+// capped by the merge's capacity (nsplit * kc candidates per query) and at 1024 rows each.
 template <typename T>
-int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq) {
-    const int key = (prec * 8 + variant) * 256 + kc;
+int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq,
+                             const void *qpad, hipStream_t st) {
     int slots = 0;
-    auto it = ctx.slots.find(key);
+    const int skey = (prec * 8 + variant) * 256 + kc;
+    auto it = ctx.slots.find(skey);
     if (it != ctx.slots.end()) {
         slots = it->second;
     } else {
         if (knn_tile_slots(prec, kp_, kc, variant, &slots) != hipSuccess || slots <= 0)
             slots = 512;
-        ctx.slots[key] = slots;
+        ctx.slots[skey] = slots;
     }
     const int64_t n = n_samples_;
     const int64_t qb = knn_tile_qb(kc);
     const int64_t qblocks = (nq + qb - 1) / qb;
     const int64_t hi = std::max<int64_t>(
         1, std::min<int64_t>(knn_merge_max_candidates() / kc, (n + 1023) / 1024));
-    const int64_t warm = 8;
-    int64_t best = 1;
-    double best_cost = 0;
-    for (int64_t s = 1; s <= hi; s++) {
-        const int64_t steps = ((n + s - 1) / s + 31) / 32;
-        const int64_t rounds = (qblocks * s + slots - 1) / slots;
-        const double cost = (double)rounds * (double)(steps + warm);
-        if (s == 1 || cost < best_cost * 0.995) {
-            best = s;
-            best_cost = cost;
+    const int64_t s1 = std::min<int64_t>(hi, std::max<int64_t>(1, (slots * 9 / 10 + qblocks - 1) / qblocks));
+    int bucket = 0;
+    while (((int64_t)1 << (bucket + 1)) <= qblocks)
+        bucket++;
+    const int key = skey * 64 + bucket;
+    auto pick = [&](int f) { return (int)std::min<int64_t>(hi, s1 * f); };
+    auto ft = ctx.split_factor.find(key);
+    if (ft != ctx.split_factor.end())
+        return pick(ft->second);
+    // Time the candidates (a first untimed launch warms the caches).
+    int best_f = 1;
+    float best_ms = -1.0f;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        if (a)
+            (void)hipEventDestroy(a);
+        return pick(1);
+    }
+    bool warmed = false;
+    for (int f : {1, 2, 4}) {
+        if (f > 1 && pick(f) == pick(f / 2))
+            break;
+        const int ns = pick(f);
+        const size_t np = (size_t)nq * ns;
+        if (ctx.part_key.reserve(np * kc * sizeof(float)) != hipSuccess ||
+            ctx.part_idx.reserve(np * kc * sizeof(int)) != hipSuccess ||
+            ctx.part_tau.reserve(np * sizeof(float)) != hipSuccess)
+            break;
+        auto launch = [&]() {
+            return launch_knn_tile(prec, qpad, nq, ypad_.ptr(), nyh_.ptr(), (int)n, kp_, ns, kc,
+                                   ctx.part_key.as<float>(), ctx.part_idx.as<int>(),
+                                   ctx.part_tau.as<float>(), variant, st);
+        };
+        if (!warmed) {
+            if (launch() != hipSuccess)
+                break;
+            warmed = true;
+        }
+        (void)hipEventRecord(a, st);
+        if (launch() != hipSuccess)
+            break;
+        (void)hipEventRecord(b, st);
+        if (hipEventSynchronize(b) != hipSuccess)
+            break;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (best_ms < 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best_f = f;
         }
     }
-    return (int)best;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    ctx.split_factor[key] = best_f;
+    return pick(best_f);
 }
 
 template <typename T>
@@ -317,10 +362,10 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
     int variant = sp.variant;
     if (variant < 0)
         variant = knn_tile_auto_variant(prec, kp_);
-    const int nsplit = tile_splits(ctx, prec, kc, variant, nq);
     const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
     DA_HIP(err, ctx.qpad.reserve((size_t)nq * kp_ * esz));
     DA_HIP(err, launch_pack_rows_prec<T>(dQ, d, nq, d, ctx.qpad.ptr(), kp_, nq, prec, st));
+    const int nsplit = tile_splits(ctx, prec, kc, variant, nq, ctx.qpad.ptr(), st);
     const size_t np = (size_t)nq * nsplit;
     DA_HIP(err, ctx.part_key.reserve(np * kc * sizeof(float)));
     DA_HIP(err, ctx.part_idx.reserve(np * kc * sizeof(int)));

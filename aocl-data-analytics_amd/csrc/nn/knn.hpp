@@ -12,7 +12,8 @@ namespace dagpu {
 // that batches overlap on the device.
 struct SearchContext {
     GpuContext *gpu = nullptr;
-    std::map<int, int> slots; // concurrent workgroups per fused-tile configuration
+    std::map<int, int> slots;        // concurrent workgroups per fused-tile configuration
+    std::map<int, int> split_factor; // tuned multiple of the minimal split count
     DeviceBuffer q_norm, keys, sel_key, sel_idx, qpad, part_key, part_idx, part_tau, fb, fb_q,
         fb_ind, fb_dist;
     // Pinned host word receiving the fallback count of the last fused launch.
@@ -68,7 +69,8 @@ template <typename T> class NNHandle : public AlgHandle {
     da_status search_tiles(SearchContext &ctx, const T *dQ, da_int nq, da_int k,
                            bool return_distance, da_int *d_ind, T *d_dist);
     bool fused_eligible(da_int k) const;
-    int tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq);
+    int tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq,
+                    const void *qpad, hipStream_t st);
     da_status prepare_fused(int prec);
 
     bool trained_ = false;
