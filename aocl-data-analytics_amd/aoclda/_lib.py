@@ -86,6 +86,7 @@ _proto("da_amd_malloc", S, C.c_size_t, P(C.c_void_p))
 _proto("da_amd_free", S, C.c_void_p)
 _proto("da_amd_memcpy_htod", S, C.c_void_p, C.c_void_p, C.c_size_t)
 _proto("da_amd_memcpy_dtoh", S, C.c_void_p, C.c_void_p, C.c_size_t)
+_proto("da_amd_memcpy_dtod", S, C.c_void_p, C.c_void_p, C.c_size_t)
 _proto("da_amd_device_synchronize", S)
 _proto("da_amd_device_name", S, C.c_char_p, da_int)
 _proto("da_amd_synchronize", S, da_handle)

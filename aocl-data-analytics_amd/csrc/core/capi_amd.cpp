@@ -28,6 +28,9 @@ da_status da_amd_memcpy_htod(void *d_dst, const void *h_src, size_t bytes) {
 da_status da_amd_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes) {
     return map(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
 }
+da_status da_amd_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes) {
+    return map(hipMemcpy(d_dst, d_src, bytes, hipMemcpyDeviceToDevice));
+}
 da_status da_amd_device_synchronize(void) { return map(hipDeviceSynchronize()); }
 da_status da_amd_device_name(char *name, da_int len) {
     if (!name || len < 1)

@@ -370,8 +370,18 @@ __device__ __forceinline__ void knn_tile_body(
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const int qblock = blockIdx.x / nsplit;
-    const int split = blockIdx.x - qblock * nsplit;
+    // XCD-aware order: workgroup b runs on XCD b % 8 and the work items (split-major: all
+    // query blocks of split 0, then split 1, ...) are dealt to the XCDs in 8 contiguous
+    // ranges, so the workgroups sharing an XCD's L2 sweep the same one or two splits at
+    // about the same time. The grid is padded to a multiple of 8; extra workgroups exit.
+    const int qblocks = (nq + QB - 1) / QB;
+    const int nwork = qblocks * nsplit;
+    const int per_xcd = (nwork + 7) / 8;
+    const int work = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (work >= nwork)
+        return;
+    const int split = work / qblocks;
+    const int qblock = work - split * qblocks;
     const int q0 = qblock * QB;
     const int t_begin = split * split_len;
     int t_end = t_begin + split_len;
@@ -744,7 +754,7 @@ static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void
     int split_len = (n + nsplit - 1) / nsplit;
     split_len = (split_len + STEP - 1) / STEP * STEP;
     const int qblocks = (nq + QB - 1) / QB;
-    const dim3 grid(qblocks * nsplit);
+    const dim3 grid((qblocks * nsplit + 7) / 8 * 8);
     hipLaunchKernelGGL(kern, grid, block, 0, s, static_cast<const in_t *>(Q), nq,
                        static_cast<const in_t *>(Y), static_cast<const ae *>(nyh), n, Kp,
                        split_len, nsplit, part_key, part_idx, part_tau);

@@ -409,15 +409,17 @@ def main():
         return
 
     if args.copy_calibration_mib > 0:
-        import torch
-        n = args.copy_calibration_mib << 18  # float32 elements
-        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
-        b = torch.empty_like(a)
+        # Two device-to-device copies of a known size (the runtime's copy kernel).
+        nb = args.copy_calibration_mib << 20
+        ca, cb = C.c_void_p(), C.c_void_p()
+        L.check(L.lib.da_amd_malloc(nb, C.byref(ca)))
+        L.check(L.lib.da_amd_malloc(nb, C.byref(cb)))
         for _ in range(2):
-            b.copy_(a)
-        torch.cuda.synchronize()
+            L.check(L.lib.da_amd_memcpy_dtod(cb, ca, nb))
+        L.check(L.lib.da_amd_device_synchronize())
+        L.lib.da_amd_free(ca)
+        L.lib.da_amd_free(cb)
         log(f"copy calibration: 2 copies of {args.copy_calibration_mib} MiB")
-        del a, b
 
     rng = np.random.default_rng(0)
     means = rng.uniform(-10, 10, (64, args.dim))

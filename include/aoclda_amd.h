@@ -21,6 +21,7 @@ da_status da_amd_malloc(size_t bytes, void **d_ptr);
 da_status da_amd_free(void *d_ptr);
 da_status da_amd_memcpy_htod(void *d_dst, const void *h_src, size_t bytes);
 da_status da_amd_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes);
+da_status da_amd_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes);
 da_status da_amd_device_synchronize(void);
 /* Name of the GPU architecture the library runs on (e.g. "gfx950"). */
 da_status da_amd_device_name(char *name, da_int len);
