@@ -96,7 +96,7 @@ _proto("da_amd_profile_get", S, da_handle, C.c_char_p, P(C.c_double), P(da_int))
 _proto("da_nn_last_query_stats", S, da_handle, P(da_int), P(da_int))
 _proto("da_kmeans_last_compute_stats", S, da_handle, P(da_int))
 _proto("da_kmeans_batch_device_s", S, da_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-       C.c_void_p, C.c_void_p, da_int, C.c_float, C.c_void_p, C.c_void_p)
+       C.c_void_p, C.c_void_p, da_int, C.c_float, da_int, C.c_void_p, C.c_void_p)
 _proto("da_kmeans_batch_tile_shape", S, P(da_int))
 _proto("da_kmeans_batch_release", S)
 _proto("da_nn_server_create", S, da_handle, da_int, C.c_size_t, P(C.c_void_p))

@@ -8,10 +8,10 @@
 //    persistent grid sized to the device, most expensive instances first;
 //  * sample chunks (instance, first sample, count) for the centroid sums, the column maxima
 //    and the inertia.
-// The assignment uses a bf16 cross term on MFMA (v_mfma_f32_16x16x32_bf16) against centres
-// staged in LDS per tile, and re-checks every sample whose two best scores are within the
-// error bound with exact double distances, so the labels are the exact argmin (ties to the
-// lowest centre). Centroid sums are 64-bit fixed point per instance and feature (the same
+// The assignment runs the cross term on MFMA (fp32 v_mfma_f32_16x16x4f32 or bf16
+// v_mfma_f32_16x16x32_bf16) against centres staged in LDS per instance, and re-checks every
+// sample whose two best scores are within the error bound with exact double distances, so
+// the labels are the exact argmin (ties to the lowest centre) in either form. Centroid sums are 64-bit fixed point per instance and feature (the same
 // scheme as the single-instance kernels), reduced in a fixed order: the result is bitwise
 // reproducible. Convergence is tested per instance on the device; converged instances drop
 // out of the later launches, so the whole run can be captured in one graph.
@@ -24,7 +24,53 @@ namespace dagpu {
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// Operand forms of the assignment cross term. E: elements per lane per MFMA; UX, UC: unit
+// roundoff of the data and of the centres in the operand type (fp32 data are exact).
+struct KbBF16 {
+    using T = __bf16;
+    using frag = v8bf;
+    static constexpr int E = 8;
+    static constexpr double UX = 1.0 / 256.0, UC = 1.0 / 256.0;
+    __device__ static v4f mfma(frag a, frag b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+struct KbF32 {
+    using T = float;
+    using frag = float;
+    static constexpr int E = 1;
+    static constexpr double UX = 0.0, UC = 1.0 / 16777216.0;
+    __device__ static v4f mfma(frag a, frag b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// NF fragments of a lane from contiguous operand elements (16- or 8-byte vector loads).
+template <class Op, int NF>
+__device__ __forceinline__ void kb_load(const typename Op::T *p, typename Op::frag *f) {
+    if constexpr (Op::E == 8) {
+#pragma unroll
+        for (int v = 0; v < NF; v++)
+            f[v] = *reinterpret_cast<const v8bf *>(p + 8 * v);
+    } else if constexpr (NF % 4 == 0) {
+#pragma unroll
+        for (int v = 0; v < NF; v += 4) {
+            const v4f x = *reinterpret_cast<const v4f *>(p + v);
+            f[v] = x[0];
+            f[v + 1] = x[1];
+            f[v + 2] = x[2];
+            f[v + 3] = x[3];
+        }
+    } else {
+        static_assert(NF == 2, "fp32 fragments come in pairs or fours");
+        const v2f x = *reinterpret_cast<const v2f *>(p);
+        f[0] = x[0];
+        f[1] = x[1];
+    }
+}
 
 
 // Column maxima |x| per instance and feature (atomic max on the bit pattern of a
@@ -85,17 +131,19 @@ __global__ void kb_escale_kernel(KBatchDesc b, int n_inst,
     escale[inst * 65 + 64] = 61 - l2n;
 }
 
-// bf16 copy of each instance's rows padded to kp features.
-__global__ __launch_bounds__(256) void kb_to_bf16_kernel(const float *__restrict__ X, KBatchDesc b,
-                                                         const KChunk *__restrict__ chunks,
-                                                         __bf16 *__restrict__ Xb) {
+// Copy of each instance's rows in the operand type, padded to kp features.
+template <class Op>
+__global__ __launch_bounds__(256) void kb_to_op_kernel(const float *__restrict__ X, KBatchDesc b,
+                                                       const KChunk *__restrict__ chunks,
+                                                       typename Op::T *__restrict__ Xo) {
+    using T = typename Op::T;
     const KChunk ch = chunks[blockIdx.x];
     const int inst = ch.inst, d = b.d[inst], kp = b.kp[inst];
     const float *x = X + b.xoff[inst] + (int64_t)ch.start * d;
-    __bf16 *y = Xb + b.xboff[inst] + (int64_t)ch.start * kp;
+    T *y = Xo + b.xboff[inst] + (int64_t)ch.start * kp;
     const int spp = 256 / kp, sub = (int)threadIdx.x / kp, l = (int)threadIdx.x - sub * kp;
     for (int i = sub; i < ch.count; i += spp)
-        y[(int64_t)i * kp + l] = l < d ? (__bf16)x[(int64_t)i * d + l] : (__bf16)0.0f;
+        y[(int64_t)i * kp + l] = l < d ? (T)x[(int64_t)i * d + l] : (T)0.0f;
 }
 
 // Initial centres (fp32, packed k x d per instance) into the double master copy.
@@ -130,15 +178,17 @@ __device__ __forceinline__ int kb_in_parity(const int *fpar, int inst, int mode,
     return mode == 0 ? par : 1 - fpar[inst];
 }
 
-// Per instance and selected parity: bf16 centres (kpad x kp, zero padded), -|c|^2/2 in fp32
-// and max |c|^2. One workgroup per instance.
+// Per instance and selected parity: centres in the operand type (kpad x kp, zero padded),
+// -|c|^2/2 in fp32 and max |c|^2. One workgroup per instance.
+template <class Op>
 __global__ __launch_bounds__(64) void kb_centre_prep_kernel(KBatchDesc b, const double *C0,
                                                             const double *C1,
                                                             const int *__restrict__ state,
                                                             const int *__restrict__ fpar, int mode,
-                                                            int par, __bf16 *__restrict__ Cb,
+                                                            int par, typename Op::T *__restrict__ Cb,
                                                             float *__restrict__ chalf,
                                                             double *__restrict__ cmax2) {
+    using T = typename Op::T;
     const int inst = blockIdx.x;
     if (!kb_runs(state, inst, mode))
         return;
@@ -151,7 +201,7 @@ __global__ __launch_bounds__(64) void kb_centre_prep_kernel(KBatchDesc b, const 
     if (c < kpad) {
         for (int l = 0; l < kp; l++) {
             const double v = (c < k && l < d) ? C[(int64_t)c * d + l] : 0.0;
-            Cb[b.cboff[inst] + (int64_t)c * kp + l] = (__bf16)(float)v;
+            Cb[b.cboff[inst] + (int64_t)c * kp + l] = (T)(float)v;
             s = fma(v, v, s);
         }
         if (c < k)
@@ -169,33 +219,39 @@ __global__ __launch_bounds__(64) void kb_centre_prep_kernel(KBatchDesc b, const 
 }
 
 // Assignment of one tile: 16*PT samples per wave against the instance's centres in LDS.
-template <int NF, int PT>
-__device__ __forceinline__ void kb_assign_tile(const __bf16 *__restrict__ Xb, int n, int kp,
-                                               int start, const __bf16 *Cs, const float *Hs,
-                                               int k, double cmax, int *__restrict__ labels,
+// Lane group g holds features g*kp/4 .. of its sample (the k-permutation), NF fragments.
+template <class Op, int NF, int PT>
+__device__ __forceinline__ void kb_assign_tile(const typename Op::T *__restrict__ Xo, int n,
+                                               int kp, int start, const typename Op::T *Cs,
+                                               const float *Hs, int k, double cmax,
+                                               int *__restrict__ labels,
                                                const int *__restrict__ prev, int *changed,
                                                int *list_count, KTile *list, int inst) {
+    using frag = typename Op::frag;
+    constexpr int KCH = NF * Op::E;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const int kch = kp / 4;
     const int kpad = (k + 15) / 16 * 16;
     const int s0 = start + w * PT * 16;
-    v8bf bb[PT][NF];
+    frag bb[PT][NF];
     float xn[PT];
 #pragma unroll
     for (int pt = 0; pt < PT; pt++) {
         const int i = s0 + pt * 16 + r;
+        kb_load<Op, NF>(Xo + (int64_t)(i < n ? i : 0) * kp + g * KCH, bb[pt]);
         float s = 0.f;
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            v8bf v = {};
-            if (i < n)
-                v = *reinterpret_cast<const v8bf *>(Xb + (int64_t)i * kp + g * kch + f * 8);
-            bb[pt][f] = v;
+            if (i >= n)
+                bb[pt][f] = frag{};
+            if constexpr (Op::E == 8) {
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const float x = (float)v[e];
-                s = fmaf(x, x, s);
+                for (int e = 0; e < 8; e++) {
+                    const float x = (float)bb[pt][f][e];
+                    s = fmaf(x, x, s);
+                }
+            } else {
+                s = fmaf(bb[pt][f], bb[pt][f], s);
             }
         }
         xn[pt] = s;
@@ -209,11 +265,8 @@ __device__ __forceinline__ void kb_assign_tile(const __bf16 *__restrict__ Xb, in
         bidx[pt] = INT_MAX;
     }
     for (int c0 = 0; c0 < kpad; c0 += 16) {
-        v8bf a[NF];
-        const __bf16 *src = Cs + (c0 + r) * kp + g * kch;
-#pragma unroll
-        for (int f = 0; f < NF; f++)
-            a[f] = *reinterpret_cast<const v8bf *>(src + f * 8);
+        frag a[NF];
+        kb_load<Op, NF>(Cs + (c0 + r) * kp + g * KCH, a);
         v4f h;
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -223,7 +276,7 @@ __device__ __forceinline__ void kb_assign_tile(const __bf16 *__restrict__ Xb, in
             v4f acc = h;
 #pragma unroll
             for (int f = 0; f < NF; f++)
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[f], bb[pt][f], acc, 0, 0, 0);
+                acc = Op::mfma(a[f], bb[pt][f], acc);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const float v = acc[i];
@@ -238,7 +291,7 @@ __device__ __forceinline__ void kb_assign_tile(const __bf16 *__restrict__ Xb, in
             }
         }
     }
-    const double u = 1.0 / 256.0, u32 = 1.0 / 16777216.0;
+    const double ux = Op::UX, uc = Op::UC, u32 = 1.0 / 16777216.0;
 #pragma unroll
     for (int pt = 0; pt < PT; pt++) {
         float xs = xn[pt];
@@ -257,39 +310,53 @@ __device__ __forceinline__ void kb_assign_tile(const __bf16 *__restrict__ Xb, in
             best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
         }
         const int i = s0 + pt * 16 + r;
-        if (g == 0 && i < n) {
-            // Same bound as the single-instance bf16 assignment: operand rounding
-            // (2u + u^2)|x| cmax, fp32 accumulation gamma_{kp+2}, the rounded -|c|^2/2.
-            const double xnorm = sqrt((double)xs) / (1.0 - u) * (1.0 + 1e-6);
+        const bool valid = g == 0 && i < n;
+        bool listed = false, moved = false;
+        if (valid) {
+            // Score error: operand rounding (ux + uc + ux uc)|x| cmax, fp32 accumulation
+            // gamma_{kp+2} over the rounded products and -|c|^2/2, the rounded -|c|^2/2;
+            // |x| from the operand copy (relative error <= ux) with a margin for the sum.
+            const double xnorm = sqrt((double)xs) / (1.0 - ux) * (1.0 + 1e-6);
             const double gam = (kp + 2) * u32 / (1.0 - (kp + 2) * u32);
             const double S = xnorm * cmax;
-            const double e = (2.0 * u + u * u) * S +
-                             gam * (S * (1.0 + u) * (1.0 + u) + 0.5 * cmax * cmax) +
+            const double e = (ux + uc + ux * uc) * S +
+                             gam * (S * (1.0 + ux) * (1.0 + uc) + 0.5 * cmax * cmax) +
                              u32 * 0.5 * cmax * cmax;
             const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
             const int lab = bidx[pt] < k ? bidx[pt] : 0;
             labels[i] = lab;
-            if (near_tie || !(best1[pt] - best1[pt] == 0.0f)) {
-                const int p = atomicAdd(list_count, 1);
-                list[p] = KTile{inst, i};
-            } else if (prev && prev[i] != lab) {
-                atomicAdd(changed, 1);
-            }
+            listed = near_tie || !(best1[pt] - best1[pt] == 0.0f);
+            moved = !listed && prev && prev[i] != lab;
         }
+        // One atomic per wave for the re-check list and one for the change count.
+        const unsigned long long lm = __ballot(listed);
+        const unsigned long long cm = __ballot(moved);
+        if (lm) {
+            const int first = __ffsll((long long)lm) - 1;
+            int base = 0;
+            if (lane == first)
+                base = atomicAdd(list_count, __popcll(lm));
+            base = __builtin_amdgcn_readlane(base, first);
+            if (listed)
+                list[base + __popcll(lm & ((1ull << lane) - 1))] = KTile{inst, i};
+        }
+        if (cm && lane == 0)
+            atomicAdd(changed, __popcll(cm));
     }
 }
 
 // Persistent assignment: workgroup b takes tiles b, b + G, b + 2G, ... of the list (sorted
 // by decreasing cost, so the round-robin deal is balanced); the centres of an instance are
 // staged once for a run of its tiles.
-template <int PT>
+template <class Op, int PT>
 __global__ __launch_bounds__(256) void kb_assign_kernel(
-    const __bf16 *__restrict__ Xb, KBatchDesc b, const KTile *__restrict__ tiles, int ntiles,
-    const int *__restrict__ state, const int *__restrict__ fpar, int mode, int par,
-    const __bf16 *__restrict__ Cb, const float *__restrict__ chalf,
+    const typename Op::T *__restrict__ Xo, KBatchDesc b, const KTile *__restrict__ tiles,
+    int ntiles, const int *__restrict__ state, const int *__restrict__ fpar, int mode, int par,
+    const typename Op::T *__restrict__ Cb, const float *__restrict__ chalf,
     const double *__restrict__ cmax2, int *__restrict__ L0, int *__restrict__ L1,
     int *__restrict__ changed, int *__restrict__ list_count, KTile *__restrict__ list) {
-    __shared__ __attribute__((aligned(16))) __bf16 Cs[64 * 64];
+    using T = typename Op::T;
+    __shared__ __attribute__((aligned(16))) T Cs[64 * 64];
     __shared__ float Hs[64];
     int staged = -1;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -304,7 +371,7 @@ __global__ __launch_bounds__(256) void kb_assign_kernel(
             const int kpad = (k + 15) / 16 * 16;
             const uint4 *src = reinterpret_cast<const uint4 *>(Cb + b.cboff[inst]);
             uint4 *dst = reinterpret_cast<uint4 *>(Cs);
-            for (int q = threadIdx.x; q < kpad * kp / 8; q += 256)
+            for (int q = threadIdx.x; q < kpad * kp * (int)sizeof(T) / 16; q += 256)
                 dst[q] = src[q];
             if (threadIdx.x < 64)
                 Hs[threadIdx.x] = (int)threadIdx.x < k ? chalf[inst * 64 + threadIdx.x]
@@ -316,13 +383,27 @@ __global__ __launch_bounds__(256) void kb_assign_kernel(
         int *lab_new = (p ? L1 : L0) + b.loff[inst];
         const int *lab_old = mode == 0 ? (p ? L0 : L1) + b.loff[inst] : nullptr;
         const double cmax = sqrt(cmax2[inst]);
-        const __bf16 *xb = Xb + b.xboff[inst];
-        if (kp == 32)
-            kb_assign_tile<1, PT>(xb, b.n[inst], kp, tl.start, Cs, Hs, k, cmax, lab_new, lab_old,
-                                  &changed[inst], list_count, list, inst);
-        else
-            kb_assign_tile<2, PT>(xb, b.n[inst], kp, tl.start, Cs, Hs, k, cmax, lab_new, lab_old,
-                                  &changed[inst], list_count, list, inst);
+        const T *xo = Xo + b.xboff[inst];
+        const int n = b.n[inst];
+#define KB_TILE(NFV)                                                                             \
+    kb_assign_tile<Op, NFV, PT>(xo, n, kp, tl.start, Cs, Hs, k, cmax, lab_new, lab_old,        \
+                                &changed[inst], list_count, list, inst)
+        if constexpr (Op::E == 8) {
+            if (kp == 32)
+                KB_TILE(1);
+            else
+                KB_TILE(2);
+        } else {
+            if (kp == 8)
+                KB_TILE(2);
+            else if (kp == 16)
+                KB_TILE(4);
+            else if (kp == 32)
+                KB_TILE(8);
+            else
+                KB_TILE(16);
+        }
+#undef KB_TILE
     }
 }
 
@@ -576,15 +657,19 @@ inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1)
 
 hipError_t kb_launch_prepare(const float *X, const float *Cinit, const KBatchDesc &b, int n_inst,
                              const KChunk *chunks, int nchunks, int64_t ctotal,
-                             unsigned long long *amax, int *escale, uint16_t *Xb, double *C0,
-                             int *state, int *fpar, int *n_iter, hipStream_t s) {
+                             unsigned long long *amax, int *escale, int prec, void *Xo,
+                             double *C0, int *state, int *fpar, int *n_iter, hipStream_t s) {
     hipLaunchKernelGGL(kb_fill_u64_kernel, dim3(blocks_for((int64_t)n_inst * 64, 256)), dim3(256),
                        0, s, amax, (int64_t)n_inst * 64);
     hipLaunchKernelGGL(kb_absmax_kernel, dim3(nchunks), dim3(256), 0, s, X, b, chunks, amax);
     hipLaunchKernelGGL(kb_escale_kernel, dim3(blocks_for(n_inst, 64)), dim3(64), 0, s, b, n_inst,
                        amax, escale);
-    hipLaunchKernelGGL(kb_to_bf16_kernel, dim3(nchunks), dim3(256), 0, s, X, b, chunks,
-                       reinterpret_cast<__bf16 *>(Xb));
+    if (prec == 1)
+        hipLaunchKernelGGL(kb_to_op_kernel<KbBF16>, dim3(nchunks), dim3(256), 0, s, X, b, chunks,
+                           reinterpret_cast<__bf16 *>(Xo));
+    else
+        hipLaunchKernelGGL(kb_to_op_kernel<KbF32>, dim3(nchunks), dim3(256), 0, s, X, b, chunks,
+                           reinterpret_cast<float *>(Xo));
     hipLaunchKernelGGL(kb_centres_in_kernel, dim3(blocks_for(ctotal, 256)), dim3(256), 0, s, Cinit,
                        ctotal, C0);
     hipLaunchKernelGGL(kb_fill_int_kernel, dim3(blocks_for(n_inst, 256)), dim3(256), 0, s, state,
@@ -596,22 +681,32 @@ hipError_t kb_launch_prepare(const float *X, const float *Cinit, const KBatchDes
     return hipGetLastError();
 }
 
+template <class Op>
+static void kb_assign_op(const KBatchBuffers &w, const KBatchDesc &b, int n_inst, int mode,
+                         int par, int pt, int grid, hipStream_t s) {
+    using T = typename Op::T;
+    T *Cb = static_cast<T *>(w.Cb);
+    const T *Xo = static_cast<const T *>(w.Xo);
+    hipLaunchKernelGGL(kb_centre_prep_kernel<Op>, dim3(n_inst), dim3(64), 0, s, b, w.C0, w.C1,
+                       w.state, w.fpar, mode, par, Cb, w.chalf, w.cmax2);
+    if (pt == 2)
+        hipLaunchKernelGGL((kb_assign_kernel<Op, 2>), dim3(grid), dim3(256), 0, s, Xo, b, w.tiles,
+                           w.ntiles, w.state, w.fpar, mode, par, Cb, w.chalf, w.cmax2, w.L0, w.L1,
+                           w.changed, w.list_count, w.list);
+    else
+        hipLaunchKernelGGL((kb_assign_kernel<Op, 4>), dim3(grid), dim3(256), 0, s, Xo, b, w.tiles,
+                           w.ntiles, w.state, w.fpar, mode, par, Cb, w.chalf, w.cmax2, w.L0, w.L1,
+                           w.changed, w.list_count, w.list);
+}
+
 hipError_t kb_launch_assign(const KBatchBuffers &w, const KBatchDesc &b, int n_inst, int mode,
                             int par, int pt, int grid, hipStream_t s) {
     hipLaunchKernelGGL(kb_reset_kernel, dim3(blocks_for(n_inst, 256)), dim3(256), 0, s,
                        w.list_count, w.changed, n_inst);
-    __bf16 *Cb = reinterpret_cast<__bf16 *>(w.Cb);
-    const __bf16 *Xb = reinterpret_cast<const __bf16 *>(w.Xb);
-    hipLaunchKernelGGL(kb_centre_prep_kernel, dim3(n_inst), dim3(64), 0, s, b, w.C0, w.C1, w.state,
-                       w.fpar, mode, par, Cb, w.chalf, w.cmax2);
-    if (pt == 2)
-        hipLaunchKernelGGL(kb_assign_kernel<2>, dim3(grid), dim3(256), 0, s, Xb, b, w.tiles,
-                           w.ntiles, w.state, w.fpar, mode, par, Cb, w.chalf, w.cmax2,
-                           w.L0, w.L1, w.changed, w.list_count, w.list);
+    if (w.prec == 1)
+        kb_assign_op<KbBF16>(w, b, n_inst, mode, par, pt, grid, s);
     else
-        hipLaunchKernelGGL(kb_assign_kernel<4>, dim3(grid), dim3(256), 0, s, Xb, b, w.tiles,
-                           w.ntiles, w.state, w.fpar, mode, par, Cb, w.chalf, w.cmax2,
-                           w.L0, w.L1, w.changed, w.list_count, w.list);
+        kb_assign_op<KbF32>(w, b, n_inst, mode, par, pt, grid, s);
     hipLaunchKernelGGL(kb_recheck_kernel, dim3(1024), dim3(256), 0, s, w.X, b, w.C0, w.C1, w.fpar,
                        mode, par, w.list_count, w.list, w.L0, w.L1, w.changed);
     return hipGetLastError();
@@ -648,12 +743,15 @@ hipError_t kb_set_partial_lds(size_t lds) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
-int kb_assign_blocks_per_cu(int pt) {
+int kb_assign_blocks_per_cu(int prec, int pt) {
     int per_cu = 0;
-    hipError_t e = pt == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                 &per_cu, kb_assign_kernel<2>, 256, 0)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                                 &per_cu, kb_assign_kernel<4>, 256, 0);
+    hipError_t e;
+    if (prec == 1)
+        e = pt == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbBF16, 2>, 256, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbBF16, 4>, 256, 0);
+    else
+        e = pt == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbF32, 2>, 256, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbF32, 4>, 256, 0);
     return e == hipSuccess && per_cu > 0 ? per_cu : 1;
 }
 

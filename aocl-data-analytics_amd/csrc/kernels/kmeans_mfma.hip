@@ -123,7 +123,9 @@ __global__ __launch_bounds__(256) void kmeans_assign_mfma_kernel(
             best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
         }
         const int i = s0 + pt * 16 + r;
-        if (g == 0 && i < n) {
+        const bool valid = g == 0 && i < n;
+        bool listed = false, moved = false;
+        if (valid) {
             // |acc - exact| <= gamma_{d+2} (|x| cmax + cmax^2/2) + unit (|x| cmax + cmax^2/2)
             const double xnorm = sqrt((double)xs);
             const double gam = (Kp + 2) * unit / (1.0 - (Kp + 2) * unit);
@@ -131,13 +133,23 @@ __global__ __launch_bounds__(256) void kmeans_assign_mfma_kernel(
             const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
             const int lab = bidx[pt] < k ? bidx[pt] : 0;
             labels[i] = lab;
-            if (near_tie || !(best1[pt] - best1[pt] == 0.0f)) {
-                const int p = atomicAdd(list_count, 1);
-                list[p] = i;
-            } else if (prev && prev[i] != lab) {
-                atomicAdd(changed, 1);
-            }
+            listed = near_tie || !(best1[pt] - best1[pt] == 0.0f);
+            moved = !listed && prev && prev[i] != lab;
         }
+        // One atomic per wave for the re-check list and one for the change count.
+        const unsigned long long lm = __ballot(listed);
+        const unsigned long long cm = __ballot(moved);
+        if (lm) {
+            const int first = __ffsll((long long)lm) - 1;
+            int base = 0;
+            if (lane == first)
+                base = atomicAdd(list_count, __popcll(lm));
+            base = __builtin_amdgcn_readlane(base, first);
+            if (listed)
+                list[base + __popcll(lm & ((1ull << lane) - 1))] = i;
+        }
+        if (cm && lane == 0)
+            atomicAdd(changed, __popcll(cm));
     }
 }
 
@@ -250,7 +262,9 @@ __global__ __launch_bounds__(256) void kmeans_assign_bf16_kernel(
             best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
         }
         const int i = s0 + pt * 16 + r;
-        if (g == 0 && i < n) {
+        const bool valid = g == 0 && i < n;
+        bool listed = false, moved = false;
+        if (valid) {
             // |x| of the data from the norm of its bf16 copy (relative error <= u), with a
             // margin for the fp32 sum. Score error: operand rounding (2u + u^2)|x| cmax, the
             // fp32 accumulation gamma_{Kp+2} over |x~| |c~| + cmax^2/2, the rounded -|c|^2/2.
@@ -262,13 +276,23 @@ __global__ __launch_bounds__(256) void kmeans_assign_bf16_kernel(
             const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
             const int lab = bidx[pt] < k ? bidx[pt] : 0;
             labels[i] = lab;
-            if (near_tie || !(best1[pt] - best1[pt] == 0.0f)) {
-                const int p = atomicAdd(list_count, 1);
-                list[p] = i;
-            } else if (prev && prev[i] != lab) {
-                atomicAdd(changed, 1);
-            }
+            listed = near_tie || !(best1[pt] - best1[pt] == 0.0f);
+            moved = !listed && prev && prev[i] != lab;
         }
+        // One atomic per wave for the re-check list and one for the change count.
+        const unsigned long long lm = __ballot(listed);
+        const unsigned long long cm = __ballot(moved);
+        if (lm) {
+            const int first = __ffsll((long long)lm) - 1;
+            int base = 0;
+            if (lane == first)
+                base = atomicAdd(list_count, __popcll(lm));
+            base = __builtin_amdgcn_readlane(base, first);
+            if (listed)
+                list[base + __popcll(lm & ((1ull << lane) - 1))] = i;
+        }
+        if (cm && lane == 0)
+            atomicAdd(changed, __popcll(cm));
     }
 }
 

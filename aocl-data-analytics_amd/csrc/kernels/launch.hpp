@@ -238,9 +238,9 @@ hipError_t launch_gather_scatter(const void *src, void *dst, const int *idx, int
 // ---- Batched k-means (kmeans_batch.hip) ----
 // Instance states.
 enum { KB_ACTIVE = 0, KB_CONV_LABELS = 1, KB_CONV_SHIFT = 2, KB_MAXIT = 3 };
-// Per-instance device arrays: sizes, padded feature count (32 or 64) and element offsets of
-// the instance's rows (X), bf16 rows (Xb), centres (k x d), bf16 centres (kpad x kp) and
-// labels.
+// Per-instance device arrays: sizes, padded feature count of the operand copy (bf16: 32 or
+// 64; fp32: 8, 16, 32 or 64) and element offsets of the instance's rows (X), operand rows
+// (Xo), centres (k x d), operand centres (kpad x kp) and labels.
 struct KBatchDesc {
     const int *n, *d, *k, *kp;
     const int64_t *xoff, *xboff, *coff, *cboff, *loff;
@@ -255,10 +255,11 @@ struct KChunk {
 // Device work buffers of a batch.
 struct KBatchBuffers {
     const float *X;
-    uint16_t *Xb;                     // bf16 rows
+    int prec;                         // cross term: 0 fp32, 1 bf16
+    void *Xo;                         // rows in the operand type, padded to kp
     double *C0, *C1;                  // centres by parity
     int *L0, *L1;                     // labels by parity
-    uint16_t *Cb;                     // bf16 centres
+    void *Cb;                         // centres in the operand type (kpad x kp)
     float *chalf;                     // -|c|^2/2, 64 per instance
     double *cmax2;                    // max |c|^2 per instance
     const KTile *tiles;               // assignment tiles
@@ -277,8 +278,8 @@ struct KBatchBuffers {
 };
 hipError_t kb_launch_prepare(const float *X, const float *Cinit, const KBatchDesc &b, int n_inst,
                              const KChunk *chunks, int nchunks, int64_t ctotal,
-                             unsigned long long *amax, int *escale, uint16_t *Xb, double *C0,
-                             int *state, int *fpar, int *n_iter, hipStream_t s);
+                             unsigned long long *amax, int *escale, int prec, void *Xo,
+                             double *C0, int *state, int *fpar, int *n_iter, hipStream_t s);
 // One assignment pass (mode 0: iteration with centre parity par; mode 1: final relabel) with
 // pt 16-sample tiles per wave on `grid` persistent workgroups, plus the exact re-check.
 hipError_t kb_launch_assign(const KBatchBuffers &w, const KBatchDesc &b, int n_inst, int mode,
@@ -289,7 +290,7 @@ hipError_t kb_launch_update(const KBatchBuffers &w, const KBatchDesc &b, int n_i
 // Final relabel, inertia and outputs.
 hipError_t kb_launch_finish(const KBatchBuffers &w, const KBatchDesc &b, int n_inst, int pt,
                             int grid, float *Cout, int *labels, double *inertia, hipStream_t s);
-int kb_assign_blocks_per_cu(int pt);
+int kb_assign_blocks_per_cu(int prec, int pt);
 // Allows the partial-sum kernel `lds` bytes of dynamic LDS (up to 160 KB).
 hipError_t kb_set_partial_lds(size_t lds);
 

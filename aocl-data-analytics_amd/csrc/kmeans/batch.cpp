@@ -26,11 +26,11 @@ using namespace dagpu;
 struct BatchKey {
     std::vector<int> n, d, k;
     const void *X, *C, *labels;
-    int max_iter;
+    int max_iter, prec;
     double tol;
     bool operator<(const BatchKey &o) const {
-        return std::tie(n, d, k, X, C, labels, max_iter, tol) <
-               std::tie(o.n, o.d, o.k, o.X, o.C, o.labels, o.max_iter, o.tol);
+        return std::tie(n, d, k, X, C, labels, max_iter, prec, tol) <
+               std::tie(o.n, o.d, o.k, o.X, o.C, o.labels, o.max_iter, o.prec, o.tol);
     }
 };
 
@@ -70,15 +70,26 @@ template <typename V> hipError_t upload(DeviceBuffer &b, const std::vector<V> &v
 }
 
 // Builds the work lists and buffers of a batch and tunes the tile shape.
+// Padded width of an instance's operand copy: the bf16 MFMA takes 32 features per step (so
+// 32 or 64); the fp32 one 4 per lane group, padded to 8, 16, 32 or 64 for vector loads.
+int op_width(int prec, int d) {
+    if (prec == 1)
+        return d <= 32 ? 32 : 64;
+    int kp = 8;
+    while (kp < d)
+        kp *= 2;
+    return kp;
+}
+
 da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, const da_int *k,
-                const float *X, int chunk_hint) {
+                const float *X, int chunk_hint, int prec) {
     hipStream_t st = r.gpu.stream();
     r.n_inst = n_inst;
     std::vector<int> hn(n, n + n_inst), hd(d, d + n_inst), hk(k, k + n_inst), hkp(n_inst);
     std::vector<int64_t> xoff(n_inst), xboff(n_inst), coff(n_inst), cboff(n_inst), loff(n_inst);
     int64_t xo = 0, xbo = 0, co = 0, cbo = 0, lo = 0;
     for (int i = 0; i < n_inst; i++) {
-        hkp[i] = hd[i] <= 32 ? 32 : 64;
+        hkp[i] = op_width(prec, hd[i]);
         xoff[i] = xo;
         xboff[i] = xbo;
         coff[i] = co;
@@ -127,12 +138,13 @@ da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, co
     BR(upload(r.poff, poff, st));
     BR(upload(r.cbeg, cbeg, st));
     BR(upload(r.cend, cend, st));
-    BR(r.xb.reserve((size_t)std::max<int64_t>(1, xbo) * 2));
+    const size_t esz = prec == 1 ? 2 : 4;
+    BR(r.xb.reserve((size_t)std::max<int64_t>(1, xbo) * esz));
     BR(r.c0.reserve((size_t)std::max<int64_t>(1, co) * sizeof(double)));
     BR(r.c1.reserve((size_t)std::max<int64_t>(1, co) * sizeof(double)));
     BR(r.l0.reserve((size_t)std::max<int64_t>(1, lo) * sizeof(int)));
     BR(r.l1.reserve((size_t)std::max<int64_t>(1, lo) * sizeof(int)));
-    BR(r.cb.reserve((size_t)std::max<int64_t>(1, cbo) * 2));
+    BR(r.cb.reserve((size_t)std::max<int64_t>(1, cbo) * esz));
     BR(r.chalf.reserve((size_t)n_inst * 64 * sizeof(float)));
     BR(r.cmax2.reserve((size_t)n_inst * sizeof(double)));
     BR(r.list_count.reserve(sizeof(int)));
@@ -152,12 +164,13 @@ da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, co
                         r.dcoff.as<int64_t>(), r.dcboff.as<int64_t>(), r.dloff.as<int64_t>()};
     KBatchBuffers &w = r.w;
     w.X = X;
-    w.Xb = r.xb.as<uint16_t>();
+    w.prec = prec;
+    w.Xo = r.xb.ptr();
     w.C0 = r.c0.as<double>();
     w.C1 = r.c1.as<double>();
     w.L0 = r.l0.as<int>();
     w.L1 = r.l1.as<int>();
-    w.Cb = r.cb.as<uint16_t>();
+    w.Cb = r.cb.ptr();
     w.chalf = r.chalf.as<float>();
     w.cmax2 = r.cmax2.as<double>();
     w.chunks = r.chunks.as<KChunk>();
@@ -190,10 +203,10 @@ da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, co
 // Assignment tiles of 64*pt samples, instances in decreasing order of per-sample cost
 // (k * kp) and size, so that the persistent grid ends on small tiles.
 std::vector<KTile> make_tiles(int n_inst, const da_int *n, const da_int *d, const da_int *k,
-                              int pt) {
+                              int pt, int prec) {
     std::vector<int> order(n_inst);
     std::iota(order.begin(), order.end(), 0);
-    auto cost = [&](int i) { return (int64_t)n[i] * k[i] * (d[i] <= 32 ? 32 : 64); };
+    auto cost = [&](int i) { return (int64_t)n[i] * k[i] * op_width(prec, d[i]); };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
     std::vector<KTile> tiles;
     const int sb = 64 * pt;
@@ -219,7 +232,8 @@ da_status enqueue_run(BatchRun &r, const float *Cinit, float *Cout, int *labels,
     hipStream_t st = r.gpu.stream();
     hipError_t e = kb_launch_prepare(r.w.X, Cinit, r.desc, r.n_inst, r.w.chunks, r.w.nchunks,
                                      r.ctotal, r.amax.as<unsigned long long>(), r.w.escale,
-                                     r.w.Xb, r.w.C0, r.w.state, r.w.fpar, r.w.n_iter, st);
+                                     r.w.prec, r.w.Xo, r.w.C0, r.w.state, r.w.fpar, r.w.n_iter,
+                                     st);
     for (int it = 0; it < max_iter && e == hipSuccess; it++) {
         const int par = it & 1;
         e = kb_launch_assign(r.w, r.desc, r.n_inst, 0, par, r.pt, r.grid, st);
@@ -243,7 +257,8 @@ da_status tune(BatchRun &r, const da_int *n, const da_int *d, const da_int *k, c
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     hipError_t e = kb_launch_prepare(r.w.X, Cinit, r.desc, r.n_inst, r.w.chunks, r.w.nchunks,
                                      r.ctotal, r.amax.as<unsigned long long>(), r.w.escale,
-                                     r.w.Xb, r.w.C0, r.w.state, r.w.fpar, r.w.n_iter, st);
+                                     r.w.prec, r.w.Xo, r.w.C0, r.w.state, r.w.fpar, r.w.n_iter,
+                                     st);
     if (e != hipSuccess)
         return hip_status(&g_err, e, "batched k-means preparation");
     hipEvent_t a, b;
@@ -252,11 +267,11 @@ da_status tune(BatchRun &r, const da_int *n, const da_int *d, const da_int *k, c
     float best = 0;
     int best_pt = 4;
     for (int pt : {2, 4}) {
-        std::vector<KTile> tiles = make_tiles(r.n_inst, n, d, k, pt);
+        std::vector<KTile> tiles = make_tiles(r.n_inst, n, d, k, pt, r.w.prec);
         da_status s = set_tiles(r, tiles);
         if (s != da_status_success)
             return s;
-        const int grid = std::max(1, cus * kb_assign_blocks_per_cu(pt));
+        const int grid = std::max(1, cus * kb_assign_blocks_per_cu(r.w.prec, pt));
         (void)kb_launch_assign(r.w, r.desc, r.n_inst, 0, 0, pt, grid, st); // warm-up
         (void)hipEventRecord(a, st);
         for (int rep = 0; rep < 3; rep++)
@@ -273,8 +288,8 @@ da_status tune(BatchRun &r, const da_int *n, const da_int *d, const da_int *k, c
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     r.pt = best_pt;
-    r.grid = std::max(1, cus * kb_assign_blocks_per_cu(best_pt));
-    return set_tiles(r, make_tiles(r.n_inst, n, d, k, best_pt));
+    r.grid = std::max(1, cus * kb_assign_blocks_per_cu(r.w.prec, best_pt));
+    return set_tiles(r, make_tiles(r.n_inst, n, d, k, best_pt, r.w.prec));
 }
 
 } // namespace
@@ -283,12 +298,13 @@ extern "C" {
 
 da_status da_kmeans_batch_device_s(da_int n_inst, const da_int *n, const da_int *d,
                                    const da_int *k, const float *X, float *C, da_int *labels,
-                                   da_int max_iter, float tol, da_int *n_iter, float *inertia) {
+                                   da_int max_iter, float tol, da_int cross_term,
+                                   da_int *n_iter, float *inertia) {
     if (n_inst < 1)
         return da_status_invalid_array_dimension;
     if (!n || !d || !k || !X || !C || !labels || !n_iter || !inertia)
         return da_status_invalid_pointer;
-    if (max_iter < 1 || !(tol >= 0.0f))
+    if (max_iter < 1 || !(tol >= 0.0f) || cross_term < 0 || cross_term > 1)
         return da_status_invalid_input;
     for (da_int i = 0; i < n_inst; i++) {
         if (n[i] < 1 || d[i] < 1 || d[i] > 64 || k[i] < 1 || k[i] > 64 || k[i] > n[i])
@@ -296,7 +312,8 @@ da_status da_kmeans_batch_device_s(da_int n_inst, const da_int *n, const da_int 
     }
     std::lock_guard<std::mutex> g(g_mu);
     BatchKey key{std::vector<int>(n, n + n_inst), std::vector<int>(d, d + n_inst),
-                 std::vector<int>(k, k + n_inst), X, C, labels, (int)max_iter, (double)tol};
+                 std::vector<int>(k, k + n_inst), X, C, labels, (int)max_iter, (int)cross_term,
+                 (double)tol};
     auto it = g_runs.find(key);
     BatchRun *r = nullptr;
     if (it == g_runs.end()) {
@@ -305,7 +322,7 @@ da_status da_kmeans_batch_device_s(da_int n_inst, const da_int *n, const da_int 
         if (s != da_status_success)
             return s;
         // Chunks of about 16 K samples (the centroid partials of a chunk are k*d*16 bytes).
-        s = build(*run, n_inst, n, d, k, X, 16384);
+        s = build(*run, n_inst, n, d, k, X, 16384, (int)cross_term);
         if (s != da_status_success)
             return s;
         s = tune(*run, n, d, k, C);

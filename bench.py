@@ -34,6 +34,25 @@ DTYPE_NAME = {"working": "f32", "fp32": "f32", "fp16": "f16", "bf16": "bf16", "f
               "bf16x3": "bf16"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "knn_tile_200000x128_20000q_k10_traffic.txt")
+FORM_KEY = {"working": "fp32", "fp32": "fp32"}
+
+
+def measured_traffic(form):
+    """HBM bytes per knn_tile launch of this form from the committed rocprofv3 counter
+    summary (two counter passes calibrated with a copy of known size), or None."""
+    key = FORM_KEY.get(form, form)
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def mixture(rng, n, means, sigma=1.0, dtype=np.float32):
     comp = rng.integers(0, len(means), n)
     return (means[comp] + sigma * rng.standard_normal((n, means.shape[1]))).astype(dtype)
@@ -312,45 +331,55 @@ def kmeans_batch_bench(args, L, aoclda):
     it = np.zeros(ni, dtype=np.int32)
     inert = np.zeros(ni, dtype=np.float32)
 
-    def run():
-        L.check(L.lib.da_amd_memcpy_htod(dc, L.ptr(C0), C0.nbytes))
-        L.check(L.lib.da_amd_device_synchronize())
-        t0 = time.perf_counter()
-        L.check(L.lib.da_kmeans_batch_device_s(ni, L.ptr(n), L.ptr(d), L.ptr(k), dx, dc, dl,
-                                               args.kb_iter, 0.0, L.ptr(it), L.ptr(inert)))
-        return time.perf_counter() - t0
+    nsum = int(n.sum())
+    for form in args.kb_forms.split(","):
+        ct = {"working": 0, "fp32": 0, "bf16": 1}[form]
+        esz = 2 if ct == 1 else 4
+        kp = np.where(d <= 32, 32, 64) if ct == 1 else np.maximum(8, 2 ** np.ceil(
+            np.log2(d)).astype(np.int64))
 
-    first = run()
-    times = [run() for _ in range(args.kb_reps)]
-    tile = L.da_int()
-    L.check(L.lib.da_kmeans_batch_tile_shape(C.byref(tile)))
-    L.check(L.lib.da_kmeans_batch_release())
+        def run():
+            L.check(L.lib.da_amd_memcpy_htod(dc, L.ptr(C0), C0.nbytes))
+            L.check(L.lib.da_amd_device_synchronize())
+            t0 = time.perf_counter()
+            L.check(L.lib.da_kmeans_batch_device_s(ni, L.ptr(n), L.ptr(d), L.ptr(k), dx, dc,
+                                                   dl, args.kb_iter, 0.0, ct, L.ptr(it),
+                                                   L.ptr(inert)))
+            return time.perf_counter() - t0
+
+        first = run()
+        times = [run() for _ in range(args.kb_reps)]
+        tile = L.da_int()
+        L.check(L.lib.da_kmeans_batch_tile_shape(C.byref(tile)))
+        L.check(L.lib.da_kmeans_batch_release())
+        t = float(np.median(times))
+        iters = int(it.sum())
+        # Algorithmic bytes of one iteration: operand rows read by the assignment, fp32 rows
+        # read by the centroid sums, labels written once and read twice.
+        byts = float((n * kp * esz).sum() + (n * d * 4).sum() + 12 * nsum)
+        flops = float((2 * n.astype(np.int64) * k * d).sum())
+        mean_it = iters / ni
+        per_it = t / max(1.0, mean_it)
+        print(json.dumps({
+            "metric": BATCH_METRIC, "value": iters / t, "unit": "instance-iterations/s",
+            "n_gpus": 1, "higher_is_better": True, "ms_per_batch": t * 1e3,
+            "first_call_ms": first * 1e3, "reps": args.kb_reps,
+            "dtype": "bf16" if ct == 1 else "f32",
+            "data": "synthetic: Gaussian blobs per instance, numpy default_rng(5)",
+            "config": {"workload": f"{ni} k-means instances, n log-uniform [1e3,2e5], d in "
+                                   f"{{8,16,32,64}}, k in [4,64], {args.kb_iter} Lloyd "
+                                   "iterations, tol 0", "total_samples": nsum,
+                       "cross_term": form},
+            "samples_per_tile": tile.value, "mean_iterations": mean_it,
+            "per_iteration_ms": per_it * 1e3,
+            "roofline": {"kernel": "whole batch iteration", "bound": "hbm",
+                         "achieved": byts / per_it / 1e9, "peak": 8000.0, "unit": "GB/s",
+                         "frac": byts / per_it / 8e12, "traffic": None,
+                         "algorithmic_bytes_per_iteration": byts,
+                         "flops_per_iteration": flops},
+            "inertia_sum": float(inert.astype(np.float64).sum())}), flush=True)
     for p in (dx, dc, dl):
         L.lib.da_amd_free(p)
-    t = float(np.median(times))
-    iters = int(it.sum())
-    kp = np.where(d <= 32, 32, 64)
-    # Algorithmic bytes of one iteration: bf16 rows read by the assignment, fp32 rows read by
-    # the centroid sums, labels written once and read twice.
-    byts = float((n * kp * 2).sum() + (n * d * 4).sum() + 12 * n.sum())
-    flops = float((2 * n.astype(np.int64) * k * d).sum())
-    mean_it = iters / ni
-    print(json.dumps({
-        "metric": BATCH_METRIC, "value": iters / t, "unit": "instance-iterations/s",
-        "n_gpus": 1, "higher_is_better": True, "ms_per_batch": t * 1e3,
-        "first_call_ms": first * 1e3, "reps": args.kb_reps, "dtype": "bf16",
-        "data": "synthetic: Gaussian blobs per instance, numpy default_rng(5)",
-        "config": {"workload": f"{ni} k-means instances, n log-uniform [1e3,2e5], d in "
-                               f"{{8,16,32,64}}, k in [4,64], {args.kb_iter} Lloyd iterations, "
-                               "tol 0", "total_samples": int(n.sum())},
-        "samples_per_tile": tile.value, "mean_iterations": mean_it,
-        "per_iteration_ms": t * 1e3 / max(1.0, mean_it),
-        "roofline": {"kernel": "whole batch iteration", "bound": "hbm",
-                     "achieved": byts / (t / max(1.0, mean_it)) / 1e9, "peak": 8000.0,
-                     "unit": "GB/s", "frac": byts / (t / max(1.0, mean_it)) / 8e12,
-                     "traffic": None, "algorithmic_bytes_per_iteration": byts,
-                     "flops_per_iteration": flops},
-        "inertia_sum": float(inert.astype(np.float64).sum())}))
 
 
 def main():
@@ -363,6 +392,8 @@ def main():
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--forms", default="working,fp16x3,bf16x3")
+    ap.add_argument("--kb-forms", default="working,bf16",
+                    help="cross-term forms of the batched k-means mode")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="fused tile kernel variant (-1: tuning table)")
     ap.add_argument("--cpu-sample", type=int, default=1000,
@@ -467,7 +498,7 @@ def main():
         issued = achieved * PRODUCTS.get(head, 1)
         roofline = {"kernel": "knn_tile", "bound": "mfma", "achieved": achieved / 1e12,
                     "peak": PEAK[head] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[head],
-                    "traffic": None, "avg_launch_ms": per_launch * 1e3,
+                    "traffic": measured_traffic(head), "avg_launch_ms": per_launch * 1e3,
                     "algorithmic_flops_per_launch": flops,
                     "issued_mfma_tflops": issued / 1e12, "issued_frac": issued / PEAK[head]}
     forms = {}

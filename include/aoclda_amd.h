@@ -111,14 +111,17 @@ void da_nn_server_destroy(da_nn_server *server);
  * Device arrays, instances packed one after another: X holds the rows (row-major, d[i]
  * columns), C holds the initial centres (k[i] x d[i], row-major) on entry and the final
  * centres on return, labels receives n[i] labels per instance. Host arrays n_iter and
- * inertia receive the iteration count and inertia of each instance. The cross term runs in
- * bf16 on MFMA with an exact double re-check of near ties, so the labels are exact.
+ * inertia receive the iteration count and inertia of each instance. cross_term selects the
+ * MFMA operand type of the assignment: 0 = fp32, 1 = bf16; either way samples whose two
+ * nearest centres are within the error bound are re-checked in double, so the labels are
+ * exact.
  * The first call for a batch layout tunes the tile shape and captures the run in a HIP graph;
  * later calls with the same sizes, pointers, max_iter and tol replay it.
  */
 da_status da_kmeans_batch_device_s(da_int n_inst, const da_int *n, const da_int *d,
                                    const da_int *k, const float *X, float *C, da_int *labels,
-                                   da_int max_iter, float tol, da_int *n_iter, float *inertia);
+                                   da_int max_iter, float tol, da_int cross_term,
+                                   da_int *n_iter, float *inertia);
 /* Samples per assignment tile chosen by the tuner for the last batch run (0 if none). */
 da_status da_kmeans_batch_tile_shape(da_int *samples_per_tile);
 /* Releases the device buffers and graphs kept for batch layouts. */

@@ -201,9 +201,11 @@ def test_bf16_cross_term_near_ties(oracle):
     np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-5, atol=1e-5)
 
 
-def test_batch_matches_single_instances(oracle):
+@pytest.mark.parametrize("precision", ["working", "bf16"])
+def test_batch_matches_single_instances(oracle, precision):
     """Batched k-means over uneven instances equals each instance run alone (oracle with
-    supplied centres): labels, iteration counts, centres and inertia."""
+    supplied centres): labels, iteration counts, centres and inertia, for both operand forms
+    of the assignment cross term."""
     rng = np.random.default_rng(50)
     datasets, inits, cfg = [], [], []
     for i in range(14):
@@ -216,7 +218,8 @@ def test_batch_matches_single_instances(oracle):
         inits.append(X[:k].copy())
         cfg.append((n, d, k))
     for max_iter, tol in ((12, 0.0), (40, 1e-4)):
-        cs, labs, n_iter, inertia = aoclda.clustering.kmeans_batch(datasets, inits, max_iter, tol)
+        cs, labs, n_iter, inertia = aoclda.clustering.kmeans_batch(datasets, inits, max_iter, tol,
+                                                                   precision)
         for i, X in enumerate(datasets):
             r = oracle.kmeans(X.astype(np.float64), cfg[i][2], "supplied",
                               inits[i].astype(np.float64), max_iter=max_iter, tol=tol)
@@ -233,3 +236,5 @@ def test_batch_argument_errors():
     with pytest.raises(aoclda.DAError):
         aoclda.clustering.kmeans_batch([np.zeros((10, 65), dtype=np.float32)],
                                        [np.zeros((2, 65), dtype=np.float32)])
+    with pytest.raises(KeyError):
+        aoclda.clustering.kmeans_batch([X], [X[:2].copy()], precision="fp8")
