@@ -43,11 +43,26 @@ constexpr int STEP = COLT * 16;           // training rows per step
 // Tile shapes per candidate-list length KC: buffer capacity CAP per query, waves per
 // workgroup WV and 16-query row tiles per wave RT. The LDS buffers (WV*RT*16*CAP*8 bytes)
 // stay at or below 64 KB so that two workgroups fit a CU.
+// Candidate-buffer shape per candidate-list length: CAP entries per query, WV waves of RT
+// row tiles (16 queries each) per workgroup.
 template <int KC> struct TileShape;
 template <> struct TileShape<16> { static constexpr int CAP = 64, WV = 4, RT = 2; };
 template <> struct TileShape<32> { static constexpr int CAP = 128, WV = 4, RT = 1; };
 template <> struct TileShape<64> { static constexpr int CAP = 128, WV = 4, RT = 1; };
 template <> struct TileShape<128> { static constexpr int CAP = 256, WV = 2, RT = 1; };
+
+// Whether a sweep keeps its buffered row offsets in 16 bits (splits of at most 65,536 rows):
+// that cuts a workgroup's buffers to 48 KB, so three workgroups fit a CU's LDS, which pays
+// only where the registers also leave room for three waves per SIMD -- the query fragments
+// plus the prefetched training fragments (ns * nf fragments of frag_regs 32-bit registers,
+// for rt query tiles and 2 x 2 training tiles) within 144 of the 168 registers a wave then
+// gets (the sweep's other state takes about 24). kc <= 32; the fp64 sweep runs one wave per
+// SIMD and never qualifies. Evaluated by the kernels and by the host's bound on the split
+// count.
+constexpr bool tile_short_index(int kc, int rt, int ns, int nf, int frag_regs, bool f64) {
+    return !f64 && kc <= 32 && ns * nf * frag_regs * (rt + 2 * 2) <= 144;
+}
+constexpr int SPLIT16_MAX = 65536; // rows per split with 16-bit offsets
 
 // Value of lane (lane ^ S) for S in {1, 2, 4, 8, 16, 32}, by register-to-register lane
 // moves (DPP within rows of 16, the gfx950 permlane swaps across rows) instead of the LDS
@@ -109,25 +124,25 @@ __device__ __forceinline__ void wave_sort64(float &key, int &idx) {
 // (key, index) and returns the new threshold: the KC-th key, or +inf while fewer than KC
 // entries have been seen. Wave-uniform; not inlined, so that the sweep's many call sites
 // share one copy.
-template <int KC, int CAP>
-__device__ __attribute__((noinline)) float compact_count(float *bkey, int *bidx, int c, int q) {
+template <int KC, int CAP, typename IT>
+__device__ __attribute__((noinline)) float compact_count(float *bkey, IT *bidx, int c, int q) {
     const int lane = threadIdx.x & 63;
     float *kk = bkey + q * CAP;
-    int *ii = bidx + q * CAP;
+    IT *ii = bidx + q * CAP;
     if constexpr (CAP == 64) {
         float key = lane < c ? kk[lane] : __builtin_huge_valf();
         int idx = lane < c ? ii[lane] : INT_MAX;
         wave_sort64(key, idx);
         if (lane < KC) {
             kk[lane] = key;
-            ii[lane] = idx;
+            ii[lane] = (IT)idx;
         }
         const float kth = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(key), KC - 1));
         return c >= KC ? kth : __builtin_huge_valf();
     } else {
         for (int t = c + lane; t < CAP; t += 64) {
             kk[t] = __builtin_huge_valf();
-            ii[t] = INT_MAX;
+            ii[t] = (IT)INT_MAX;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -138,7 +153,7 @@ __device__ __attribute__((noinline)) float compact_count(float *bkey, int *bidx,
                     const int j = i + stride;
                     const bool asc = (i & size) == 0;
                     const float ki = kk[i], kj = kk[j];
-                    const int xi = ii[i], xj = ii[j];
+                    const IT xi = ii[i], xj = ii[j];
                     const bool gt = (kj < ki) || (kj == ki && xj < xi);
                     if (gt == asc) {
                         kk[i] = kj;
@@ -238,16 +253,17 @@ __device__ __forceinline__ void wave_fence() {
 }
 
 // Candidate bookkeeping of one wave after a step. Lane (r, g) holds, for row tile rt and
-// accumulator element i, the scores of query rt*16 + row(g, i) against training rows
-// tb + ct*16 + r. Scores above the query's threshold are appended to its LDS buffer at
+// accumulator element i, the scores of query rt*16 + row(g, i) against the training rows at
+// offsets tb + ct*16 + r of the split. Scores above the query's threshold are appended to its
+// LDS buffer at
 // positions from a ballot: the 16 lanes of a lane group serve one query per (rt, i), so a
 // hit's slot is the group's running count (cq, held in registers by the group's lanes)
 // plus the number of hits of the lower lanes of the group. A buffer with more than
 // CAP - STEP entries is compacted to its KC best, which sets the new threshold.
-template <class Op, int KC, int CAP, int ROWT>
+template <class Op, int KC, int CAP, int ROWT, typename IT>
 __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][COLT],
                                           typename Op::acc_e (&ts)[ROWT][4],
-                                          int (&cq)[ROWT][4], float *bkey, int *bidx,
+                                          int (&cq)[ROWT][4], float *bkey, IT *bidx,
                                           int qbase, int tb) {
     const int lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
@@ -262,14 +278,15 @@ __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][
 #pragma unroll
             for (int ct = 0; ct < COLT; ct++) {
                 const bool h = acc[rt][ct][i] > ts[rt][i];
-                const unsigned long long m = __ballot(h);
+                // The compare's own lane mask (no materialised boolean).
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(h);
                 if (m == 0)
                     continue;
                 const unsigned long long mg = m & gmask;
                 if (h) {
                     const int pos = cq[rt][i] + __popcll(mg & below);
                     bkey[q * CAP + pos] = (float)(-2.0 * (double)acc[rt][ct][i]);
-                    bidx[q * CAP + pos] = tb + ct * 16 + r;
+                    bidx[q * CAP + pos] = (IT)(tb + ct * 16 + r);
                 }
                 cq[rt][i] += __popcll(mg);
             }
@@ -281,7 +298,7 @@ __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][
 #pragma unroll
         for (int i = 0; i < 4; i++)
             need |= cq[rt][i] > CAP - STEP;
-    if (__ballot(need) == 0)
+    if (__builtin_amdgcn_ballot_w64(need) == 0)
         return;
     wave_fence();
 #pragma unroll
@@ -289,13 +306,15 @@ __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             // one representative lane (r = 0) per group
-            unsigned long long m2 = __ballot(cq[rt][i] > CAP - STEP) & 0x0001000100010001ull;
+            unsigned long long m2 =
+                __builtin_amdgcn_ballot_w64(cq[rt][i] > CAP - STEP) & 0x0001000100010001ull;
             while (m2) {
                 const int l0 = __ffsll((long long)m2) - 1;
                 m2 &= m2 - 1;
                 const int gg = l0 >> 4;
                 const int c = __builtin_amdgcn_readlane(cq[rt][i], l0);
-                const float t = compact_count<KC, CAP>(bkey, bidx, c, qbase + rt * 16 + row_of(i, gg));
+                const float t =
+                    compact_count<KC, CAP>(bkey, bidx, c, qbase + rt * 16 + row_of(i, gg));
                 if (g == gg) {
                     cq[rt][i] = c < KC ? c : KC;
                     ts[rt][i] = (typename Op::acc_e)(-0.5 * (double)t);
@@ -307,10 +326,12 @@ __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][
 }
 
 // Final compaction of a wave's queries and the per-(query, split) output: the KC best keys
-// and indices and the split threshold tau (every key of the split not kept is >= tau).
-template <class Op, int KC, int CAP, int ROWT>
-__device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bkey, int *bidx,
-                                            int qbase, int q0, int nq, int nsplit, int split,
+// and row indices (t_begin + buffered offset) and the split threshold tau (every key of the
+// split not kept is >= tau).
+template <class Op, int KC, int CAP, int ROWT, typename IT>
+__device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bkey, IT *bidx,
+                                            int t_begin, int qbase, int q0, int nq, int nsplit,
+                                            int split,
                                             float *__restrict__ part_key,
                                             int *__restrict__ part_idx,
                                             float *__restrict__ part_tau) {
@@ -331,7 +352,7 @@ __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bke
                 const int valid = c < KC ? c : KC;
                 for (int u = lane; u < KC; u += 64) {
                     part_key[o + u] = u < valid ? bkey[ql * CAP + u] : __builtin_huge_valf();
-                    part_idx[o + u] = u < valid ? bidx[ql * CAP + u] : -1;
+                    part_idx[o + u] = u < valid ? t_begin + (int)bidx[ql * CAP + u] : -1;
                 }
                 if (lane == 0)
                     part_tau[(int64_t)q * nsplit + split] = t;
@@ -339,6 +360,16 @@ __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bke
         }
     }
 }
+
+// Per-kernel configuration: 16-bit buffered offsets and three waves per SIMD, or 32-bit
+// offsets and two.
+template <class Op, int KCH, int KC, bool SPLIT> struct TileCfg {
+    static constexpr bool SHORT =
+        tile_short_index(KC, TileShape<KC>::RT, SPLIT ? 2 : 1, KCH / Op::E,
+                         (int)sizeof(typename Op::frag_t) / 4, std::is_same_v<Op, OpF64>);
+    static constexpr int WPE = SHORT ? 3 : 2;
+    using idx_t = std::conditional_t<SHORT, unsigned short, int>;
+};
 
 // Sweep of one training split for 128 queries. Scores are x.y - |y|^2/2 (the accumulator
 // starts at -|y|^2/2), so a larger score is a nearer row; keys -2*score are what the
@@ -364,8 +395,9 @@ __device__ __forceinline__ void knn_tile_body(
     constexpr int CAP = TileShape<KC>::CAP;
     constexpr int ROWT = TileShape<KC>::RT;
     constexpr int QB = TileShape<KC>::WV * ROWT * 16;
+    using idx_t = typename TileCfg<Op, KCH, KC, SPLIT>::idx_t;
     __shared__ float bkey[QB * CAP];
-    __shared__ int bidx[QB * CAP];
+    __shared__ idx_t bidx[QB * CAP];
 
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
@@ -462,9 +494,9 @@ __device__ __forceinline__ void knn_tile_body(
 #pragma unroll
                 for (int ct = 0; ct < COLT; ct++)
                     hit |= acc[rt][ct][i] > ts[rt][i];
-        if (__ballot(hit) == 0)
+        if (__builtin_amdgcn_ballot_w64(hit) == 0)
             return;
-        take_hits<Op, KC, CAP, ROWT>(acc, ts, cq, bkey, bidx, qbase, tb);
+        take_hits<Op, KC, CAP, ROWT>(acc, ts, cq, bkey, bidx, qbase, tb - t_begin);
     };
 
     if constexpr (PF) {
@@ -495,12 +527,13 @@ __device__ __forceinline__ void knn_tile_body(
             check_step(tb, acc);
         }
     }
-    write_split<Op, KC, CAP, ROWT>(cq, bkey, bidx, qbase, q0, nq, nsplit, split, part_key,
+    write_split<Op, KC, CAP, ROWT>(cq, bkey, bidx, t_begin, qbase, q0, nq, nsplit, split,
+                                   part_key,
                                    part_idx, part_tau);
 }
 
-// The sweep as a kernel: K2 asks the compiler for at most 256 registers per lane, two waves
-// per SIMD (the LDS of two workgroups fits a CU); the fp64 operands need more registers
+// The sweep as a kernel: K2 asks the compiler for registers that leave room for WPE waves
+// per SIMD (as many workgroups as fit a CU's LDS); the fp64 operands need more registers
 // and run one wave per SIMD without spilling.
 template <class Op, int KCH, int KC, bool SPLIT, bool PF>
 __global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
@@ -512,7 +545,7 @@ __global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
 }
 template <class Op, int KCH, int KC, bool SPLIT, bool PF>
 __global__ __launch_bounds__(64 * TileShape<KC>::WV)
-__attribute__((amdgpu_waves_per_eu(2))) void knn_tile_kernel_k2(
+__attribute__((amdgpu_waves_per_eu(TileCfg<Op, KCH, KC, SPLIT>::WPE))) void knn_tile_kernel_k2(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
     float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
@@ -719,6 +752,17 @@ int knn_tile_qb(int kc) {
 
 int knn_merge_max_candidates() { return MMAX; }
 
+int knn_tile_min_splits(int prec, int Kp, int n, int kc) {
+    const bool split = prec == KP_F16X3 || prec == KP_BF16X3;
+    const bool wide = prec == KP_F16 || prec == KP_BF16 || split;   // 8 elements, 4 registers
+    const int frag_regs = wide ? 4 : prec == KP_F64 ? 2 : 1;
+    const int nf = wide ? Kp / 4 / 8 : Kp / 4;
+    const int rt = kc <= 16 ? TileShape<16>::RT : 1;
+    if (!tile_short_index(kc, rt, split ? 2 : 1, nf, frag_regs, prec == KP_F64))
+        return 1;
+    return (n + SPLIT16_MAX - 1) / SPLIT16_MAX;
+}
+
 
 // Launches the fused tile for one operand type, candidate-list length, width and variant
 // (0: loads of the next step under the current step's bookkeeping, 1: load then compute
@@ -753,6 +797,9 @@ static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void
     }
     int split_len = (n + nsplit - 1) / nsplit;
     split_len = (split_len + STEP - 1) / STEP * STEP;
+    // Buffered row offsets must fit the index type.
+    if (TileCfg<Op, KCH, KC, SPLIT>::SHORT && split_len > SPLIT16_MAX)
+        return hipErrorInvalidValue;
     const int qblocks = (nq + QB - 1) / QB;
     const dim3 grid((qblocks * nsplit + 7) / 8 * 8);
     hipLaunchKernelGGL(kern, grid, block, 0, s, static_cast<const in_t *>(Q), nq,

@@ -68,7 +68,16 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
     // 0 = next step's loads under the current step's bookkeeping, 1 = load then compute).
     opts.add_int("tile variant", "Kernel variant of the fused kNN distance tile (-1 automatic).",
                  -1, LowerBound::greater_equal, 1, UpperBound::less_equal, -1);
+    // MI355X extension: minimum length of the candidate list each split of the fused tile
+    // keeps per query (rounded up to 16, 32, 64 or 128; 0 = the smallest that holds k). A
+    // longer list lowers the split thresholds, so a low-precision form with a wider error
+    // bound still certifies its queries instead of sending them to the exact fallback.
+    opts.add_int("candidates per split",
+                 "Minimum candidates kept per query and split by the fused kNN tile (0 = k).", 0,
+                 LowerBound::greater_equal, 128, UpperBound::less_equal, 0);
 }
+
+int SearchParams::tile_kc(int k) const { return knn_tile_kc(k > kc_min ? k : kc_min); }
 
 // Tuning table of the fused tile variant per operand precision and padded width: the
 // pipelined sweep (variant 0) for every form.
@@ -179,6 +188,13 @@ da_status NNHandle<T>::set_data(da_int n_samples, da_int n_features, const T *X,
 
 // Squared-euclidean metrics are searched by the fused MFMA kernel (up to 128 features, k up
 // to 128); everything else uses the key-tile + selection path.
+// Most training splits of one fused search: the merge takes nsplit * kc candidates per
+// query, and a split has at least 1,024 rows.
+template <typename T> int NNHandle<T>::split_cap(int kc) const {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(knn_merge_max_candidates() / kc,
+                                                       ((int64_t)n_samples_ + 1023) / 1024));
+}
+
 template <typename T> bool NNHandle<T>::fused_eligible(da_int k) const {
     if (!fused_ready_)
         return false;
@@ -245,6 +261,7 @@ template <typename T> SearchParams NNHandle<T>::params() const {
     SearchParams p;
     p.prec = tile_prec();
     p.variant = opts.ival("tile variant");
+    p.kc_min = opts.ival("candidates per split");
     return p;
 }
 
@@ -285,9 +302,10 @@ int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, 
     const int64_t n = n_samples_;
     const int64_t qb = knn_tile_qb(kc);
     const int64_t qblocks = (nq + qb - 1) / qb;
-    const int64_t hi = std::max<int64_t>(
-        1, std::min<int64_t>(knn_merge_max_candidates() / kc, (n + 1023) / 1024));
-    const int64_t s1 = std::min<int64_t>(hi, std::max<int64_t>(1, (slots * 9 / 10 + qblocks - 1) / qblocks));
+    const int64_t hi = split_cap(kc);
+    const int64_t lo = knn_tile_min_splits(prec, kp_, (int)n, kc);
+    const int64_t s1 = std::min<int64_t>(
+        hi, std::max<int64_t>(lo, (slots * 9 / 10 + qblocks - 1) / qblocks));
     int bucket = 0;
     while (((int64_t)1 << (bucket + 1)) <= qblocks)
         bucket++;
@@ -348,7 +366,9 @@ template <typename T>
 da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp, const T *dQ,
                                      da_int nq, da_int k, bool return_distance, da_int *d_ind,
                                      T *d_dist, bool &fused) {
-    fused = fused_eligible(k) && sp.prec == ypad_prec_;
+    const int kc = sp.tile_kc(k);
+    fused = fused_eligible(k) && sp.prec == ypad_prec_ &&
+            knn_tile_min_splits(sp.prec, kp_, n_samples_, kc) <= split_cap(kc);
     ctx.last_fallback = 0;
     ctx.last_candidates = 0;
     if (!fused)
@@ -356,7 +376,6 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
     hipStream_t st = ctx.gpu->stream();
     const int n = n_samples_, d = n_features_;
     const int prec = sp.prec;
-    const int kc = knn_tile_kc(k);
     if (!ctx.h_fb)
         DA_HIP(err, hipHostMalloc((void **)&ctx.h_fb, sizeof(int)));
     int variant = sp.variant;

@@ -26,6 +26,9 @@ struct SearchContext {
 struct SearchParams {
     int prec = 0;     // operand precision of the fused tile (KP_*)
     int variant = -1; // fused tile kernel variant (-1: tuning table)
+    int kc_min = 0;   // minimum candidates kept per (query, split) on the fused tile
+    // Candidate-list length of the fused tile for k neighbours.
+    int tile_kc(int k) const;
 };
 
 template <typename T> class NNHandle : public AlgHandle {
@@ -69,6 +72,7 @@ template <typename T> class NNHandle : public AlgHandle {
     da_status search_tiles(SearchContext &ctx, const T *dQ, da_int nq, da_int k,
                            bool return_distance, da_int *d_ind, T *d_dist);
     bool fused_eligible(da_int k) const;
+    int split_cap(int kc) const;
     int tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq,
                     const void *qpad, hipStream_t st);
     da_status prepare_fused(int prec);

@@ -227,10 +227,10 @@ template <typename T> class Server : public ServerBase {
         k = 1;
         if (queue_.empty())
             return false;
-        const int kc = knn_tile_kc(queue_.front()->k);
+        const int kc = params_.tile_kc(queue_.front()->k);
         for (auto it = queue_.begin(); it != queue_.end() && nq < batch_cap_;) {
             Request *r = *it;
-            if (knn_tile_kc(r->k) > kc) {
+            if (params_.tile_kc(r->k) > kc) {
                 ++it;
                 continue;
             }

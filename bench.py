@@ -38,6 +38,13 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "knn_tile_200000x128_20000q_k10_tr
 FORM_KEY = {"working": "fp32", "fp32": "fp32"}
 
 
+def split_form(form):
+    """A form is a precision, optionally followed by /N: the minimum candidate-list length
+    per query and split of the fused tile (e.g. fp16/32)."""
+    prec, _, kc = form.partition("/")
+    return prec, int(kc) if kc else 0
+
+
 def measured_traffic(form):
     """HBM bytes per knn_tile launch of this form from the committed rocprofv3 counter
     summary (two counter passes calibrated with a copy of known size), or None."""
@@ -63,8 +70,9 @@ def log(msg):
 
 
 def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1):
-    nn = aoclda.neighbors.nearest_neighbors(n_neighbors=k, precision=precision,
-                                            tile_variant=variant).fit(Xt)
+    prec, kc_min = split_form(precision)
+    nn = aoclda.neighbors.nearest_neighbors(n_neighbors=k, precision=prec, tile_variant=variant,
+                                            candidates_per_split=kc_min).fit(Xt)
     h = nn.handle
     di, dd = C.c_void_p(), C.c_void_p()
     L.check(L.lib.da_amd_malloc(nq * k * 4, C.byref(di)))
@@ -495,12 +503,13 @@ def main():
         tot_ms, cnt = H["kernels"]["knn_tile"]
         per_launch = tot_ms / cnt * 1e-3
         achieved = flops / per_launch  # one tile launch per step covers the whole batch
-        issued = achieved * PRODUCTS.get(head, 1)
+        hp = split_form(head)[0]
+        issued = achieved * PRODUCTS.get(hp, 1)
         roofline = {"kernel": "knn_tile", "bound": "mfma", "achieved": achieved / 1e12,
-                    "peak": PEAK[head] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[head],
+                    "peak": PEAK[hp] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[hp],
                     "traffic": measured_traffic(head), "avg_launch_ms": per_launch * 1e3,
                     "algorithmic_flops_per_launch": flops,
-                    "issued_mfma_tflops": issued / 1e12, "issued_frac": issued / PEAK[head]}
+                    "issued_mfma_tflops": issued / 1e12, "issued_frac": issued / PEAK[hp]}
     forms = {}
     for f, r in results.items():
         forms[f] = {k2: r[k2] for k2 in ("qps", "ms_per_step", "p50_ms", "p99_ms", "fallback",
@@ -512,7 +521,7 @@ def main():
         "metric": METRIC, "value": H["qps"], "unit": "queries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": H["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": DTYPE_NAME.get(head, head),
+        "dtype": DTYPE_NAME.get(split_form(head)[0], head),
         "data": "synthetic: 64-component isotropic Gaussian mixture (means U[-10,10]^128, "
                 "sigma 1), numpy default_rng(0)",
         "config": {"workload": f"brute-force kNN, train {args.n_train}x{d} fp32, {nq} queries, "
