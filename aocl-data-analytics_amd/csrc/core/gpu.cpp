@@ -1,4 +1,5 @@
 #include "gpu.hpp"
+#include <cstring>
 #include <cstdio>
 
 namespace dagpu {
@@ -21,6 +22,74 @@ da_status require_gpu(ErrorStack *err) {
         return da_status_arch_not_supported;
     }
     return da_status_success;
+}
+
+hipError_t copy_2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                   size_t rows, hipMemcpyKind kind, hipStream_t st) {
+    if (rows == 0 || width == 0)
+        return hipSuccess;
+    if (kind == hipMemcpyDeviceToDevice)
+        return hipMemcpy2DAsync(dst, dpitch, src, spitch, width, rows, kind, st);
+    const bool dc = dpitch == width || rows == 1, sc = spitch == width || rows == 1;
+    const size_t bytes = width * rows;
+    if (dc && sc)
+        return hipMemcpyAsync(dst, src, bytes, kind, st);
+    hipError_t e = hipSuccess;
+    if (kind == hipMemcpyHostToDevice) {
+        const char *h = static_cast<const char *>(src);
+        std::vector<char> pack;
+        if (!sc) {
+            pack.resize(bytes);
+            for (size_t r = 0; r < rows; r++)
+                std::memcpy(pack.data() + r * width, h + r * spitch, width);
+            h = pack.data();
+        }
+        if (dc) {
+            e = hipMemcpyAsync(dst, h, bytes, kind, st);
+        } else {
+            DeviceBuffer stage;
+            e = stage.reserve(bytes);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(stage.ptr(), h, bytes, kind, st);
+            if (e == hipSuccess)
+                e = hipMemcpy2DAsync(dst, dpitch, stage.ptr(), width, width, rows,
+                                     hipMemcpyDeviceToDevice, st);
+            if (e == hipSuccess)
+                e = hipStreamSynchronize(st);
+            return e;
+        }
+        if (e == hipSuccess && !sc)
+            e = hipStreamSynchronize(st);
+        return e;
+    }
+    // Device to host.
+    DeviceBuffer stage;
+    const void *d = src;
+    if (!sc) {
+        e = stage.reserve(bytes);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(stage.ptr(), width, src, spitch, width, rows,
+                                 hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess)
+            return e;
+        d = stage.ptr();
+    }
+    if (dc) {
+        e = hipMemcpyAsync(dst, d, bytes, kind, st);
+        if (e == hipSuccess && !sc)
+            e = hipStreamSynchronize(st);
+        return e;
+    }
+    std::vector<char> buf(bytes);
+    e = hipMemcpyAsync(buf.data(), d, bytes, kind, st);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+    if (e != hipSuccess)
+        return e;
+    char *h = static_cast<char *>(dst);
+    for (size_t r = 0; r < rows; r++)
+        std::memcpy(h + r * dpitch, buf.data() + r * width, width);
+    return hipSuccess;
 }
 
 hipError_t DeviceBuffer::reserve(size_t bytes) {

@@ -42,6 +42,14 @@ class DeviceBuffer {
     size_t bytes_ = 0;
 };
 
+// Strided copy with the arguments of hipMemcpy2DAsync. Host sides are handled with one
+// contiguous transfer: rows of a strided host array are packed (or unpacked) on the host
+// around a single copy, because a pitched copy from pageable memory moves one row at a time
+// (seconds for millions of short rows). Strided device sides use the device's own 2-D copy.
+// Packed transfers are complete when the call returns; the others follow stream order.
+hipError_t copy_2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                   size_t rows, hipMemcpyKind kind, hipStream_t st);
+
 class GpuContext {
   public:
     GpuContext() = default;

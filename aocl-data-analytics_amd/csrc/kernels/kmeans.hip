@@ -13,6 +13,7 @@
 // and the totals are exact to about 2^-100 of the data range. The exponents are chosen on
 // the host from the column maxima so that no partial or total can overflow.
 #include "launch.hpp"
+#include <algorithm>
 #include <climits>
 #include <hip/hip_runtime.h>
 
@@ -23,15 +24,43 @@ namespace {
 constexpr int CS_ELEMS = 4096; // centres staged in LDS per chunk (doubles)
 
 template <typename T>
-__global__ void col_absmax_kernel(const T *__restrict__ X, int n, int d, double *out) {
-    const int64_t total = (int64_t)n * d;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int l = (int)(e % d);
-        double v = fabs((double)X[e]);
-        if (v == v && v > 0.0)
-            atomicMax(reinterpret_cast<unsigned long long *>(out + l),
-                      (unsigned long long)__double_as_longlong(v));
+__global__ __launch_bounds__(256) void col_absmax_kernel(const T *__restrict__ X, int n, int d,
+                                                         double *out) {
+    // Thread t takes feature t % d of every (256/d)-th sample of the workgroup's stride
+    // (d <= 256), or features t, t + 256, ... of every sample (d > 256); maxima meet in LDS
+    // and each workgroup issues one atomic per feature. Non-negative doubles order like their
+    // bit patterns.
+    __shared__ double m[256];
+    auto fold = [](double mx, T x) {
+        const double v = fabs((double)x);
+        return (v == v && v > mx) ? v : mx;
+    };
+    if (d <= 256) {
+        const int spp = 256 / d, sub = (int)threadIdx.x / d, l = (int)threadIdx.x - sub * d;
+        double mx = 0.0;
+        if (sub < spp)
+            for (int64_t i = (int64_t)blockIdx.x * spp + sub; i < n;
+                 i += (int64_t)gridDim.x * spp)
+                mx = fold(mx, X[i * d + l]);
+        m[threadIdx.x] = mx;
+        __syncthreads();
+        if ((int)threadIdx.x < d) {
+            double r = m[threadIdx.x];
+            for (int t = 1; t < spp; t++)
+                r = fmax(r, m[threadIdx.x + t * d]);
+            if (r > 0.0)
+                atomicMax(reinterpret_cast<unsigned long long *>(out + threadIdx.x),
+                          (unsigned long long)__double_as_longlong(r));
+        }
+    } else {
+        for (int l = threadIdx.x; l < d; l += 256) {
+            double mx = 0.0;
+            for (int64_t i = blockIdx.x; i < n; i += gridDim.x)
+                mx = fold(mx, X[i * d + l]);
+            if (mx > 0.0)
+                atomicMax(reinterpret_cast<unsigned long long *>(out + l),
+                          (unsigned long long)__double_as_longlong(mx));
+        }
     }
 }
 
@@ -91,93 +120,155 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
     }
 }
 
+// Exclusive prefix sum of v[0..m) in LDS by a 1024-thread workgroup: each thread scans a
+// contiguous chunk, the chunk totals are scanned across the waves, then added back.
+__device__ void block_exclusive_scan(int *v, int m, int *wsum) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int per = (m + 1023) / 1024;
+    const int lo = t * per, hi = min(m, lo + per);
+    int s = 0;
+    for (int j = lo; j < hi; j++)
+        s += v[j];
+    // Inclusive scan of the chunk totals within the wave.
+    int x = s;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o)
+            x += y;
+    }
+    if (lane == 63)
+        wsum[w] = x;
+    __syncthreads();
+    if (t == 0) {
+        int run = 0;
+        for (int j = 0; j < 16; j++) {
+            const int c = wsum[j];
+            wsum[j] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    int run = wsum[w] + x - s;
+    for (int j = lo; j < hi; j++) {
+        const int c = v[j];
+        v[j] = run;
+        run += c;
+    }
+    __syncthreads();
+}
+
+// Fixed-point totals per (cluster, feature) of the points [b*ppb, (b+1)*ppb) whose label is
+// in [c0, c0+kc): the points are counting-sorted by label in LDS (two 32-bit LDS atomics per
+// point), then thread pairs (cluster, feature) sum their cluster's values in registers and
+// add the workgroup totals to the device-wide accumulators. Integer sums are exact, so
+// neither the order inside a cluster nor the order of the workgroups changes the result.
 template <typename T>
 __global__ __launch_bounds__(1024) void kmeans_partial_kernel(
     const T *__restrict__ X, int n, int d, const int *__restrict__ labels, int c0, int kc,
     const int *__restrict__ escale, int ppb, long long *__restrict__ part,
     int *__restrict__ cnt) {
-    extern __shared__ long long acc[];
-    const int words = kc * d * 2;
-    int *lcnt = reinterpret_cast<int *>(acc + words);
-    for (int t = threadIdx.x; t < words; t += 1024)
-        acc[t] = 0;
-    for (int t = threadIdx.x; t < kc; t += 1024)
-        lcnt[t] = 0;
-    __syncthreads();
+    extern __shared__ int sh[];
+    int *off = sh;                 // kc + 1 offsets
+    int *fill = off + kc + 1;      // kc fill counters
+    int *list = fill + kc;         // ppb point offsets, sorted by label
+    __shared__ int wsum[16];
     const int b = blockIdx.x;
     const int64_t start = (int64_t)b * ppb;
-    int64_t end = start + ppb;
-    if (end > n)
-        end = n;
-    const int lshift = escale[d];
-    // Thread t takes feature t % d of samples start + t / d, start + t / d + spp, ... (spp
-    // samples per pass of the 1024-thread block), or, for d > 1024, features t, t + 1024, ...
-    // of every sample: no per-element division.
-    auto add = [&](int64_t i, int l, int lab) {
-        const double v = (double)X[i * d + l];
-        if (v == v && v - v == 0.0) {
-            const double sv = ldexp(v, escale[l]);
-            const double h = rint(sv);
-            const long long H = (long long)h;
-            const long long L = (long long)rint(ldexp(sv - h, lshift));
-            atomicAdd(reinterpret_cast<unsigned long long *>(&acc[(lab * d + l) * 2]),
-                      (unsigned long long)H);
-            atomicAdd(reinterpret_cast<unsigned long long *>(&acc[(lab * d + l) * 2 + 1]),
-                      (unsigned long long)L);
-        }
-        if (l == 0)
-            atomicAdd(&lcnt[lab], 1);
-    };
-    if (d <= 1024) {
-        const int spp = 1024 / d;
-        const int sub = (int)threadIdx.x / d, l = (int)threadIdx.x - sub * d;
-        if (sub < spp) {
-            for (int64_t i = start + sub; i < end; i += spp) {
-                const int lab = labels[i] - c0;
-                if (lab >= 0 && lab < kc)
-                    add(i, l, lab);
-            }
-        }
-    } else {
-        for (int64_t i = start; i < end; i++) {
-            const int lab = labels[i] - c0;
-            if (lab < 0 || lab >= kc)
-                continue;
-            for (int l = threadIdx.x; l < d; l += 1024)
-                add(i, l, lab);
-        }
+    const int m = (int)min((int64_t)ppb, (int64_t)n - start);
+    for (int t = threadIdx.x; t <= kc; t += 1024)
+        off[t] = 0;
+    for (int t = threadIdx.x; t < kc; t += 1024)
+        fill[t] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += 1024) {
+        const int lab = labels[start + i] - c0;
+        if (lab >= 0 && lab < kc)
+            atomicAdd(&off[lab], 1);
     }
     __syncthreads();
-    long long *pb = part + (int64_t)b * words;
-    for (int t = threadIdx.x; t < words; t += 1024)
-        pb[t] = acc[t];
-    for (int t = threadIdx.x; t < kc; t += 1024)
-        cnt[(int64_t)b * kc + t] = lcnt[t];
+    block_exclusive_scan(off, kc + 1, wsum);
+    for (int i = threadIdx.x; i < m; i += 1024) {
+        const int lab = labels[start + i] - c0;
+        if (lab >= 0 && lab < kc)
+            list[off[lab] + atomicAdd(&fill[lab], 1)] = i;
+    }
+    __syncthreads();
+    const int lshift = escale[d];
+    const int pairs = kc * d;
+    for (int p = threadIdx.x; p < pairs; p += 1024) {
+        const int c = p / d, l = p - c * d;
+        const int j0 = off[c], j1 = off[c + 1];
+        if (j0 == j1)
+            continue;
+        const int el = escale[l];
+        long long H = 0, L = 0;
+        auto add = [&](T x) {
+            const double v = (double)x;
+            if (v == v && v - v == 0.0) {
+                const double sv = ldexp(v, el);
+                const double h = rint(sv);
+                H += (long long)h;
+                L += (long long)rint(ldexp(sv - h, lshift));
+            }
+        };
+        const T *xs = X + start * d + l;
+        int j = j0;
+        // Eight independent gathers in flight per thread.
+        for (; j + 8 <= j1; j += 8) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                v[u] = xs[(int64_t)list[j + u] * d];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                add(v[u]);
+        }
+        for (; j < j1; j++)
+            add(xs[(int64_t)list[j] * d]);
+        if (H != 0)
+            atomicAdd(reinterpret_cast<unsigned long long *>(&part[2 * (int64_t)p]),
+                      (unsigned long long)H);
+        if (L != 0)
+            atomicAdd(reinterpret_cast<unsigned long long *>(&part[2 * (int64_t)p + 1]),
+                      (unsigned long long)L);
+        if (l == 0)
+            atomicAdd(&cnt[c], j1 - j0);
+    }
 }
 
-__global__ void kmeans_reduce_kernel(const long long *__restrict__ part,
-                                     const int *__restrict__ cnt, int nblocks, int c0, int kc,
-                                     int d, const int *__restrict__ escale,
-                                     double *__restrict__ C, int *__restrict__ counts) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= kc * d)
-        return;
-    const int c = e / d, l = e - c * d;
-    const int words = kc * d * 2;
-    long long H = 0, L = 0;
-    int count = 0;
-    for (int b = 0; b < nblocks; b++) {
-        H += part[(int64_t)b * words + 2 * e];
-        L += part[(int64_t)b * words + 2 * e + 1];
-        count += cnt[(int64_t)b * kc + c];
+// One workgroup per cluster: the centre from the device-wide totals, which are then cleared
+// (the count once every thread of the workgroup has read it).
+__global__ __launch_bounds__(256) void kmeans_reduce_kernel(long long *__restrict__ part,
+                                                            int *__restrict__ cnt, int c0, int d,
+                                                            const int *__restrict__ escale,
+                                                            double *__restrict__ C,
+                                                            int *__restrict__ counts) {
+    const int c = blockIdx.x;
+    const int count = cnt[c];
+    const int lshift = escale[d];
+    for (int l = threadIdx.x; l < d; l += blockDim.x) {
+        const int64_t e = (int64_t)c * d + l;
+        const long long H = part[2 * e], L = part[2 * e + 1];
+        const int ex = escale[l];
+        const double sum = ldexp((double)H, -ex) + ldexp((double)L, -(ex + lshift));
+        // An empty cluster keeps count 1 and a zero sum, i.e. the origin
+        // (scale_current_cluster_centres, kmeans_lloyd.hpp:363-373).
+        C[(int64_t)(c0 + c) * d + l] = sum / (double)(count > 0 ? count : 1);
+        part[2 * e] = 0;
+        part[2 * e + 1] = 0;
     }
-    const int ex = escale[l], lshift = escale[d];
-    const double sum = ldexp((double)H, -ex) + ldexp((double)L, -(ex + lshift));
-    // An empty cluster keeps count 1 and a zero sum, i.e. the origin
-    // (scale_current_cluster_centres, kmeans_lloyd.hpp:363-373).
-    C[(int64_t)(c0 + c) * d + l] = sum / (double)(count > 0 ? count : 1);
-    if (l == 0)
+    __syncthreads();
+    if (threadIdx.x == 0) {
         counts[c0 + c] = count;
+        cnt[c] = 0;
+    }
+}
+
+__global__ void zero_words_kernel(long long *__restrict__ p, int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        p[t] = 0;
 }
 
 __global__ __launch_bounds__(1024) void kmeans_shift_kernel(const double *__restrict__ a,
@@ -483,8 +574,10 @@ __host__ inline unsigned grid_for(int64_t n, int block, int64_t cap = 65536) {
 
 template <typename T>
 hipError_t launch_col_absmax(const T *X, int n, int d, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(col_absmax_kernel<T>, dim3(grid_for((int64_t)n * d, 256, 4096)), dim3(256),
-                       0, s, X, n, d, out);
+    const int64_t spp = d <= 256 ? 256 / d : 1;
+    const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n + spp - 1) / spp, 1024));
+    hipLaunchKernelGGL(col_absmax_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, X, n, d,
+                       out);
     return hipGetLastError();
 }
 
@@ -514,7 +607,7 @@ template <typename T>
 hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *labels, int c0,
                                       int kc, const int *escale, int ppb, int nblocks,
                                       long long *part, int *cnt, hipStream_t s) {
-    const size_t lds = (size_t)kc * d * 2 * sizeof(long long) + (size_t)kc * sizeof(int);
+    const size_t lds = kmeans_partial_lds(kc, ppb);
     if (lds > 160 * 1024)
         return hipErrorInvalidValue;
     if (lds > 65536)
@@ -525,11 +618,22 @@ hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *label
     return hipGetLastError();
 }
 
-hipError_t launch_kmeans_reduce(const long long *part, const int *cnt, int nblocks, int c0,
-                                int kc, int d, const int *escale, double *C, int *counts,
-                                hipStream_t s) {
-    hipLaunchKernelGGL(kmeans_reduce_kernel, dim3((kc * d + 255) / 256), dim3(256), 0, s, part,
-                       cnt, nblocks, c0, kc, d, escale, C, counts);
+size_t kmeans_partial_lds(int kc, int ppb) {
+    return (size_t)(2 * kc + 1 + ppb) * sizeof(int);
+}
+
+hipError_t launch_kmeans_reduce(long long *part, int *cnt, int c0, int kc, int d,
+                                const int *escale, double *C, int *counts, hipStream_t s) {
+    hipLaunchKernelGGL(kmeans_reduce_kernel, dim3(kc), dim3(d < 256 ? (d + 63) / 64 * 64 : 256),
+                       0, s, part, cnt, c0, d, escale, C, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_words(long long *p, int64_t n, hipStream_t s) {
+    if (n <= 0)
+        return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

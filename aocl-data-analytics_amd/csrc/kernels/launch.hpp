@@ -84,19 +84,25 @@ template <typename T>
 hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k, int *labels,
                                 const int *prev, int *changed, double *mind, hipStream_t s);
 
-// Fixed-point partial sums of the points of each cluster in [c0, c0+kc): each workgroup
-// b handles points [b*ppb, (b+1)*ppb) and writes part[(b*kc + c)*d*W + l*W + w] (W = 2
-// words: the value scaled by 2^e[l] rounded to int64 and the remainder scaled by
-// 2^(e[l]+40)) and cnt[b*kc + c].
+// Fixed-point sums of the points of each cluster in [c0, c0+kc): each workgroup b sorts the
+// points [b*ppb, (b+1)*ppb) by label in LDS, sums each cluster's values in registers and
+// adds its totals into part[(c*d + l)*2 + w] (w = 0:
+// the value scaled by 2^e[l] and rounded to int64, w = 1: the remainder scaled by
+// 2^(e[l]+lshift)) and cnt[c] with 64-bit integer atomics. Integer addition is exact and
+// associative, so the totals do not depend on the order of the workgroups. part (kc*d*2
+// words) and cnt (kc) must be zero on entry; launch_kmeans_reduce leaves them zero.
 template <typename T>
 hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *labels, int c0,
                                       int kc, const int *escale, int ppb, int nblocks,
                                       long long *part, int *cnt, hipStream_t s);
-// Reduces the partials over blocks (exactly, in 128-bit integers) and writes
-// C[(c0+c)*d + l] = sum / max(count, 1), counts[c0+c] = count.
-hipError_t launch_kmeans_reduce(const long long *part, const int *cnt, int nblocks, int c0,
-                                int kc, int d, const int *escale, double *C, int *counts,
-                                hipStream_t s);
+// C[(c0+c)*d + l] = sum / max(count, 1), counts[c0+c] = count from the totals of
+// launch_kmeans_partial_sums, which it then clears for the next pass.
+hipError_t launch_kmeans_reduce(long long *part, int *cnt, int c0, int kc, int d,
+                                const int *escale, double *C, int *counts, hipStream_t s);
+// Fills n 64-bit words with zero (a kernel, so that it can be captured in a graph).
+hipError_t launch_zero_words(long long *p, int64_t n, hipStream_t s);
+// LDS bytes of one partial-sum workgroup for kc clusters and ppb points.
+size_t kmeans_partial_lds(int kc, int ppb);
 // out[0] = ||Cprev||_F^2, out[1] = ||Cprev - Cnew||_F^2 (single workgroup, fixed order).
 hipError_t launch_kmeans_shift(const double *Cprev, const double *Cnew, int len, double *out,
                                hipStream_t s);

@@ -87,12 +87,12 @@ da_status KMeansHandle<T>::upload_data(const T *src, da_int ld, DeviceBuffer &ds
     const size_t bytes = (size_t)n_samples_ * n_features_ * sizeof(T);
     DA_HIP(err, dst.reserve(bytes));
     if (data_order_ == row_major) {
-        DA_HIP(err, hipMemcpy2DAsync(dst.ptr(), n_features_ * sizeof(T), src, (size_t)ld * sizeof(T),
+        DA_HIP(err, copy_2d(dst.ptr(), n_features_ * sizeof(T), src, (size_t)ld * sizeof(T),
                                      n_features_ * sizeof(T), n_samples_, hipMemcpyHostToDevice, st));
     } else {
         DeviceBuffer stage;
         DA_HIP(err, stage.reserve(bytes));
-        DA_HIP(err, hipMemcpy2DAsync(stage.ptr(), n_samples_ * sizeof(T), src, (size_t)ld * sizeof(T),
+        DA_HIP(err, copy_2d(stage.ptr(), n_samples_ * sizeof(T), src, (size_t)ld * sizeof(T),
                                      n_samples_ * sizeof(T), n_features_, hipMemcpyHostToDevice, st));
         DA_HIP(err, launch_colmajor_to_rowmajor<T>(stage.as<T>(), n_samples_, n_samples_, n_features_,
                                                    dst.as<T>(), st));
@@ -310,8 +310,8 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
             DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n, d, lab_new, c0, kcc, escale_.as<int>(),
                                                       ppb_, nblocks_, part_.as<long long>(),
                                                       pcnt_.as<int>(), st));
-            DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), nblocks_, c0,
-                                             kcc, d, escale_.as<int>(), Cout, counts_.as<int>(), st));
+            DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), c0, kcc, d,
+                                             escale_.as<int>(), Cout, counts_.as<int>(), st));
         }
         DA_HIP(err, launch_kmeans_shift(Cin, Cout, k * d, small_.as<double>(), st));
     }
@@ -336,7 +336,7 @@ da_status KMeansHandle<T>::centres_from_labels(const U *dX, const int *labels, d
         DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n_samples_, n_features_, labels, c0, kcc,
                                                   escale_.as<int>(), ppb_, nblocks_,
                                                   part_.as<long long>(), pcnt_.as<int>(), st));
-        DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), nblocks_, c0, kcc,
+        DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), c0, kcc,
                                          n_features_, escale_.as<int>(), C, counts_.as<int>(), st));
     }
     return da_status_success;
@@ -674,17 +674,31 @@ template <typename T> da_status KMeansHandle<T>::compute() {
     DA_HIP(err, cp_.reserve((size_t)std::max(n, 256 * 64) * sizeof(double)));
     // Clusters per partial-sum pass: as many as fit the 160 KB of LDS of a CU (all of them
     // for k = 256 at d = 32).
-    kc_ = std::max<da_int>(1, std::min<da_int>(k, (160 * 1024 - 64) / (16 * d + 4)));
-    nblocks_ = std::min<da_int>(512, (n + 255) / 256);
-    ppb_ = (n + nblocks_ - 1) / nblocks_;
-    nblocks_ = (n + ppb_ - 1) / ppb_;
-    DA_HIP(err, part_.reserve((size_t)nblocks_ * kc_ * d * 2 * sizeof(long long)));
+    // Partial-sum workgroups sort up to 8,192 points by label in LDS: at least one round of
+    // two workgroups per CU, more for large n. Clusters per pass: all of them up to 12,288
+    // (the label counters share the 160 KB of LDS with the point list).
+    {
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus < 1)
+            cus = 256;
+        nblocks_ = std::max<da_int>(1, std::min<da_int>(2 * cus, (n + 255) / 256));
+        ppb_ = (n + nblocks_ - 1) / nblocks_;
+        if (ppb_ > 8192)
+            ppb_ = 8192;
+        nblocks_ = (n + ppb_ - 1) / ppb_;
+    }
+    kc_ = std::max<da_int>(1, std::min<da_int>(k, 12288));
+    DA_HIP(err, part_.reserve((size_t)kc_ * d * 2 * sizeof(long long)));
+    DA_HIP(err, launch_zero_words(part_.as<long long>(), (int64_t)kc_ * d * 2, gpu->stream()));
     if (mfma_ok_) {
         DA_HIP(err, c32_.reserve((size_t)(k + 16) * kp_ * sizeof(float)));
         DA_HIP(err, chalf_.reserve((size_t)(k + 16) * sizeof(float)));
         DA_HIP(err, list_.reserve((size_t)n * sizeof(int)));
     }
-    DA_HIP(err, pcnt_.reserve((size_t)nblocks_ * kc_ * sizeof(int)));
+    DA_HIP(err, pcnt_.reserve((size_t)(kc_ + 1) / 2 * sizeof(long long)));
+    DA_HIP(err, launch_zero_words(pcnt_.as<long long>(), (kc_ + 1) / 2, gpu->stream()));
     Ccur_ = Ca_.as<double>();
     Cprev_ = Cb_.as<double>();
     lab_cur_ = lab_a_.as<int>();
@@ -793,11 +807,11 @@ da_status KMeansHandle<T>::predict(da_int k_samples, da_int k_features, const T 
     const size_t bytes = (size_t)k_samples * k_features * sizeof(T);
     DA_HIP(err, q_.reserve(bytes));
     if (order() == row_major) {
-        DA_HIP(err, hipMemcpy2DAsync(q_.ptr(), k_features * sizeof(T), Y, (size_t)ldy * sizeof(T),
+        DA_HIP(err, copy_2d(q_.ptr(), k_features * sizeof(T), Y, (size_t)ldy * sizeof(T),
                                      k_features * sizeof(T), k_samples, hipMemcpyHostToDevice, st));
     } else {
         DA_HIP(err, q_stage_.reserve(bytes));
-        DA_HIP(err, hipMemcpy2DAsync(q_stage_.ptr(), k_samples * sizeof(T), Y, (size_t)ldy * sizeof(T),
+        DA_HIP(err, copy_2d(q_stage_.ptr(), k_samples * sizeof(T), Y, (size_t)ldy * sizeof(T),
                                      k_samples * sizeof(T), k_features, hipMemcpyHostToDevice, st));
         DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), k_samples, k_samples,
                                                    k_features, q_.as<T>(), st));
@@ -836,11 +850,11 @@ da_status KMeansHandle<T>::transform(da_int m_samples, da_int m_features, const 
     const size_t bytes = (size_t)m_samples * m_features * sizeof(T);
     DA_HIP(err, q_.reserve(bytes));
     if (order() == row_major) {
-        DA_HIP(err, hipMemcpy2DAsync(q_.ptr(), m_features * sizeof(T), X, (size_t)ldx * sizeof(T),
+        DA_HIP(err, copy_2d(q_.ptr(), m_features * sizeof(T), X, (size_t)ldx * sizeof(T),
                                      m_features * sizeof(T), m_samples, hipMemcpyHostToDevice, st));
     } else {
         DA_HIP(err, q_stage_.reserve(bytes));
-        DA_HIP(err, hipMemcpy2DAsync(q_stage_.ptr(), m_samples * sizeof(T), X, (size_t)ldx * sizeof(T),
+        DA_HIP(err, copy_2d(q_stage_.ptr(), m_samples * sizeof(T), X, (size_t)ldx * sizeof(T),
                                      m_samples * sizeof(T), m_features, hipMemcpyHostToDevice, st));
         DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), m_samples, m_samples,
                                                    m_features, q_.as<T>(), st));
@@ -851,12 +865,12 @@ da_status KMeansHandle<T>::transform(da_int m_samples, da_int m_features, const 
     DA_HIP(err, launch_kmeans_transform<T>(q_.as<T>(), m_samples, m_features, Cbest_.as<double>(),
                                            k, out_.as<T>(), k, st));
     if (order() == row_major) {
-        DA_HIP(err, hipMemcpy2DAsync(X_transform, (size_t)ldx_transform * sizeof(T), out_.ptr(),
+        DA_HIP(err, copy_2d(X_transform, (size_t)ldx_transform * sizeof(T), out_.ptr(),
                                      k * sizeof(T), k * sizeof(T), m_samples, hipMemcpyDeviceToHost, st));
     } else {
         T *tr = out_.as<T>() + (size_t)m_samples * k;
         DA_HIP(err, launch_colmajor_to_rowmajor<T>(out_.as<T>(), k, k, m_samples, tr, st));
-        DA_HIP(err, hipMemcpy2DAsync(X_transform, (size_t)ldx_transform * sizeof(T), tr,
+        DA_HIP(err, copy_2d(X_transform, (size_t)ldx_transform * sizeof(T), tr,
                                      m_samples * sizeof(T), m_samples * sizeof(T), k,
                                      hipMemcpyDeviceToHost, st));
     }

@@ -33,11 +33,11 @@ da_status upload_rows(ErrorStack *err, hipStream_t st, da_order order, da_int m,
     const size_t bytes = (size_t)m * k * sizeof(T);
     DA_HIP(err, dst.reserve(bytes));
     if (order == row_major) {
-        DA_HIP(err, hipMemcpy2DAsync(dst.ptr(), k * sizeof(T), A, (size_t)lda * sizeof(T),
+        DA_HIP(err, copy_2d(dst.ptr(), k * sizeof(T), A, (size_t)lda * sizeof(T),
                                      k * sizeof(T), m, hipMemcpyHostToDevice, st));
     } else {
         DA_HIP(err, stage.reserve(bytes));
-        DA_HIP(err, hipMemcpy2DAsync(stage.ptr(), m * sizeof(T), A, (size_t)lda * sizeof(T),
+        DA_HIP(err, copy_2d(stage.ptr(), m * sizeof(T), A, (size_t)lda * sizeof(T),
                                      m * sizeof(T), k, hipMemcpyHostToDevice, st));
         DA_HIP(err, launch_colmajor_to_rowmajor<T>(stage.as<T>(), m, m, k, dst.as<T>(), st));
     }
@@ -176,7 +176,7 @@ da_status pairwise(da_order order, da_int m, da_int n, da_int k, const T *X, da_
                                                   S.dbuf.as<T>(), nb_cols,
                                                   na ? na + r0 : nullptr, nb, (double)p,
                                                   zero_diag, (int)r0, s)));
-        DA_HIP(err, hipMemcpy2DAsync(D + (size_t)r0 * ldd, (size_t)ldd * sizeof(T),
+        DA_HIP(err, copy_2d(D + (size_t)r0 * ldd, (size_t)ldd * sizeof(T),
                                      S.dbuf.ptr(), row_bytes, row_bytes, rows,
                                      hipMemcpyDeviceToHost, s));
     }

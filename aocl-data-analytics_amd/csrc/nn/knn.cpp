@@ -156,13 +156,13 @@ da_status NNHandle<T>::set_data(da_int n_samples, da_int n_features, const T *X,
     const size_t bytes = (size_t)n_samples * n_features * sizeof(T);
     DA_HIP(err, train_.reserve(bytes));
     if (order() == row_major) {
-        DA_HIP(err, hipMemcpy2DAsync(train_.ptr(), n_features * sizeof(T), X,
+        DA_HIP(err, copy_2d(train_.ptr(), n_features * sizeof(T), X,
                                      (size_t)ldx * sizeof(T), n_features * sizeof(T), n_samples,
                                      hipMemcpyHostToDevice, st));
     } else {
         DeviceBuffer stage;
         DA_HIP(err, stage.reserve(bytes));
-        DA_HIP(err, hipMemcpy2DAsync(stage.ptr(), n_samples * sizeof(T), X,
+        DA_HIP(err, copy_2d(stage.ptr(), n_samples * sizeof(T), X,
                                      (size_t)ldx * sizeof(T), n_samples * sizeof(T), n_features,
                                      hipMemcpyHostToDevice, st));
         DA_HIP(err, launch_colmajor_to_rowmajor<T>(stage.as<T>(), n_samples, n_samples,
@@ -521,12 +521,12 @@ da_status NNHandle<T>::kneighbors(da_int n_queries, da_int n_features, const T *
     const size_t qbytes = (size_t)n_queries * n_features * sizeof(T);
     DA_HIP(err, q_.reserve(qbytes));
     if (order() == row_major) {
-        DA_HIP(err, hipMemcpy2DAsync(q_.ptr(), n_features * sizeof(T), X_test,
+        DA_HIP(err, copy_2d(q_.ptr(), n_features * sizeof(T), X_test,
                                      (size_t)ldx_test * sizeof(T), n_features * sizeof(T),
                                      n_queries, hipMemcpyHostToDevice, st));
     } else {
         DA_HIP(err, q_stage_.reserve(qbytes));
-        DA_HIP(err, hipMemcpy2DAsync(q_stage_.ptr(), n_queries * sizeof(T), X_test,
+        DA_HIP(err, copy_2d(q_stage_.ptr(), n_queries * sizeof(T), X_test,
                                      (size_t)ldx_test * sizeof(T), n_queries * sizeof(T),
                                      n_features, hipMemcpyHostToDevice, st));
         DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), n_queries, n_queries,
@@ -590,7 +590,7 @@ da_status NNHandle<T>::kneighbors_device(da_int n_queries, da_int n_features, co
     const T *dq = d_X_test;
     if (ldx_test != n_features) {
         DA_HIP(err, q_.reserve((size_t)n_queries * n_features * sizeof(T)));
-        DA_HIP(err, hipMemcpy2DAsync(q_.ptr(), n_features * sizeof(T), d_X_test,
+        DA_HIP(err, copy_2d(q_.ptr(), n_features * sizeof(T), d_X_test,
                                      (size_t)ldx_test * sizeof(T), n_features * sizeof(T),
                                      n_queries, hipMemcpyDeviceToDevice, gpu->stream()));
         dq = q_.as<T>();

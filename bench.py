@@ -109,6 +109,17 @@ def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1):
         if cnt.value:
             kern[name] = (ms.value, cnt.value)
     L.check(L.lib.da_amd_profile_enable(h.h, 0))
+    # The reference's entry point with host arrays: queries cross PCIe to the device and the
+    # results come back inside each call (not the headline value, which starts and ends in
+    # device memory).
+    Xq_host = np.empty((nq, d), dtype=np.float32)
+    L.check(L.lib.da_amd_memcpy_dtoh(L.ptr(Xq_host), dq, Xq_host.nbytes))
+    nn.kneighbors(Xq_host, k)
+    host_steps = max(1, min(steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(host_steps):
+        nn.kneighbors(Xq_host, k)
+    host_qps = nq * host_steps / (time.perf_counter() - t0)
     ind = np.empty((nq, k), dtype=np.int32)
     dist = np.empty((nq, k), dtype=np.float32)
     L.check(L.lib.da_amd_memcpy_dtoh(L.ptr(ind), di, ind.nbytes))
@@ -120,7 +131,7 @@ def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1):
     return {"elapsed": elapsed, "qps": nq * steps / elapsed, "ms_per_step": elapsed / steps * 1e3,
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
             "kernels": kern, "ind": ind, "dist": dist, "fallback": nfb.value,
-            "candidates_per_query": ncand.value}
+            "candidates_per_query": ncand.value, "host_buffers_qps": host_qps}
 
 
 SERVE_METRIC = ("kNN serving: sustained queries/s with p50/p99 request latency, resident "
@@ -514,7 +525,8 @@ def main():
     for f, r in results.items():
         forms[f] = {k2: r[k2] for k2 in ("qps", "ms_per_step", "p50_ms", "p99_ms", "fallback",
                                          "candidates_per_query", "recall_at_10", "index_match",
-                                         "max_rel_dist_err", "mean_rel_dist_err") if k2 in r}
+                                         "max_rel_dist_err", "mean_rel_dist_err",
+                                         "host_buffers_qps") if k2 in r}
         forms[f]["kernels_ms"] = {n: {"total_ms": v[0], "launches": v[1]}
                                   for n, v in r["kernels"].items()}
     out = {
@@ -531,6 +543,7 @@ def main():
         "recall_at_10": H.get("recall_at_10"),
         "lowp_speedup": (H["qps"] / plain["qps"]) if plain and head != "working" else None,
         "plain_fp32_qps": plain["qps"] if plain else None,
+        "host_buffers_qps": H["host_buffers_qps"],
         "forms": forms, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
