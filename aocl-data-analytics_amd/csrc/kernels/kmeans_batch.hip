@@ -463,47 +463,119 @@ __global__ __launch_bounds__(1024) void kb_partial_kernel(
     const int64_t *__restrict__ poff, const int *__restrict__ state, int par,
     const int *__restrict__ L0, const int *__restrict__ L1, const int *__restrict__ escale,
     long long *__restrict__ part, int *__restrict__ pcnt) {
-    extern __shared__ long long acc[];
+    // The chunk's samples are counting-sorted by label in LDS (two 32-bit LDS atomics per
+    // sample); then thread pairs (cluster, feature) sum their cluster's values in registers
+    // and write the chunk's partial, which the reduce adds up in chunk order.
+    extern __shared__ int sh[];
+    int *off = sh;        // k + 1 (k <= 64)
+    int *fill = sh + 65;  // k
+    int *list = sh + 130; // chunk samples sorted by label (8-byte aligned)
     const KChunk ch = chunks[blockIdx.x];
     const int inst = ch.inst;
     if (state[inst] != KB_ACTIVE)
         return;
     const int d = b.d[inst], k = b.k[inst];
-    const int words = k * d * 2;
-    int *lcnt = reinterpret_cast<int *>(acc + words);
-    for (int t = threadIdx.x; t < words; t += 1024)
-        acc[t] = 0;
-    for (int t = threadIdx.x; t < k; t += 1024)
-        lcnt[t] = 0;
-    __syncthreads();
-    const int *lab = (par ? L1 : L0) + b.loff[inst];
-    const float *x = X + b.xoff[inst];
+    const int *lab = (par ? L1 : L0) + b.loff[inst] + ch.start;
+    const float *x = X + b.xoff[inst] + (int64_t)ch.start * d;
     const int *es = escale + inst * 65;
     const int lshift = es[64];
-    const int spp = 1024 / d, sub = (int)threadIdx.x / d, l = (int)threadIdx.x - sub * d;
-    for (int64_t i = ch.start + sub; sub < spp && i < ch.start + ch.count; i += spp) {
-        const int c = lab[i];
-        const double v = (double)x[i * d + l];
-        if (v == v && v - v == 0.0) {
-            const double sv = ldexp(v, es[l]);
-            const double h = rint(sv);
-            const long long H = (long long)h;
-            const long long Lw = (long long)rint(ldexp(sv - h, lshift));
-            atomicAdd(reinterpret_cast<unsigned long long *>(&acc[(c * d + l) * 2]),
-                      (unsigned long long)H);
-            atomicAdd(reinterpret_cast<unsigned long long *>(&acc[(c * d + l) * 2 + 1]),
-                      (unsigned long long)Lw);
+    if (threadIdx.x < 65)
+        off[threadIdx.x] = 0;
+    if (threadIdx.x < 64)
+        fill[threadIdx.x] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < ch.count; i += 1024)
+        atomicAdd(&off[lab[i]], 1);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // Exclusive scan of the k <= 64 counts in one wave; off[k] = chunk count.
+        const int lane = threadIdx.x;
+        const int v = lane < k ? off[lane] : 0;
+        int incl = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o)
+                incl += y;
         }
-        if (l == 0)
-            atomicAdd(&lcnt[c], 1);
+        __builtin_amdgcn_wave_barrier();
+        if (lane < k)
+            off[lane] = incl - v;
+        if (lane == k - 1)
+            off[k] = incl;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < ch.count; i += 1024) {
+        const int c = lab[i];
+        list[off[c] + atomicAdd(&fill[c], 1)] = i;
     }
     __syncthreads();
     long long *pb = part + poff[blockIdx.x];
-    for (int t = threadIdx.x; t < words; t += 1024)
-        pb[t] = acc[t];
     int *cb = pcnt + (int64_t)blockIdx.x * 64;
-    for (int t = threadIdx.x; t < k; t += 1024)
-        cb[t] = lcnt[t];
+    // R threads per pair when the pairs are fewer than the threads: thread (p, r) takes every
+    // R-th sample of the cluster from the r-th on.
+    const int P = k * d;
+    const int R = P >= 1024 ? 1 : 1024 / P;
+    auto pair_sum = [&](int p, int r, long long &H, long long &Lw) {
+        const int c = p / d, l = p - c * d;
+        const int j1 = off[c + 1];
+        const int el = es[l];
+        H = 0;
+        Lw = 0;
+        auto add = [&](float xv) {
+            const double v = (double)xv;
+            if (v == v && v - v == 0.0) {
+                const double sv = ldexp(v, el);
+                const double h = rint(sv);
+                H += (long long)h;
+                Lw += (long long)rint(ldexp(sv - h, lshift));
+            }
+        };
+        const float *xs = x + l;
+        int j = off[c] + r;
+        // Eight independent gathers in flight per thread.
+        for (; j + 7 * R < j1; j += 8 * R) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                v[u] = xs[(int64_t)list[j + u * R] * d];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                add(v[u]);
+        }
+        for (; j < j1; j += R)
+            add(xs[(int64_t)list[j] * d]);
+    };
+    if (R == 1) {
+        for (int p = threadIdx.x; p < P; p += 1024) {
+            long long H, Lw;
+            pair_sum(p, 0, H, Lw);
+            pb[2 * p] = H;
+            pb[2 * p + 1] = Lw;
+        }
+    } else {
+        const int t = threadIdx.x;
+        long long H = 0, Lw = 0;
+        if (t < P * R)
+            pair_sum(t % P, t / P, H, Lw);
+        // The sample list is dead: its space holds the per-thread sums (16 KB).
+        __syncthreads();
+        long long *red = reinterpret_cast<long long *>(list);
+        red[t] = H;
+        red[1024 + t] = Lw;
+        __syncthreads();
+        if (t < P) {
+            long long sh_ = 0, sl = 0;
+            for (int r = 0; r < R; r++) {
+                sh_ += red[r * P + t];
+                sl += red[1024 + r * P + t];
+            }
+            pb[2 * t] = sh_;
+            pb[2 * t + 1] = sl;
+        }
+    }
+    if ((int)threadIdx.x < k)
+        cb[threadIdx.x] = off[threadIdx.x + 1] - off[threadIdx.x];
 }
 
 // Per instance: new centres from its chunks' partials (fixed order), the centre shift and

@@ -188,12 +188,12 @@ da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, co
     w.fpar = r.fpar.as<int>();
     w.n_iter = r.n_iter.as<int>();
     w.cin = r.cin.as<double>();
-    int kmax = 1, dmax = 1;
-    for (int i = 0; i < n_inst; i++) {
-        kmax = std::max(kmax, hk[i]);
-        dmax = std::max(dmax, hd[i]);
-    }
-    r.lds = (size_t)kmax * dmax * 2 * sizeof(long long) + (size_t)kmax * sizeof(int);
+    // Centroid sums: label offsets and fill counters (130 ints) and the chunk's samples
+    // sorted by label, a region that later holds 2,048 per-thread int64 sums.
+    int cmax = 1;
+    for (const KChunk &c : chunks)
+        cmax = std::max(cmax, c.count);
+    r.lds = (size_t)(130 + std::max(cmax, 4096)) * sizeof(int);
     BR(kb_set_partial_lds(r.lds));
     BR(hipStreamSynchronize(st));
 #undef BR
