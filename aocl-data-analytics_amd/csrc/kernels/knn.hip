@@ -37,12 +37,8 @@ typedef double v2d __attribute__((ext_vector_type(2)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-constexpr int COLT = 2;                   // 16-row training tiles per step
-constexpr int STEP = COLT * 16;           // training rows per step
+constexpr int STEP = 32; // split lengths are multiples of 32 rows (whole steps of any sweep)
 
-// Tile shapes per candidate-list length KC: buffer capacity CAP per query, waves per
-// workgroup WV and 16-query row tiles per wave RT. The LDS buffers (WV*RT*16*CAP*8 bytes)
-// stay at or below 64 KB so that two workgroups fit a CU.
 // Candidate-buffer shape per candidate-list length: CAP entries per query, WV waves of RT
 // row tiles (16 queries each) per workgroup.
 template <int KC> struct TileShape;
@@ -54,13 +50,14 @@ template <> struct TileShape<128> { static constexpr int CAP = 256, WV = 2, RT =
 // Whether a sweep keeps its buffered row offsets in 16 bits (splits of at most 65,536 rows):
 // that cuts a workgroup's buffers to 48 KB, so three workgroups fit a CU's LDS, which pays
 // only where the registers also leave room for three waves per SIMD -- the query fragments
-// plus the prefetched training fragments (ns * nf fragments of frag_regs 32-bit registers,
-// for rt query tiles and 2 x 2 training tiles) within 144 of the 168 registers a wave then
-// gets (the sweep's other state takes about 24). kc <= 32; the fp64 sweep runs one wave per
-// SIMD and never qualifies. Evaluated by the kernels and by the host's bound on the split
-// count.
-constexpr bool tile_short_index(int kc, int rt, int ns, int nf, int frag_regs, bool f64) {
-    return !f64 && kc <= 32 && ns * nf * frag_regs * (rt + 2 * 2) <= 144;
+// plus the current and prefetched training fragments (ns * nf fragments of frag_regs 32-bit
+// registers, for rt query tiles and 2 x colt training tiles, colt 16-row tiles per step)
+// within 144 of the 168 registers a wave then gets (the sweep's other state takes about
+// 24). kc <= 32; the fp64 sweep runs one wave per SIMD and never qualifies. Evaluated by the
+// kernels and by the host's bound on the split count.
+constexpr bool tile_short_index(int kc, int rt, int ns, int nf, int frag_regs, bool f64,
+                                int colt) {
+    return !f64 && kc <= 32 && ns * nf * frag_regs * (rt + 2 * colt) <= 144;
 }
 constexpr int SPLIT16_MAX = 65536; // rows per split with 16-bit offsets
 
@@ -259,8 +256,9 @@ __device__ __forceinline__ void wave_fence() {
 // positions from a ballot: the 16 lanes of a lane group serve one query per (rt, i), so a
 // hit's slot is the group's running count (cq, held in registers by the group's lanes)
 // plus the number of hits of the lower lanes of the group. A buffer with more than
-// CAP - STEP entries is compacted to its KC best, which sets the new threshold.
-template <class Op, int KC, int CAP, int ROWT, typename IT>
+// CAP - COLT*16 entries (room for one more step) is compacted to its KC best, which sets
+// the new threshold.
+template <class Op, int KC, int CAP, int ROWT, int COLT, typename IT>
 __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][COLT],
                                           typename Op::acc_e (&ts)[ROWT][4],
                                           int (&cq)[ROWT][4], float *bkey, IT *bidx,
@@ -297,7 +295,7 @@ __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][
     for (int rt = 0; rt < ROWT; rt++)
 #pragma unroll
         for (int i = 0; i < 4; i++)
-            need |= cq[rt][i] > CAP - STEP;
+            need |= cq[rt][i] > CAP - COLT * 16;
     if (__builtin_amdgcn_ballot_w64(need) == 0)
         return;
     wave_fence();
@@ -307,7 +305,7 @@ __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][
         for (int i = 0; i < 4; i++) {
             // one representative lane (r = 0) per group
             unsigned long long m2 =
-                __builtin_amdgcn_ballot_w64(cq[rt][i] > CAP - STEP) & 0x0001000100010001ull;
+                __builtin_amdgcn_ballot_w64(cq[rt][i] > CAP - COLT * 16) & 0x0001000100010001ull;
             while (m2) {
                 const int l0 = __ffsll((long long)m2) - 1;
                 m2 &= m2 - 1;
@@ -361,12 +359,12 @@ __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bke
     }
 }
 
-// Per-kernel configuration: 16-bit buffered offsets and three waves per SIMD, or 32-bit
-// offsets and two.
-template <class Op, int KCH, int KC, bool SPLIT> struct TileCfg {
+// Per-kernel configuration: COLT 16-row training tiles per step; 16-bit buffered offsets and
+// three waves per SIMD, or 32-bit offsets and two.
+template <class Op, int KCH, int KC, bool SPLIT, int COLT> struct TileCfg {
     static constexpr bool SHORT =
         tile_short_index(KC, TileShape<KC>::RT, SPLIT ? 2 : 1, KCH / Op::E,
-                         (int)sizeof(typename Op::frag_t) / 4, std::is_same_v<Op, OpF64>);
+                         (int)sizeof(typename Op::frag_t) / 4, std::is_same_v<Op, OpF64>, COLT);
     static constexpr int WPE = SHORT ? 3 : 2;
     using idx_t = std::conditional_t<SHORT, unsigned short, int>;
 };
@@ -379,7 +377,7 @@ template <class Op, int KCH, int KC, bool SPLIT> struct TileCfg {
 // hi*hi + hi*lo + lo*hi. Training operands are loaded from the tiled layout straight into
 // registers (PF: the next step's loads are issued right after the current step's MFMAs and
 // overlap its candidate bookkeeping; otherwise each step loads, then computes).
-template <class Op, int KCH, int KC, bool SPLIT, bool PF>
+template <class Op, int KCH, int KC, bool SPLIT, bool PF, int COLT>
 __device__ __forceinline__ void knn_tile_body(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
@@ -395,7 +393,8 @@ __device__ __forceinline__ void knn_tile_body(
     constexpr int CAP = TileShape<KC>::CAP;
     constexpr int ROWT = TileShape<KC>::RT;
     constexpr int QB = TileShape<KC>::WV * ROWT * 16;
-    using idx_t = typename TileCfg<Op, KCH, KC, SPLIT>::idx_t;
+    using idx_t = typename TileCfg<Op, KCH, KC, SPLIT, COLT>::idx_t;
+    constexpr int TSTEP = COLT * 16; // training rows per step
     __shared__ float bkey[QB * CAP];
     __shared__ idx_t bidx[QB * CAP];
 
@@ -485,7 +484,8 @@ __device__ __forceinline__ void knn_tile_body(
                 }
         }
     };
-    auto check_step = [&](int tb, const acc_t (&acc)[ROWT][COLT]) {
+    // Whether any score of the step beats its query's threshold (wave-uniform).
+    auto any_hit = [&](const acc_t (&acc)[ROWT][COLT]) {
         bool hit = false;
 #pragma unroll
         for (int rt = 0; rt < ROWT; rt++)
@@ -494,29 +494,39 @@ __device__ __forceinline__ void knn_tile_body(
 #pragma unroll
                 for (int ct = 0; ct < COLT; ct++)
                     hit |= acc[rt][ct][i] > ts[rt][i];
-        if (__builtin_amdgcn_ballot_w64(hit) == 0)
-            return;
-        take_hits<Op, KC, CAP, ROWT>(acc, ts, cq, bkey, bidx, qbase, tb - t_begin);
+        return __builtin_amdgcn_ballot_w64(hit) != 0;
+    };
+    auto check_step = [&](int tb, const acc_t (&acc)[ROWT][COLT]) {
+        if (any_hit(acc))
+            take_hits<Op, KC, CAP, ROWT, COLT>(acc, ts, cq, bkey, bidx, qbase, tb - t_begin);
     };
 
     if constexpr (PF) {
-        // The next step's operands are loaded once this step's MFMAs have consumed the
-        // registers, so the loads run under this step's candidate bookkeeping.
+        // The next step's operands are loaded after this step's MFMAs and run under its
+        // candidate bookkeeping. The hit test comes first: it reads the accumulators, so it
+        // waits for the last MFMA, and only then may the loads and their address arithmetic
+        // reuse registers. An MFMA still in flight can read its accumulator input after the
+        // instructions that follow it have issued, so a register it reads must not be written
+        // before its result has been consumed.
         if (t_begin < t_end) {
             frag_t b[COLT][NS][NF];
             ae hv[COLT];
             load_step(t_begin, b, hv);
-            for (int tb = t_begin; tb < t_end; tb += STEP) {
+            for (int tb = t_begin; tb < t_end; tb += TSTEP) {
                 acc_t acc[ROWT][COLT];
                 mfma_step(b, hv, acc);
-                if (tb + STEP < t_end)
-                    load_step(tb + STEP, b, hv);
+                const bool hit = any_hit(acc);
                 __builtin_amdgcn_sched_barrier(0);
-                check_step(tb, acc);
+                if (tb + TSTEP < t_end)
+                    load_step(tb + TSTEP, b, hv);
+                __builtin_amdgcn_sched_barrier(0);
+                if (hit)
+                    take_hits<Op, KC, CAP, ROWT, COLT>(acc, ts, cq, bkey, bidx, qbase,
+                                                       tb - t_begin);
             }
         }
     } else {
-        for (int tb = t_begin; tb < t_end; tb += STEP) {
+        for (int tb = t_begin; tb < t_end; tb += TSTEP) {
             frag_t b[COLT][NS][NF];
             ae hv[COLT];
             load_step(tb, b, hv);
@@ -540,17 +550,19 @@ __global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
     float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
-    knn_tile_body<Op, KCH, KC, SPLIT, PF>(Q, nq, Y, nyh, n, Kp, split_len, nsplit, part_key,
-                                          part_idx, part_tau);
+    knn_tile_body<Op, KCH, KC, SPLIT, PF, 2>(Q, nq, Y, nyh, n, Kp, split_len, nsplit, part_key,
+                                             part_idx, part_tau);
 }
-template <class Op, int KCH, int KC, bool SPLIT, bool PF>
+template <class Op, int KCH, int KC, bool SPLIT, bool PF, int COLT>
 __global__ __launch_bounds__(64 * TileShape<KC>::WV)
-__attribute__((amdgpu_waves_per_eu(TileCfg<Op, KCH, KC, SPLIT>::WPE))) void knn_tile_kernel_k2(
-    const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
-    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
-    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
-    knn_tile_body<Op, KCH, KC, SPLIT, PF>(Q, nq, Y, nyh, n, Kp, split_len, nsplit, part_key,
-                                          part_idx, part_tau);
+__attribute__((amdgpu_waves_per_eu(TileCfg<Op, KCH, KC, SPLIT, COLT>::WPE))) void
+knn_tile_kernel_k2(const typename Op::in_t *__restrict__ Q, int nq,
+                   const typename Op::in_t *__restrict__ Y,
+                   const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len,
+                   int nsplit, float *__restrict__ part_key, int *__restrict__ part_idx,
+                   float *__restrict__ part_tau) {
+    knn_tile_body<Op, KCH, KC, SPLIT, PF, COLT>(Q, nq, Y, nyh, n, Kp, split_len, nsplit,
+                                                part_key, part_idx, part_tau);
 }
 
 constexpr int MMAX = 2048; // candidates merged per query (nsplit * kc)
@@ -752,22 +764,24 @@ int knn_tile_qb(int kc) {
 
 int knn_merge_max_candidates() { return MMAX; }
 
-int knn_tile_min_splits(int prec, int Kp, int n, int kc) {
+int knn_tile_min_splits(int prec, int Kp, int n, int kc, int variant) {
     const bool split = prec == KP_F16X3 || prec == KP_BF16X3;
     const bool wide = prec == KP_F16 || prec == KP_BF16 || split;   // 8 elements, 4 registers
     const int frag_regs = wide ? 4 : prec == KP_F64 ? 2 : 1;
     const int nf = wide ? Kp / 4 / 8 : Kp / 4;
     const int rt = kc <= 16 ? TileShape<16>::RT : 1;
-    if (!tile_short_index(kc, rt, split ? 2 : 1, nf, frag_regs, prec == KP_F64))
+    const int colt = variant == 2 && prec != KP_F64 ? 1 : 2;
+    if (!tile_short_index(kc, rt, split ? 2 : 1, nf, frag_regs, prec == KP_F64, colt))
         return 1;
     return (n + SPLIT16_MAX - 1) / SPLIT16_MAX;
 }
 
 
 // Launches the fused tile for one operand type, candidate-list length, width and variant
-// (0: loads of the next step under the current step's bookkeeping, 1: load then compute
-// step by step) or, with `slots` set, reports how many of its workgroups the device runs at
-// once instead.
+// (0: two 16-row training tiles per step, the next step's loads under the current step's
+// bookkeeping; 1: two tiles per step, load then compute; 2: as 0 with one tile per step,
+// which halves the training fragments held in registers) or, with `slots` set, reports how
+// many of its workgroups the device runs at once instead.
 template <class Op, int KCH, int KC, bool SPLIT>
 static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                 int Kp, int nsplit, float *part_key, int *part_idx,
@@ -778,12 +792,18 @@ static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void
                               float *, int *, float *);
     constexpr int QB = TileShape<KC>::WV * TileShape<KC>::RT * 16;
     kernel_t kern;
-    if constexpr (std::is_same_v<Op, OpF64>)
+    bool short_index = false;
+    if constexpr (std::is_same_v<Op, OpF64>) {
         kern = variant == 1 ? knn_tile_kernel<Op, KCH, KC, SPLIT, false>
                             : knn_tile_kernel<Op, KCH, KC, SPLIT, true>;
-    else
-        kern = variant == 1 ? knn_tile_kernel_k2<Op, KCH, KC, SPLIT, false>
-                            : knn_tile_kernel_k2<Op, KCH, KC, SPLIT, true>;
+    } else if (variant == 2) {
+        kern = knn_tile_kernel_k2<Op, KCH, KC, SPLIT, true, 1>;
+        short_index = TileCfg<Op, KCH, KC, SPLIT, 1>::SHORT;
+    } else {
+        kern = variant == 1 ? knn_tile_kernel_k2<Op, KCH, KC, SPLIT, false, 2>
+                            : knn_tile_kernel_k2<Op, KCH, KC, SPLIT, true, 2>;
+        short_index = TileCfg<Op, KCH, KC, SPLIT, 2>::SHORT;
+    }
     const dim3 block(64 * TileShape<KC>::WV);
     if (slots) {
         int dev = 0, cus = 0, per_cu = 0;
@@ -798,7 +818,7 @@ static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void
     int split_len = (n + nsplit - 1) / nsplit;
     split_len = (split_len + STEP - 1) / STEP * STEP;
     // Buffered row offsets must fit the index type.
-    if (TileCfg<Op, KCH, KC, SPLIT>::SHORT && split_len > SPLIT16_MAX)
+    if (short_index && split_len > SPLIT16_MAX)
         return hipErrorInvalidValue;
     const int qblocks = (nq + QB - 1) / QB;
     const dim3 grid((qblocks * nsplit + 7) / 8 * 8);

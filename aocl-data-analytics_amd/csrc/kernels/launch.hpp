@@ -177,7 +177,7 @@ int knn_merge_max_candidates();
 // Fewest training splits the fused tile accepts for n rows at candidate-list length kc,
 // operand precision prec and padded width Kp (sweeps that buffer 16-bit row offsets take
 // splits of at most 65,536 rows).
-int knn_tile_min_splits(int prec, int Kp, int n, int kc);
+int knn_tile_min_splits(int prec, int Kp, int n, int kc, int variant);
 // Operand precision of the fused tile.
 enum KnnPrec : int { KP_F32 = 0, KP_F64 = 1, KP_F16 = 2, KP_BF16 = 3, KP_F16X3 = 4, KP_BF16X3 = 5 };
 // Bytes per padded operand row element group: split forms store [hi Kp][lo Kp] per row.

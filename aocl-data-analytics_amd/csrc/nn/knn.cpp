@@ -65,9 +65,11 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
                      {"bf16x3", 5}},
                     "working");
     // MI355X extension: kernel variant of the fused tile (-1 = chosen by the tuning table,
-    // 0 = next step's loads under the current step's bookkeeping, 1 = load then compute).
+    // 0 = two 16-row training tiles per step with the next step's loads under the current
+    // step's bookkeeping, 1 = two tiles per step, load then compute, 2 = as 0 with one tile
+    // per step).
     opts.add_int("tile variant", "Kernel variant of the fused kNN distance tile (-1 automatic).",
-                 -1, LowerBound::greater_equal, 1, UpperBound::less_equal, -1);
+                 -1, LowerBound::greater_equal, 2, UpperBound::less_equal, -1);
     // MI355X extension: minimum length of the candidate list each split of the fused tile
     // keeps per query (rounded up to 16, 32, 64 or 128; 0 = the smallest that holds k). A
     // longer list lowers the split thresholds, so a low-precision form with a wider error
@@ -282,10 +284,12 @@ SearchContext::~SearchContext() {
 // queries per workgroup the grid has qblocks * nsplit workgroups and the device runs `slots`
 // of them at once; s1 = the fewest splits that fill about 90% of the slots. More splits
 // shorten each workgroup's sweep but restart more candidate lists (each start costs a burst
-// of threshold updates), so the best multiple of s1 depends on the operand form and the
-// data. The first batch of each size class (log2 of qblocks) times the multiples 1, 2 and 4
-// of s1 on its own queries and keeps the fastest; later batches reuse the choice. Splits are
-// capped by the merge's capacity (nsplit * kc candidates per query) and at 1024 rows each.
+// of threshold updates), and a grid just over a whole number of rounds leaves the device
+// idle in its last round, so the best count depends on the operand form and the data. The
+// first batch of each size class (log2 of qblocks) times the candidates on its own queries
+// and keeps the fastest; later batches reuse the choice. Splits are capped by the merge's
+// capacity (nsplit * kc candidates per query) and at 1024 rows each, and kept at or above
+// the minimum of sweeps with 16-bit row offsets.
 template <typename T>
 int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq,
                              const void *qpad, hipStream_t st) {
@@ -303,31 +307,36 @@ int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, 
     const int64_t qb = knn_tile_qb(kc);
     const int64_t qblocks = (nq + qb - 1) / qb;
     const int64_t hi = split_cap(kc);
-    const int64_t lo = knn_tile_min_splits(prec, kp_, (int)n, kc);
+    const int64_t lo = knn_tile_min_splits(prec, kp_, (int)n, kc, variant);
     const int64_t s1 = std::min<int64_t>(
         hi, std::max<int64_t>(lo, (slots * 9 / 10 + qblocks - 1) / qblocks));
     int bucket = 0;
     while (((int64_t)1 << (bucket + 1)) <= qblocks)
         bucket++;
     const int key = skey * 64 + bucket;
-    auto pick = [&](int f) { return (int)std::min<int64_t>(hi, s1 * f); };
+    auto clamp = [&](int64_t v) { return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, v)); };
     auto ft = ctx.split_factor.find(key);
     if (ft != ctx.split_factor.end())
-        return pick(ft->second);
-    // Time the candidates (a first untimed launch warms the caches).
-    int best_f = 1;
+        return ft->second;
+    // Candidates: the most splits whose workgroups all run in one round, and 1, 2 and 4
+    // times the count that fills 90% of the slots. Time them (a first untimed launch warms
+    // the caches) and keep the fastest.
+    std::vector<int> cand;
+    for (int64_t v : {(int64_t)slots / std::max<int64_t>(1, qblocks), s1, 2 * s1, 4 * s1}) {
+        const int c = clamp(v);
+        if (std::find(cand.begin(), cand.end(), c) == cand.end())
+            cand.push_back(c);
+    }
+    int best = cand[0];
     float best_ms = -1.0f;
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
         if (a)
             (void)hipEventDestroy(a);
-        return pick(1);
+        return clamp(s1);
     }
     bool warmed = false;
-    for (int f : {1, 2, 4}) {
-        if (f > 1 && pick(f) == pick(f / 2))
-            break;
-        const int ns = pick(f);
+    for (int ns : cand) {
         const size_t np = (size_t)nq * ns;
         if (ctx.part_key.reserve(np * kc * sizeof(float)) != hipSuccess ||
             ctx.part_idx.reserve(np * kc * sizeof(int)) != hipSuccess ||
@@ -353,13 +362,13 @@ int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, 
         (void)hipEventElapsedTime(&ms, a, b);
         if (best_ms < 0.0f || ms < best_ms) {
             best_ms = ms;
-            best_f = f;
+            best = ns;
         }
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    ctx.split_factor[key] = best_f;
-    return pick(best_f);
+    ctx.split_factor[key] = best;
+    return best;
 }
 
 template <typename T>
@@ -367,8 +376,11 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
                                      da_int nq, da_int k, bool return_distance, da_int *d_ind,
                                      T *d_dist, bool &fused) {
     const int kc = sp.tile_kc(k);
+    int variant = sp.variant;
+    if (variant < 0)
+        variant = knn_tile_auto_variant(sp.prec, kp_);
     fused = fused_eligible(k) && sp.prec == ypad_prec_ &&
-            knn_tile_min_splits(sp.prec, kp_, n_samples_, kc) <= split_cap(kc);
+            knn_tile_min_splits(sp.prec, kp_, n_samples_, kc, variant) <= split_cap(kc);
     ctx.last_fallback = 0;
     ctx.last_candidates = 0;
     if (!fused)
@@ -378,9 +390,6 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
     const int prec = sp.prec;
     if (!ctx.h_fb)
         DA_HIP(err, hipHostMalloc((void **)&ctx.h_fb, sizeof(int)));
-    int variant = sp.variant;
-    if (variant < 0)
-        variant = knn_tile_auto_variant(prec, kp_);
     const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
     DA_HIP(err, ctx.qpad.reserve((size_t)nq * kp_ * esz));
     DA_HIP(err, launch_pack_rows_prec<T>(dQ, d, nq, d, ctx.qpad.ptr(), kp_, nq, prec, st));

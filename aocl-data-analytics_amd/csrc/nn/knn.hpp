@@ -13,7 +13,7 @@ namespace dagpu {
 struct SearchContext {
     GpuContext *gpu = nullptr;
     std::map<int, int> slots;        // concurrent workgroups per fused-tile configuration
-    std::map<int, int> split_factor; // tuned multiple of the minimal split count
+    std::map<int, int> split_factor; // tuned split count per (form, list length, size class)
     DeviceBuffer q_norm, keys, sel_key, sel_idx, qpad, part_key, part_idx, part_tau, fb, fb_q,
         fb_ind, fb_dist;
     // Pinned host word receiving the fallback count of the last fused launch.

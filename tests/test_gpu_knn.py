@@ -212,13 +212,15 @@ def test_fused_large_k(oracle, k, precision):
     assert ncand.value >= k  # the fused tile path ran
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-@pytest.mark.parametrize("precision", ["working", "fp64", "fp16", "fp16x3"])
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("precision", ["working", "fp64", "fp16", "fp16x3", "bf16x3"])
 @pytest.mark.parametrize("d", [20, 64, 128])
-def test_tile_variants(oracle, variant, precision, d):
-    """Both sweeps of the fused tile (loads under the bookkeeping, load-then-compute) give the
-    exact result, including training sizes that end inside a step and splits of unequal
-    length."""
+@pytest.mark.parametrize("k", [12, 30])
+def test_tile_variants(oracle, variant, precision, d, k):
+    """Every sweep of the fused tile (two training tiles per step with the loads under the
+    bookkeeping, load-then-compute, one tile per step) gives the exact result, for short
+    (16) and longer (32) candidate lists, including training sizes that end inside a step
+    and splits of unequal length."""
     rng = np.random.default_rng(200 + d)
     means = rng.uniform(-5, 5, (8, d))
     for n in (1000, 4133):
@@ -228,7 +230,23 @@ def test_tile_variants(oracle, variant, precision, d):
         prec = "working" if precision == "fp64" else precision
         nn = aoclda.neighbors.nearest_neighbors(precision=prec, tile_variant=variant).fit(
             Xt.astype(dtype))
-        dist, ind = nn.kneighbors(Xq.astype(dtype), n_neighbors=12)
-        ei, ed = oracle.knn(Xt, Xq, 12)
+        dist, ind = nn.kneighbors(Xq.astype(dtype), n_neighbors=k)
+        ei, ed = oracle.knn(Xt, Xq, k)
         np.testing.assert_array_equal(ind, ei)
         np.testing.assert_allclose(dist, ed, rtol=1e-12 if dtype == np.float64 else 1e-5)
+
+
+@pytest.mark.parametrize("variant", [0, 2])
+def test_long_splits_16bit_offsets(oracle, variant):
+    """A training set of more than 65,536 rows: sweeps that buffer 16-bit row offsets take
+    at least two splits and still return the exact neighbours with the right indices."""
+    rng = np.random.default_rng(77)
+    d = 24
+    means = rng.uniform(-5, 5, (16, d))
+    Xt = gaussian_mixture(rng, 70001, means)
+    Xq = gaussian_mixture(rng, 48, means)
+    nn = aoclda.neighbors.nearest_neighbors(precision="bf16x3", tile_variant=variant).fit(Xt)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=10)
+    ei, ed = oracle.knn(Xt, Xq, 10)
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-5)
