@@ -11,11 +11,12 @@ class nearest_neighbors:
     def __init__(self, n_neighbors=5, weights="uniform", algorithm="auto", leaf_size=30,
                  metric="euclidean", p=2.0, radius=1.0, outlier_handling=None,
                  check_data=False, precision="working", tile_variant=-1,
-                 candidates_per_split=0):
+                 candidates_per_split=0, seed_thresholds=False):
         # precision: operand precision of the MFMA distance tiles ("working", "fp32",
         # "fp16", "bf16", "fp16x3", "bf16x3"); results are re-ranked exactly whatever the
         # choice. candidates_per_split: minimum candidate-list length per query and split of
-        # the fused tile (0 = k). Both are MI355X options.
+        # the fused tile (0 = k). seed_thresholds: bound the tile's thresholds with a sweep of
+        # a row subset first. All three are MI355X options that change speed only.
         if not isinstance(n_neighbors, int) and n_neighbors is not None:
             raise ValueError("Number of neighbors must be an integer or None. Got n_neighbors="
                              + str(n_neighbors))
@@ -26,6 +27,7 @@ class nearest_neighbors:
         self.precision = precision
         self.tile_variant = tile_variant
         self.candidates_per_split = candidates_per_split
+        self.seed_thresholds = seed_thresholds
         self._h = None
         self.order = None
         self.dtype = None
@@ -42,6 +44,7 @@ class nearest_neighbors:
         h.set_string("cross term precision", self.precision)
         h.set_int("tile variant", self.tile_variant)
         h.set_int("candidates per split", self.candidates_per_split)
+        h.set_string("seed thresholds", "yes" if self.seed_thresholds else "no")
 
     def fit(self, X, y=None):
         X, self.order, self.dtype = L.prepare(X)

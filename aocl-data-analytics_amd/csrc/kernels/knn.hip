@@ -325,14 +325,16 @@ __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][
 
 // Final compaction of a wave's queries and the per-(query, split) output: the KC best keys
 // and row indices (t_begin + buffered offset) and the split threshold tau (every key of the
-// split not kept is >= tau).
+// split not kept is >= tau): the KC-th kept key, or the seeded threshold tau0 where that is
+// lower (a row at or above the seed was never buffered).
 template <class Op, int KC, int CAP, int ROWT, typename IT>
 __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bkey, IT *bidx,
                                             int t_begin, int qbase, int q0, int nq, int nsplit,
                                             int split,
                                             float *__restrict__ part_key,
                                             int *__restrict__ part_idx,
-                                            float *__restrict__ part_tau) {
+                                            float *__restrict__ part_tau,
+                                            const float *__restrict__ tau0) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int rt = 0; rt < ROWT; rt++) {
@@ -353,7 +355,7 @@ __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bke
                     part_idx[o + u] = u < valid ? t_begin + (int)bidx[ql * CAP + u] : -1;
                 }
                 if (lane == 0)
-                    part_tau[(int64_t)q * nsplit + split] = t;
+                    part_tau[(int64_t)q * nsplit + split] = tau0 ? fminf(t, tau0[q]) : t;
             }
         }
     }
@@ -381,7 +383,8 @@ template <class Op, int KCH, int KC, bool SPLIT, bool PF, int COLT>
 __device__ __forceinline__ void knn_tile_body(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
-    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
+    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau,
+    const float *__restrict__ tau0) {
     using frag_t = typename Op::frag_t;
     using acc_t = typename Op::acc_t;
     using ae = typename Op::acc_e;
@@ -437,12 +440,16 @@ __device__ __forceinline__ void knn_tile_body(
             }
         }
     }
+    // Thresholds in score space (score > ts <=> key < -2 ts): -inf, or the seeded key bound
+    // of the query when one is given.
     ae ts[ROWT][4];
 #pragma unroll
     for (int rt = 0; rt < ROWT; rt++)
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-            ts[rt][i] = -__builtin_huge_val();
+        for (int i = 0; i < 4; i++) {
+            const int q = q0 + qbase + rt * 16 + (Op::rows_strided ? g + 4 * i : 4 * g + i);
+            ts[rt][i] = (tau0 && q < nq) ? (ae)(-0.5 * (double)tau0[q]) : (ae)-__builtin_huge_val();
+        }
     wave_fence();
 
     // Training tile t (rows 16t .. 16t+15) is stored as [half][chunk][lane][16 bytes]: the
@@ -538,8 +545,7 @@ __device__ __forceinline__ void knn_tile_body(
         }
     }
     write_split<Op, KC, CAP, ROWT>(cq, bkey, bidx, t_begin, qbase, q0, nq, nsplit, split,
-                                   part_key,
-                                   part_idx, part_tau);
+                                   part_key, part_idx, part_tau, tau0);
 }
 
 // The sweep as a kernel: K2 asks the compiler for registers that leave room for WPE waves
@@ -549,9 +555,10 @@ template <class Op, int KCH, int KC, bool SPLIT, bool PF>
 __global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
-    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau) {
+    float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau,
+    const float *__restrict__ tau0) {
     knn_tile_body<Op, KCH, KC, SPLIT, PF, 2>(Q, nq, Y, nyh, n, Kp, split_len, nsplit, part_key,
-                                             part_idx, part_tau);
+                                             part_idx, part_tau, tau0);
 }
 template <class Op, int KCH, int KC, bool SPLIT, bool PF, int COLT>
 __global__ __launch_bounds__(64 * TileShape<KC>::WV)
@@ -560,9 +567,9 @@ knn_tile_kernel_k2(const typename Op::in_t *__restrict__ Q, int nq,
                    const typename Op::in_t *__restrict__ Y,
                    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len,
                    int nsplit, float *__restrict__ part_key, int *__restrict__ part_idx,
-                   float *__restrict__ part_tau) {
+                   float *__restrict__ part_tau, const float *__restrict__ tau0) {
     knn_tile_body<Op, KCH, KC, SPLIT, PF, COLT>(Q, nq, Y, nyh, n, Kp, split_len, nsplit,
-                                                part_key, part_idx, part_tau);
+                                                part_key, part_idx, part_tau, tau0);
 }
 
 constexpr int MMAX = 2048; // candidates merged per query (nsplit * kc)
@@ -785,11 +792,12 @@ int knn_tile_min_splits(int prec, int Kp, int n, int kc, int variant) {
 template <class Op, int KCH, int KC, bool SPLIT>
 static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                 int Kp, int nsplit, float *part_key, int *part_idx,
-                                float *part_tau, int variant, int *slots, hipStream_t s) {
+                                float *part_tau, const float *tau0, int variant, int *slots,
+                                hipStream_t s) {
     using in_t = typename Op::in_t;
     using ae = typename Op::acc_e;
     using kernel_t = void (*)(const in_t *, int, const in_t *, const ae *, int, int, int, int,
-                              float *, int *, float *);
+                              float *, int *, float *, const float *);
     constexpr int QB = TileShape<KC>::WV * TileShape<KC>::RT * 16;
     kernel_t kern;
     bool short_index = false;
@@ -824,23 +832,24 @@ static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void
     const dim3 grid((qblocks * nsplit + 7) / 8 * 8);
     hipLaunchKernelGGL(kern, grid, block, 0, s, static_cast<const in_t *>(Q), nq,
                        static_cast<const in_t *>(Y), static_cast<const ae *>(nyh), n, Kp,
-                       split_len, nsplit, part_key, part_idx, part_tau);
+                       split_len, nsplit, part_key, part_idx, part_tau, tau0);
     return hipGetLastError();
 }
 
 template <class Op, int KC, bool SPLIT>
 static hipError_t launch_tile_op(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                  int Kp, int nsplit, float *part_key, int *part_idx,
-                                 float *part_tau, int variant, int *slots, hipStream_t s) {
+                                 float *part_tau, const float *tau0, int variant, int *slots,
+                                hipStream_t s) {
     switch (Kp / 4) {
     case 8:
-        return launch_tile_w<Op, 8, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+        return launch_tile_w<Op, 8, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
     case 16:
-        return launch_tile_w<Op, 16, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+        return launch_tile_w<Op, 16, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
     case 24:
-        return launch_tile_w<Op, 24, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+        return launch_tile_w<Op, 24, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
     case 32:
-        return launch_tile_w<Op, 32, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+        return launch_tile_w<Op, 32, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
     default:
         return hipErrorInvalidValue;
     }
@@ -855,10 +864,11 @@ int knn_tile_kp(int d) {
 template <int KC>
 static hipError_t launch_tile_kc(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                                  int n, int Kp, int nsplit, float *part_key, int *part_idx,
-                                 float *part_tau, int variant, int *slots, hipStream_t s) {
+                                 float *part_tau, const float *tau0, int variant, int *slots,
+                                hipStream_t s) {
 #define DA_OP(OP, SPLIT)                                                                          \
     launch_tile_op<OP, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau,     \
-                                  variant, slots, s)
+                                  tau0, variant, slots, s)
     switch (prec) {
     case KP_F32: return DA_OP(OpF32, false);
     case KP_F64: return DA_OP(OpF64, false);
@@ -873,16 +883,17 @@ static hipError_t launch_tile_kc(int prec, const void *Q, int nq, const void *Y,
 
 static hipError_t tile_dispatch(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                                 int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
-                                float *part_tau, int variant, int *slots, hipStream_t s) {
+                                float *part_tau, const float *tau0, int variant, int *slots,
+                                hipStream_t s) {
     switch (kc) {
     case 16:
-        return launch_tile_kc<16>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+        return launch_tile_kc<16>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
     case 32:
-        return launch_tile_kc<32>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+        return launch_tile_kc<32>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
     case 64:
-        return launch_tile_kc<64>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+        return launch_tile_kc<64>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
     case 128:
-        return launch_tile_kc<128>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, variant, slots, s);
+        return launch_tile_kc<128>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
     default:
         return hipErrorInvalidValue;
     }
@@ -890,14 +901,38 @@ static hipError_t tile_dispatch(int prec, const void *Q, int nq, const void *Y, 
 
 hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                            int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
-                           float *part_tau, int variant, hipStream_t s) {
+                           float *part_tau, const float *tau0, int variant, hipStream_t s) {
     return tile_dispatch(prec, Q, nq, Y, nyh, n, Kp, nsplit, kc, part_key, part_idx, part_tau,
-                         variant, nullptr, s);
+                         tau0, variant, nullptr, s);
 }
 
 hipError_t knn_tile_slots(int prec, int Kp, int kc, int variant, int *slots) {
     return tile_dispatch(prec, nullptr, 0, nullptr, nullptr, 0, Kp, 1, kc, nullptr, nullptr,
-                         nullptr, variant, slots, nullptr);
+                         nullptr, nullptr, variant, slots, nullptr);
+}
+
+namespace {
+// tau0[q] = the smallest split threshold of a sweep over a subset of the rows: the subset
+// holds kc rows with keys at most that value, so the kc-th key of the whole set is no larger.
+__global__ void knn_seed_kernel(const float *__restrict__ part_tau, int nq, int nsplit,
+                                float *__restrict__ tau0) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq)
+        return;
+    float t = __builtin_huge_valf();
+    for (int s = 0; s < nsplit; s++)
+        t = fminf(t, part_tau[(int64_t)q * nsplit + s]);
+    tau0[q] = t;
+}
+} // namespace
+
+hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau0,
+                           hipStream_t s) {
+    if (nq <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(knn_seed_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, part_tau, nq,
+                       nsplit, tau0);
+    return hipGetLastError();
 }
 
 template <typename T>

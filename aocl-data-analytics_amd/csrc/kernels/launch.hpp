@@ -190,12 +190,19 @@ int knn_tile_kp(int d);
 // layout of launch_knn_tile_operand, padded to a multiple of 32 rows; nyh: -|y|^2/2 per row
 // (double for KP_F64, float otherwise).
 // Writes part_key/part_idx[(q*nsplit + s)*kc + i] (keys |y|^2 - 2 q.y as float) and
-// part_tau[q*nsplit + s]. kc in {16, 32, 64, 128}. variant 0: the next step's loads are
-// issued under the current step's candidate bookkeeping; 1: each step loads, then computes.
+// part_tau[q*nsplit + s]. kc in {16, 32, 64, 128}. variant 0: two 16-row training tiles
+// per step, the next step's loads issued under the current step's candidate bookkeeping;
+// 1: two tiles per step, each step loads, then computes; 2: as 0 with one tile per step.
+// tau0 (may be null): per-query key bound that seeds the thresholds; rows with keys at or
+// above it are not buffered and part_tau is at most tau0[q].
 // nyh must hold a multiple of 32 entries (the padded rows at -inf).
 hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                            int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
-                           float *part_tau, int variant, hipStream_t s);
+                           float *part_tau, const float *tau0, int variant, hipStream_t s);
+// tau0[q] = min over the nsplit thresholds part_tau[q*nsplit + s] of a sweep (a valid seed
+// for a sweep of any superset of the swept rows).
+hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau0,
+                           hipStream_t s);
 // Q: nq x d and Y: n x d unpadded rows (type T); u_in: relative representation error of the
 // tile operands (0 when they are the data themselves); abs_in: absolute representation error
 // per element (fp16 subnormals); u_acc: unit roundoff of the tile accumulation;

@@ -15,7 +15,7 @@ struct SearchContext {
     std::map<int, int> slots;        // concurrent workgroups per fused-tile configuration
     std::map<int, int> split_factor; // tuned split count per (form, list length, size class)
     DeviceBuffer q_norm, keys, sel_key, sel_idx, qpad, part_key, part_idx, part_tau, fb, fb_q,
-        fb_ind, fb_dist;
+        fb_ind, fb_dist, tau0;
     // Pinned host word receiving the fallback count of the last fused launch.
     int *h_fb = nullptr;
     da_int last_fallback = 0, last_candidates = 0;
@@ -27,6 +27,7 @@ struct SearchParams {
     int prec = 0;     // operand precision of the fused tile (KP_*)
     int variant = -1; // fused tile kernel variant (-1: tuning table)
     int kc_min = 0;   // minimum candidates kept per (query, split) on the fused tile
+    bool seed = false; // seed the fused tile's thresholds from a sweep of a row subset
     // Candidate-list length of the fused tile for k neighbours.
     int tile_kc(int k) const;
 };
@@ -74,7 +75,7 @@ template <typename T> class NNHandle : public AlgHandle {
     bool fused_eligible(da_int k) const;
     int split_cap(int kc) const;
     int tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq,
-                    const void *qpad, hipStream_t st);
+                    const void *qpad, const float *seed, hipStream_t st);
     da_status prepare_fused(int prec);
 
     bool trained_ = false;

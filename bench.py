@@ -69,10 +69,11 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1):
+def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1, seed=True):
     prec, kc_min = split_form(precision)
     nn = aoclda.neighbors.nearest_neighbors(n_neighbors=k, precision=prec, tile_variant=variant,
-                                            candidates_per_split=kc_min).fit(Xt)
+                                            candidates_per_split=kc_min,
+                                            seed_thresholds=seed).fit(Xt)
     h = nn.handle
     di, dd = C.c_void_p(), C.c_void_p()
     L.check(L.lib.da_amd_malloc(nq * k * 4, C.byref(di)))
@@ -103,7 +104,7 @@ def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1):
         step()
     L.check(L.lib.da_amd_synchronize(h.h))
     kern = {}
-    for name in ("knn_tile", "knn_merge", "pairwise_direct", "row_topk"):
+    for name in ("knn_seed", "knn_tile", "knn_merge", "pairwise_direct", "row_topk"):
         ms, cnt = C.c_double(), L.da_int()
         L.check(L.lib.da_amd_profile_get(h.h, name.encode(), C.byref(ms), C.byref(cnt)))
         if cnt.value:
@@ -415,6 +416,8 @@ def main():
                     help="cross-term forms of the batched k-means mode")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="fused tile kernel variant (-1: tuning table)")
+    ap.add_argument("--seed", action="store_true",
+                    help="seed the tile thresholds from a sweep of a row subset")
     ap.add_argument("--cpu-sample", type=int, default=1000,
                     help="queries checked against (and timed with) the host oracle")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -484,7 +487,7 @@ def main():
     results = {}
     for form in args.forms.split(","):
         results[form] = run_form(L, aoclda, Xt, dq, nq, d, k, form, args.steps, args.warmup,
-                                 args.tile_variant)
+                                 args.tile_variant, args.seed)
     L.lib.da_amd_free(dq)
 
     cpu = None

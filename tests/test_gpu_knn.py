@@ -250,3 +250,24 @@ def test_long_splits_16bit_offsets(oracle, variant):
     ei, ed = oracle.knn(Xt, Xq, 10)
     np.testing.assert_array_equal(ind, ei)
     np.testing.assert_allclose(dist, ed, rtol=1e-5)
+
+@pytest.mark.parametrize("precision", ["working", "fp16", "bf16x3"])
+@pytest.mark.parametrize("k", [10, 30, 100])
+def test_seeded_thresholds(oracle, precision, k):
+    """Seeding the thresholds from a sweep of the first sixteenth of the rows (training sets
+    of 32,768 rows or more) returns exactly the neighbours of the unseeded sweep and of the
+    oracle, also for queries far from every training row (whose seeds are loose) and for
+    plain fp16, whose wider error bound sends queries to the exact fallback."""
+    rng = np.random.default_rng(91)
+    d = 40
+    means = rng.uniform(-5, 5, (32, d))
+    Xt = gaussian_mixture(rng, 40000, means)
+    Xq = np.concatenate([gaussian_mixture(rng, 90, means),
+                         rng.uniform(-40, 40, (10, d)).astype(np.float32)])
+    ei, ed = oracle.knn(Xt, Xq, k)
+    for seed in (True, False):
+        nn = aoclda.neighbors.nearest_neighbors(precision=precision,
+                                                seed_thresholds=seed).fit(Xt)
+        dist, ind = nn.kneighbors(Xq, n_neighbors=k)
+        np.testing.assert_array_equal(ind, ei, err_msg=f"seed={seed}")
+        np.testing.assert_allclose(dist, ed, rtol=1e-5)
