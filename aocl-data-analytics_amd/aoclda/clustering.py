@@ -151,8 +151,8 @@ class kmeans:
 def kmeans_batch(datasets, init_centres, max_iter=300, tol=1.0e-4, precision="working"):
     """Batched Lloyd k-means (MI355X extension da_kmeans_batch_device_s): instance i clusters
     the fp32 rows datasets[i] (n_i x d_i, d_i <= 64) from the initial centres init_centres[i]
-    (k_i x d_i, k_i <= 64). precision: MFMA operand type of the assignment, "working" (fp32)
-    or "bf16" (labels exact either way). Returns (centres, labels, n_iter, inertia) as lists /
+    (k_i x d_i, k_i <= 64). precision: MFMA operand type of the assignment, "working" (fp32),
+    "bf16" or "bf16x3" (hi + lo split; labels exact in every form). Returns (centres, labels, n_iter, inertia) as lists /
     arrays, one entry per instance."""
     X = [np.ascontiguousarray(a, dtype=np.float32) for a in datasets]
     C0 = [np.ascontiguousarray(c, dtype=np.float32) for c in init_centres]
@@ -172,7 +172,7 @@ def kmeans_batch(datasets, init_centres, max_iter=300, tol=1.0e-4, precision="wo
         L.check(L.lib.da_amd_memcpy_htod(dc, L.ptr(cs), cs.nbytes))
         n_iter = np.zeros(len(X), dtype=np.int32)
         inertia = np.zeros(len(X), dtype=np.float32)
-        ct = {"working": 0, "fp32": 0, "bf16": 1}[precision]
+        ct = {"working": 0, "fp32": 0, "bf16": 1, "bf16x3": 2}[precision]
         L.check(L.lib.da_kmeans_batch_device_s(len(X), L.ptr(n), L.ptr(d), L.ptr(k), dx, dc, dl,
                                                max_iter, tol, ct, L.ptr(n_iter), L.ptr(inertia)))
         cout = np.empty_like(cs)

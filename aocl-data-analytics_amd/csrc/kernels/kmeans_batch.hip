@@ -27,13 +27,16 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef float v2f __attribute__((ext_vector_type(2)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
-// Operand forms of the assignment cross term. E: elements per lane per MFMA; UX, UC: unit
-// roundoff of the data and of the centres in the operand type (fp32 data are exact).
+// Operand forms of the assignment cross term. E: elements per lane per MFMA; NS: parts per
+// value (2: hi + lo in bf16, rows stored [hi kp][lo kp], products hi*hi + hi*lo + lo*hi);
+// UX, UC: relative error of a product term charged to the data and to the centres (fp32
+// data are exact in fp32; centres are rounded from double).
+constexpr double KB_U32 = 1.0 / 16777216.0, KB_UB16 = 1.0 / 256.0;
 struct KbBF16 {
     using T = __bf16;
     using frag = v8bf;
-    static constexpr int E = 8;
-    static constexpr double UX = 1.0 / 256.0, UC = 1.0 / 256.0;
+    static constexpr int E = 8, NS = 1;
+    static constexpr double UX = KB_UB16, UC = KB_UB16;
     __device__ static v4f mfma(frag a, frag b, v4f c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
@@ -41,10 +44,23 @@ struct KbBF16 {
 struct KbF32 {
     using T = float;
     using frag = float;
-    static constexpr int E = 1;
-    static constexpr double UX = 0.0, UC = 1.0 / 16777216.0;
+    static constexpr int E = 1, NS = 1;
+    static constexpr double UX = 0.0, UC = KB_U32;
     __device__ static v4f mfma(frag a, frag b, v4f c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+// hi + lo split: |x - (hi + lo)| <= ub16^2 |x| and the dropped lo*lo <= ub16^2 (1 + ub16)^2
+// |x||c|, so a product carries at most 3 ub16^2 (1 + ub16)^2 |x||c|, charged half to each
+// operand; the centres add their rounding to fp32.
+struct KbBF16X3 {
+    using T = __bf16;
+    using frag = v8bf;
+    static constexpr int E = 8, NS = 2;
+    static constexpr double UX = 1.5 * KB_UB16 * KB_UB16 * (1 + KB_UB16) * (1 + KB_UB16),
+                            UC = UX + KB_U32;
+    __device__ static v4f mfma(frag a, frag b, v4f c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 };
 
@@ -140,10 +156,16 @@ __global__ __launch_bounds__(256) void kb_to_op_kernel(const float *__restrict__
     const KChunk ch = chunks[blockIdx.x];
     const int inst = ch.inst, d = b.d[inst], kp = b.kp[inst];
     const float *x = X + b.xoff[inst] + (int64_t)ch.start * d;
-    T *y = Xo + b.xboff[inst] + (int64_t)ch.start * kp;
+    const int rs = kp * Op::NS;
+    T *y = Xo + b.xboff[inst] + (int64_t)ch.start * rs;
     const int spp = 256 / kp, sub = (int)threadIdx.x / kp, l = (int)threadIdx.x - sub * kp;
-    for (int i = sub; i < ch.count; i += spp)
-        y[(int64_t)i * kp + l] = l < d ? (T)x[(int64_t)i * d + l] : (T)0.0f;
+    for (int i = sub; i < ch.count; i += spp) {
+        const float v = l < d ? x[(int64_t)i * d + l] : 0.0f;
+        const T hi = (T)v;
+        y[(int64_t)i * rs + l] = hi;
+        if constexpr (Op::NS == 2)
+            y[(int64_t)i * rs + kp + l] = (T)(v - (float)hi);
+    }
 }
 
 // Initial centres (fp32, packed k x d per instance) into the double master copy.
@@ -199,9 +221,14 @@ __global__ __launch_bounds__(64) void kb_centre_prep_kernel(KBatchDesc b, const 
     const int c = threadIdx.x;
     double s = 0.0;
     if (c < kpad) {
+        T *row = Cb + b.cboff[inst] + (int64_t)c * kp * Op::NS;
         for (int l = 0; l < kp; l++) {
             const double v = (c < k && l < d) ? C[(int64_t)c * d + l] : 0.0;
-            Cb[b.cboff[inst] + (int64_t)c * kp + l] = (T)(float)v;
+            const float f = (float)v;
+            const T hi = (T)f;
+            row[l] = hi;
+            if constexpr (Op::NS == 2)
+                row[kp + l] = (T)(f - (float)hi);
             s = fma(v, v, s);
         }
         if (c < k)
@@ -219,7 +246,8 @@ __global__ __launch_bounds__(64) void kb_centre_prep_kernel(KBatchDesc b, const 
 }
 
 // Assignment of one tile: 16*PT samples per wave against the instance's centres in LDS.
-// Lane group g holds features g*kp/4 .. of its sample (the k-permutation), NF fragments.
+// Lane group g holds features g*kp/4 .. of its sample (the k-permutation), NF fragments per
+// part (NS parts: the value, or its hi and lo).
 template <class Op, int NF, int PT>
 __device__ __forceinline__ void kb_assign_tile(const typename Op::T *__restrict__ Xo, int n,
                                                int kp, int start, const typename Op::T *Cs,
@@ -233,25 +261,35 @@ __device__ __forceinline__ void kb_assign_tile(const typename Op::T *__restrict_
     const int r = lane & 15, g = lane >> 4;
     const int kpad = (k + 15) / 16 * 16;
     const int s0 = start + w * PT * 16;
-    frag bb[PT][NF];
+    constexpr int NS = Op::NS;
+    frag bb[PT][NS][NF];
     float xn[PT];
 #pragma unroll
     for (int pt = 0; pt < PT; pt++) {
         const int i = s0 + pt * 16 + r;
-        kb_load<Op, NF>(Xo + (int64_t)(i < n ? i : 0) * kp + g * KCH, bb[pt]);
+#pragma unroll
+        for (int h = 0; h < NS; h++) {
+            kb_load<Op, NF>(Xo + (int64_t)(i < n ? i : 0) * kp * NS + h * kp + g * KCH,
+                            bb[pt][h]);
+            if (i >= n) {
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+                    bb[pt][h][f] = frag{};
+            }
+        }
         float s = 0.f;
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-            if (i >= n)
-                bb[pt][f] = frag{};
             if constexpr (Op::E == 8) {
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
-                    const float x = (float)bb[pt][f][e];
+                    float x = (float)bb[pt][0][f][e];
+                    if constexpr (NS == 2)
+                        x += (float)bb[pt][NS - 1][f][e];
                     s = fmaf(x, x, s);
                 }
             } else {
-                s = fmaf(bb[pt][f], bb[pt][f], s);
+                s = fmaf(bb[pt][0][f], bb[pt][0][f], s);
             }
         }
         xn[pt] = s;
@@ -265,8 +303,10 @@ __device__ __forceinline__ void kb_assign_tile(const typename Op::T *__restrict_
         bidx[pt] = INT_MAX;
     }
     for (int c0 = 0; c0 < kpad; c0 += 16) {
-        frag a[NF];
-        kb_load<Op, NF>(Cs + (c0 + r) * kp + g * KCH, a);
+        frag a[NS][NF];
+#pragma unroll
+        for (int h = 0; h < NS; h++)
+            kb_load<Op, NF>(Cs + (c0 + r) * kp * NS + h * kp + g * KCH, a[h]);
         v4f h;
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -275,8 +315,13 @@ __device__ __forceinline__ void kb_assign_tile(const typename Op::T *__restrict_
         for (int pt = 0; pt < PT; pt++) {
             v4f acc = h;
 #pragma unroll
-            for (int f = 0; f < NF; f++)
-                acc = Op::mfma(a[f], bb[pt][f], acc);
+            for (int f = 0; f < NF; f++) {
+                acc = Op::mfma(a[0][f], bb[pt][0][f], acc);
+                if constexpr (NS == 2) {
+                    acc = Op::mfma(a[0][f], bb[pt][NS - 1][f], acc);
+                    acc = Op::mfma(a[NS - 1][f], bb[pt][0][f], acc);
+                }
+            }
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const float v = acc[i];
@@ -317,7 +362,9 @@ __device__ __forceinline__ void kb_assign_tile(const typename Op::T *__restrict_
             // gamma_{kp+2} over the rounded products and -|c|^2/2, the rounded -|c|^2/2;
             // |x| from the operand copy (relative error <= ux) with a margin for the sum.
             const double xnorm = sqrt((double)xs) / (1.0 - ux) * (1.0 + 1e-6);
-            const double gam = (kp + 2) * u32 / (1.0 - (kp + 2) * u32);
+            // fp32 accumulation of NS == 2 ? 3 kp : kp products and -|c|^2/2.
+            const int terms = (NS == 2 ? 3 : 1) * kp + 2;
+            const double gam = terms * u32 / (1.0 - terms * u32);
             const double S = xnorm * cmax;
             const double e = (ux + uc + ux * uc) * S +
                              gam * (S * (1.0 + ux) * (1.0 + uc) + 0.5 * cmax * cmax) +
@@ -345,9 +392,11 @@ __device__ __forceinline__ void kb_assign_tile(const typename Op::T *__restrict_
     }
 }
 
-// Persistent assignment: workgroup b takes tiles b, b + G, b + 2G, ... of the list (sorted
-// by decreasing cost, so the round-robin deal is balanced); the centres of an instance are
-// staged once for a run of its tiles.
+// Persistent assignment: the tile list (sorted by decreasing cost, the tiles of an instance
+// adjacent) is dealt in runs of KB_RUN tiles, workgroup b taking runs b, b + G, b + 2G, ...:
+// the round-robin deal keeps the grid balanced as instances converge and drop out, and the
+// centres of an instance are staged once per run.
+constexpr int KB_RUN = 4;
 template <class Op, int PT>
 __global__ __launch_bounds__(256) void kb_assign_kernel(
     const typename Op::T *__restrict__ Xo, KBatchDesc b, const KTile *__restrict__ tiles,
@@ -356,10 +405,11 @@ __global__ __launch_bounds__(256) void kb_assign_kernel(
     const double *__restrict__ cmax2, int *__restrict__ L0, int *__restrict__ L1,
     int *__restrict__ changed, int *__restrict__ list_count, KTile *__restrict__ list) {
     using T = typename Op::T;
-    __shared__ __attribute__((aligned(16))) T Cs[64 * 64];
+    __shared__ __attribute__((aligned(16))) T Cs[64 * 64 * Op::NS];
     __shared__ float Hs[64];
     int staged = -1;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = blockIdx.x * KB_RUN; t < ntiles;
+         t += (t % KB_RUN == KB_RUN - 1) ? (gridDim.x - 1) * KB_RUN + 1 : 1) {
         const KTile tl = tiles[t];
         const int inst = tl.inst;
         if (!kb_runs(state, inst, mode))
@@ -371,7 +421,7 @@ __global__ __launch_bounds__(256) void kb_assign_kernel(
             const int kpad = (k + 15) / 16 * 16;
             const uint4 *src = reinterpret_cast<const uint4 *>(Cb + b.cboff[inst]);
             uint4 *dst = reinterpret_cast<uint4 *>(Cs);
-            for (int q = threadIdx.x; q < kpad * kp * (int)sizeof(T) / 16; q += 256)
+            for (int q = threadIdx.x; q < kpad * kp * Op::NS * (int)sizeof(T) / 16; q += 256)
                 dst[q] = src[q];
             if (threadIdx.x < 64)
                 Hs[threadIdx.x] = (int)threadIdx.x < k ? chalf[inst * 64 + threadIdx.x]
@@ -739,6 +789,9 @@ hipError_t kb_launch_prepare(const float *X, const float *Cinit, const KBatchDes
     if (prec == 1)
         hipLaunchKernelGGL(kb_to_op_kernel<KbBF16>, dim3(nchunks), dim3(256), 0, s, X, b, chunks,
                            reinterpret_cast<__bf16 *>(Xo));
+    else if (prec == 2)
+        hipLaunchKernelGGL(kb_to_op_kernel<KbBF16X3>, dim3(nchunks), dim3(256), 0, s, X, b,
+                           chunks, reinterpret_cast<__bf16 *>(Xo));
     else
         hipLaunchKernelGGL(kb_to_op_kernel<KbF32>, dim3(nchunks), dim3(256), 0, s, X, b, chunks,
                            reinterpret_cast<float *>(Xo));
@@ -777,6 +830,8 @@ hipError_t kb_launch_assign(const KBatchBuffers &w, const KBatchDesc &b, int n_i
                        w.list_count, w.changed, n_inst);
     if (w.prec == 1)
         kb_assign_op<KbBF16>(w, b, n_inst, mode, par, pt, grid, s);
+    else if (w.prec == 2)
+        kb_assign_op<KbBF16X3>(w, b, n_inst, mode, par, pt, grid, s);
     else
         kb_assign_op<KbF32>(w, b, n_inst, mode, par, pt, grid, s);
     hipLaunchKernelGGL(kb_recheck_kernel, dim3(1024), dim3(256), 0, s, w.X, b, w.C0, w.C1, w.fpar,
@@ -821,6 +876,9 @@ int kb_assign_blocks_per_cu(int prec, int pt) {
     if (prec == 1)
         e = pt == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbBF16, 2>, 256, 0)
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbBF16, 4>, 256, 0);
+    else if (prec == 2)
+        e = pt == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbBF16X3, 2>, 256, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbBF16X3, 4>, 256, 0);
     else
         e = pt == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbF32, 2>, 256, 0)
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_assign_kernel<KbF32, 4>, 256, 0);

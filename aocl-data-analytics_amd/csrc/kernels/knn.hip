@@ -48,16 +48,18 @@ template <> struct TileShape<64> { static constexpr int CAP = 128, WV = 4, RT = 
 template <> struct TileShape<128> { static constexpr int CAP = 256, WV = 2, RT = 1; };
 
 // Whether a sweep keeps its buffered row offsets in 16 bits (splits of at most 65,536 rows):
-// that cuts a workgroup's buffers to 48 KB, so three workgroups fit a CU's LDS, which pays
-// only where the registers also leave room for three waves per SIMD -- the query fragments
-// plus the current and prefetched training fragments (ns * nf fragments of frag_regs 32-bit
-// registers, for rt query tiles and 2 x colt training tiles, colt 16-row tiles per step)
-// within 144 of the 168 registers a wave then gets (the sweep's other state takes about
-// 24). kc <= 32; the fp64 sweep runs one wave per SIMD and never qualifies. Evaluated by the
-// kernels and by the host's bound on the split count.
+// that cuts a workgroup's buffers from 64 to 48 KB, so three workgroups fit a CU's LDS,
+// which pays only where the registers also leave room for three waves per SIMD -- the query
+// fragments plus the current and prefetched training fragments (ns * nf fragments of
+// frag_regs 32-bit registers, for rt query tiles and 2 x colt training tiles, colt 16-row
+// tiles per step) within 144 of the 168 registers a wave then gets (the sweep's other state
+// takes about 24). kc <= 64. The longest lists (kc = 128) keep 32-bit offsets: the extra
+// splits that 65,536-row splits would force cost more (each split restarts a 128-entry list)
+// than half a wave more per SIMD gains. The fp64 sweep runs one wave per SIMD and never
+// qualifies. Evaluated by the kernels and by the host's bound on the split count.
 constexpr bool tile_short_index(int kc, int rt, int ns, int nf, int frag_regs, bool f64,
                                 int colt) {
-    return !f64 && kc <= 32 && ns * nf * frag_regs * (rt + 2 * colt) <= 144;
+    return !f64 && kc <= 64 && ns * nf * frag_regs * (rt + 2 * colt) <= 144;
 }
 constexpr int SPLIT16_MAX = 65536; // rows per split with 16-bit offsets
 

@@ -272,11 +272,11 @@ struct KChunk {
 // Device work buffers of a batch.
 struct KBatchBuffers {
     const float *X;
-    int prec;                         // cross term: 0 fp32, 1 bf16
-    void *Xo;                         // rows in the operand type, padded to kp
+    int prec;                         // cross term: 0 fp32, 1 bf16, 2 bf16 hi + lo
+    void *Xo;                         // rows in the operand type, padded to kp (x2 for hi + lo)
     double *C0, *C1;                  // centres by parity
     int *L0, *L1;                     // labels by parity
-    void *Cb;                         // centres in the operand type (kpad x kp)
+    void *Cb;                         // centres in the operand type (kpad x kp, x2 for hi + lo)
     float *chalf;                     // -|c|^2/2, 64 per instance
     double *cmax2;                    // max |c|^2 per instance
     const KTile *tiles;               // assignment tiles

@@ -71,9 +71,10 @@ template <typename V> hipError_t upload(DeviceBuffer &b, const std::vector<V> &v
 
 // Builds the work lists and buffers of a batch and tunes the tile shape.
 // Padded width of an instance's operand copy: the bf16 MFMA takes 32 features per step (so
-// 32 or 64); the fp32 one 4 per lane group, padded to 8, 16, 32 or 64 for vector loads.
+// 32 or 64, for bf16 and the hi + lo split); the fp32 one 4 per lane group, padded to 8, 16,
+// 32 or 64 for vector loads.
 int op_width(int prec, int d) {
-    if (prec == 1)
+    if (prec == 1 || prec == 2)
         return d <= 32 ? 32 : 64;
     int kp = 8;
     while (kp < d)
@@ -88,6 +89,7 @@ da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, co
     std::vector<int> hn(n, n + n_inst), hd(d, d + n_inst), hk(k, k + n_inst), hkp(n_inst);
     std::vector<int64_t> xoff(n_inst), xboff(n_inst), coff(n_inst), cboff(n_inst), loff(n_inst);
     int64_t xo = 0, xbo = 0, co = 0, cbo = 0, lo = 0;
+    const int parts = prec == 2 ? 2 : 1; // operand rows [hi kp][lo kp] for the split form
     for (int i = 0; i < n_inst; i++) {
         hkp[i] = op_width(prec, hd[i]);
         xoff[i] = xo;
@@ -96,9 +98,9 @@ da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, co
         cboff[i] = cbo;
         loff[i] = lo;
         xo += (int64_t)hn[i] * hd[i];
-        xbo += (int64_t)hn[i] * hkp[i];
+        xbo += (int64_t)hn[i] * hkp[i] * parts;
         co += (int64_t)hk[i] * hd[i];
-        cbo += (int64_t)((hk[i] + 15) / 16 * 16) * hkp[i];
+        cbo += (int64_t)((hk[i] + 15) / 16 * 16) * hkp[i] * parts;
         lo += hn[i];
     }
     r.ctotal = co;
@@ -138,7 +140,7 @@ da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, co
     BR(upload(r.poff, poff, st));
     BR(upload(r.cbeg, cbeg, st));
     BR(upload(r.cend, cend, st));
-    const size_t esz = prec == 1 ? 2 : 4;
+    const size_t esz = prec == 0 ? 4 : 2;
     BR(r.xb.reserve((size_t)std::max<int64_t>(1, xbo) * esz));
     BR(r.c0.reserve((size_t)std::max<int64_t>(1, co) * sizeof(double)));
     BR(r.c1.reserve((size_t)std::max<int64_t>(1, co) * sizeof(double)));
@@ -201,7 +203,8 @@ da_status build(BatchRun &r, da_int n_inst, const da_int *n, const da_int *d, co
 }
 
 // Assignment tiles of 64*pt samples, instances in decreasing order of per-sample cost
-// (k * kp) and size, so that the persistent grid ends on small tiles.
+// (k * kp) and size, the tiles of an instance adjacent, so that the persistent grid ends on
+// small tiles and a run of dealt tiles mostly shares one instance's centres.
 std::vector<KTile> make_tiles(int n_inst, const da_int *n, const da_int *d, const da_int *k,
                               int pt, int prec) {
     std::vector<int> order(n_inst);
@@ -304,7 +307,7 @@ da_status da_kmeans_batch_device_s(da_int n_inst, const da_int *n, const da_int 
         return da_status_invalid_array_dimension;
     if (!n || !d || !k || !X || !C || !labels || !n_iter || !inertia)
         return da_status_invalid_pointer;
-    if (max_iter < 1 || !(tol >= 0.0f) || cross_term < 0 || cross_term > 1)
+    if (max_iter < 1 || !(tol >= 0.0f) || cross_term < 0 || cross_term > 2)
         return da_status_invalid_input;
     for (da_int i = 0; i < n_inst; i++) {
         if (n[i] < 1 || d[i] < 1 || d[i] > 64 || k[i] < 1 || k[i] > 64 || k[i] > n[i])

@@ -77,11 +77,10 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
     opts.add_int("candidates per split",
                  "Minimum candidates kept per query and split by the fused kNN tile (0 = k).", 0,
                  LowerBound::greater_equal, 128, UpperBound::less_equal, 0);
-    // MI355X extension: a sweep of the first sixteenth of the rows (training sets of 32,768
-    // rows or more) bounds each query's candidate-list threshold before the full sweep, so
-    // that the full sweep buffers far fewer rows. Speed only: the result is exact either way.
-    // Off by default: on a 200,000-row set the sweep saves 9% of the full sweep and costs
-    // more than that itself.
+    // MI355X extension: a short sweep of the first 1/64 of the rows (training sets of
+    // 131,072 rows or more) bounds each query's candidate-list threshold before the full
+    // sweep, so that the full sweep buffers far fewer rows. Speed only: the result is exact
+    // either way.
     opts.add_string("seed thresholds",
                     "Seed the fused kNN tile's thresholds from a sweep of a row subset.",
                     {{"yes", 1}, {"no", 0}}, "no");
@@ -402,28 +401,23 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
     const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
     DA_HIP(err, ctx.qpad.reserve((size_t)nq * kp_ * esz));
     DA_HIP(err, launch_pack_rows_prec<T>(dQ, d, nq, d, ctx.qpad.ptr(), kp_, nq, prec, st));
-    // Threshold seeds from a sweep of the first sixteenth of the rows, with as many splits
-    // as fill the device (its thresholds bound each query's kc-th key over the whole set).
+    // Threshold seeds from a 16-candidate sweep of the first 1/64 of the rows in splits of at
+    // least 8,192 rows (few list restarts). Its split thresholds bound each query's 16th key
+    // over the subset, about the 1,024th over the whole set: far above the k-th (k <= 128),
+    // so the seeded sweep still certifies, while it buffers only the rows below the seed.
     const float *seed = nullptr;
-    const int sub = sp.seed && n >= 32768 ? (n / 16 + 31) / 32 * 32 : 0;
+    const int sub = sp.seed && n >= 131072 ? (n / 64 + 31) / 32 * 32 : 0;
     if (sub > 0) {
         KernelScope ks(*ctx.gpu, "knn_seed");
-        int slots = 0;
-        if (knn_tile_slots(prec, kp_, kc, variant, &slots) != hipSuccess || slots <= 0)
-            slots = 512;
-        const int64_t qblocks = (nq + knn_tile_qb(kc) - 1) / knn_tile_qb(kc);
-        const int64_t hi = std::max<int64_t>(
-            1, std::min<int64_t>(knn_merge_max_candidates() / kc, ((int64_t)sub + 1023) / 1024));
-        const int64_t lo = knn_tile_min_splits(prec, kp_, sub, kc, variant);
-        const int ns = (int)std::max<int64_t>(
-            lo, std::min<int64_t>(hi, (slots * 9 / 10 + qblocks - 1) / qblocks));
+        const int lo = knn_tile_min_splits(prec, kp_, sub, 16, variant);
+        const int ns = std::max(lo, std::min(knn_merge_max_candidates() / 16, sub / 8192));
         const size_t np0 = (size_t)nq * ns;
-        DA_HIP(err, ctx.part_key.reserve(np0 * kc * sizeof(float)));
-        DA_HIP(err, ctx.part_idx.reserve(np0 * kc * sizeof(int)));
+        DA_HIP(err, ctx.part_key.reserve(np0 * 16 * sizeof(float)));
+        DA_HIP(err, ctx.part_idx.reserve(np0 * 16 * sizeof(int)));
         DA_HIP(err, ctx.part_tau.reserve(np0 * sizeof(float)));
         DA_HIP(err, ctx.tau0.reserve((size_t)nq * sizeof(float)));
         DA_HIP(err, launch_knn_tile(prec, ctx.qpad.ptr(), nq, ypad_.ptr(), nyh_.ptr(), sub, kp_,
-                                    ns, kc, ctx.part_key.as<float>(), ctx.part_idx.as<int>(),
+                                    ns, 16, ctx.part_key.as<float>(), ctx.part_idx.as<int>(),
                                     ctx.part_tau.as<float>(), nullptr, variant, st));
         DA_HIP(err, launch_knn_seed(ctx.part_tau.as<float>(), nq, ns, ctx.tau0.as<float>(), st));
         seed = ctx.tau0.as<float>();

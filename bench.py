@@ -353,9 +353,9 @@ def kmeans_batch_bench(args, L, aoclda):
 
     nsum = int(n.sum())
     for form in args.kb_forms.split(","):
-        ct = {"working": 0, "fp32": 0, "bf16": 1}[form]
-        esz = 2 if ct == 1 else 4
-        kp = np.where(d <= 32, 32, 64) if ct == 1 else np.maximum(8, 2 ** np.ceil(
+        ct = {"working": 0, "fp32": 0, "bf16": 1, "bf16x3": 2}[form]
+        esz = {0: 4, 1: 2, 2: 4}[ct]  # bytes per padded element (the split form: hi + lo)
+        kp = np.where(d <= 32, 32, 64) if ct else np.maximum(8, 2 ** np.ceil(
             np.log2(d)).astype(np.int64))
 
         def run():
@@ -384,7 +384,7 @@ def kmeans_batch_bench(args, L, aoclda):
             "metric": BATCH_METRIC, "value": iters / t, "unit": "instance-iterations/s",
             "n_gpus": 1, "higher_is_better": True, "ms_per_batch": t * 1e3,
             "first_call_ms": first * 1e3, "reps": args.kb_reps,
-            "dtype": "bf16" if ct == 1 else "f32",
+            "dtype": "f32" if ct == 0 else "bf16",
             "data": "synthetic: Gaussian blobs per instance, numpy default_rng(5)",
             "config": {"workload": f"{ni} k-means instances, n log-uniform [1e3,2e5], d in "
                                    f"{{8,16,32,64}}, k in [4,64], {args.kb_iter} Lloyd "
@@ -412,7 +412,7 @@ def main():
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--forms", default="working,fp16x3,bf16x3")
-    ap.add_argument("--kb-forms", default="working,bf16",
+    ap.add_argument("--kb-forms", default="working,bf16x3,bf16",
                     help="cross-term forms of the batched k-means mode")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="fused tile kernel variant (-1: tuning table)")

@@ -112,9 +112,9 @@ void da_nn_server_destroy(da_nn_server *server);
  * columns), C holds the initial centres (k[i] x d[i], row-major) on entry and the final
  * centres on return, labels receives n[i] labels per instance. Host arrays n_iter and
  * inertia receive the iteration count and inertia of each instance. cross_term selects the
- * MFMA operand type of the assignment: 0 = fp32, 1 = bf16; either way samples whose two
- * nearest centres are within the error bound are re-checked in double, so the labels are
- * exact.
+ * MFMA operand type of the assignment: 0 = fp32, 1 = bf16, 2 = bf16x3 (each value as hi + lo
+ * in bf16, three products); in every form samples whose two nearest centres are within the
+ * error bound are re-checked in double, so the labels are exact.
  * The first call for a batch layout tunes the tile shape and captures the run in a HIP graph;
  * later calls with the same sizes, pointers, max_iter and tol replay it.
  */

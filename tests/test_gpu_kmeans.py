@@ -201,7 +201,7 @@ def test_bf16_cross_term_near_ties(oracle):
     np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-5, atol=1e-5)
 
 
-@pytest.mark.parametrize("precision", ["working", "bf16"])
+@pytest.mark.parametrize("precision", ["working", "bf16", "bf16x3"])
 def test_batch_matches_single_instances(oracle, precision):
     """Batched k-means over uneven instances equals each instance run alone (oracle with
     supplied centres): labels, iteration counts, centres and inertia, for both operand forms

@@ -237,31 +237,33 @@ def test_tile_variants(oracle, variant, precision, d, k):
 
 
 @pytest.mark.parametrize("variant", [0, 2])
-def test_long_splits_16bit_offsets(oracle, variant):
-    """A training set of more than 65,536 rows: sweeps that buffer 16-bit row offsets take
-    at least two splits and still return the exact neighbours with the right indices."""
+@pytest.mark.parametrize("k", [10, 100])
+def test_long_splits_16bit_offsets(oracle, variant, k):
+    """A training set of more than 65,536 rows: sweeps that buffer 16-bit row offsets (16
+    and 128 candidates per split here) take at least two splits and still return the exact
+    neighbours with the right indices."""
     rng = np.random.default_rng(77)
     d = 24
     means = rng.uniform(-5, 5, (16, d))
     Xt = gaussian_mixture(rng, 70001, means)
     Xq = gaussian_mixture(rng, 48, means)
     nn = aoclda.neighbors.nearest_neighbors(precision="bf16x3", tile_variant=variant).fit(Xt)
-    dist, ind = nn.kneighbors(Xq, n_neighbors=10)
-    ei, ed = oracle.knn(Xt, Xq, 10)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=k)
+    ei, ed = oracle.knn(Xt, Xq, k)
     np.testing.assert_array_equal(ind, ei)
     np.testing.assert_allclose(dist, ed, rtol=1e-5)
 
 @pytest.mark.parametrize("precision", ["working", "fp16", "bf16x3"])
 @pytest.mark.parametrize("k", [10, 30, 100])
 def test_seeded_thresholds(oracle, precision, k):
-    """Seeding the thresholds from a sweep of the first sixteenth of the rows (training sets
-    of 32,768 rows or more) returns exactly the neighbours of the unseeded sweep and of the
+    """Seeding the thresholds from a short sweep of the first 1/64 of the rows (training sets
+    of 131,072 rows or more) returns exactly the neighbours of the unseeded sweep and of the
     oracle, also for queries far from every training row (whose seeds are loose) and for
     plain fp16, whose wider error bound sends queries to the exact fallback."""
     rng = np.random.default_rng(91)
     d = 40
     means = rng.uniform(-5, 5, (32, d))
-    Xt = gaussian_mixture(rng, 40000, means)
+    Xt = gaussian_mixture(rng, 140000, means)
     Xq = np.concatenate([gaussian_mixture(rng, 90, means),
                          rng.uniform(-40, 40, (10, d)).astype(np.float32)])
     ei, ed = oracle.knn(Xt, Xq, k)
