@@ -44,6 +44,8 @@ METRICS = {"euclidean": 0, "l2": 0, "sqeuclidean": 1, "minkowski": 2, "manhattan
            "l1": 3, "cityblock": 3, "cosine": 4, "euclidean_gemm": 5,
            "sqeuclidean_gemm": 6, "inner_product": 7}
 RINFO, TRAINED, KMEANS_CENTRES, KMEANS_LABELS = 1, 2, 501, 502
+(RNN_COUNT, RNN_OFFSETS, RNN_INDICES, RNN_DISTANCES, RNN_INDICES_INDEX,
+ RNN_DISTANCES_INDEX) = range(601, 607)
 
 
 def _proto(name, restype, *argtypes):
@@ -63,6 +65,8 @@ for _t, _ct in (("d", C.c_double), ("s", C.c_float)):
            C.c_void_p, C.c_void_p, da_int, da_int)
     _proto(f"da_nn_kneighbors_device_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int,
            C.c_void_p, C.c_void_p, da_int, da_int)
+    _proto(f"da_nn_radius_neighbors_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int,
+           _ct, da_int, da_int)
     _proto(f"da_kmeans_set_data_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int)
     _proto(f"da_kmeans_set_init_centres_{_t}", S, da_handle, C.c_void_p, da_int)
     _proto(f"da_kmeans_compute_{_t}", S, da_handle)

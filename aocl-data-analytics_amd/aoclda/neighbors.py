@@ -45,6 +45,8 @@ class nearest_neighbors:
         h.set_int("tile variant", self.tile_variant)
         h.set_int("candidates per split", self.candidates_per_split)
         h.set_string("seed thresholds", "yes" if self.seed_thresholds else "no")
+        if self.radius is not None:
+            h.set_real("radius", self.radius)
 
     def fit(self, X, y=None):
         X, self.order, self.dtype = L.prepare(X)
@@ -67,6 +69,52 @@ class nearest_neighbors:
         self._h.call("da_nn_kneighbors", nq, d, L.ptr(X), ld, L.ptr(ind),
                      L.ptr(dist) if return_distance else None, k, 1 if return_distance else 0)
         return (dist, ind) if return_distance else ind
+
+    def radius_neighbors(self, X, radius=None, return_distance=True, sort_results=False):
+        """Training rows within `radius` of each query (neighbors.py:232-330 of the
+        reference): object arrays of per-query index (and distance) arrays, in ascending
+        training index, or by (distance, index) with sort_results."""
+        if not return_distance and sort_results:
+            raise ValueError("Cannot sort results if distances are not returned.")
+        if X is None:
+            raise ValueError("radius_neighbors() is not implemented for X=None")
+        if radius is not None and not isinstance(radius, (int, float)):
+            raise ValueError("Radius must be a number or None. Got radius=" + str(radius))
+        if radius is None and self.radius is None:
+            raise ValueError("radius cannot be None in radius_neighbors() if it was None in "
+                             "the constructor.")
+        if radius is not None and radius < 0.0:
+            raise ValueError("Radius must be non-negative or None. Got radius=" + str(radius))
+        X, _, _ = L.prepare(X, dtype=self.dtype, order=self.order)
+        nq, d = X.shape
+        ld = nq if self.order == L.COLUMN_MAJOR else d
+        self._h.call("da_nn_radius_neighbors", nq, d, L.ptr(X), ld,
+                     -1.0 if radius is None else float(radius), 1 if return_distance else 0,
+                     1 if sort_results else 0)
+        counts = self._int_result(L.RNN_COUNT, nq + 1)
+        total = int(counts[nq])
+        flat_ind = self._int_result(L.RNN_INDICES, max(total, 1))[:total]
+        bounds = np.concatenate(([0], np.cumsum(counts[:nq])))
+        inds = np.empty(nq, dtype=object)
+        for q in range(nq):
+            inds[q] = flat_ind[bounds[q]:bounds[q + 1]].copy()
+        if not return_distance:
+            return inds
+        flat_d = np.zeros(max(total, 1), dtype=self.dtype)
+        n = L.da_int(flat_d.size)
+        L.check(getattr(L.lib, f"da_handle_get_result_{L.suffix(self.dtype)}")(
+            self._h.h, L.RNN_DISTANCES, C.byref(n), L.ptr(flat_d)), self._h.h)
+        dists = np.empty(nq, dtype=object)
+        for q in range(nq):
+            dists[q] = flat_d[bounds[q]:bounds[q + 1]].copy()
+        return dists, inds
+
+    def _int_result(self, query, size):
+        out = np.zeros(size, dtype=np.int32)
+        n = L.da_int(size)
+        L.check(L.lib.da_handle_get_result_int(self._h.h, query, C.byref(n), L.ptr(out)),
+                self._h.h)
+        return out
 
     @property
     def handle(self):

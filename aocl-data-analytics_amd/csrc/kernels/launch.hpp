@@ -174,6 +174,25 @@ int knn_tile_kc(int k);
 int knn_tile_qb(int kc);
 // Largest nsplit * kc the merge accepts.
 int knn_merge_max_candidates();
+// Radius search on a rows x n tile of distance keys D (row-major, leading dimension ldd):
+// counts[r] = #{j : D[r][j] <= thr}; the fill writes, from offs[r] on, the indices j of
+// those entries in ascending order (and their keys when key is not null); the sort orders
+// each segment [offs[r], offs[r+1]) by key, stably (ties stay in index order), with
+// temporary storage of radius_sort_bytes bytes; the sqrt pass takes the square root of n
+// values in place.
+template <typename U>
+hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int *counts,
+                               hipStream_t s);
+template <typename U>
+hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, const int *offs,
+                              int *ind, U *key, hipStream_t s);
+template <typename U> hipError_t radius_sort_bytes(int total, int rows, size_t *bytes);
+template <typename U>
+hipError_t launch_radius_sort(void *temp, size_t bytes, const U *key_in, U *key_out,
+                              const int *ind_in, int *ind_out, int total, int rows,
+                              const int *offs, hipStream_t s);
+template <typename U> hipError_t launch_radius_sqrt(U *v, int64_t n, hipStream_t s);
+
 // Fewest training splits the fused tile accepts for n rows at candidate-list length kc,
 // operand precision prec and padded width Kp (sweeps that buffer 16-bit row offsets take
 // splits of at most 65,536 rows).

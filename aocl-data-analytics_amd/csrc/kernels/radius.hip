@@ -1,0 +1,133 @@
+// Radius search over materialised distance-key tiles for gfx950: per query, the training
+// rows whose key is at most a threshold (the reference's brute-force radius search,
+// nearest_neighbors.cpp:2080-2215: D <= radius, or <= radius^2 on squared euclidean keys).
+//
+// A query's hits are found by one wave sweeping its key row 64 entries at a time: a ballot
+// of the compares gives the count, and in the fill pass the position of each hit (the
+// running count plus the hits of lower lanes), so every query's neighbours come out in
+// ascending training index without atomics. Optional ordering by distance is a stable
+// segmented radix sort of (key, index) pairs, which keeps ties in index order.
+#include "launch.hpp"
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+namespace dagpu {
+
+namespace {
+
+template <typename U>
+__global__ __launch_bounds__(256) void radius_count_kernel(const U *__restrict__ D, int64_t ldd,
+                                                           int rows, int n, U thr,
+                                                           int *__restrict__ counts) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (row >= rows)
+        return;
+    const U *d = D + (int64_t)row * ldd;
+    int c = 0;
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int j = j0 + lane;
+        const bool hit = j < n && d[j] <= thr;
+        c += __popcll(__builtin_amdgcn_ballot_w64(hit));
+    }
+    if (lane == 0)
+        counts[row] = c;
+}
+
+template <typename U>
+__global__ __launch_bounds__(256) void radius_fill_kernel(const U *__restrict__ D, int64_t ldd,
+                                                          int rows, int n, U thr,
+                                                          const int *__restrict__ offs,
+                                                          int *__restrict__ ind,
+                                                          U *__restrict__ key) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (row >= rows)
+        return;
+    const U *d = D + (int64_t)row * ldd;
+    const unsigned long long below = (1ull << lane) - 1;
+    int pos = offs[row];
+    for (int j0 = 0; j0 < n; j0 += 64) {
+        const int j = j0 + lane;
+        const U v = j < n ? d[j] : thr;
+        const bool hit = j < n && v <= thr;
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
+        if (hit) {
+            const int p = pos + __popcll(m & below);
+            ind[p] = j;
+            if (key)
+                key[p] = v;
+        }
+        pos += __popcll(m);
+    }
+}
+
+template <typename U>
+__global__ void radius_sqrt_kernel(U *__restrict__ v, int64_t n) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x)
+        v[t] = sqrt(v[t]);
+}
+
+} // namespace
+
+template <typename U>
+hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int *counts,
+                               hipStream_t s) {
+    if (rows <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(radius_count_kernel<U>, dim3((rows + 3) / 4), dim3(256), 0, s, D, ldd,
+                       rows, n, thr, counts);
+    return hipGetLastError();
+}
+
+template <typename U>
+hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, const int *offs,
+                              int *ind, U *key, hipStream_t s) {
+    if (rows <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(radius_fill_kernel<U>, dim3((rows + 3) / 4), dim3(256), 0, s, D, ldd,
+                       rows, n, thr, offs, ind, key);
+    return hipGetLastError();
+}
+
+template <typename U>
+hipError_t radius_sort_bytes(int total, int rows, size_t *bytes) {
+    return hipcub::DeviceSegmentedRadixSort::SortPairs(
+        nullptr, *bytes, (const U *)nullptr, (U *)nullptr, (const int *)nullptr, (int *)nullptr,
+        total, rows, (const int *)nullptr, (const int *)nullptr, 0, (int)sizeof(U) * 8);
+}
+
+template <typename U>
+hipError_t launch_radius_sort(void *temp, size_t bytes, const U *key_in, U *key_out,
+                              const int *ind_in, int *ind_out, int total, int rows,
+                              const int *offs, hipStream_t s) {
+    if (total <= 0)
+        return hipSuccess;
+    return hipcub::DeviceSegmentedRadixSort::SortPairs(temp, bytes, key_in, key_out, ind_in,
+                                                       ind_out, total, rows, offs, offs + 1, 0,
+                                                       (int)sizeof(U) * 8, s);
+}
+
+template <typename U> hipError_t launch_radius_sqrt(U *v, int64_t n, hipStream_t s) {
+    if (n <= 0)
+        return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(radius_sqrt_kernel<U>, dim3((unsigned)blocks), dim3(256), 0, s, v, n);
+    return hipGetLastError();
+}
+
+#define DA_RADIUS_INST(U)                                                                       \
+    template hipError_t launch_radius_count<U>(const U *, int64_t, int, int, U, int *,          \
+                                               hipStream_t);                                    \
+    template hipError_t launch_radius_fill<U>(const U *, int64_t, int, int, U, const int *,     \
+                                              int *, U *, hipStream_t);                         \
+    template hipError_t radius_sort_bytes<U>(int, int, size_t *);                               \
+    template hipError_t launch_radius_sort<U>(void *, size_t, const U *, U *, const int *,      \
+                                              int *, int, int, const int *, hipStream_t);      \
+    template hipError_t launch_radius_sqrt<U>(U *, int64_t, hipStream_t);
+DA_RADIUS_INST(float)
+DA_RADIUS_INST(double)
+#undef DA_RADIUS_INST
+
+} // namespace dagpu

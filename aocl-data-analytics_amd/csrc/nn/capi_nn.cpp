@@ -59,7 +59,33 @@ static da_status kneighbors_dev(da_handle h, da_int nq, da_int nf, const T *X, d
     return nn->kneighbors_device(nq, nf, X, ldx, ind, dist, k, rd);
 }
 
+template <typename T>
+static da_status radius_search(da_handle h, da_int nq, da_int nf, const T *X, da_int ldx, T r,
+                        da_int rd, da_int sort) {
+    if (!h)
+        return da_status_handle_not_initialized;
+    h->err.clear();
+    da_status s;
+    NNHandle<T> *nn = nn_of<T>(h, s);
+    if (!nn)
+        return s;
+    return nn->radius_neighbors(nq, nf, X, ldx, r, rd, sort);
+}
+
 extern "C" {
+
+da_status da_nn_radius_neighbors_d(da_handle handle, da_int n_queries, da_int n_features,
+                                   const double *X_test, da_int ldx_test, double radius,
+                                   da_int return_distance, da_int sort_results) {
+    return radius_search<double>(handle, n_queries, n_features, X_test, ldx_test, radius,
+                                 return_distance, sort_results);
+}
+da_status da_nn_radius_neighbors_s(da_handle handle, da_int n_queries, da_int n_features,
+                                   const float *X_test, da_int ldx_test, float radius,
+                                   da_int return_distance, da_int sort_results) {
+    return radius_search<float>(handle, n_queries, n_features, X_test, ldx_test, radius,
+                                return_distance, sort_results);
+}
 
 da_status da_nn_set_data_d(da_handle handle, da_int n_samples, da_int n_features,
                            const double *X_train, da_int ldx_train) {

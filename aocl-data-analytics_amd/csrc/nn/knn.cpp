@@ -751,9 +751,299 @@ da_status NNHandle<T>::search_tiles(SearchContext &ctx, const T *dQ, da_int nq, 
     return da_status_success;
 }
 
+// Radius neighbours by brute force (nearest_neighbors.cpp:1918-2215): the squared-euclidean
+// forms compare squared distances with radius^2 and return square roots; the others compare
+// the distance itself. Keys are computed exactly (double direct differences) and rounded to
+// T, then compared in T as the reference does. Each query's neighbours are in ascending
+// training index, or ordered by (distance, index) with sort_results.
+template <typename T>
+da_status NNHandle<T>::radius_neighbors(da_int n_queries, da_int n_features, const T *X_test,
+                                        da_int ldx_test, T radius, da_int return_distance,
+                                        da_int sort_results) {
+    if (!return_distance && sort_results)
+        return err->error(da_status_invalid_input,
+                          "Cannot sort results if distances are not returned.");
+    if (!trained_)
+        return err->error(da_status_no_data, "No training data have been set. Please call "
+                                             "da_nn_set_data_s or da_nn_set_data_d.");
+    da_status s = check_options_update();
+    if (s != da_status_success)
+        return s;
+    if (metric_ == da_inner_product)
+        return err->error(da_status_incompatible_options,
+                          "The inner product metric is not compatible with radius neighbors. "
+                          "Use da_nn_kneighbors to find the k most similar training vectors "
+                          "by inner product.");
+    if (X_test == nullptr)
+        return err->error(da_status_invalid_pointer, "X_test is not a valid pointer.");
+    if (n_features != n_features_)
+        return err->error(da_status_invalid_array_dimension,
+                          "n_features = " + std::to_string(n_features) +
+                              " doesn't match the expected value " +
+                              std::to_string(n_features_) + ".");
+    s = check_2D_array<T>(order(), n_queries, n_features, X_test, ldx_test, "n_queries",
+                          "n_features", "X_test", "ldx_test");
+    if (s != da_status_success)
+        return s;
+    const T opt_radius = (T)opts.rval("radius");
+    if (radius < 0 && opt_radius < 0)
+        return err->error(da_status_invalid_input,
+                          "Radius of requested neighbors must be non-negative.");
+    if (radius < 0)
+        radius = opt_radius;
+    rnn_done_ = false;
+    model_trained = false;
+    rnn_count_.assign(n_queries, 0);
+    rnn_ind_.clear();
+    rnn_d_.clear();
+
+    hipStream_t st = gpu->stream();
+    const da_int d = n_features_, n = n_samples_;
+    const size_t qbytes = (size_t)n_queries * d * sizeof(T);
+    DA_HIP(err, q_.reserve(qbytes));
+    if (order() == row_major) {
+        DA_HIP(err, copy_2d(q_.ptr(), d * sizeof(T), X_test, (size_t)ldx_test * sizeof(T),
+                            d * sizeof(T), n_queries, hipMemcpyHostToDevice, st));
+    } else {
+        DA_HIP(err, q_stage_.reserve(qbytes));
+        DA_HIP(err, copy_2d(q_stage_.ptr(), n_queries * sizeof(T), X_test,
+                            (size_t)ldx_test * sizeof(T), n_queries * sizeof(T), d,
+                            hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), n_queries, n_queries, d,
+                                                   q_.as<T>(), st));
+    }
+    const T *dQ = q_.as<T>();
+    // Keys and thresholds per metric (set_params, nearest_neighbors.cpp:337-347).
+    int km;
+    bool squares = false, need_qsq = false, need_qcos = false;
+    switch (metric_) {
+    case da_euclidean:
+        km = KM_SQEUCLIDEAN;
+        squares = true;
+        break;
+    case da_euclidean_gemm:
+        km = KM_SQEUCLIDEAN_GEMM;
+        squares = true;
+        need_qsq = true;
+        break;
+    case da_sqeuclidean:
+        km = KM_SQEUCLIDEAN;
+        break;
+    case da_sqeuclidean_gemm:
+        km = KM_SQEUCLIDEAN_GEMM;
+        need_qsq = true;
+        break;
+    case da_manhattan:
+        km = KM_MANHATTAN;
+        break;
+    case da_cosine:
+        km = KM_COSINE;
+        need_qcos = true;
+        break;
+    case da_minkowski:
+        if (p_ == 2.0) {
+            km = KM_SQEUCLIDEAN;
+            squares = true;
+        } else {
+            km = p_ == 1.0 ? KM_MANHATTAN : KM_MINKOWSKI;
+        }
+        break;
+    default:
+        return err->error(da_status_not_implemented, "Unknown metric.");
+    }
+    const T thr = squares ? radius * radius : radius;
+    const double *nq_norm = nullptr, *nt_norm = nullptr;
+    if (need_qsq || need_qcos) {
+        DA_HIP(err, main_ctx.q_norm.reserve((size_t)n_queries * sizeof(double)));
+        DA_HIP(err, launch_row_sqnorms<T>(dQ, d, n_queries, d, main_ctx.q_norm.as<double>(), st));
+        if (need_qcos) {
+            DA_HIP(err, launch_cosine_norms(main_ctx.q_norm.as<double>(), n_queries,
+                                            std::numeric_limits<T>::epsilon(), st));
+            nt_norm = train_cos_.as<double>();
+        } else {
+            nt_norm = train_sq_.as<double>();
+        }
+        nq_norm = main_ctx.q_norm.as<double>();
+    }
+    // Query blocks whose key tile stays within 256 MB (and 2^28 entries).
+    const size_t max_keys = ((size_t)256 << 20) / sizeof(T);
+    const da_int chunk = (da_int)std::max<size_t>(
+        1, std::min<size_t>(n_queries, std::min<size_t>(max_keys, (size_t)1 << 28) / (size_t)n));
+    DA_HIP(err, rnn_keys_.reserve((size_t)chunk * n * sizeof(T)));
+    DA_HIP(err, rnn_cnt_.reserve((size_t)chunk * sizeof(int)));
+    DA_HIP(err, rnn_off_.reserve((size_t)(chunk + 1) * sizeof(int)));
+    std::vector<int> cnt(chunk), off(chunk + 1);
+    for (da_int q0 = 0; q0 < n_queries; q0 += chunk) {
+        const da_int rows = std::min(chunk, n_queries - q0);
+        {
+            KernelScope ks(*gpu, "pairwise_direct");
+            DA_HIP(err, (launch_pairwise_direct<T, T>(km, dQ + (size_t)q0 * d, train_.as<T>(),
+                                                      rows, n, d, rnn_keys_.as<T>(), n,
+                                                      nq_norm ? nq_norm + q0 : nullptr, nt_norm,
+                                                      p_, 0, 0, st)));
+        }
+        KernelScope ks(*gpu, "radius_select");
+        DA_HIP(err, launch_radius_count<T>(rnn_keys_.as<T>(), n, rows, n, thr,
+                                           rnn_cnt_.as<int>(), st));
+        DA_HIP(err, hipMemcpyAsync(cnt.data(), rnn_cnt_.ptr(), rows * sizeof(int),
+                                   hipMemcpyDeviceToHost, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+        off[0] = 0;
+        for (da_int r = 0; r < rows; r++)
+            off[r + 1] = off[r] + cnt[r];
+        const int total = off[rows];
+        for (da_int r = 0; r < rows; r++)
+            rnn_count_[q0 + r] = cnt[r];
+        if (total == 0)
+            continue;
+        DA_HIP(err, hipMemcpyAsync(rnn_off_.ptr(), off.data(), (rows + 1) * sizeof(int),
+                                   hipMemcpyHostToDevice, st));
+        DA_HIP(err, rnn_ind_buf_.reserve((size_t)total * sizeof(int)));
+        DA_HIP(err, rnn_key_buf_.reserve((size_t)total * sizeof(T)));
+        DA_HIP(err, launch_radius_fill<T>(rnn_keys_.as<T>(), n, rows, n, thr, rnn_off_.as<int>(),
+                                          rnn_ind_buf_.as<int>(),
+                                          return_distance ? rnn_key_buf_.as<T>() : nullptr, st));
+        int *ind = rnn_ind_buf_.as<int>();
+        T *key = rnn_key_buf_.as<T>();
+        if (sort_results) {
+            size_t bytes = 0;
+            DA_HIP(err, radius_sort_bytes<T>(total, rows, &bytes));
+            DA_HIP(err, rnn_tmp_.reserve(std::max<size_t>(bytes, 1)));
+            DA_HIP(err, rnn_ind2_.reserve((size_t)total * sizeof(int)));
+            DA_HIP(err, rnn_key2_.reserve((size_t)total * sizeof(T)));
+            DA_HIP(err, launch_radius_sort<T>(rnn_tmp_.ptr(), bytes, key, rnn_key2_.as<T>(), ind,
+                                              rnn_ind2_.as<int>(), total, rows,
+                                              rnn_off_.as<int>(), st));
+            ind = rnn_ind2_.as<int>();
+            key = rnn_key2_.as<T>();
+        }
+        if (return_distance && squares)
+            DA_HIP(err, launch_radius_sqrt<T>(key, total, st));
+        const size_t base = rnn_ind_.size();
+        rnn_ind_.resize(base + total);
+        DA_HIP(err, hipMemcpyAsync(rnn_ind_.data() + base, ind, (size_t)total * sizeof(int),
+                                   hipMemcpyDeviceToHost, st));
+        if (return_distance) {
+            rnn_d_.resize(base + total);
+            DA_HIP(err, hipMemcpyAsync(rnn_d_.data() + base, key, (size_t)total * sizeof(T),
+                                       hipMemcpyDeviceToHost, st));
+        }
+        DA_HIP(err, hipStreamSynchronize(st));
+    }
+    rnn_dist_ = return_distance != 0;
+    rnn_done_ = true;
+    model_trained = true;
+    return da_status_success;
+}
+
+// Radius results (nearest_neighbors.cpp get_result, :180-300 and :2284-2330).
 template <typename T>
 da_status NNHandle<T>::get_result(da_result query, da_int *dim, da_int *result) {
-    return AlgHandle::get_result(query, dim, result);
+    switch (query) {
+    case da_nn_radius_neighbors_count:
+    case da_nn_radius_neighbors_offsets:
+    case da_nn_radius_neighbors_indices:
+    case da_nn_radius_neighbors_indices_index:
+        break;
+    default:
+        return AlgHandle::get_result(query, dim, result);
+    }
+    if (!rnn_done_)
+        return err->warn(da_status_no_data, "Radius neighbors have not been computed. Please "
+                                            "call da_nn_radius_neighbors first.");
+    const da_int nq = (da_int)rnn_count_.size();
+    auto too_small = [&](da_int need) {
+        *dim = need;
+        return err->warn(da_status_invalid_array_dimension,
+                         "The array is too small. Please provide an array of at least size: " +
+                             std::to_string(need) + ".");
+    };
+    const da_int total = (da_int)rnn_ind_.size();
+    switch (query) {
+    case da_nn_radius_neighbors_count: {
+        if (*dim < nq + 1)
+            return too_small(nq + 1);
+        for (da_int i = 0; i < nq; i++)
+            result[i] = rnn_count_[i];
+        result[nq] = total;
+        return da_status_success;
+    }
+    case da_nn_radius_neighbors_offsets: {
+        if (*dim < nq + 1)
+            return too_small(nq + 1);
+        da_int off = 0;
+        for (da_int i = 0; i < nq; i++) {
+            result[i] = rnn_count_[i] > 0 ? off : -1;
+            off += rnn_count_[i];
+        }
+        result[nq] = off;
+        return da_status_success;
+    }
+    case da_nn_radius_neighbors_indices: {
+        if (*dim < total)
+            return too_small(total);
+        std::copy(rnn_ind_.begin(), rnn_ind_.end(), result);
+        return da_status_success;
+    }
+    default: { // indices of one query, its index passed in result[0]
+        const da_int q = result[0];
+        if (q < 0 || q >= nq)
+            return err->warn(da_status_invalid_input,
+                             "The provided index is out of bounds. It should be in the range "
+                             "[0, " + std::to_string(nq - 1) + "].");
+        if (*dim < rnn_count_[q])
+            return too_small(rnn_count_[q]);
+        da_int off = 0;
+        for (da_int i = 0; i < q; i++)
+            off += rnn_count_[i];
+        std::copy(rnn_ind_.begin() + off, rnn_ind_.begin() + off + rnn_count_[q], result);
+        return da_status_success;
+    }
+    }
+}
+
+template <typename T>
+da_status NNHandle<T>::get_result(da_result query, da_int *dim, T *result) {
+    if (query != da_nn_radius_neighbors_distances &&
+        query != da_nn_radius_neighbors_distances_index)
+        return AlgHandle::get_result(query, dim, result);
+    if (!rnn_done_)
+        return err->warn(da_status_no_data, "Radius neighbors have not been computed. Please "
+                                            "call da_nn_radius_neighbors_s or "
+                                            "da_nn_radius_neighbors_d first.");
+    if (!rnn_dist_)
+        return err->warn(da_status_no_data,
+                         "Distances were not requested during radius neighbors computation. "
+                         "Please set return_distance to true in da_nn_radius_neighbors_s or "
+                         "da_nn_radius_neighbors_d call.");
+    const da_int nq = (da_int)rnn_count_.size();
+    if (query == da_nn_radius_neighbors_distances) {
+        const da_int total = (da_int)rnn_d_.size();
+        if (*dim < total) {
+            *dim = total;
+            return err->warn(da_status_invalid_array_dimension,
+                             "The array is too small. Please provide an array of at least "
+                             "size: " + std::to_string(total) + ".");
+        }
+        std::copy(rnn_d_.begin(), rnn_d_.end(), result);
+        return da_status_success;
+    }
+    const da_int q = (da_int)result[0];
+    if (q < 0 || q >= nq)
+        return err->warn(da_status_invalid_input,
+                         "The provided index is out of bounds. It should be in the range [0, " +
+                             std::to_string(nq - 1) + "].");
+    if (*dim < rnn_count_[q]) {
+        *dim = rnn_count_[q];
+        return err->warn(da_status_invalid_array_dimension,
+                         "The array is too small. Please provide an array of at least size: " +
+                             std::to_string(rnn_count_[q]) + ".");
+    }
+    da_int off = 0;
+    for (da_int i = 0; i < q; i++)
+        off += rnn_count_[i];
+    std::copy(rnn_d_.begin() + off, rnn_d_.begin() + off + rnn_count_[q], result);
+    return da_status_success;
 }
 
 template class NNHandle<float>;

@@ -4,6 +4,7 @@
 #pragma once
 #include "../core/handle.hpp"
 #include <map>
+#include <vector>
 
 namespace dagpu {
 
@@ -42,7 +43,14 @@ template <typename T> class NNHandle : public AlgHandle {
     da_status kneighbors_device(da_int n_queries, da_int n_features, const T *d_X_test,
                                 da_int ldx_test, da_int *d_ind, T *d_dist, da_int k,
                                 da_int return_distance);
+    // Radius neighbours (nearest_neighbors.cpp:1918-2215): results are kept in the handle
+    // for da_handle_get_result_? (counts, offsets, indices, distances).
+    da_status radius_neighbors(da_int n_queries, da_int n_features, const T *X_test,
+                               da_int ldx_test, T radius, da_int return_distance,
+                               da_int sort_results);
+    using AlgHandle::get_result;
     da_status get_result(da_result query, da_int *dim, da_int *result) override;
+    da_status get_result(da_result query, da_int *dim, T *result) override;
     void refresh() override {}
 
     // Search entry points shared with the serving loop. dQ: row-major n_features columns;
@@ -80,6 +88,12 @@ template <typename T> class NNHandle : public AlgHandle {
 
     bool trained_ = false;
     da_int n_samples_ = 0, n_features_ = 0;
+    // Radius results: per-query counts, neighbours in query order, their distances.
+    bool rnn_done_ = false, rnn_dist_ = false;
+    std::vector<da_int> rnn_count_, rnn_ind_;
+    std::vector<T> rnn_d_;
+    DeviceBuffer rnn_keys_, rnn_cnt_, rnn_off_, rnn_ind_buf_, rnn_key_buf_, rnn_ind2_, rnn_key2_,
+        rnn_tmp_;
     // Options as read at set_data (nearest_neighbors.cpp:295-352).
     da_int n_neighbors_ = 5, algo_ = 3, metric_ = 0;
     double p_ = 2.0;

@@ -33,6 +33,20 @@ da_status da_nn_kneighbors_s(da_handle handle, da_int n_queries, da_int n_featur
                              const float *X_test, da_int ldx_test, da_int *n_ind,
                              float *n_dist, da_int k, da_int return_distance);
 
+/* aoclda_nearest_neighbors.h:188-193. Results through da_handle_get_result_?:
+ * da_nn_radius_neighbors_count (n_queries + 1, total last), _offsets (n_queries + 1, -1 for
+ * a query without neighbours, total last), _indices / _distances (all queries' neighbours
+ * in query order), _indices_index / _distances_index (one query, its index in result[0]).
+ * Neighbours of a query are in ascending training index, or ascending distance (ties by
+ * index) with sort_results = 1, which requires return_distance = 1. radius < 0 takes the
+ * "radius" option. */
+da_status da_nn_radius_neighbors_d(da_handle handle, da_int n_queries, da_int n_features,
+                                   const double *X_test, da_int ldx_test, double radius,
+                                   da_int return_distance, da_int sort_results);
+da_status da_nn_radius_neighbors_s(da_handle handle, da_int n_queries, da_int n_features,
+                                   const float *X_test, da_int ldx_test, float radius,
+                                   da_int return_distance, da_int sort_results);
+
 #ifdef __cplusplus
 }
 #endif

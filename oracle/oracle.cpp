@@ -490,4 +490,61 @@ int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *
     return warn ? 21 : 0;
 }
 
+/*
+ * Brute-force radius neighbours restated (nearest_neighbors.cpp:1918-2080 and the brute-force
+ * kernel :2080-2215, set_params :337-347, extraction :2310-2330): the euclidean forms
+ * (euclidean, euclidean_gemm, minkowski p = 2) compare squared euclidean distances with
+ * radius^2 and return square roots; sqeuclidean(_gemm), manhattan, cosine and minkowski
+ * (p != 2, with its root) compare the distance itself with radius. Inclusive (<=). With
+ * single != 0 the distance and the threshold are rounded to float before the comparison,
+ * as the reference's single-precision path computes them in float. Neighbours of a query
+ * are listed in ascending index (the single-threaded insertion order), or, with sort != 0,
+ * stably sorted by distance (sorted_n_dist_n_ind). counts[q] receives the number of
+ * neighbours of query q; ind/dist (capacity cap) the neighbours of all queries in query
+ * order. Returns the total number of neighbours.
+ */
+long long oracle_radius(int n, int d, const double *Xt, int nq, const double *Xq, int metric,
+                        double p, double r, int single, int sort, int *counts, int *ind,
+                        double *dist, long long cap, double eps) {
+    const bool sq = metric == M_EUCLID || metric == M_EUCLID_GEMM ||
+                    (metric == M_MINKOWSKI && p == 2.0);
+    int km = metric;
+    if (sq)
+        km = metric == M_EUCLID_GEMM ? M_SQEUCLID_GEMM : M_SQEUCLID;
+    double thr = sq ? r * r : r;
+    if (single) {
+        const float rf = (float)r;
+        thr = sq ? (double)(rf * rf) : (double)rf;
+    }
+    std::vector<double> D(n);
+    std::vector<int> hit;
+    long long total = 0;
+    for (int q = 0; q < nq; q++) {
+        oracle_pairwise(0, 1, n, d, Xq + (size_t)q * d, d, Xt, d, D.data(), n, p, km, eps);
+        hit.clear();
+        for (int j = 0; j < n; j++) {
+            double v = D[j];
+            if (single)
+                v = (double)(float)v;
+            D[j] = v;
+            if (v <= thr)
+                hit.push_back(j);
+        }
+        if (sort)
+            std::stable_sort(hit.begin(), hit.end(), [&](int a, int b) { return D[a] < D[b]; });
+        counts[q] = (int)hit.size();
+        for (int j : hit) {
+            if (total < cap) {
+                ind[total] = j;
+                double v = D[j];
+                if (sq)
+                    v = single ? (double)std::sqrt((float)v) : std::sqrt(v);
+                dist[total] = v;
+            }
+            total++;
+        }
+    }
+    return total;
+}
+
 } // extern "C"

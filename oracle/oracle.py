@@ -64,6 +64,32 @@ def knn(Xtrain, Xq, k, metric="euclidean", p=2.0, eps=np.finfo(np.float64).eps):
     return ind, dist
 
 
+_lib.oracle_radius.restype = C.c_longlong
+_lib.oracle_radius.argtypes = [C.c_int, C.c_int, _d, C.c_int, _d, C.c_int, C.c_double, C.c_double,
+                               C.c_int, C.c_int, _i, _i, _d, C.c_longlong, C.c_double]
+
+
+def radius(Xtrain, Xq, r, metric="euclidean", p=2.0, single=False, sort=False,
+           eps=np.finfo(np.float64).eps):
+    """(counts, indices, distances) of the radius neighbours of every query: indices and
+    distances flattened in query order, each query's list in ascending index or, with sort,
+    by distance. single: distances and threshold compared in float."""
+    Xt = np.ascontiguousarray(Xtrain, dtype=np.float64)
+    Q = np.ascontiguousarray(Xq, dtype=np.float64)
+    nq, n = Q.shape[0], Xt.shape[0]
+    counts = np.empty(nq, dtype=np.int32)
+    cap = max(1, nq * min(n, 64))
+    while True:
+        ind = np.empty(cap, dtype=np.int32)
+        dist = np.empty(cap)
+        total = _lib.oracle_radius(n, Xt.shape[1], _p(Xt), nq, _p(Q), METRICS[metric],
+                                   float(p), float(r), int(single), int(sort), _p(counts, _i),
+                                   _p(ind, _i), _p(dist), cap, float(eps))
+        if total <= cap:
+            return counts, ind[:total].copy(), dist[:total].copy()
+        cap = total  # the first pass only counted: run again with room for every neighbour
+
+
 INIT = {"random": 0, "k-means++": 1, "supplied": 2, "random partitions": 3}
 EMPTY = {"ignore": 0, "error": 1, "split": 2}
 

@@ -85,6 +85,15 @@ known = {
                              5.477225575051661, 3.7416573867739413, 4.242640687119285,
                              5.656854249492381, 5.385164807134504, 5.0990195135927845,
                              6.164414002968976],
+        "radius_source": "tests/unit_tests/nearest_neighbors/nearest_neighbors_tests.hpp:"
+                         "147-159,431-432 (radius 5, results sorted by distance)",
+        "radius": 5.0,
+        "radius_euclidean_ind": [[1, 0, 3], [2, 0, 1], [3]],
+        "radius_euclidean_dist": [[3., 3.3166247903554, 3.7416573867739413],
+                                  [2., 3.1622776601683795, 4.242640687119285],
+                                  [4.58257569495584]],
+        "radius_sqeuclidean_ind": [[], [2], []],
+        "radius_sqeuclidean_dist": [[], [4.], []],
     },
     "kmeans": [
         {

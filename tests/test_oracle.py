@@ -155,3 +155,50 @@ def test_kmeanspp_quality_and_determinism(oracle):
     np.testing.assert_array_equal(a["centres"], b["centres"])
     # well separated blobs: k-means++ finds every blob, so the inertia is close to n*d*0.09
     assert a["inertia"] < 1.5 * 800 * 3 * 0.09
+
+def _split(counts, flat):
+    out, o = [], 0
+    for c in counts:
+        out.append(list(flat[o:o + c]))
+        o += c
+    return out
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_radius_known_answers(known, oracle, single):
+    """The reference's radius-neighbour answers on the 6x3 example (radius 5, sorted): the
+    euclidean search compares squared distances with 25, the squared one with 5."""
+    c = known["knn_example"]
+    Xt = np.array(c["X_train_colmajor"]).reshape((c["n_features"], c["n_samples"])).T
+    Xq = np.array(c["X_test_colmajor"]).reshape((c["n_features"], c["n_queries"])).T
+    for metric in ("euclidean", "sqeuclidean"):
+        counts, ind, dist = oracle.radius(Xt, Xq, c["radius"], metric, single=single, sort=True)
+        assert _split(counts, ind) == c[f"radius_{metric}_ind"]
+        for got, exp in zip(_split(counts, dist), c[f"radius_{metric}_dist"]):
+            np.testing.assert_allclose(got, exp, rtol=1e-6 if single else 1e-14)
+    # Unsorted: the same sets in ascending index.
+    counts, ind, _ = oracle.radius(Xt, Xq, c["radius"], "euclidean", single=single)
+    assert _split(counts, ind) == [sorted(v) for v in c["radius_euclidean_ind"]]
+
+
+def test_radius_vs_sklearn_and_brute_force(oracle):
+    """Radius neighbours against scikit-learn's brute force (sorted) and against a direct
+    numpy restatement of the inclusive threshold (unsorted, ascending index)."""
+    from sklearn.neighbors import NearestNeighbors
+    rng = np.random.default_rng(12)
+    Xt = rng.standard_normal((400, 7))
+    Xq = rng.standard_normal((50, 7))
+    for metric, r in (("euclidean", 2.6), ("manhattan", 6.0), ("cosine", 0.4)):
+        nn = NearestNeighbors(radius=r, algorithm="brute", metric=metric).fit(Xt)
+        sd, si = nn.radius_neighbors(Xq, sort_results=True)
+        counts, ind, dist = oracle.radius(Xt, Xq, r, metric, sort=True)
+        got_i, got_d = _split(counts, ind), _split(counts, dist)
+        for q in range(len(Xq)):
+            np.testing.assert_array_equal(got_i[q], si[q])
+            np.testing.assert_allclose(got_d[q], sd[q], rtol=1e-12, atol=1e-12)
+    D = np.sqrt(((Xq[:, None, :] - Xt[None, :, :]) ** 2).sum(-1))
+    counts, ind, dist = oracle.radius(Xt, Xq, 2.6, "sqeuclidean")
+    for q, (iv, dv) in enumerate(zip(_split(counts, ind), _split(counts, dist))):
+        exp = np.nonzero(D[q] ** 2 <= 2.6)[0]
+        np.testing.assert_array_equal(iv, exp)
+        np.testing.assert_allclose(dv, D[q, exp] ** 2, rtol=1e-12)
