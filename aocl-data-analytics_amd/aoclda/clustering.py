@@ -1,4 +1,5 @@
-"""k-means (python_interface/python_package/aoclda/clustering.py:36-280 of the reference)."""
+"""k-means and DBSCAN (python_interface/python_package/aoclda/clustering.py:36-441 of the
+reference)."""
 import ctypes as C
 
 import numpy as np
@@ -191,3 +192,69 @@ def kmeans_batch(datasets, init_centres, max_iter=300, tol=1.0e-4, precision="wo
         co += k[i] * d[i]
         lo += n[i]
     return centres, labels, n_iter, inertia
+
+
+class DBSCAN:
+    """DBSCAN clustering (clustering.py:329-441 of the reference): the eps-neighbourhoods, core
+    points, connected clusters and labels are computed on the GPU."""
+
+    def __init__(self, min_samples=5, metric="euclidean", algorithm="auto", leaf_size=30,
+                 eps=0.5, power=2.0, check_data=False):
+        self.min_samples, self.metric, self.algorithm = min_samples, metric, algorithm
+        self.leaf_size, self.eps, self.power, self.check_data = leaf_size, eps, power, check_data
+        self._h = None
+
+    def fit(self, A):
+        A, self.order, self.dtype = L.prepare(A)
+        h = L.Handle(L.HANDLE_DBSCAN, self.dtype)
+        self._h = h
+        h.set_int("min samples", self.min_samples)
+        h.set_string("metric", self.metric)
+        h.set_string("algorithm", self.algorithm.replace("_", " "))
+        h.set_int("leaf size", self.leaf_size)
+        h.set_real("eps", self.eps)
+        h.set_real("power", self.power)
+        h.set_string("check data", "yes" if self.check_data else "no")
+        h.set_string("storage order", "column-major" if self.order == L.COLUMN_MAJOR else "row-major")
+        n, d = A.shape
+        h.call("da_dbscan_set_data", n, d, L.ptr(A), n if self.order == L.COLUMN_MAJOR else d)
+        h.call("da_dbscan_compute")
+        self._n = n
+        return self
+
+    def _int(self, query, size):
+        r = np.zeros(max(size, 1), dtype=np.int32)
+        dim = L.da_int(max(size, 1))
+        L.check(L.lib.da_handle_get_result_int(self._h.h, query, C.byref(dim), L.ptr(r)),
+                self._h.h)
+        return r[:size]
+
+    def _rinfo(self):
+        r = np.zeros(9, dtype=self.dtype)
+        dim = L.da_int(9)
+        self._h.call("da_handle_get_result", L.RINFO, C.byref(dim), L.ptr(r))
+        return r
+
+    @property
+    def labels(self):
+        return self._int(L.DBSCAN_LABELS, self._n)
+
+    @property
+    def core_sample_indices(self):
+        return self._int(L.DBSCAN_CORE_INDICES, self.n_core_samples)
+
+    @property
+    def n_clusters(self):
+        return int(self._int(L.DBSCAN_N_CLUSTERS, 1)[0])
+
+    @property
+    def n_core_samples(self):
+        return int(self._int(L.DBSCAN_N_CORE, 1)[0])
+
+    @property
+    def n_samples(self):
+        return int(self._rinfo()[0])
+
+    @property
+    def n_features(self):
+        return int(self._rinfo()[1])

@@ -2,6 +2,7 @@
 // buffer, and a per-handle context holding the HIP stream and the optional per-kernel
 // event timer.
 #pragma once
+#include <utility>
 #include "aoclda_error.h"
 #include "errors.hpp"
 #include <hip/hip_runtime_api.h>
@@ -36,6 +37,10 @@ class DeviceBuffer {
     void *ptr() const { return ptr_; }
     template <typename T> T *as() const { return static_cast<T *>(ptr_); }
     size_t size() const { return bytes_; }
+    void swap(DeviceBuffer &o) {
+        std::swap(ptr_, o.ptr_);
+        std::swap(bytes_, o.bytes_);
+    }
 
   private:
     void *ptr_ = nullptr;

@@ -175,17 +175,39 @@ int knn_tile_qb(int kc);
 // Largest nsplit * kc the merge accepts.
 int knn_merge_max_candidates();
 // Radius search on a rows x n tile of distance keys D (row-major, leading dimension ldd):
-// counts[r] = #{j : D[r][j] <= thr}; the fill writes, from offs[r] on, the indices j of
+// counts[r] = #{j : D[r][j] <= thr, j != self0 + r when self0 >= 0}; the fill writes, from offs[r] on, the indices j of
 // those entries in ascending order (and their keys when key is not null); the sort orders
 // each segment [offs[r], offs[r+1]) by key, stably (ties stay in index order), with
 // temporary storage of radius_sort_bytes bytes; the sqrt pass takes the square root of n
 // values in place.
 template <typename U>
-hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int *counts,
-                               hipStream_t s);
+hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
+                               int *counts, hipStream_t s);
 template <typename U>
-hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, const int *offs,
-                              int *ind, U *key, hipStream_t s);
+hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
+                              const int *offs, int *ind, U *key, hipStream_t s);
+
+// ---- DBSCAN over a neighbour graph in CSR form (offs: n + 1 64-bit offsets, ind) ----
+// parent[i] = i.
+hipError_t launch_dbscan_init(int *parent, int n, hipStream_t s);
+// Joins the trees of every pair of core points that are neighbours (core[i] != 0), with
+// lock-free union by smaller root: after the launch each connected set of core points forms
+// one tree whose root is its smallest index.
+hipError_t launch_dbscan_union(const long long *offs, const int *ind, const unsigned char *core,
+                               int *parent, int n, hipStream_t s);
+// For core points parent[i] becomes the root of i; is_root[i] = 1 for a core point that is
+// its own root, 0 otherwise.
+hipError_t launch_dbscan_flatten(const unsigned char *core, int *parent, int *is_root, int n,
+                                 hipStream_t s);
+// Exclusive prefix sum of n ints (hipCUB), with its temporary-storage query.
+hipError_t dbscan_scan_bytes(int n, size_t *bytes);
+hipError_t launch_dbscan_scan(void *temp, size_t bytes, const int *in, int *out, int n,
+                              hipStream_t s);
+// labels[i]: for a core point, the cluster number of its root (rid = scan of is_root); for a
+// non-core point, the smallest cluster number among its core neighbours, or -1 (noise).
+hipError_t launch_dbscan_label(const long long *offs, const int *ind, const unsigned char *core,
+                               const int *parent, const int *rid, int *labels, int n,
+                               hipStream_t s);
 template <typename U> hipError_t radius_sort_bytes(int total, int rows, size_t *bytes);
 template <typename U>
 hipError_t launch_radius_sort(void *temp, size_t bytes, const U *key_in, U *key_out,

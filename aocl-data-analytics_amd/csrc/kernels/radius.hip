@@ -6,7 +6,9 @@
 // of the compares gives the count, and in the fill pass the position of each hit (the
 // running count plus the hits of lower lanes), so every query's neighbours come out in
 // ascending training index without atomics. Optional ordering by distance is a stable
-// segmented radix sort of (key, index) pairs, which keeps ties in index order.
+// segmented radix sort of (key, index) pairs, which keeps ties in index order. With
+// self0 >= 0 the entry of row r at column self0 + r is skipped (a point is not its own
+// neighbour when a set is compared with itself).
 #include "launch.hpp"
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -17,17 +19,18 @@ namespace {
 
 template <typename U>
 __global__ __launch_bounds__(256) void radius_count_kernel(const U *__restrict__ D, int64_t ldd,
-                                                           int rows, int n, U thr,
+                                                           int rows, int n, U thr, int self0,
                                                            int *__restrict__ counts) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     if (row >= rows)
         return;
     const U *d = D + (int64_t)row * ldd;
+    const int self = self0 >= 0 ? self0 + row : -1;
     int c = 0;
     for (int j0 = 0; j0 < n; j0 += 64) {
         const int j = j0 + lane;
-        const bool hit = j < n && d[j] <= thr;
+        const bool hit = j < n && d[j] <= thr && j != self;
         c += __popcll(__builtin_amdgcn_ballot_w64(hit));
     }
     if (lane == 0)
@@ -36,7 +39,7 @@ __global__ __launch_bounds__(256) void radius_count_kernel(const U *__restrict__
 
 template <typename U>
 __global__ __launch_bounds__(256) void radius_fill_kernel(const U *__restrict__ D, int64_t ldd,
-                                                          int rows, int n, U thr,
+                                                          int rows, int n, U thr, int self0,
                                                           const int *__restrict__ offs,
                                                           int *__restrict__ ind,
                                                           U *__restrict__ key) {
@@ -46,11 +49,12 @@ __global__ __launch_bounds__(256) void radius_fill_kernel(const U *__restrict__ 
         return;
     const U *d = D + (int64_t)row * ldd;
     const unsigned long long below = (1ull << lane) - 1;
+    const int self = self0 >= 0 ? self0 + row : -1;
     int pos = offs[row];
     for (int j0 = 0; j0 < n; j0 += 64) {
         const int j = j0 + lane;
         const U v = j < n ? d[j] : thr;
-        const bool hit = j < n && v <= thr;
+        const bool hit = j < n && v <= thr && j != self;
         const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
         if (hit) {
             const int p = pos + __popcll(m & below);
@@ -72,22 +76,22 @@ __global__ void radius_sqrt_kernel(U *__restrict__ v, int64_t n) {
 } // namespace
 
 template <typename U>
-hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int *counts,
-                               hipStream_t s) {
+hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
+                               int *counts, hipStream_t s) {
     if (rows <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(radius_count_kernel<U>, dim3((rows + 3) / 4), dim3(256), 0, s, D, ldd,
-                       rows, n, thr, counts);
+                       rows, n, thr, self0, counts);
     return hipGetLastError();
 }
 
 template <typename U>
-hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, const int *offs,
-                              int *ind, U *key, hipStream_t s) {
+hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
+                              const int *offs, int *ind, U *key, hipStream_t s) {
     if (rows <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(radius_fill_kernel<U>, dim3((rows + 3) / 4), dim3(256), 0, s, D, ldd,
-                       rows, n, thr, offs, ind, key);
+                       rows, n, thr, self0, offs, ind, key);
     return hipGetLastError();
 }
 
@@ -118,10 +122,10 @@ template <typename U> hipError_t launch_radius_sqrt(U *v, int64_t n, hipStream_t
 }
 
 #define DA_RADIUS_INST(U)                                                                       \
-    template hipError_t launch_radius_count<U>(const U *, int64_t, int, int, U, int *,          \
+    template hipError_t launch_radius_count<U>(const U *, int64_t, int, int, U, int, int *,     \
                                                hipStream_t);                                    \
-    template hipError_t launch_radius_fill<U>(const U *, int64_t, int, int, U, const int *,     \
-                                              int *, U *, hipStream_t);                         \
+    template hipError_t launch_radius_fill<U>(const U *, int64_t, int, int, U, int,             \
+                                              const int *, int *, U *, hipStream_t);            \
     template hipError_t radius_sort_bytes<U>(int, int, size_t *);                               \
     template hipError_t launch_radius_sort<U>(void *, size_t, const U *, U *, const int *,      \
                                               int *, int, int, const int *, hipStream_t);      \

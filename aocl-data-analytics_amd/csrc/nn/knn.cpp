@@ -883,7 +883,7 @@ da_status NNHandle<T>::radius_neighbors(da_int n_queries, da_int n_features, con
                                                       p_, 0, 0, st)));
         }
         KernelScope ks(*gpu, "radius_select");
-        DA_HIP(err, launch_radius_count<T>(rnn_keys_.as<T>(), n, rows, n, thr,
+        DA_HIP(err, launch_radius_count<T>(rnn_keys_.as<T>(), n, rows, n, thr, -1,
                                            rnn_cnt_.as<int>(), st));
         DA_HIP(err, hipMemcpyAsync(cnt.data(), rnn_cnt_.ptr(), rows * sizeof(int),
                                    hipMemcpyDeviceToHost, st));
@@ -900,7 +900,8 @@ da_status NNHandle<T>::radius_neighbors(da_int n_queries, da_int n_features, con
                                    hipMemcpyHostToDevice, st));
         DA_HIP(err, rnn_ind_buf_.reserve((size_t)total * sizeof(int)));
         DA_HIP(err, rnn_key_buf_.reserve((size_t)total * sizeof(T)));
-        DA_HIP(err, launch_radius_fill<T>(rnn_keys_.as<T>(), n, rows, n, thr, rnn_off_.as<int>(),
+        DA_HIP(err, launch_radius_fill<T>(rnn_keys_.as<T>(), n, rows, n, thr, -1,
+                                          rnn_off_.as<int>(),
                                           rnn_ind_buf_.as<int>(),
                                           return_distance ? rnn_key_buf_.as<T>() : nullptr, st));
         int *ind = rnn_ind_buf_.as<int>();

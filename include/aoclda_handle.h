@@ -1,7 +1,7 @@
 /*
  * Opaque handle of the AOCL-DA C API. Enum values follow
  * source/include/aoclda_handle.h:45-88 and the functions :118-150 of the reference.
- * Only da_handle_nn and da_handle_kmeans are backed by an implementation in this
+ * Only da_handle_nn, da_handle_kmeans and da_handle_dbscan are backed by an implementation in this
  * library; initialising any other type returns da_status_not_implemented.
  */
 #ifndef AOCLDA_HANDLE

@@ -547,4 +547,79 @@ long long oracle_radius(int n, int d, const double *Xt, int nq, const double *Xq
     return total;
 }
 
+/*
+ * DBSCAN restated (dbscan.cpp:214-291 compute, :395-445 the serial cluster formation,
+ * :449-470 core points; neighbour lists of radius_neighbors.cpp:47-200). Neighbours of i are
+ * the j != i whose key is <= the threshold: metric 5 ("euclidean") compares squared
+ * distances with eps^2; 0 ("l2") the distance with eps; 6 ("sqeuclidean") the squared
+ * distance with eps; minkowski with p = 2 the squared distance with eps itself, p = 1 the
+ * manhattan distance, other p the minkowski distance; manhattan and cosine the distance.
+ * Keys are exact (direct differences in double); with single != 0 the key and the
+ * threshold are rounded to float first, as a single-precision build computes them in float.
+ * A point with at least min_samples - 1 neighbours is a core point. Clusters are grown
+ * breadth first from each unvisited core point in index order; a non-core point reached
+ * from a cluster joins it, otherwise it is noise (-1). Returns the number of clusters;
+ * core[i] receives 1 for core points. Inputs row-major n x d.
+ */
+int oracle_dbscan(int n, int d, const double *X, int metric, double p, double eps,
+                  int min_samples, int single, int *labels, int *core) {
+    int km = metric;
+    double thr = eps;
+    if (metric == M_EUCLID_GEMM) {
+        km = M_SQEUCLID;
+        thr = single ? (double)((float)eps * (float)eps) : eps * eps;
+    } else if (metric == M_SQEUCLID_GEMM) {
+        km = M_SQEUCLID;
+    } else if (metric == M_MINKOWSKI && p == 2.0) {
+        km = M_SQEUCLID;
+    } else if (metric == M_MINKOWSKI && p == 1.0) {
+        km = M_MANHATTAN;
+    }
+    if (single)
+        thr = (double)(float)thr;
+    std::vector<std::vector<int>> nb(n);
+    std::vector<double> D(n);
+    for (int i = 0; i < n; i++) {
+        oracle_pairwise(0, 1, n, d, X + (size_t)i * d, d, X, d, D.data(), n, p, km,
+                        single ? 1.1920928955078125e-07 : 2.220446049250313e-16);
+        for (int j = 0; j < n; j++) {
+            double v = D[j];
+            if (single)
+                v = (double)(float)v;
+            if (j != i && v <= thr)
+                nb[i].push_back(j);
+        }
+    }
+    for (int i = 0; i < n; i++)
+        core[i] = (int)nb[i].size() >= min_samples - 1 ? 1 : 0;
+    const int UNVISITED = -2, NOISE = -1;
+    for (int i = 0; i < n; i++)
+        labels[i] = UNVISITED;
+    int nc = 0;
+    for (int i = 0; i < n; i++) {
+        if (labels[i] != UNVISITED)
+            continue;
+        if (!core[i]) {
+            labels[i] = NOISE;
+            continue;
+        }
+        labels[i] = nc;
+        std::vector<int> search(nb[i]);
+        for (size_t t = 0; t < search.size(); t++) {
+            const int j = search[t];
+            if (labels[j] == NOISE) {
+                labels[j] = nc;
+                continue;
+            }
+            if (labels[j] != UNVISITED)
+                continue;
+            labels[j] = nc;
+            if (core[j])
+                search.insert(search.end(), nb[j].begin(), nb[j].end());
+        }
+        nc++;
+    }
+    return nc;
+}
+
 } // extern "C"

@@ -90,6 +90,25 @@ def radius(Xtrain, Xq, r, metric="euclidean", p=2.0, single=False, sort=False,
         cap = total  # the first pass only counted: run again with room for every neighbour
 
 
+_lib.oracle_dbscan.restype = C.c_int
+_lib.oracle_dbscan.argtypes = [C.c_int, C.c_int, _d, C.c_int, C.c_double, C.c_double, C.c_int,
+                               C.c_int, _i, _i]
+DBSCAN_METRICS = {"euclidean": 5, "l2": 0, "sqeuclidean": 6, "manhattan": 3, "l1": 3,
+                  "cityblock": 3, "cosine": 4, "minkowski": 2}
+
+
+def dbscan(X, eps, min_samples, metric="euclidean", p=2.0, single=False):
+    """(labels, core flags, n_clusters) of the restated serial DBSCAN; single: keys and
+    threshold compared in float."""
+    A = np.ascontiguousarray(X, dtype=np.float64)
+    n, d = A.shape
+    labels = np.empty(n, dtype=np.int32)
+    core = np.empty(n, dtype=np.int32)
+    nc = _lib.oracle_dbscan(n, d, _p(A), DBSCAN_METRICS[metric], float(p), float(eps),
+                            int(min_samples), int(single), _p(labels, _i), _p(core, _i))
+    return labels, core.astype(bool), nc
+
+
 INIT = {"random": 0, "k-means++": 1, "supplied": 2, "random partitions": 3}
 EMPTY = {"ignore": 0, "error": 1, "split": 2}
 

@@ -206,6 +206,78 @@ known = {
         "comment": "the empty-cluster case with 'empty clusters'='error' returns da_status_empty_clusters",
     },
 }
+
+
+def _three_blobs(n, d, seed):
+    """The reference's 30 x d three-cluster data (dbscan_test_data.hpp:99-131): entry f of
+    the column-major array is U[-0.1, 0.1] plus 2, -2 or 0 by f % 3, so sample i sits near
+    (c, ..., c) with c chosen by i % 3. The draw is seeded here (the reference seeds from
+    std::random_device); the expected labels hold for every draw."""
+    r = np.random.default_rng(seed).uniform(-0.1, 0.1, n * d)
+    f = np.arange(n * d)
+    return (r + np.where(f % 3 == 0, 2.0, np.where(f % 3 == 1, -2.0, 0.0))).tolist()
+
+
+_A25 = [0, 1.0, 1.0, -1.0, -1.0, 10.0, -7.0, -6.0, -8.0, -6.5, -5.0, -8.0, -6.5, -5.0, -12.0,
+        -12.1, -11.9, 15.0, -5.0, -5.1, -4.9, -5.0, -5.1, -5.1, -5.0, 0, 1.0, -1.0, 1.0, -1.0,
+        10.0, 2.0, 2.0, 3.0, 3.1, 3.0, 1.0, 1.0, 1.0, -12.0, -12.1, -11.9, 0.0, -5.0, -5.1,
+        -4.9, -5.0, -5.1, -5.1, -5.0]
+_L25 = [0, 0, 0, 0, 0, -1, 1, 1, 1, 1, 1, 1, 1, 1, -1, -1, -1, -1, 2, 2, 2, 2, 2, 2, 2]
+_C25 = [0, 6, 7, 12, 18, 19, 20, 21, 22, 23, 24]
+_ABC = [0, 1, 2] * 10
+
+
+def _db(name, src, n, d, A, lda, order, min_samples, eps, labels, core, rinfo, n_clusters,
+        algorithm="brute", metric="euclidean", power=2.0, leaf_size=30):
+    return {"name": name, "source": "tests/unit_tests/dbscan/dbscan_test_data.hpp:" + src,
+            "n_samples": n, "n_features": d, "A": A, "lda": lda, "order": order,
+            "min_samples": min_samples, "eps": eps, "algorithm": algorithm, "metric": metric,
+            "power": power, "leaf_size": leaf_size, "labels": labels, "core": core,
+            "rinfo": rinfo, "n_clusters": n_clusters, "n_core": len(core)}
+
+
+_a25_row = np.array(_A25).reshape(2, 25).T.ravel().tolist()
+_a25_sub = _A25[:25] + [0, 0] + _A25[25:] + [0, 0]
+known["dbscan"] = [
+    _db("1 by 1", "40-55,260-268", 1, 1, [2.1], 1, "column-major", 1, 0.5, [0], [0],
+        [1, 1, 1, 0.5, 1, 30, 2, 1, 1], 1),
+    _db("2 by 25, k-d tree", "57-78,280-288", 2, 25, [float(i % 2) for i in range(50)], 2,
+        "column-major", 1, 100.0, [0, 0], [0, 1], [2, 25, 2, 100, 1, 1, 2, 2, 1], 1,
+        algorithm="kd tree", leaf_size=1),
+    _db("2 by 25, auto", "57-78,280-288", 2, 25, [float(i % 2) for i in range(50)], 2,
+        "column-major", 1, 100.0, [0, 0], [0, 1], [2, 25, 2, 100, 1, 1, 2, 2, 1], 1,
+        algorithm="auto", leaf_size=1),
+    _db("zeros", "80-97,270-278", 25, 5, [0.0] * 125, 25, "column-major", 2, 0.5, [0] * 25,
+        list(range(25)), [25, 5, 25, 0.5, 2, 30, 2, 25, 1], 1),
+    _db("25 by 2", "166-190,316-321", 25, 2, _A25, 25, "column-major", 4, 1.5, _L25, _C25,
+        [25, 2, 25, 1.5, 4, 30, 2, 11, 3], 3),
+    _db("25 by 2 row-major", "248-258", 25, 2, _a25_row, 2, "row-major", 4, 1.5, _L25, _C25,
+        [25, 2, 2, 1.5, 4, 30, 2, 11, 3], 3),
+    _db("25 by 2 tiny eps", "322-334", 25, 2, _A25, 25, "column-major", 4, 0.0001, [-1] * 25,
+        [], [25, 2, 25, 0.0001, 4, 30, 2, 0, 0], 0),
+    _db("25 by 2 subarray", "335-348", 25, 2, _a25_sub, 27, "column-major", 4, 1.5, _L25, _C25,
+        [25, 2, 27, 1.5, 4, 30, 2, 11, 3], 3),
+]
+for alg in ("brute", "kd tree", "auto", "ball tree"):
+    known["dbscan"].append(
+        _db(f"30 by 3, {alg}", "99-131,290-306", 30, 3, _three_blobs(30, 3, 11), 30,
+            "column-major", 3, 0.5, _ABC, list(range(30)), [30, 3, 30, 0.5, 3, 30, 2, 30, 3], 3,
+            algorithm=alg))
+for metric, alg, power, eps in (("euclidean", "brute", 2.0, 0.5), ("manhattan", "brute", 2.0, 0.5),
+                                ("minkowski", "brute", 2.0, 0.5),
+                                ("minkowski", "brute", 2.0000001, 0.5),
+                                ("euclidean", "kd tree", 2.0000001, 0.5),
+                                ("euclidean", "ball tree", 2.0000001, 0.5),
+                                ("sqeuclidean", "brute", 2.0000001, 0.25)):
+    known["dbscan"].append(
+        _db(f"30 by 1, {metric}, {alg}, p={power}", "133-164,207-246", 30, 1,
+            _three_blobs(30, 1, 12), 30, "column-major", 3, eps, _ABC, list(range(30)),
+            [30, 1, 30, eps, 3, 30, power, 30, 3], 3, algorithm=alg, metric=metric, power=power))
+known["dbscan_errors"] = {
+    "source": "tests/unit_tests/dbscan/dbscan_public.cpp:309-395",
+    "A_colmajor": [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12], "n_samples": 4, "n_features": 3,
+    "lda": 4,
+}
 with open(os.path.join(HERE, "known_answers.json"), "w") as f:
     json.dump(known, f, indent=1)
 print("fixtures written to", HERE)

@@ -202,3 +202,61 @@ def test_radius_vs_sklearn_and_brute_force(oracle):
         exp = np.nonzero(D[q] ** 2 <= 2.6)[0]
         np.testing.assert_array_equal(iv, exp)
         np.testing.assert_allclose(dv, D[q, exp] ** 2, rtol=1e-12)
+
+
+def _first_seen(labels):
+    """Renames cluster labels in order of first appearance (noise stays -1), as the
+    reference's DBSCAN tests compare them (dbscan_public.cpp:42-63)."""
+    m, out = {}, []
+    for v in labels:
+        if v == -1:
+            out.append(-1)
+        else:
+            out.append(m.setdefault(v, len(m)))
+    return out
+
+
+def _dbscan_case_data(c):
+    A = np.array(c["A"], dtype=np.float64)
+    n, d, lda = c["n_samples"], c["n_features"], c["lda"]
+    if c["order"] == "row-major":
+        return A.reshape(n, lda)[:, :d]
+    return A.reshape(d, lda).T[:n, :]
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_dbscan_known_answers(known, oracle, single):
+    """The reference's DBSCAN cases (dbscan_test_data.hpp): labels up to renaming, core
+    samples, cluster counts; tree algorithms give the same neighbourhoods as brute force."""
+    for c in known["dbscan"]:
+        X = _dbscan_case_data(c)
+        labels, core, nc = oracle.dbscan(X, c["eps"], c["min_samples"], c["metric"], c["power"],
+                                         single=single)
+        assert nc == c["n_clusters"], c["name"]
+        assert _first_seen(labels) == _first_seen(c["labels"]), c["name"]
+        assert list(np.nonzero(core)[0]) == c["core"], c["name"]
+
+
+def test_dbscan_vs_sklearn(oracle):
+    """Labels equal to scikit-learn's DBSCAN (brute force), including its numbering: clusters
+    in the order of their first core point, a border point in the first cluster reaching it."""
+    from sklearn.cluster import DBSCAN
+    rng = np.random.default_rng(21)
+    centres = rng.uniform(-6, 6, (6, 4))
+    X = np.vstack([c + rng.normal(0, 0.6, (120, 4)) for c in centres] +
+                  [rng.uniform(-9, 9, (60, 4))])
+    X = X[rng.permutation(len(X))]
+    for metric, eps, p in (("euclidean", 0.9, 2.0), ("manhattan", 1.6, 2.0),
+                           ("cosine", 0.02, 2.0), ("minkowski", 0.8, 3.0)):
+        for ms in (1, 5, 12):
+            sk = DBSCAN(eps=eps, min_samples=ms, metric=metric, p=p, algorithm="brute").fit(X)
+            labels, core, nc = oracle.dbscan(X, eps, ms, metric, p)
+            np.testing.assert_array_equal(labels, sk.labels_, err_msg=f"{metric} {ms}")
+            np.testing.assert_array_equal(np.nonzero(core)[0], sk.core_sample_indices_)
+            assert nc == len(set(sk.labels_) - {-1})
+    # "sqeuclidean" compares squared distances with eps; "l2" the distance; minkowski p = 2
+    # is evaluated as the squared distance and compared with eps itself, as the reference does
+    sk = DBSCAN(eps=0.9, min_samples=5, algorithm="brute").fit(X)
+    np.testing.assert_array_equal(oracle.dbscan(X, 0.81, 5, "sqeuclidean")[0], sk.labels_)
+    np.testing.assert_array_equal(oracle.dbscan(X, 0.9, 5, "l2")[0], sk.labels_)
+    np.testing.assert_array_equal(oracle.dbscan(X, 0.81, 5, "minkowski", 2.0)[0], sk.labels_)
