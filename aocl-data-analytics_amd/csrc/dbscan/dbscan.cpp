@@ -107,11 +107,12 @@ template <typename T> da_status DBSCANHandle<T>::neighbour_graph(int km, T thr) 
     // Row blocks whose key tile stays within 256 MB (and 2^28 entries).
     const size_t max_keys = std::min<size_t>(((size_t)256 << 20) / sizeof(T), (size_t)1 << 28);
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n, max_keys / (size_t)n));
+    const int seg = radius_segment_len(n), nseg = (n + seg - 1) / seg;
     DA_HIP(err, keys_.reserve((size_t)chunk * n * sizeof(T)));
-    DA_HIP(err, cnt_.reserve((size_t)n * sizeof(int)));
-    DA_HIP(err, boff_.reserve((size_t)(chunk + 1) * sizeof(int)));
+    DA_HIP(err, cnt_.reserve((size_t)chunk * nseg * sizeof(int)));
+    DA_HIP(err, boff_.reserve((size_t)chunk * nseg * sizeof(int)));
     count_.assign(n, 0);
-    std::vector<int> boff(chunk + 1);
+    std::vector<int> scnt((size_t)chunk * nseg), boff((size_t)chunk * nseg);
     long long E = 0;
     size_t cap = ind_.size() / sizeof(int);
     const T *X = X_.as<T>();
@@ -125,16 +126,23 @@ template <typename T> da_status DBSCANHandle<T>::neighbour_graph(int km, T thr) 
         }
         {
             KernelScope ks(*gpu, "radius_count");
-            DA_HIP(err, launch_radius_count<T>(keys_.as<T>(), n, rows, n, thr, r0,
-                                               cnt_.as<int>() + r0, st));
+            DA_HIP(err, launch_radius_count<T>(keys_.as<T>(), n, rows, n, thr, r0, cnt_.as<int>(),
+                                               st));
         }
-        DA_HIP(err, hipMemcpyAsync(count_.data() + r0, cnt_.as<int>() + r0, rows * sizeof(int),
+        DA_HIP(err, hipMemcpyAsync(scnt.data(), cnt_.ptr(), (size_t)rows * nseg * sizeof(int),
                                    hipMemcpyDeviceToHost, st));
         DA_HIP(err, hipStreamSynchronize(st));
-        boff[0] = 0;
-        for (int r = 0; r < rows; r++)
-            boff[r + 1] = boff[r] + count_[r0 + r];
-        const long long tb = boff[rows];
+        // Offsets of each row segment within the block's part of the neighbour list.
+        int tb = 0;
+        for (int r = 0; r < rows; r++) {
+            int c = 0;
+            for (int g = 0; g < nseg; g++) {
+                boff[(size_t)r * nseg + g] = tb + c;
+                c += scnt[(size_t)r * nseg + g];
+            }
+            count_[r0 + r] = c;
+            tb += c;
+        }
         if (tb == 0)
             continue;
         if ((size_t)(E + tb) > cap) {
@@ -149,7 +157,7 @@ template <typename T> da_status DBSCANHandle<T>::neighbour_graph(int km, T thr) 
             ind_spare_.release();
             cap = ncap;
         }
-        DA_HIP(err, hipMemcpyAsync(boff_.ptr(), boff.data(), (rows + 1) * sizeof(int),
+        DA_HIP(err, hipMemcpyAsync(boff_.ptr(), boff.data(), (size_t)rows * nseg * sizeof(int),
                                    hipMemcpyHostToDevice, st));
         KernelScope ks(*gpu, "radius_fill");
         DA_HIP(err, launch_radius_fill<T>(keys_.as<T>(), n, rows, n, thr, r0, boff_.as<int>(),

@@ -174,18 +174,21 @@ int knn_tile_kc(int k);
 int knn_tile_qb(int kc);
 // Largest nsplit * kc the merge accepts.
 int knn_merge_max_candidates();
-// Radius search on a rows x n tile of distance keys D (row-major, leading dimension ldd):
-// counts[r] = #{j : D[r][j] <= thr, j != self0 + r when self0 >= 0}; the fill writes, from offs[r] on, the indices j of
-// those entries in ascending order (and their keys when key is not null); the sort orders
-// each segment [offs[r], offs[r+1]) by key, stably (ties stay in index order), with
+// Radius search on a rows x n tile of distance keys D (row-major, leading dimension ldd).
+// Each row is cut into nseg = ceil(n / radius_segment_len(n)) segments of columns. The
+// count writes segcnt[r * nseg + s] = #{j in segment s : D[r][j] <= thr, j != self0 + r when
+// self0 >= 0}; the fill writes, from segoff[r * nseg + s] on, the indices j of those entries
+// in ascending order (and their keys when key is not null); the sort orders
+// each row's range [offs[r], offs[r+1]) by key, stably (ties stay in index order), with
 // temporary storage of radius_sort_bytes bytes; the sqrt pass takes the square root of n
 // values in place.
 template <typename U>
 hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
-                               int *counts, hipStream_t s);
+                               int *segcnt, hipStream_t s);
 template <typename U>
 hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
-                              const int *offs, int *ind, U *key, hipStream_t s);
+                              const int *segoff, int *ind, U *key, hipStream_t s);
+int radius_segment_len(int n);
 
 // ---- DBSCAN over a neighbour graph in CSR form (offs: n + 1 64-bit offsets, ind) ----
 // parent[i] = i.

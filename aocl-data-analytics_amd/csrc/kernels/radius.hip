@@ -2,10 +2,11 @@
 // rows whose key is at most a threshold (the reference's brute-force radius search,
 // nearest_neighbors.cpp:2080-2215: D <= radius, or <= radius^2 on squared euclidean keys).
 //
-// A query's hits are found by one wave sweeping its key row 64 entries at a time: a ballot
-// of the compares gives the count, and in the fill pass the position of each hit (the
-// running count plus the hits of lower lanes), so every query's neighbours come out in
-// ascending training index without atomics. Optional ordering by distance is a stable
+// A row of keys is cut into segments of radius_segment_len(n) columns, each swept by one
+// wave 64 entries at a time: a ballot of the compares gives the segment's count, and in the
+// fill pass the position of each hit (the segment's offset, the running count and the hits
+// of lower lanes), so every query's neighbours come out in ascending training index without
+// atomics. Optional ordering by distance is a stable
 // segmented radix sort of (key, index) pairs, which keeps ties in index order. With
 // self0 >= 0 the entry of row r at column self0 + r is skipped (a point is not its own
 // neighbour when a set is compared with itself).
@@ -17,52 +18,75 @@ namespace dagpu {
 
 namespace {
 
+// A wave sweeps one segment of one row, four 64-entry steps per iteration so that four
+// loads are in flight; the grid is (row groups of 4) x (segments).
 template <typename U>
 __global__ __launch_bounds__(256) void radius_count_kernel(const U *__restrict__ D, int64_t ldd,
                                                            int rows, int n, U thr, int self0,
-                                                           int *__restrict__ counts) {
+                                                           int seg, int nseg,
+                                                           int *__restrict__ segcnt) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     if (row >= rows)
         return;
+    const int c0 = (int)blockIdx.y * seg, c1 = min(n, c0 + seg);
     const U *d = D + (int64_t)row * ldd;
     const int self = self0 >= 0 ? self0 + row : -1;
     int c = 0;
-    for (int j0 = 0; j0 < n; j0 += 64) {
-        const int j = j0 + lane;
-        const bool hit = j < n && d[j] <= thr && j != self;
-        c += __popcll(__builtin_amdgcn_ballot_w64(hit));
+    for (int j0 = c0; j0 < c1; j0 += 256) {
+        U v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int j = j0 + u * 64 + lane;
+            v[u] = j < c1 ? d[j] : thr;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int j = j0 + u * 64 + lane;
+            const bool hit = j < c1 && v[u] <= thr && j != self;
+            c += __popcll(__builtin_amdgcn_ballot_w64(hit));
+        }
     }
     if (lane == 0)
-        counts[row] = c;
+        segcnt[(int64_t)row * nseg + blockIdx.y] = c;
 }
 
 template <typename U>
 __global__ __launch_bounds__(256) void radius_fill_kernel(const U *__restrict__ D, int64_t ldd,
                                                           int rows, int n, U thr, int self0,
-                                                          const int *__restrict__ offs,
+                                                          int seg, int nseg,
+                                                          const int *__restrict__ segoff,
                                                           int *__restrict__ ind,
                                                           U *__restrict__ key) {
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     if (row >= rows)
         return;
+    const int c0 = (int)blockIdx.y * seg, c1 = min(n, c0 + seg);
     const U *d = D + (int64_t)row * ldd;
     const unsigned long long below = (1ull << lane) - 1;
     const int self = self0 >= 0 ? self0 + row : -1;
-    int pos = offs[row];
-    for (int j0 = 0; j0 < n; j0 += 64) {
-        const int j = j0 + lane;
-        const U v = j < n ? d[j] : thr;
-        const bool hit = j < n && v <= thr && j != self;
-        const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
-        if (hit) {
-            const int p = pos + __popcll(m & below);
-            ind[p] = j;
-            if (key)
-                key[p] = v;
+    int pos = segoff[(int64_t)row * nseg + blockIdx.y];
+    for (int j0 = c0; j0 < c1; j0 += 256) {
+        U v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int j = j0 + u * 64 + lane;
+            v[u] = j < c1 ? d[j] : thr;
         }
-        pos += __popcll(m);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int j = j0 + u * 64 + lane;
+            const bool hit = j < c1 && v[u] <= thr && j != self;
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(hit);
+            if (hit) {
+                const int p = pos + __popcll(m & below);
+                ind[p] = j;
+                if (key)
+                    key[p] = v[u];
+            }
+            pos += __popcll(m);
+        }
     }
 }
 
@@ -75,23 +99,33 @@ __global__ void radius_sqrt_kernel(U *__restrict__ v, int64_t n) {
 
 } // namespace
 
+int radius_segment_len(int n) {
+    // 2,048 columns a segment, longer when the segment count would pass the grid's y limit.
+    int seg = 2048;
+    while ((int64_t)(n + seg - 1) / seg > 65535)
+        seg *= 2;
+    return seg;
+}
+
 template <typename U>
 hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
-                               int *counts, hipStream_t s) {
-    if (rows <= 0)
+                               int *segcnt, hipStream_t s) {
+    if (rows <= 0 || n <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(radius_count_kernel<U>, dim3((rows + 3) / 4), dim3(256), 0, s, D, ldd,
-                       rows, n, thr, self0, counts);
+    const int seg = radius_segment_len(n), nseg = (n + seg - 1) / seg;
+    hipLaunchKernelGGL(radius_count_kernel<U>, dim3((rows + 3) / 4, nseg), dim3(256), 0, s, D,
+                       ldd, rows, n, thr, self0, seg, nseg, segcnt);
     return hipGetLastError();
 }
 
 template <typename U>
 hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
-                              const int *offs, int *ind, U *key, hipStream_t s) {
-    if (rows <= 0)
+                              const int *segoff, int *ind, U *key, hipStream_t s) {
+    if (rows <= 0 || n <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(radius_fill_kernel<U>, dim3((rows + 3) / 4), dim3(256), 0, s, D, ldd,
-                       rows, n, thr, self0, offs, ind, key);
+    const int seg = radius_segment_len(n), nseg = (n + seg - 1) / seg;
+    hipLaunchKernelGGL(radius_fill_kernel<U>, dim3((rows + 3) / 4, nseg), dim3(256), 0, s, D,
+                       ldd, rows, n, thr, self0, seg, nseg, segoff, ind, key);
     return hipGetLastError();
 }
 

@@ -869,10 +869,12 @@ da_status NNHandle<T>::radius_neighbors(da_int n_queries, da_int n_features, con
     const size_t max_keys = ((size_t)256 << 20) / sizeof(T);
     const da_int chunk = (da_int)std::max<size_t>(
         1, std::min<size_t>(n_queries, std::min<size_t>(max_keys, (size_t)1 << 28) / (size_t)n));
+    const int seg = radius_segment_len(n), nseg = (n + seg - 1) / seg;
     DA_HIP(err, rnn_keys_.reserve((size_t)chunk * n * sizeof(T)));
-    DA_HIP(err, rnn_cnt_.reserve((size_t)chunk * sizeof(int)));
+    DA_HIP(err, rnn_cnt_.reserve((size_t)chunk * nseg * sizeof(int)));
+    DA_HIP(err, rnn_segoff_.reserve((size_t)chunk * nseg * sizeof(int)));
     DA_HIP(err, rnn_off_.reserve((size_t)(chunk + 1) * sizeof(int)));
-    std::vector<int> cnt(chunk), off(chunk + 1);
+    std::vector<int> scnt((size_t)chunk * nseg), soff((size_t)chunk * nseg), off(chunk + 1);
     for (da_int q0 = 0; q0 < n_queries; q0 += chunk) {
         const da_int rows = std::min(chunk, n_queries - q0);
         {
@@ -885,23 +887,31 @@ da_status NNHandle<T>::radius_neighbors(da_int n_queries, da_int n_features, con
         KernelScope ks(*gpu, "radius_select");
         DA_HIP(err, launch_radius_count<T>(rnn_keys_.as<T>(), n, rows, n, thr, -1,
                                            rnn_cnt_.as<int>(), st));
-        DA_HIP(err, hipMemcpyAsync(cnt.data(), rnn_cnt_.ptr(), rows * sizeof(int),
+        DA_HIP(err, hipMemcpyAsync(scnt.data(), rnn_cnt_.ptr(), (size_t)rows * nseg * sizeof(int),
                                    hipMemcpyDeviceToHost, st));
         DA_HIP(err, hipStreamSynchronize(st));
+        // Offsets per row (for the sort) and per row segment (for the fill).
         off[0] = 0;
-        for (da_int r = 0; r < rows; r++)
-            off[r + 1] = off[r] + cnt[r];
+        for (da_int r = 0; r < rows; r++) {
+            int c = 0;
+            for (int g = 0; g < nseg; g++) {
+                soff[(size_t)r * nseg + g] = off[r] + c;
+                c += scnt[(size_t)r * nseg + g];
+            }
+            off[r + 1] = off[r] + c;
+            rnn_count_[q0 + r] = c;
+        }
         const int total = off[rows];
-        for (da_int r = 0; r < rows; r++)
-            rnn_count_[q0 + r] = cnt[r];
         if (total == 0)
             continue;
         DA_HIP(err, hipMemcpyAsync(rnn_off_.ptr(), off.data(), (rows + 1) * sizeof(int),
                                    hipMemcpyHostToDevice, st));
+        DA_HIP(err, hipMemcpyAsync(rnn_segoff_.ptr(), soff.data(),
+                                   (size_t)rows * nseg * sizeof(int), hipMemcpyHostToDevice, st));
         DA_HIP(err, rnn_ind_buf_.reserve((size_t)total * sizeof(int)));
         DA_HIP(err, rnn_key_buf_.reserve((size_t)total * sizeof(T)));
         DA_HIP(err, launch_radius_fill<T>(rnn_keys_.as<T>(), n, rows, n, thr, -1,
-                                          rnn_off_.as<int>(),
+                                          rnn_segoff_.as<int>(),
                                           rnn_ind_buf_.as<int>(),
                                           return_distance ? rnn_key_buf_.as<T>() : nullptr, st));
         int *ind = rnn_ind_buf_.as<int>();

@@ -402,6 +402,98 @@ def kmeans_batch_bench(args, L, aoclda):
         L.lib.da_amd_free(p)
 
 
+DBSCAN_METRIC = "DBSCAN fits per second (eps-neighbourhoods, core points, clusters, labels), 1 GPU"
+
+
+def dbscan_bench(args, L, aoclda):
+    """DBSCAN on 100,000 x 16 fp32 samples: 32 Gaussian blobs (means U[-10,10]^16, sigma 0.5)
+    plus 5% uniform noise in [-12,12]^16 (numpy default_rng(9)), euclidean, eps 1.8,
+    min_samples 10. A step is one da_dbscan_compute_s with the samples resident in GPU
+    memory: exact key tiles, threshold counts and neighbour lists, union-find, labels. The
+    GPU labels on a 12,000-sample subset are compared with the oracle's."""
+    import ctypes as C
+    rng = np.random.default_rng(9)
+    n, d, eps, ms = args.db_n, 16, 1.8, 10
+    means = rng.uniform(-10, 10, (32, d))
+    nb = n - n // 20
+    X = np.vstack([means[rng.integers(0, 32, nb)] + 0.5 * rng.standard_normal((nb, d)),
+                   rng.uniform(-12, 12, (n - nb, d))]).astype(np.float32)
+    X = X[rng.permutation(n)]
+
+    def handle(data):
+        h = L.Handle(L.HANDLE_DBSCAN, np.float32)
+        h.set_string("storage order", "row-major")
+        h.set_string("algorithm", "brute")
+        h.set_int("min samples", ms)
+        h.set_real("eps", eps)
+        data = np.ascontiguousarray(data)
+        h.call("da_dbscan_set_data", data.shape[0], d, L.ptr(data), d)
+        return h, data
+
+    h, Xc = handle(X)
+    L.check(L.lib.da_amd_profile_enable(h.h, 1))
+    for _ in range(args.warmup):
+        h.call("da_dbscan_compute")
+    L.check(L.lib.da_amd_profile_reset(h.h))
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        h.call("da_dbscan_compute")
+        times.append(time.perf_counter() - t0)
+    names = ("pairwise_direct", "radius_count", "radius_fill", "dbscan_union", "dbscan_label")
+    kern = {}
+    for nm in names:
+        ms_, cnt = C.c_double(), L.da_int()
+        L.check(L.lib.da_amd_profile_get(h.h, nm.encode(), C.byref(ms_), C.byref(cnt)))
+        if cnt.value:
+            kern[nm] = {"total_ms": ms_.value, "launches": cnt.value}
+    r = np.zeros(9, np.float32)
+    dim = L.da_int(9)
+    h.call("da_handle_get_result", L.RINFO, C.byref(dim), L.ptr(r))
+    step = float(np.median(times))
+    log(f"dbscan: {n}x{d}, step {step * 1e3:.2f} ms, clusters {int(r[8])}, core {int(r[7])}")
+    # Dominant kernel: the exact key tile (3d fp64 operations per pair, the tile written once).
+    pd = kern.get("pairwise_direct", {"total_ms": 0.0, "launches": 1})
+    pd_s = pd["total_ms"] / args.steps * 1e-3
+    flops = 3.0 * n * n * d
+    byts = 4.0 * n * n + 4.0 * n * d
+    t_f, t_b = flops / 78.6e12, byts / 8e12
+    if t_b >= t_f:
+        roof = {"kernel": "pairwise_direct", "bound": "hbm", "achieved": byts / pd_s / 1e9,
+                "peak": 8000.0, "unit": "GB/s", "frac": byts / pd_s / 8e12, "traffic": None}
+    else:
+        roof = {"kernel": "pairwise_direct", "bound": "fp64", "achieved": flops / pd_s / 1e12,
+                "peak": 78.6, "unit": "TFLOP/s", "frac": flops / pd_s / 78.6e12, "traffic": None}
+    roof["algorithmic_bytes_per_fit"] = byts
+    roof["fp64_ops_per_fit"] = flops
+    # Parity and CPU baseline on a subset.
+    sub = X[:args.db_cpu_sample]
+    hs, _ = handle(sub)
+    hs.call("da_dbscan_compute")
+    lab = np.zeros(len(sub), np.int32)
+    dim = L.da_int(len(sub))
+    L.check(L.lib.da_handle_get_result_int(hs.h, L.DBSCAN_LABELS, C.byref(dim), L.ptr(lab)))
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    el, _, _ = O.dbscan(sub.astype(np.float64), eps, ms, single=True)
+    cpu_s = time.perf_counter() - t0
+    scale = (n / len(sub)) ** 2
+    print(json.dumps({
+        "metric": DBSCAN_METRIC, "value": 1.0 / step, "unit": "fits/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": step * 1e3,
+        "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic: 32 Gaussian blobs + 5% uniform noise, numpy default_rng(9)",
+        "config": {"workload": f"DBSCAN, {n}x{d} fp32, euclidean, eps {eps}, min_samples {ms}"},
+        "n_clusters": int(r[8]), "n_core_samples": int(r[7]),
+        "pair_evaluations_per_s": n * n / step, "kernels_ms": kern, "roofline": roof,
+        "sample_labels_identical": bool(np.array_equal(lab, el)),
+        "cpu_baseline": {"value": 1.0 / (cpu_s * scale), "unit": "fits/s", "cores": 1,
+                         "kind": "port",
+                         "sample": f"oracle DBSCAN (serial, exact keys) on the first "
+                                   f"{len(sub)} samples, {cpu_s:.2f} s, scaled by "
+                                   f"(n / sample)^2 = {scale:.1f} to the full fit"}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -433,6 +525,10 @@ def main():
     ap.add_argument("--kb-instances", type=int, default=1024)
     ap.add_argument("--kb-iter", type=int, default=15)
     ap.add_argument("--kb-reps", type=int, default=3)
+    ap.add_argument("--dbscan", action="store_true",
+                    help="DBSCAN workload (100,000 x 16, eps 1.8, min_samples 10)")
+    ap.add_argument("--db-n", type=int, default=100_000)
+    ap.add_argument("--db-cpu-sample", type=int, default=12_000)
     ap.add_argument("--serving", action="store_true",
                     help="run the continuous-batching serving benchmark instead")
     ap.add_argument("--serve-rows", type=int, default=2_000_000)
@@ -459,6 +555,9 @@ def main():
         return
     if args.kmeans_batch:
         kmeans_batch_bench(args, L, aoclda)
+        return
+    if args.dbscan:
+        dbscan_bench(args, L, aoclda)
         return
 
     if args.copy_calibration_mib > 0:
