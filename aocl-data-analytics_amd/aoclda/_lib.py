@@ -32,7 +32,7 @@ STATUS = {
     17: "invalid_array_dimension", 18: "unknown_query", 19: "incorrect_output",
     20: "no_data", 21: "maxit", 22: "maxtime", 23: "numerical_difficulties",
     24: "bad_derivatives", 25: "arch_not_supported", 26: "io_error",
-    400: "option_not_found", 401: "option_locked", 402: "option_wrong_type",
+    700: "outlier_warning", 400: "option_not_found", 401: "option_locked", 402: "option_wrong_type",
     403: "option_invalid_bounds", 404: "option_invalid_value", 900: "empty_clusters",
 }
 CODE = {v: k for k, v in STATUS.items()}
@@ -68,6 +68,12 @@ for _t, _ct in (("d", C.c_double), ("s", C.c_float)):
            C.c_void_p, C.c_void_p, da_int, da_int)
     _proto(f"da_nn_radius_neighbors_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int,
            _ct, da_int, da_int)
+    _proto(f"da_nn_set_labels_{_t}", S, da_handle, da_int, C.c_void_p)
+    _proto(f"da_nn_set_targets_{_t}", S, da_handle, da_int, C.c_void_p)
+    _proto(f"da_nn_classes_{_t}", S, da_handle, P(da_int), C.c_void_p)
+    for _f in ("classifier_predict_proba", "classifier_predict", "regressor_predict"):
+        _proto(f"da_nn_{_f}_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int, C.c_void_p,
+               C.c_int)
     _proto(f"da_dbscan_set_data_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int)
     _proto(f"da_dbscan_compute_{_t}", S, da_handle)
     _proto(f"da_kmeans_set_data_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int)

@@ -23,6 +23,8 @@ class nearest_neighbors:
         self.n_neighbors = n_neighbors
         self.weights, self.algorithm, self.leaf_size = weights, algorithm, leaf_size
         self.metric, self.p, self.radius = metric, p, radius
+        self.outlier_handling = outlier_handling
+        self.y = None
         self.check_data = check_data
         self.precision = precision
         self.tile_variant = tile_variant
@@ -56,6 +58,7 @@ class nearest_neighbors:
         ld = n if self.order == L.COLUMN_MAJOR else d
         self._h.call("da_nn_set_data", n, d, L.ptr(X), ld)
         self.n_samples, self.n_features = n, d
+        self.y = None if y is None else np.asarray(y)
         return self
 
     def kneighbors(self, X, n_neighbors=None, return_distance=True):
@@ -108,6 +111,83 @@ class nearest_neighbors:
         for q in range(nq):
             dists[q] = flat_d[bounds[q]:bounds[q + 1]].copy()
         return dists, inds
+
+    _MODES = {"knn": 0, "radius_neighbors": 1}
+
+    def _predict_setup(self, search_mode, classification):
+        """Labels or targets and the outlier options of a prediction (neighbors.py:335-535 of
+        the reference)."""
+        if search_mode not in self._MODES:
+            raise ValueError("search_mode must be 'knn' or 'radius_neighbors'")
+        if self.y is None:
+            raise ValueError("y was not provided during fit().")
+        if search_mode == "radius_neighbors" and self.radius is None:
+            raise ValueError("radius was None in the constructor.")
+        h = self._h
+        if classification:
+            y = np.ascontiguousarray(self.y, dtype=np.int32)
+            h.call("da_nn_set_labels", len(y), L.ptr(y))
+        else:
+            y = np.ascontiguousarray(self.y, dtype=self.dtype)
+            h.call("da_nn_set_targets", len(y), L.ptr(y))
+        self._y_dev = y
+        mode = "none"
+        if search_mode == "radius_neighbors" and self.outlier_handling is not None:
+            if self.outlier_handling == "most_frequent":
+                mode = "most frequent"
+            else:
+                mode = "manual"
+                if classification:
+                    if not isinstance(self.outlier_handling, (int, np.integer)):
+                        raise ValueError("outlier_handling must be None, 'most_frequent', or "
+                                         "an integer for classification.")
+                    h.set_int("outlier label", int(self.outlier_handling))
+                else:
+                    h.set_real("outlier target", float(self.outlier_handling))
+        h.set_string("outlier handling", mode)
+        return self._MODES[search_mode]
+
+    def _queries(self, X):
+        X, _, _ = L.prepare(X, dtype=self.dtype, order=self.order)
+        nq, d = X.shape
+        return X, nq, d, (nq if self.order == L.COLUMN_MAJOR else d)
+
+    def classifier_predict_proba(self, X, search_mode="knn"):
+        """Class probabilities (n_queries x n_classes, classes ascending) from the k nearest
+        neighbours or the neighbours within the radius."""
+        m = self._predict_setup(search_mode, True)
+        X, nq, d, ld = self._queries(X)
+        n = L.da_int(0)
+        self._h.call("da_nn_classes", C.byref(n), None)
+        proba = np.empty((nq, n.value), dtype=self.dtype,
+                         order="F" if self.order == L.COLUMN_MAJOR else "C")
+        self.status = self._h.call("da_nn_classifier_predict_proba", nq, d, L.ptr(X), ld,
+                                   L.ptr(proba), m, allow=(L.CODE["outlier_warning"],))
+        return proba
+
+    def classifier_predict(self, X, search_mode="knn"):
+        m = self._predict_setup(search_mode, True)
+        X, nq, d, ld = self._queries(X)
+        out = np.empty(nq, dtype=np.int32)
+        self.status = self._h.call("da_nn_classifier_predict", nq, d, L.ptr(X), ld, L.ptr(out),
+                                   m, allow=(L.CODE["outlier_warning"],))
+        return out
+
+    def regressor_predict(self, X, search_mode="knn"):
+        m = self._predict_setup(search_mode, False)
+        X, nq, d, ld = self._queries(X)
+        out = np.empty(nq, dtype=self.dtype)
+        self.status = self._h.call("da_nn_regressor_predict", nq, d, L.ptr(X), ld, L.ptr(out), m,
+                                   allow=(L.CODE["outlier_warning"],))
+        return out
+
+    @property
+    def classes(self):
+        n = L.da_int(0)
+        self._h.call("da_nn_classes", C.byref(n), None)
+        out = np.zeros(max(n.value, 1), dtype=np.int32)
+        self._h.call("da_nn_classes", C.byref(n), L.ptr(out))
+        return out[:n.value]
 
     def _int_result(self, query, size):
         out = np.zeros(size, dtype=np.int32)

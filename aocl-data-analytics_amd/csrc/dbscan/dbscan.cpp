@@ -104,8 +104,9 @@ template <typename T> da_status DBSCANHandle<T>::neighbour_graph(int km, T thr) 
                                         st));
         nrm = norms_.as<double>();
     }
-    // Row blocks whose key tile stays within 256 MB (and 2^28 entries).
-    const size_t max_keys = std::min<size_t>(((size_t)256 << 20) / sizeof(T), (size_t)1 << 28);
+    // Row blocks whose key tile stays within 1 GiB (and 2^28 entries): each block costs a
+    // host round trip for its counts, so blocks are made large.
+    const size_t max_keys = std::min<size_t>(((size_t)1 << 30) / sizeof(T), (size_t)1 << 28);
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>(n, max_keys / (size_t)n));
     const int seg = radius_segment_len(n), nseg = (n + seg - 1) / seg;
     DA_HIP(err, keys_.reserve((size_t)chunk * n * sizeof(T)));

@@ -190,6 +190,25 @@ hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, i
                               const int *segoff, int *ind, U *key, hipStream_t s);
 int radius_segment_len(int n);
 
+// ---- k-NN classification and regression from neighbour lists ----
+// Neighbours of query q: ind[b + t], t < cnt, with b = off[q], cnt = off[q+1] - off[q] (a
+// radius search) or b = q*k, cnt = k when off is null; dist alongside (read only when
+// weights = 1, i.e. 1/d weights). Classification writes proba (nq x ncls, row per query)
+// from cls (class index of each training row) and, when pred is not null, the label
+// classes[argmax] (fallback_label when the largest probability is 0). A query without
+// neighbours lowers *first_empty to qbase + its index; outlier_mode 0 leaves its row zero, 1 and 2
+// set proba[fill_idx] = 1 (none when fill_idx < 0). Regression writes the mean target (or
+// the 1/d-weighted mean), and fallback for a query without neighbours.
+template <typename T>
+hipError_t launch_nn_classify(int nq, const int *off, int k, const int *ind, const T *dist,
+                              int weights, const int *cls, const int *classes, int ncls,
+                              int outlier_mode, int fill_idx, int fallback_label, T *proba,
+                              int *pred, int qbase, int *first_empty, hipStream_t s);
+template <typename T>
+hipError_t launch_nn_regress(int nq, const int *off, int k, const int *ind, const T *dist,
+                             int weights, const T *tgt, int outlier_mode, T fallback, T *out,
+                             int qbase, int *first_empty, hipStream_t s);
+
 // ---- DBSCAN over a neighbour graph in CSR form (offs: n + 1 64-bit offsets, ind) ----
 // parent[i] = i.
 hipError_t launch_dbscan_init(int *parent, int n, hipStream_t s);

@@ -137,3 +137,81 @@ da_status da_nn_last_query_stats(da_handle handle, da_int *n_fallback, da_int *n
 }
 
 } // extern "C"
+
+// ---- classification and regression (nearest_neighbors_public.cpp of the reference) ----
+
+#define DA_NN_PROLOGUE(T)                                                                        \
+    if (!handle)                                                                                 \
+        return da_status_handle_not_initialized;                                                 \
+    handle->err.clear();                                                                         \
+    da_status s_;                                                                                \
+    NNHandle<T> *nn = nn_of<T>(handle, s_);                                                      \
+    if (!nn)                                                                                     \
+        return s_;
+
+extern "C" {
+
+da_status da_nn_set_labels_s(da_handle handle, da_int n_samples, const da_int *y_train) {
+    DA_NN_PROLOGUE(float)
+    return nn->set_labels(n_samples, y_train);
+}
+da_status da_nn_set_labels_d(da_handle handle, da_int n_samples, const da_int *y_train) {
+    DA_NN_PROLOGUE(double)
+    return nn->set_labels(n_samples, y_train);
+}
+da_status da_nn_set_targets_d(da_handle handle, da_int n_samples, const double *y_train) {
+    DA_NN_PROLOGUE(double)
+    return nn->set_targets(n_samples, y_train);
+}
+da_status da_nn_set_targets_s(da_handle handle, da_int n_samples, const float *y_train) {
+    DA_NN_PROLOGUE(float)
+    return nn->set_targets(n_samples, y_train);
+}
+da_status da_nn_classes_d(da_handle handle, da_int *n_classes, da_int *classes) {
+    DA_NN_PROLOGUE(double)
+    return nn->classes(n_classes, classes);
+}
+da_status da_nn_classes_s(da_handle handle, da_int *n_classes, da_int *classes) {
+    DA_NN_PROLOGUE(float)
+    return nn->classes(n_classes, classes);
+}
+da_status da_nn_classifier_predict_proba_d(da_handle handle, da_int n_queries,
+                                           da_int n_features, const double *X_test,
+                                           da_int ldx_test, double *proba,
+                                           da_nn_search_mode search_mode) {
+    DA_NN_PROLOGUE(double)
+    return nn->predict_proba(n_queries, n_features, X_test, ldx_test, proba, search_mode);
+}
+da_status da_nn_classifier_predict_proba_s(da_handle handle, da_int n_queries,
+                                           da_int n_features, const float *X_test,
+                                           da_int ldx_test, float *proba,
+                                           da_nn_search_mode search_mode) {
+    DA_NN_PROLOGUE(float)
+    return nn->predict_proba(n_queries, n_features, X_test, ldx_test, proba, search_mode);
+}
+da_status da_nn_classifier_predict_d(da_handle handle, da_int n_queries, da_int n_features,
+                                     const double *X_test, da_int ldx_test, da_int *y_test,
+                                     da_nn_search_mode search_mode) {
+    DA_NN_PROLOGUE(double)
+    return nn->predict(n_queries, n_features, X_test, ldx_test, y_test, search_mode);
+}
+da_status da_nn_classifier_predict_s(da_handle handle, da_int n_queries, da_int n_features,
+                                     const float *X_test, da_int ldx_test, da_int *y_test,
+                                     da_nn_search_mode search_mode) {
+    DA_NN_PROLOGUE(float)
+    return nn->predict(n_queries, n_features, X_test, ldx_test, y_test, search_mode);
+}
+da_status da_nn_regressor_predict_d(da_handle handle, da_int n_queries, da_int n_features,
+                                    const double *X_test, da_int ldx_test, double *y_test,
+                                    da_nn_search_mode search_mode) {
+    DA_NN_PROLOGUE(double)
+    return nn->predict(n_queries, n_features, X_test, ldx_test, y_test, search_mode);
+}
+da_status da_nn_regressor_predict_s(da_handle handle, da_int n_queries, da_int n_features,
+                                    const float *X_test, da_int ldx_test, float *y_test,
+                                    da_nn_search_mode search_mode) {
+    DA_NN_PROLOGUE(float)
+    return nn->predict(n_queries, n_features, X_test, ldx_test, y_test, search_mode);
+}
+
+} // extern "C"

@@ -2,7 +2,9 @@
 #include "../core/factories.hpp"
 #include "../kernels/launch.hpp"
 #include "aoclda_metrics.h"
+#include "aoclda_nearest_neighbors.h"
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <vector>
 #include <limits>
@@ -751,6 +753,28 @@ da_status NNHandle<T>::search_tiles(SearchContext &ctx, const T *dQ, da_int nq, 
     return da_status_success;
 }
 
+// Copies n_queries x n_features queries (handle storage order, leading dimension ld) into
+// q_ as compact row-major rows.
+template <typename T>
+da_status NNHandle<T>::upload_queries(da_int n_queries, const T *X_test, da_int ldx_test) {
+    hipStream_t st = gpu->stream();
+    const da_int d = n_features_;
+    const size_t qbytes = (size_t)n_queries * d * sizeof(T);
+    DA_HIP(err, q_.reserve(qbytes));
+    if (order() == row_major) {
+        DA_HIP(err, copy_2d(q_.ptr(), d * sizeof(T), X_test, (size_t)ldx_test * sizeof(T),
+                            d * sizeof(T), n_queries, hipMemcpyHostToDevice, st));
+    } else {
+        DA_HIP(err, q_stage_.reserve(qbytes));
+        DA_HIP(err, copy_2d(q_stage_.ptr(), n_queries * sizeof(T), X_test,
+                            (size_t)ldx_test * sizeof(T), n_queries * sizeof(T), d,
+                            hipMemcpyHostToDevice, st));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), n_queries, n_queries, d,
+                                                   q_.as<T>(), st));
+    }
+    return da_status_success;
+}
+
 // Radius neighbours by brute force (nearest_neighbors.cpp:1918-2215): the squared-euclidean
 // forms compare squared distances with radius^2 and return square roots; the others compare
 // the distance itself. Keys are computed exactly (double direct differences) and rounded to
@@ -793,26 +817,33 @@ da_status NNHandle<T>::radius_neighbors(da_int n_queries, da_int n_features, con
         radius = opt_radius;
     rnn_done_ = false;
     model_trained = false;
-    rnn_count_.assign(n_queries, 0);
-    rnn_ind_.clear();
-    rnn_d_.clear();
+    s = upload_queries(n_queries, X_test, ldx_test);
+    if (s != da_status_success)
+        return s;
+    s = radius_core(q_.as<T>(), n_queries, radius, return_distance != 0, sort_results != 0,
+                    nullptr);
+    if (s != da_status_success)
+        return s;
+    rnn_dist_ = return_distance != 0;
+    rnn_done_ = true;
+    model_trained = true;
+    return da_status_success;
+}
 
+// The search itself, on row-major queries dQ in GPU memory. Without a sink the results go to
+// the handle's host arrays (counts, indices, distances); with one, each query block's lists
+// stay in GPU memory and are handed to the sink: offsets (rows + 1, from 0), indices and
+// distances (when want_dist) of queries q0 .. q0 + rows - 1.
+template <typename T>
+da_status NNHandle<T>::radius_core(const T *dQ, da_int n_queries, T radius, bool want_dist,
+                                   bool sort_results, const RadiusSink *sink) {
+    if (!sink) {
+        rnn_count_.assign(n_queries, 0);
+        rnn_ind_.clear();
+        rnn_d_.clear();
+    }
     hipStream_t st = gpu->stream();
     const da_int d = n_features_, n = n_samples_;
-    const size_t qbytes = (size_t)n_queries * d * sizeof(T);
-    DA_HIP(err, q_.reserve(qbytes));
-    if (order() == row_major) {
-        DA_HIP(err, copy_2d(q_.ptr(), d * sizeof(T), X_test, (size_t)ldx_test * sizeof(T),
-                            d * sizeof(T), n_queries, hipMemcpyHostToDevice, st));
-    } else {
-        DA_HIP(err, q_stage_.reserve(qbytes));
-        DA_HIP(err, copy_2d(q_stage_.ptr(), n_queries * sizeof(T), X_test,
-                            (size_t)ldx_test * sizeof(T), n_queries * sizeof(T), d,
-                            hipMemcpyHostToDevice, st));
-        DA_HIP(err, launch_colmajor_to_rowmajor<T>(q_stage_.as<T>(), n_queries, n_queries, d,
-                                                   q_.as<T>(), st));
-    }
-    const T *dQ = q_.as<T>();
     // Keys and thresholds per metric (set_params, nearest_neighbors.cpp:337-347).
     int km;
     bool squares = false, need_qsq = false, need_qcos = false;
@@ -899,24 +930,25 @@ da_status NNHandle<T>::radius_neighbors(da_int n_queries, da_int n_features, con
                 c += scnt[(size_t)r * nseg + g];
             }
             off[r + 1] = off[r] + c;
-            rnn_count_[q0 + r] = c;
+            if (!sink)
+                rnn_count_[q0 + r] = c;
         }
         const int total = off[rows];
-        if (total == 0)
+        if (total == 0 && !sink)
             continue;
         DA_HIP(err, hipMemcpyAsync(rnn_off_.ptr(), off.data(), (rows + 1) * sizeof(int),
                                    hipMemcpyHostToDevice, st));
         DA_HIP(err, hipMemcpyAsync(rnn_segoff_.ptr(), soff.data(),
                                    (size_t)rows * nseg * sizeof(int), hipMemcpyHostToDevice, st));
-        DA_HIP(err, rnn_ind_buf_.reserve((size_t)total * sizeof(int)));
-        DA_HIP(err, rnn_key_buf_.reserve((size_t)total * sizeof(T)));
+        DA_HIP(err, rnn_ind_buf_.reserve((size_t)std::max(total, 1) * sizeof(int)));
+        DA_HIP(err, rnn_key_buf_.reserve((size_t)std::max(total, 1) * sizeof(T)));
         DA_HIP(err, launch_radius_fill<T>(rnn_keys_.as<T>(), n, rows, n, thr, -1,
                                           rnn_segoff_.as<int>(),
                                           rnn_ind_buf_.as<int>(),
-                                          return_distance ? rnn_key_buf_.as<T>() : nullptr, st));
+                                          want_dist ? rnn_key_buf_.as<T>() : nullptr, st));
         int *ind = rnn_ind_buf_.as<int>();
         T *key = rnn_key_buf_.as<T>();
-        if (sort_results) {
+        if (sort_results && total > 0) {
             size_t bytes = 0;
             DA_HIP(err, radius_sort_bytes<T>(total, rows, &bytes));
             DA_HIP(err, rnn_tmp_.reserve(std::max<size_t>(bytes, 1)));
@@ -928,22 +960,422 @@ da_status NNHandle<T>::radius_neighbors(da_int n_queries, da_int n_features, con
             ind = rnn_ind2_.as<int>();
             key = rnn_key2_.as<T>();
         }
-        if (return_distance && squares)
+        if (want_dist && squares)
             DA_HIP(err, launch_radius_sqrt<T>(key, total, st));
+        if (sink) {
+            da_status ss = (*sink)(q0, rows, rnn_off_.as<int>(), ind, key);
+            if (ss != da_status_success)
+                return ss;
+            continue;
+        }
         const size_t base = rnn_ind_.size();
         rnn_ind_.resize(base + total);
         DA_HIP(err, hipMemcpyAsync(rnn_ind_.data() + base, ind, (size_t)total * sizeof(int),
                                    hipMemcpyDeviceToHost, st));
-        if (return_distance) {
+        if (want_dist) {
             rnn_d_.resize(base + total);
             DA_HIP(err, hipMemcpyAsync(rnn_d_.data() + base, key, (size_t)total * sizeof(T),
                                        hipMemcpyDeviceToHost, st));
         }
         DA_HIP(err, hipStreamSynchronize(st));
     }
-    rnn_dist_ = return_distance != 0;
-    rnn_done_ = true;
-    model_trained = true;
+    return da_status_success;
+}
+
+// ---- Classification and regression (nearest_neighbors.cpp:515-566, 1021-1910) ----
+
+template <typename T>
+da_status NNHandle<T>::set_labels(da_int n_samples, const da_int *y_train) {
+    if (trained_ && n_samples != n_samples_)
+        return err->error(da_status_invalid_array_dimension,
+                          "n_samples = " + std::to_string(n_samples) +
+                              " doesn't match the training data size " +
+                              std::to_string(n_samples_) + ".");
+    if (n_samples < 1)
+        return err->error(da_status_invalid_array_dimension,
+                          "The function was called with n_samples = " +
+                              std::to_string(n_samples) + ". Constraint: n_samples >= 1.");
+    if (y_train == nullptr)
+        return err->error(da_status_invalid_pointer, "The array y_train_class is null.");
+    y_class_.assign(y_train, y_train + n_samples);
+    has_labels_ = true;
+    classes_ready_ = false;
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::set_targets(da_int n_samples, const T *y_train) {
+    if (trained_ && n_samples != n_samples_)
+        return err->error(da_status_invalid_array_dimension,
+                          "n_samples = " + std::to_string(n_samples) +
+                              " doesn't match the training data size " +
+                              std::to_string(n_samples_) + ".");
+    if (n_samples < 1)
+        return err->error(da_status_invalid_array_dimension,
+                          "The function was called with n_samples = " +
+                              std::to_string(n_samples) + ". Constraint: n_samples >= 1.");
+    if (y_train == nullptr)
+        return err->error(da_status_invalid_pointer, "The array y_train_reg is null.");
+    y_reg_.assign(y_train, y_train + n_samples);
+    has_targets_ = true;
+    DA_HIP(err, tgt_.reserve((size_t)n_samples * sizeof(T)));
+    DA_HIP(err, hipMemcpyAsync(tgt_.ptr(), y_reg_.data(), (size_t)n_samples * sizeof(T),
+                               hipMemcpyHostToDevice, gpu->stream()));
+    DA_HIP(err, hipStreamSynchronize(gpu->stream()));
+    return da_status_success;
+}
+
+// Sorted distinct labels (available_classes, :1060-1086) and each training row's position
+// among them, in GPU memory.
+template <typename T> da_status NNHandle<T>::available_classes() {
+    if (!trained_)
+        return err->error(da_status_no_data, "No training data have been set. Please call "
+                                             "da_nn_set_data_s or da_nn_set_data_d.");
+    if (!has_labels_)
+        return err->error(da_status_no_data, "No classification labels have been set. Please "
+                                             "call da_nn_set_labels_s or da_nn_set_labels_d.");
+    if (classes_ready_)
+        return da_status_success;
+    if ((da_int)y_class_.size() != n_samples_)
+        return err->error(da_status_invalid_array_dimension,
+                          "The number of labels doesn't match the training data size " +
+                              std::to_string(n_samples_) + ".");
+    classes_ = y_class_;
+    std::sort(classes_.begin(), classes_.end());
+    classes_.erase(std::unique(classes_.begin(), classes_.end()), classes_.end());
+    std::vector<int> idx(n_samples_);
+    for (da_int i = 0; i < n_samples_; i++)
+        idx[i] = (int)(std::lower_bound(classes_.begin(), classes_.end(), y_class_[i]) -
+                       classes_.begin());
+    hipStream_t st = gpu->stream();
+    DA_HIP(err, cls_idx_.reserve((size_t)n_samples_ * sizeof(int)));
+    DA_HIP(err, classes_d_.reserve(classes_.size() * sizeof(int)));
+    DA_HIP(err, hipMemcpyAsync(cls_idx_.ptr(), idx.data(), (size_t)n_samples_ * sizeof(int),
+                               hipMemcpyHostToDevice, st));
+    DA_HIP(err, hipMemcpyAsync(classes_d_.ptr(), classes_.data(), classes_.size() * sizeof(int),
+                               hipMemcpyHostToDevice, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    classes_ready_ = true;
+    return da_status_success;
+}
+
+template <typename T> da_status NNHandle<T>::classes(da_int *n_classes, da_int *classes) {
+    // nearest_neighbors_public.hpp:79-98: n_classes <= 0 queries the count.
+    if (n_classes == nullptr)
+        return err->error(da_status_invalid_pointer, "n_classes is not a valid pointer.");
+    da_status s = available_classes();
+    if (s != da_status_success)
+        return s;
+    if (*n_classes <= 0) {
+        *n_classes = (da_int)classes_.size();
+        return da_status_success;
+    }
+    if (classes == nullptr)
+        return err->error(da_status_invalid_pointer, "classes is not a valid pointer.");
+    const da_int m = std::min<da_int>(*n_classes, (da_int)classes_.size());
+    std::copy(classes_.begin(), classes_.begin() + m, classes);
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::check_queries(da_int n_queries, da_int n_features, const T *X_test,
+                                     da_int ldx_test) {
+    if (X_test == nullptr)
+        return err->error(da_status_invalid_pointer, "X_test is not a valid pointer.");
+    if (n_features != n_features_)
+        return err->error(da_status_invalid_array_dimension,
+                          "n_features = " + std::to_string(n_features) +
+                              " doesn't match the expected value " +
+                              std::to_string(n_features_) + ".");
+    return check_2D_array<T>(order(), n_queries, n_features, X_test, ldx_test, "n_queries",
+                             "n_features", "X_test", "ldx_test");
+}
+
+// The most frequent label, the smallest at equal counts (most_frequent_element, :1035-1058).
+static da_int most_frequent_label(const std::vector<da_int> &y) {
+    std::map<da_int, da_int> freq;
+    for (da_int v : y)
+        freq[v]++;
+    da_int best = y[0], bc = 0;
+    for (const auto &f : freq)
+        if (f.second > bc) {
+            best = f.first;
+            bc = f.second;
+        }
+    return best;
+}
+
+template <typename T>
+da_status NNHandle<T>::classify(da_int nq, da_int search_mode, T *d_proba, int *d_pred) {
+    const int weights = (int)opts.skey("weights");
+    if (metric_ == da_inner_product && weights == 1)
+        return err->error(da_status_incompatible_options,
+                          "The inner product metric is not compatible with weights=\"distance\". "
+                          "Use weights=\"uniform\" instead.");
+    da_status s = available_classes();
+    if (s != da_status_success)
+        return s;
+    hipStream_t st = gpu->stream();
+    const int ncls = (int)classes_.size();
+    const int big = std::numeric_limits<int>::max();
+    DA_HIP(err, vote_flag_.reserve(sizeof(int)));
+    DA_HIP(err, hipMemcpyAsync(vote_flag_.ptr(), &big, sizeof(int), hipMemcpyHostToDevice, st));
+    int *flag = vote_flag_.as<int>();
+    const int *cls = cls_idx_.as<int>(), *cl = classes_d_.as<int>();
+    const int mode = (int)opts.skey("outlier handling");
+    int fill_idx = -1;
+    da_int fallback = 0;
+    if (search_mode == knn_search_mode) {
+        const da_int k = n_neighbors_;
+        if (k > n_samples_)
+            return err->error(da_status_invalid_input,
+                              "The number of neighbors (" + std::to_string(k) +
+                                  ") must not exceed the number of training samples.");
+        DA_HIP(err, out_ind_.reserve((size_t)nq * k * sizeof(da_int)));
+        DA_HIP(err, out_dist_.reserve((size_t)nq * k * sizeof(T)));
+        s = search(q_.as<T>(), nq, k, weights == 1, out_ind_.as<da_int>(),
+                   weights == 1 ? out_dist_.as<T>() : nullptr);
+        if (s != da_status_success)
+            return s;
+        KernelScope ks(*gpu, "nn_vote");
+        DA_HIP(err, launch_nn_classify<T>(nq, nullptr, k, out_ind_.as<int>(), out_dist_.as<T>(),
+                                          weights, cls, cl, ncls, 0, -1, 0, d_proba, d_pred, 0,
+                                          flag, st));
+    } else if (search_mode == radius_search_mode) {
+        if (metric_ == da_inner_product)
+            return err->error(da_status_incompatible_options,
+                              "The inner product metric is not compatible with radius "
+                              "neighbors.");
+        if (mode == 1) {
+            fallback = opts.ival("outlier label");
+            auto it = std::find(classes_.begin(), classes_.end(), fallback);
+            fill_idx = it == classes_.end() ? -1 : (int)(it - classes_.begin());
+        } else if (mode == 2) {
+            fallback = most_frequent_label(y_class_);
+            fill_idx = (int)(std::lower_bound(classes_.begin(), classes_.end(), fallback) -
+                             classes_.begin());
+        }
+        RadiusSink sink = [&](da_int q0, int rows, const int *off, const int *ind,
+                              const T *dist) -> da_status {
+            KernelScope ks(*gpu, "nn_vote");
+            DA_HIP(err, launch_nn_classify<T>(rows, off, 0, ind, dist, weights, cls, cl, ncls,
+                                              mode, fill_idx, (int)fallback,
+                                              d_proba + (size_t)q0 * ncls,
+                                              d_pred ? d_pred + q0 : nullptr, (int)q0, flag, st));
+            return da_status_success;
+        };
+        s = radius_core(q_.as<T>(), nq, (T)opts.rval("radius"), weights == 1, false, &sink);
+        if (s != da_status_success)
+            return s;
+    } else {
+        return err->error(da_status_invalid_input,
+                          "Unknown search mode: " + std::to_string(search_mode) + ".");
+    }
+    int first = big;
+    DA_HIP(err, hipMemcpyAsync(&first, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    if (first < nq && search_mode == radius_search_mode) {
+        if (mode == 0)
+            return err->error(da_status_operation_failed,
+                              "Failed to compute probabilities for query " +
+                                  std::to_string(first) +
+                                  " since it does not have any neighbors within the specified "
+                                  "radius.");
+        if (mode == 1 && fill_idx == -1)
+            return err->warn(da_status_outlier_warning,
+                             "The manually set outlier label is not present in the training "
+                             "labels. Setting probabilities to 0.0.");
+    }
+    return da_status_success;
+}
+
+template <typename T>
+da_status NNHandle<T>::predict_proba(da_int n_queries, da_int n_features, const T *X_test,
+                                     da_int ldx_test, T *proba, da_int search_mode) {
+    if (!trained_)
+        return err->error(da_status_no_data, "No training data have been set. Please call "
+                                             "da_nn_set_data_s or da_nn_set_data_d.");
+    if (!has_labels_)
+        return err->error(da_status_no_data, "No classification labels have been set. Please "
+                                             "call da_nn_set_labels_s or da_nn_set_labels_d.");
+    da_status s = check_options_update();
+    if (s != da_status_success)
+        return s;
+    if (proba == nullptr)
+        return err->error(da_status_invalid_pointer, "proba is not a valid pointer.");
+    if (n_queries < 1)
+        return err->error(da_status_invalid_array_dimension,
+                          "Number of queries must be greater than zero.");
+    if (n_features < 1)
+        return err->error(da_status_invalid_array_dimension,
+                          "Number of features must be greater than zero.");
+    s = check_queries(n_queries, n_features, X_test, ldx_test);
+    if (s != da_status_success)
+        return s;
+    s = available_classes();
+    if (s != da_status_success)
+        return s;
+    s = upload_queries(n_queries, X_test, ldx_test);
+    if (s != da_status_success)
+        return s;
+    const size_t ncls = classes_.size(), total = (size_t)n_queries * ncls;
+    DA_HIP(err, vote_out_.reserve(total * sizeof(T)));
+    s = classify(n_queries, search_mode, vote_out_.as<T>(), nullptr);
+    if (s != da_status_success && s != da_status_outlier_warning)
+        return s;
+    hipStream_t st = gpu->stream();
+    // n_queries x n_classes in the handle's storage order (:1150-1156).
+    if (order() == row_major) {
+        DA_HIP(err, hipMemcpyAsync(proba, vote_out_.ptr(), total * sizeof(T),
+                                   hipMemcpyDeviceToHost, st));
+    } else {
+        DA_HIP(err, q_stage_.reserve(total * sizeof(T)));
+        DA_HIP(err, launch_colmajor_to_rowmajor<T>(vote_out_.as<T>(), ncls, ncls, n_queries,
+                                                   q_stage_.as<T>(), st));
+        DA_HIP(err, hipMemcpyAsync(proba, q_stage_.ptr(), total * sizeof(T),
+                                   hipMemcpyDeviceToHost, st));
+    }
+    DA_HIP(err, hipStreamSynchronize(st));
+    return s;
+}
+
+template <typename T>
+da_status NNHandle<T>::predict(da_int n_queries, da_int n_features, const T *X_test,
+                               da_int ldx_test, da_int *y_test, da_int search_mode) {
+    da_status s = check_options_update();
+    if (s != da_status_success)
+        return s;
+    if (!trained_)
+        return err->error(da_status_no_data, "No training data have been set. Please call "
+                                             "da_nn_set_data_s or da_nn_set_data_d.");
+    if (!has_labels_)
+        return err->error(da_status_no_data, "No classification labels have been set. Please "
+                                             "call da_nn_set_labels_s or da_nn_set_labels_d.");
+    if (y_test == nullptr)
+        return err->error(da_status_invalid_pointer, "y_test is not a valid pointer.");
+    if (n_queries < 1)
+        return err->error(da_status_invalid_array_dimension, "n_queries must be greater than 0.");
+    s = check_queries(n_queries, n_features, X_test, ldx_test);
+    if (s != da_status_success)
+        return s;
+    s = available_classes();
+    if (s != da_status_success)
+        return s;
+    s = upload_queries(n_queries, X_test, ldx_test);
+    if (s != da_status_success)
+        return s;
+    DA_HIP(err, vote_out_.reserve((size_t)n_queries * classes_.size() * sizeof(T)));
+    DA_HIP(err, vote_pred_.reserve((size_t)n_queries * sizeof(int)));
+    s = classify(n_queries, search_mode, vote_out_.as<T>(), vote_pred_.as<int>());
+    if (s != da_status_success && s != da_status_outlier_warning)
+        return s;
+    DA_HIP(err, hipMemcpyAsync(y_test, vote_pred_.ptr(), (size_t)n_queries * sizeof(int),
+                               hipMemcpyDeviceToHost, gpu->stream()));
+    DA_HIP(err, hipStreamSynchronize(gpu->stream()));
+    return s;
+}
+
+template <typename T>
+da_status NNHandle<T>::predict(da_int n_queries, da_int n_features, const T *X_test,
+                               da_int ldx_test, T *y_test, da_int search_mode) {
+    if (!trained_)
+        return err->error(da_status_no_data, "No training data have been set. Please call "
+                                             "da_nn_set_data_s or da_nn_set_data_d.");
+    if (!has_targets_)
+        return err->error(da_status_no_data, "No regression targets have been set. Please call "
+                                             "da_nn_set_targets_s or da_nn_set_targets_d.");
+    da_status s = check_options_update();
+    if (s != da_status_success)
+        return s;
+    if (y_test == nullptr)
+        return err->error(da_status_invalid_pointer, "y_test is not a valid pointer.");
+    if (n_queries < 1)
+        return err->error(da_status_invalid_array_dimension, "n_queries must be greater than 0.");
+    s = check_queries(n_queries, n_features, X_test, ldx_test);
+    if (s != da_status_success)
+        return s;
+    if ((da_int)y_reg_.size() != n_samples_)
+        return err->error(da_status_invalid_array_dimension,
+                          "The number of targets doesn't match the training data size " +
+                              std::to_string(n_samples_) + ".");
+    const int weights = (int)opts.skey("weights");
+    if (metric_ == da_inner_product && weights == 1)
+        return err->error(da_status_incompatible_options,
+                          "The inner product metric is not compatible with weights=\"distance\". "
+                          "Use weights=\"uniform\" instead.");
+    s = upload_queries(n_queries, X_test, ldx_test);
+    if (s != da_status_success)
+        return s;
+    hipStream_t st = gpu->stream();
+    const int big = std::numeric_limits<int>::max();
+    DA_HIP(err, vote_flag_.reserve(sizeof(int)));
+    DA_HIP(err, hipMemcpyAsync(vote_flag_.ptr(), &big, sizeof(int), hipMemcpyHostToDevice, st));
+    DA_HIP(err, vote_out_.reserve((size_t)n_queries * sizeof(T)));
+    int *flag = vote_flag_.as<int>();
+    T *out = vote_out_.as<T>();
+    const int mode = (int)opts.skey("outlier handling");
+    T fallback = 0;
+    if (search_mode == knn_search_mode) {
+        const da_int k = n_neighbors_;
+        if (k > n_samples_)
+            return err->error(da_status_invalid_input,
+                              "The number of neighbors (" + std::to_string(k) +
+                                  ") must not exceed the number of training samples.");
+        DA_HIP(err, out_ind_.reserve((size_t)n_queries * k * sizeof(da_int)));
+        DA_HIP(err, out_dist_.reserve((size_t)n_queries * k * sizeof(T)));
+        s = search(q_.as<T>(), n_queries, k, weights == 1, out_ind_.as<da_int>(),
+                   weights == 1 ? out_dist_.as<T>() : nullptr);
+        if (s != da_status_success)
+            return s;
+        KernelScope ks(*gpu, "nn_vote");
+        DA_HIP(err, launch_nn_regress<T>(n_queries, nullptr, k, out_ind_.as<int>(),
+                                         out_dist_.as<T>(), weights, tgt_.as<T>(), 0, (T)0, out,
+                                         0, flag, st));
+    } else if (search_mode == radius_search_mode) {
+        if (metric_ == da_inner_product)
+            return err->error(da_status_incompatible_options,
+                              "The inner product metric is not compatible with radius "
+                              "neighbors.");
+        if (mode == 1) {
+            fallback = (T)opts.rval("outlier target");
+        } else if (mode == 2) {
+            // Mean of the training targets (basic_statistics mean, summed in T).
+            T sum = 0;
+            for (T v : y_reg_)
+                sum += v;
+            fallback = sum / (T)n_samples_;
+        }
+        RadiusSink sink = [&](da_int q0, int rows, const int *off, const int *ind,
+                              const T *dist) -> da_status {
+            KernelScope ks(*gpu, "nn_vote");
+            DA_HIP(err, launch_nn_regress<T>(rows, off, 0, ind, dist, weights, tgt_.as<T>(), mode,
+                                             fallback, out + q0, (int)q0, flag, st));
+            return da_status_success;
+        };
+        s = radius_core(q_.as<T>(), n_queries, (T)opts.rval("radius"), weights == 1, false,
+                        &sink);
+        if (s != da_status_success)
+            return s;
+    } else {
+        return err->error(da_status_invalid_input,
+                          "Unknown search mode: " + std::to_string(search_mode) + ".");
+    }
+    int first = big;
+    DA_HIP(err, hipMemcpyAsync(&first, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    if (first < n_queries && search_mode == radius_search_mode && mode == 0)
+        return err->error(da_status_operation_failed,
+                          "Failed to predict targets for query " + std::to_string(first) +
+                              " since it does not have any neighbors within the specified "
+                              "radius.");
+    DA_HIP(err, hipMemcpyAsync(y_test, out, (size_t)n_queries * sizeof(T),
+                               hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    if (first < n_queries && search_mode == radius_search_mode && mode == 1)
+        return err->warn(da_status_outlier_warning,
+                         "One or more samples have no neighbors within the specified radius. "
+                         "Setting the corresponding targets to " +
+                             std::to_string((double)fallback) + ".");
     return da_status_success;
 }
 

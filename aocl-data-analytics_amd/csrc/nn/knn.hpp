@@ -3,6 +3,7 @@
 // training set resident in GPU memory.
 #pragma once
 #include "../core/handle.hpp"
+#include <functional>
 #include <map>
 #include <vector>
 
@@ -48,6 +49,17 @@ template <typename T> class NNHandle : public AlgHandle {
     da_status radius_neighbors(da_int n_queries, da_int n_features, const T *X_test,
                                da_int ldx_test, T radius, da_int return_distance,
                                da_int sort_results);
+    // Classification labels and regression targets (nearest_neighbors.cpp:515-566), copied
+    // into GPU memory; k-NN or radius-neighbour prediction (:1094-1910).
+    da_status set_labels(da_int n_samples, const da_int *y_train);
+    da_status set_targets(da_int n_samples, const T *y_train);
+    da_status classes(da_int *n_classes, da_int *classes);
+    da_status predict_proba(da_int n_queries, da_int n_features, const T *X_test,
+                            da_int ldx_test, T *proba, da_int search_mode);
+    da_status predict(da_int n_queries, da_int n_features, const T *X_test, da_int ldx_test,
+                      da_int *y_test, da_int search_mode);
+    da_status predict(da_int n_queries, da_int n_features, const T *X_test, da_int ldx_test,
+                      T *y_test, da_int search_mode);
     using AlgHandle::get_result;
     da_status get_result(da_result query, da_int *dim, da_int *result) override;
     da_status get_result(da_result query, da_int *dim, T *result) override;
@@ -73,6 +85,17 @@ template <typename T> class NNHandle : public AlgHandle {
     da_int last_fallback = 0, last_candidates = 0;
 
   private:
+    using RadiusSink =
+        std::function<da_status(da_int q0, int rows, const int *off, const int *ind, const T *dist)>;
+    da_status upload_queries(da_int n_queries, const T *X_test, da_int ldx_test);
+    da_status radius_core(const T *dQ, da_int n_queries, T radius, bool want_dist,
+                          bool sort_results, const RadiusSink *sink);
+    da_status check_queries(da_int n_queries, da_int n_features, const T *X_test,
+                            da_int ldx_test);
+    da_status available_classes();
+    // Class probabilities (row per query) and, when d_pred is set, labels of the queries in
+    // q_, by k-NN or radius search; reports outliers as the reference does.
+    da_status classify(da_int n_queries, da_int search_mode, T *d_proba, int *d_pred);
     da_status set_params();
     da_status check_options_update();
     da_status resolve_k(da_int &k);
@@ -87,6 +110,11 @@ template <typename T> class NNHandle : public AlgHandle {
     da_status prepare_fused(int prec);
 
     bool trained_ = false;
+    // Labels / targets: host copies, class list, per-row class index in GPU memory.
+    bool has_labels_ = false, has_targets_ = false, classes_ready_ = false;
+    std::vector<da_int> y_class_, classes_;
+    std::vector<T> y_reg_;
+    DeviceBuffer cls_idx_, classes_d_, tgt_, vote_out_, vote_pred_, vote_flag_;
     da_int n_samples_ = 0, n_features_ = 0;
     // Radius results: per-query counts, neighbours in query order, their distances.
     bool rnn_done_ = false, rnn_dist_ = false;

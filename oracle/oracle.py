@@ -109,6 +109,97 @@ def dbscan(X, eps, min_samples, metric="euclidean", p=2.0, single=False):
     return labels, core.astype(bool), nc
 
 
+def _lists(Xt, Xq, k, r, metric, p, single):
+    """Neighbour lists (indices, distances) per query: the k nearest by (distance, index), or
+    those within r in ascending index, with distances rounded to float when single."""
+    if r is None:
+        ind, dist = knn(Xt, Xq, k, metric, p)
+        if single:
+            dist = dist.astype(np.float32)
+        return [(ind[q], dist[q]) for q in range(len(Xq))]
+    counts, ind, dist = radius(Xt, Xq, r, metric, p, single=single)
+    b = np.concatenate(([0], np.cumsum(counts)))
+    return [(ind[b[q]:b[q + 1]], dist[b[q]:b[q + 1]]) for q in range(len(Xq))]
+
+
+def _weights(d, weights, T):
+    # get_weights (nearest_neighbors.cpp:1021-1033): 1, or 1/d with d <= eps(T) counted as 1
+    if weights == "uniform":
+        return [T(1)] * len(d)
+    eps = np.finfo(T).eps
+    return [T(1) if T(v) <= eps else T(1) / T(v) for v in d]
+
+
+def nn_classify(Xt, y, Xq, k=None, r=None, metric="euclidean", p=2.0, weights="uniform",
+                outlier="none", outlier_label=0, single=False):
+    """(proba, labels, first query without neighbours or -1) of the k-NN / radius classifier
+    restated from nearest_neighbors.cpp:1166-1600: per class the weights of the neighbours in
+    list order, normalised by the sum of the class totals; the label of the first largest
+    probability; radius queries without neighbours handled by outlier ('none', 'manual',
+    'most_frequent'). Arithmetic in float when single."""
+    T = np.float32 if single else np.float64
+    y = np.asarray(y)
+    classes = np.unique(y)
+    pos = {c: i for i, c in enumerate(classes)}
+    vals, cnts = np.unique(y, return_counts=True)
+    mf = vals[np.argmax(cnts)]  # smallest label among the most frequent
+    proba = np.zeros((len(Xq), len(classes)), dtype=T)
+    labels = np.zeros(len(Xq), dtype=np.int64)
+    first = -1
+    for q, (ind, d) in enumerate(_lists(Xt, Xq, k, r, metric, p, single)):
+        row = [T(0)] * len(classes)
+        if len(ind) == 0:
+            first = q if first < 0 else first
+            fill = pos.get(outlier_label, -1) if outlier == "manual" else (
+                pos[mf] if outlier == "most_frequent" else -1)
+            if fill >= 0:
+                row[fill] = T(1)
+        else:
+            w = _weights(d, weights, T)
+            for j, wj in zip(ind, w):
+                row[pos[y[j]]] = T(row[pos[y[j]]] + wj)
+            den = T(0)
+            for v in row:
+                den = T(den + v)
+            row = [T(v / den) for v in row]
+        proba[q] = row
+        best = int(np.argmax(proba[q]))
+        if proba[q, best] == 0:
+            labels[q] = outlier_label if outlier == "manual" else mf
+        else:
+            labels[q] = classes[best]
+    return proba, labels, first
+
+
+def nn_regress(Xt, t, Xq, k=None, r=None, metric="euclidean", p=2.0, weights="uniform",
+               outlier="none", outlier_target=0.0, single=False):
+    """(targets, first query without neighbours or -1) of the k-NN / radius regressor
+    restated from nearest_neighbors.cpp:1610-1910: the mean of the neighbours' targets, or
+    sum(t w) / sum(w); radius outliers get outlier_target ('manual') or the mean target
+    ('most_frequent')."""
+    T = np.float32 if single else np.float64
+    t = np.asarray(t, dtype=T)
+    out = np.zeros(len(Xq), dtype=T)
+    first = -1
+    mean = T(0)
+    for v in t:
+        mean = T(mean + v)
+    mean = T(mean / T(len(t)))
+    for q, (ind, d) in enumerate(_lists(Xt, Xq, k, r, metric, p, single)):
+        if len(ind) == 0:
+            first = q if first < 0 else first
+            out[q] = T(outlier_target) if outlier == "manual" else (
+                mean if outlier == "most_frequent" else T(0))
+            continue
+        w = _weights(d, weights, T)
+        num, den = T(0), T(0)
+        for j, wj in zip(ind, w):
+            num = T(num + T(t[j] * wj))
+            den = T(den + wj)
+        out[q] = T(num / den) if weights == "distance" else T(num / T(len(ind)))
+    return out, first
+
+
 INIT = {"random": 0, "k-means++": 1, "supplied": 2, "random partitions": 3}
 EMPTY = {"ignore": 0, "error": 1, "split": 2}
 

@@ -273,6 +273,72 @@ for metric, alg, power, eps in (("euclidean", "brute", 2.0, 0.5), ("manhattan", 
         _db(f"30 by 1, {metric}, {alg}, p={power}", "133-164,207-246", 30, 1,
             _three_blobs(30, 1, 12), 30, "column-major", 3, eps, _ABC, list(range(30)),
             [30, 1, 30, eps, 3, 30, power, 30, 3], 3, algorithm=alg, metric=metric, power=power))
+# k-NN / radius classification and regression of the 6 x 3 example
+# (nearest_neighbors_tests.hpp:164-427, data :446-466): proba column-major n_queries x
+# n_classes; "euclidean" stands for euclidean, l2, minkowski and euclidean_gemm,
+# "sqeuclidean" for sqeuclidean and sqeuclidean_gemm; radius searches use the "radius"
+# option 21.
+known["knn_predict"] = {
+    "source": "tests/unit_tests/nearest_neighbors/nearest_neighbors_tests.hpp:164-466",
+    "X_train_colmajor": [-1., -2., -3., 1., 2., 3., -1., -1., -2., 3., 5., -1., 2., 3., -1., 1.,
+                         1., 2.],
+    "X_test_colmajor": [-2., -1., 2., 2., -2., 1., 3., -1., -3.],
+    "y_class": [1, 2, 0, 1, 2, 2], "y_reg": [1.5, 2.2, 0.5, 1, 2.8, 5.2], "radius": 21.0,
+    "proba_knn": {
+        "5": {"uniform": {"euclidean": [0.2, 0.2, 0.2, 0.4, 0.4, 0.4, 0.4, 0.4, 0.4],
+                          "sqeuclidean": [0.2, 0.2, 0.2, 0.4, 0.4, 0.4, 0.4, 0.4, 0.4]},
+              "distance": {
+                  "euclidean": [0.1379511568268668, 0.3515868265794006, 0.1798440493222374,
+                                0.4507346784224799, 0.3447698547319956, 0.4217676420329797,
+                                0.4113141647506533, 0.3036433186886039, 0.3983883086447829],
+                  "sqeuclidean": [0.0895917616770872, 0.5270701941190925, 0.1596502898413,
+                                  0.4799558661272527, 0.2747153739044967, 0.4485412905065095,
+                                  0.4304523721956602, 0.1982144319764108, 0.3918084196521904]}},
+        "3": {"uniform": {"euclidean": [0., 1 / 3, 0., 2 / 3, 1 / 3, 1 / 3, 1 / 3, 1 / 3, 2 / 3],
+                          "sqeuclidean": [0., 1 / 3, 0., 2 / 3, 1 / 3, 1 / 3, 1 / 3, 1 / 3,
+                                          2 / 3]},
+              "distance": {
+                  "euclidean": [0., 0.47531678671182, 0., 0.6304942401901729, 0.3006167312243614,
+                                0.3778214838715254, 0.3695057598098272, 0.2240664820638185,
+                                0.6221785161284746],
+                  "sqeuclidean": [0., 0.6164383561643836, 0., 0.5936675461741425,
+                                  0.2465753424657534, 0.4244031830238727, 0.4063324538258575,
+                                  0.136986301369863, 0.5755968169761273]}},
+    },
+    "labels_knn": {"5": {"uniform": [1, 1, 1], "distance": [1, 0, 1]},
+                   "3": {"uniform": [1, 0, 2], "distance": [1, 0, 2]}},
+    "targets_knn": {
+        "5": {"uniform": {"euclidean": [1.6, 2.08, 2.2], "sqeuclidean": [1.6, 2.08, 2.2]},
+              "distance": {"euclidean": [1.6323653119203947, 1.7134711834322838,
+                                         2.202877674352422],
+                           "sqeuclidean": [1.6673041559396837, 1.3229994266524698,
+                                           2.190592036491495]}},
+        "3": {"uniform": {"euclidean": [1.5666666666666667, 1.4000000000000001, 3.0],
+                          "sqeuclidean": [1.5666666666666667, 1.4000000000000001, 3.0]},
+              "distance": {"euclidean": [1.6105221794760263, 1.1815297507328528,
+                                         2.8785808475618806],
+                           "sqeuclidean": [1.6538258575197888, 0.9794520547945206,
+                                           2.749071618037135]}},
+    },
+    "proba_radius": {
+        "uniform": {"euclidean": [1 / 6, 1 / 6, 1 / 6, 1 / 3, 1 / 3, 1 / 3, 0.5, 0.5, 0.5],
+                    "sqeuclidean": [0.0, 1 / 3, 0.0, 2 / 3, 1 / 3, 1.0, 1 / 3, 1 / 3, 0.0]},
+        "distance": {
+            "euclidean": [0.12165522957880101, 0.32276306991001125, 0.1566384758301224,
+                          0.3974901845254738, 0.31650496637875825, 0.3673462694566996,
+                          0.4808545858957253, 0.3607319637112305, 0.47601525471317796],
+            "sqeuclidean": [0.0, 0.6164383561643836, 0.0, 0.5936675461741425,
+                            0.24657534246575344, 1.0, 0.40633245382585753, 0.136986301369863,
+                            0.0]},
+    },
+    "labels_radius": {"euclidean": [2, 2, 2], "sqeuclidean": [1, 0, 1]},
+    "targets_radius": {
+        "uniform": {"euclidean": [2.1999999999999997, 2.1999999999999997, 2.1999999999999997],
+                    "sqeuclidean": [1.5666666666666667, 1.4000000000000001, 1.0]},
+        "distance": {"euclidean": [2.053803715641692, 1.802546882108823, 2.2025063632975206],
+                     "sqeuclidean": [1.6538258575197888, 0.9794520547945206, 1.0]},
+    },
+}
 known["dbscan_errors"] = {
     "source": "tests/unit_tests/dbscan/dbscan_public.cpp:309-395",
     "A_colmajor": [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12], "n_samples": 4, "n_features": 3,

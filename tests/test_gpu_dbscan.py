@@ -116,11 +116,11 @@ def test_vs_oracle(oracle, metric, eps, p, dtype, order):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_row_blocks_and_long_chains(oracle, dtype):
-    """12,000 samples take several 256 MB key blocks and a neighbour list past its first
+    """13,000 samples take two 1 GiB key blocks in fp64 and a neighbour list past its first
     allocation; a chain of points spaced just under eps forms one cluster through thousands
     of links."""
     rng = np.random.default_rng(17)
-    X = blobs(rng, 10000, 3, 5, 0.35, 500)
+    X = blobs(rng, 11000, 3, 5, 0.35, 500)
     chain = np.zeros((1500, 3))
     chain[:, 0] = np.arange(1500) * 0.29 - 200.0
     X = np.vstack([X, chain]).astype(dtype)

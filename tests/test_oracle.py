@@ -260,3 +260,81 @@ def test_dbscan_vs_sklearn(oracle):
     np.testing.assert_array_equal(oracle.dbscan(X, 0.81, 5, "sqeuclidean")[0], sk.labels_)
     np.testing.assert_array_equal(oracle.dbscan(X, 0.9, 5, "l2")[0], sk.labels_)
     np.testing.assert_array_equal(oracle.dbscan(X, 0.81, 5, "minkowski", 2.0)[0], sk.labels_)
+
+
+def _knn_predict_data(c):
+    Xt = np.array(c["X_train_colmajor"]).reshape(3, 6).T
+    Xq = np.array(c["X_test_colmajor"]).reshape(3, 3).T
+    return Xt, Xq
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_nn_predict_known_answers(known, oracle, single):
+    """The reference's k-NN and radius classification / regression answers on its 6 x 3
+    example (k 3 and 5, uniform and 1/d weights, euclidean and squared euclidean, radius 21)."""
+    c = known["knn_predict"]
+    Xt, Xq = _knn_predict_data(c)
+    tol = 1e-6 if single else 1e-12
+    for metric in ("euclidean", "sqeuclidean"):
+        for w in ("uniform", "distance"):
+            for k in ("3", "5"):
+                proba, labels, _ = oracle.nn_classify(Xt, c["y_class"], Xq, k=int(k), metric=metric,
+                                                      weights=w, single=single)
+                exp = np.array(c["proba_knn"][k][w][metric]).reshape(3, 3).T
+                np.testing.assert_allclose(proba, exp, rtol=tol, atol=tol)
+                assert list(labels) == c["labels_knn"][k][w]
+                t, _ = oracle.nn_regress(Xt, c["y_reg"], Xq, k=int(k), metric=metric, weights=w,
+                                         single=single)
+                np.testing.assert_allclose(t, c["targets_knn"][k][w][metric], rtol=tol)
+            proba, labels, first = oracle.nn_classify(Xt, c["y_class"], Xq, r=c["radius"],
+                                                      metric=metric, weights=w, single=single)
+            exp = np.array(c["proba_radius"][w][metric]).reshape(3, 3).T
+            np.testing.assert_allclose(proba, exp, rtol=tol, atol=tol)
+            assert list(labels) == c["labels_radius"][metric] and first == -1
+            t, _ = oracle.nn_regress(Xt, c["y_reg"], Xq, r=c["radius"], metric=metric, weights=w,
+                                     single=single)
+            np.testing.assert_allclose(t, c["targets_radius"][w][metric], rtol=tol)
+
+
+def test_nn_predict_vs_sklearn(oracle):
+    """k-NN and radius classifiers / regressors against scikit-learn's brute-force estimators
+    (no query at distance 0, where scikit-learn's 1/d weighting differs by design)."""
+    from sklearn.neighbors import (KNeighborsClassifier, KNeighborsRegressor,
+                                   RadiusNeighborsClassifier, RadiusNeighborsRegressor)
+    rng = np.random.default_rng(31)
+    Xt = rng.standard_normal((500, 5))
+    Xq = rng.standard_normal((80, 5))
+    y = rng.choice([-3, 7, 10, 42], 500)
+    t = rng.standard_normal(500)
+    for w in ("uniform", "distance"):
+        for metric in ("euclidean", "manhattan"):
+            for k in (1, 6, 25):
+                sk = KNeighborsClassifier(k, weights=w, metric=metric, algorithm="brute").fit(Xt, y)
+                proba, labels, _ = oracle.nn_classify(Xt, y, Xq, k=k, metric=metric, weights=w)
+                np.testing.assert_allclose(proba, sk.predict_proba(Xq), rtol=1e-12, atol=1e-12)
+                np.testing.assert_array_equal(labels, sk.predict(Xq))
+                skr = KNeighborsRegressor(k, weights=w, metric=metric, algorithm="brute").fit(Xt, t)
+                np.testing.assert_allclose(oracle.nn_regress(Xt, t, Xq, k=k, metric=metric,
+                                                             weights=w)[0], skr.predict(Xq),
+                                           rtol=1e-12, atol=1e-12)
+            r = 2.2 if metric == "euclidean" else 4.0
+            sk = RadiusNeighborsClassifier(r, weights=w, metric=metric, algorithm="brute",
+                                           outlier_label=None).fit(Xt, y)
+            proba, labels, first = oracle.nn_classify(Xt, y, Xq, r=r, metric=metric, weights=w)
+            assert first == -1
+            np.testing.assert_allclose(proba, sk.predict_proba(Xq), rtol=1e-12, atol=1e-12)
+            np.testing.assert_array_equal(labels, sk.predict(Xq))
+            skr = RadiusNeighborsRegressor(r, weights=w, metric=metric, algorithm="brute").fit(Xt, t)
+            np.testing.assert_allclose(oracle.nn_regress(Xt, t, Xq, r=r, metric=metric,
+                                                         weights=w)[0], skr.predict(Xq),
+                                       rtol=1e-12, atol=1e-12)
+    # outliers: the manual label / target, or the most frequent label / mean target
+    Xfar = np.vstack([Xq[:3], np.full((2, 5), 50.0)])
+    proba, labels, first = oracle.nn_classify(Xt, y, Xfar, r=2.2, outlier="manual",
+                                              outlier_label=99)
+    assert first == 3 and list(labels[3:]) == [99, 99] and not proba[3:].any()
+    vals, cnt = np.unique(y, return_counts=True)
+    proba, labels, _ = oracle.nn_classify(Xt, y, Xfar, r=2.2, outlier="most_frequent")
+    assert list(labels[3:]) == [vals[np.argmax(cnt)]] * 2
+    tv, first = oracle.nn_regress(Xt, t, Xfar, r=2.2, outlier="most_frequent")
+    assert first == 3 and np.allclose(tv[3:], t.mean())
