@@ -39,12 +39,13 @@ CODE = {v: k for k, v in STATUS.items()}
 
 # Enums.
 ROW_MAJOR, COLUMN_MAJOR = 0, 1
-HANDLE_KMEANS, HANDLE_DBSCAN, HANDLE_NN = 3, 4, 8
+HANDLE_KMEANS, HANDLE_DBSCAN, HANDLE_NN, HANDLE_APPROX_NN = 3, 4, 8, 11
 METRICS = {"euclidean": 0, "l2": 0, "sqeuclidean": 1, "minkowski": 2, "manhattan": 3,
            "l1": 3, "cityblock": 3, "cosine": 4, "euclidean_gemm": 5,
            "sqeuclidean_gemm": 6, "inner_product": 7}
 RINFO, TRAINED, KMEANS_CENTRES, KMEANS_LABELS = 1, 2, 501, 502
 DBSCAN_LABELS, DBSCAN_N_CLUSTERS, DBSCAN_N_CORE, DBSCAN_CORE_INDICES = 503, 504, 505, 506
+ANN_CENTROIDS, ANN_LIST_SIZES = 801, 802
 (RNN_COUNT, RNN_OFFSETS, RNN_INDICES, RNN_DISTANCES, RNN_INDICES_INDEX,
  RNN_DISTANCES_INDEX) = range(601, 607)
 
@@ -74,6 +75,13 @@ for _t, _ct in (("d", C.c_double), ("s", C.c_float)):
     for _f in ("classifier_predict_proba", "classifier_predict", "regressor_predict"):
         _proto(f"da_nn_{_f}_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int, C.c_void_p,
                C.c_int)
+    _proto(f"da_approx_nn_set_training_data_{_t}", S, da_handle, da_int, da_int, C.c_void_p,
+           da_int)
+    _proto(f"da_approx_nn_train_{_t}", S, da_handle)
+    _proto(f"da_approx_nn_train_and_add_{_t}", S, da_handle)
+    _proto(f"da_approx_nn_add_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int)
+    _proto(f"da_approx_nn_kneighbors_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int,
+           C.c_void_p, C.c_void_p, da_int, da_int)
     _proto(f"da_dbscan_set_data_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int)
     _proto(f"da_dbscan_compute_{_t}", S, da_handle)
     _proto(f"da_kmeans_set_data_{_t}", S, da_handle, da_int, da_int, C.c_void_p, da_int)

@@ -1,5 +1,6 @@
-"""k-nearest neighbours (python_interface/python_package/aoclda/neighbors.py:39-233 of the
-reference): brute force on the GPU with the training set resident in GPU memory."""
+"""k-nearest neighbours and IVF-Flat approximate neighbours
+(python_interface/python_package/aoclda/neighbors.py:39-780 of the reference): brute force
+and probed inverted lists on the GPU, with the indexed vectors resident in GPU memory."""
 import ctypes as C
 
 import numpy as np
@@ -249,3 +250,99 @@ class nn_server:
             self.close()
         except Exception:
             pass
+
+
+class approximate_neighbors:
+    """IVF-Flat approximate nearest neighbours (neighbors.py:578-780 of the reference):
+    k-means centroids, inverted lists and probed-list search on the GPU."""
+
+    def __init__(self, n_neighbors=5, algorithm="ivfflat", metric="sqeuclidean", n_list=1,
+                 n_probe=1, kmeans_iter=10, train_fraction=1.0, seed=0, check_data=False):
+        self.n_neighbors, self.algorithm, self.metric = n_neighbors, algorithm, metric
+        self.n_list, self._n_probe, self.kmeans_iter = n_list, n_probe, kmeans_iter
+        self.train_fraction, self.seed, self.check_data = train_fraction, seed, check_data
+        self._h = None
+        self.order = None
+        self.dtype = None
+
+    @property
+    def n_probe(self):
+        return self._n_probe
+
+    @n_probe.setter
+    def n_probe(self, value):
+        self._n_probe = value
+        if self._h is not None:
+            self._h.set_int("n_probe", value)
+
+    def _prepare(self, X):
+        X, self.order, self.dtype = L.prepare(X)
+        h = L.Handle(L.HANDLE_APPROX_NN, self.dtype)
+        h.set_int("number of neighbors", self.n_neighbors)
+        h.set_string("algorithm", self.algorithm)
+        h.set_string("metric", self.metric.replace("_", " "))
+        h.set_int("n_list", self.n_list)
+        h.set_int("n_probe", self._n_probe)
+        h.set_int("k-means_iter", self.kmeans_iter)
+        h.set_int("seed", self.seed)
+        h.set_real("train fraction", self.train_fraction)
+        h.set_string("check data", "yes" if self.check_data else "no")
+        h.set_string("storage order", "column-major" if self.order == L.COLUMN_MAJOR else "row-major")
+        n, d = X.shape
+        self._X = X  # the C API keeps the training pointer until training
+        h.call("da_approx_nn_set_training_data", n, d, L.ptr(X), n if self.order == L.COLUMN_MAJOR else d)
+        self._h = h
+        return X
+
+    def train(self, X_train):
+        self._prepare(X_train)
+        self._h.call("da_approx_nn_train")
+        return self
+
+    def train_and_add(self, X_train):
+        self._prepare(X_train)
+        self._h.call("da_approx_nn_train_and_add")
+        return self
+
+    def add(self, X_add):
+        X, _, _ = L.prepare(X_add, dtype=self.dtype, order=self.order)
+        n, d = X.shape
+        self._h.call("da_approx_nn_add", n, d, L.ptr(X), n if self.order == L.COLUMN_MAJOR else d)
+        return self
+
+    def kneighbors(self, X_test, n_neighbors=0, return_distance=True):
+        X, _, _ = L.prepare(X_test, dtype=self.dtype, order=self.order)
+        nq, d = X.shape
+        k = n_neighbors if n_neighbors > 0 else self.n_neighbors
+        fo = "F" if self.order == L.COLUMN_MAJOR else "C"
+        ind = np.empty((nq, k), dtype=np.int32, order=fo)
+        dist = np.empty((nq, k), dtype=self.dtype, order=fo)
+        self._h.call("da_approx_nn_kneighbors", nq, d, L.ptr(X), nq if fo == "F" else d,
+                     L.ptr(ind), L.ptr(dist) if return_distance else None, k,
+                     1 if return_distance else 0)
+        return (dist, ind) if return_distance else ind
+
+    @property
+    def cluster_centroids(self):
+        d = self._X.shape[1]
+        r = np.zeros(self.n_list * d, dtype=self.dtype)
+        dim = L.da_int(r.size)
+        self._h.call("da_handle_get_result", L.ANN_CENTROIDS, C.byref(dim), L.ptr(r))
+        if self.order == L.COLUMN_MAJOR:
+            return r.reshape((self.n_list, d), order="F")
+        return r.reshape((self.n_list, d))
+
+    @property
+    def list_sizes(self):
+        r = np.zeros(self.n_list, dtype=np.int32)
+        dim = L.da_int(self.n_list)
+        L.check(L.lib.da_handle_get_result_int(self._h.h, L.ANN_LIST_SIZES, C.byref(dim), L.ptr(r)),
+                self._h.h)
+        return r
+
+    @property
+    def rinfo(self):
+        r = np.zeros(4, dtype=self.dtype)
+        dim = L.da_int(4)
+        self._h.call("da_handle_get_result", L.RINFO, C.byref(dim), L.ptr(r))
+        return r

@@ -35,11 +35,14 @@ template <typename T> static da_status handle_init(da_handle *handle, da_handle_
         case da_handle_dbscan:
             h->alg = make_dbscan_handle(h->precision, h->err, h->gpu);
             break;
+        case da_handle_approx_nn:
+            h->alg = make_approx_nn_handle(h->precision, h->err, h->gpu);
+            break;
         default:
             return h->err.error(da_status_not_implemented,
                                 "This handle type is not implemented by the MI355X build: "
-                                "only da_handle_nn, da_handle_kmeans and da_handle_dbscan are "
-                                "available.");
+                                "only da_handle_nn, da_handle_kmeans, da_handle_dbscan and "
+                                "da_handle_approx_nn are available.");
         }
     } catch (std::bad_alloc &) {
         return h->err.error(da_status_memory_error, "Memory allocation error");

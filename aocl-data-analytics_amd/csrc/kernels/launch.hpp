@@ -209,6 +209,23 @@ hipError_t launch_nn_regress(int nq, const int *off, int k, const int *ind, cons
                              int weights, const T *tgt, int outlier_mode, T fallback, T *out,
                              int qbase, int *first_empty, hipStream_t s);
 
+// ---- IVF-Flat search ----
+// Scans, for each (query q < nq, probe p < n_probe) pair, the inverted list probes[q*n_probe+p]
+// (vectors X[perm[r]] for r in [list_off[c], list_off[c+1])) and writes the exact squared
+// distances (rounded to T) and global ids to keys/idx[q*ld + seg[q*n_probe+p] + r - b].
+template <typename T>
+hipError_t launch_ivf_scan(const T *Q, int nq, int n_probe, int d, const T *X, const int *perm,
+                           const int *list_off, const int *probes, const int *seg, int64_t ld,
+                           T *keys, int *idx, hipStream_t s);
+// keys = +inf, idx = INT_MAX over n entries (candidate padding).
+template <typename T> hipError_t launch_ivf_pad(T *keys, int *idx, int64_t n, hipStream_t s);
+// Selected (key, column) pairs of n = rows x k -> (distance, index): the index is
+// cand_idx[(t / k) * ld + column]; INT_MAX there becomes -1 with distance +inf; square roots
+// when take_sqrt. dist may be null.
+template <typename T>
+hipError_t launch_ivf_finish(const T *key, const int *sel, const int *cand_idx, int64_t ld, int k,
+                             int64_t n, int take_sqrt, T *dist, int *ind, hipStream_t s);
+
 // ---- DBSCAN over a neighbour graph in CSR form (offs: n + 1 64-bit offsets, ind) ----
 // parent[i] = i.
 hipError_t launch_dbscan_init(int *parent, int n, hipStream_t s);

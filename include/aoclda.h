@@ -6,6 +6,7 @@
 #ifndef AOCLDA_H
 #define AOCLDA_H
 
+#include "aoclda_approximate_neighbors.h"
 #include "aoclda_dbscan.h"
 #include "aoclda_error.h"
 #include "aoclda_handle.h"

@@ -1,8 +1,8 @@
 /*
  * Opaque handle of the AOCL-DA C API. Enum values follow
  * source/include/aoclda_handle.h:45-88 and the functions :118-150 of the reference.
- * Only da_handle_nn, da_handle_kmeans and da_handle_dbscan are backed by an implementation in this
- * library; initialising any other type returns da_status_not_implemented.
+ * Only da_handle_nn, da_handle_kmeans, da_handle_dbscan and da_handle_approx_nn are backed
+ * by an implementation in this library; initialising any other type returns da_status_not_implemented.
  */
 #ifndef AOCLDA_HANDLE
 #define AOCLDA_HANDLE

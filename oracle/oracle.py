@@ -200,6 +200,40 @@ def nn_regress(Xt, t, Xq, k=None, r=None, metric="euclidean", p=2.0, weights="un
     return out, first
 
 
+def ivf_search(C, X, Xq, k, n_probe, metric="sqeuclidean", single=False):
+    """IVF-Flat add and search restated (approximate_neighbors.cpp:565-842 add, :944-1315
+    search) for given centroids C: every row of X (in order of addition) joins the list of
+    its nearest centroid (ties to the lower centroid); a query scans the n_probe lists of its
+    nearest centroids (by (distance, centroid)) in list order and keeps its k nearest
+    candidates, equal distances resolved for the lower list and the earlier position in it as
+    the reference's first-seen heap does; padded with index -1 and distance +inf; euclidean
+    distances are square-rooted. Exact squared
+    distances, rounded to float when single. Returns (ind, dist, list_sizes)."""
+    T = np.float32 if single else np.float64
+    C = np.asarray(C, dtype=np.float64)
+    X = np.asarray(X, dtype=np.float64)
+    Q = np.asarray(Xq, dtype=np.float64)
+
+    def sq(A, B):
+        return ((A[:, None, :] - B[None, :, :]) ** 2).sum(-1).astype(T)
+
+    lab = np.argmin(sq(X, C), axis=1)
+    sizes = np.bincount(lab, minlength=len(C))
+    cd = sq(Q, C)
+    ind = np.full((len(Q), k), -1, dtype=np.int64)
+    dist = np.full((len(Q), k), np.inf, dtype=T)
+    for q in range(len(Q)):
+        probes = sorted(range(len(C)), key=lambda c: (cd[q, c], c))[:n_probe]
+        cand = np.nonzero(np.isin(lab, probes))[0]
+        dq = sq(Q[q:q + 1], X[cand])[0]
+        order = sorted(range(len(cand)), key=lambda t: (dq[t], lab[cand[t]], cand[t]))[:k]
+        ind[q, :len(order)] = cand[order]
+        dist[q, :len(order)] = dq[order]
+    if metric == "euclidean":
+        dist = np.where(np.isfinite(dist), np.sqrt(dist), dist).astype(T)
+    return ind, dist, sizes
+
+
 INIT = {"random": 0, "k-means++": 1, "supplied": 2, "random partitions": 3}
 EMPTY = {"ignore": 0, "error": 1, "split": 2}
 

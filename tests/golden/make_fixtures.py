@@ -339,6 +339,27 @@ known["knn_predict"] = {
                      "sqeuclidean": [1.6538258575197888, 0.9794520547945206, 1.0]},
     },
 }
+# IVF-Flat cases with fixed answers (ann_tests.hpp:393-575): squared euclidean, all-zero
+# training data (n_list 2, n_probe 1, k 2: distances only, the indices tie) and the 1 x 1
+# case (n_list 1, n_probe 1, k 1), in column- and row-major order.
+known["ivf"] = [
+    {"name": "col zero", "source": "tests/unit_tests/approximate_neighbors/ann_tests.hpp:393-412",
+     "order": "column-major", "n_list": 2, "n_probe": 1, "k": 2, "n_samples": 8, "n_features": 4,
+     "n_queries": 2, "X_train": [0.0] * 32, "X_test": [1., 2., 3., 0.5, 1.5, 2.5, -0.5, -1],
+     "kdist": [12.5, 11.5, 12.5, 11.5], "kind": None},
+    {"name": "row zero", "source": "tests/unit_tests/approximate_neighbors/ann_tests.hpp:413-431",
+     "order": "row-major", "n_list": 2, "n_probe": 1, "k": 2, "n_samples": 8, "n_features": 4,
+     "n_queries": 2, "X_train": [0.0] * 32, "X_test": [1., 3., 1.5, -0.5, 2., 0.5, 2.5, -1],
+     "kdist": [12.5, 12.5, 11.5, 11.5], "kind": None},
+    {"name": "col 1x1", "source": "tests/unit_tests/approximate_neighbors/ann_tests.hpp:499-521",
+     "order": "column-major", "n_list": 1, "n_probe": 1, "k": 1, "n_samples": 1, "n_features": 1,
+     "n_queries": 3, "X_train": [0.5], "X_test": [-1., 0., 1.], "kdist": [2.25, 0.25, 0.25],
+     "kind": [0, 0, 0]},
+    {"name": "row 1x1", "source": "tests/unit_tests/approximate_neighbors/ann_tests.hpp:522-547",
+     "order": "row-major", "n_list": 1, "n_probe": 1, "k": 1, "n_samples": 1, "n_features": 1,
+     "n_queries": 3, "X_train": [0.5], "X_test": [-1., 0., 1.], "kdist": [2.25, 0.25, 0.25],
+     "kind": [0, 0, 0]},
+]
 known["dbscan_errors"] = {
     "source": "tests/unit_tests/dbscan/dbscan_public.cpp:309-395",
     "A_colmajor": [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12], "n_samples": 4, "n_features": 3,

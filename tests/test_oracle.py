@@ -338,3 +338,41 @@ def test_nn_predict_vs_sklearn(oracle):
     assert list(labels[3:]) == [vals[np.argmax(cnt)]] * 2
     tv, first = oracle.nn_regress(Xt, t, Xfar, r=2.2, outlier="most_frequent")
     assert first == 3 and np.allclose(tv[3:], t.mean())
+
+
+def test_ivf_oracle(known, oracle):
+    """IVF-Flat: the reference's fixed cases; all lists probed = exact k-NN; one list probed =
+    brute force restricted to the query's nearest list (a numpy statement)."""
+    for c in known["ivf"]:
+        Xt = np.array(c["X_train"]).reshape(
+            (c["n_features"], c["n_samples"]) if c["order"] == "column-major"
+            else (c["n_samples"], c["n_features"]))
+        Xq = np.array(c["X_test"]).reshape(
+            (c["n_features"], c["n_queries"]) if c["order"] == "column-major"
+            else (c["n_queries"], c["n_features"]))
+        if c["order"] == "column-major":
+            Xt, Xq = Xt.T, Xq.T
+        C = Xt[:c["n_list"]]  # k-means of these cases returns rows of the data
+        ind, dist, _ = oracle.ivf_search(C, Xt, Xq, c["k"], c["n_probe"])
+        got = dist.T.ravel() if c["order"] == "column-major" else dist.ravel()
+        np.testing.assert_allclose(got, c["kdist"])
+        if c["kind"] is not None:
+            assert (ind.T.ravel() if c["order"] == "column-major" else ind.ravel()).tolist() == c["kind"]
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((600, 6))
+    Q = rng.standard_normal((40, 6))
+    C = X[rng.choice(600, 12, replace=False)]
+    ind, dist, sizes = oracle.ivf_search(C, X, Q, 9, 12, metric="euclidean")
+    ei, ed = oracle.knn(X, Q, 9)
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-12)
+    assert sizes.sum() == 600
+    ind, dist, _ = oracle.ivf_search(C, X, Q, 5, 1)
+    lab = np.argmin(((X[:, None] - C[None]) ** 2).sum(-1), 1)
+    for q in range(len(Q)):
+        cq = np.argmin(((C - Q[q]) ** 2).sum(-1))
+        members = np.nonzero(lab == cq)[0]
+        d2 = ((X[members] - Q[q]) ** 2).sum(-1)
+        best = members[np.lexsort((members, d2))][:5]
+        got = ind[q][ind[q] >= 0]
+        np.testing.assert_array_equal(got, best)
