@@ -16,15 +16,19 @@ namespace dagpu {
 
 namespace {
 
-// One wave per (query, probe) pair; each lane takes one vector of the list at a time and
-// accumulates its squared distance to the query in double.
+// One wave per (query, probe) pair. Sixteen lanes share a vector: lane s of a group sums
+// features s, s + 16, ... (consecutive lanes read consecutive features, so each group's
+// loads are contiguous), then the group adds its partial sums with lane shuffles. A wave
+// therefore takes four vectors of the list at a time; distances accumulate in double.
 template <typename T>
 __global__ __launch_bounds__(256) void ivf_scan_kernel(
     const T *__restrict__ Q, int nq, int n_probe, int d, const T *__restrict__ X,
     const int *__restrict__ perm, const int *__restrict__ list_off,
     const int *__restrict__ probes, const int *__restrict__ seg, int64_t ld,
     T *__restrict__ keys, int *__restrict__ idx) {
+    constexpr int G = 16;
     const int lane = threadIdx.x & 63;
+    const int sub = lane & (G - 1), slot = lane / G;
     const int64_t pair = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (pair >= (int64_t)nq * n_probe)
         return;
@@ -34,16 +38,24 @@ __global__ __launch_bounds__(256) void ivf_scan_kernel(
     const T *qv = Q + (int64_t)q * d;
     T *kr = keys + (int64_t)q * ld + seg[pair];
     int *ir = idx + (int64_t)q * ld + seg[pair];
-    for (int r = b + lane; r < e; r += 64) {
-        const int g = perm[r];
+    for (int r0 = b; r0 < e; r0 += 64 / G) {
+        const int r = r0 + slot;
+        const bool live = r < e;
+        const int g = live ? perm[r] : 0;
         const T *xv = X + (int64_t)g * d;
         double acc = 0.0;
-        for (int l = 0; l < d; l++) {
-            const double t = (double)qv[l] - (double)xv[l];
-            acc += t * t;
+        if (live)
+            for (int l = sub; l < d; l += G) {
+                const double t = (double)qv[l] - (double)xv[l];
+                acc += t * t;
+            }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1)
+            acc += __shfl_xor(acc, off, G);
+        if (live && sub == 0) {
+            kr[r - b] = (T)acc;
+            ir[r - b] = g;
         }
-        kr[r - b] = (T)acc;
-        ir[r - b] = g;
     }
 }
 

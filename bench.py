@@ -494,6 +494,79 @@ def dbscan_bench(args, L, aoclda):
                                    f"(n / sample)^2 = {scale:.1f} to the full fit"}}))
 
 
+IVF_METRIC = "IVF-Flat approximate kNN queries/s at recall@10 against exact search, 1 GPU"
+
+
+def ivf_bench(args, L, aoclda):
+    """IVF-Flat on 1,000,000 x 96 fp32 vectors from a 1,024-component Gaussian mixture (means
+    U[-4,4]^96, sigma 1; numpy default_rng(21)), n_list 1,024, 10 k-means iterations,
+    10,000 queries from the same mixture, k = 10, squared euclidean. A step is one
+    da_approx_nn_kneighbors_s over all queries (host arrays in and out), at each n_probe of
+    --ivf-probes; recall@10 against the exact GPU brute-force search of the same vectors."""
+    import ctypes as C
+    rng = np.random.default_rng(21)
+    n, d, nq, k, nlist = args.ivf_n, 96, 10_000, 10, 1024
+    means = rng.uniform(-4, 4, (1024, d)).astype(np.float32)
+    X = means[rng.integers(0, 1024, n)] + rng.standard_normal((n, d), dtype=np.float32)
+    Q = means[rng.integers(0, 1024, nq)] + rng.standard_normal((nq, d), dtype=np.float32)
+    ann = aoclda.neighbors.approximate_neighbors(n_list=nlist, kmeans_iter=10, seed=5)
+    t0 = time.perf_counter()
+    ann.train(X)
+    t_train = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ann.add(X)
+    t_add = time.perf_counter() - t0
+    L.check(L.lib.da_amd_profile_enable(ann._h.h, 1))
+    exact = aoclda.neighbors.nearest_neighbors(n_neighbors=k, precision="bf16x3").fit(X)
+    _, ei = exact.kneighbors(Q)
+    log(f"ivf: trained {t_train:.2f}s, added {t_add:.2f}s")
+    probes = {}
+    for npb in [int(v) for v in args.ivf_probes.split(",")]:
+        ann.n_probe = npb
+        for _ in range(args.warmup):
+            ann.kneighbors(Q, n_neighbors=k)
+        L.check(L.lib.da_amd_profile_reset(ann._h.h))
+        ts = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            _, ind = ann.kneighbors(Q, n_neighbors=k)
+            ts.append(time.perf_counter() - t0)
+        kern = {}
+        for nm in ("ivf_coarse", "ivf_scan", "ivf_select"):
+            ms_, cnt = C.c_double(), L.da_int()
+            L.check(L.lib.da_amd_profile_get(ann._h.h, nm.encode(), C.byref(ms_), C.byref(cnt)))
+            if cnt.value:
+                kern[nm] = {"total_ms": ms_.value, "launches": cnt.value}
+        recall = float(np.mean([len(set(ind[q]) & set(ei[q])) / k for q in range(nq)]))
+        step = float(np.median(ts))
+        probes[npb] = {"qps": nq / step, "ms_per_step": step * 1e3, "recall_at_10": recall,
+                       "kernels_ms": kern}
+        log(f"ivf n_probe {npb}: {nq / step:.0f} q/s, recall {recall:.4f}")
+    sizes = ann.list_sizes
+    head = max((p for p in probes if probes[p]["recall_at_10"] >= 0.95), default=max(probes),
+               key=lambda p: (probes[p]["recall_at_10"] >= 0.99, -p))
+    H = probes[head]
+    # The scan: each candidate vector read once per query that probes it (4d bytes) and
+    # 3d fp64 operations; candidates per query = the probed lists' sizes.
+    sk = H["kernels_ms"].get("ivf_scan", {"total_ms": 0.0, "launches": 1})
+    scan_s = sk["total_ms"] / args.steps * 1e-3
+    cand = nq * head * float(np.mean(sizes))
+    byts = cand * 4 * d
+    roof = {"kernel": "ivf_scan", "bound": "hbm", "achieved": byts / scan_s / 1e9 if scan_s else None,
+            "peak": 8000.0, "unit": "GB/s", "frac": byts / scan_s / 8e12 if scan_s else None,
+            "traffic": None, "candidates_per_step": cand, "fp64_tflops": 3 * d * cand / scan_s / 1e12 if scan_s else None}
+    print(json.dumps({
+        "metric": IVF_METRIC, "value": H["qps"], "unit": "queries/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": H["ms_per_step"],
+        "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic: 1,024-component Gaussian mixture, numpy default_rng(21)",
+        "config": {"workload": f"IVF-Flat, {n}x{d} fp32, n_list {nlist}, {nq} queries, k {k}, "
+                               "squared euclidean", "n_probe": head},
+        "train_s": t_train, "add_s": t_add, "recall_at_10": H["recall_at_10"],
+        "list_size_min_max": [int(sizes.min()), int(sizes.max())],
+        "by_n_probe": probes, "roofline": roof}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -525,6 +598,10 @@ def main():
     ap.add_argument("--kb-instances", type=int, default=1024)
     ap.add_argument("--kb-iter", type=int, default=15)
     ap.add_argument("--kb-reps", type=int, default=3)
+    ap.add_argument("--ivf", action="store_true",
+                    help="IVF-Flat workload (1,000,000 x 96, n_list 1024, 10,000 queries)")
+    ap.add_argument("--ivf-n", type=int, default=1_000_000)
+    ap.add_argument("--ivf-probes", default="8,16,32")
     ap.add_argument("--dbscan", action="store_true",
                     help="DBSCAN workload (100,000 x 16, eps 1.8, min_samples 10)")
     ap.add_argument("--db-n", type=int, default=100_000)
@@ -558,6 +635,9 @@ def main():
         return
     if args.dbscan:
         dbscan_bench(args, L, aoclda)
+        return
+    if args.ivf:
+        ivf_bench(args, L, aoclda)
         return
 
     if args.copy_calibration_mib > 0:
