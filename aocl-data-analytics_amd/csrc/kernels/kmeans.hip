@@ -13,6 +13,7 @@
 // and the totals are exact to about 2^-100 of the data range. The exponents are chosen on
 // the host from the column maxima so that no partial or total can overflow.
 #include "launch.hpp"
+#include <type_traits>
 #include <algorithm>
 #include <climits>
 #include <hip/hip_runtime.h>
@@ -72,11 +73,13 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
     __shared__ double Cs[CS_ELEMS];
     const int i = blockIdx.x * 256 + threadIdx.x;
     const bool valid = i < n;
-    double x[DMAX > 0 ? DMAX : 1];
+    // The sample stays in registers in its own type (widened exactly to double when used),
+    // which halves the registers of a float sample.
+    T x[DMAX > 0 ? DMAX : 1];
     if constexpr (DMAX > 0) {
 #pragma unroll
         for (int l = 0; l < DMAX; l++)
-            x[l] = (valid && l < d) ? (double)X[(int64_t)i * d + l] : 0.0;
+            x[l] = (valid && l < d) ? X[(int64_t)i * d + l] : (T)0;
     }
     double best = __builtin_huge_val();
     int bl = 0;
@@ -95,7 +98,7 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
 #pragma unroll
                 for (int l = 0; l < DMAX; l++) {
                     if (l < d) {
-                        const double t = x[l] - Cs[c * d + l];
+                        const double t = (double)x[l] - Cs[c * d + l];
                         acc = fma(t, t, acc);
                     }
                 }
@@ -596,6 +599,12 @@ hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k
                            labels, prev, changed, mind);
     else if (d <= 64)
         hipLaunchKernelGGL((kmeans_assign_kernel<T, 64>), grid, block, 0, s, X, n, d, C, k,
+                           labels, prev, changed, mind);
+    else if (std::is_same_v<T, float> && d <= 96)
+        hipLaunchKernelGGL((kmeans_assign_kernel<T, 96>), grid, block, 0, s, X, n, d, C, k,
+                           labels, prev, changed, mind);
+    else if (std::is_same_v<T, float> && d <= 128)
+        hipLaunchKernelGGL((kmeans_assign_kernel<T, 128>), grid, block, 0, s, X, n, d, C, k,
                            labels, prev, changed, mind);
     else
         hipLaunchKernelGGL((kmeans_assign_kernel<T, 0>), grid, block, 0, s, X, n, d, C, k, labels,
