@@ -15,8 +15,9 @@ class nearest_neighbors:
                  candidates_per_split=0, seed_thresholds=False):
         # precision: operand precision of the MFMA distance tiles ("working", "fp32",
         # "fp16", "bf16", "fp16x3", "bf16x3"); results are re-ranked exactly whatever the
-        # choice. candidates_per_split: minimum candidate-list length per query and split of
-        # the fused tile (0 = k). seed_thresholds: bound the tile's thresholds with a sweep of
+        # choice. candidates_per_split: candidate-list length per query and split of
+        # the fused tile (0 = from k; shorter than k is allowed and stays exact).
+        # seed_thresholds: bound the tile's thresholds with a sweep of
         # a row subset first. All three are MI355X options that change speed only.
         if not isinstance(n_neighbors, int) and n_neighbors is not None:
             raise ValueError("Number of neighbors must be an integer or None. Got n_neighbors="

@@ -943,7 +943,8 @@ hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const fl
                             double u_in, double abs_in, double u_acc, double ymax, int post,
                             T *out_dist, int *out_idx, int *fb_count, int *fb_list, hipStream_t s) {
     const int m = nsplit * kc;
-    if (m > MMAX || d > 1024 || k > 128 || k > kc)
+    // The lists may be shorter than k; their union must hold k candidates.
+    if (m > MMAX || d > 1024 || k > 128 || k > m)
         return hipErrorInvalidValue;
     if (nq == 0)
         return hipSuccess;

@@ -273,3 +273,20 @@ def test_seeded_thresholds(oracle, precision, k):
         dist, ind = nn.kneighbors(Xq, n_neighbors=k)
         np.testing.assert_array_equal(ind, ei, err_msg=f"seed={seed}")
         np.testing.assert_allclose(dist, ed, rtol=1e-5)
+
+
+@pytest.mark.parametrize("precision", ["working", "bf16x3", "fp16x3"])
+@pytest.mark.parametrize("k,kc", [(100, 64), (40, 16), (60, 32), (128, 64)])
+def test_lists_shorter_than_k(oracle, precision, k, kc):
+    """Candidate lists shorter than k per split ("candidates per split" < k) return the exact
+    neighbours: a split's rejected rows lie above its threshold, so only queries whose k
+    nearest rows were all kept are certified; the others take the exact fallback."""
+    rng = np.random.default_rng(71)
+    means = rng.uniform(-5, 5, (10, 24))
+    Xt = gaussian_mixture(rng, 60000, means).astype(np.float32)
+    Xq = gaussian_mixture(rng, 300, means).astype(np.float32)
+    nn = aoclda.neighbors.nearest_neighbors(precision=precision, candidates_per_split=kc).fit(Xt)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=k)
+    ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), k)
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-5)
