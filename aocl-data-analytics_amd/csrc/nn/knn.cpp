@@ -73,7 +73,7 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
     opts.add_int("tile variant", "Kernel variant of the fused kNN distance tile (-1 automatic).",
                  -1, LowerBound::greater_equal, 2, UpperBound::less_equal, -1);
     // MI355X extension: length of the candidate list each split of the fused tile keeps per
-    // query (rounded up to 16, 32, 64 or 128; 0 = the smallest that holds k). A longer list
+    // query (rounded up to 16, 32, 64 or 128; 0 = automatic). A longer list
     // lowers the split thresholds, so a low-precision form with a wider error bound still
     // certifies its queries instead of sending them to the exact fallback; a list shorter
     // than k, spread over enough splits, makes the sweep's bookkeeping cheaper.

@@ -215,7 +215,7 @@ def test_fused_large_k(oracle, k, precision):
 @pytest.mark.parametrize("variant", [0, 1, 2])
 @pytest.mark.parametrize("precision", ["working", "fp64", "fp16", "fp16x3", "bf16x3"])
 @pytest.mark.parametrize("d", [20, 64, 128])
-@pytest.mark.parametrize("k", [12, 30])
+@pytest.mark.parametrize("k", [5, 12, 30])
 def test_tile_variants(oracle, variant, precision, d, k):
     """Every sweep of the fused tile (two training tiles per step with the loads under the
     bookkeeping, load-then-compute, one tile per step) gives the exact result, for short
