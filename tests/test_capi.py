@@ -172,3 +172,54 @@ def test_no_gpu_is_an_error_not_a_fallback():
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True,
                          timeout=120)
     assert out.stdout.strip() == str(st("arch_not_supported")), out.stderr
+
+
+def test_dbscan_and_ann_handles_without_gpu():
+    """Host-side behaviour of the DBSCAN and IVF-Flat handles: options, call order and
+    argument checks (dbscan_public.cpp, ann_public.cpp of the reference) come before any
+    device work."""
+    lib = L.lib
+    for t in (L.HANDLE_DBSCAN, L.HANDLE_APPROX_NN):
+        h = L.da_handle()
+        assert lib.da_handle_init_d(C.byref(h), t) == 0
+        lib.da_handle_destroy(C.byref(h))
+    h = L.Handle(L.HANDLE_DBSCAN, np.float64)
+    v = L.da_int()
+    assert lib.da_options_get_int(h.h, b"min samples", C.byref(v)) == 0 and v.value == 5
+    assert lib.da_options_set_int(h.h, b"min samples", 0) == st("option_invalid_bounds")
+    assert lib.da_options_set_string(h.h, b"metric", b"inner product") == st("option_invalid_value")
+    assert lib.da_options_set_real_d(h.h, b"eps", -1.0) == st("option_invalid_bounds")
+    assert lib.da_dbscan_compute_d(h.h) == st("no_data")
+    X = np.ones((4, 3))
+    assert lib.da_dbscan_set_data_d(h.h, 0, 3, L.ptr(X), 4) == st("invalid_array_dimension")
+    assert lib.da_dbscan_set_data_d(h.h, 4, 3, None, 4) == st("invalid_pointer")
+    assert lib.da_dbscan_set_data_d(h.h, 4, 3, L.ptr(X), 3) == st("invalid_leading_dimension")
+    assert lib.da_dbscan_set_data_s(h.h, 4, 3, L.ptr(X), 4) == st("wrong_type")
+    assert lib.da_dbscan_compute_d(None) == st("handle_not_initialized")
+    a = L.Handle(L.HANDLE_APPROX_NN, np.float64)
+    buf = C.create_string_buffer(32)
+    n = L.da_int(32)
+    assert lib.da_options_get_string(a.h, b"metric", buf, C.byref(n)) == 0
+    assert buf.value == b"sqeuclidean"
+    assert lib.da_options_set_real_d(a.h, b"train fraction", 0.0) == st("option_invalid_bounds")
+    assert lib.da_approx_nn_train_d(a.h) == st("no_data")
+    assert lib.da_approx_nn_set_training_data_d(a.h, 4, 3, L.ptr(X), 2) == st("invalid_leading_dimension")
+    ind = np.zeros(4, np.int32)
+    assert lib.da_approx_nn_kneighbors_d(a.h, 1, 3, L.ptr(X), 1, L.ptr(ind), None, 1, 0) == st("no_data")
+    assert lib.da_approx_nn_add_d(a.h, 4, 3, L.ptr(X), 4) == st("no_data")
+
+
+def test_nn_prediction_checks_without_data():
+    h = L.Handle(L.HANDLE_NN, np.float64)
+    lib = L.lib
+    X = np.ones((3, 2))
+    y = np.zeros(3, np.int32)
+    out = np.zeros(3, np.int32)
+    assert lib.da_nn_classifier_predict_d(h.h, 3, 2, L.ptr(X), 3, L.ptr(out), 0) == st("no_data")
+    assert lib.da_nn_regressor_predict_d(h.h, 3, 2, L.ptr(X), 3, L.ptr(X), 0) == st("no_data")
+    assert lib.da_nn_set_labels_d(h.h, 3, None) == st("invalid_pointer")
+    assert lib.da_nn_set_labels_d(h.h, 0, L.ptr(y)) == st("invalid_array_dimension")
+    n = L.da_int(0)
+    assert lib.da_nn_classes_d(h.h, C.byref(n), None) == st("no_data")
+    assert lib.da_nn_radius_neighbors_d(h.h, 3, 2, L.ptr(X), 3, 1.0, 0, 1) == st("invalid_input")
+    assert lib.da_nn_radius_neighbors_d(h.h, 3, 2, L.ptr(X), 3, 1.0, 1, 0) == st("no_data")
