@@ -91,8 +91,8 @@ da_status DBSCANHandle<T>::set_data(da_int n_samples, da_int n_features, const T
 
 // The eps-neighbour graph in CSR form (radius_neighbors.cpp:47-200 builds the same lists):
 // per block of rows, the exact key tile against all samples (double accumulation, rounded
-// to T), a count of the keys <= thr other than the point itself, then a fill of the
-// neighbour indices in ascending order. Leaves count_ (host), offs_ (n + 1 int64) and ind_.
+// to T) with the count of the keys <= thr other than the point itself taken as the tile is
+// written, then a fill of the neighbour indices in ascending order. Leaves count_ (host), offs_ (n + 1 int64) and ind_.
 template <typename T> da_status DBSCANHandle<T>::neighbour_graph(int km, T thr) {
     hipStream_t st = gpu->stream();
     const int n = n_samples_, d = n_features_;
@@ -111,6 +111,7 @@ template <typename T> da_status DBSCANHandle<T>::neighbour_graph(int km, T thr) 
     const int seg = radius_segment_len(n), nseg = (n + seg - 1) / seg;
     DA_HIP(err, keys_.reserve((size_t)chunk * n * sizeof(T)));
     DA_HIP(err, cnt_.reserve((size_t)chunk * nseg * sizeof(int)));
+    DA_HIP(err, tile_.reserve((size_t)chunk * ((n + 63) / 64) * sizeof(int)));
     DA_HIP(err, boff_.reserve((size_t)chunk * nseg * sizeof(int)));
     count_.assign(n, 0);
     std::vector<int> scnt((size_t)chunk * nseg), boff((size_t)chunk * nseg);
@@ -121,14 +122,14 @@ template <typename T> da_status DBSCANHandle<T>::neighbour_graph(int km, T thr) 
         const int rows = std::min(chunk, n - r0);
         {
             KernelScope ks(*gpu, "pairwise_direct");
-            DA_HIP(err, (launch_pairwise_direct<T, T>(km, X + (size_t)r0 * d, X, rows, n, d,
-                                                      keys_.as<T>(), n, nrm ? nrm + r0 : nullptr,
-                                                      nrm, (double)p_, 0, 0, st)));
+            DA_HIP(err, (launch_pairwise_direct_count<T, T>(
+                            km, X + (size_t)r0 * d, X, rows, n, d, keys_.as<T>(), n,
+                            nrm ? nrm + r0 : nullptr, nrm, (double)p_, thr, r0, tile_.as<int>(),
+                            st)));
         }
         {
             KernelScope ks(*gpu, "radius_count");
-            DA_HIP(err, launch_radius_count<T>(keys_.as<T>(), n, rows, n, thr, r0, cnt_.as<int>(),
-                                               st));
+            DA_HIP(err, launch_radius_segsum(tile_.as<int>(), rows, n, cnt_.as<int>(), st));
         }
         DA_HIP(err, hipMemcpyAsync(scnt.data(), cnt_.ptr(), (size_t)rows * nseg * sizeof(int),
                                    hipMemcpyDeviceToHost, st));

@@ -37,7 +37,7 @@ template <typename T> class DBSCANHandle : public AlgHandle {
     long long edges_ = 0;
     // Device state: samples (row-major n x d), norms, key tile, neighbour graph (CSR),
     // union-find links and the cluster numbering.
-    DeviceBuffer X_, norms_, keys_, cnt_, boff_, ind_, ind_spare_, offs_, core_d_, parent_,
+    DeviceBuffer X_, norms_, keys_, tile_, cnt_, boff_, ind_, ind_spare_, offs_, core_d_, parent_,
         is_root_, rid_, labels_d_, tmp_;
 };
 

@@ -54,11 +54,14 @@ __device__ __forceinline__ double finish(double acc, int i, int j, const double 
     }
 }
 
-template <typename T, typename U, int METRIC>
+// With COUNT, the tile also counts per row the stored keys <= thr (skipping column
+// self0 + row when self0 >= 0) and writes the count to tilecnt[row * gridDim.x + blockIdx.x],
+// which saves a radius search one read of the key tile.
+template <typename T, typename U, int METRIC, bool COUNT>
 __global__ __launch_bounds__(256) void pairwise_direct_kernel(
     const T *__restrict__ X, const T *__restrict__ Y, int m, int n, int k, U *__restrict__ D,
     int64_t ldd, const double *__restrict__ nx, const double *__restrict__ ny, double p,
-    int zero_diag, int diag_offset) {
+    int zero_diag, int diag_offset, U thr, int self0, int *__restrict__ tilecnt) {
     __shared__ double Xs[KC][TB + 1];
     __shared__ double Ys[KC][TB + 1];
     const int tid = threadIdx.x;
@@ -109,6 +112,7 @@ __global__ __launch_bounds__(256) void pairwise_direct_kernel(
         __syncthreads();
     }
 
+    int hits[TR] = {};
 #pragma unroll
     for (int r = 0; r < TR; r++) {
         const int i = i0 + ty + 16 * r;
@@ -122,7 +126,23 @@ __global__ __launch_bounds__(256) void pairwise_direct_kernel(
             double v = finish<METRIC>(acc[r][c], i, j, nx, ny, p);
             if (zero_diag && i + diag_offset == j)
                 v = 0.0;
-            D[(int64_t)i * ldd + j] = (U)v;
+            const U key = (U)v;
+            D[(int64_t)i * ldd + j] = key;
+            if constexpr (COUNT)
+                hits[r] += key <= thr && !(self0 >= 0 && j == self0 + i);
+        }
+    }
+    if constexpr (COUNT) {
+        // The 16 lanes of equal ty hold one row's 64 columns between them.
+#pragma unroll
+        for (int r = 0; r < TR; r++) {
+            int h = hits[r];
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1)
+                h += __shfl_xor(h, off, 16);
+            const int i = i0 + ty + 16 * r;
+            if (tx == 0 && i < m)
+                tilecnt[(int64_t)i * gridDim.x + blockIdx.x] = h;
         }
     }
 }
@@ -259,8 +279,9 @@ hipError_t launch_pairwise_direct(int metric, const T *X, const T *Y, int m, int
     dim3 block(256);
 #define DA_PW_CASE(M)                                                                        \
     case M:                                                                                  \
-        hipLaunchKernelGGL((pairwise_direct_kernel<T, U, M>), grid, block, 0, s, X, Y, m, n,  \
-                           k, D, ldd, nx, ny, p, zero_diag, diag_offset);                                 \
+        hipLaunchKernelGGL((pairwise_direct_kernel<T, U, M, false>), grid, block, 0, s, X, Y, \
+                           m, n, k, D, ldd, nx, ny, p, zero_diag, diag_offset, (U)0, -1,      \
+                           nullptr);                                                         \
         break;
     switch (metric) {
         DA_PW_CASE(KM_SQEUCLIDEAN)
@@ -278,6 +299,40 @@ hipError_t launch_pairwise_direct(int metric, const T *X, const T *Y, int m, int
 #undef DA_PW_CASE
     return hipGetLastError();
 }
+
+template <typename T, typename U>
+hipError_t launch_pairwise_direct_count(int metric, const T *X, const T *Y, int m, int n, int k,
+                                        U *D, int64_t ldd, const double *nx, const double *ny,
+                                        double p, U thr, int self0, int *tilecnt,
+                                        hipStream_t s) {
+    dim3 grid((n + TB - 1) / TB, (m + TB - 1) / TB);
+    dim3 block(256);
+#define DA_PWC_CASE(M)                                                                       \
+    case M:                                                                                  \
+        hipLaunchKernelGGL((pairwise_direct_kernel<T, U, M, true>), grid, block, 0, s, X, Y,  \
+                           m, n, k, D, ldd, nx, ny, p, 0, 0, thr, self0, tilecnt);           \
+        break;
+    switch (metric) {
+        DA_PWC_CASE(KM_SQEUCLIDEAN)
+        DA_PWC_CASE(KM_EUCLIDEAN)
+        DA_PWC_CASE(KM_MANHATTAN)
+        DA_PWC_CASE(KM_MINKOWSKI)
+        DA_PWC_CASE(KM_COSINE)
+        DA_PWC_CASE(KM_SQEUCLIDEAN_GEMM)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef DA_PWC_CASE
+    return hipGetLastError();
+}
+template hipError_t launch_pairwise_direct_count<float, float>(int, const float *, const float *,
+                                                               int, int, int, float *, int64_t,
+                                                               const double *, const double *,
+                                                               double, float, int, int *,
+                                                               hipStream_t);
+template hipError_t launch_pairwise_direct_count<double, double>(
+    int, const double *, const double *, int, int, int, double *, int64_t, const double *,
+    const double *, double, double, int, int *, hipStream_t);
 
 template <typename T>
 hipError_t launch_pack_rows_prec(const T *src, int64_t lds, int rows, int cols, void *dst,

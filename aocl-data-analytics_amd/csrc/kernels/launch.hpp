@@ -175,16 +175,22 @@ int knn_tile_qb(int kc);
 // Largest nsplit * kc the merge accepts.
 int knn_merge_max_candidates();
 // Radius search on a rows x n tile of distance keys D (row-major, leading dimension ldd).
-// Each row is cut into nseg = ceil(n / radius_segment_len(n)) segments of columns. The
-// count writes segcnt[r * nseg + s] = #{j in segment s : D[r][j] <= thr, j != self0 + r when
-// self0 >= 0}; the fill writes, from segoff[r * nseg + s] on, the indices j of those entries
-// in ascending order (and their keys when key is not null); the sort orders
+// Each row is cut into nseg = ceil(n / radius_segment_len(n)) segments of columns (a
+// multiple of 64). launch_pairwise_direct_count writes the keys and, per row and 64-column
+// tile, the number of keys <= thr other than column self0 + row (self0 >= 0);
+// launch_radius_segsum sums those into segcnt[r * nseg + s]. The fill writes, from
+// segoff[r * nseg + s] on, the indices j of the keys <= thr of segment s in ascending order
+// (and their keys when key is not null); the sort orders
 // each row's range [offs[r], offs[r+1]) by key, stably (ties stay in index order), with
 // temporary storage of radius_sort_bytes bytes; the sqrt pass takes the square root of n
 // values in place.
-template <typename U>
-hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
-                               int *segcnt, hipStream_t s);
+template <typename T, typename U>
+hipError_t launch_pairwise_direct_count(int metric, const T *X, const T *Y, int m, int n, int k,
+                                        U *D, int64_t ldd, const double *nx, const double *ny,
+                                        double p, U thr, int self0, int *tilecnt,
+                                        hipStream_t s);
+hipError_t launch_radius_segsum(const int *tilecnt, int rows, int n, int *segcnt,
+                                hipStream_t s);
 template <typename U>
 hipError_t launch_radius_fill(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
                               const int *segoff, int *ind, U *key, hipStream_t s);

@@ -18,39 +18,25 @@ namespace dagpu {
 
 namespace {
 
-// A wave sweeps one segment of one row, four 64-entry steps per iteration so that four
-// loads are in flight; the grid is (row groups of 4) x (segments).
-template <typename U>
-__global__ __launch_bounds__(256) void radius_count_kernel(const U *__restrict__ D, int64_t ldd,
-                                                           int rows, int n, U thr, int self0,
-                                                           int seg, int nseg,
-                                                           int *__restrict__ segcnt) {
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
-    if (row >= rows)
-        return;
-    const int c0 = (int)blockIdx.y * seg, c1 = min(n, c0 + seg);
-    const U *d = D + (int64_t)row * ldd;
-    const int self = self0 >= 0 ? self0 + row : -1;
-    int c = 0;
-    for (int j0 = c0; j0 < c1; j0 += 256) {
-        U v[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int j = j0 + u * 64 + lane;
-            v[u] = j < c1 ? d[j] : thr;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int j = j0 + u * 64 + lane;
-            const bool hit = j < c1 && v[u] <= thr && j != self;
-            c += __popcll(__builtin_amdgcn_ballot_w64(hit));
-        }
+// Per-row counts of a row segment from the per-tile counts of the key-tile kernel: the
+// segment's tiles_per_seg 64-column tiles (the last segment may hold fewer).
+__global__ void radius_segsum_kernel(const int *__restrict__ tilecnt, int rows, int ntiles,
+                                     int tiles_per_seg, int nseg, int *__restrict__ segcnt) {
+    const int64_t total = (int64_t)rows * nseg;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / nseg;
+        const int g = (int)(t - r * nseg);
+        const int b = g * tiles_per_seg, e = min(ntiles, b + tiles_per_seg);
+        int c = 0;
+        for (int j = b; j < e; j++)
+            c += tilecnt[r * ntiles + j];
+        segcnt[t] = c;
     }
-    if (lane == 0)
-        segcnt[(int64_t)row * nseg + blockIdx.y] = c;
 }
 
+// A wave sweeps one segment of one row, four 64-entry steps per iteration so that four
+// loads are in flight; the grid is (row groups of 4) x (segments).
 template <typename U>
 __global__ __launch_bounds__(256) void radius_fill_kernel(const U *__restrict__ D, int64_t ldd,
                                                           int rows, int n, U thr, int self0,
@@ -107,14 +93,17 @@ int radius_segment_len(int n) {
     return seg;
 }
 
-template <typename U>
-hipError_t launch_radius_count(const U *D, int64_t ldd, int rows, int n, U thr, int self0,
-                               int *segcnt, hipStream_t s) {
+hipError_t launch_radius_segsum(const int *tilecnt, int rows, int n, int *segcnt,
+                                hipStream_t s) {
     if (rows <= 0 || n <= 0)
         return hipSuccess;
     const int seg = radius_segment_len(n), nseg = (n + seg - 1) / seg;
-    hipLaunchKernelGGL(radius_count_kernel<U>, dim3((rows + 3) / 4, nseg), dim3(256), 0, s, D,
-                       ldd, rows, n, thr, self0, seg, nseg, segcnt);
+    const int ntiles = (n + 63) / 64;
+    const int64_t total = (int64_t)rows * nseg;
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256,
+                                                                              4096));
+    hipLaunchKernelGGL(radius_segsum_kernel, dim3(blocks), dim3(256), 0, s, tilecnt, rows, ntiles,
+                       seg / 64, nseg, segcnt);
     return hipGetLastError();
 }
 
@@ -156,8 +145,6 @@ template <typename U> hipError_t launch_radius_sqrt(U *v, int64_t n, hipStream_t
 }
 
 #define DA_RADIUS_INST(U)                                                                       \
-    template hipError_t launch_radius_count<U>(const U *, int64_t, int, int, U, int, int *,     \
-                                               hipStream_t);                                    \
     template hipError_t launch_radius_fill<U>(const U *, int64_t, int, int, U, int,             \
                                               const int *, int *, U *, hipStream_t);            \
     template hipError_t radius_sort_bytes<U>(int, int, size_t *);                               \

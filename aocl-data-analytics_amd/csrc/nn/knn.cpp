@@ -923,6 +923,7 @@ da_status NNHandle<T>::radius_core(const T *dQ, da_int n_queries, T radius, bool
     const int seg = radius_segment_len(n), nseg = (n + seg - 1) / seg;
     DA_HIP(err, rnn_keys_.reserve((size_t)chunk * n * sizeof(T)));
     DA_HIP(err, rnn_cnt_.reserve((size_t)chunk * nseg * sizeof(int)));
+    DA_HIP(err, rnn_tile_.reserve((size_t)chunk * ((n + 63) / 64) * sizeof(int)));
     DA_HIP(err, rnn_segoff_.reserve((size_t)chunk * nseg * sizeof(int)));
     DA_HIP(err, rnn_off_.reserve((size_t)(chunk + 1) * sizeof(int)));
     std::vector<int> scnt((size_t)chunk * nseg), soff((size_t)chunk * nseg), off(chunk + 1);
@@ -930,14 +931,13 @@ da_status NNHandle<T>::radius_core(const T *dQ, da_int n_queries, T radius, bool
         const da_int rows = std::min(chunk, n_queries - q0);
         {
             KernelScope ks(*gpu, "pairwise_direct");
-            DA_HIP(err, (launch_pairwise_direct<T, T>(km, dQ + (size_t)q0 * d, train_.as<T>(),
-                                                      rows, n, d, rnn_keys_.as<T>(), n,
-                                                      nq_norm ? nq_norm + q0 : nullptr, nt_norm,
-                                                      p_, 0, 0, st)));
+            DA_HIP(err, (launch_pairwise_direct_count<T, T>(
+                            km, dQ + (size_t)q0 * d, train_.as<T>(), rows, n, d, rnn_keys_.as<T>(),
+                            n, nq_norm ? nq_norm + q0 : nullptr, nt_norm, p_, thr, -1,
+                            rnn_tile_.as<int>(), st)));
         }
         KernelScope ks(*gpu, "radius_select");
-        DA_HIP(err, launch_radius_count<T>(rnn_keys_.as<T>(), n, rows, n, thr, -1,
-                                           rnn_cnt_.as<int>(), st));
+        DA_HIP(err, launch_radius_segsum(rnn_tile_.as<int>(), rows, n, rnn_cnt_.as<int>(), st));
         DA_HIP(err, hipMemcpyAsync(scnt.data(), rnn_cnt_.ptr(), (size_t)rows * nseg * sizeof(int),
                                    hipMemcpyDeviceToHost, st));
         DA_HIP(err, hipStreamSynchronize(st));

@@ -120,7 +120,7 @@ template <typename T> class NNHandle : public AlgHandle {
     bool rnn_done_ = false, rnn_dist_ = false;
     std::vector<da_int> rnn_count_, rnn_ind_;
     std::vector<T> rnn_d_;
-    DeviceBuffer rnn_keys_, rnn_cnt_, rnn_off_, rnn_segoff_, rnn_ind_buf_, rnn_key_buf_, rnn_ind2_, rnn_key2_,
+    DeviceBuffer rnn_keys_, rnn_tile_, rnn_cnt_, rnn_off_, rnn_segoff_, rnn_ind_buf_, rnn_key_buf_, rnn_ind2_, rnn_key2_,
         rnn_tmp_;
     // Options as read at set_data (nearest_neighbors.cpp:295-352).
     da_int n_neighbors_ = 5, algo_ = 3, metric_ = 0;
