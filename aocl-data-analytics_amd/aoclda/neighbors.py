@@ -347,3 +347,8 @@ class approximate_neighbors:
         dim = L.da_int(4)
         self._h.call("da_handle_get_result", L.RINFO, C.byref(dim), L.ptr(r))
         return r
+
+    @property
+    def n_index(self):
+        """Number of vectors added to the index."""
+        return int(self.rinfo[1])
