@@ -92,13 +92,15 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
 // The candidate-list length: from k, or as set by "candidates per split". A list shorter
 // than k is exact as well: every row a split leaves out has a key at or above that split's
 // threshold, so the merge certifies a query only when all k nearest rows were kept.
-// By default a split keeps about k/2 candidates for k > 32 (the next list length up):
-// measured at k = 100 1.7-2.3x faster than 128-long lists (2,000,000 x 96 and
-// 200,000 x 128) and at k = 50 1.2x faster than 64-long lists, every query certified. At
-// k <= 32 a 16-long list leaves too many queries uncertified (k = 30: 455 of 20,000), so
-// the list holds k.
+// By default a split keeps 32 candidates for 32 < k <= 128 (max(32, ceil(k / 4)), rounded
+// up to a list length). Against 64-long lists, measured with bf16x3 at 200,000 x 128
+// (20,000 queries) and 2,000,000 x 96 (2,048 queries): k = 50 1.2x faster at both; k = 100
+// 1.27x faster at 2,000,000 x 96, equal at 200,000 x 128 (10 of 20,000 queries then take the
+// exact fallback); k = 128 slightly faster (91 fallbacks). 64-long lists had in turn been
+// 1.7-2.3x faster than 128-long ones at k = 100. At k <= 32 a 16-long list leaves too many
+// queries uncertified (k = 30: 455 of 20,000), so the list holds k.
 int SearchParams::tile_kc(int k) const {
-    return knn_tile_kc(kc_min > 0 ? kc_min : (k > 32 ? (k + 1) / 2 : k));
+    return knn_tile_kc(kc_min > 0 ? kc_min : (k > 32 ? std::max(32, (k + 3) / 4) : k));
 }
 
 // Tuning table of the fused tile variant per operand precision and padded width: the
@@ -315,14 +317,15 @@ template <typename T>
 int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq,
                              const void *qpad, const float *seed, hipStream_t st) {
     int slots = 0;
+    SearchContext &tc = ctx.tuning ? *ctx.tuning : ctx;
     const int skey = ((prec * 8 + variant) * 2 + (seed ? 1 : 0)) * 256 + kc;
-    auto it = ctx.slots.find(skey);
-    if (it != ctx.slots.end()) {
+    auto it = tc.slots.find(skey);
+    if (it != tc.slots.end()) {
         slots = it->second;
     } else {
         if (knn_tile_slots(prec, kp_, kc, variant, &slots) != hipSuccess || slots <= 0)
             slots = 512;
-        ctx.slots[skey] = slots;
+        tc.slots[skey] = slots;
     }
     const int64_t n = n_samples_;
     const int64_t qb = knn_tile_qb(kc);
@@ -336,8 +339,8 @@ int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, 
         bucket++;
     const int key = skey * 64 + bucket;
     auto clamp = [&](int64_t v) { return (int)std::max<int64_t>(lo, std::min<int64_t>(hi, v)); };
-    auto ft = ctx.split_factor.find(key);
-    if (ft != ctx.split_factor.end())
+    auto ft = tc.split_factor.find(key);
+    if (ft != tc.split_factor.end())
         return ft->second;
     // Candidates: the most splits whose workgroups all run in one round, and 1, 2 and 4
     // times the count that fills 90% of the slots. Time them (a first untimed launch warms
@@ -388,7 +391,7 @@ int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, 
     }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    ctx.split_factor[key] = best;
+    tc.split_factor[key] = best;
     return best;
 }
 
@@ -396,7 +399,13 @@ template <typename T>
 da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp, const T *dQ,
                                      da_int nq, da_int k, bool return_distance, da_int *d_ind,
                                      T *d_dist, bool &fused) {
-    const int kc = sp.tile_kc(k);
+    int kc = sp.tile_kc(k);
+    // With the automatic length, a list shorter than k needs 2k candidates over the splits
+    // the training set allows (at most one per 1,024 rows): on a small set the next length up
+    // keeps the fused tile instead of the exact key tiles.
+    if (sp.kc_min <= 0)
+        while (kc < k && kc < 128 && (int64_t)split_cap(kc) * kc < 2 * (int64_t)k)
+            kc *= 2;
     int variant = sp.variant;
     if (variant < 0)
         variant = knn_tile_auto_variant(sp.prec, kp_);

@@ -16,6 +16,9 @@ struct SearchContext {
     GpuContext *gpu = nullptr;
     std::map<int, int> slots;        // concurrent workgroups per fused-tile configuration
     std::map<int, int> split_factor; // tuned split count per (form, list length, size class)
+    // When set, the tuned maps above are read and written in this context instead: contexts
+    // that search the same handle from one thread share their tuning.
+    SearchContext *tuning = nullptr;
     DeviceBuffer q_norm, keys, sel_key, sel_idx, qpad, part_key, part_idx, part_tau, fb, fb_q,
         fb_ind, fb_dist, tau0;
     // Pinned host word receiving the fallback count of the last fused launch.

@@ -89,6 +89,9 @@ template <typename T> class Server : public ServerBase {
             DA_HIP(err, sl.dq.reserve((size_t)batch_cap_ * d_ * sizeof(T)));
             DA_HIP(err, hipHostMalloc((void **)&sl.hq, (size_t)batch_cap_ * d_ * sizeof(T)));
         }
+        // Both slots are driven by the one worker thread: they share one set of tuned split
+        // counts, so each (list length, batch size class) is tuned once, not once per slot.
+        slots_[1].ctx.tuning = &slots_[0].ctx;
         worker_ = std::thread([this] { run(); });
         return da_status_success;
     }

@@ -186,6 +186,12 @@ def serving(args, L, aoclda):
             d2, i2 = nn.kneighbors(pool[o:o + q], n_neighbors=int(k))
             if not (np.array_equal(ind, i2) and np.array_equal(dist, d2)):
                 raise RuntimeError("served result differs from the direct search")
+        # The first batch of each (candidate-list length, batch size class) runs the split
+        # count tuner, a few trial launches: a service warms every class it can form before
+        # it takes traffic. One request of 2^j queries for each list length does that here.
+        for k in (10, 100):
+            for j in range(13):
+                srv.wait(srv.submit(pool[:1 << j], k))
         # Closed loop: capacity.
         nqs, ks, offs = requests(args.serve_closed, 12)
         t0 = time.perf_counter()
