@@ -616,10 +616,10 @@ def main():
                     help="run the continuous-batching serving benchmark instead")
     ap.add_argument("--serve-rows", type=int, default=2_000_000)
     ap.add_argument("--serve-forms", default="fp16x3")
-    ap.add_argument("--serve-batch", type=int, default=2048)
+    ap.add_argument("--serve-batch", type=int, default=4096)
     ap.add_argument("--serve-bytes", type=int, default=1 << 30)
     ap.add_argument("--serve-closed", type=int, default=200)
-    ap.add_argument("--serve-loads", default="0.3,0.5,0.7")
+    ap.add_argument("--serve-loads", default="0.05,0.1,0.3,0.5")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     args = ap.parse_args()
     if args.gpus != 1:
