@@ -128,7 +128,7 @@ template <typename T> da_status DBSCANHandle<T>::neighbour_graph(int km, T thr) 
                             st)));
         }
         {
-            KernelScope ks(*gpu, "radius_count");
+            KernelScope ks(*gpu, "radius_segsum");
             DA_HIP(err, launch_radius_segsum(tile_.as<int>(), rows, n, cnt_.as<int>(), st));
         }
         DA_HIP(err, hipMemcpyAsync(scnt.data(), cnt_.ptr(), (size_t)rows * nseg * sizeof(int),

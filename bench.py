@@ -446,7 +446,7 @@ def dbscan_bench(args, L, aoclda):
         t0 = time.perf_counter()
         h.call("da_dbscan_compute")
         times.append(time.perf_counter() - t0)
-    names = ("pairwise_direct", "radius_count", "radius_fill", "dbscan_union", "dbscan_label")
+    names = ("pairwise_direct", "radius_segsum", "radius_fill", "dbscan_union", "dbscan_label")
     kern = {}
     for nm in names:
         ms_, cnt = C.c_double(), L.da_int()
