@@ -78,6 +78,27 @@ def test_vs_oracle(oracle, metric, dtype, order):
             np.testing.assert_array_equal(ind, ei, err_msg=f"n_probe {n_probe} k {k}")
             np.testing.assert_allclose(dist, ed, rtol=rtol)
     np.testing.assert_array_equal(ann.list_sizes, sizes)
+
+
+@pytest.mark.parametrize("d", [12, 40, 96, 128])
+@pytest.mark.parametrize("order", ["C", "F"])
+def test_vs_oracle_chunked_rows(oracle, d, order):
+    """fp32 rows of whole 16-byte chunks (d a multiple of 4, up to 128) are scanned in
+    chunks: one, two, three and four chunks per lane here, including partly used ones."""
+    rng = np.random.default_rng(60 + d)
+    means = rng.uniform(-6, 6, (24, d))
+    X = (means[rng.integers(0, 24, 4000)] + rng.standard_normal((4000, d))).astype(np.float32)
+    Q = (means[rng.integers(0, 24, 200)] + rng.standard_normal((200, d))).astype(np.float32)
+    ann = aoclda.neighbors.approximate_neighbors(n_list=32, kmeans_iter=8,
+                                                 seed=3).train_and_add(np.asarray(X, order=order))
+    C = np.asarray(ann.cluster_centroids, dtype=np.float64)
+    for n_probe in (1, 5, 32):
+        ann.n_probe = n_probe
+        for k in (1, 10, 50):
+            dist, ind = ann.kneighbors(np.asarray(Q, order=order), n_neighbors=k)
+            ei, ed, _ = oracle.ivf_search(C, X, Q, k, n_probe, "sqeuclidean", single=True)
+            np.testing.assert_array_equal(ind, ei, err_msg=f"n_probe {n_probe} k {k}")
+            np.testing.assert_allclose(dist, ed, rtol=1e-6)
     # every list probed: the exact k nearest neighbours
     ann.n_probe = 32
     dist, ind = ann.kneighbors(Q, n_neighbors=10)
