@@ -60,16 +60,44 @@ __global__ __launch_bounds__(256) void ivf_scan_kernel(
     }
 }
 
-// The same scan for fp32 rows of d = 4 * d4 <= 128 features: eight lanes share a vector and
-// read it as 16-byte chunks (lane s takes chunks s, s + 8, ...), so a wave takes eight
-// vectors per iteration with a quarter of the load instructions; the query's chunks stay
-// in registers for the whole list. Distances accumulate in double, chunk by chunk.
-template <int NJ>
-__global__ __launch_bounds__(256) void ivf_scan4_kernel(
-    const float *__restrict__ Q, int nq, int n_probe, int d4, const float *__restrict__ X,
+// 16-byte chunks of a row: four floats or two doubles. acc adds the squared differences of
+// a chunk's elements in element order.
+template <typename T> struct Chunk16;
+template <> struct Chunk16<float> {
+    using type = float4;
+    __device__ static type zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+    __device__ static void acc(double &a, const type &q, const type &x) {
+        const double t0 = (double)q.x - (double)x.x, t1 = (double)q.y - (double)x.y;
+        const double t2 = (double)q.z - (double)x.z, t3 = (double)q.w - (double)x.w;
+        a += t0 * t0;
+        a += t1 * t1;
+        a += t2 * t2;
+        a += t3 * t3;
+    }
+};
+template <> struct Chunk16<double> {
+    using type = double2;
+    __device__ static type zero() { return make_double2(0.0, 0.0); }
+    __device__ static void acc(double &a, const type &q, const type &x) {
+        const double t0 = q.x - x.x, t1 = q.y - x.y;
+        a += t0 * t0;
+        a += t1 * t1;
+    }
+};
+
+// The same scan for rows of whole 16-byte chunks (dc chunks, at most 8 * NJ): eight lanes
+// share a vector and read it chunk by chunk (lane s takes chunks s, s + 8, ...), so a wave
+// takes eight vectors per iteration with a quarter (fp32) or half (fp64) of the load
+// instructions; the query's chunks stay in registers for the whole list. Distances
+// accumulate in double, chunk by chunk.
+template <typename T, int NJ>
+__global__ __launch_bounds__(256) void ivf_scan_chunk_kernel(
+    const T *__restrict__ Q, int nq, int n_probe, int dc, const T *__restrict__ X,
     const int *__restrict__ perm, const int *__restrict__ list_off,
     const int *__restrict__ probes, const int *__restrict__ seg, int64_t ld,
-    float *__restrict__ keys, int *__restrict__ idx) {
+    T *__restrict__ keys, int *__restrict__ idx) {
+    using CT = Chunk16<T>;
+    using ch_t = typename CT::type;
     constexpr int G = 8;
     const int lane = threadIdx.x & 63;
     const int sub = lane & (G - 1), slot = lane / G;
@@ -79,43 +107,34 @@ __global__ __launch_bounds__(256) void ivf_scan4_kernel(
     const int q = (int)(pair / n_probe);
     const int c = probes[pair];
     const int b = list_off[c], e = list_off[c + 1];
-    const float4 *qv = reinterpret_cast<const float4 *>(Q) + (int64_t)q * d4;
-    float4 qr[NJ];
+    const ch_t *qv = reinterpret_cast<const ch_t *>(Q) + (int64_t)q * dc;
+    ch_t qr[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
         const int ch = sub + G * j;
-        qr[j] = ch < d4 ? qv[ch] : make_float4(0.f, 0.f, 0.f, 0.f);
+        qr[j] = ch < dc ? qv[ch] : CT::zero();
     }
-    float *kr = keys + (int64_t)q * ld + seg[pair];
+    T *kr = keys + (int64_t)q * ld + seg[pair];
     int *ir = idx + (int64_t)q * ld + seg[pair];
     for (int r0 = b; r0 < e; r0 += 64 / G) {
         const int r = r0 + slot;
         const bool live = r < e;
         const int g = live ? perm[r] : 0;
-        const float4 *xv = reinterpret_cast<const float4 *>(X) + (int64_t)g * d4;
+        const ch_t *xv = reinterpret_cast<const ch_t *>(X) + (int64_t)g * dc;
         double acc = 0.0;
         if (live) {
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
                 const int ch = sub + G * j;
-                if (ch < d4) {
-                    const float4 x = xv[ch];
-                    const double t0 = (double)qr[j].x - (double)x.x;
-                    const double t1 = (double)qr[j].y - (double)x.y;
-                    const double t2 = (double)qr[j].z - (double)x.z;
-                    const double t3 = (double)qr[j].w - (double)x.w;
-                    acc += t0 * t0;
-                    acc += t1 * t1;
-                    acc += t2 * t2;
-                    acc += t3 * t3;
-                }
+                if (ch < dc)
+                    CT::acc(acc, qr[j], xv[ch]);
             }
         }
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1)
             acc += __shfl_xor(acc, off, G);
         if (live && sub == 0) {
-            kr[r - b] = (float)acc;
+            kr[r - b] = (T)acc;
             ir[r - b] = g;
         }
     }
@@ -168,23 +187,29 @@ hipError_t launch_ivf_scan(const T *Q, int nq, int n_probe, int d, const T *X, c
     if (pairs <= 0)
         return hipSuccess;
     const dim3 grid((unsigned)((pairs + 3) / 4));
-    if constexpr (std::is_same_v<T, float>) {
-        // 16-byte chunks need rows of whole chunks (d % 4 == 0; the buffers are allocated
-        // 256-byte aligned), at most 32 chunks.
-        const int d4 = d / 4;
-        if (d % 4 == 0 && d4 <= 32) {
-            auto run = [&](auto kern) {
-                hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, Q, nq, n_probe, d4, X, perm,
-                                   list_off, probes, seg, ld, keys, idx);
-                return hipGetLastError();
-            };
-            if (d4 <= 8)
-                return run(ivf_scan4_kernel<1>);
-            if (d4 <= 16)
-                return run(ivf_scan4_kernel<2>);
-            if (d4 <= 24)
-                return run(ivf_scan4_kernel<3>);
-            return run(ivf_scan4_kernel<4>);
+    // Rows of whole 16-byte chunks (d a multiple of 4 for fp32, of 2 for fp64; the buffers
+    // are allocated 256-byte aligned), at most 128 elements, take the chunked scan.
+    constexpr int per = 16 / (int)sizeof(T);
+    const int dc = d / per;
+    if (d % per == 0 && d <= 128) {
+        auto run = [&](auto kern) {
+            hipLaunchKernelGGL(kern, grid, dim3(256), 0, s, Q, nq, n_probe, dc, X, perm,
+                               list_off, probes, seg, ld, keys, idx);
+            return hipGetLastError();
+        };
+        switch ((dc + 7) / 8) {
+        case 1: return run(ivf_scan_chunk_kernel<T, 1>);
+        case 2: return run(ivf_scan_chunk_kernel<T, 2>);
+        case 3: return run(ivf_scan_chunk_kernel<T, 3>);
+        case 4: return run(ivf_scan_chunk_kernel<T, 4>);
+        }
+        if constexpr (per == 2) {
+            switch ((dc + 7) / 8) {
+            case 5: return run(ivf_scan_chunk_kernel<T, 5>);
+            case 6: return run(ivf_scan_chunk_kernel<T, 6>);
+            case 7: return run(ivf_scan_chunk_kernel<T, 7>);
+            case 8: return run(ivf_scan_chunk_kernel<T, 8>);
+            }
         }
     }
     hipLaunchKernelGGL(ivf_scan_kernel<T>, grid, dim3(256), 0, s, Q, nq, n_probe, d, X, perm,

