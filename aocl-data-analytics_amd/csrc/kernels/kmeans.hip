@@ -92,7 +92,45 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
         __syncthreads();
         if (!valid)
             continue;
-        for (int c = 0; c < nc; c++) {
+        int c = 0;
+        if constexpr (DMAX > 0) {
+            // Four centres at a time: each sample element is widened once for four
+            // distances. Each distance keeps its own accumulation order and the centres are
+            // compared in ascending order, so the result is that of one centre at a time.
+            for (; c + 4 <= nc; c += 4) {
+                const double *p = Cs + c * d;
+                double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+#pragma unroll
+                for (int l = 0; l < DMAX; l++) {
+                    if (l < d) {
+                        const double xl = (double)x[l];
+                        const double t0 = xl - p[l], t1 = xl - p[d + l];
+                        const double t2 = xl - p[2 * d + l], t3 = xl - p[3 * d + l];
+                        a0 = fma(t0, t0, a0);
+                        a1 = fma(t1, t1, a1);
+                        a2 = fma(t2, t2, a2);
+                        a3 = fma(t3, t3, a3);
+                    }
+                }
+                if (a0 < best) {
+                    best = a0;
+                    bl = c0 + c;
+                }
+                if (a1 < best) {
+                    best = a1;
+                    bl = c0 + c + 1;
+                }
+                if (a2 < best) {
+                    best = a2;
+                    bl = c0 + c + 2;
+                }
+                if (a3 < best) {
+                    best = a3;
+                    bl = c0 + c + 3;
+                }
+            }
+        }
+        for (; c < nc; c++) {
             double acc = 0.0;
             if constexpr (DMAX > 0) {
 #pragma unroll
