@@ -211,8 +211,8 @@ int oracle_knn(int n, int d, const double *Xt, int nq, const double *Xq, int k, 
 }
 
 /* ---------------------------------------------------------------------------------------
- * k-means (Lloyd; Elkan gives the same iterates), restated from
- * source/core/clustering/kmeans/kmeans.cpp and kmeans_lloyd.hpp. Data row-major n x d.
+ * k-means, restated from source/core/clustering/kmeans/kmeans.cpp, kmeans_lloyd.hpp and
+ * kmeans_elkan.hpp. Data row-major n x d, centres row-major k x d, all arithmetic double.
  */
 struct KMOut {
     int n_iter;
@@ -228,13 +228,38 @@ static double sqd(const double *x, const double *c, int d) {
     return s;
 }
 
+/* Rounding to the lower precision of the mixed-precision phase (kmeans.cpp:1824-1876):
+ * mode 1 rounds to float (the phase of double data), mode 2 to IEEE half through float
+ * (the phase of float data), both to nearest even. */
+static double round_lp(double v, int mode) {
+    if (mode == 0)
+        return v;
+    const float f = (float)v;
+    if (mode == 1 || !std::isfinite(f))
+        return f;
+    const double a = std::fabs((double)f);
+    double q;
+    if (a >= 65520.0)
+        q = std::numeric_limits<double>::infinity();
+    else if (a < 6.103515625e-05)              // below the smallest normal half: quantum 2^-24
+        q = std::nearbyint(a * 16777216.0) / 16777216.0;
+    else {
+        int e;
+        std::frexp(a, &e);                     // a in [2^(e-1), 2^e): 11 significant bits
+        const double quantum = std::ldexp(1.0, e - 11);
+        q = std::nearbyint(a / quantum) * quantum;
+    }
+    return std::copysign(q, (double)f);
+}
+
+double oracle_round_lp(double v, int mode) { return round_lp(v, mode); }
+
 /* lloyd_iteration (kmeans_lloyd.hpp:105-271): labels by argmin with ties to the lowest
- * centre (lloyd_iteration_kernel_scalar, kmeans_kernels.cpp:209-260), sums per label,
- * scale_current_cluster_centres :363-405 (empty -> count 1 -> origin), shift
- * compute_centre_shift kmeans.cpp:1334-1355. Returns the number of changed labels. */
+ * centre (lloyd_iteration_kernel_scalar, kmeans_kernels.cpp:209-260); with update, the sums
+ * per label and scale_current_cluster_centres :363-405 (empty -> count 1 -> origin).
+ * Returns the number of changed labels. */
 static int lloyd_iter(int n, int d, int k, const double *X, const double *Cprev, double *Cnew,
-                      int *lab, const int *lab_prev, bool update, double *normc2,
-                      double *shift2) {
+                      int *lab, const int *lab_prev, bool update) {
     int changed = 0;
     for (int i = 0; i < n; i++) {
         double best = std::numeric_limits<double>::infinity();
@@ -261,15 +286,131 @@ static int lloyd_iter(int n, int d, int k, const double *X, const double *Cprev,
         for (int c = 0; c < k; c++)
             for (int l = 0; l < d; l++)
                 Cnew[(size_t)c * d + l] /= (cnt[c] == 0 ? 1 : cnt[c]);
-        double a = 0, b = 0;
-        for (size_t t = 0; t < (size_t)k * d; t++) {
-            a += Cprev[t] * Cprev[t];
-            b += (Cprev[t] - Cnew[t]) * (Cprev[t] - Cnew[t]);
-        }
-        *normc2 = a;
-        *shift2 = b;
     }
     return changed;
+}
+
+/* Elkan's bounds (kmeans_elkan.hpp:143-566, kmeans_kernels.cpp:116-135). u: upper bound of
+ * each sample's distance to its centre; l: n x k lower bounds; cc: half centre-to-centre
+ * distances; nxt: each centre's half distance to its nearest other centre. */
+struct Elkan {
+    std::vector<double> u, l, cc, nxt;
+};
+
+/* compute_centre_half_distances, kmeans_elkan.hpp:515-566 (distances, not squares). */
+static void elkan_half_distances(int k, int d, const double *C, Elkan &e) {
+    e.cc.assign((size_t)k * k, 0.0);
+    e.nxt.assign(k, std::numeric_limits<double>::infinity());
+    for (int j = 0; j < k; j++)
+        for (int i = 0; i < j; i++) {
+            const double t = 0.5 * std::sqrt(sqd(C + (size_t)i * d, C + (size_t)j * d, d));
+            e.cc[(size_t)i * k + j] = e.cc[(size_t)j * k + i] = t;
+            e.nxt[i] = std::min(e.nxt[i], t);
+            e.nxt[j] = std::min(e.nxt[j], t);
+        }
+}
+
+/* init_elkan, kmeans_elkan.hpp:143-233: centre 0 first, a centre j is measured only when the
+ * best distance so far exceeds half the distance between the best centre and j; strict <. */
+static void elkan_init(int n, int d, int k, const double *X, const double *C, int *lab,
+                       Elkan &e) {
+    elkan_half_distances(k, d, C, e);
+    e.u.assign(n, 0.0);
+    e.l.assign((size_t)n * k, 0.0);
+    for (int i = 0; i < n; i++) {
+        const double *x = X + (size_t)i * d;
+        int label = 0;
+        double sd = std::sqrt(sqd(x, C, d));
+        e.l[(size_t)i * k] = sd;
+        for (int j = 1; j < k; j++) {
+            e.l[(size_t)i * k + j] = 0.0;
+            if (sd > e.cc[(size_t)label * k + j]) {
+                const double dist = std::sqrt(sqd(x, C + (size_t)j * d, d));
+                e.l[(size_t)i * k + j] = dist;
+                if (dist < sd) {
+                    label = j;
+                    sd = dist;
+                }
+            }
+        }
+        lab[i] = label;
+        e.u[i] = sd;
+    }
+}
+
+/* elkan_iteration_assign_block, kmeans_elkan.hpp:427-513: samples whose upper bound exceeds
+ * the half distance to the next centre test each other centre j against its lower bound and
+ * the half distance; the first such test tightens the upper bound; a measured distance
+ * replaces the label only when strictly smaller. With update, the sums per label as in
+ * Lloyd's. */
+static int elkan_assign(int n, int d, int k, const double *X, const double *Cprev,
+                        double *Cnew, int *lab, const int *lab_prev, bool update, Elkan &e) {
+    int changed = 0;
+    for (int i = 0; i < n; i++) {
+        const double *x = X + (size_t)i * d;
+        double *lb = e.l.data() + (size_t)i * k;
+        int label = lab_prev[i];
+        double u = e.u[i];
+        bool tight = false;
+        if (u > e.nxt[label]) {
+            for (int j = 0; j < k; j++) {
+                const double l_j = lb[j];
+                const double h = e.cc[(size_t)label * k + j];
+                if (j != label && u > l_j && u > h) {
+                    if (!tight) {
+                        u = std::sqrt(sqd(x, Cprev + (size_t)label * d, d));
+                        lb[label] = u;
+                        tight = true;
+                    }
+                    if (u > l_j || u > h) {
+                        const double dist = std::sqrt(sqd(x, Cprev + (size_t)j * d, d));
+                        lb[j] = dist;
+                        if (dist < u) {
+                            u = dist;
+                            label = j;
+                        }
+                    }
+                }
+            }
+        }
+        e.u[i] = u;
+        if (label != lab_prev[i])
+            changed++;
+        lab[i] = label;
+    }
+    if (update) {
+        std::vector<int> cnt(k, 0);
+        std::fill(Cnew, Cnew + (size_t)k * d, 0.0);
+        for (int i = 0; i < n; i++) {
+            cnt[lab[i]]++;
+            for (int l = 0; l < d; l++)
+                Cnew[(size_t)lab[i] * d + l] += X[(size_t)i * d + l];
+        }
+        for (int c = 0; c < k; c++)
+            for (int l = 0; l < d; l++)
+                Cnew[(size_t)c * d + l] /= (cnt[c] == 0 ? 1 : cnt[c]);
+    }
+    return changed;
+}
+
+/* The bound update after the centres moved (elkan_iteration, kmeans_elkan.hpp:372-418, and
+ * elkan_iteration_kernel_scalar, kmeans_kernels.cpp:116-131): u += shift of the sample's
+ * centre, l_j = max(l_j - shift_j, 0), then the half distances of the new centres. */
+static void elkan_update(int n, int d, int k, const double *Cprev, const double *Cnew,
+                         const int *lab, Elkan &e) {
+    std::vector<double> shift(k);
+    for (int c = 0; c < k; c++)
+        shift[c] = std::sqrt(sqd(Cprev + (size_t)c * d, Cnew + (size_t)c * d, d));
+    for (int i = 0; i < n; i++) {
+        e.u[i] += shift[lab[i]];
+        for (int j = 0; j < k; j++) {
+            double &v = e.l[(size_t)i * k + j];
+            v -= shift[j];
+            if (v < 0)
+                v = 0.0;
+        }
+    }
+    elkan_half_distances(k, d, Cnew, e);
 }
 
 /* handle_empty_clusters, kmeans.cpp:1127-1330 (split mode in double arithmetic). Returns
@@ -319,30 +460,54 @@ static int empty_clusters(int n, int d, int k, const double *X, double *C, int *
     return 1;
 }
 
-/* perform_kmeans, kmeans.cpp:996-1058, and compute_current_inertia :1060-1124. C holds the
- * initial centres on entry and the final centres on exit. */
+/* perform_kmeans, kmeans.cpp:996-1058, convergence_test :1358-1397 and
+ * compute_current_inertia :1060-1124, for Lloyd (algorithm 0) or Elkan (1). C holds the
+ * initial centres on entry and the final centres on exit. lp_mode != 0 runs the phase of
+ * mixed-precision refinement in the lower precision (kmeans.cpp:1824-1876): X is already
+ * rounded, and the centres are rounded after every update, as kmeans<float> or
+ * kmeans<_Float16> stores them. */
 static KMOut perform(int n, int d, int k, const double *X, double *C, int *lab, int max_iter,
-                     double tol, int empty_mode) {
+                     double tol, int empty_mode, int algorithm, int lp_mode) {
     std::vector<double> Cp((size_t)k * d);
     std::vector<int> lp(n, 0);
     std::vector<double> Cc(C, C + (size_t)k * d);
     std::vector<int> lc(n, 0);
     KMOut o{0, 0, 0.0, 0};
+    const bool elkan = algorithm == 1;
+    Elkan e;
+    if (elkan)
+        elkan_init(n, d, k, X, Cc.data(), lc.data(), e);
     int conv = 0, it;
-    double normc2 = 0, shift2 = 0;
     for (it = 0; it < max_iter; it++) {
         std::swap(Cp, Cc);
         std::swap(lp, lc);
-        const int changed = lloyd_iter(n, d, k, X, Cp.data(), Cc.data(), lc.data(), lp.data(), true,
-                                       &normc2, &shift2);
+        const int changed =
+            elkan ? elkan_assign(n, d, k, X, Cp.data(), Cc.data(), lc.data(), lp.data(), true, e)
+                  : lloyd_iter(n, d, k, X, Cp.data(), Cc.data(), lc.data(), lp.data(), true);
+        for (auto &v : Cc)
+            v = round_lp(v, lp_mode);
+        // compute_centre_shift, kmeans.cpp:1334-1355 (Frobenius norms).
+        double normc2 = 0, shift2 = 0;
+        for (size_t t = 0; t < (size_t)k * d; t++) {
+            normc2 += Cp[t] * Cp[t];
+            shift2 += (Cp[t] - Cc[t]) * (Cp[t] - Cc[t]);
+        }
+        if (elkan)
+            elkan_update(n, d, k, Cp.data(), Cc.data(), lc.data(), e);
         if (empty_mode != 0) {
             const int r = empty_clusters(n, d, k, X, Cc.data(), lc.data(), empty_mode);
             if (r == 2) {
                 o.empty_found = 1;
                 return o;
             }
-            if (r == 1)
+            if (r == 1) {
+                if (elkan) {
+                    // Bounds invalidated after a split (kmeans.cpp:1312-1317).
+                    std::fill(e.u.begin(), e.u.end(), std::numeric_limits<double>::max());
+                    std::fill(e.l.begin(), e.l.end(), 0.0);
+                }
                 continue;
+            }
         }
         conv = 0;
         if (it > 1 && changed == 0)
@@ -357,7 +522,10 @@ static KMOut perform(int n, int d, int k, const double *X, double *C, int *lab, 
     if (conv == 1 || it == max_iter) {
         std::swap(lp, lc);
         std::swap(Cp, Cc);
-        lloyd_iter(n, d, k, X, Cp.data(), nullptr, lc.data(), nullptr, false, nullptr, nullptr);
+        if (elkan)
+            elkan_assign(n, d, k, X, Cp.data(), nullptr, lc.data(), lp.data(), false, e);
+        else
+            lloyd_iter(n, d, k, X, Cp.data(), nullptr, lc.data(), nullptr, false);
         std::swap(Cp, Cc);
     }
     double inertia = 0;
@@ -427,23 +595,91 @@ static void kmeanspp(int n, int d, int k, const double *X, double *C, std::mt199
             C[(size_t)c * d + l] = X[(size_t)chosen[c] * d + l];
 }
 
+/* afk_mcmc_init, kmeans.cpp:1667-1811 (Bachem et al. 2016): the first centre uniform; the
+ * proposal q(x) = d(x,c1)^2 / (2 sum) + 1/(2n) (uniform when every distance is zero); for
+ * each further centre a Markov chain of m steps over draws from q, accepting y with
+ * probability min(1, d(y)q(x) / (d(x)q(y))), where d is the squared distance to the
+ * nearest centre chosen so far; a zero d(x)q(y) accepts without a uniform draw. */
+static void afkmc2(int n, int d, int k, const double *X, double *C, int m, std::mt19937_64 &mt) {
+    std::uniform_int_distribution<> dis_int(0, n - 1);
+    const int c1 = dis_int(mt);
+    std::copy(X + (size_t)c1 * d, X + (size_t)c1 * d + d, C);
+    std::vector<double> w(n), q(n);
+    double sum = 0;
+    for (int i = 0; i < n; i++) {
+        w[i] = std::max(0.0, sqd(X + (size_t)i * d, X + (size_t)c1 * d, d));
+        sum += w[i];
+    }
+    const double inv_2n = 1.0 / (2.0 * n);
+    for (int i = 0; i < n; i++)
+        q[i] = (std::isfinite(sum) && sum > 0) ? 0.5 * w[i] / sum + inv_2n : 1.0 / n;
+    auto nearest = [&](int idx, int nc) {
+        double best = std::numeric_limits<double>::infinity();
+        for (int c = 0; c < nc; c++)
+            best = std::min(best, sqd(X + (size_t)idx * d, C + (size_t)c * d, d));
+        return best;
+    };
+    std::discrete_distribution<> qd(q.begin(), q.end());
+    std::uniform_real_distribution<double> unif(0.0, 1.0);
+    for (int i = 1; i < k; i++) {
+        int x = qd(mt);
+        double dx = nearest(x, i);
+        for (int s = 0; s < m; s++) {
+            const int y = qd(mt);
+            const double dy = nearest(y, i);
+            const double dxqy = dx * q[y];
+            if (dxqy == 0.0 || (dy * q[x]) / dxqy > unif(mt)) {
+                x = y;
+                dx = dy;
+            }
+        }
+        std::copy(X + (size_t)x * d, X + (size_t)x * d + d, C + (size_t)i * d);
+    }
+}
+
+/* The initial centres of the first run (k-means++ init 1, AFK-MC2 init 4) for the given
+ * seed, as oracle_kmeans draws them; exposed so that tests can check the draws. */
+int oracle_kmeans_init(int n, int d, const double *X, int k, int init, long long seed,
+                       int afk_m, double *C_out) {
+    std::mt19937_64 mt((unsigned long long)seed);
+    if (init == 1)
+        kmeanspp(n, d, k, X, C_out, mt);
+    else if (init == 4)
+        afkmc2(n, d, k, X, C_out, afk_m, mt);
+    else
+        return -1;
+    return 0;
+}
+
 /*
- * kmeans<T>::compute restated (kmeans.cpp:351-783) for Lloyd/Elkan with initialisation
- * 0 random (std::sample, :1404-1425), 1 k-means++, 2 supplied (C_init), 3 random
- * partitions (:1426-1457). empty_mode: 0 ignore, 1 error, 2 split. Returns 0 on success,
- * 900 when every run hit an empty cluster (error mode), 21 when the best run hit max_iter.
- * out_info = {n_iter, inertia}. Centres row-major k x d.
+ * kmeans<T>::compute restated (kmeans.cpp:351-783) for Lloyd (algorithm 0) or Elkan (1)
+ * with initialisation 0 random (std::sample, :1404-1425), 1 k-means++, 2 supplied (C_init),
+ * 3 random partitions (:1426-1457), 4 AFK-MC2 with afk_m chain steps. empty_mode: 0
+ * ignore, 1 error, 2 split. lp_mode: 0 none, 1 mixed precision of double data (float
+ * phase), 2 of float data (half phase), with lp_tol and lp_max_iter. Returns 0 on success,
+ * 900 when every run hit an empty cluster (error mode), 21 when the best run hit max_iter,
+ * 41 when lp_tol <= tol (incompatible options). out_info = {n_iter, inertia, lp_n_iter}.
+ * Centres row-major k x d.
  */
 int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *C_init,
                   int n_init, int max_iter, double tol, long long seed, int empty_mode,
+                  int algorithm, int afk_m, int lp_mode, double lp_tol, int lp_max_iter,
                   double *C_out, int *labels_out, double *out_info) {
     if (n_init > 1 && init == 2)
         n_init = 1;
+    if (lp_mode != 0 && lp_tol <= tol)
+        return 41;
     std::mt19937_64 mt;
     if (init != 2)
         mt.seed((unsigned long long)seed);
     std::vector<double> C((size_t)k * d);
     std::vector<int> lab(n);
+    std::vector<double> Xlp;
+    if (lp_mode != 0) {
+        Xlp.resize((size_t)n * d);
+        for (size_t t = 0; t < Xlp.size(); t++)
+            Xlp[t] = round_lp(X[t], lp_mode);
+    }
     double best = std::numeric_limits<double>::infinity();
     bool valid = false, warn = false;
     for (int run = 0; run < n_init; run++) {
@@ -469,10 +705,26 @@ int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *
             for (int c = 0; c < k; c++)
                 for (int l = 0; l < d; l++)
                     C[(size_t)c * d + l] /= cnt[c];
+        } else if (init == 4) {
+            afkmc2(n, d, k, X, C.data(), afk_m, mt);
         } else {
             kmeanspp(n, d, k, X, C.data(), mt);
         }
-        KMOut o = perform(n, d, k, X, C.data(), lab.data(), max_iter, tol, empty_mode);
+        int lp_iter = 0;
+        if (lp_mode != 0) {
+            for (auto &v : C)
+                v = round_lp(v, lp_mode);
+            std::vector<int> lab_lp(n);
+            KMOut lo = perform(n, d, k, Xlp.data(), C.data(), lab_lp.data(), lp_max_iter, lp_tol,
+                               empty_mode, algorithm, lp_mode);
+            if (lo.empty_found)
+                return 900;
+            lp_iter = lo.n_iter;
+            for (auto &v : C)
+                v = round_lp(v, lp_mode);
+        }
+        KMOut o = perform(n, d, k, X, C.data(), lab.data(), max_iter, tol, empty_mode, algorithm,
+                          0);
         if (o.empty_found)
             continue;
         valid = true;
@@ -483,6 +735,7 @@ int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *
             std::copy(lab.begin(), lab.end(), labels_out);
             out_info[0] = o.n_iter;
             out_info[1] = o.inertia;
+            out_info[2] = lp_iter;
         }
     }
     if (!valid)

@@ -33,7 +33,13 @@ _lib.oracle_knn.argtypes = [C.c_int, C.c_int, _d, C.c_int, _d, C.c_int, C.c_int,
                             _i, _d, C.c_double]
 _lib.oracle_kmeans.restype = C.c_int
 _lib.oracle_kmeans.argtypes = [C.c_int, C.c_int, _d, C.c_int, C.c_int, _d, C.c_int, C.c_int,
-                               C.c_double, C.c_longlong, C.c_int, _d, _i, _d]
+                               C.c_double, C.c_longlong, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_double, C.c_int, _d, _i, _d]
+_lib.oracle_kmeans_init.restype = C.c_int
+_lib.oracle_kmeans_init.argtypes = [C.c_int, C.c_int, _d, C.c_int, C.c_int, C.c_longlong,
+                                    C.c_int, _d]
+_lib.oracle_round_lp.restype = C.c_double
+_lib.oracle_round_lp.argtypes = [C.c_double, C.c_int]
 
 
 def _p(a, t=_d):
@@ -234,21 +240,44 @@ def ivf_search(C, X, Xq, k, n_probe, metric="sqeuclidean", single=False):
     return ind, dist, sizes
 
 
-INIT = {"random": 0, "k-means++": 1, "supplied": 2, "random partitions": 3}
+INIT = {"random": 0, "k-means++": 1, "supplied": 2, "random partitions": 3, "afk-mc2": 4}
 EMPTY = {"ignore": 0, "error": 1, "split": 2}
+ALGORITHM = {"lloyd": 0, "elkan": 1}
 
 
 def kmeans(X, n_clusters, init="k-means++", C=None, n_init=1, max_iter=300, tol=1e-4, seed=0,
-           empty_clusters="ignore"):
-    """Returns dict(status, centres, labels, n_iter, inertia) of the restated k-means."""
+           empty_clusters="ignore", algorithm="lloyd", afk_samples=50, mixed_precision=None,
+           lp_tol=1e-2, lp_max_iter=200):
+    """Returns dict(status, centres, labels, n_iter, inertia, lp_n_iter) of the restated
+    k-means. mixed_precision: None, "float" (the lower-precision phase of double data) or
+    "half" (that of float data)."""
     X = np.ascontiguousarray(X, dtype=np.float64)
     n, d = X.shape
     Ci = None if C is None else np.ascontiguousarray(C, dtype=np.float64)
     Co = np.zeros((n_clusters, d))
     lab = np.zeros(n, dtype=np.int32)
-    info = np.zeros(2)
+    info = np.zeros(3)
+    lp = {None: 0, "float": 1, "half": 2}[mixed_precision]
     st = _lib.oracle_kmeans(n, d, _p(X), n_clusters, INIT[init], _p(Ci), n_init, max_iter,
-                            float(tol), int(seed), EMPTY[empty_clusters], _p(Co), _p(lab, _i),
-                            _p(info))
+                            float(tol), int(seed), EMPTY[empty_clusters], ALGORITHM[algorithm],
+                            int(afk_samples), lp, float(lp_tol), int(lp_max_iter), _p(Co),
+                            _p(lab, _i), _p(info))
     return {"status": st, "centres": Co, "labels": lab, "n_iter": int(info[0]),
-            "inertia": float(info[1])}
+            "inertia": float(info[1]), "lp_n_iter": int(info[2])}
+
+
+def kmeans_init(X, n_clusters, init="k-means++", seed=0, afk_samples=50):
+    """Initial centres (n_clusters x d) of the first run, k-means++ or AFK-MC2."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Co = np.zeros((n_clusters, X.shape[1]))
+    st = _lib.oracle_kmeans_init(X.shape[0], X.shape[1], _p(X), n_clusters, INIT[init],
+                                 int(seed), int(afk_samples), _p(Co))
+    assert st == 0
+    return Co
+
+
+def round_lp(x, mode):
+    """The oracle's rounding to the lower precision of mixed-precision refinement (mode 1
+    float, 2 half through float), elementwise."""
+    f = np.vectorize(lambda v: _lib.oracle_round_lp(float(v), int(mode)))
+    return f(np.asarray(x, dtype=np.float64))

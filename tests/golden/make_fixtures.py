@@ -365,6 +365,109 @@ known["dbscan_errors"] = {
     "A_colmajor": [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12], "n_samples": 4, "n_features": 3,
     "lda": 4,
 }
+# k-means cases of the reference whose answers are bounds or iteration counts
+# (kmeans_test_data.hpp). "base" names the entry of known["kmeans"] whose data, initial
+# centres (for "supplied") and options are reused; the fields given here override them.
+# Random initialisations are checked as the reference checks them (kmeans_public.cpp:272-276):
+# inertia <= max_inertia + check_tol with check_tol = 100*sqrt(eps) of the test's type.
+_3c = "three clusters, lloyd"
+known["kmeans_random"] = [
+    dict(base=_3c, source="kmeans_test_data.hpp:604-624", init=i, algorithm="elkan",
+         n_init=10, max_iter=30, seed=78, n_clusters=3, max_inertia=0.185475)
+    for i in ("k-means++", "random", "random partitions", "afk-mc2")
+] + [
+    dict(base=_3c, source="kmeans_test_data.hpp:625-636", init="k-means++", algorithm=a,
+         n_init=1, max_iter=30, seed=78, n_clusters=1, max_inertia=34.89176)
+    for a in ("elkan", "lloyd")
+] + [
+    dict(base=_3c, source="kmeans_test_data.hpp:643-649", init="k-means++", algorithm="lloyd",
+         n_init=1, max_iter=30, seed=78, n_clusters=10, max_inertia=0.11),
+    dict(base=_3c, source="kmeans_test_data.hpp:655-663", init="random partitions",
+         algorithm="elkan", n_init=5, max_iter=30, seed=78, n_clusters=10, max_inertia=0.11),
+    dict(base=_3c, source="kmeans_test_data.hpp:664-672", init="k-means++", algorithm="lloyd",
+         n_init=10, max_iter=30, seed=78, n_clusters=2, max_inertia=15.7915238095238),
+    dict(base=_3c, source="kmeans_test_data.hpp:673-676", init="afk-mc2", algorithm="lloyd",
+         n_init=10, max_iter=30, seed=78, n_clusters=2, max_inertia=15.7915238095238),
+    dict(base=_3c, source="kmeans_test_data.hpp:677-680", init="random", algorithm="elkan",
+         n_init=10, max_iter=30, seed=78, n_clusters=2, max_inertia=15.7915238095238),
+    dict(base=_3c, source="kmeans_test_data.hpp:690-695", init="k-means++", algorithm="lloyd",
+         n_init=10, max_iter=30, seed=78, n_clusters=5, max_inertia=0.076),
+    dict(base=_3c, source="kmeans_test_data.hpp:696-701", init="random", algorithm="elkan",
+         n_init=10, max_iter=30, seed=78, n_clusters=5, max_inertia=0.11),
+    dict(base="three clusters, row-major", source="kmeans_test_data.hpp:923-942",
+         init="k-means++", algorithm="elkan", n_init=1, max_iter=50, seed=78, n_clusters=3,
+         max_inertia=0.185475),
+    dict(base="three clusters, row-major", source="kmeans_test_data.hpp:929-933",
+         init="afk-mc2", algorithm="elkan", n_init=1, max_iter=50, seed=78, n_clusters=3,
+         max_inertia=0.185475),
+    dict(base="three clusters, row-major", source="kmeans_test_data.hpp:934-942",
+         init="random", algorithm="elkan", n_init=1, max_iter=50, seed=12345, n_clusters=3,
+         max_inertia=0.185475),
+    dict(base="three clusters, row-major", source="kmeans_test_data.hpp:939-942",
+         init="random partitions", algorithm="elkan", n_init=1, max_iter=50, seed=12345,
+         n_clusters=3, max_inertia=0.185475),
+    dict(base="pseudorandom 20 x 2", source="kmeans_test_data.hpp:530-556,746-755",
+         init="random", algorithm="elkan", n_init=1, max_iter=30, seed=593228, n_clusters=3,
+         max_inertia=4.8, check_tol="1000*sqrt(eps)"),
+]
+# The row-major 3-cluster data (kmeans_test_data.hpp:310-370) is the column-major data of
+# known["kmeans"][0] transposed; its expected answers are the same.
+known["kmeans_extra_data"] = {
+    "three clusters, row-major": {
+        "source": "kmeans_test_data.hpp:310-370", "order": "row-major",
+        "A_rowmajor": [1.0, 1.0, 1.1, 1.2, 0.5, -2.0, 0.49, -1.9, -2.0, 0.5,
+                       -2.0, 0.51, 0.53, -2.1, 0.9, 0.95, 1.2, 0.8, -1.8, 0.6],
+        "C_rowmajor": [0.5, 0.5, 0.7, -1.7, -1.3, 0.2],
+        "n_samples": 10, "n_features": 2},
+    "pseudorandom 20 x 2": {
+        "source": "kmeans_test_data.hpp:530-556", "order": "column-major",
+        "A_colmajor": [0.31, 0.61, 0.65, -0.49, -0.7, -0.35, 0.53, 0.29, 0.23, -0.58, 1.0,
+                       0.23, -0.04, -0.79, 0.25, 0.51, -0.41, 0.95, -0.81, -0.61, -0.41, 0.12,
+                       0.75, 0.49, -0.47, -0.85, -0.53, 0.19, 0.25, -0.82, 0.52, -0.26, -0.01,
+                       -0.49, 0.56, 0.51, -0.61, 0.95, 0.83, -0.76],
+        "n_samples": 20, "n_features": 2},
+}
+# Deterministic cases with the Elkan algorithm: the reference expects the same answers as
+# the base entry (kmeans_test_data.hpp:600-603 three clusters, :757-766 the larger matrix,
+# :785-797 empty cluster split, :918-922 row-major), i.e. rinfo n_iter 1, 4, 1 and 1.
+known["kmeans_elkan"] = [
+    {"base": "three clusters, lloyd", "source": "kmeans_test_data.hpp:600-603"},
+    {"base": "larger data matrix, lloyd", "source": "kmeans_test_data.hpp:757-766"},
+    {"base": "empty cluster, split", "source": "kmeans_test_data.hpp:785-797"},
+]
+# Cases with every answer fixed (is_random false): all-zero 5 x 3 data, k = 2, max_iter 3
+# (kmeans_test_data.hpp:195-248,718-744), expected rinfo {5, 3, 2, 2, 0, 0}, centres and
+# transform zero, labels 0; and the 1 x 1 matrix (:88-139,558-584), rinfo {1, 1, 1, 0, 0, 0}.
+known["kmeans_fixed_random_init"] = [
+    dict(name="zero data", source="kmeans_test_data.hpp:195-248,718-728", init="k-means++",
+         algorithm="elkan", n_init=3, seed=-1, n_samples=5, n_features=3, n_clusters=2,
+         max_iter=3, tol=1e-4, A_colmajor=[0.0] * 15, rinfo=[5.0, 3.0, 2.0, 2.0, 0.0, 0.0],
+         centres_colmajor=[0.0] * 6, labels=[0] * 5, X_colmajor=[0.0] * 9, m_samples=3,
+         X_transform_colmajor=[0.0] * 6, Y_colmajor=[0.0] * 6, k_samples=2, Y_labels=[0, 0]),
+] + [
+    dict(name="zero data " + i, source="kmeans_test_data.hpp:729-737", init=i,
+         algorithm="lloyd", n_init=3, seed=-1, n_samples=5, n_features=3, n_clusters=2,
+         max_iter=3, tol=1e-4, A_colmajor=[0.0] * 15, rinfo=[5.0, 3.0, 2.0, 2.0, 0.0, 0.0],
+         centres_colmajor=[0.0] * 6, labels=[0] * 5, X_colmajor=[0.0] * 9, m_samples=3,
+         X_transform_colmajor=[0.0] * 6, Y_colmajor=[0.0] * 6, k_samples=2, Y_labels=[0, 0])
+    for i in ("random", "afk-mc2")
+] + [
+    dict(name="1 by 1 " + i + " " + a, source="kmeans_test_data.hpp:88-139,558-584", init=i,
+         algorithm=a, n_init=1, seed=78, n_samples=1, n_features=1, n_clusters=1,
+         max_iter=30, tol=1e-4, A_colmajor=[2.1], rinfo=[1.0, 1.0, 1.0, 0.0, 0.0, 0.0],
+         centres_colmajor=[2.1], labels=[0], X_colmajor=[3.3], m_samples=1,
+         X_transform_colmajor=[1.2], Y_colmajor=[1.3], k_samples=1, Y_labels=[0])
+    for i, a in (("k-means++", "elkan"), ("random", "elkan"), ("random", "lloyd"),
+                 ("random partitions", "lloyd"))
+]
+# Mixed-precision iterative refinement (kmeans_test_data.hpp:944-969): rinfo[3] (iterations
+# in the working precision) and rinfo[5] (iterations of the lower-precision phase).
+known["kmeans_mixed"] = [
+    dict(base="three clusters, lloyd", source="kmeans_test_data.hpp:947-957", lp_max_iter=1,
+         lp_tol=1e-2, n_iter={"double": 0, "float": 1}, lp_n_iter=1),
+    dict(base="larger data matrix, lloyd", source="kmeans_test_data.hpp:958-968",
+         lp_max_iter=3, lp_tol=1e-3, n_iter={"double": 1, "float": 1}, lp_n_iter=3),
+]
 with open(os.path.join(HERE, "known_answers.json"), "w") as f:
     json.dump(known, f, indent=1)
 print("fixtures written to", HERE)

@@ -156,6 +156,149 @@ def test_kmeanspp_quality_and_determinism(oracle):
     # well separated blobs: k-means++ finds every blob, so the inertia is close to n*d*0.09
     assert a["inertia"] < 1.5 * 800 * 3 * 0.09
 
+def _random_case(known, c):
+    from _kmeans_cases import base_data
+    A, _, _, _ = base_data(known, c["base"])
+    return A
+
+
+@pytest.mark.parametrize("i", range(17))
+def test_kmeans_reference_random_inits(known, oracle, i):
+    """The reference's checks of random initialisations (kmeans_public.cpp:272-276): the
+    best inertia of n_init runs is at most the stated bound."""
+    from _kmeans_cases import check_tol
+    c = known["kmeans_random"][i]
+    A = _random_case(known, c)
+    r = oracle.kmeans(A, c["n_clusters"], c["init"], n_init=c["n_init"],
+                      max_iter=c["max_iter"], seed=c["seed"], algorithm=c["algorithm"])
+    assert r["status"] in (0, 21)
+    assert r["inertia"] <= c["max_inertia"] + check_tol(c.get("check_tol", "100*sqrt(eps)"),
+                                                        np.float64)
+
+
+@pytest.mark.parametrize("i", range(7))
+def test_kmeans_reference_fixed_random_inits(known, oracle, i):
+    """Random initialisations whose answers the reference fixes: all-zero data (every
+    initialisation coincides) and the 1 x 1 matrix."""
+    c = known["kmeans_fixed_random_init"][i]
+    n, d, k = c["n_samples"], c["n_features"], c["n_clusters"]
+    A = _colmajor(c["A_colmajor"], n, d)
+    r = oracle.kmeans(A, k, c["init"], n_init=c["n_init"], max_iter=c["max_iter"],
+                      tol=c["tol"], seed=abs(c["seed"]), algorithm=c["algorithm"])
+    assert r["status"] == 0
+    assert [n, d, k, r["n_iter"], r["inertia"], r["lp_n_iter"]] == c["rinfo"]
+    np.testing.assert_array_equal(r["centres"], _colmajor(c["centres_colmajor"], k, d))
+    np.testing.assert_array_equal(r["labels"], c["labels"])
+
+
+@pytest.mark.parametrize("i", range(3))
+def test_kmeans_reference_elkan(known, oracle, i):
+    """Elkan's algorithm returns the answers the reference expects of its base cases."""
+    from _kmeans_cases import base_data, check_tol
+    c = known["kmeans_elkan"][i]
+    A, C0, _, b = base_data(known, c["base"])
+    k, d = C0.shape
+    r = oracle.kmeans(A, k, "supplied", C0, max_iter=b["max_iter"], tol=b["tol"],
+                      empty_clusters=b["empty_clusters"], algorithm="elkan")
+    assert r["status"] == 0
+    tol = max(check_tol(b["tol_check"], np.float64), 1e-6)
+    assert r["n_iter"] == b["rinfo"][3]
+    assert abs(r["inertia"] - b["rinfo"][4]) <= tol
+    np.testing.assert_allclose(r["centres"], _colmajor(b["centres_colmajor"], k, d), atol=tol)
+    np.testing.assert_array_equal(r["labels"], b["labels"])
+
+
+@pytest.mark.parametrize("i", range(2))
+@pytest.mark.parametrize("dtype", ["double", "float"])
+def test_kmeans_reference_mixed_precision(known, oracle, i, dtype):
+    """Iteration counts of mixed-precision refinement (kmeans_test_data.hpp:944-969): the
+    lower-precision phase runs in float for double data and in half for float data."""
+    from _kmeans_cases import base_data
+    c = known["kmeans_mixed"][i]
+    A, C0, _, b = base_data(known, c["base"])
+    if dtype == "float":
+        A, C0 = A.astype(np.float32).astype(np.float64), C0.astype(np.float32).astype(np.float64)
+    r = oracle.kmeans(A, C0.shape[0], "supplied", C0, max_iter=b["max_iter"], tol=b["tol"],
+                      mixed_precision={"double": "float", "float": "half"}[dtype],
+                      lp_tol=c["lp_tol"], lp_max_iter=c["lp_max_iter"])
+    assert r["status"] == 0
+    assert r["lp_n_iter"] == c["lp_n_iter"]
+    assert r["n_iter"] == c["n_iter"][dtype]
+    # the refinement ends at the answer of the working precision alone
+    np.testing.assert_array_equal(r["labels"], b["labels"])
+
+
+def test_kmeans_mixed_precision_incompatible_tolerance(oracle):
+    X = np.arange(12.0).reshape(6, 2)
+    assert oracle.kmeans(X, 2, "supplied", X[:2], tol=1e-2, mixed_precision="float",
+                         lp_tol=1e-2)["status"] == 41
+
+
+def test_round_lp_matches_ieee(oracle):
+    """Rounding to float and to half (through float), round to nearest even, against numpy."""
+    rng = np.random.default_rng(3)
+    v = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-9, 6, 2000),
+                        [65504.0, 65519.99, 65520.0, 1e5, 6.1e-5, 2.0 ** -24, 3 * 2.0 ** -25,
+                         2.0 ** -26, 0.0, -0.0, 1.0 + 2.0 ** -11, 1.0 + 3 * 2.0 ** -11]])
+    np.testing.assert_array_equal(oracle.round_lp(v, 1), v.astype(np.float32).astype(np.float64))
+    with np.errstate(over="ignore"):
+        half = v.astype(np.float32).astype(np.float16).astype(np.float64)
+    np.testing.assert_array_equal(oracle.round_lp(v, 2), half)
+
+
+@pytest.mark.parametrize("seed", [0, 7, 78])
+@pytest.mark.parametrize("k", [2, 5, 12])
+def test_kmeanspp_draws_equal_independent_restatement(oracle, seed, k):
+    """On integer data (exact squared distances, as the reference's GEMM form also gives
+    there) the oracle's k-means++ picks exactly the rows of an independent restatement of
+    kmeans.cpp:1524-1664 over libstdc++'s engine and distributions: D^2 weights, 2 + ln k
+    trials, the cheapest candidate, first centre uniform."""
+    import _stdrng
+    rng = np.random.default_rng(seed + 100 * k)
+    X = rng.integers(-6, 7, size=(40, 3)).astype(np.float64)
+    ref = _stdrng.rows(X, _stdrng.kmeanspp(X, k, seed))
+    np.testing.assert_array_equal(oracle.kmeans_init(X, k, "k-means++", seed=seed), ref)
+
+
+@pytest.mark.parametrize("seed", [0, 78])
+@pytest.mark.parametrize("k,m", [(3, 1), (6, 5), (9, 50)])
+def test_afkmc2_draws_equal_independent_restatement(oracle, seed, k, m):
+    """The same check for AFK-MC2 (kmeans.cpp:1667-1811): the proposal, the chain of m steps
+    and its acceptance rule, including the acceptance without a draw when d(x) is zero."""
+    import _stdrng
+    rng = np.random.default_rng(seed + 10 * k + m)
+    X = rng.integers(-5, 6, size=(30, 2)).astype(np.float64)
+    X[5] = X[6]  # a duplicate row: a chain at an existing centre has d(x) = 0
+    ref = _stdrng.rows(X, _stdrng.afkmc2(X, k, seed, m))
+    np.testing.assert_array_equal(oracle.kmeans_init(X, k, "afk-mc2", seed=seed,
+                                                     afk_samples=m), ref)
+
+
+def test_kmeanspp_picks_every_distinct_point(oracle):
+    """With k equal to the number of distinct rows, k-means++ picks each of them once (a
+    chosen row has weight zero), so the centres are the distinct rows."""
+    rng = np.random.default_rng(11)
+    P = rng.integers(-20, 20, size=(7, 4)).astype(np.float64)
+    X = P[rng.integers(0, 7, size=60)]
+    X[:7] = P
+    C = oracle.kmeans_init(X, 7, "k-means++", seed=3)
+    assert sorted(map(tuple, C)) == sorted(map(tuple, P))
+
+
+def test_elkan_equals_lloyd_without_ties(oracle):
+    """Elkan's bounds only skip distances that cannot change the label, so on data without
+    equidistant centres it follows Lloyd's iterates exactly."""
+    for seed, (n, d, k) in enumerate([(400, 3, 7), (300, 8, 20), (250, 2, 40)]):
+        rng = np.random.default_rng(seed)
+        X = rng.standard_normal((n, d)) + rng.integers(0, 4, size=(n, 1)) * 3.0
+        C0 = X[rng.choice(n, k, replace=False)]
+        a = oracle.kmeans(X, k, "supplied", C0, max_iter=60, tol=1e-6, algorithm="lloyd")
+        b = oracle.kmeans(X, k, "supplied", C0, max_iter=60, tol=1e-6, algorithm="elkan")
+        np.testing.assert_array_equal(a["labels"], b["labels"])
+        np.testing.assert_array_equal(a["centres"], b["centres"])
+        assert a["n_iter"] == b["n_iter"]
+
+
 def _split(counts, flat):
     out, o = [], 0
     for c in counts:
