@@ -83,7 +83,30 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
     }
     double best = __builtin_huge_val();
     int bl = 0;
-    const int cc = CS_ELEMS / d > 0 ? CS_ELEMS / d : 1;
+    if (DMAX == 0 && d > CS_ELEMS) {
+        // A centre does not fit the LDS stage: read the centres from global memory (L2).
+        if (valid) {
+            for (int c = 0; c < k; c++) {
+                const double *p = C + (int64_t)c * d;
+                double acc = 0.0;
+                for (int l = 0; l < d; l++) {
+                    const double t = (double)X[(int64_t)i * d + l] - p[l];
+                    acc = fma(t, t, acc);
+                }
+                if (acc < best) {
+                    best = acc;
+                    bl = c;
+                }
+            }
+            labels[i] = bl;
+            if (mind)
+                mind[i] = best;
+            if (prev && prev[i] != bl)
+                atomicAdd(changed, 1);
+        }
+        return;
+    }
+    const int cc = CS_ELEMS / d;
     for (int c0 = 0; c0 < k; c0 += cc) {
         const int nc = (k - c0) < cc ? (k - c0) : cc;
         __syncthreads();
@@ -407,18 +430,37 @@ __device__ __forceinline__ double sqdist_rows(const T *__restrict__ X, int d, in
     return acc;
 }
 
+// Squared distance of rows i and j of X, widened to double (the row j read from global
+// memory: used where a row does not fit the LDS stage).
+template <typename T>
+__device__ __forceinline__ double sqdist_xx(const T *__restrict__ X, int d, int64_t i, int64_t j) {
+    double acc = 0.0;
+    for (int l = 0; l < d; l++) {
+        const double t = (double)X[i * d + l] - (double)X[j * d + l];
+        acc = fma(t, t, acc);
+    }
+    return acc;
+}
+
+// Rows of at most ROW_STAGE_MAX features are staged in LDS by the distance passes of the
+// initialisations; longer rows are read from global memory.
+constexpr int ROW_STAGE_MAX = 8192;
+
 template <typename T>
 __global__ __launch_bounds__(256) void dist_to_row_kernel(const T *__restrict__ X, int n, int d,
                                                           int ci, double *__restrict__ w,
                                                           int mode) {
     extern __shared__ double cs[];
-    for (int t = threadIdx.x; t < d; t += 256)
-        cs[t] = (double)X[(int64_t)ci * d + t];
-    __syncthreads();
+    const bool staged = d <= ROW_STAGE_MAX;
+    if (staged) {
+        for (int t = threadIdx.x; t < d; t += 256)
+            cs[t] = (double)X[(int64_t)ci * d + t];
+        __syncthreads();
+    }
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n)
         return;
-    double v = sqdist_rows(X, d, i, cs);
+    double v = staged ? sqdist_rows(X, d, i, cs) : sqdist_xx(X, d, i, ci);
     if (mode == 1) {
         // min update: max(0, min(w, v))
         const double o = w[i];
@@ -531,13 +573,16 @@ __global__ __launch_bounds__(256) void candidate_cost_kernel(const T *__restrict
                                                              const double *__restrict__ w,
                                                              const int *__restrict__ cand, int nt,
                                                              double *__restrict__ part, int nb) {
-    extern __shared__ double cs[]; // nt x d
+    extern __shared__ double cs[]; // nt x d when staged
     __shared__ double red[256];
-    for (int t = threadIdx.x; t < nt * d; t += 256) {
-        const int c = t / d, l = t - c * d;
-        cs[t] = (double)X[(int64_t)cand[c] * d + l];
+    const bool staged = (int64_t)nt * d <= ROW_STAGE_MAX;
+    if (staged) {
+        for (int t = threadIdx.x; t < nt * d; t += 256) {
+            const int c = t / d, l = t - c * d;
+            cs[t] = (double)X[(int64_t)cand[c] * d + l];
+        }
+        __syncthreads();
     }
-    __syncthreads();
     double acc[NT_MAX];
 #pragma unroll
     for (int c = 0; c < NT_MAX; c++)
@@ -547,7 +592,8 @@ __global__ __launch_bounds__(256) void candidate_cost_kernel(const T *__restrict
 #pragma unroll
         for (int c = 0; c < NT_MAX; c++) {
             if (c < nt) {
-                double v = sqdist_rows(X, d, i, cs + c * d);
+                double v = staged ? sqdist_rows(X, d, i, cs + c * d)
+                                  : sqdist_xx(X, d, i, (int64_t)cand[c]);
                 v = wi < v ? wi : v;
                 acc[c] += v > 0.0 ? v : 0.0;
             }
@@ -715,16 +761,22 @@ hipError_t launch_gather_rows(const T *X, int d, const int *idx, int m, double *
     return hipGetLastError();
 }
 
+// Dynamic LDS of a pass that stages `elems` doubles: at most ROW_STAGE_MAX (64 KB, within
+// the default limit), none when the rows are read from global memory instead.
+static size_t row_stage_bytes(int64_t elems) {
+    return elems <= ROW_STAGE_MAX ? (size_t)elems * sizeof(double) : 0;
+}
+
 template <typename T>
 hipError_t launch_dist_to_row(const T *X, int n, int d, int ci, double *w, hipStream_t s) {
     hipLaunchKernelGGL(dist_to_row_kernel<T>, dim3((n + 255) / 256), dim3(256),
-                       (size_t)d * sizeof(double), s, X, n, d, ci, w, 0);
+                       row_stage_bytes(d), s, X, n, d, ci, w, 0);
     return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_min_update(const T *X, int n, int d, int ci, double *w, hipStream_t s) {
     hipLaunchKernelGGL(dist_to_row_kernel<T>, dim3((n + 255) / 256), dim3(256),
-                       (size_t)d * sizeof(double), s, X, n, d, ci, w, 1);
+                       row_stage_bytes(d), s, X, n, d, ci, w, 1);
     return hipGetLastError();
 }
 
@@ -750,7 +802,7 @@ hipError_t launch_candidate_cost(const T *X, int n, int d, const double *w, cons
     if (nt > NT_MAX)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(candidate_cost_kernel<T>, dim3(nb), dim3(256),
-                       (size_t)nt * d * sizeof(double), s, X, n, d, w, cand, nt, part, nb);
+                       row_stage_bytes((int64_t)nt * d), s, X, n, d, w, cand, nt, part, nb);
     return hipGetLastError();
 }
 

@@ -398,3 +398,33 @@ int kb_assign_blocks_per_cu(int prec, int pt);
 hipError_t kb_set_partial_lds(size_t lds);
 
 } // namespace dagpu
+
+namespace dagpu {
+// ---- Elkan's k-means bounds (kmeans_elkan.hip) ----
+// cc[i*k+j] = |C_i - C_j| / 2, nxt[i] = min_{j != i} cc[i*k+j].
+hipError_t launch_elkan_half(const double *C, int k, int d, double *cc, double *nxt,
+                             hipStream_t s);
+// Initial labels and bounds: lab (n), u (n), lb (n x k).
+template <typename T>
+hipError_t launch_elkan_init(const T *X, int n, int d, const double *C, int k, const double *cc,
+                             int *lab, double *u, double *lb, hipStream_t s);
+// One bounded assignment from the labels prev; counts changed labels into *changed.
+template <typename T>
+hipError_t launch_elkan_assign(const T *X, int n, int d, const double *C, int k, const double *cc,
+                               const double *nxt, const int *prev, int *lab, double *u,
+                               double *lb, int *changed, hipStream_t s);
+// Centre shifts (shift, k doubles), then u += shift[lab], lb = max(lb - shift, 0).
+hipError_t launch_elkan_bounds(const double *Cprev, const double *Cnew, int n, int k, int d,
+                               const int *lab, double *shift, double *u, double *lb,
+                               hipStream_t s);
+// u = DBL_MAX, lb = 0.
+hipError_t launch_elkan_reset(int n, int k, double *u, double *lb, hipStream_t s);
+// ---- AFK-MC2 seeding ----
+// Proposal q[i] = w[i]/(2 sum) + 1/(2n), or 1/n for a zero or non-finite sum.
+hipError_t launch_afk_proposal(const double *w, int n, const double *sum, double *q,
+                               hipStream_t s);
+// One Markov chain of m steps over the canonical draws u (at least 2m + 1): out[0] = the
+// chosen sample, out[1] = draws consumed.
+hipError_t launch_afk_chain(const double *cp, const double *q, const double *dmin, int n, int m,
+                            const double *u, int *out, hipStream_t s);
+} // namespace dagpu

@@ -304,17 +304,39 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
                                             mind_.as<double>(), st));
     }
     if (update) {
-        KernelScope ks(*gpu, "kmeans_update");
-        for (int c0 = 0; c0 < k; c0 += kc_) {
-            const int kcc = std::min<int>(kc_, k - c0);
-            DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n, d, lab_new, c0, kcc, escale_.as<int>(),
-                                                      ppb_, nblocks_, part_.as<long long>(),
-                                                      pcnt_.as<int>(), st));
-            DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), c0, kcc, d,
-                                             escale_.as<int>(), Cout, counts_.as<int>(), st));
-        }
-        DA_HIP(err, launch_kmeans_shift(Cin, Cout, k * d, small_.as<double>(), st));
+        da_status s = update_centres<U>(dX, lab_new, Cin, Cout);
+        if (s != da_status_success)
+            return s;
     }
+    return read_step(changed, normc2, shift2);
+}
+
+template <typename T>
+template <typename U>
+da_status KMeansHandle<T>::update_centres(const U *dX, const int *lab_new, const double *Cin,
+                                          double *Cout) {
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    KernelScope ks(*gpu, "kmeans_update");
+    for (int c0 = 0; c0 < k; c0 += kc_) {
+        const int kcc = std::min<int>(kc_, k - c0);
+        DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n, d, lab_new, c0, kcc, escale_.as<int>(),
+                                                  ppb_, nblocks_, part_.as<long long>(),
+                                                  pcnt_.as<int>(), st));
+        DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), c0, kcc, d,
+                                         escale_.as<int>(), Cout, counts_.as<int>(), st));
+    }
+    // The low-precision phase stores its centres in the lower precision (kmeans<float> or
+    // kmeans<_Float16> in kmeans.cpp:1824-1876); the move is measured after the rounding.
+    if (lp_mode_ != 0)
+        DA_HIP(err, launch_round_through(Cout, (int64_t)k * d, lp_mode_, st));
+    DA_HIP(err, launch_kmeans_shift(Cin, Cout, k * d, small_.as<double>(), st));
+    return da_status_success;
+}
+
+template <typename T>
+da_status KMeansHandle<T>::read_step(int &changed, double &normc2, double &shift2) {
+    hipStream_t st = gpu->stream();
     double h[5];
     DA_HIP(err, hipMemcpyAsync(h, small_.ptr(), 5 * sizeof(double), hipMemcpyDeviceToHost, st));
     DA_HIP(err, hipStreamSynchronize(st));
@@ -325,6 +347,35 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
     normc2 = h[0];
     shift2 = h[1];
     return da_status_success;
+}
+
+// One iteration of Elkan's algorithm (elkan_iteration, kmeans_elkan.hpp:372-513): the
+// bounded assignment from lab_old, then (with update) the centre update, the bound update
+// with the centre shifts and the half distances of the new centres.
+template <typename T>
+template <typename U>
+da_status KMeansHandle<T>::elkan_step(const U *dX, const double *Cin, double *Cout, int *lab_new,
+                                      const int *lab_old, bool update, int &changed,
+                                      double &normc2, double &shift2) {
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    int *dchanged = reinterpret_cast<int *>(small_.as<double>() + 4);
+    DA_HIP(err, hipMemsetAsync(dchanged, 0, 2 * sizeof(int), st));
+    {
+        KernelScope ks(*gpu, "kmeans_assign");
+        DA_HIP(err, launch_elkan_assign<U>(dX, n, d, Cin, k, ecc_.as<double>(), enxt_.as<double>(),
+                                           lab_old, lab_new, eu_.as<double>(), el_.as<double>(),
+                                           dchanged, st));
+    }
+    if (update) {
+        da_status s = update_centres<U>(dX, lab_new, Cin, Cout);
+        if (s != da_status_success)
+            return s;
+        DA_HIP(err, launch_elkan_bounds(Cin, Cout, n, k, d, lab_new, eshift_.as<double>(),
+                                        eu_.as<double>(), el_.as<double>(), st));
+        DA_HIP(err, launch_elkan_half(Cout, k, d, ecc_.as<double>(), enxt_.as<double>(), st));
+    }
+    return read_step(changed, normc2, shift2);
 }
 
 template <typename T>
@@ -431,10 +482,29 @@ da_status KMeansHandle<T>::run_lloyd(const U *dX, double tol, da_int max_iter, R
     int changed = 0;
     double normc2 = 0, shift2 = 0;
     res = RunResult{};
+    auto step = [&](const double *Cin, double *Cout, int *lab_new, const int *lab_old,
+                    bool update) {
+        return elkan_ ? elkan_step<U>(dX, Cin, Cout, lab_new, lab_old, update, changed, normc2,
+                                      shift2)
+                      : lloyd_step<U>(dX, Cin, Cout, lab_new, lab_old, update, changed, normc2,
+                                      shift2);
+    };
+    if (elkan_) {
+        // init_elkan (kmeans_elkan.hpp:143-233): labels and bounds of the initial centres.
+        const int n = n_samples_, d = n_features_, k = n_clusters_;
+        DA_HIP(err, eu_.reserve((size_t)n * sizeof(double)));
+        DA_HIP(err, el_.reserve((size_t)n * k * sizeof(double)));
+        DA_HIP(err, ecc_.reserve((size_t)k * k * sizeof(double)));
+        DA_HIP(err, enxt_.reserve((size_t)k * sizeof(double)));
+        DA_HIP(err, eshift_.reserve((size_t)k * sizeof(double)));
+        DA_HIP(err, launch_elkan_half(Ccur_, k, d, ecc_.as<double>(), enxt_.as<double>(), st));
+        DA_HIP(err, launch_elkan_init<U>(dX, n, d, Ccur_, k, ecc_.as<double>(), lab_cur_,
+                                         eu_.as<double>(), el_.as<double>(), st));
+    }
     for (it = 0; it < max_iter; it++) {
         std::swap(Cprev_, Ccur_);
         std::swap(lab_prev_, lab_cur_);
-        da_status s = lloyd_step<U>(dX, Cprev_, Ccur_, lab_cur_, lab_prev_, true, changed, normc2, shift2);
+        da_status s = step(Cprev_, Ccur_, lab_cur_, lab_prev_, true);
         if (s != da_status_success)
             return s;
         if (empty_mode_ != EMPTY_IGNORE) {
@@ -446,8 +516,13 @@ da_status KMeansHandle<T>::run_lloyd(const U *dX, double tol, da_int max_iter, R
                 res.empty_found = true;
                 return da_status_success;
             }
-            if (split)
+            if (split) {
+                // The bounds do not hold after a split (kmeans.cpp:1312-1317).
+                if (elkan_)
+                    DA_HIP(err, launch_elkan_reset(n_samples_, n_clusters_, eu_.as<double>(),
+                                                   el_.as<double>(), st));
                 continue;
+            }
         }
         // convergence_test, kmeans.cpp:1359-1397.
         conv = 0;
@@ -463,7 +538,8 @@ da_status KMeansHandle<T>::run_lloyd(const U *dX, double tol, da_int max_iter, R
     if (conv == 1 || it == max_iter) {
         std::swap(lab_prev_, lab_cur_);
         std::swap(Cprev_, Ccur_);
-        da_status s = lloyd_step<U>(dX, Cprev_, nullptr, lab_cur_, nullptr, false, changed, normc2, shift2);
+        da_status s = elkan_ ? step(Cprev_, nullptr, lab_cur_, lab_prev_, false)
+                             : step(Cprev_, nullptr, lab_cur_, nullptr, false);
         if (s != da_status_success)
             return s;
         std::swap(Cprev_, Ccur_);
@@ -599,10 +675,65 @@ template <typename T> da_status KMeansHandle<T>::initialise_centres(da_int metho
     }
     case INIT_KMEANSPP:
         return kmeans_plusplus();
+    case INIT_AFK_MCMC:
+        return afk_mc2();
     default:
-        return err->error(da_status_not_implemented,
-                          "The AFK-MC2 initialization method is not implemented by the MI355X build.");
+        return err->error(da_status_invalid_option, "Unknown initialization method.");
     }
+}
+
+// AFK-MC2 (afk_mcmc_init, kmeans.cpp:1667-1811): the first centre uniform; the proposal
+// q(x) = d(x, c1)^2 / (2 sum) + 1/(2n) sampled through std::discrete_distribution; each further
+// centre is the end of a Markov chain of "afk-mc2 samples" steps whose acceptance compares
+// the squared distances to the nearest chosen centre. The engine draws are made on the host
+// with the same libstdc++ engine and distributions; the distance passes, the proposal, its
+// cumulative distribution and the chain itself run on the GPU. A chain takes one canonical
+// draw per proposal and one per acceptance test that needs a uniform, so the host prepares
+// the most a chain can use (2m + 1) and then advances its engine by the number consumed.
+template <typename T> da_status KMeansHandle<T>::afk_mc2() {
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    const T *dX = X_.as<T>();
+    const int m = (int)opts.ival("afk-mc2 samples");
+    std::vector<int> chosen(k, 0);
+    std::uniform_int_distribution<> dis_int(0, n - 1);
+    chosen[0] = dis_int(mt_);
+    // w: squared distance of each sample to its nearest chosen centre.
+    double *w = w_.as<double>();
+    DA_HIP(err, launch_dist_to_row<T>(dX, n, d, chosen[0], w, st));
+    if (k > 1) {
+        double *total = small_.as<double>() + 3;
+        int *dout = reinterpret_cast<int *>(small_.as<double>() + 10);
+        DA_HIP(err, aq_.reserve((size_t)n * sizeof(double)));
+        DA_HIP(err, au_.reserve((size_t)(2 * m + 1) * sizeof(double)));
+        double *q = aq_.as<double>(), *cp = cp_.as<double>();
+        DA_HIP(err, launch_sum_double(w, n, tmp_.as<double>(), total, st));
+        DA_HIP(err, launch_afk_proposal(w, n, total, q, st));
+        // std::discrete_distribution: the weights normalised by their sum, accumulated, the
+        // last entry set to 1.
+        DA_HIP(err, launch_sum_double(q, n, tmp_.as<double>(), total, st));
+        DA_HIP(err, launch_normalised_scan(q, n, total, cp, tmp_.as<double>(), st));
+        std::vector<double> u(2 * (size_t)m + 1);
+        for (int c = 1; c < k; c++) {
+            const std::mt19937_64 saved = mt_;
+            for (auto &v : u)
+                v = std::generate_canonical<double, std::numeric_limits<double>::digits>(mt_);
+            DA_HIP(err, hipMemcpyAsync(au_.ptr(), u.data(), u.size() * sizeof(double),
+                                       hipMemcpyHostToDevice, st));
+            DA_HIP(err, launch_afk_chain(cp, q, w, n, m, au_.as<double>(), dout, st));
+            int out[2] = {0, 0};
+            DA_HIP(err, hipMemcpyAsync(out, dout, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+            DA_HIP(err, hipStreamSynchronize(st));
+            mt_ = saved;
+            mt_.discard((unsigned long long)out[1]);
+            chosen[c] = out[0];
+            DA_HIP(err, launch_min_update<T>(dX, n, d, out[0], w, st));
+        }
+    }
+    DA_HIP(err, hipMemcpyAsync(idx_.ptr(), chosen.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
+    DA_HIP(err, launch_gather_rows<T>(dX, d, idx_.as<int>(), k, Ccur_, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    return da_status_success;
 }
 
 template <typename T> da_status KMeansHandle<T>::compute() {
@@ -623,6 +754,8 @@ template <typename T> da_status KMeansHandle<T>::compute() {
     const bool mixed = opts.skey("mixed precision") == 1;
     empty_mode_ = opts.skey("empty clusters");
     const bool spherical = opts.skey("distance") == 1;
+    elkan_ = algorithm == KM_ELKAN;
+    lp_mode_ = 0;
     opts.rebound_int("n_clusters", std::numeric_limits<da_int>::max(), UpperBound::p_inf);
 
     da_status status = da_status_success;
@@ -736,7 +869,9 @@ template <typename T> da_status KMeansHandle<T>::compute() {
             DA_HIP(err, launch_round_data<T>(X_.as<T>(), (int64_t)n * d, mode, Xlp_.as<T>(), st));
             DA_HIP(err, launch_round_through(Ccur_, (int64_t)k * d, mode, st));
             RunResult lp;
+            lp_mode_ = mode;
             s = run_lloyd<T>(Xlp_.as<T>(), lp_tol, lp_max, lp);
+            lp_mode_ = 0;
             if (s != da_status_success)
                 return s;
             if (lp.empty_found)

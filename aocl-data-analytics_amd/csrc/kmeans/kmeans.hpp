@@ -33,12 +33,24 @@ template <typename T> class KMeansHandle : public AlgHandle {
     da_status upload_data(const T *src, da_int ld, DeviceBuffer &dst);
     da_status initialise_centres(da_int method);
     da_status kmeans_plusplus();
+    da_status afk_mc2();
+    // One run from the centres in Ccur_: Lloyd's iteration, or Elkan's when elkan_ is set.
     template <typename U>
     da_status run_lloyd(const U *dX, double tol, da_int max_iter, RunResult &res);
     template <typename U>
     da_status lloyd_step(const U *dX, const double *Cin, double *Cout, int *lab_new,
                          const int *lab_old, bool update, int &changed, double &normc2,
                          double &shift2);
+    template <typename U>
+    da_status elkan_step(const U *dX, const double *Cin, double *Cout, int *lab_new,
+                         const int *lab_old, bool update, int &changed, double &normc2,
+                         double &shift2);
+    // Centre update from the labels lab_new (exact sums, then rounding to the lower
+    // precision in the low-precision phase) and the Frobenius norms of the move.
+    template <typename U>
+    da_status update_centres(const U *dX, const int *lab_new, const double *Cin, double *Cout);
+    // Reads back the changed-label count, the re-check count and the norms of the move.
+    da_status read_step(int &changed, double &normc2, double &shift2);
     template <typename U> da_status centres_from_labels(const U *dX, const int *labels, double *C);
     template <typename U>
     da_status handle_empty_clusters(const U *dX, double *C, int *labels, bool &split,
@@ -79,6 +91,15 @@ template <typename T> class KMeansHandle : public AlgHandle {
 
   private:
     da_int empty_mode_ = 0;
+    // Rounding of the centres after each update: 0 none, 1 float, 2 half (the low-precision
+    // phase of mixed-precision refinement).
+    int lp_mode_ = 0;
+    // Elkan's bounds: upper bounds (n), lower bounds (n x k), half centre distances (k x k),
+    // each centre's half distance to its nearest other centre (k), centre shifts (k).
+    bool elkan_ = false;
+    DeviceBuffer eu_, el_, ecc_, enxt_, eshift_;
+    // AFK-MC2: proposal distribution (n) and the chain's draws.
+    DeviceBuffer aq_, au_;
     double *Ccur_ = nullptr, *Cprev_ = nullptr;
     int *lab_cur_ = nullptr, *lab_prev_ = nullptr;
 

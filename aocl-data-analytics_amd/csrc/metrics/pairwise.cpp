@@ -4,6 +4,7 @@
 #include "../core/gpu.hpp"
 #include "../kernels/launch.hpp"
 #include "aoclda_metrics.h"
+#include "pairwise_common.hpp"
 #include <algorithm>
 #include <cfloat>
 #include <limits>
@@ -47,61 +48,12 @@ da_status upload_rows(ErrorStack *err, hipStream_t st, da_order order, da_int m,
 template <typename T>
 da_status pairwise(da_order order, da_int m, da_int n, da_int k, const T *X, da_int ldx,
                    const T *Y, da_int ldy, T *D, da_int ldd, T p, da_metric metric) {
-    // Argument checks in the order of pairwise_distances.cpp:85-111.
-    if (m < 1 || k < 1)
-        return da_status_invalid_array_dimension;
-    if (order == column_major && ldx < m)
-        return da_status_invalid_leading_dimension;
-    if (order == row_major && ldx < k)
-        return da_status_invalid_leading_dimension;
-    if (X == nullptr || D == nullptr)
-        return da_status_invalid_pointer;
-    if (Y != nullptr) {
-        if (n < 1)
-            return da_status_invalid_array_dimension;
-        if (order == column_major) {
-            if (ldy < n || ldd < m)
-                return da_status_invalid_leading_dimension;
-        } else {
-            if (ldy < k || ldd < n)
-                return da_status_invalid_leading_dimension;
-        }
-    } else {
-        if (ldd < m)
-            return da_status_invalid_leading_dimension;
-    }
-    if (metric == da_minkowski && p <= 0)
-        return da_status_invalid_input;
-
-    int km;
-    switch (metric) {
-    case da_euclidean:
-        km = KM_EUCLIDEAN;
-        break;
-    case da_sqeuclidean:
-        km = KM_SQEUCLIDEAN;
-        break;
-    case da_manhattan:
-        km = KM_MANHATTAN;
-        break;
-    case da_cosine:
-        km = KM_COSINE;
-        break;
-    case da_minkowski:
-        km = (p == (T)2.0) ? KM_EUCLIDEAN : (p == (T)1.0) ? KM_MANHATTAN : KM_MINKOWSKI;
-        break;
-    case da_euclidean_gemm:
-        km = KM_EUCLIDEAN_GEMM;
-        break;
-    case da_sqeuclidean_gemm:
-        km = KM_SQEUCLIDEAN_GEMM;
-        break;
-    case da_inner_product:
-        km = KM_INNER;
-        break;
-    default:
+    da_status chk = pairwise_check<T>(order, m, n, k, X, ldx, Y, ldy, D, ldd, p, metric);
+    if (chk != da_status_success)
+        return chk;
+    const int km = pairwise_kmetric<T>(metric, p);
+    if (km < 0)
         return da_status_not_implemented;
-    }
 
     PairwiseState &S = state();
     std::lock_guard<std::mutex> lock(S.mu);
@@ -130,7 +82,7 @@ da_status pairwise(da_order order, da_int m, da_int n, da_int k, const T *X, da_
     }
 
     double *dnx = nullptr, *dny = nullptr;
-    if (km == KM_COSINE || km == KM_EUCLIDEAN_GEMM || km == KM_SQEUCLIDEAN_GEMM) {
+    if (pairwise_needs_norms(km)) {
         DA_HIP(err, S.nx.reserve((size_t)m * sizeof(double)));
         DA_HIP(err, launch_row_sqnorms<T>(dX, k, m, k, S.nx.as<double>(), s));
         dnx = S.nx.as<double>();
@@ -148,11 +100,7 @@ da_status pairwise(da_order order, da_int m, da_int n, da_int k, const T *X, da_
                 DA_HIP(err, launch_cosine_norms(dny, n, eps, s));
         }
     }
-    // The reference forces a zero diagonal for cosine (cosine_distance.cpp:57-61) and for
-    // the GEMM form of X against itself (euclidean_gemm_distance.cpp:196-226); for the
-    // direct forms the diagonal is exactly zero already.
-    const int zero_diag = x_is_y && (km == KM_COSINE || km == KM_EUCLIDEAN_GEMM ||
-                                     km == KM_SQEUCLIDEAN_GEMM);
+    const int zero_diag = pairwise_zero_diag(km, x_is_y);
 
     // Row-major output is computed as rows of X against Y. Column-major output is the
     // transposed problem (rows of Y against X, written row by row), which keeps the device

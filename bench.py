@@ -69,6 +69,45 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
+def cpu_baseline(Xt, Xq, k, ei, args):
+    """Times the CPU restatement of the reference's blocked brute-force kNN
+    (oracle/cpu_knn.cpp, built -O3 -march=native -fopenmp for this host) on the host cores:
+    all usable physical cores and one thread, one warm-up then the median of 5 runs, each on a
+    bounded sample of the queries against the whole training set. Its neighbours are checked
+    against the fp64 oracle's on the checked sample."""
+    from oracle import cpu_baseline as cb
+    phys = cb.physical_cores()
+    usable = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = min(x for x in (phys, usable, int(env) if env and env.isdigit() else None) if x)
+    nt, d = Xt.shape
+
+    def timed(nq_s, th):
+        cb.knn(Xt, Xq[:nq_s], k, th)
+        runs = []
+        for _ in range(5):
+            t = time.perf_counter()
+            cb.knn(Xt, Xq[:nq_s], k, th)
+            runs.append(time.perf_counter() - t)
+        return nq_s / float(np.median(runs))
+
+    s_all = min(args.cpu_sample_all, len(Xq))
+    qps_all = timed(s_all, threads)
+    s_one = min(args.cpu_sample_one, len(Xq))
+    qps_one = timed(s_one, 1)
+    ind, _ = cb.knn(Xt, Xq[:len(ei)], k, threads)
+    log(f"cpu baseline: {qps_all:.0f} q/s on {threads} threads, {qps_one:.0f} q/s on 1")
+    return {"value": qps_all, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"median of 5 runs (after 1 warm-up) of {s_all} queries x {nt} training "
+                      f"rows ({d} features, fp32), on {threads} threads; the restated blocked "
+                      "host algorithm of the reference (packed direct-difference tiles with its "
+                      "MR/NR/MC/NC/KC, 512 x 1024 kNN blocks, replace-the-maximum top-k), built "
+                      "-O3 -march=native -fopenmp",
+            "one_thread_qps": qps_one, "one_thread_sample": f"{s_one} queries",
+            "physical_cores": phys, "usable_cpus": usable,
+            "vector_bits": cb.vector_bits(), "index_match_oracle": bool((ind == ei).all())}
+
+
 def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1, seed=True):
     prec, kc_min = split_form(precision)
     nn = aoclda.neighbors.nearest_neighbors(n_neighbors=k, precision=prec, tile_variant=variant,
@@ -589,8 +628,12 @@ def main():
                     help="fused tile kernel variant (-1: tuning table)")
     ap.add_argument("--seed", action="store_true",
                     help="seed the tile thresholds from a sweep of a row subset")
+    ap.add_argument("--cpu-sample-all", type=int, default=4000,
+                    help="queries per timed run of the CPU baseline on all threads")
+    ap.add_argument("--cpu-sample-one", type=int, default=300,
+                    help="queries per timed run of the CPU baseline on one thread")
     ap.add_argument("--cpu-sample", type=int, default=1000,
-                    help="queries checked against (and timed with) the host oracle")
+                    help="queries checked against the fp64 host oracle")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (two launches), the "
                          "reference point that scales memory counters to bytes")
@@ -679,12 +722,9 @@ def main():
     s = min(args.cpu_sample, nq)
     if s > 0:
         from oracle import oracle
-        t = time.perf_counter()
+        # The checker: exact fp64 neighbours of the first s queries.
         ei, ed = oracle.knn(Xt.astype(np.float64), Xq[:s].astype(np.float64), k)
-        ct = time.perf_counter() - t
-        cpu = {"value": s / ct, "unit": "queries/s", "cores": 1, "kind": "port",
-               "sample": f"{s} queries x {args.n_train} training rows ({d} features), exact "
-                         "fp64 restatement of the brute-force search, 1 thread"}
+        cpu = cpu_baseline(Xt, Xq, k, ei, args) if args.cpu_sample_all > 0 else None
         for form, r in results.items():
             r["recall_at_10"] = sum(len(set(r["ind"][q]) & set(ei[q])) for q in range(s)) / (s * k)
             r["index_match"] = bool((r["ind"][:s] == ei).all())

@@ -15,11 +15,17 @@
  *     tests/unit_tests/kmeans/kmeans_test_data.hpp (3 clusters :250-308, large :444-528,
  *     empty clusters :141-193 incl. the split variant :789-805) and
  *     sklearn.cluster.KMeans(init=C, n_init=1, algorithm='lloyd') where no cluster
- *     empties. The k-means++ centre sequence (kmeans.cpp:1524-1664) is restated with the
- *     same libstdc++ engine and distributions; for it only the distribution of draws is
- *     pinned (inertia bound of kmeans_test_data.hpp:586-700 style tests), the exact
- *     sequence of the reference is "parity unpinned" because its weights carry BLAS
- *     rounding.
+ *     empties; Elkan by the reference's Elkan cases (:600-603) and by equality with Lloyd's
+ *     iterates where no two centres are equidistant; mixed-precision refinement by the
+ *     reference's iteration counts (:944-969) and an IEEE rounding check of round_lp.
+ *   - the random initialisations (k-means++ kmeans.cpp:1524-1664, AFK-MC2 :1667-1811,
+ *     random :1404-1425, random partitions :1426-1457): the reference's inertia bounds of
+ *     its random-initialisation cases (:604-716) and its fixed answers on all-zero data and
+ *     the 1 x 1 matrix (:195-248,718-745); the k-means++ and AFK-MC2 draws are pinned
+ *     against an independent restatement over libstdc++'s engine and distributions on
+ *     integer data (tests/_stdrng.py), where the reference's GEMM-form weights are exact
+ *     too. On general data the reference's weights carry BLAS rounding, so a draw that
+ *     falls within an ulp of a boundary of the cumulative weights can pick another sample.
  */
 #include <algorithm>
 #include <cmath>

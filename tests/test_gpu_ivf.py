@@ -102,3 +102,25 @@ def test_vs_oracle_chunked_rows(oracle, d, dtype, order):
             ei, ed, _ = oracle.ivf_search(C, X, Q, k, n_probe, "sqeuclidean", single=single)
             np.testing.assert_array_equal(ind, ei, err_msg=f"n_probe {n_probe} k {k}")
             np.testing.assert_allclose(dist, ed, rtol=1e-6 if single else 1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_vs_oracle_many_features(oracle, dtype):
+    """Rows of 5,000 features: the list assignment of add() reads centroids wider than its
+    LDS stage from global memory."""
+    rng = np.random.default_rng(77)
+    d = 5000
+    means = rng.uniform(-2, 2, (6, d))
+    X = (means[rng.integers(0, 6, 400)] + rng.standard_normal((400, d))).astype(dtype)
+    Q = (means[rng.integers(0, 6, 20)] + rng.standard_normal((20, d))).astype(dtype)
+    single = dtype == np.float32
+    ann = aoclda.neighbors.approximate_neighbors(n_list=6, kmeans_iter=5,
+                                                 seed=3).train_and_add(X)
+    C = np.asarray(ann.cluster_centroids, dtype=np.float64)
+    for n_probe in (1, 3):
+        ann.n_probe = n_probe
+        dist, ind = ann.kneighbors(Q, n_neighbors=5)
+        ei, ed, sizes = oracle.ivf_search(C, X, Q, 5, n_probe, "sqeuclidean", single=single)
+        np.testing.assert_array_equal(ind, ei, err_msg=f"n_probe {n_probe}")
+        np.testing.assert_allclose(dist, ed, rtol=1e-6 if single else 1e-12)
+    np.testing.assert_array_equal(ann.list_sizes, sizes)
