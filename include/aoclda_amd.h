@@ -10,6 +10,7 @@
 
 #include "aoclda_error.h"
 #include "aoclda_handle.h"
+#include "aoclda_metrics.h"
 #include "aoclda_types.h"
 
 #ifdef __cplusplus
@@ -43,6 +44,23 @@ da_status da_nn_kneighbors_device_d(da_handle handle, da_int n_queries, da_int n
                                     const double *d_X_test, da_int ldx_test,
                                     da_int *d_n_ind, double *d_n_dist, da_int k,
                                     da_int return_distance);
+
+/*
+ * Pairwise distances that allocate nothing. No counterpart in the reference: this is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Arguments, checks, metrics and results are those
+ * of da_pairwise_distances_s, with X, Y (may be NULL) and D in device memory; device memory
+ * for compact row-major copies of the operands (column-major input, or leading dimensions
+ * above k) and for row norms (cosine and the *_gemm metrics) comes from a fixed array of
+ * 32 MiB per GPU, held until the process ends: inputs whose copies and norms exceed it
+ * return da_status_invalid_input, host pointers return da_status_invalid_pointer. The work
+ * runs on the null stream and the call returns when it is done. Calls on one GPU share the
+ * array, so calls that overlap on one GPU (from several host threads) race.
+ */
+da_status da_pairwise_distances_device_noalloc_s(da_order order, da_int m, da_int n, da_int k,
+                                                 const float *d_X, da_int ldx,
+                                                 const float *d_Y, da_int ldy, float *d_D,
+                                                 da_int ldd, float p, da_metric metric);
 
 /*
  * Per-kernel timing. When enabled, every kernel launch of the handle is bracketed by HIP
