@@ -14,7 +14,8 @@ class nearest_neighbors:
                  check_data=False, precision="working", tile_variant=-1,
                  candidates_per_split=0, seed_thresholds=False):
         # precision: operand precision of the MFMA distance tiles ("working", "fp32",
-        # "fp16", "bf16", "fp16x3", "bf16x3"); results are re-ranked exactly whatever the
+        # "fp16", "bf16", "fp16x3", "bf16x3", "fp16c": fp16 rows stored about k-means group
+        # centres); results are re-ranked exactly whatever the
         # choice. candidates_per_split: candidate-list length per query and split of
         # the fused tile (0 = from k; shorter than k is allowed and stays exact).
         # seed_thresholds: bound the tile's thresholds with a sweep of

@@ -345,6 +345,7 @@ hipError_t launch_pack_rows_prec(const T *src, int64_t lds, int rows, int cols, 
         return launch_pack_rows<T, float>(src, lds, rows, cols, static_cast<float *>(dst), ldd,
                                           rows_pad, s);
     case 2: // KP_F16
+    case 6: // KP_F16C (the queries are plain fp16)
         return launch_pack_rows<T, _Float16>(src, lds, rows, cols, static_cast<_Float16 *>(dst),
                                              ldd, rows_pad, s);
     case 3: // KP_BF16

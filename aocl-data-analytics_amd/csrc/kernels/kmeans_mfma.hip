@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void kmeans_assign_bf16_kernel(
 // takes centres c, c + 64, ..., then a wave argmin by (distance, index).
 template <typename T>
 __global__ __launch_bounds__(256) void kmeans_recheck_kernel(
-    const T *__restrict__ X, int ldx, int d, const double *__restrict__ C, int k,
+    const T *__restrict__ X, int ldx, int d, const double *__restrict__ CT, int k,
     const int *__restrict__ list_count, const int *__restrict__ list, int *__restrict__ labels,
     const int *__restrict__ prev, int *__restrict__ changed) {
     const int cnt = *list_count;
@@ -314,10 +314,10 @@ __global__ __launch_bounds__(256) void kmeans_recheck_kernel(
         double best = __builtin_huge_val();
         int bl = INT_MAX;
         for (int c = lane; c < k; c += 64) {
-            const double *cc = C + (int64_t)c * d;
+            // Centres transposed (feature-major): the 64 lanes read 64 consecutive centres.
             double acc = 0.0;
             for (int l = 0; l < d; l++) {
-                const double v = (double)x[l] - cc[l];
+                const double v = (double)x[l] - CT[(int64_t)l * k + c];
                 acc = fma(v, v, acc);
             }
             if (acc < best) {
@@ -489,19 +489,32 @@ hipError_t launch_kmeans_assign_mfma(const float *X, int n, int Kp, const float 
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void transpose_centres_kernel(const double *__restrict__ C,
+                                                                int k, int d,
+                                                                double *__restrict__ CT) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (int64_t)k * d)
+        return;
+    const int c = (int)(t / d), l = (int)(t - (int64_t)c * d);
+    CT[(int64_t)l * k + c] = C[t];
+}
+
 template <typename T>
 hipError_t launch_kmeans_recheck(const T *X, int ldx, int d, const double *C, int k,
                                  const int *list_count, const int *list, int *labels,
-                                 const int *prev, int *changed, hipStream_t s) {
-    hipLaunchKernelGGL(kmeans_recheck_kernel<T>, dim3(1024), dim3(256), 0, s, X, ldx, d, C, k,
+                                 const int *prev, int *changed, double *CT, hipStream_t s) {
+    const int64_t kd = (int64_t)k * d;
+    hipLaunchKernelGGL(transpose_centres_kernel, dim3((unsigned)((kd + 255) / 256)), dim3(256), 0,
+                       s, C, k, d, CT);
+    hipLaunchKernelGGL(kmeans_recheck_kernel<T>, dim3(1024), dim3(256), 0, s, X, ldx, d, CT, k,
                        list_count, list, labels, prev, changed);
     return hipGetLastError();
 }
 template hipError_t launch_kmeans_recheck<float>(const float *, int, int, const double *, int,
                                                  const int *, const int *, int *, const int *,
-                                                 int *, hipStream_t);
+                                                 int *, double *, hipStream_t);
 template hipError_t launch_kmeans_recheck<double>(const double *, int, int, const double *, int,
                                                   const int *, const int *, int *, const int *,
-                                                  int *, hipStream_t);
+                                                  int *, double *, hipStream_t);
 
 } // namespace dagpu

@@ -183,6 +183,14 @@ struct OpF32 {
     __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static constexpr int FPC = 4; // fragments per 16-byte chunk
+    template <int CS, int NF> __device__ static void load_chunk(const in_t *p, int c, frag_t *f) {
+        const v4f x = *reinterpret_cast<const v4f *>(p + c * CS);
+        f[4 * c] = x[0];
+        f[4 * c + 1] = x[1];
+        f[4 * c + 2] = x[2];
+        f[4 * c + 3] = x[3];
+    }
     template <int NF, int CS> __device__ static void load(const in_t *p, frag_t *f) {
 #pragma unroll
         for (int v = 0; v < NF; v += 4) {
@@ -204,6 +212,12 @@ struct OpF64 {
     __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static constexpr int FPC = 2;
+    template <int CS, int NF> __device__ static void load_chunk(const in_t *p, int c, frag_t *f) {
+        const v2d x = *reinterpret_cast<const v2d *>(p + c * CS);
+        f[2 * c] = x[0];
+        f[2 * c + 1] = x[1];
+    }
     template <int NF, int CS> __device__ static void load(const in_t *p, frag_t *f) {
 #pragma unroll
         for (int v = 0; v < NF; v += 2) {
@@ -223,6 +237,10 @@ struct OpF16 {
     __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
+    static constexpr int FPC = 1;
+    template <int CS, int NF> __device__ static void load_chunk(const in_t *p, int c, frag_t *f) {
+        f[c] = *reinterpret_cast<const v8h *>(p + c * CS);
+    }
     template <int NF, int CS> __device__ static void load(const in_t *p, frag_t *f) {
 #pragma unroll
         for (int v = 0; v < NF; v++)
@@ -238,6 +256,10 @@ struct OpBF16 {
     static constexpr bool rows_strided = false;
     __device__ static acc_t mfma(frag_t a, frag_t b, acc_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+    static constexpr int FPC = 1;
+    template <int CS, int NF> __device__ static void load_chunk(const in_t *p, int c, frag_t *f) {
+        f[c] = *reinterpret_cast<const v8bf *>(p + c * CS);
     }
     template <int NF, int CS> __device__ static void load(const in_t *p, frag_t *f) {
 #pragma unroll
@@ -336,7 +358,8 @@ __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bke
                                             float *__restrict__ part_key,
                                             int *__restrict__ part_idx,
                                             float *__restrict__ part_tau,
-                                            const float *__restrict__ tau0) {
+                                            const float *__restrict__ tau0,
+                                            const int *__restrict__ perm) {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int rt = 0; rt < ROWT; rt++) {
@@ -354,7 +377,8 @@ __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bke
                 const int valid = c < KC ? c : KC;
                 for (int u = lane; u < KC; u += 64) {
                     part_key[o + u] = u < valid ? bkey[ql * CAP + u] : __builtin_huge_valf();
-                    part_idx[o + u] = u < valid ? t_begin + (int)bidx[ql * CAP + u] : -1;
+                    const int row = t_begin + (int)bidx[ql * CAP + u];
+                    part_idx[o + u] = u < valid ? (perm ? perm[row] : row) : -1;
                 }
                 if (lane == 0)
                     part_tau[(int64_t)q * nsplit + split] = tau0 ? fminf(t, tau0[q]) : t;
@@ -379,14 +403,17 @@ template <class Op, int KCH, int KC, bool SPLIT, int COLT> struct TileCfg {
 // so no workgroup barrier is needed inside the sweep. SPLIT operands carry each value as
 // hi + lo in the 16-bit type (query rows stored as [hi Kp][lo Kp]); the product is
 // hi*hi + hi*lo + lo*hi. Training operands are loaded from the tiled layout straight into
-// registers (PF: the next step's loads are issued right after the current step's MFMAs and
-// overlap its candidate bookkeeping; otherwise each step loads, then computes).
-template <class Op, int KCH, int KC, bool SPLIT, bool PF, int COLT>
+// registers. PF selects the schedule: 0, each step loads, then computes; 1, the next step's
+// loads are issued right after the current step's MFMAs and overlap its candidate
+// bookkeeping; 2 (rolling), the loads of the next step's fragment f are issued as soon as
+// the current step's MFMAs that read fragment f have issued, so that every load has about a
+// step of MFMA work and bookkeeping in front of its first use, with no extra registers.
+template <class Op, int KCH, int KC, bool SPLIT, int PF, int COLT, bool CEN = false>
 __device__ __forceinline__ void knn_tile_body(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
     float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau,
-    const float *__restrict__ tau0) {
+    const float *__restrict__ tau0, const KnnCentring cen) {
     using frag_t = typename Op::frag_t;
     using acc_t = typename Op::acc_t;
     using ae = typename Op::acc_e;
@@ -510,7 +537,84 @@ __device__ __forceinline__ void knn_tile_body(
             take_hits<Op, KC, CAP, ROWT, COLT>(acc, ts, cq, bkey, bidx, qbase, tb - t_begin);
     };
 
-    if constexpr (PF) {
+    if constexpr (PF == 2) {
+        // Rolling prefetch. The address of the next step's tile is computed before the
+        // step's MFMAs (the last step reloads its own tile, in bounds), so that only loads
+        // with immediate offsets sit between the MFMA groups: no VALU instruction writes a
+        // register while the MFMAs are in flight. Each MFMA reads its A and B operands when
+        // it issues, so a fragment can be reloaded once every MFMA that reads it has issued.
+        if (t_begin < t_end) {
+            frag_t b[COLT][NS][NF];
+            ae hv[COLT];
+            load_step(t_begin, b, hv);
+            // Centred rows: the products of this wave's queries with the centre of the group
+            // the current step belongs to (a step lies in one group), reloaded when the group
+            // changes.
+            ae qcv[ROWT][4] = {};
+            int cur_g = -1;
+            for (int tb = t_begin; tb < t_end; tb += TSTEP) {
+                const int nb = tb + TSTEP < t_end ? tb + TSTEP : tb;
+                const in_t *ntile = Y + (int64_t)nb * rowlen + lane * CSR;
+                if constexpr (CEN) {
+                    const int grp = cen.step_group[tb / TSTEP];
+                    if (grp != cur_g) {
+                        cur_g = grp;
+#pragma unroll
+                        for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+                            for (int i = 0; i < 4; i++) {
+                                const int q = q0 + qbase + rt * 16 + 4 * g + i;
+                                qcv[rt][i] = q < nq ? cen.qc[(int64_t)q * cen.G + grp] : (ae)0;
+                            }
+                    }
+                }
+                acc_t acc[ROWT][COLT];
+#pragma unroll
+                for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+                    for (int rt = 0; rt < ROWT; rt++) {
+                        if constexpr (CEN)
+                            acc[rt][ct] = acc_t{hv[ct] + qcv[rt][0], hv[ct] + qcv[rt][1],
+                                                hv[ct] + qcv[rt][2], hv[ct] + qcv[rt][3]};
+                        else
+                            acc[rt][ct] = acc_t{hv[ct], hv[ct], hv[ct], hv[ct]};
+                    }
+#pragma unroll
+                for (int ct = 0; ct < COLT; ct++)
+                    hv[ct] = nyh[nb + ct * 16 + r];
+                __builtin_amdgcn_sched_barrier(0);
+                constexpr int FPC = Op::FPC;
+#pragma unroll
+                for (int c = 0; c < NF / FPC; c++) {
+#pragma unroll
+                    for (int f = c * FPC; f < (c + 1) * FPC; f++)
+#pragma unroll
+                        for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+                            for (int rt = 0; rt < ROWT; rt++) {
+                                acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][0][f], acc[rt][ct]);
+                                if constexpr (SPLIT) {
+                                    acc[rt][ct] =
+                                        Op::mfma(a[0][rt][f], b[ct][NS - 1][f], acc[rt][ct]);
+                                    acc[rt][ct] =
+                                        Op::mfma(a[NS - 1][rt][f], b[ct][0][f], acc[rt][ct]);
+                                }
+                            }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+                        for (int h = 0; h < NS; h++)
+                            Op::template load_chunk<CST, NF>(
+                                ntile + (int64_t)ct * 16 * rowlen + h * 16 * Kp, c, b[ct][h]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (any_hit(acc))
+                    take_hits<Op, KC, CAP, ROWT, COLT>(acc, ts, cq, bkey, bidx, qbase,
+                                                       tb - t_begin);
+            }
+        }
+    } else if constexpr (PF == 1) {
         // The next step's operands are loaded after this step's MFMAs and run under its
         // candidate bookkeeping. The hit test comes first: it reads the accumulators, so it
         // waits for the last MFMA, and only then may the loads and their address arithmetic
@@ -547,31 +651,32 @@ __device__ __forceinline__ void knn_tile_body(
         }
     }
     write_split<Op, KC, CAP, ROWT>(cq, bkey, bidx, t_begin, qbase, q0, nq, nsplit, split,
-                                   part_key, part_idx, part_tau, tau0);
+                                   part_key, part_idx, part_tau, tau0, CEN ? cen.perm : nullptr);
 }
 
 // The sweep as a kernel: K2 asks the compiler for registers that leave room for WPE waves
 // per SIMD (as many workgroups as fit a CU's LDS); the fp64 operands need more registers
 // and run one wave per SIMD without spilling.
-template <class Op, int KCH, int KC, bool SPLIT, bool PF>
+template <class Op, int KCH, int KC, bool SPLIT, int PF>
 __global__ __launch_bounds__(64 * TileShape<KC>::WV) void knn_tile_kernel(
     const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
     const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit,
     float *__restrict__ part_key, int *__restrict__ part_idx, float *__restrict__ part_tau,
-    const float *__restrict__ tau0) {
+    const float *__restrict__ tau0, const KnnCentring cen) {
     knn_tile_body<Op, KCH, KC, SPLIT, PF, 2>(Q, nq, Y, nyh, n, Kp, split_len, nsplit, part_key,
-                                             part_idx, part_tau, tau0);
+                                             part_idx, part_tau, tau0, cen);
 }
-template <class Op, int KCH, int KC, bool SPLIT, bool PF, int COLT>
+template <class Op, int KCH, int KC, bool SPLIT, int PF, int COLT, bool CEN = false>
 __global__ __launch_bounds__(64 * TileShape<KC>::WV)
 __attribute__((amdgpu_waves_per_eu(TileCfg<Op, KCH, KC, SPLIT, COLT>::WPE))) void
 knn_tile_kernel_k2(const typename Op::in_t *__restrict__ Q, int nq,
                    const typename Op::in_t *__restrict__ Y,
                    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len,
                    int nsplit, float *__restrict__ part_key, int *__restrict__ part_idx,
-                   float *__restrict__ part_tau, const float *__restrict__ tau0) {
-    knn_tile_body<Op, KCH, KC, SPLIT, PF, COLT>(Q, nq, Y, nyh, n, Kp, split_len, nsplit,
-                                                part_key, part_idx, part_tau, tau0);
+                   float *__restrict__ part_tau, const float *__restrict__ tau0,
+                   const KnnCentring cen) {
+    knn_tile_body<Op, KCH, KC, SPLIT, PF, COLT, CEN>(Q, nq, Y, nyh, n, Kp, split_len, nsplit,
+                                                     part_key, part_idx, part_tau, tau0, cen);
 }
 
 constexpr int MMAX = 2048; // candidates merged per query (nsplit * kc)
@@ -583,7 +688,7 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
     const float *__restrict__ part_key, const int *__restrict__ part_idx,
     const float *__restrict__ part_tau, int nsplit, int kc, const T *__restrict__ Q,
     const T *__restrict__ Yrows, int d, int k, double u_in, double abs_in, double u_acc,
-    double ymax, int post,
+    double ymax, double rmax, double cmax, int post,
     T *__restrict__ out_dist, int *__restrict__ out_idx, int *__restrict__ fb_count,
     int *__restrict__ fb_list) {
     __shared__ float skey[MM];
@@ -634,19 +739,24 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
             __syncthreads();
         }
     }
-    // Error bound of the stored key |y|^2 - 2 q.y. With S = sum|q_l y_l| <= |q| ymax: the
-    // operands rounded to the input precision (2u_in + u_in^2) S; the d+1-term accumulation
-    // from -|y|^2/2 (gamma_{d+2}); the rounded -|y|^2/2; the float storage of the key.
+    // Error bound of the stored key |y|^2 - 2 q.y. The tile accumulates the products of
+    // the query with the operand rows r (the rows themselves, or their residuals about a
+    // group centre c) from h = -|y|^2/2 (+ q.c for a centred form). With S = sum|q_l r_l|
+    // <= |q| rmax and |h| <= H = ymax^2/2 + |q| cmax: the operands rounded to the input
+    // precision (2u_in + u_in^2) S; the d+1-term accumulation from h (gamma_{d+2}); the
+    // rounding of h (of -|y|^2/2 alone, or of -|y|^2/2, q.c and their sum); the float
+    // storage of the key.
     const double qnorm = sqrt(qn);
-    const double S = qnorm * ymax;
+    const double S = qnorm * rmax;
+    const double H = 0.5 * ymax * ymax + qnorm * cmax;
     const double gam = (d + 2) * u_acc / (1.0 - (d + 2) * u_acc);
     const double u32 = 1.0 / 16777216.0;
+    const double herr = cmax > 0.0 ? 2.0 * u32 * H : u_acc * 0.5 * ymax * ymax;
     const double E = 2.0 * ((2.0 * u_in + u_in * u_in) * S +
-                            gam * (S * (1.0 + u_in) * (1.0 + u_in) + 0.5 * ymax * ymax) +
-                            u_acc * 0.5 * ymax * ymax) +
-                     u32 * (ymax * ymax + 2.01 * S) +
+                            gam * (S * (1.0 + u_in) * (1.0 + u_in) + H) + herr) +
+                     u32 * (2.0 * H + 2.01 * S) +
                      // operands below the normal range of fp16 carry an absolute error
-                     2.0 * abs_in * sqrt((double)d) * (qnorm + ymax) * 1.01;
+                     2.0 * abs_in * sqrt((double)d) * (qnorm + rmax) * 1.01;
     const float ak = skey[k - 1];
     const double lim = (double)ak + 2.0 * E;
     // Candidates within 2E of the k-th key, re-ranked with exact distances.
@@ -775,11 +885,11 @@ int knn_merge_max_candidates() { return MMAX; }
 
 int knn_tile_min_splits(int prec, int Kp, int n, int kc, int variant) {
     const bool split = prec == KP_F16X3 || prec == KP_BF16X3;
-    const bool wide = prec == KP_F16 || prec == KP_BF16 || split;   // 8 elements, 4 registers
+    const bool wide = prec == KP_F16 || prec == KP_F16C || prec == KP_BF16 || split; // 8 elements
     const int frag_regs = wide ? 4 : prec == KP_F64 ? 2 : 1;
     const int nf = wide ? Kp / 4 / 8 : Kp / 4;
     const int rt = kc <= 16 ? TileShape<16>::RT : 1;
-    const int colt = variant == 2 && prec != KP_F64 ? 1 : 2;
+    const int colt = variant == 2 && prec != KP_F64 && prec != KP_F16C ? 1 : 2;
     if (!tile_short_index(kc, rt, split ? 2 : 1, nf, frag_regs, prec == KP_F64, colt))
         return 1;
     return (n + SPLIT16_MAX - 1) / SPLIT16_MAX;
@@ -789,29 +899,47 @@ int knn_tile_min_splits(int prec, int Kp, int n, int kc, int variant) {
 // Launches the fused tile for one operand type, candidate-list length, width and variant
 // (0: two 16-row training tiles per step, the next step's loads under the current step's
 // bookkeeping; 1: two tiles per step, load then compute; 2: as 0 with one tile per step,
-// which halves the training fragments held in registers) or, with `slots` set, reports how
-// many of its workgroups the device runs at once instead.
+// which halves the training fragments held in registers; 3: two tiles per step with the
+// rolling prefetch, each fragment reloaded for the next step as soon as its MFMAs have
+// issued) or, with `slots` set, reports how many of its workgroups the device runs at once
+// instead.
 template <class Op, int KCH, int KC, bool SPLIT>
 static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                 int Kp, int nsplit, float *part_key, int *part_idx,
                                 float *part_tau, const float *tau0, int variant, int *slots,
-                                hipStream_t s) {
+                                hipStream_t s, const KnnCentring *cen) {
     using in_t = typename Op::in_t;
     using ae = typename Op::acc_e;
     using kernel_t = void (*)(const in_t *, int, const in_t *, const ae *, int, int, int, int,
-                              float *, int *, float *, const float *);
+                              float *, int *, float *, const float *, const KnnCentring);
     constexpr int QB = TileShape<KC>::WV * TileShape<KC>::RT * 16;
     kernel_t kern;
     bool short_index = false;
-    if constexpr (std::is_same_v<Op, OpF64>) {
-        kern = variant == 1 ? knn_tile_kernel<Op, KCH, KC, SPLIT, false>
-                            : knn_tile_kernel<Op, KCH, KC, SPLIT, true>;
+    // Centred operand rows: the fp16 form with the rolling sweep.
+    constexpr bool CEN_OK = std::is_same_v<Op, OpF16> && !SPLIT;
+    if (cen && !CEN_OK)
+        return hipErrorInvalidValue;
+    if constexpr (CEN_OK) {
+        if (cen) {
+            kern = knn_tile_kernel_k2<Op, KCH, KC, SPLIT, 2, 2, true>;
+            short_index = TileCfg<Op, KCH, KC, SPLIT, 2>::SHORT;
+        }
+    }
+    if (cen) {
+        // chosen above
+    } else if constexpr (std::is_same_v<Op, OpF64>) {
+        kern = variant == 1   ? knn_tile_kernel<Op, KCH, KC, SPLIT, 0>
+               : variant == 3 ? knn_tile_kernel<Op, KCH, KC, SPLIT, 2>
+                              : knn_tile_kernel<Op, KCH, KC, SPLIT, 1>;
+    } else if (variant == 3) {
+        kern = knn_tile_kernel_k2<Op, KCH, KC, SPLIT, 2, 2>;
+        short_index = TileCfg<Op, KCH, KC, SPLIT, 2>::SHORT;
     } else if (variant == 2) {
-        kern = knn_tile_kernel_k2<Op, KCH, KC, SPLIT, true, 1>;
+        kern = knn_tile_kernel_k2<Op, KCH, KC, SPLIT, 1, 1>;
         short_index = TileCfg<Op, KCH, KC, SPLIT, 1>::SHORT;
     } else {
-        kern = variant == 1 ? knn_tile_kernel_k2<Op, KCH, KC, SPLIT, false, 2>
-                            : knn_tile_kernel_k2<Op, KCH, KC, SPLIT, true, 2>;
+        kern = variant == 1 ? knn_tile_kernel_k2<Op, KCH, KC, SPLIT, 0, 2>
+                            : knn_tile_kernel_k2<Op, KCH, KC, SPLIT, 1, 2>;
         short_index = TileCfg<Op, KCH, KC, SPLIT, 2>::SHORT;
     }
     const dim3 block(64 * TileShape<KC>::WV);
@@ -832,9 +960,10 @@ static hipError_t launch_tile_w(const void *Q, int nq, const void *Y, const void
         return hipErrorInvalidValue;
     const int qblocks = (nq + QB - 1) / QB;
     const dim3 grid((qblocks * nsplit + 7) / 8 * 8);
+    const KnnCentring none{};
     hipLaunchKernelGGL(kern, grid, block, 0, s, static_cast<const in_t *>(Q), nq,
                        static_cast<const in_t *>(Y), static_cast<const ae *>(nyh), n, Kp,
-                       split_len, nsplit, part_key, part_idx, part_tau, tau0);
+                       split_len, nsplit, part_key, part_idx, part_tau, tau0, cen ? *cen : none);
     return hipGetLastError();
 }
 
@@ -842,16 +971,16 @@ template <class Op, int KC, bool SPLIT>
 static hipError_t launch_tile_op(const void *Q, int nq, const void *Y, const void *nyh, int n,
                                  int Kp, int nsplit, float *part_key, int *part_idx,
                                  float *part_tau, const float *tau0, int variant, int *slots,
-                                hipStream_t s) {
+                                 hipStream_t s, const KnnCentring *cen) {
     switch (Kp / 4) {
     case 8:
-        return launch_tile_w<Op, 8, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
+        return launch_tile_w<Op, 8, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s, cen);
     case 16:
-        return launch_tile_w<Op, 16, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
+        return launch_tile_w<Op, 16, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s, cen);
     case 24:
-        return launch_tile_w<Op, 24, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
+        return launch_tile_w<Op, 24, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s, cen);
     case 32:
-        return launch_tile_w<Op, 32, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
+        return launch_tile_w<Op, 32, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s, cen);
     default:
         return hipErrorInvalidValue;
     }
@@ -867,14 +996,17 @@ template <int KC>
 static hipError_t launch_tile_kc(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                                  int n, int Kp, int nsplit, float *part_key, int *part_idx,
                                  float *part_tau, const float *tau0, int variant, int *slots,
-                                hipStream_t s) {
+                                 hipStream_t s, const KnnCentring *cen) {
 #define DA_OP(OP, SPLIT)                                                                          \
     launch_tile_op<OP, KC, SPLIT>(Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau,     \
-                                  tau0, variant, slots, s)
+                                  tau0, variant, slots, s, cen)
+    if ((prec == KP_F16C) != (cen != nullptr))
+        return hipErrorInvalidValue;
     switch (prec) {
     case KP_F32: return DA_OP(OpF32, false);
     case KP_F64: return DA_OP(OpF64, false);
-    case KP_F16: return DA_OP(OpF16, false);
+    case KP_F16:
+    case KP_F16C: return DA_OP(OpF16, false);
     case KP_BF16: return DA_OP(OpBF16, false);
     case KP_F16X3: return DA_OP(OpF16, true);
     case KP_BF16X3: return DA_OP(OpBF16, true);
@@ -886,16 +1018,16 @@ static hipError_t launch_tile_kc(int prec, const void *Q, int nq, const void *Y,
 static hipError_t tile_dispatch(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                                 int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
                                 float *part_tau, const float *tau0, int variant, int *slots,
-                                hipStream_t s) {
+                                hipStream_t s, const KnnCentring *cen) {
     switch (kc) {
     case 16:
-        return launch_tile_kc<16>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
+        return launch_tile_kc<16>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s, cen);
     case 32:
-        return launch_tile_kc<32>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
+        return launch_tile_kc<32>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s, cen);
     case 64:
-        return launch_tile_kc<64>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
+        return launch_tile_kc<64>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s, cen);
     case 128:
-        return launch_tile_kc<128>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s);
+        return launch_tile_kc<128>(prec, Q, nq, Y, nyh, n, Kp, nsplit, part_key, part_idx, part_tau, tau0, variant, slots, s, cen);
     default:
         return hipErrorInvalidValue;
     }
@@ -903,17 +1035,74 @@ static hipError_t tile_dispatch(int prec, const void *Q, int nq, const void *Y, 
 
 hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                            int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
-                           float *part_tau, const float *tau0, int variant, hipStream_t s) {
+                           float *part_tau, const float *tau0, int variant, hipStream_t s,
+                           const KnnCentring *cen) {
     return tile_dispatch(prec, Q, nq, Y, nyh, n, Kp, nsplit, kc, part_key, part_idx, part_tau,
-                         tau0, variant, nullptr, s);
+                         tau0, variant, nullptr, s, cen);
 }
 
 hipError_t knn_tile_slots(int prec, int Kp, int kc, int variant, int *slots) {
+    // The centred form's kernel has the registers and LDS of the plain fp16 sweep.
+    static const KnnCentring probe{};
     return tile_dispatch(prec, nullptr, 0, nullptr, nullptr, 0, Kp, 1, kc, nullptr, nullptr,
-                         nullptr, nullptr, variant, slots, nullptr);
+                         nullptr, nullptr, variant, slots, nullptr,
+                         prec == KP_F16C ? &probe : nullptr);
 }
 
 namespace {
+// One wave per stored row: the residual of its original row about its group centre, rounded
+// to fp16 and zero-padded to Kp features, and the residual's squared norm (in double).
+template <typename T>
+__global__ __launch_bounds__(256) void knn_centre_rows_kernel(
+    const T *__restrict__ X, int d, const int *__restrict__ perm,
+    const int *__restrict__ step_group, const double *__restrict__ C, int rows_pad, int Kp,
+    _Float16 *__restrict__ dst, double *__restrict__ rnorm2) {
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= rows_pad)
+        return;
+    const int j = perm[p];
+    const int g = step_group[p / STEP];
+    double acc = 0.0;
+    for (int l = lane; l < Kp; l += 64) {
+        double v = 0.0;
+        if (j >= 0 && l < d)
+            v = (double)X[(int64_t)j * d + l] - C[(int64_t)g * d + l];
+        dst[(int64_t)p * Kp + l] = (_Float16)v;
+        acc = fma(v, v, acc);
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        acc += __shfl_xor(acc, o, 64);
+    if (lane == 0)
+        rnorm2[p] = acc;
+}
+
+// qc[q*G + g] = Q_q . C_g: a 16 x 16 block of (query, centre) pairs per workgroup, the rows
+// staged in LDS in slices of 64 features, accumulated in double.
+template <typename T>
+__global__ __launch_bounds__(256) void knn_query_centre_kernel(const T *__restrict__ Q, int nq,
+                                                               int d, const double *__restrict__ C,
+                                                               int G, float *__restrict__ qc) {
+    __shared__ double qs[16][65], cs[16][65];
+    const int tq = threadIdx.x >> 4, tg = threadIdx.x & 15;
+    const int q0 = blockIdx.y * 16, g0 = blockIdx.x * 16;
+    double acc = 0.0;
+    for (int l0 = 0; l0 < d; l0 += 64) {
+        for (int t = threadIdx.x; t < 16 * 64; t += 256) {
+            const int r = t >> 6, l = t & 63;
+            qs[r][l] = (q0 + r < nq && l0 + l < d) ? (double)Q[(int64_t)(q0 + r) * d + l0 + l] : 0.0;
+            cs[r][l] = (g0 + r < G && l0 + l < d) ? C[(int64_t)(g0 + r) * d + l0 + l] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 16
+        for (int l = 0; l < 64; l++)
+            acc = fma(qs[tq][l], cs[tg][l], acc);
+        __syncthreads();
+    }
+    if (q0 + tq < nq && g0 + tg < G)
+        qc[(int64_t)(q0 + tq) * G + g0 + tg] = (float)acc;
+}
+
 // tau0[q] = the smallest split threshold of a sweep over a subset of the rows: the subset
 // holds kc rows with keys at most that value, so the kc-th key of the whole set is no larger.
 __global__ void knn_seed_kernel(const float *__restrict__ part_tau, int nq, int nsplit,
@@ -928,6 +1117,48 @@ __global__ void knn_seed_kernel(const float *__restrict__ part_tau, int nq, int 
 }
 } // namespace
 
+template <typename T>
+hipError_t launch_knn_centre_rows(const T *X, int d, const int *perm, const int *step_group,
+                                  const double *C, int rows_pad, int Kp, void *dst,
+                                  double *rnorm2, hipStream_t s) {
+    if (rows_pad % STEP != 0 || Kp < d)
+        return hipErrorInvalidValue;
+    if (rows_pad == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(knn_centre_rows_kernel<T>, dim3((rows_pad + 3) / 4), dim3(256), 0, s, X, d,
+                       perm, step_group, C, rows_pad, Kp, static_cast<_Float16 *>(dst), rnorm2);
+    return hipGetLastError();
+}
+template hipError_t launch_knn_centre_rows<float>(const float *, int, const int *, const int *,
+                                                  const double *, int, int, void *, double *,
+                                                  hipStream_t);
+template hipError_t launch_knn_centre_rows<double>(const double *, int, const int *, const int *,
+                                                   const double *, int, int, void *, double *,
+                                                   hipStream_t);
+
+template <typename T>
+hipError_t launch_knn_query_centre(const T *Q, int nq, int d, const double *C, int G, float *qc,
+                                   hipStream_t s) {
+    if (nq <= 0 || G <= 0)
+        return hipSuccess;
+    // At most 65,535 blocks of 16 queries per launch (the grid's y extent).
+    const int chunk = 65535 * 16;
+    for (int q0 = 0; q0 < nq; q0 += chunk) {
+        const int m = nq - q0 < chunk ? nq - q0 : chunk;
+        hipLaunchKernelGGL(knn_query_centre_kernel<T>, dim3((G + 15) / 16, (m + 15) / 16),
+                           dim3(256), 0, s, Q + (int64_t)q0 * d, m, d, C, G,
+                           qc + (int64_t)q0 * G);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+template hipError_t launch_knn_query_centre<float>(const float *, int, int, const double *, int,
+                                                   float *, hipStream_t);
+template hipError_t launch_knn_query_centre<double>(const double *, int, int, const double *, int,
+                                                    float *, hipStream_t);
+
 hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau0,
                            hipStream_t s) {
     if (nq <= 0)
@@ -940,8 +1171,9 @@ hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau
 template <typename T>
 hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const float *part_tau,
                             int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,
-                            double u_in, double abs_in, double u_acc, double ymax, int post,
-                            T *out_dist, int *out_idx, int *fb_count, int *fb_list, hipStream_t s) {
+                            double u_in, double abs_in, double u_acc, double ymax, double rmax,
+                            double cmax, int post, T *out_dist, int *out_idx, int *fb_count,
+                            int *fb_list, hipStream_t s) {
     const int m = nsplit * kc;
     // The lists may be shorter than k; their union must hold k candidates.
     if (m > MMAX || d > 1024 || k > 128 || k > m)
@@ -953,18 +1185,18 @@ hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const fl
               : m <= 1024 ? knn_merge_kernel<T, 1024>
                           : knn_merge_kernel<T, 2048>;
     hipLaunchKernelGGL(kern, dim3(nq), dim3(64), 0, s, part_key, part_idx, part_tau, nsplit, kc,
-                       Q, Y, d, k, u_in, abs_in, u_acc, ymax, post, out_dist, out_idx, fb_count,
-                       fb_list);
+                       Q, Y, d, k, u_in, abs_in, u_acc, ymax, rmax, cmax, post, out_dist, out_idx,
+                       fb_count, fb_list);
     return hipGetLastError();
 }
 
 template hipError_t launch_knn_merge<float>(const float *, const int *, const float *, int, int,
                                             const float *, const float *, int, int, int, double,
-                                            double, double, double, int, float *, int *, int *,
-                                            int *, hipStream_t);
+                                            double, double, double, double, double, int, float *,
+                                            int *, int *, int *, hipStream_t);
 template hipError_t launch_knn_merge<double>(const float *, const int *, const float *, int, int,
                                              const double *, const double *, int, int, int,
-                                             double, double, double, double, int, double *, int *,
-                                             int *, int *, hipStream_t);
+                                             double, double, double, double, double, double, int,
+                                             double *, int *, int *, int *, hipStream_t);
 
 } // namespace dagpu

@@ -265,9 +265,25 @@ template <typename U> hipError_t launch_radius_sqrt(U *v, int64_t n, hipStream_t
 // splits of at most 65,536 rows).
 int knn_tile_min_splits(int prec, int Kp, int n, int kc, int variant);
 // Operand precision of the fused tile.
-enum KnnPrec : int { KP_F32 = 0, KP_F64 = 1, KP_F16 = 2, KP_BF16 = 3, KP_F16X3 = 4, KP_BF16X3 = 5 };
+// KP_F16C: fp16 operands of training rows stored relative to the centre of their group
+// (the queries are plain fp16); the tile adds each query's product with the group centre.
+enum KnnPrec : int {
+    KP_F32 = 0, KP_F64 = 1, KP_F16 = 2, KP_BF16 = 3, KP_F16X3 = 4, KP_BF16X3 = 5, KP_F16C = 6
+};
 // Bytes per padded operand row element group: split forms store [hi Kp][lo Kp] per row.
-inline int knn_prec_row_factor(int prec) { return prec >= KP_F16X3 ? 2 : 1; }
+inline int knn_prec_row_factor(int prec) {
+    return prec == KP_F16X3 || prec == KP_BF16X3 ? 2 : 1;
+}
+// Centred training rows of the KP_F16C form. The rows are stored in group order, each group
+// padded to a whole number of 32-row steps (padded rows have -|y|^2/2 = -inf); step_group[t]
+// is the group of rows 32t .. 32t+31, qc[q*G + g] the product of query q with the centre of
+// group g (float), perm[p] the original index of stored row p.
+struct KnnCentring {
+    const int *step_group = nullptr;
+    const float *qc = nullptr;
+    int G = 0;
+    const int *perm = nullptr;
+};
 inline int knn_prec_elem_size(int prec) { return prec == KP_F64 ? 8 : prec == KP_F32 ? 4 : 2; }
 // Padded feature count used by the fused tile for d features (a multiple of 32, <= 128).
 int knn_tile_kp(int d);
@@ -284,7 +300,19 @@ int knn_tile_kp(int d);
 // nyh must hold a multiple of 32 entries (the padded rows at -inf).
 hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const void *nyh,
                            int n, int Kp, int nsplit, int kc, float *part_key, int *part_idx,
-                           float *part_tau, const float *tau0, int variant, hipStream_t s);
+                           float *part_tau, const float *tau0, int variant, hipStream_t s,
+                           const KnnCentring *cen = nullptr);
+// Centred operand rows of the KP_F16C form: for stored row p with original row j = perm[p]
+// (-1: a padded row) of group g = step_group[p / 32], dst row p = fp16(X_j - C_g) zero-padded
+// to Kp features (row-major, rows_pad rows) and rnorm2[p] = |X_j - C_g|^2 (0 for padded rows).
+template <typename T>
+hipError_t launch_knn_centre_rows(const T *X, int d, const int *perm, const int *step_group,
+                                  const double *C, int rows_pad, int Kp, void *dst,
+                                  double *rnorm2, hipStream_t s);
+// qc[q*G + g] = (float) sum_l Q[q*d + l] C[g*d + l], accumulated in double.
+template <typename T>
+hipError_t launch_knn_query_centre(const T *Q, int nq, int d, const double *C, int G, float *qc,
+                                   hipStream_t s);
 // tau0[q] = min over the nsplit thresholds part_tau[q*nsplit + s] of a sweep (a valid seed
 // for a sweep of any superset of the swept rows).
 hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau0,
@@ -292,14 +320,17 @@ hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau
 // Q: nq x d and Y: n x d unpadded rows (type T); u_in: relative representation error of the
 // tile operands (0 when they are the data themselves); abs_in: absolute representation error
 // per element (fp16 subnormals); u_acc: unit roundoff of the tile accumulation;
-// ymax: max row norm of Y. Requires nsplit * kc <= knn_merge_max_candidates() and k <= kc.
+// ymax: max row norm of Y; rmax: max norm of the operand rows of the tile (ymax, or the
+// largest residual of a centred form); cmax: max centre norm of a centred form (0 otherwise).
+// Requires nsplit * kc <= knn_merge_max_candidates() and k <= kc.
 // Certified queries get out_dist/out_idx (row-major nq x k); the others are appended to
 // fb_list (fb_count must be zeroed before).
 template <typename T>
 hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const float *part_tau,
                             int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,
-                            double u_in, double abs_in, double u_acc, double ymax, int post,
-                            T *out_dist, int *out_idx, int *fb_count, int *fb_list, hipStream_t s);
+                            double u_in, double abs_in, double u_acc, double ymax, double rmax,
+                            double cmax, int post, T *out_dist, int *out_idx, int *fb_count,
+                            int *fb_list, hipStream_t s);
 } // namespace dagpu
 
 namespace dagpu {
@@ -330,7 +361,7 @@ hipError_t launch_rows_to_bf16(const float *X, int n, int d, int ldx, void *Y, i
 template <typename T>
 hipError_t launch_kmeans_recheck(const T *X, int ldx, int d, const double *C, int k,
                                  const int *list_count, const int *list, int *labels,
-                                 const int *prev, int *changed, hipStream_t s);
+                                 const int *prev, int *changed, double *CT, hipStream_t s);
 } // namespace dagpu
 
 namespace dagpu {

@@ -152,7 +152,33 @@ da_status KMeansHandle<T>::set_data(da_int n_samples, da_int n_features, const T
     s = upload_data(A, lda, X_);
     if (s != da_status_success)
         return s;
-    s = prepare_scales_for<T>(X_.as<T>());
+    return finish_set_data();
+}
+
+template <typename T>
+da_status KMeansHandle<T>::set_data_device(da_int n_samples, da_int n_features, const T *dX) {
+    refresh();
+    initdone_ = false;
+    data_order_ = row_major;
+    if (n_samples < 1 || n_features < 1)
+        return err->error(da_status_invalid_array_dimension, "The data has no rows or columns.");
+    if (dX == nullptr)
+        return err->error(da_status_invalid_pointer, "The data pointer is null.");
+    da_status s = gpu->init(err);
+    if (s != da_status_success)
+        return s;
+    (void)opts.set_string("storage order", "row-major");
+    n_samples_ = n_samples;
+    n_features_ = n_features;
+    const size_t bytes = (size_t)n_samples * n_features * sizeof(T);
+    DA_HIP(err, X_.reserve(bytes));
+    DA_HIP(err, hipMemcpyAsync(X_.ptr(), dX, bytes, hipMemcpyDeviceToDevice, gpu->stream()));
+    return finish_set_data();
+}
+
+template <typename T> da_status KMeansHandle<T>::finish_set_data() {
+    const da_int n_samples = n_samples_, n_features = n_features_;
+    da_status s = prepare_scales_for<T>(X_.as<T>());
     if (s != da_status_success)
         return s;
     mfma_ok_ = false;
@@ -275,8 +301,10 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
         }
         {
             KernelScope ks(*gpu, "kmeans_recheck");
+            DA_HIP(err, ct_.reserve((size_t)k * d * sizeof(double)));
             DA_HIP(err, launch_kmeans_recheck<U>(dX, d, d, Cin, k, dlist_count, list_.as<int>(),
-                                                 lab_new, lab_old, dchanged, st));
+                                                 lab_new, lab_old, dchanged, ct_.as<double>(),
+                                                 st));
         }
     } else if (use_mfma) {
         // fp32 cross term on MFMA, exact re-check of near ties (kmeans_mfma.hip).
@@ -295,8 +323,10 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
         }
         {
             KernelScope ks(*gpu, "kmeans_recheck");
+            DA_HIP(err, ct_.reserve((size_t)k * d * sizeof(double)));
             DA_HIP(err, launch_kmeans_recheck<U>(dX, d, d, Cin, k, dlist_count, list_.as<int>(),
-                                                 lab_new, lab_old, dchanged, st));
+                                                 lab_new, lab_old, dchanged, ct_.as<double>(),
+                                                 st));
         }
     } else {
         KernelScope ks(*gpu, "kmeans_assign");

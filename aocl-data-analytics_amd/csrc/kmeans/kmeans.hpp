@@ -11,6 +11,12 @@ template <typename T> class KMeansHandle : public AlgHandle {
   public:
     KMeansHandle(ErrorStack &err, GpuContext &gpu);
     da_status set_data(da_int n_samples, da_int n_features, const T *A, da_int lda);
+    // Data already in device memory (row-major n_samples x n_features, copied): for callers
+    // inside the library that keep their rows in HBM.
+    da_status set_data_device(da_int n_samples, da_int n_features, const T *dX);
+    // Results of the best run, row-major centres k x d and labels (after compute).
+    const std::vector<double> &centres() const { return best_centres_; }
+    const std::vector<da_int> &labels() const { return best_labels_; }
     da_status set_init_centres(const T *C, da_int ldc);
     da_status compute();
     da_status transform(da_int m_samples, da_int m_features, const T *X, da_int ldx,
@@ -31,6 +37,7 @@ template <typename T> class KMeansHandle : public AlgHandle {
         double inertia = 0;
     };
     da_status upload_data(const T *src, da_int ld, DeviceBuffer &dst);
+    da_status finish_set_data();
     da_status initialise_centres(da_int method);
     da_status kmeans_plusplus();
     da_status afk_mc2();
@@ -75,8 +82,8 @@ template <typename T> class KMeansHandle : public AlgHandle {
     da_int ppb_ = 0, nblocks_ = 0, kc_ = 0;
     std::vector<int> escale_host_;
     // fp32 MFMA assignment: zero-padded copy of the data (n x kp_), fp32 centres, -|c|^2/2,
-    // the re-check list.
-    DeviceBuffer xpad_, c32_, chalf_, list_;
+    // the re-check list, the centres transposed for the re-check.
+    DeviceBuffer xpad_, c32_, chalf_, list_, ct_;
     int kp_ = 0;
     bool mfma_ok_ = false;
     // bf16 cross term: bf16 copy of the data padded to kp16_ (a multiple of 32) features,

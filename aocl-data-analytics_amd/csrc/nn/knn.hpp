@@ -3,6 +3,7 @@
 // training set resident in GPU memory.
 #pragma once
 #include "../core/handle.hpp"
+#include "../kernels/launch.hpp"
 #include <functional>
 #include <map>
 #include <vector>
@@ -20,7 +21,7 @@ struct SearchContext {
     // that search the same handle from one thread share their tuning.
     SearchContext *tuning = nullptr;
     DeviceBuffer q_norm, keys, sel_key, sel_idx, qpad, part_key, part_idx, part_tau, fb, fb_q,
-        fb_ind, fb_dist, tau0;
+        fb_ind, fb_dist, tau0, qc;
     // Pinned host word receiving the fallback count of the last fused launch.
     int *h_fb = nullptr;
     da_int last_fallback = 0, last_candidates = 0;
@@ -109,7 +110,8 @@ template <typename T> class NNHandle : public AlgHandle {
     bool fused_eligible(da_int k) const;
     int split_cap(int kc) const;
     int tile_splits(SearchContext &ctx, int prec, int kc, int variant, da_int nq,
-                    const void *qpad, const float *seed, hipStream_t st);
+                    const void *qpad, const float *seed, hipStream_t st,
+                    const KnnCentring *cen = nullptr);
     da_status prepare_fused(int prec);
 
     bool trained_ = false;
@@ -139,6 +141,16 @@ template <typename T> class NNHandle : public AlgHandle {
     int kp_ = 0;
     int ypad_prec_ = -1; // operand precision the padded copy was built for
     double ymax_ = 0;
+    // Centred fp16 form: the stored rows are residuals about the centres of cen_G_ groups
+    // (a k-means of the training rows), kept in group order, each group padded to whole
+    // 32-row steps (cen_rows_ stored rows). Group centres (G x d, double), the group of each
+    // step, the original index of each stored row, the largest residual and centre norms.
+    DeviceBuffer cen_C_, cen_step_, cen_perm_;
+    int cen_G_ = 0, cen_rows_ = 0;
+    double rmax_ = 0, cmax_ = 0;
+    da_status prepare_centred();
+    // Rows the fused tile sweeps: the padded stored rows of the centred form, else n_samples.
+    int tile_rows(int prec) const { return prec == KP_F16C ? cen_rows_ : (int)n_samples_; }
     bool fused_ready_ = false;
     int tile_prec() const;
 };

@@ -44,6 +44,7 @@ struct Request {
     int remaining = 0; // queries not yet completed
     Clock::time_point t_submit, t_done;
     bool done = false;
+    bool claimed = false; // a caller is waiting for (or has taken) the result
     da_status status = da_status_success;
 };
 
@@ -156,10 +157,15 @@ template <typename T> class Server : public ServerBase {
         {
             std::unique_lock<std::mutex> g(mu_);
             auto it = all_.find(id);
-            if (it == all_.end())
+            // A result is handed to one waiter only: a second wait on the same request
+            // (also one that started while the first was still waiting) is refused.
+            if (it == all_.end() || it->second->claimed)
                 return da_status_invalid_input;
             Request *rp = it->second.get();
+            rp->claimed = true;
             done_cv_.wait(g, [&] { return rp->done; });
+            // std::map iterators stay valid across the insertions of concurrent submits, and
+            // only the claiming waiter erases this entry.
             r = std::move(it->second);
             all_.erase(it);
         }
@@ -251,7 +257,7 @@ template <typename T> class Server : public ServerBase {
     }
 
     da_status launch(Slot &sl) {
-        ErrorStack *err = nn_->errors();
+        ErrorStack *err = &werr_;
         hipStream_t st = sl.gpu.stream();
         size_t off = 0;
         for (auto &sg : sl.segs) {
@@ -278,7 +284,7 @@ template <typename T> class Server : public ServerBase {
     }
 
     da_status finish(Slot &sl) {
-        ErrorStack *err = nn_->errors();
+        ErrorStack *err = &werr_;
         hipStream_t st = sl.gpu.stream();
         da_status s = sl.launch_status;
         if (s == da_status_success)
@@ -372,6 +378,9 @@ template <typename T> class Server : public ServerBase {
     std::condition_variable cv_, done_cv_;
     std::deque<Request *> queue_;
     std::map<int, std::unique_ptr<Request>> all_;
+    // Messages of the serving thread's own device calls (the handle's stack belongs to the
+    // caller's thread).
+    ErrorStack werr_;
     int next_id_ = 1;
     bool stop_ = false;
     long long completed_ = 0, queries_done_ = 0, batches_ = 0, queries_batched_ = 0,

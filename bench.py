@@ -27,11 +27,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "kNN queries/s + p99 latency at recall@10=1.0; lowp vs FP32 speedup, 1 GPU"
 PEAK = {"working": 157.3e12, "fp32": 157.3e12, "fp16": 2.5e15, "bf16": 2.5e15, "fp16x3": 2.5e15,
-        "bf16x3": 2.5e15, "f64": 78.6e12}
+        "bf16x3": 2.5e15, "fp16c": 2.5e15, "f64": 78.6e12}
 # MFMA products per contraction term: the split forms issue three.
 PRODUCTS = {"fp16x3": 3, "bf16x3": 3}
 DTYPE_NAME = {"working": "f32", "fp32": "f32", "fp16": "f16", "bf16": "bf16", "fp16x3": "f16",
-              "bf16x3": "bf16"}
+              "bf16x3": "bf16", "fp16c": "f16"}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "knn_tile_200000x128_20000q_k10_traffic.txt")
@@ -621,7 +621,7 @@ def main():
     ap.add_argument("--n-query", type=int, default=20_000)
     ap.add_argument("--dim", type=int, default=128)
     ap.add_argument("--k", type=int, default=10)
-    ap.add_argument("--forms", default="working,fp16x3,bf16x3")
+    ap.add_argument("--forms", default="working,fp16x3,bf16x3,fp16c")
     ap.add_argument("--kb-forms", default="working,bf16x3,bf16",
                     help="cross-term forms of the batched k-means mode")
     ap.add_argument("--tile-variant", type=int, default=-1,

@@ -90,7 +90,8 @@ da_status da_kmeans_last_compute_stats(da_handle handle, da_int *n_rechecked);
 
 /*
  * k-NN serving loop over the resident training set of a kNN handle (da_nn_set_data_? done;
- * the handle must outlive the server and not be modified while it runs). Requests are
+ * the handle must outlive the server and not be used by other calls while it runs: the
+ * serving thread searches through it). Requests are
  * queued by da_nn_server_submit_? (the queries are copied, the call does not block) and
  * served by a worker thread that coalesces queued requests into batches of at most
  * max_batch_queries queries whose device buffers fit in max_device_bytes (admission by
@@ -99,7 +100,8 @@ da_status da_kmeans_last_compute_stats(da_handle handle, da_int *n_rechecked);
  * Every query's result is the exact k nearest neighbours, identical whether the request is
  * served alone or together with others. da_nn_server_wait_? blocks until a request is
  * complete and returns its n_queries x k results (row-major) and its latency (submission to
- * completion, milliseconds).
+ * completion, milliseconds); a request's result goes to one waiter, and a second wait on
+ * the same request id returns da_status_invalid_input.
  */
 typedef struct _da_nn_server *da_nn_server;
 da_status da_nn_server_create(da_handle handle, da_int max_batch_queries,

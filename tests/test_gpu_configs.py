@@ -60,7 +60,7 @@ def _check_knn(Xt, Xq, ind, dist, ei, ed, sample, rtol):
         assert ((Ds >= kth * (1 - 1e-9) - 1e-9) | inres).all()
 
 
-@pytest.mark.parametrize("precision", ["working", "bf16x3", "fp16x3"])
+@pytest.mark.parametrize("precision", ["working", "bf16x3", "fp16x3", "fp16c"])
 def test_knn_200000x128_fp32(knn_data, precision):
     Xt, Xq, ei, ed, sample = knn_data
     nn = aoclda.neighbors.nearest_neighbors(n_neighbors=10, precision=precision).fit(Xt)

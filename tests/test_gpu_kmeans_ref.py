@@ -52,7 +52,8 @@ def test_reference_fixed_random_inits(known, i, dtype):
               n_init=c["n_init"], max_iter=c["max_iter"], tol=c["tol"], seed=c["seed"],
               algorithm=c["algorithm"])
     assert [n, d, k, km.n_iter, km.inertia, km.lp_n_iter] == c["rinfo"]
-    np.testing.assert_array_equal(km.cluster_centres, colmajor(c["centres_colmajor"], k, d))
+    np.testing.assert_array_equal(km.cluster_centres,
+                                  colmajor(c["centres_colmajor"], k, d).astype(dtype))
     np.testing.assert_array_equal(km.labels, c["labels"])
     X = colmajor(c["X_colmajor"], c["m_samples"], d).astype(dtype)
     # the reference compares the transform within its tolerance: 3.3 - 2.1 is not 1.2 in

@@ -149,7 +149,8 @@ def gaussian_mixture(rng, n, means, sigma=1.0):
     return means[comp] + sigma * rng.standard_normal((n, means.shape[1]))
 
 
-@pytest.mark.parametrize("precision", ["working", "fp32", "fp16", "bf16", "fp16x3", "bf16x3"])
+@pytest.mark.parametrize("precision", ["working", "fp32", "fp16", "bf16", "fp16x3", "bf16x3",
+                                       "fp16c"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("d", [20, 96, 128])
 def test_precision_forms_exact(oracle, precision, dtype, d):
@@ -212,13 +213,14 @@ def test_fused_large_k(oracle, k, precision):
     assert ncand.value >= k  # the fused tile path ran
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
 @pytest.mark.parametrize("precision", ["working", "fp64", "fp16", "fp16x3", "bf16x3"])
 @pytest.mark.parametrize("d", [20, 64, 128])
 @pytest.mark.parametrize("k", [5, 12, 30])
 def test_tile_variants(oracle, variant, precision, d, k):
     """Every sweep of the fused tile (two training tiles per step with the loads under the
-    bookkeeping, load-then-compute, one tile per step) gives the exact result, for short
+    bookkeeping, load-then-compute, one tile per step, the rolling prefetch) gives the exact
+    result, for short
     (16) and longer (32) candidate lists, including training sizes that end inside a step
     and splits of unequal length."""
     rng = np.random.default_rng(200 + d)
@@ -236,7 +238,7 @@ def test_tile_variants(oracle, variant, precision, d, k):
         np.testing.assert_allclose(dist, ed, rtol=1e-12 if dtype == np.float64 else 1e-5)
 
 
-@pytest.mark.parametrize("variant", [0, 2])
+@pytest.mark.parametrize("variant", [0, 2, 3])
 @pytest.mark.parametrize("k", [10, 100])
 def test_long_splits_16bit_offsets(oracle, variant, k):
     """A training set of more than 65,536 rows: sweeps that buffer 16-bit row offsets (16
@@ -290,3 +292,35 @@ def test_lists_shorter_than_k(oracle, precision, k, kc):
     ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), k)
     np.testing.assert_array_equal(ind, ei)
     np.testing.assert_allclose(dist, ed, rtol=1e-5)
+
+
+@pytest.mark.parametrize("n,d", [(16, 8), (383, 40), (1000, 20), (4133, 64), (52000, 128),
+                                 (70001, 24)])
+@pytest.mark.parametrize("k", [1, 5, 16, 30, 100])
+def test_centred_fp16_form(oracle, n, d, k):
+    """fp16 training rows stored about the centres of k-means groups (one group per 384
+    rows, padded to whole 32-row steps, in a random order): the exact neighbours with the
+    original indices, on data far from the origin (so that the centring matters), with
+    duplicated rows (ties go to the lower index) and sizes from a single group to
+    several 16-bit-offset splits; most queries are certified by the fused tile."""
+    if k > n:
+        pytest.skip("k > n")
+    rng = np.random.default_rng(n + d)
+    means = rng.uniform(-30, 30, (24, d))
+    Xt = gaussian_mixture(rng, n, means).astype(np.float32)
+    if n > 100:
+        Xt[n // 2: n // 2 + 20] = Xt[:20]  # exact duplicates
+    Xq = gaussian_mixture(rng, 300, means).astype(np.float32)
+    Xq[:5] = Xt[:5]
+    nn = aoclda.neighbors.nearest_neighbors(precision="fp16c").fit(Xt)
+    dist, ind = nn.kneighbors(Xq, n_neighbors=k)
+    ei, ed = oracle.knn(Xt.astype(np.float64), Xq.astype(np.float64), k)
+    np.testing.assert_array_equal(ind, ei)
+    np.testing.assert_allclose(dist, ed, rtol=1e-5, atol=1e-6)
+    nfb, ncand = L.da_int(), L.da_int()
+    L.check(L.lib.da_nn_last_query_stats(nn.handle.h, C.byref(nfb), C.byref(ncand)))
+    assert ncand.value > 0
+    # a handle with the split form returns the same indices
+    nn2 = aoclda.neighbors.nearest_neighbors(precision="bf16x3").fit(Xt)
+    d2, i2 = nn2.kneighbors(Xq, n_neighbors=k)
+    np.testing.assert_array_equal(i2, ind)
