@@ -12,6 +12,11 @@
 // identical whatever the order in which points are visited or workgroups are scheduled,
 // and the totals are exact to about 2^-100 of the data range. The exponents are chosen on
 // the host from the column maxima so that no partial or total can overflow.
+//
+// Spherical k-means (the "distance" option "cosine" of the reference) shares these kernels:
+// the assignment takes the argmax of x.c (the reference's GEMM form with zero centre norms,
+// kmeans_lloyd.hpp:131-146), the sums are of x or of x/|x| (kmeans_lloyd.hpp:279-300) and
+// the centres are normalised rather than divided by the counts (:363-405).
 #include "launch.hpp"
 #include <type_traits>
 #include <algorithm>
@@ -23,6 +28,18 @@ namespace dagpu {
 namespace {
 
 constexpr int CS_ELEMS = 4096; // centres staged in LDS per chunk (doubles)
+
+// One term of the assignment key: (x - c)^2 for the Euclidean k-means, -x c for the
+// spherical one (the key -x.c is minimised, so the largest x.c wins with the same ties).
+template <bool SPH>
+__device__ __forceinline__ double km_term(double x, double c, double acc) {
+    if constexpr (SPH) {
+        return fma(-x, c, acc);
+    } else {
+        const double t = x - c;
+        return fma(t, t, acc);
+    }
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void col_absmax_kernel(const T *__restrict__ X, int n, int d,
@@ -65,7 +82,7 @@ __global__ __launch_bounds__(256) void col_absmax_kernel(const T *__restrict__ X
     }
 }
 
-template <typename T, int DMAX>
+template <typename T, int DMAX, bool SPH>
 __global__ __launch_bounds__(256) void kmeans_assign_kernel(
     const T *__restrict__ X, int n, int d, const double *__restrict__ C, int k,
     int *__restrict__ labels, const int *__restrict__ prev, int *__restrict__ changed,
@@ -89,10 +106,8 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
             for (int c = 0; c < k; c++) {
                 const double *p = C + (int64_t)c * d;
                 double acc = 0.0;
-                for (int l = 0; l < d; l++) {
-                    const double t = (double)X[(int64_t)i * d + l] - p[l];
-                    acc = fma(t, t, acc);
-                }
+                for (int l = 0; l < d; l++)
+                    acc = km_term<SPH>((double)X[(int64_t)i * d + l], p[l], acc);
                 if (acc < best) {
                     best = acc;
                     bl = c;
@@ -127,12 +142,10 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
                 for (int l = 0; l < DMAX; l++) {
                     if (l < d) {
                         const double xl = (double)x[l];
-                        const double t0 = xl - p[l], t1 = xl - p[d + l];
-                        const double t2 = xl - p[2 * d + l], t3 = xl - p[3 * d + l];
-                        a0 = fma(t0, t0, a0);
-                        a1 = fma(t1, t1, a1);
-                        a2 = fma(t2, t2, a2);
-                        a3 = fma(t3, t3, a3);
+                        a0 = km_term<SPH>(xl, p[l], a0);
+                        a1 = km_term<SPH>(xl, p[d + l], a1);
+                        a2 = km_term<SPH>(xl, p[2 * d + l], a2);
+                        a3 = km_term<SPH>(xl, p[3 * d + l], a3);
                     }
                 }
                 if (a0 < best) {
@@ -158,16 +171,12 @@ __global__ __launch_bounds__(256) void kmeans_assign_kernel(
             if constexpr (DMAX > 0) {
 #pragma unroll
                 for (int l = 0; l < DMAX; l++) {
-                    if (l < d) {
-                        const double t = (double)x[l] - Cs[c * d + l];
-                        acc = fma(t, t, acc);
-                    }
+                    if (l < d)
+                        acc = km_term<SPH>((double)x[l], Cs[c * d + l], acc);
                 }
             } else {
-                for (int l = 0; l < d; l++) {
-                    const double t = (double)X[(int64_t)i * d + l] - Cs[c * d + l];
-                    acc = fma(t, t, acc);
-                }
+                for (int l = 0; l < d; l++)
+                    acc = km_term<SPH>((double)X[(int64_t)i * d + l], Cs[c * d + l], acc);
             }
             if (acc < best) {
                 best = acc;
@@ -227,11 +236,12 @@ __device__ void block_exclusive_scan(int *v, int m, int *wsum) {
 // point), then thread pairs (cluster, feature) sum their cluster's values in registers and
 // add the workgroup totals to the device-wide accumulators. Integer sums are exact, so
 // neither the order inside a cluster nor the order of the workgroups changes the result.
+// With winv, point i contributes x_i * winv[i] (the normalised samples of spherical k-means).
 template <typename T>
 __global__ __launch_bounds__(1024) void kmeans_partial_kernel(
     const T *__restrict__ X, int n, int d, const int *__restrict__ labels, int c0, int kc,
     const int *__restrict__ escale, int ppb, long long *__restrict__ part,
-    int *__restrict__ cnt) {
+    int *__restrict__ cnt, const double *__restrict__ winv) {
     extern __shared__ int sh[];
     int *off = sh;                 // kc + 1 offsets
     int *fill = off + kc + 1;      // kc fill counters
@@ -267,8 +277,9 @@ __global__ __launch_bounds__(1024) void kmeans_partial_kernel(
             continue;
         const int el = escale[l];
         long long H = 0, L = 0;
-        auto add = [&](T x) {
-            const double v = (double)x;
+        const double *wv = winv ? winv + start : nullptr;
+        auto add = [&](T x, int j) {
+            const double v = wv ? (double)x * wv[list[j]] : (double)x;
             if (v == v && v - v == 0.0) {
                 const double sv = ldexp(v, el);
                 const double h = rint(sv);
@@ -286,10 +297,10 @@ __global__ __launch_bounds__(1024) void kmeans_partial_kernel(
                 v[u] = xs[(int64_t)list[j + u] * d];
 #pragma unroll
             for (int u = 0; u < 8; u++)
-                add(v[u]);
+                add(v[u], j + u);
         }
         for (; j < j1; j++)
-            add(xs[(int64_t)list[j] * d]);
+            add(xs[(int64_t)list[j] * d], j);
         if (H != 0)
             atomicAdd(reinterpret_cast<unsigned long long *>(&part[2 * (int64_t)p]),
                       (unsigned long long)H);
@@ -302,15 +313,19 @@ __global__ __launch_bounds__(1024) void kmeans_partial_kernel(
 }
 
 // One workgroup per cluster: the centre from the device-wide totals, which are then cleared
-// (the count once every thread of the workgroup has read it).
+// (the count once every thread of the workgroup has read it). sph: the sum normalised to unit
+// length instead of divided by the count, a zero sum left at the origin (normalize_rows_inplace,
+// da_utils.cpp:807-855).
 __global__ __launch_bounds__(256) void kmeans_reduce_kernel(long long *__restrict__ part,
                                                             int *__restrict__ cnt, int c0, int d,
                                                             const int *__restrict__ escale,
                                                             double *__restrict__ C,
-                                                            int *__restrict__ counts) {
+                                                            int *__restrict__ counts, int sph) {
+    __shared__ double red[256];
     const int c = blockIdx.x;
     const int count = cnt[c];
     const int lshift = escale[d];
+    double ss = 0.0;
     for (int l = threadIdx.x; l < d; l += blockDim.x) {
         const int64_t e = (int64_t)c * d + l;
         const long long H = part[2 * e], L = part[2 * e + 1];
@@ -318,9 +333,23 @@ __global__ __launch_bounds__(256) void kmeans_reduce_kernel(long long *__restric
         const double sum = ldexp((double)H, -ex) + ldexp((double)L, -(ex + lshift));
         // An empty cluster keeps count 1 and a zero sum, i.e. the origin
         // (scale_current_cluster_centres, kmeans_lloyd.hpp:363-373).
-        C[(int64_t)(c0 + c) * d + l] = sum / (double)(count > 0 ? count : 1);
+        C[(int64_t)(c0 + c) * d + l] = sph ? sum : sum / (double)(count > 0 ? count : 1);
+        ss = fma(sum, sum, ss);
         part[2 * e] = 0;
         part[2 * e + 1] = 0;
+    }
+    if (sph) {
+        red[threadIdx.x] = ss;
+        __syncthreads();
+        for (int m = 128; m > 0; m >>= 1) {
+            if ((int)threadIdx.x < m && threadIdx.x + m < blockDim.x)
+                red[threadIdx.x] += red[threadIdx.x + m];
+            __syncthreads();
+        }
+        const double nrm = sqrt(red[0]);
+        const double inv = nrm == 0.0 ? 1.0 : 1.0 / nrm;
+        for (int l = threadIdx.x; l < d; l += blockDim.x)
+            C[(int64_t)(c0 + c) * d + l] *= inv;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -442,6 +471,43 @@ __device__ __forceinline__ double sqdist_xx(const T *__restrict__ X, int d, int6
     return acc;
 }
 
+// Dot products of the same operands (spherical k-means).
+template <typename T>
+__device__ __forceinline__ double dot_rows(const T *__restrict__ X, int d, int64_t i,
+                                           const double *__restrict__ cs) {
+    double acc = 0.0;
+    for (int l = 0; l < d; l++)
+        acc = fma((double)X[i * d + l], cs[l], acc);
+    return acc;
+}
+template <typename T>
+__device__ __forceinline__ double dot_xx(const T *__restrict__ X, int d, int64_t i, int64_t j) {
+    double acc = 0.0;
+    for (int l = 0; l < d; l++)
+        acc = fma((double)X[i * d + l], (double)X[j * d + l], acc);
+    return acc;
+}
+
+// The distance of samples i and j used by the initialisations: the squared Euclidean
+// distance, or for spherical k-means (sph) max(0, 1 - x_i.x_j inv_i inv_j) with the inverse
+// norms inv (compute_distances_to_point, kmeans.cpp:1488-1522; without inv the samples are
+// taken as unit rows). cs: row j staged in LDS, or null to read it from global memory.
+struct KmGeom {
+    int sph;
+    const double *inv;
+};
+template <typename T>
+__device__ __forceinline__ double sample_dist(const T *__restrict__ X, int d, int64_t i,
+                                              int64_t j, const double *cs, KmGeom g) {
+    if (!g.sph)
+        return cs ? sqdist_rows(X, d, i, cs) : sqdist_xx(X, d, i, j);
+    double cosv = cs ? dot_rows(X, d, i, cs) : dot_xx(X, d, i, j);
+    if (g.inv)
+        cosv = cosv * g.inv[i] * g.inv[j];
+    const double v = 1.0 - cosv;
+    return v > 0.0 ? v : 0.0;
+}
+
 // Rows of at most ROW_STAGE_MAX features are staged in LDS by the distance passes of the
 // initialisations; longer rows are read from global memory.
 constexpr int ROW_STAGE_MAX = 8192;
@@ -449,7 +515,7 @@ constexpr int ROW_STAGE_MAX = 8192;
 template <typename T>
 __global__ __launch_bounds__(256) void dist_to_row_kernel(const T *__restrict__ X, int n, int d,
                                                           int ci, double *__restrict__ w,
-                                                          int mode) {
+                                                          int mode, KmGeom g) {
     extern __shared__ double cs[];
     const bool staged = d <= ROW_STAGE_MAX;
     if (staged) {
@@ -460,7 +526,7 @@ __global__ __launch_bounds__(256) void dist_to_row_kernel(const T *__restrict__ 
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n)
         return;
-    double v = staged ? sqdist_rows(X, d, i, cs) : sqdist_xx(X, d, i, ci);
+    double v = sample_dist(X, d, i, ci, staged ? cs : nullptr, g);
     if (mode == 1) {
         // min update: max(0, min(w, v))
         const double o = w[i];
@@ -572,7 +638,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void candidate_cost_kernel(const T *__restrict__ X, int n, int d,
                                                              const double *__restrict__ w,
                                                              const int *__restrict__ cand, int nt,
-                                                             double *__restrict__ part, int nb) {
+                                                             double *__restrict__ part, int nb,
+                                                             KmGeom g) {
     extern __shared__ double cs[]; // nt x d when staged
     __shared__ double red[256];
     const bool staged = (int64_t)nt * d <= ROW_STAGE_MAX;
@@ -592,8 +659,7 @@ __global__ __launch_bounds__(256) void candidate_cost_kernel(const T *__restrict
 #pragma unroll
         for (int c = 0; c < NT_MAX; c++) {
             if (c < nt) {
-                double v = staged ? sqdist_rows(X, d, i, cs + c * d)
-                                  : sqdist_xx(X, d, i, (int64_t)cand[c]);
+                double v = sample_dist(X, d, i, (int64_t)cand[c], staged ? cs + c * d : nullptr, g);
                 v = wi < v ? wi : v;
                 acc[c] += v > 0.0 ? v : 0.0;
             }
@@ -631,21 +697,48 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const double *__restri
         out[t] = s[0];
 }
 
+// Distances of rows of X to the centres: Euclidean, or for spherical k-means (sph) the cosine
+// distance 1 - clamp(x.c inv_x, -1, 1) (kmeans.cpp:78-105, 821-863), inv_x = 1 without inv.
 template <typename T>
 __global__ __launch_bounds__(256) void kmeans_transform_kernel(const T *__restrict__ X, int n, int d,
                                                                const double *__restrict__ C, int k,
-                                                               T *__restrict__ D, int64_t ldd) {
+                                                               T *__restrict__ D, int64_t ldd,
+                                                               KmGeom g) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= (int64_t)n * k)
         return;
     const int64_t i = e / k;
     const int c = (int)(e - i * k);
+    if (g.sph) {
+        double cosv = dot_rows(X, d, i, C + (int64_t)c * d);
+        if (g.inv)
+            cosv *= g.inv[i];
+        cosv = cosv < -1.0 ? -1.0 : (cosv > 1.0 ? 1.0 : cosv);
+        D[i * ldd + c] = (T)(1.0 - cosv);
+        return;
+    }
     double acc = 0.0;
     for (int l = 0; l < d; l++) {
         const double t = (double)X[i * d + l] - C[(int64_t)c * d + l];
         acc = fma(t, t, acc);
     }
     D[i * ldd + c] = (T)sqrt(acc);
+}
+
+// inv[i] = 1/|X_i| (0 for a zero row), the norm in double.
+template <typename T>
+__global__ __launch_bounds__(256) void row_inv_norms_kernel(const T *__restrict__ X, int n, int d,
+                                                            double *__restrict__ inv) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    double s = 0.0;
+    for (int l = 0; l < d; l++) {
+        const double v = (double)X[i * d + l];
+        s = fma(v, v, s);
+    }
+    const double nrm = sqrt(s);
+    inv[i] = nrm > 0.0 ? 1.0 / nrm : 0.0;
 }
 
 __host__ inline unsigned grid_for(int64_t n, int block, int64_t cap = 65536) {
@@ -668,38 +761,47 @@ hipError_t launch_col_absmax(const T *X, int n, int d, double *out, hipStream_t 
     return hipGetLastError();
 }
 
+template <typename T, bool SPH>
+static void assign_dispatch(dim3 grid, dim3 block, hipStream_t s, const T *X, int n, int d,
+                            const double *C, int k, int *labels, const int *prev, int *changed,
+                            double *mind) {
+#define DA_KM_ASSIGN(DM)                                                                       \
+    hipLaunchKernelGGL((kmeans_assign_kernel<T, DM, SPH>), grid, block, 0, s, X, n, d, C, k,  \
+                       labels, prev, changed, mind)
+    if (d <= 8)
+        DA_KM_ASSIGN(8);
+    else if (d <= 16)
+        DA_KM_ASSIGN(16);
+    else if (d <= 32)
+        DA_KM_ASSIGN(32);
+    else if (d <= 64)
+        DA_KM_ASSIGN(64);
+    else if (std::is_same_v<T, float> && d <= 96)
+        DA_KM_ASSIGN(96);
+    else if (std::is_same_v<T, float> && d <= 128)
+        DA_KM_ASSIGN(128);
+    else
+        DA_KM_ASSIGN(0);
+#undef DA_KM_ASSIGN
+}
+
 template <typename T>
 hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k, int *labels,
-                                const int *prev, int *changed, double *mind, hipStream_t s) {
+                                const int *prev, int *changed, double *mind, hipStream_t s,
+                                int sph) {
     dim3 grid((n + 255) / 256), block(256);
-    if (d <= 8)
-        hipLaunchKernelGGL((kmeans_assign_kernel<T, 8>), grid, block, 0, s, X, n, d, C, k, labels,
-                           prev, changed, mind);
-    else if (d <= 16)
-        hipLaunchKernelGGL((kmeans_assign_kernel<T, 16>), grid, block, 0, s, X, n, d, C, k,
-                           labels, prev, changed, mind);
-    else if (d <= 32)
-        hipLaunchKernelGGL((kmeans_assign_kernel<T, 32>), grid, block, 0, s, X, n, d, C, k,
-                           labels, prev, changed, mind);
-    else if (d <= 64)
-        hipLaunchKernelGGL((kmeans_assign_kernel<T, 64>), grid, block, 0, s, X, n, d, C, k,
-                           labels, prev, changed, mind);
-    else if (std::is_same_v<T, float> && d <= 96)
-        hipLaunchKernelGGL((kmeans_assign_kernel<T, 96>), grid, block, 0, s, X, n, d, C, k,
-                           labels, prev, changed, mind);
-    else if (std::is_same_v<T, float> && d <= 128)
-        hipLaunchKernelGGL((kmeans_assign_kernel<T, 128>), grid, block, 0, s, X, n, d, C, k,
-                           labels, prev, changed, mind);
+    if (sph)
+        assign_dispatch<T, true>(grid, block, s, X, n, d, C, k, labels, prev, changed, mind);
     else
-        hipLaunchKernelGGL((kmeans_assign_kernel<T, 0>), grid, block, 0, s, X, n, d, C, k, labels,
-                           prev, changed, mind);
+        assign_dispatch<T, false>(grid, block, s, X, n, d, C, k, labels, prev, changed, mind);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *labels, int c0,
                                       int kc, const int *escale, int ppb, int nblocks,
-                                      long long *part, int *cnt, hipStream_t s) {
+                                      long long *part, int *cnt, hipStream_t s,
+                                      const double *winv) {
     const size_t lds = kmeans_partial_lds(kc, ppb);
     if (lds > 160 * 1024)
         return hipErrorInvalidValue;
@@ -707,7 +809,7 @@ hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *label
         (void)hipFuncSetAttribute((const void *)kmeans_partial_kernel<T>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(kmeans_partial_kernel<T>, dim3(nblocks), dim3(1024), lds, s, X, n, d, labels,
-                       c0, kc, escale, ppb, part, cnt);
+                       c0, kc, escale, ppb, part, cnt, winv);
     return hipGetLastError();
 }
 
@@ -716,9 +818,10 @@ size_t kmeans_partial_lds(int kc, int ppb) {
 }
 
 hipError_t launch_kmeans_reduce(long long *part, int *cnt, int c0, int kc, int d,
-                                const int *escale, double *C, int *counts, hipStream_t s) {
+                                const int *escale, double *C, int *counts, hipStream_t s,
+                                int sph) {
     hipLaunchKernelGGL(kmeans_reduce_kernel, dim3(kc), dim3(d < 256 ? (d + 63) / 64 * 64 : 256),
-                       0, s, part, cnt, c0, d, escale, C, counts);
+                       0, s, part, cnt, c0, d, escale, C, counts, sph);
     return hipGetLastError();
 }
 
@@ -768,15 +871,17 @@ static size_t row_stage_bytes(int64_t elems) {
 }
 
 template <typename T>
-hipError_t launch_dist_to_row(const T *X, int n, int d, int ci, double *w, hipStream_t s) {
+hipError_t launch_dist_to_row(const T *X, int n, int d, int ci, double *w, hipStream_t s, int sph,
+                              const double *inv) {
     hipLaunchKernelGGL(dist_to_row_kernel<T>, dim3((n + 255) / 256), dim3(256),
-                       row_stage_bytes(d), s, X, n, d, ci, w, 0);
+                       row_stage_bytes(d), s, X, n, d, ci, w, 0, KmGeom{sph, inv});
     return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_min_update(const T *X, int n, int d, int ci, double *w, hipStream_t s) {
+hipError_t launch_min_update(const T *X, int n, int d, int ci, double *w, hipStream_t s, int sph,
+                             const double *inv) {
     hipLaunchKernelGGL(dist_to_row_kernel<T>, dim3((n + 255) / 256), dim3(256),
-                       row_stage_bytes(d), s, X, n, d, ci, w, 1);
+                       row_stage_bytes(d), s, X, n, d, ci, w, 1, KmGeom{sph, inv});
     return hipGetLastError();
 }
 
@@ -798,11 +903,13 @@ hipError_t launch_lower_bound(const double *cp, int n, const double *u, int nt, 
 
 template <typename T>
 hipError_t launch_candidate_cost(const T *X, int n, int d, const double *w, const int *cand,
-                                 int nt, double *part, int nb, hipStream_t s) {
+                                 int nt, double *part, int nb, hipStream_t s, int sph,
+                                 const double *inv) {
     if (nt > NT_MAX)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(candidate_cost_kernel<T>, dim3(nb), dim3(256),
-                       row_stage_bytes((int64_t)nt * d), s, X, n, d, w, cand, nt, part, nb);
+                       row_stage_bytes((int64_t)nt * d), s, X, n, d, w, cand, nt, part, nb,
+                       KmGeom{sph, inv});
     return hipGetLastError();
 }
 
@@ -813,30 +920,71 @@ hipError_t launch_reduce_rows(const double *part, int nt, int nb, double *out, h
 
 template <typename T>
 hipError_t launch_kmeans_transform(const T *X, int n, int d, const double *C, int k, T *D,
-                                   int64_t ldd, hipStream_t s) {
+                                   int64_t ldd, hipStream_t s, int sph, const double *inv) {
     const int64_t total = (int64_t)n * k;
     hipLaunchKernelGGL(kmeans_transform_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256),
-                       0, s, X, n, d, C, k, D, ldd);
+                       0, s, X, n, d, C, k, D, ldd, KmGeom{sph, inv});
+    return hipGetLastError();
+}
+
+// One workgroup per row of C (k x d doubles): the row scaled to unit length, a zero row left
+// as it is (normalize_rows_inplace, da_utils.cpp:807-855).
+__global__ __launch_bounds__(256) void normalize_rows_kernel(double *__restrict__ C, int d) {
+    __shared__ double red[256];
+    double *row = C + (int64_t)blockIdx.x * d;
+    double ss = 0.0;
+    for (int l = threadIdx.x; l < d; l += 256)
+        ss = fma(row[l], row[l], ss);
+    red[threadIdx.x] = ss;
+    __syncthreads();
+    for (int m = 128; m > 0; m >>= 1) {
+        if ((int)threadIdx.x < m)
+            red[threadIdx.x] += red[threadIdx.x + m];
+        __syncthreads();
+    }
+    const double nrm = sqrt(red[0]);
+    const double inv = nrm == 0.0 ? 1.0 : 1.0 / nrm;
+    for (int l = threadIdx.x; l < d; l += 256)
+        row[l] *= inv;
+}
+
+hipError_t launch_normalize_rows(double *C, int k, int d, hipStream_t s) {
+    if (k <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(normalize_rows_kernel, dim3(k), dim3(256), 0, s, C, d);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_row_inv_norms(const T *X, int n, int d, double *inv, hipStream_t s) {
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(row_inv_norms_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, s, X, n, d,
+                       inv);
     return hipGetLastError();
 }
 
 #define DA_KM_INST(T)                                                                           \
     template hipError_t launch_col_absmax<T>(const T *, int, int, double *, hipStream_t);       \
     template hipError_t launch_kmeans_assign<T>(const T *, int, int, const double *, int, int *, \
-                                                const int *, int *, double *, hipStream_t);     \
+                                                const int *, int *, double *, hipStream_t, int); \
     template hipError_t launch_kmeans_partial_sums<T>(const T *, int, int, const int *, int, int, \
                                                       const int *, int, int, long long *, int *, \
-                                                      hipStream_t);                              \
+                                                      hipStream_t, const double *);              \
     template hipError_t launch_convert<T>(const T *, int64_t, double *, hipStream_t);           \
     template hipError_t launch_convert_back<T>(const double *, int64_t, T *, hipStream_t);      \
     template hipError_t launch_gather_rows<T>(const T *, int, const int *, int, double *,       \
                                               hipStream_t);                                      \
-    template hipError_t launch_dist_to_row<T>(const T *, int, int, int, double *, hipStream_t); \
-    template hipError_t launch_min_update<T>(const T *, int, int, int, double *, hipStream_t);  \
+    template hipError_t launch_dist_to_row<T>(const T *, int, int, int, double *, hipStream_t,  \
+                                              int, const double *);                              \
+    template hipError_t launch_min_update<T>(const T *, int, int, int, double *, hipStream_t,   \
+                                             int, const double *);                               \
     template hipError_t launch_candidate_cost<T>(const T *, int, int, const double *,           \
-                                                 const int *, int, double *, int, hipStream_t); \
+                                                 const int *, int, double *, int, hipStream_t,  \
+                                                 int, const double *);                           \
     template hipError_t launch_kmeans_transform<T>(const T *, int, int, const double *, int, T *, \
-                                                   int64_t, hipStream_t);
+                                                   int64_t, hipStream_t, int, const double *);  \
+    template hipError_t launch_row_inv_norms<T>(const T *, int, int, double *, hipStream_t);
 DA_KM_INST(float)
 DA_KM_INST(double)
 #undef DA_KM_INST
@@ -845,15 +993,24 @@ DA_KM_INST(double)
 
 namespace dagpu {
 namespace {
+// The inertia term of each sample: squared distance to its centre, or for spherical k-means
+// (sph) 1 - x.c inv_x (compute_current_inertia, kmeans.cpp:1060-1124; inv_x = 1 without inv).
 template <typename T>
 __global__ void label_sqdist_kernel(const T *__restrict__ X, int n, int d,
                                     const double *__restrict__ C, const int *__restrict__ labels,
-                                    double *__restrict__ out) {
+                                    double *__restrict__ out, int sph,
+                                    const double *__restrict__ inv) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     const double *c = C + (int64_t)labels[i] * d;
     double acc = 0.0;
+    if (sph) {
+        for (int l = 0; l < d; l++)
+            acc = fma((double)X[i * d + l], c[l], acc);
+        out[i] = 1.0 - (inv ? acc * inv[i] : acc);
+        return;
+    }
     for (int l = 0; l < d; l++) {
         const double t = (double)X[i * d + l] - c[l];
         acc = fma(t, t, acc);
@@ -863,15 +1020,17 @@ __global__ void label_sqdist_kernel(const T *__restrict__ X, int n, int d,
 } // namespace
 template <typename T>
 hipError_t launch_label_sqdist(const T *X, int n, int d, const double *C, const int *labels,
-                               double *out, hipStream_t s) {
+                               double *out, hipStream_t s, int sph, const double *inv) {
     hipLaunchKernelGGL(label_sqdist_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, s, X, n, d, C,
-                       labels, out);
+                       labels, out, sph, inv);
     return hipGetLastError();
 }
 template hipError_t launch_label_sqdist<float>(const float *, int, int, const double *,
-                                               const int *, double *, hipStream_t);
+                                               const int *, double *, hipStream_t, int,
+                                               const double *);
 template hipError_t launch_label_sqdist<double>(const double *, int, int, const double *,
-                                                const int *, double *, hipStream_t);
+                                                const int *, double *, hipStream_t, int,
+                                                const double *);
 } // namespace dagpu
 
 namespace dagpu {

@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void kmeans_assign_bf16_kernel(
 // Exact re-check of listed samples: argmin of double squared distances, ties to the lowest
 // centre. One wave per listed sample (fixed grid, the count is read on the device): lane c
 // takes centres c, c + 64, ..., then a wave argmin by (distance, index).
-template <typename T>
+template <typename T, bool SPH>
 __global__ __launch_bounds__(256) void kmeans_recheck_kernel(
     const T *__restrict__ X, int ldx, int d, const double *__restrict__ CT, int k,
     const int *__restrict__ list_count, const int *__restrict__ list, int *__restrict__ labels,
@@ -317,8 +317,13 @@ __global__ __launch_bounds__(256) void kmeans_recheck_kernel(
             // Centres transposed (feature-major): the 64 lanes read 64 consecutive centres.
             double acc = 0.0;
             for (int l = 0; l < d; l++) {
-                const double v = (double)x[l] - CT[(int64_t)l * k + c];
-                acc = fma(v, v, acc);
+                const double cv = CT[(int64_t)l * k + c];
+                if constexpr (SPH) {
+                    acc = fma(-(double)x[l], cv, acc); // key -x.c
+                } else {
+                    const double v = (double)x[l] - cv;
+                    acc = fma(v, v, acc);
+                }
             }
             if (acc < best) {
                 best = acc;
@@ -343,7 +348,7 @@ __global__ __launch_bounds__(256) void kmeans_recheck_kernel(
 
 __global__ void centre_prep_kernel(const double *__restrict__ C, int k, int d, int Kp,
                                    float *__restrict__ C32, float *__restrict__ chalf,
-                                   double *__restrict__ cmax2) {
+                                   double *__restrict__ cmax2, int sph) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= k)
         return;
@@ -354,7 +359,7 @@ __global__ void centre_prep_kernel(const double *__restrict__ C, int k, int d, i
         C32[(int64_t)c * Kp + l] = f;
         s = fma(v, v, s);
     }
-    chalf[c] = (float)(-0.5 * s);
+    chalf[c] = sph ? 0.0f : (float)(-0.5 * s);
     atomicMax(reinterpret_cast<unsigned long long *>(cmax2), (unsigned long long)__double_as_longlong(s));
 }
 
@@ -362,7 +367,8 @@ __global__ void centre_prep_kernel(const double *__restrict__ C, int k, int d, i
 // the largest ||c||^2 (atomic max on the bit pattern of a non-negative double).
 __global__ void centre_prep_bf16_kernel(const double *__restrict__ C, int k, int kpad, int d,
                                         int Kp, __bf16 *__restrict__ C16,
-                                        float *__restrict__ chalf, double *__restrict__ cmax2) {
+                                        float *__restrict__ chalf, double *__restrict__ cmax2,
+                                        int sph) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= kpad)
         return;
@@ -373,7 +379,7 @@ __global__ void centre_prep_bf16_kernel(const double *__restrict__ C, int k, int
         s = fma(v, v, s);
     }
     if (c < k) {
-        chalf[c] = (float)(-0.5 * s);
+        chalf[c] = sph ? 0.0f : (float)(-0.5 * s);
         atomicMax(reinterpret_cast<unsigned long long *>(cmax2),
                   (unsigned long long)__double_as_longlong(s));
     }
@@ -392,10 +398,10 @@ __global__ void to_bf16_rows_kernel(const float *__restrict__ X, int n, int d, i
 } // namespace
 
 hipError_t launch_centre_prep_bf16(const double *C, int k, int d, int Kp, void *C16,
-                                   float *chalf, double *cmax2, hipStream_t s) {
+                                   float *chalf, double *cmax2, hipStream_t s, int sph) {
     const int kpad = (k + 15) / 16 * 16;
     hipLaunchKernelGGL(centre_prep_bf16_kernel, dim3((kpad + 255) / 256), dim3(256), 0, s, C, k,
-                       kpad, d, Kp, static_cast<__bf16 *>(C16), chalf, cmax2);
+                       kpad, d, Kp, static_cast<__bf16 *>(C16), chalf, cmax2, sph);
     return hipGetLastError();
 }
 
@@ -446,9 +452,9 @@ hipError_t launch_kmeans_assign_bf16(const void *X, int n, int Kp, const void *C
 }
 
 hipError_t launch_centre_prep(const double *C, int k, int d, int Kp, float *C32, float *chalf,
-                              double *cmax2, hipStream_t s) {
+                              double *cmax2, hipStream_t s, int sph) {
     hipLaunchKernelGGL(centre_prep_kernel, dim3((k + 255) / 256), dim3(256), 0, s, C, k, d, Kp,
-                       C32, chalf, cmax2);
+                       C32, chalf, cmax2, sph);
     return hipGetLastError();
 }
 
@@ -502,19 +508,24 @@ __global__ __launch_bounds__(256) void transpose_centres_kernel(const double *__
 template <typename T>
 hipError_t launch_kmeans_recheck(const T *X, int ldx, int d, const double *C, int k,
                                  const int *list_count, const int *list, int *labels,
-                                 const int *prev, int *changed, double *CT, hipStream_t s) {
+                                 const int *prev, int *changed, double *CT, hipStream_t s,
+                                 int sph) {
     const int64_t kd = (int64_t)k * d;
     hipLaunchKernelGGL(transpose_centres_kernel, dim3((unsigned)((kd + 255) / 256)), dim3(256), 0,
                        s, C, k, d, CT);
-    hipLaunchKernelGGL(kmeans_recheck_kernel<T>, dim3(1024), dim3(256), 0, s, X, ldx, d, CT, k,
-                       list_count, list, labels, prev, changed);
+    if (sph)
+        hipLaunchKernelGGL((kmeans_recheck_kernel<T, true>), dim3(1024), dim3(256), 0, s, X, ldx,
+                           d, CT, k, list_count, list, labels, prev, changed);
+    else
+        hipLaunchKernelGGL((kmeans_recheck_kernel<T, false>), dim3(1024), dim3(256), 0, s, X, ldx,
+                           d, CT, k, list_count, list, labels, prev, changed);
     return hipGetLastError();
 }
 template hipError_t launch_kmeans_recheck<float>(const float *, int, int, const double *, int,
                                                  const int *, const int *, int *, const int *,
-                                                 int *, double *, hipStream_t);
+                                                 int *, double *, hipStream_t, int);
 template hipError_t launch_kmeans_recheck<double>(const double *, int, int, const double *, int,
                                                   const int *, const int *, int *, const int *,
-                                                  int *, double *, hipStream_t);
+                                                  int *, double *, hipStream_t, int);
 
 } // namespace dagpu

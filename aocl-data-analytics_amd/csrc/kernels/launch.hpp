@@ -78,11 +78,13 @@ template <typename T>
 hipError_t launch_col_absmax(const T *X, int n, int d, double *out, hipStream_t s);
 
 // Exact assignment: labels[i] = argmin_c sum_l (X[i,l]-C[c,l])^2 in double, ties to the
-// lowest c. mind[i] receives the minimum squared distance (may be null). When prev is
-// non-null, *changed is incremented for every i with labels[i] != prev[i].
+// lowest c (sph: argmin of -X_i.C_c, the largest dot product). mind[i] receives the minimum
+// key (may be null). When prev is non-null, *changed is incremented for every i with
+// labels[i] != prev[i].
 template <typename T>
 hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k, int *labels,
-                                const int *prev, int *changed, double *mind, hipStream_t s);
+                                const int *prev, int *changed, double *mind, hipStream_t s,
+                                int sph = 0);
 
 // Fixed-point sums of the points of each cluster in [c0, c0+kc): each workgroup b sorts the
 // points [b*ppb, (b+1)*ppb) by label in LDS, sums each cluster's values in registers and
@@ -90,15 +92,19 @@ hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k
 // the value scaled by 2^e[l] and rounded to int64, w = 1: the remainder scaled by
 // 2^(e[l]+lshift)) and cnt[c] with 64-bit integer atomics. Integer addition is exact and
 // associative, so the totals do not depend on the order of the workgroups. part (kc*d*2
-// words) and cnt (kc) must be zero on entry; launch_kmeans_reduce leaves them zero.
+// words) and cnt (kc) must be zero on entry; launch_kmeans_reduce leaves them zero. With
+// winv (n doubles) sample i contributes X_i * winv[i].
 template <typename T>
 hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *labels, int c0,
                                       int kc, const int *escale, int ppb, int nblocks,
-                                      long long *part, int *cnt, hipStream_t s);
-// C[(c0+c)*d + l] = sum / max(count, 1), counts[c0+c] = count from the totals of
-// launch_kmeans_partial_sums, which it then clears for the next pass.
+                                      long long *part, int *cnt, hipStream_t s,
+                                      const double *winv = nullptr);
+// C[(c0+c)*d + l] = sum / max(count, 1) (sph: the sum scaled to unit length, a zero sum left
+// at zero), counts[c0+c] = count from the totals of launch_kmeans_partial_sums, which it then
+// clears for the next pass.
 hipError_t launch_kmeans_reduce(long long *part, int *cnt, int c0, int kc, int d,
-                                const int *escale, double *C, int *counts, hipStream_t s);
+                                const int *escale, double *C, int *counts, hipStream_t s,
+                                int sph = 0);
 // Fills n 64-bit words with zero (a kernel, so that it can be captured in a graph).
 hipError_t launch_zero_words(long long *p, int64_t n, hipStream_t s);
 // LDS bytes of one partial-sum workgroup for kc clusters and ppb points.
@@ -119,9 +125,12 @@ hipError_t launch_gather_rows(const T *X, int d, const int *idx, int m, double *
                               hipStream_t s);
 
 // k-means++ helpers. w[i] = squared distance of X_i to row `ci` of X (exact, double), with
-// negative or non-finite values set to 0 and w[ci] = 0.
+// negative or non-finite values set to 0 and w[ci] = 0. sph: the cosine distance
+// max(0, 1 - X_i.X_ci inv[i] inv[ci]) instead (inv null: rows taken as unit rows); the same
+// choice for launch_candidate_cost and launch_min_update.
 template <typename T>
-hipError_t launch_dist_to_row(const T *X, int n, int d, int ci, double *w, hipStream_t s);
+hipError_t launch_dist_to_row(const T *X, int n, int d, int ci, double *w, hipStream_t s,
+                              int sph = 0, const double *inv = nullptr);
 // cp[i] = inclusive prefix sum of w[i]/sum (sum = *total), cp[n-1] = 1. tmp: >= 2*1024 doubles.
 hipError_t launch_normalised_scan(const double *w, int n, const double *total, double *cp,
                                   double *tmp, hipStream_t s);
@@ -131,23 +140,34 @@ hipError_t launch_lower_bound(const double *cp, int n, const double *u, int nt, 
 // For each candidate row cand[t]: part[t*nb + b] = partial of sum_i max(0, min(w[i], d_i(t))).
 template <typename T>
 hipError_t launch_candidate_cost(const T *X, int n, int d, const double *w, const int *cand,
-                                 int nt, double *part, int nb, hipStream_t s);
+                                 int nt, double *part, int nb, hipStream_t s, int sph = 0,
+                                 const double *inv = nullptr);
 // w[i] = max(0, min(w[i], d_i(ci))), w[ci] = 0.
 template <typename T>
-hipError_t launch_min_update(const T *X, int n, int d, int ci, double *w, hipStream_t s);
+hipError_t launch_min_update(const T *X, int n, int d, int ci, double *w, hipStream_t s,
+                             int sph = 0, const double *inv = nullptr);
 // Reduces part[t*nb + b] over b for each t into out[t] (fixed order).
 hipError_t launch_reduce_rows(const double *part, int nt, int nb, double *out, hipStream_t s);
-// Direct distances of rows of X to centres (double), sqrt applied: D[i*ldd + c].
+// Direct distances of rows of X to centres (double), sqrt applied: D[i*ldd + c]. sph: the
+// cosine distance 1 - clamp(X_i.C_c inv[i], -1, 1) (inv null: 1 - clamp(X_i.C_c)).
 template <typename T>
 hipError_t launch_kmeans_transform(const T *X, int n, int d, const double *C, int k, T *D,
-                                   int64_t ldd, hipStream_t s);
+                                   int64_t ldd, hipStream_t s, int sph = 0,
+                                   const double *inv = nullptr);
+// Each row of C (k x d) scaled to unit length; a zero row is left as it is.
+hipError_t launch_normalize_rows(double *C, int k, int d, hipStream_t s);
+// inv[i] = 1/|X_i| in double, 0 for a zero row.
+template <typename T>
+hipError_t launch_row_inv_norms(const T *X, int n, int d, double *inv, hipStream_t s);
 } // namespace dagpu
 
 namespace dagpu {
-// out[i] = ||X_i - C[labels[i]]||^2 in double.
+// out[i] = ||X_i - C[labels[i]]||^2 in double (sph: 1 - X_i.C[labels[i]] inv[i], inv null:
+// 1 - X_i.C[labels[i]]).
 template <typename T>
 hipError_t launch_label_sqdist(const T *X, int n, int d, const double *C, const int *labels,
-                               double *out, hipStream_t s);
+                               double *out, hipStream_t s, int sph = 0,
+                               const double *inv = nullptr);
 } // namespace dagpu
 
 namespace dagpu {
@@ -337,8 +357,9 @@ namespace dagpu {
 // ---- MFMA k-means assignment (fp32 data) ----
 // C32[c*Kp + l] = (float)C[c*d + l] (zero padded), chalf[c] = -||C_c||^2/2, *cmax2 =
 // max ||C_c||^2 (must be zeroed before).
+// sph: chalf = 0 (spherical k-means: the argmax of x.c, the reference's zero centre norms).
 hipError_t launch_centre_prep(const double *C, int k, int d, int Kp, float *C32, float *chalf,
-                              double *cmax2, hipStream_t s);
+                              double *cmax2, hipStream_t s, int sph = 0);
 size_t kmeans_assign_mfma_lds(int k, int Kp);
 // X: n x Kp zero-padded fp32 rows, Kp in {16, 32, 48, 64, 96, 128}. Writes labels; samples
 // whose best two centres are within the fp32 error bound are appended to list (count in
@@ -355,13 +376,16 @@ hipError_t launch_kmeans_assign_bf16(const void *X, int n, int Kp, const void *C
                                      const int *prev, int *changed, int *list_count, int *list,
                                      hipStream_t s);
 hipError_t launch_centre_prep_bf16(const double *C, int k, int d, int Kp, void *C16,
-                                   float *chalf, double *cmax2, hipStream_t s);
+                                   float *chalf, double *cmax2, hipStream_t s, int sph = 0);
 hipError_t launch_rows_to_bf16(const float *X, int n, int d, int ldx, void *Y, int Kp,
                                hipStream_t s);
+// Exact double re-check of the listed samples (sph: argmax of x.c); CT: k*d doubles of
+// scratch for the transposed centres.
 template <typename T>
 hipError_t launch_kmeans_recheck(const T *X, int ldx, int d, const double *C, int k,
                                  const int *list_count, const int *list, int *labels,
-                                 const int *prev, int *changed, double *CT, hipStream_t s);
+                                 const int *prev, int *changed, double *CT, hipStream_t s,
+                                 int sph = 0);
 } // namespace dagpu
 
 namespace dagpu {

@@ -114,6 +114,17 @@ da_status KMeansHandle<T>::prepare_scales_for(const U *dX) {
     std::vector<double> amax(d);
     DA_HIP(err, hipMemcpyAsync(amax.data(), tmp_.ptr(), d * sizeof(double), hipMemcpyDeviceToHost, st));
     DA_HIP(err, hipStreamSynchronize(st));
+    da_status s = upload_scales(amax, escale_);
+    if (s != da_status_success)
+        return s;
+    // Sums of normalised samples (spherical k-means): every value is at most 1 in magnitude.
+    return upload_scales(std::vector<double>(d, 1.0), escale_unit_);
+}
+
+template <typename T>
+da_status KMeansHandle<T>::upload_scales(const std::vector<double> &amax, DeviceBuffer &dst) {
+    hipStream_t st = gpu->stream();
+    const int d = n_features_;
     int l2n = 0;
     while (((int64_t)1 << l2n) < (int64_t)n_samples_ + 1)
         l2n++;
@@ -126,8 +137,8 @@ da_status KMeansHandle<T>::prepare_scales_for(const U *dX) {
         }
     }
     escale_host_[d] = 61 - l2n;
-    DA_HIP(err, escale_.reserve((d + 1) * sizeof(int)));
-    DA_HIP(err, hipMemcpyAsync(escale_.ptr(), escale_host_.data(), (d + 1) * sizeof(int),
+    DA_HIP(err, dst.reserve((d + 1) * sizeof(int)));
+    DA_HIP(err, hipMemcpyAsync(dst.ptr(), escale_host_.data(), (d + 1) * sizeof(int),
                                hipMemcpyHostToDevice, st));
     DA_HIP(err, hipStreamSynchronize(st));
     return da_status_success;
@@ -288,7 +299,7 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
         DA_HIP(err, list_.reserve((size_t)n * sizeof(int)));
         DA_HIP(err, hipMemsetAsync(cmax2, 0, sizeof(double), st));
         DA_HIP(err, launch_centre_prep_bf16(Cin, k, d, kp16_, c16_.ptr(), chalf_.as<float>(),
-                                            cmax2, st));
+                                            cmax2, st, sph_));
         double hc = 0;
         DA_HIP(err, hipMemcpyAsync(&hc, cmax2, sizeof(double), hipMemcpyDeviceToHost, st));
         DA_HIP(err, hipStreamSynchronize(st));
@@ -304,14 +315,15 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
             DA_HIP(err, ct_.reserve((size_t)k * d * sizeof(double)));
             DA_HIP(err, launch_kmeans_recheck<U>(dX, d, d, Cin, k, dlist_count, list_.as<int>(),
                                                  lab_new, lab_old, dchanged, ct_.as<double>(),
-                                                 st));
+                                                 st, sph_));
         }
     } else if (use_mfma) {
         // fp32 cross term on MFMA, exact re-check of near ties (kmeans_mfma.hip).
         int *dlist_count = dchanged + 1;
         double *cmax2 = small_.as<double>() + 6;
         DA_HIP(err, hipMemsetAsync(cmax2, 0, sizeof(double), st));
-        DA_HIP(err, launch_centre_prep(Cin, k, d, kp_, c32_.as<float>(), chalf_.as<float>(), cmax2, st));
+        DA_HIP(err, launch_centre_prep(Cin, k, d, kp_, c32_.as<float>(), chalf_.as<float>(), cmax2,
+                                       st, sph_));
         double hc = 0;
         DA_HIP(err, hipMemcpyAsync(&hc, cmax2, sizeof(double), hipMemcpyDeviceToHost, st));
         DA_HIP(err, hipStreamSynchronize(st));
@@ -326,12 +338,12 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
             DA_HIP(err, ct_.reserve((size_t)k * d * sizeof(double)));
             DA_HIP(err, launch_kmeans_recheck<U>(dX, d, d, Cin, k, dlist_count, list_.as<int>(),
                                                  lab_new, lab_old, dchanged, ct_.as<double>(),
-                                                 st));
+                                                 st, sph_));
         }
     } else {
         KernelScope ks(*gpu, "kmeans_assign");
         DA_HIP(err, launch_kmeans_assign<U>(dX, n, d, Cin, k, lab_new, lab_old, dchanged,
-                                            mind_.as<double>(), st));
+                                            mind_.as<double>(), st, sph_));
     }
     if (update) {
         da_status s = update_centres<U>(dX, lab_new, Cin, Cout);
@@ -350,11 +362,14 @@ da_status KMeansHandle<T>::update_centres(const U *dX, const int *lab_new, const
     KernelScope ks(*gpu, "kmeans_update");
     for (int c0 = 0; c0 < k; c0 += kc_) {
         const int kcc = std::min<int>(kc_, k - c0);
-        DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n, d, lab_new, c0, kcc, escale_.as<int>(),
-                                                  ppb_, nblocks_, part_.as<long long>(),
-                                                  pcnt_.as<int>(), st));
-        DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), c0, kcc, d,
-                                         escale_.as<int>(), Cout, counts_.as<int>(), st));
+        // Spherical k-means: sums of x (or of x/|x|), normalised (kmeans_lloyd.hpp:279-300,
+        // 363-405).
+        const int *esc = winv() ? escale_unit_.as<int>() : escale_.as<int>();
+        DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n, d, lab_new, c0, kcc, esc, ppb_, nblocks_,
+                                                  part_.as<long long>(), pcnt_.as<int>(), st,
+                                                  winv()));
+        DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), c0, kcc, d, esc,
+                                         Cout, counts_.as<int>(), st, sph_));
     }
     // The low-precision phase stores its centres in the lower precision (kmeans<float> or
     // kmeans<_Float16> in kmeans.cpp:1824-1876); the move is measured after the rounding.
@@ -451,17 +466,40 @@ da_status KMeansHandle<T>::handle_empty_clusters(const U *dX, double *C, int *la
         return da_status_success;
     }
     std::vector<U> A((size_t)n * d);
-    std::vector<double> cen((size_t)k * d);
+    std::vector<double> cen((size_t)k * d), inv;
     DA_HIP(err, hipMemcpyAsync(A.data(), dX, A.size() * sizeof(U), hipMemcpyDeviceToHost, st));
     DA_HIP(err, hipMemcpyAsync(cen.data(), C, cen.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (winv()) {
+        inv.resize(n);
+        DA_HIP(err, hipMemcpyAsync(inv.data(), winv(), n * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     DA_HIP(err, hipStreamSynchronize(st));
+    // Distance of a sample to a centre: squared Euclidean, or for spherical k-means
+    // 1 - x.c/|x| (1 - x.c for data taken as unit rows), kmeans.cpp:1171-1199.
     auto sqd = [&](int i, int c) {
         double s = 0;
+        if (sph_) {
+            for (int j = 0; j < d; j++)
+                s += (double)A[(size_t)i * d + j] * cen[(size_t)c * d + j];
+            return 1.0 - (inv.empty() ? s : s * inv[i]);
+        }
         for (int j = 0; j < d; j++) {
             double t = (double)A[(size_t)i * d + j] - cen[(size_t)c * d + j];
             s += t * t;
         }
         return s;
+    };
+    // Spherical k-means relocates x/|x| (x for data taken as unit rows) and normalises both
+    // centres it changes (kmeans.cpp:1224-1265).
+    auto normalise = [&](int c) {
+        double ss = 0;
+        for (int j = 0; j < d; j++)
+            ss += cen[(size_t)c * d + j] * cen[(size_t)c * d + j];
+        if (ss > 0) {
+            const double r = 1.0 / std::sqrt(ss);
+            for (int j = 0; j < d; j++)
+                cen[(size_t)c * d + j] *= r;
+        }
     };
     std::vector<double> dist(n);
     for (int i = 0; i < n; i++)
@@ -480,12 +518,17 @@ da_status KMeansHandle<T>::handle_empty_clusters(const U *dX, double *C, int *la
             break;
         const int old = lab[best];
         const int n_old = count[old];
+        const double wb = inv.empty() ? 1.0 : inv[best];
         for (int j = 0; j < d; j++) {
             double &cv = cen[(size_t)old * d + j];
-            cv = (cv * n_old - (double)A[(size_t)best * d + j]) / (n_old - 1);
+            cv = (cv * n_old - (double)A[(size_t)best * d + j] * wb) / (n_old - 1);
         }
         for (int j = 0; j < d; j++)
-            cen[(size_t)c * d + j] = (double)A[(size_t)best * d + j];
+            cen[(size_t)c * d + j] = (double)A[(size_t)best * d + j] * wb;
+        if (sph_) {
+            normalise(old);
+            normalise(c);
+        }
         count[old] -= 1;
         count[c] = 1;
         lab[best] = c;
@@ -576,7 +619,9 @@ da_status KMeansHandle<T>::run_lloyd(const U *dX, double tol, da_int max_iter, R
     }
     // compute_current_inertia (kmeans.cpp:1060-1124), exact squared distances summed in a
     // fixed order.
-    DA_HIP(err, launch_label_sqdist<U>(dX, n_samples_, n_features_, Ccur_, lab_cur_, mind_.as<double>(), st));
+    // Spherical k-means: sum of 1 - x.c/|x| (1 - x.c when the data are taken as unit rows).
+    DA_HIP(err, launch_label_sqdist<U>(dX, n_samples_, n_features_, Ccur_, lab_cur_,
+                                       mind_.as<double>(), st, sph_, winv()));
     DA_HIP(err, launch_sum_double(mind_.as<double>(), n_samples_, tmp_.as<double>(), small_.as<double>() + 2, st));
     double inertia = 0;
     DA_HIP(err, hipMemcpyAsync(&inertia, small_.as<double>() + 2, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -598,7 +643,7 @@ template <typename T> da_status KMeansHandle<T>::kmeans_plusplus() {
     const int first = dis_int(mt_);
     chosen[0] = first;
     double *w = w_.as<double>();
-    DA_HIP(err, launch_dist_to_row<T>(dX, n, d, first, w, st));
+    DA_HIP(err, launch_dist_to_row<T>(dX, n, d, first, w, st, sph_, winv()));
     double *total = small_.as<double>() + 3;
     DA_HIP(err, launch_sum_double(w, n, tmp_.as<double>(), total, st));
     double htot = 0;
@@ -629,7 +674,8 @@ template <typename T> da_status KMeansHandle<T>::kmeans_plusplus() {
             } else {
                 std::fill(cand.begin(), cand.end(), 0);
             }
-            DA_HIP(err, launch_candidate_cost<T>(dX, n, d, w, dcand, n_trials, part, nb, st));
+            DA_HIP(err, launch_candidate_cost<T>(dX, n, d, w, dcand, n_trials, part, nb, st, sph_,
+                                                 winv()));
             DA_HIP(err, launch_reduce_rows(part, n_trials, nb, dcost, st));
             DA_HIP(err, hipMemcpyAsync(cost.data(), dcost, n_trials * sizeof(double), hipMemcpyDeviceToHost, st));
             DA_HIP(err, hipStreamSynchronize(st));
@@ -647,7 +693,7 @@ template <typename T> da_status KMeansHandle<T>::kmeans_plusplus() {
                 }
             }
             chosen[c] = best;
-            DA_HIP(err, launch_min_update<T>(dX, n, d, best, w, st));
+            DA_HIP(err, launch_min_update<T>(dX, n, d, best, w, st, sph_, winv()));
         }
     }
     DA_HIP(err, hipMemcpyAsync(idx_.ptr(), chosen.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
@@ -730,7 +776,7 @@ template <typename T> da_status KMeansHandle<T>::afk_mc2() {
     chosen[0] = dis_int(mt_);
     // w: squared distance of each sample to its nearest chosen centre.
     double *w = w_.as<double>();
-    DA_HIP(err, launch_dist_to_row<T>(dX, n, d, chosen[0], w, st));
+    DA_HIP(err, launch_dist_to_row<T>(dX, n, d, chosen[0], w, st, sph_, winv()));
     if (k > 1) {
         double *total = small_.as<double>() + 3;
         int *dout = reinterpret_cast<int *>(small_.as<double>() + 10);
@@ -757,7 +803,7 @@ template <typename T> da_status KMeansHandle<T>::afk_mc2() {
             mt_ = saved;
             mt_.discard((unsigned long long)out[1]);
             chosen[c] = out[0];
-            DA_HIP(err, launch_min_update<T>(dX, n, d, out[0], w, st));
+            DA_HIP(err, launch_min_update<T>(dX, n, d, out[0], w, st, sph_, winv()));
         }
     }
     DA_HIP(err, hipMemcpyAsync(idx_.ptr(), chosen.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
@@ -784,7 +830,12 @@ template <typename T> da_status KMeansHandle<T>::compute() {
     const bool mixed = opts.skey("mixed precision") == 1;
     empty_mode_ = opts.skey("empty clusters");
     const bool spherical = opts.skey("distance") == 1;
-    elkan_ = algorithm == KM_ELKAN;
+    // Spherical k-means with Elkan's algorithm runs Lloyd's iteration: Elkan's angular bounds
+    // (kmeans_elkan.hpp:143-233) only skip distance evaluations, the labels are the argmax of
+    // x.c either way (an exact tie goes to the lowest centre here).
+    elkan_ = algorithm == KM_ELKAN && !spherical;
+    sph_ = spherical;
+    sph_norm_ = spherical && opts.skey("normalize data") == 1;
     lp_mode_ = 0;
     opts.rebound_int("n_clusters", std::numeric_limits<da_int>::max(), UpperBound::p_inf);
 
@@ -799,13 +850,18 @@ template <typename T> da_status KMeansHandle<T>::compute() {
     if (algorithm == KM_HARTIGAN_WONG && (k == 1 || k >= n_samples_))
         return err->error(da_status_incompatible_options,
                           "The Hartigan-Wong algorithm requires 1 < k < n_samples.");
+    if (spherical && algorithm == KM_HARTIGAN_WONG)
+        return err->error(da_status_incompatible_options,
+                          "Cosine distance is not compatible with the Hartigan-Wong algorithm. "
+                          "Please use Lloyd, Elkan, or MacQueen.");
     if (algorithm == KM_HARTIGAN_WONG || algorithm == KM_MACQUEEN)
         return err->error(da_status_not_implemented,
                           "Only the Lloyd and Elkan k-means algorithms are implemented by the "
                           "MI355X build.");
-    if (spherical)
+    if (spherical && mixed)
         return err->error(da_status_not_implemented,
-                          "Spherical (cosine) k-means is not implemented by the MI355X build.");
+                          "Mixed-precision refinement of spherical (cosine) k-means is not "
+                          "implemented by the MI355X build.");
     if (k > n_samples_)
         return err->error(da_status_incompatible_options,
                           "n_clusters = " + std::to_string(k) + ", and n_samples = " +
@@ -866,6 +922,11 @@ template <typename T> da_status KMeansHandle<T>::compute() {
     Cprev_ = Cb_.as<double>();
     lab_cur_ = lab_a_.as<int>();
     lab_prev_ = lab_b_.as<int>();
+    if (sph_norm_) {
+        // data_inv_norms, kmeans.cpp:681-703: safe_inv(|x_i|).
+        DA_HIP(err, inv_.reserve((size_t)n * sizeof(double)));
+        DA_HIP(err, launch_row_inv_norms<T>(X_.as<T>(), n, d, inv_.as<double>(), st));
+    }
 
     // initialize_rng, kmeans.cpp:1814-1822.
     if (init != INIT_SUPPLIED) {
@@ -883,6 +944,9 @@ template <typename T> da_status KMeansHandle<T>::compute() {
         da_status s = initialise_centres(init);
         if (s != da_status_success)
             return s;
+        // Spherical k-means starts from unit centres (kmeans.cpp:1478-1486).
+        if (sph_)
+            DA_HIP(err, launch_normalize_rows(Ccur_, k, d, st));
         da_int lp_iter = 0;
         if (mixed) {
             // lower_precision_init, kmeans.cpp:1824-1876: a full k-means on the data rounded
@@ -982,8 +1046,11 @@ da_status KMeansHandle<T>::predict(da_int k_samples, da_int k_features, const T 
                                                    k_features, q_.as<T>(), st));
     }
     DA_HIP(err, out_.reserve((size_t)k_samples * sizeof(int)));
+    // Spherical k-means: the largest y.c (the reference's -C Y^T with zero centre norms,
+    // kmeans.cpp:975-985).
     DA_HIP(err, launch_kmeans_assign<T>(q_.as<T>(), k_samples, k_features, Cbest_.as<double>(),
-                                        n_clusters_, out_.as<int>(), nullptr, nullptr, nullptr, st));
+                                        n_clusters_, out_.as<int>(), nullptr, nullptr, nullptr, st,
+                                        sph_));
     std::vector<int> lab(k_samples);
     DA_HIP(err, hipMemcpyAsync(lab.data(), out_.ptr(), k_samples * sizeof(int), hipMemcpyDeviceToHost, st));
     DA_HIP(err, hipStreamSynchronize(st));
@@ -1027,8 +1094,17 @@ da_status KMeansHandle<T>::transform(da_int m_samples, da_int m_features, const 
     const int k = n_clusters_;
     const size_t ob = (size_t)m_samples * k * sizeof(T);
     DA_HIP(err, out_.reserve(2 * ob));
+    // Spherical k-means: cosine distances 1 - x.c/|x| (1 - x.c when the data are taken as
+    // unit rows), kmeans.cpp:821-863.
+    const double *qinv = nullptr;
+    if (sph_ && sph_norm_) {
+        DA_HIP(err, qinv_.reserve((size_t)m_samples * sizeof(double)));
+        DA_HIP(err, launch_row_inv_norms<T>(q_.as<T>(), m_samples, m_features, qinv_.as<double>(),
+                                            st));
+        qinv = qinv_.as<double>();
+    }
     DA_HIP(err, launch_kmeans_transform<T>(q_.as<T>(), m_samples, m_features, Cbest_.as<double>(),
-                                           k, out_.as<T>(), k, st));
+                                           k, out_.as<T>(), k, st, sph_, qinv));
     if (order() == row_major) {
         DA_HIP(err, copy_2d(X_transform, (size_t)ldx_transform * sizeof(T), out_.ptr(),
                                      k * sizeof(T), k * sizeof(T), m_samples, hipMemcpyDeviceToHost, st));

@@ -63,6 +63,10 @@ template <typename T> class KMeansHandle : public AlgHandle {
     da_status handle_empty_clusters(const U *dX, double *C, int *labels, bool &split,
                                     bool &empty_error);
     template <typename U> da_status prepare_scales_for(const U *dX);
+    // Fixed-point exponents of the sums for column maxima amax (see kmeans.hip) into dst.
+    da_status upload_scales(const std::vector<double> &amax, DeviceBuffer &dst);
+    // Spherical k-means with normalised data: 1/|x_i| per sample (else null).
+    const double *winv() const { return sph_ && sph_norm_ ? inv_.as<double>() : nullptr; }
     da_status upload_rows_double(const T *src, da_int rows, da_int ld, da_order ord,
                                  DeviceBuffer &dst);
 
@@ -79,6 +83,11 @@ template <typename T> class KMeansHandle : public AlgHandle {
     // Device state.
     DeviceBuffer X_, Xlp_, escale_, Ca_, Cb_, lab_a_, lab_b_, counts_, part_, pcnt_, mind_,
         small_, w_, cp_, tmp_, idx_, Cbest_, q_, q_stage_, out_;
+    // Spherical k-means (the "distance" option "cosine"): sph_ set by compute, sph_norm_ when
+    // the data are normalised; inv_ the inverse sample norms; escale_unit_ the exponents of
+    // sums of unit rows; qinv_ the inverse norms of transformed samples.
+    bool sph_ = false, sph_norm_ = false;
+    DeviceBuffer inv_, escale_unit_, qinv_;
     da_int ppb_ = 0, nblocks_ = 0, kc_ = 0;
     std::vector<int> escale_host_;
     // fp32 MFMA assignment: zero-padded copy of the data (n x kp_), fp32 centres, -|c|^2/2,

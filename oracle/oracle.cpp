@@ -260,18 +260,67 @@ static double round_lp(double v, int mode) {
 
 double oracle_round_lp(double v, int mode) { return round_lp(v, mode); }
 
+/* Spherical k-means (the "distance" option "cosine", kmeans.cpp:400-405): the inverse data
+ * norms safe_inv(|x|) when the data are normalised (kmeans.cpp:681-703), else none. A null
+ * Sph pointer below means the Euclidean k-means. */
+struct Sph {
+    int normalize;
+    std::vector<double> inv;
+};
+
+static double dotp(const double *a, const double *b, int d) {
+    double s = 0;
+    for (int l = 0; l < d; l++)
+        s += a[l] * b[l];
+    return s;
+}
+
+/* normalize_rows_inplace, da_utils.cpp:807-855: a zero row is left as it is. */
+static void normalize_rows(int k, int d, double *C) {
+    for (int c = 0; c < k; c++) {
+        const double nrm = std::sqrt(dotp(C + (size_t)c * d, C + (size_t)c * d, d));
+        const double inv = nrm == 0 ? 1.0 : 1.0 / nrm;
+        for (int l = 0; l < d; l++)
+            C[(size_t)c * d + l] *= inv;
+    }
+}
+
+/* The distance of sample i to a unit centre c: 1 - x.c/|x| (or 1 - x.c for data taken as
+ * normalised), kmeans.cpp:1171-1199; squared Euclidean without g. */
+static double centre_dist(int d, const double *X, const Sph *g, int i, const double *c) {
+    if (!g)
+        return sqd(X + (size_t)i * d, c, d);
+    double cs = dotp(X + (size_t)i * d, c, d);
+    if (g->normalize)
+        cs *= g->inv[i];
+    return 1.0 - cs;
+}
+
+/* compute_distances_to_point, kmeans.cpp:1488-1522: the distance of sample i to sample j,
+ * for spherical k-means max(0, 1 - cos). */
+static double sample_dist(int d, const double *X, const Sph *g, int i, int j) {
+    if (!g)
+        return sqd(X + (size_t)i * d, X + (size_t)j * d, d);
+    double cs = dotp(X + (size_t)i * d, X + (size_t)j * d, d);
+    if (g->normalize)
+        cs *= g->inv[i] * g->inv[j];
+    return std::max(0.0, 1.0 - cs);
+}
+
 /* lloyd_iteration (kmeans_lloyd.hpp:105-271): labels by argmin with ties to the lowest
  * centre (lloyd_iteration_kernel_scalar, kmeans_kernels.cpp:209-260); with update, the sums
  * per label and scale_current_cluster_centres :363-405 (empty -> count 1 -> origin).
  * Returns the number of changed labels. */
 static int lloyd_iter(int n, int d, int k, const double *X, const double *Cprev, double *Cnew,
-                      int *lab, const int *lab_prev, bool update) {
+                      int *lab, const int *lab_prev, bool update, const Sph *g = nullptr) {
     int changed = 0;
     for (int i = 0; i < n; i++) {
         double best = std::numeric_limits<double>::infinity();
         int bl = 0;
         for (int c = 0; c < k; c++) {
-            const double v = sqd(X + (size_t)i * d, Cprev + (size_t)c * d, d);
+            // spherical: the GEMM form -x.c of kmeans_lloyd.hpp:144-146 (centre norms zero)
+            const double v = g ? -dotp(X + (size_t)i * d, Cprev + (size_t)c * d, d)
+                               : sqd(X + (size_t)i * d, Cprev + (size_t)c * d, d);
             if (v < best) {
                 best = v;
                 bl = c;
@@ -286,12 +335,18 @@ static int lloyd_iter(int n, int d, int k, const double *X, const double *Cprev,
         std::fill(Cnew, Cnew + (size_t)k * d, 0.0);
         for (int i = 0; i < n; i++) {
             cnt[lab[i]]++;
+            // normalised spherical data: sums of x/|x| (kmeans_lloyd.hpp:279-300)
+            const double w = (g && g->normalize) ? g->inv[i] : 1.0;
             for (int l = 0; l < d; l++)
-                Cnew[(size_t)lab[i] * d + l] += X[(size_t)i * d + l];
+                Cnew[(size_t)lab[i] * d + l] += X[(size_t)i * d + l] * w;
         }
-        for (int c = 0; c < k; c++)
-            for (int l = 0; l < d; l++)
-                Cnew[(size_t)c * d + l] /= (cnt[c] == 0 ? 1 : cnt[c]);
+        if (g) {
+            normalize_rows(k, d, Cnew);
+        } else {
+            for (int c = 0; c < k; c++)
+                for (int l = 0; l < d; l++)
+                    Cnew[(size_t)c * d + l] /= (cnt[c] == 0 ? 1 : cnt[c]);
+        }
     }
     return changed;
 }
@@ -421,7 +476,8 @@ static void elkan_update(int n, int d, int k, const double *Cprev, const double 
 
 /* handle_empty_clusters, kmeans.cpp:1127-1330 (split mode in double arithmetic). Returns
  * 1 if a split was made, 2 if empty clusters exist in error mode, 0 otherwise. */
-static int empty_clusters(int n, int d, int k, const double *X, double *C, int *lab, int mode) {
+static int empty_clusters(int n, int d, int k, const double *X, double *C, int *lab, int mode,
+                          const Sph *g = nullptr) {
     std::vector<int> cnt(k, 0);
     for (int i = 0; i < n; i++)
         cnt[lab[i]]++;
@@ -434,7 +490,7 @@ static int empty_clusters(int n, int d, int k, const double *X, double *C, int *
         return 2;
     std::vector<double> dist(n);
     for (int i = 0; i < n; i++)
-        dist[i] = sqd(X + (size_t)i * d, C + (size_t)lab[i] * d, d);
+        dist[i] = centre_dist(d, X, g, i, C + (size_t)lab[i] * d);
     for (int c = 0; c < k; c++) {
         if (cnt[c] != 0)
             continue;
@@ -448,12 +504,17 @@ static int empty_clusters(int n, int d, int k, const double *X, double *C, int *
         if (best < 0)
             break;
         const int old = lab[best], n_old = cnt[old];
+        const double wb = (g && g->normalize) ? g->inv[best] : 1.0;
         for (int l = 0; l < d; l++) {
             double &cv = C[(size_t)old * d + l];
-            cv = (cv * n_old - X[(size_t)best * d + l]) / (n_old - 1);
+            cv = (cv * n_old - X[(size_t)best * d + l] * wb) / (n_old - 1);
         }
         for (int l = 0; l < d; l++)
-            C[(size_t)c * d + l] = X[(size_t)best * d + l];
+            C[(size_t)c * d + l] = X[(size_t)best * d + l] * wb;
+        if (g) {
+            normalize_rows(1, d, C + (size_t)old * d);
+            normalize_rows(1, d, C + (size_t)c * d);
+        }
         cnt[old]--;
         cnt[c] = 1;
         lab[best] = c;
@@ -461,7 +522,7 @@ static int empty_clusters(int n, int d, int k, const double *X, double *C, int *
         if (n_empty > 1)
             for (int i = 0; i < n; i++)
                 if (lab[i] == old)
-                    dist[i] = sqd(X + (size_t)i * d, C + (size_t)old * d, d);
+                    dist[i] = centre_dist(d, X, g, i, C + (size_t)old * d);
     }
     return 1;
 }
@@ -473,7 +534,8 @@ static int empty_clusters(int n, int d, int k, const double *X, double *C, int *
  * rounded, and the centres are rounded after every update, as kmeans<float> or
  * kmeans<_Float16> stores them. */
 static KMOut perform(int n, int d, int k, const double *X, double *C, int *lab, int max_iter,
-                     double tol, int empty_mode, int algorithm, int lp_mode) {
+                     double tol, int empty_mode, int algorithm, int lp_mode,
+                     const Sph *g = nullptr) {
     std::vector<double> Cp((size_t)k * d);
     std::vector<int> lp(n, 0);
     std::vector<double> Cc(C, C + (size_t)k * d);
@@ -489,7 +551,7 @@ static KMOut perform(int n, int d, int k, const double *X, double *C, int *lab, 
         std::swap(lp, lc);
         const int changed =
             elkan ? elkan_assign(n, d, k, X, Cp.data(), Cc.data(), lc.data(), lp.data(), true, e)
-                  : lloyd_iter(n, d, k, X, Cp.data(), Cc.data(), lc.data(), lp.data(), true);
+                  : lloyd_iter(n, d, k, X, Cp.data(), Cc.data(), lc.data(), lp.data(), true, g);
         for (auto &v : Cc)
             v = round_lp(v, lp_mode);
         // compute_centre_shift, kmeans.cpp:1334-1355 (Frobenius norms).
@@ -501,7 +563,7 @@ static KMOut perform(int n, int d, int k, const double *X, double *C, int *lab, 
         if (elkan)
             elkan_update(n, d, k, Cp.data(), Cc.data(), lc.data(), e);
         if (empty_mode != 0) {
-            const int r = empty_clusters(n, d, k, X, Cc.data(), lc.data(), empty_mode);
+            const int r = empty_clusters(n, d, k, X, Cc.data(), lc.data(), empty_mode, g);
             if (r == 2) {
                 o.empty_found = 1;
                 return o;
@@ -531,26 +593,28 @@ static KMOut perform(int n, int d, int k, const double *X, double *C, int *lab, 
         if (elkan)
             elkan_assign(n, d, k, X, Cp.data(), nullptr, lc.data(), lp.data(), false, e);
         else
-            lloyd_iter(n, d, k, X, Cp.data(), nullptr, lc.data(), nullptr, false);
+            lloyd_iter(n, d, k, X, Cp.data(), nullptr, lc.data(), nullptr, false, g);
         std::swap(Cp, Cc);
     }
     double inertia = 0;
     for (int i = 0; i < n; i++)
-        inertia += sqd(X + (size_t)i * d, Cc.data() + (size_t)lc[i] * d, d);
+        inertia += centre_dist(d, X, g, i, Cc.data() + (size_t)lc[i] * d);
     o.inertia = inertia;
     std::copy(Cc.begin(), Cc.end(), C);
     std::copy(lc.begin(), lc.end(), lab);
     return o;
 }
 
-/* kmeans_plusplus, kmeans.cpp:1524-1664, with exact squared distances as weights. */
-static void kmeanspp(int n, int d, int k, const double *X, double *C, std::mt19937_64 &mt) {
+/* kmeans_plusplus, kmeans.cpp:1524-1664, with exact squared distances (spherical: 1 - cos,
+ * clamped at zero) as weights. */
+static void kmeanspp(int n, int d, int k, const double *X, double *C, std::mt19937_64 &mt,
+                     const Sph *g = nullptr) {
     const int n_trials = 2 + (int)std::log((double)k);
     std::uniform_int_distribution<> dis_int(0, n - 1);
     const int first = dis_int(mt);
     std::vector<double> w(n), w2(n), w5(n), w4(n);
     for (int i = 0; i < n; i++) {
-        double v = sqd(X + (size_t)i * d, X + (size_t)first * d, d);
+        double v = sample_dist(d, X, g, i, first);
         w[i] = (!std::isfinite(v) || v < 0) ? 0.0 : v;
     }
     w[first] = 0;
@@ -581,7 +645,7 @@ static void kmeanspp(int n, int d, int k, const double *X, double *C, std::mt199
                     continue;
                 double cost = 0;
                 for (int i = 0; i < n; i++) {
-                    w4[i] = sqd(X + (size_t)i * d, X + (size_t)cand[t] * d, d);
+                    w4[i] = sample_dist(d, X, g, i, cand[t]);
                     w5[i] = std::max(0.0, std::min(w[i], w4[i]));
                     cost += w5[i];
                 }
@@ -605,15 +669,23 @@ static void kmeanspp(int n, int d, int k, const double *X, double *C, std::mt199
  * proposal q(x) = d(x,c1)^2 / (2 sum) + 1/(2n) (uniform when every distance is zero); for
  * each further centre a Markov chain of m steps over draws from q, accepting y with
  * probability min(1, d(y)q(x) / (d(x)q(y))), where d is the squared distance to the
- * nearest centre chosen so far; a zero d(x)q(y) accepts without a uniform draw. */
-static void afkmc2(int n, int d, int k, const double *X, double *C, int m, std::mt19937_64 &mt) {
+ * nearest centre chosen so far; a zero d(x)q(y) accepts without a uniform draw. Spherical
+ * k-means: the distances max(0, 1 - cos) (the proposal between samples, the chain against the
+ * centres stored as x/|x|, or as x for data taken as unit rows). */
+static void afkmc2(int n, int d, int k, const double *X, double *C, int m, std::mt19937_64 &mt,
+                   const Sph *g = nullptr) {
     std::uniform_int_distribution<> dis_int(0, n - 1);
     const int c1 = dis_int(mt);
-    std::copy(X + (size_t)c1 * d, X + (size_t)c1 * d + d, C);
+    auto place = [&](int c, int idx) {
+        const double wv = (g && g->normalize) ? g->inv[idx] : 1.0;
+        for (int l = 0; l < d; l++)
+            C[(size_t)c * d + l] = X[(size_t)idx * d + l] * wv;
+    };
+    place(0, c1);
     std::vector<double> w(n), q(n);
     double sum = 0;
     for (int i = 0; i < n; i++) {
-        w[i] = std::max(0.0, sqd(X + (size_t)i * d, X + (size_t)c1 * d, d));
+        w[i] = std::max(0.0, sample_dist(d, X, g, i, c1));
         sum += w[i];
     }
     const double inv_2n = 1.0 / (2.0 * n);
@@ -622,7 +694,8 @@ static void afkmc2(int n, int d, int k, const double *X, double *C, int m, std::
     auto nearest = [&](int idx, int nc) {
         double best = std::numeric_limits<double>::infinity();
         for (int c = 0; c < nc; c++)
-            best = std::min(best, sqd(X + (size_t)idx * d, C + (size_t)c * d, d));
+            best = std::min(best, g ? std::max(0.0, centre_dist(d, X, g, idx, C + (size_t)c * d))
+                                    : sqd(X + (size_t)idx * d, C + (size_t)c * d, d));
         return best;
     };
     std::discrete_distribution<> qd(q.begin(), q.end());
@@ -639,7 +712,7 @@ static void afkmc2(int n, int d, int k, const double *X, double *C, int m, std::
                 dx = dy;
             }
         }
-        std::copy(X + (size_t)x * d, X + (size_t)x * d + d, C + (size_t)i * d);
+        place(i, x);
     }
 }
 
@@ -667,10 +740,10 @@ int oracle_kmeans_init(int n, int d, const double *X, int k, int init, long long
  * 41 when lp_tol <= tol (incompatible options). out_info = {n_iter, inertia, lp_n_iter}.
  * Centres row-major k x d.
  */
-int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *C_init,
-                  int n_init, int max_iter, double tol, long long seed, int empty_mode,
-                  int algorithm, int afk_m, int lp_mode, double lp_tol, int lp_max_iter,
-                  double *C_out, int *labels_out, double *out_info) {
+static int kmeans_impl(int n, int d, const double *X, int k, int init, const double *C_init,
+                       int n_init, int max_iter, double tol, long long seed, int empty_mode,
+                       int algorithm, int afk_m, int lp_mode, double lp_tol, int lp_max_iter,
+                       double *C_out, int *labels_out, double *out_info, const Sph *g) {
     if (n_init > 1 && init == 2)
         n_init = 1;
     if (lp_mode != 0 && lp_tol <= tol)
@@ -712,10 +785,13 @@ int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *
                 for (int l = 0; l < d; l++)
                     C[(size_t)c * d + l] /= cnt[c];
         } else if (init == 4) {
-            afkmc2(n, d, k, X, C.data(), afk_m, mt);
+            afkmc2(n, d, k, X, C.data(), afk_m, mt, g);
         } else {
-            kmeanspp(n, d, k, X, C.data(), mt);
+            kmeanspp(n, d, k, X, C.data(), mt, g);
         }
+        // spherical: the initial centres normalised (kmeans.cpp:1478-1486)
+        if (g)
+            normalize_rows(k, d, C.data());
         int lp_iter = 0;
         if (lp_mode != 0) {
             for (auto &v : C)
@@ -730,7 +806,7 @@ int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *
                 v = round_lp(v, lp_mode);
         }
         KMOut o = perform(n, d, k, X, C.data(), lab.data(), max_iter, tol, empty_mode, algorithm,
-                          0);
+                          0, g);
         if (o.empty_found)
             continue;
         valid = true;
@@ -747,6 +823,39 @@ int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *
     if (!valid)
         return 900;
     return warn ? 21 : 0;
+}
+
+int oracle_kmeans(int n, int d, const double *X, int k, int init, const double *C_init,
+                  int n_init, int max_iter, double tol, long long seed, int empty_mode,
+                  int algorithm, int afk_m, int lp_mode, double lp_tol, int lp_max_iter,
+                  double *C_out, int *labels_out, double *out_info) {
+    return kmeans_impl(n, d, X, k, init, C_init, n_init, max_iter, tol, seed, empty_mode,
+                       algorithm, afk_m, lp_mode, lp_tol, lp_max_iter, C_out, labels_out,
+                       out_info, nullptr);
+}
+
+/* Spherical k-means (the "distance" option "cosine"), Lloyd, full precision, initialisation
+ * 0 random, 1 k-means++, 2 supplied, 3 random partitions or 4 AFK-MC2 with afk_m chain steps
+ * (kmeans.cpp:1667-1811), empty_mode 0 ignore, 1 error or
+ * 2 split (kmeans.cpp:1160-1320, the relocated sample and the donor centre normalised), with
+ * the data normalised (normalize 1, kmeans.cpp:681-703) or taken as unit rows (0): labels by
+ * the largest x.c (ties to the lowest centre), centres the normalised sums of the members (of
+ * x/|x| when normalised), inertia the sum of 1 - x.c/|x| (1 - x.c). Returns as
+ * oracle_kmeans; out_info = {n_iter, inertia, 0}. */
+int oracle_kmeans_sph(int n, int d, const double *X, int k, int init, const double *C_init,
+                      int n_init, int max_iter, double tol, long long seed, int empty_mode,
+                      int afk_m, int normalize, double *C_out, int *labels_out,
+                      double *out_info) {
+    Sph g{normalize, {}};
+    if (normalize) {
+        g.inv.resize(n);
+        for (int i = 0; i < n; i++) {
+            const double nrm = std::sqrt(dotp(X + (size_t)i * d, X + (size_t)i * d, d));
+            g.inv[i] = nrm != 0 ? 1.0 / nrm : 0.0;
+        }
+    }
+    return kmeans_impl(n, d, X, k, init, C_init, n_init, max_iter, tol, seed, empty_mode, 0,
+                       afk_m, 0, 0.0, 0, C_out, labels_out, out_info, &g);
 }
 
 /*

@@ -38,6 +38,10 @@ _lib.oracle_kmeans.argtypes = [C.c_int, C.c_int, _d, C.c_int, C.c_int, _d, C.c_i
 _lib.oracle_kmeans_init.restype = C.c_int
 _lib.oracle_kmeans_init.argtypes = [C.c_int, C.c_int, _d, C.c_int, C.c_int, C.c_longlong,
                                     C.c_int, _d]
+_lib.oracle_kmeans_sph.restype = C.c_int
+_lib.oracle_kmeans_sph.argtypes = [C.c_int, C.c_int, _d, C.c_int, C.c_int, _d, C.c_int, C.c_int,
+                                   C.c_double, C.c_longlong, C.c_int, C.c_int, C.c_int, _d, _i,
+                                   _d]
 _lib.oracle_round_lp.restype = C.c_double
 _lib.oracle_round_lp.argtypes = [C.c_double, C.c_int]
 
@@ -264,6 +268,22 @@ def kmeans(X, n_clusters, init="k-means++", C=None, n_init=1, max_iter=300, tol=
                             _p(lab, _i), _p(info))
     return {"status": st, "centres": Co, "labels": lab, "n_iter": int(info[0]),
             "inertia": float(info[1]), "lp_n_iter": int(info[2])}
+
+
+def kmeans_sph(X, n_clusters, init="k-means++", C=None, n_init=1, max_iter=300, tol=1e-4,
+               seed=0, empty_clusters="ignore", normalize=True, afk_samples=50):
+    """Spherical (cosine) k-means, Lloyd: dict(status, centres, labels, n_iter, inertia)."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, d = X.shape
+    Ci = None if C is None else np.ascontiguousarray(C, dtype=np.float64)
+    Co = np.zeros((n_clusters, d))
+    lab = np.zeros(n, dtype=np.int32)
+    info = np.zeros(3)
+    st = _lib.oracle_kmeans_sph(n, d, _p(X), n_clusters, INIT[init], _p(Ci), n_init, max_iter,
+                                float(tol), int(seed), EMPTY[empty_clusters], int(afk_samples),
+                                int(bool(normalize)), _p(Co), _p(lab, _i), _p(info))
+    return {"status": st, "centres": Co, "labels": lab, "n_iter": int(info[0]),
+            "inertia": float(info[1])}
 
 
 def kmeans_init(X, n_clusters, init="k-means++", seed=0, afk_samples=50):

@@ -112,12 +112,13 @@ def _sq(X, i, j):
     return float(sum((X[i, l] - X[j, l]) ** 2 for l in range(X.shape[1])))
 
 
-def kmeanspp(X, k, seed):
-    """Indices of the rows k-means++ picks (kmeans.cpp:1524-1664), exact weights."""
+def kmeanspp(X, k, seed, dist=_sq):
+    """Indices of the rows k-means++ picks (kmeans.cpp:1524-1664), exact weights dist(X, i, j)
+    (squared Euclidean by default)."""
     g = MT19937_64(seed)
     n = X.shape[0]
     first = uniform_int(g, 0, n - 1)
-    w = [_sq(X, i, first) for i in range(n)]
+    w = [dist(X, i, first) for i in range(n)]
     w[first] = 0.0
     if not any(v > 0 for v in w):
         return list(range(k))
@@ -130,7 +131,7 @@ def kmeanspp(X, k, seed):
         for t, c in enumerate(cand):
             if c in cand[:t]:
                 continue
-            w5 = [max(0.0, min(w[i], _sq(X, i, c))) for i in range(n)]
+            w5 = [max(0.0, min(w[i], dist(X, i, c))) for i in range(n)]
             cost = 0.0
             for v in w5:
                 cost += v
@@ -142,12 +143,13 @@ def kmeanspp(X, k, seed):
     return chosen
 
 
-def afkmc2(X, k, seed, m):
-    """Indices of the rows AFK-MC2 picks (kmeans.cpp:1667-1811), exact distances."""
+def afkmc2(X, k, seed, m, dist=_sq):
+    """Indices of the rows AFK-MC2 picks (kmeans.cpp:1667-1811), exact distances dist(X, i, j)
+    (squared Euclidean by default)."""
     g = MT19937_64(seed)
     n = X.shape[0]
     c1 = uniform_int(g, 0, n - 1)
-    w = [max(0.0, _sq(X, i, c1)) for i in range(n)]
+    w = [max(0.0, dist(X, i, c1)) for i in range(n)]
     s = 0.0
     for v in w:
         s += v
@@ -156,7 +158,7 @@ def afkmc2(X, k, seed, m):
     qd = Discrete(q)
 
     def near(i):
-        return min(_sq(X, i, c) for c in chosen)
+        return min(max(0.0, dist(X, i, c)) for c in chosen)
 
     for _ in range(1, k):
         x = qd(g)
