@@ -89,11 +89,6 @@ template <typename T> da_status ApproxNNHandle<T>::train() {
         return err->error(da_status_invalid_array_dimension,
                           "n_samples = " + std::to_string(n_samples_) +
                               " must be at least as large as n_list = " + std::to_string(n_list_));
-    if (metric_ == ANN_INNER || metric_ == ANN_COSINE)
-        return err->error(da_status_not_implemented,
-                          "The inner product and cosine metrics need spherical k-means "
-                          "training, which this build does not provide; use euclidean or "
-                          "sqeuclidean.");
     da_status s = gpu->init(err);
     if (s != da_status_success)
         return s;
@@ -134,9 +129,37 @@ template <typename T> da_status ApproxNNHandle<T>::train() {
         ld = n_features_;
         ord = row_major;
     }
+    // The cosine metric trains on normalised rows (:419-426), as the GPU normalises the rows
+    // it adds: 1/|x| in double, the product rounded to T, zero rows unchanged.
+    std::vector<T> unit;
+    if (metric_ == ANN_COSINE) {
+        const size_t d = n_features_;
+        unit.resize((size_t)n_train * d);
+        for (da_int i = 0; i < n_train; i++) {
+            double ss = 0.0;
+            for (size_t j = 0; j < d; j++) {
+                const double v = (double)(ord == row_major ? rows[(size_t)i * ld + j]
+                                                           : rows[i + j * (size_t)ld]);
+                unit[i * d + j] = (T)v;
+                ss += v * v;
+            }
+            const double nrm = std::sqrt(ss);
+            if (nrm > 0.0)
+                for (size_t j = 0; j < d; j++)
+                    unit[i * d + j] = (T)((double)unit[i * d + j] * (1.0 / nrm));
+        }
+        rows = unit.data();
+        ld = n_features_;
+        ord = row_major;
+    }
     // Lloyd k-means on the GPU with centres drawn from the rows (train_ivfflat, :334-490):
-    // one initialisation, tolerance sqrt(epsilon).
+    // one initialisation, tolerance sqrt(epsilon); spherical k-means without normalising the
+    // data for the inner-product and cosine metrics (:458-461).
     KMeansHandle<T> km(*err, *gpu);
+    if (metric_ == ANN_INNER || metric_ == ANN_COSINE) {
+        km.opts.set_string("distance", "cosine");
+        km.opts.set_string("normalize data", "no");
+    }
     km.opts.set_string("storage order", ord == row_major ? "row-major" : "column-major");
     km.opts.set_string("initialization method", "random");
     km.opts.set_string("algorithm", "lloyd");
@@ -238,12 +261,18 @@ da_status ApproxNNHandle<T>::add(da_int n_add, da_int n_features, const T *X, da
     if (s != da_status_success)
         return s;
     hipStream_t st = gpu->stream();
+    T *added = X_d_.as<T>() + (size_t)n_index_ * n_features_;
+    // Cosine: the vectors are stored normalised (add_ivfflat, :577-598).
+    if (metric_ == ANN_COSINE)
+        DA_HIP(err, launch_ivf_normalize<T>(added, n_add, (int)n_features_, st));
     DA_HIP(err, lab_d_.reserve((size_t)n_add * sizeof(int)));
     {
+        // Nearest centroid: squared distance, or the largest x.c for the inner-product and
+        // cosine metrics (the GEMM with alpha = -1 of :715-731).
         KernelScope ks(*gpu, "ivf_assign");
-        DA_HIP(err, launch_kmeans_assign<T>(X_d_.as<T>() + (size_t)n_index_ * n_features_, n_add,
-                                            n_features_, cdouble_d_.as<double>(), n_list_,
-                                            lab_d_.as<int>(), nullptr, nullptr, nullptr, st));
+        DA_HIP(err, launch_kmeans_assign<T>(added, n_add, n_features_, cdouble_d_.as<double>(),
+                                            n_list_, lab_d_.as<int>(), nullptr, nullptr, nullptr,
+                                            st, metric_ == ANN_INNER || metric_ == ANN_COSINE));
     }
     std::vector<int> lab(n_add);
     DA_HIP(err, hipMemcpyAsync(lab.data(), lab_d_.ptr(), (size_t)n_add * sizeof(int),
@@ -325,6 +354,10 @@ da_status ApproxNNHandle<T>::kneighbors(da_int n_queries, da_int n_features, con
     if (s != da_status_success)
         return s;
     const int d = n_features_;
+    const bool ip = metric_ == ANN_INNER || metric_ == ANN_COSINE;
+    // Cosine: queries normalised like the stored vectors (:1068-1081).
+    if (metric_ == ANN_COSINE)
+        DA_HIP(err, launch_ivf_normalize<T>(q_d_.as<T>(), n_queries, d, st));
     const da_int max_list = *std::max_element(list_sizes_.begin(), list_sizes_.end());
     // Query blocks whose candidate rows (n_probe lists of at most max_list vectors) and
     // coarse keys stay within 2^26 entries.
@@ -345,15 +378,17 @@ da_status ApproxNNHandle<T>::kneighbors(da_int n_queries, da_int n_features, con
     const T *Q = q_d_.as<T>();
     for (da_int q0 = 0; q0 < n_queries; q0 += qb) {
         const int rows = (int)std::min<da_int>(qb, n_queries - q0);
-        // Coarse step: the n_probe nearest centroids of each query, by (distance, index).
+        // Coarse step: the n_probe nearest centroids of each query, by (distance, index);
+        // for the inner-product and cosine metrics the largest q.c (:1093-1127).
         {
             KernelScope ks(*gpu, "ivf_coarse");
-            DA_HIP(err, (launch_pairwise_direct<T, T>(KM_SQEUCLIDEAN, Q + (size_t)q0 * d,
-                                                      C_d_.as<T>(), rows, n_list_, d,
-                                                      ckeys_.as<T>(), n_list_, nullptr, nullptr,
-                                                      2.0, 0, 0, st)));
-            DA_HIP(err, launch_row_topk<T>(ckeys_.as<T>(), n_list_, rows, n_list_, n_probe, 0,
-                                           probe_key_.as<T>(), probe_d_.as<int>(), st));
+            DA_HIP(err, (launch_pairwise_direct<T, T>(ip ? KM_INNER : KM_SQEUCLIDEAN,
+                                                      Q + (size_t)q0 * d, C_d_.as<T>(), rows,
+                                                      n_list_, d, ckeys_.as<T>(), n_list_,
+                                                      nullptr, nullptr, 2.0, 0, 0, st)));
+            DA_HIP(err, launch_row_topk<T>(ckeys_.as<T>(), n_list_, rows, n_list_, n_probe,
+                                           ip ? 1 : 0, probe_key_.as<T>(), probe_d_.as<int>(),
+                                           st));
         }
         DA_HIP(err, hipMemcpyAsync(probes.data(), probe_d_.ptr(),
                                    (size_t)rows * n_probe * sizeof(int), hipMemcpyDeviceToHost,
@@ -389,15 +424,18 @@ da_status ApproxNNHandle<T>::kneighbors(da_int n_queries, da_int n_features, con
             DA_HIP(err, launch_ivf_scan<T>(Q + (size_t)q0 * d, rows, n_probe, d, X_d_.as<T>(),
                                            perm_d_.as<int>(), off_d_.as<int>(), probe_d_.as<int>(),
                                            seg_d_.as<int>(), ld, cand_key_.as<T>(),
-                                           cand_idx_.as<int>(), st));
+                                           cand_idx_.as<int>(), st, ip ? 1 : 0));
         }
         {
             KernelScope ks(*gpu, "ivf_select");
             DA_HIP(err, launch_row_topk<T>(cand_key_.as<T>(), ld, rows, (int)ld, k, 0,
                                            sel_key_.as<T>(), sel_idx_.as<int>(), st));
+            const int post = metric_ == ANN_EUCLIDEAN ? 1
+                             : metric_ == ANN_COSINE  ? 2
+                             : metric_ == ANN_INNER   ? 3
+                                                      : 0;
             DA_HIP(err, launch_ivf_finish<T>(sel_key_.as<T>(), sel_idx_.as<int>(),
-                                             cand_idx_.as<int>(), ld, k, (int64_t)rows * k,
-                                             metric_ == ANN_EUCLIDEAN,
+                                             cand_idx_.as<int>(), ld, k, (int64_t)rows * k, post,
                                              out_dist_.as<T>() + (size_t)q0 * k,
                                              out_ind_.as<int>() + (size_t)q0 * k, st));
         }

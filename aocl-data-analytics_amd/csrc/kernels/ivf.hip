@@ -7,6 +7,10 @@
 // ids of its vectors in list order. Candidate row q of a query block holds the keys of list
 // probes[q][p] at columns seg[q][p] + r; the host places the probed lists in ascending list
 // number, so that column order is (list, position in list).
+//
+// Keys: the squared euclidean distance, or for the inner-product and cosine metrics (IP)
+// the negated dot product -q.x (the reference's GEMM with alpha = -1, approximate_neighbors.cpp
+// :1145-1151), so that the smallest key is the nearest vector for every metric.
 #include "launch.hpp"
 #include <climits>
 #include <type_traits>
@@ -21,7 +25,7 @@ namespace {
 // features s, s + 16, ... (consecutive lanes read consecutive features, so each group's
 // loads are contiguous), then the group adds its partial sums with lane shuffles. A wave
 // therefore takes four vectors of the list at a time; distances accumulate in double.
-template <typename T>
+template <typename T, bool IP>
 __global__ __launch_bounds__(256) void ivf_scan_kernel(
     const T *__restrict__ Q, int nq, int n_probe, int d, const T *__restrict__ X,
     const int *__restrict__ perm, const int *__restrict__ list_off,
@@ -47,21 +51,25 @@ __global__ __launch_bounds__(256) void ivf_scan_kernel(
         double acc = 0.0;
         if (live)
             for (int l = sub; l < d; l += G) {
-                const double t = (double)qv[l] - (double)xv[l];
-                acc += t * t;
+                if constexpr (IP) {
+                    acc += (double)qv[l] * (double)xv[l];
+                } else {
+                    const double t = (double)qv[l] - (double)xv[l];
+                    acc += t * t;
+                }
             }
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1)
             acc += __shfl_xor(acc, off, G);
         if (live && sub == 0) {
-            kr[r - b] = (T)acc;
+            kr[r - b] = (T)(IP ? -acc : acc);
             ir[r - b] = g;
         }
     }
 }
 
 // 16-byte chunks of a row: four floats or two doubles. acc adds the squared differences of
-// a chunk's elements in element order.
+// a chunk's elements in element order, dot their products.
 template <typename T> struct Chunk16;
 template <> struct Chunk16<float> {
     using type = float4;
@@ -74,6 +82,12 @@ template <> struct Chunk16<float> {
         a += t2 * t2;
         a += t3 * t3;
     }
+    __device__ static void dot(double &a, const type &q, const type &x) {
+        a += (double)q.x * (double)x.x;
+        a += (double)q.y * (double)x.y;
+        a += (double)q.z * (double)x.z;
+        a += (double)q.w * (double)x.w;
+    }
 };
 template <> struct Chunk16<double> {
     using type = double2;
@@ -83,6 +97,10 @@ template <> struct Chunk16<double> {
         a += t0 * t0;
         a += t1 * t1;
     }
+    __device__ static void dot(double &a, const type &q, const type &x) {
+        a += q.x * x.x;
+        a += q.y * x.y;
+    }
 };
 
 // The same scan for rows of whole 16-byte chunks (dc chunks, at most 8 * NJ): eight lanes
@@ -90,7 +108,7 @@ template <> struct Chunk16<double> {
 // takes eight vectors per iteration with a quarter (fp32) or half (fp64) of the load
 // instructions; the query's chunks stay in registers for the whole list. Distances
 // accumulate in double, chunk by chunk.
-template <typename T, int NJ>
+template <typename T, int NJ, bool IP>
 __global__ __launch_bounds__(256) void ivf_scan_chunk_kernel(
     const T *__restrict__ Q, int nq, int n_probe, int dc, const T *__restrict__ X,
     const int *__restrict__ perm, const int *__restrict__ list_off,
@@ -126,15 +144,19 @@ __global__ __launch_bounds__(256) void ivf_scan_chunk_kernel(
 #pragma unroll
             for (int j = 0; j < NJ; j++) {
                 const int ch = sub + G * j;
-                if (ch < dc)
-                    CT::acc(acc, qr[j], xv[ch]);
+                if (ch < dc) {
+                    if constexpr (IP)
+                        CT::dot(acc, qr[j], xv[ch]);
+                    else
+                        CT::acc(acc, qr[j], xv[ch]);
+                }
             }
         }
 #pragma unroll
         for (int off = G / 2; off > 0; off >>= 1)
             acc += __shfl_xor(acc, off, G);
         if (live && sub == 0) {
-            kr[r - b] = (T)acc;
+            kr[r - b] = (T)(IP ? -acc : acc);
             ir[r - b] = g;
         }
     }
@@ -150,12 +172,14 @@ __global__ void ivf_pad_kernel(T *__restrict__ keys, int *__restrict__ idx, int6
 }
 
 // Selected candidates (key, column of the candidate row) -> reported neighbours: the global
-// id stored at that column, index -1 and distance +inf where a query's probed lists held
-// fewer than k vectors; the square root for the euclidean metric.
+// id stored at that column, index -1 where a query's probed lists held fewer than k vectors.
+// Distances (approximate_neighbors.cpp:1287-1312): post 0 the key, 1 its square root
+// (euclidean), 2 1 + key (cosine), 3 -key (inner product); a missing neighbour has key +inf,
+// so its distance is +inf, or -inf for the inner product.
 template <typename T>
 __global__ void ivf_finish_kernel(const T *__restrict__ key, const int *__restrict__ sel,
                                   const int *__restrict__ cand_idx, int64_t ld, int k, int64_t n,
-                                  int take_sqrt, T *__restrict__ dist, int *__restrict__ ind) {
+                                  int post, T *__restrict__ dist, int *__restrict__ ind) {
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (int64_t)gridDim.x * blockDim.x) {
         const int g = cand_idx[(t / k) * ld + sel[t]];
@@ -165,11 +189,34 @@ __global__ void ivf_finish_kernel(const T *__restrict__ key, const int *__restri
             v = std::numeric_limits<T>::infinity();
         } else {
             ind[t] = g;
-            if (take_sqrt && v >= (T)0)
-                v = sqrt(v);
         }
+        if (post == 1 && v >= (T)0)
+            v = sqrt(v);
+        else if (post == 2)
+            v = (T)1 + v;
+        else if (post == 3)
+            v = -v;
         if (dist)
             dist[t] = v;
+    }
+}
+
+// Rows scaled to unit length in place (normalize_rows, da_utils.cpp:857-905): 1/|x| in
+// double, the product rounded to T; a zero row is left as it is. One thread per row.
+template <typename T>
+__global__ void ivf_normalize_kernel(T *__restrict__ X, int64_t n, int d) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    T *x = X + i * d;
+    double s = 0.0;
+    for (int l = 0; l < d; l++)
+        s += (double)x[l] * (double)x[l];
+    const double nrm = sqrt(s);
+    if (nrm > 0.0) {
+        const double inv = 1.0 / nrm;
+        for (int l = 0; l < d; l++)
+            x[l] = (T)((double)x[l] * inv);
     }
 }
 
@@ -179,10 +226,11 @@ inline unsigned blocks_for(int64_t n) {
 
 } // namespace
 
-template <typename T>
-hipError_t launch_ivf_scan(const T *Q, int nq, int n_probe, int d, const T *X, const int *perm,
-                           const int *list_off, const int *probes, const int *seg, int64_t ld,
-                           T *keys, int *idx, hipStream_t s) {
+template <typename T, bool IP>
+static hipError_t ivf_scan_dispatch(const T *Q, int nq, int n_probe, int d, const T *X,
+                                    const int *perm, const int *list_off, const int *probes,
+                                    const int *seg, int64_t ld, T *keys, int *idx,
+                                    hipStream_t s) {
     const int64_t pairs = (int64_t)nq * n_probe;
     if (pairs <= 0)
         return hipSuccess;
@@ -198,22 +246,40 @@ hipError_t launch_ivf_scan(const T *Q, int nq, int n_probe, int d, const T *X, c
             return hipGetLastError();
         };
         switch ((dc + 7) / 8) {
-        case 1: return run(ivf_scan_chunk_kernel<T, 1>);
-        case 2: return run(ivf_scan_chunk_kernel<T, 2>);
-        case 3: return run(ivf_scan_chunk_kernel<T, 3>);
-        case 4: return run(ivf_scan_chunk_kernel<T, 4>);
+        case 1: return run(ivf_scan_chunk_kernel<T, 1, IP>);
+        case 2: return run(ivf_scan_chunk_kernel<T, 2, IP>);
+        case 3: return run(ivf_scan_chunk_kernel<T, 3, IP>);
+        case 4: return run(ivf_scan_chunk_kernel<T, 4, IP>);
         }
         if constexpr (per == 2) {
             switch ((dc + 7) / 8) {
-            case 5: return run(ivf_scan_chunk_kernel<T, 5>);
-            case 6: return run(ivf_scan_chunk_kernel<T, 6>);
-            case 7: return run(ivf_scan_chunk_kernel<T, 7>);
-            case 8: return run(ivf_scan_chunk_kernel<T, 8>);
+            case 5: return run(ivf_scan_chunk_kernel<T, 5, IP>);
+            case 6: return run(ivf_scan_chunk_kernel<T, 6, IP>);
+            case 7: return run(ivf_scan_chunk_kernel<T, 7, IP>);
+            case 8: return run(ivf_scan_chunk_kernel<T, 8, IP>);
             }
         }
     }
-    hipLaunchKernelGGL(ivf_scan_kernel<T>, grid, dim3(256), 0, s, Q, nq, n_probe, d, X, perm,
-                       list_off, probes, seg, ld, keys, idx);
+    hipLaunchKernelGGL((ivf_scan_kernel<T, IP>), grid, dim3(256), 0, s, Q, nq, n_probe, d, X,
+                       perm, list_off, probes, seg, ld, keys, idx);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ivf_scan(const T *Q, int nq, int n_probe, int d, const T *X, const int *perm,
+                           const int *list_off, const int *probes, const int *seg, int64_t ld,
+                           T *keys, int *idx, hipStream_t s, int ip) {
+    return ip ? ivf_scan_dispatch<T, true>(Q, nq, n_probe, d, X, perm, list_off, probes, seg,
+                                           ld, keys, idx, s)
+              : ivf_scan_dispatch<T, false>(Q, nq, n_probe, d, X, perm, list_off, probes, seg,
+                                            ld, keys, idx, s);
+}
+
+template <typename T> hipError_t launch_ivf_normalize(T *X, int64_t n, int d, hipStream_t s) {
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(ivf_normalize_kernel<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       s, X, n, d);
     return hipGetLastError();
 }
 
@@ -226,18 +292,19 @@ template <typename T> hipError_t launch_ivf_pad(T *keys, int *idx, int64_t n, hi
 
 template <typename T>
 hipError_t launch_ivf_finish(const T *key, const int *sel, const int *cand_idx, int64_t ld, int k,
-                             int64_t n, int take_sqrt, T *dist, int *ind, hipStream_t s) {
+                             int64_t n, int post, T *dist, int *ind, hipStream_t s) {
     if (n <= 0)
         return hipSuccess;
     hipLaunchKernelGGL(ivf_finish_kernel<T>, dim3(blocks_for(n)), dim3(256), 0, s, key, sel,
-                       cand_idx, ld, k, n, take_sqrt, dist, ind);
+                       cand_idx, ld, k, n, post, dist, ind);
     return hipGetLastError();
 }
 
 #define DA_IVF_INST(T)                                                                           \
     template hipError_t launch_ivf_scan<T>(const T *, int, int, int, const T *, const int *,     \
                                            const int *, const int *, const int *, int64_t, T *,  \
-                                           int *, hipStream_t);                                  \
+                                           int *, hipStream_t, int);                             \
+    template hipError_t launch_ivf_normalize<T>(T *, int64_t, int, hipStream_t);                 \
     template hipError_t launch_ivf_pad<T>(T *, int *, int64_t, hipStream_t);                     \
     template hipError_t launch_ivf_finish<T>(const T *, const int *, const int *, int64_t, int,  \
                                              int64_t, int, T *, int *, hipStream_t);

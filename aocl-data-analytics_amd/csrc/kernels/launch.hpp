@@ -238,19 +238,22 @@ hipError_t launch_nn_regress(int nq, const int *off, int k, const int *ind, cons
 // ---- IVF-Flat search ----
 // Scans, for each (query q < nq, probe p < n_probe) pair, the inverted list probes[q*n_probe+p]
 // (vectors X[perm[r]] for r in [list_off[c], list_off[c+1])) and writes the exact squared
-// distances (rounded to T) and global ids to keys/idx[q*ld + seg[q*n_probe+p] + r - b].
+// distances (ip: the negated dot products -q.x), rounded to T, and global ids to
+// keys/idx[q*ld + seg[q*n_probe+p] + r - b].
 template <typename T>
 hipError_t launch_ivf_scan(const T *Q, int nq, int n_probe, int d, const T *X, const int *perm,
                            const int *list_off, const int *probes, const int *seg, int64_t ld,
-                           T *keys, int *idx, hipStream_t s);
+                           T *keys, int *idx, hipStream_t s, int ip = 0);
 // keys = +inf, idx = INT_MAX over n entries (candidate padding).
 template <typename T> hipError_t launch_ivf_pad(T *keys, int *idx, int64_t n, hipStream_t s);
 // Selected (key, column) pairs of n = rows x k -> (distance, index): the index is
-// cand_idx[(t / k) * ld + column]; INT_MAX there becomes -1 with distance +inf; square roots
-// when take_sqrt. dist may be null.
+// cand_idx[(t / k) * ld + column]; INT_MAX there becomes -1 with key +inf. Distance: post 0
+// the key, 1 its square root, 2 1 + key (cosine), 3 -key (inner product). dist may be null.
 template <typename T>
 hipError_t launch_ivf_finish(const T *key, const int *sel, const int *cand_idx, int64_t ld, int k,
-                             int64_t n, int take_sqrt, T *dist, int *ind, hipStream_t s);
+                             int64_t n, int post, T *dist, int *ind, hipStream_t s);
+// n rows of d values scaled to unit length in place (1/|x| in double; zero rows unchanged).
+template <typename T> hipError_t launch_ivf_normalize(T *X, int64_t n, int d, hipStream_t s);
 
 // ---- DBSCAN over a neighbour graph in CSR form (offs: n + 1 64-bit offsets, ind) ----
 // parent[i] = i.

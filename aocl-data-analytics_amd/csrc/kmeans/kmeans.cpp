@@ -450,12 +450,11 @@ da_status KMeansHandle<T>::handle_empty_clusters(const U *dX, double *C, int *la
     empty_error = false;
     hipStream_t st = gpu->stream();
     const int n = n_samples_, d = n_features_, k = n_clusters_;
-    std::vector<int> lab(n);
-    DA_HIP(err, hipMemcpyAsync(lab.data(), labels, n * sizeof(int), hipMemcpyDeviceToHost, st));
-    DA_HIP(err, hipStreamSynchronize(st));
+    // The cluster sizes of the update just made (k integers) decide whether anything is to
+    // be done; the labels and the data only come to the host when a cluster is empty.
     std::vector<int> count(k, 0);
-    for (int i = 0; i < n; i++)
-        count[lab[i]]++;
+    DA_HIP(err, hipMemcpyAsync(count.data(), counts_.ptr(), k * sizeof(int), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
     int n_empty = 0;
     for (int c = 0; c < k; c++)
         n_empty += count[c] == 0;
@@ -465,6 +464,9 @@ da_status KMeansHandle<T>::handle_empty_clusters(const U *dX, double *C, int *la
         empty_error = true;
         return da_status_success;
     }
+    std::vector<int> lab(n);
+    DA_HIP(err, hipMemcpyAsync(lab.data(), labels, n * sizeof(int), hipMemcpyDeviceToHost, st));
+    DA_HIP(err, hipStreamSynchronize(st));
     std::vector<U> A((size_t)n * d);
     std::vector<double> cen((size_t)k * d), inv;
     DA_HIP(err, hipMemcpyAsync(A.data(), dX, A.size() * sizeof(U), hipMemcpyDeviceToHost, st));
@@ -719,10 +721,19 @@ template <typename T> da_status KMeansHandle<T>::initialise_centres(da_int metho
         return da_status_success;
     }
     case INIT_RANDOM: {
-        std::vector<da_int> all(n), pick(k);
-        std::iota(all.begin(), all.end(), 0);
-        std::sample(all.begin(), all.end(), pick.begin(), k, mt_);
-        std::vector<int> p32(pick.begin(), pick.end());
+        // da_std::sample (da_std.hpp:102-118), selection sampling: sample i of the remaining
+        // `left` is taken when a draw of uniform_int_distribution(0, left - 1) is below the
+        // number still needed (one draw per sample visited, unlike std::sample).
+        std::vector<int> p32;
+        p32.reserve(k);
+        da_int need = k, left = n;
+        for (da_int i = 0; need > 0; i++, left--) {
+            std::uniform_int_distribution<da_int> u(0, left - 1);
+            if (u(mt_) < need) {
+                p32.push_back((int)i);
+                need--;
+            }
+        }
         DA_HIP(err, hipMemcpyAsync(idx_.ptr(), p32.data(), k * sizeof(int), hipMemcpyHostToDevice, st));
         DA_HIP(err, launch_gather_rows<T>(X_.as<T>(), d, idx_.as<int>(), k, Ccur_, st));
         DA_HIP(err, hipStreamSynchronize(st));

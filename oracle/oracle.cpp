@@ -732,7 +732,7 @@ int oracle_kmeans_init(int n, int d, const double *X, int k, int init, long long
 
 /*
  * kmeans<T>::compute restated (kmeans.cpp:351-783) for Lloyd (algorithm 0) or Elkan (1)
- * with initialisation 0 random (std::sample, :1404-1425), 1 k-means++, 2 supplied (C_init),
+ * with initialisation 0 random (da_std::sample, :1404-1425), 1 k-means++, 2 supplied (C_init),
  * 3 random partitions (:1426-1457), 4 AFK-MC2 with afk_m chain steps. empty_mode: 0
  * ignore, 1 error, 2 split. lp_mode: 0 none, 1 mixed precision of double data (float
  * phase), 2 of float data (half phase), with lp_tol and lp_max_iter. Returns 0 on success,
@@ -765,9 +765,16 @@ static int kmeans_impl(int n, int d, const double *X, int k, int init, const dou
         if (init == 2) {
             std::copy(C_init, C_init + (size_t)k * d, C.begin());
         } else if (init == 0) {
-            std::vector<int> all(n), pick(k);
-            std::iota(all.begin(), all.end(), 0);
-            std::sample(all.begin(), all.end(), pick.begin(), k, mt);
+            // da_std::sample, da_std.hpp:102-118 (selection sampling, one draw per sample).
+            std::vector<int> pick;
+            long long need = k, left = n;
+            for (int i = 0; need > 0; i++, left--) {
+                std::uniform_int_distribution<long long> u(0, left - 1);
+                if (u(mt) < need) {
+                    pick.push_back(i);
+                    need--;
+                }
+            }
             for (int c = 0; c < k; c++)
                 for (int l = 0; l < d; l++)
                     C[(size_t)c * d + l] = X[(size_t)pick[c] * d + l];

@@ -210,24 +210,41 @@ def nn_regress(Xt, t, Xq, k=None, r=None, metric="euclidean", p=2.0, weights="un
     return out, first
 
 
+def _unit_rows(A, T):
+    """normalize_rows (da_utils.cpp:857-905) as the GPU stores it: 1/|x| in double, the
+    product rounded to the data type, a zero row left at zero."""
+    A = np.asarray(A, dtype=np.float64)
+    n = np.sqrt((A * A).sum(axis=1))
+    inv = np.where(n > 0, 1.0 / np.where(n > 0, n, 1.0), 1.0)
+    return (A * inv[:, None]).astype(T).astype(np.float64)
+
+
 def ivf_search(C, X, Xq, k, n_probe, metric="sqeuclidean", single=False):
     """IVF-Flat add and search restated (approximate_neighbors.cpp:565-842 add, :944-1315
     search) for given centroids C: every row of X (in order of addition) joins the list of
     its nearest centroid (ties to the lower centroid); a query scans the n_probe lists of its
     nearest centroids (by (distance, centroid)) in list order and keeps its k nearest
     candidates, equal distances resolved for the lower list and the earlier position in it as
-    the reference's first-seen heap does; padded with index -1 and distance +inf; euclidean
-    distances are square-rooted. Exact squared
-    distances, rounded to float when single. Returns (ind, dist, list_sizes)."""
+    the reference's first-seen heap does; padded with index -1 and distance +inf. Distances:
+    squared euclidean (square-rooted for "euclidean"); for "inner product" the key -x.y,
+    reported as x.y (padding -inf); for "cosine" the rows and queries normalised first, the
+    key -x.y reported as 1 - x.y. Exact keys, rounded to float when single. Returns (ind,
+    dist, list_sizes)."""
     T = np.float32 if single else np.float64
+    metric = metric.replace("_", " ")
+    ip = metric in ("inner product", "cosine")
     C = np.asarray(C, dtype=np.float64)
     X = np.asarray(X, dtype=np.float64)
     Q = np.asarray(Xq, dtype=np.float64)
+    if metric == "cosine":
+        X, Q = _unit_rows(X, T), _unit_rows(Q, T)
 
     def sq(A, B):
+        if ip:
+            return (-(A @ B.T)).astype(T)
         return ((A[:, None, :] - B[None, :, :]) ** 2).sum(-1).astype(T)
 
-    lab = np.argmin(sq(X, C), axis=1)
+    lab = np.argmin(-(X @ C.T) if ip else sq(X, C), axis=1)
     sizes = np.bincount(lab, minlength=len(C))
     cd = sq(Q, C)
     ind = np.full((len(Q), k), -1, dtype=np.int64)
@@ -241,6 +258,10 @@ def ivf_search(C, X, Xq, k, n_probe, metric="sqeuclidean", single=False):
         dist[q, :len(order)] = dq[order]
     if metric == "euclidean":
         dist = np.where(np.isfinite(dist), np.sqrt(dist), dist).astype(T)
+    elif metric == "cosine":
+        dist = (T(1) + dist).astype(T)
+    elif metric == "inner product":
+        dist = (-dist).astype(T)
     return ind, dist, sizes
 
 

@@ -1,5 +1,5 @@
 """Independent restatement, for tests, of the C++ random draws the reference's k-means
-initialisations make (kmeans.cpp:1524-1811 use std::mt19937_64 with
+initialisations make (kmeans.cpp:1404-1811 use std::mt19937_64 with
 std::uniform_int_distribution<int>, std::discrete_distribution<int> and
 std::uniform_real_distribution<double> as libstdc++ implements them), and of the two
 initialisations themselves in exact arithmetic. On integer-valued data the squared
@@ -171,6 +171,22 @@ def afkmc2(X, k, seed, m, dist=_sq):
                 x, dx = y, dy
         chosen.append(x)
     return chosen
+
+
+def selection_sample(n, k, seed):
+    """Rows of the random initialisation (kmeans.cpp:1404-1425 through da_std::sample,
+    da_std.hpp:102-118): row i of the `left` not yet visited is taken when a uniform draw
+    from [0, left - 1] is below the number still needed."""
+    g = MT19937_64(seed)
+    out, need, left = [], k, n
+    i = 0
+    while need > 0:
+        if uniform_int(g, 0, left - 1) < need:
+            out.append(i)
+            need -= 1
+        i += 1
+        left -= 1
+    return out
 
 
 def rows(X, idx):

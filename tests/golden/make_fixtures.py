@@ -468,6 +468,72 @@ known["kmeans_mixed"] = [
     dict(base="larger data matrix, lloyd", source="kmeans_test_data.hpp:958-968",
          lp_max_iter=3, lp_tol=1e-3, n_iter={"double": 1, "float": 1}, lp_n_iter=3),
 ]
+
+
+# Every IVF-Flat case of ann_tests.hpp:395-1850 with fixed expectations, read from the test
+# source as data: the constructor (n_list, n_probe, k, metric, algorithm, order), the scalar
+# fields it sets (sizes, leading dimensions, seed, k-means iterations, train fraction) and the
+# vectors X_train, X_test, expected_kind and expected_kdist. "...Euclidean" cases are the
+# squared-euclidean ones with the metric changed and the distances square-rooted
+# (ConvertToEuclidean, :141-153).
+def _ann_cases():
+    import math
+    import re
+    src = open(os.path.join(REF, "tests", "unit_tests", "approximate_neighbors",
+                            "ann_tests.hpp")).read()
+    lines = src.split("\n")
+    heads = [(m.start(), m.group(1)) for m in
+             re.finditer(r"void (\w+)\(std::vector<ANNParamType<T>> &params\) \{", src)]
+    bodies = {}
+    for i, (pos, name) in enumerate(heads):
+        end = heads[i + 1][0] if i + 1 < len(heads) else len(src)
+        bodies[name] = (src.count("\n", 0, pos) + 1, src[pos:end])
+
+    def num(t):
+        return float(t.strip().rstrip("f"))
+
+    def parse(name):
+        line0, b = bodies[name]
+        m = re.search(r'ANNParamType<T> test\((\d+), (\d+), (\d+), "([^"]+)", "([^"]+)", '
+                      r'"([^"]+)"\)', b)
+        c = {"name": name, "source": "tests/unit_tests/approximate_neighbors/ann_tests.hpp:%d"
+             % line0, "n_list": int(m.group(1)), "n_probe": int(m.group(2)),
+             "k": int(m.group(3)), "metric": m.group(4), "algorithm": m.group(5),
+             "order": m.group(6), "seed": 0,
+             "kmeans_iter": 10, "train_fraction": 1.0}
+        for f, v in re.findall(r"test\.(n_features|n_samples|n_queries|ldx_train|ldx_test|"
+                               r"seed|kmeans_iter|train_fraction) = ([^;]+);", b):
+            v = v.strip()
+            mm = re.fullmatch(r"test\.(\w+)", v)
+            c[f] = c[mm.group(1)] if mm else (float(v) if "." in v else int(v))
+        for f, typ in (("X_train", "T"), ("X_test", "T"), ("expected_kind", "da_int"),
+                       ("expected_kdist", "T")):
+            mm = re.search(r"std::vector<%s> %s\{([^}]*)\}" % (typ, f), b, re.S)
+            c[f] = None if mm is None else [num(t) for t in mm.group(1).split(",")
+                                             if t.strip()]
+        if c["expected_kind"] is not None:
+            c["expected_kind"] = [int(v) for v in c["expected_kind"]]
+        return c
+
+    out = []
+    for name in bodies:
+        b = bodies[name][1]
+        if "ANNParamType<T> test(" in b and "csvname" not in b:
+            c = parse(name)
+            if c["X_train"] is None or c["expected_kdist"] is None:
+                continue
+            out.append(c)
+        conv = re.search(r"(\w+SqEuclidean)\(params\);\s*ConvertToEuclidean", b)
+        if conv:
+            c = parse(conv.group(1))
+            c["name"] = name
+            c["metric"] = "euclidean"
+            c["expected_kdist"] = [math.sqrt(v) for v in c["expected_kdist"]]
+            out.append(c)
+    return out
+
+
+known["ivf_cases"] = _ann_cases()
 with open(os.path.join(HERE, "known_answers.json"), "w") as f:
     json.dump(known, f, indent=1)
 print("fixtures written to", HERE)

@@ -124,6 +124,27 @@ def test_ivf_train_add_search_match_oracle(faiss, oracle, dtype, order):
     np.testing.assert_array_equal(idx._ann.list_sizes, sizes)
 
 
+@pytest.mark.parametrize("dtype,order", DTYPES_ORDERS)
+def test_ivf_inner_product_matches_oracle(faiss, oracle, dtype, order):
+    """METRIC_INNER_PRODUCT: spherical k-means lists, the largest inner products first,
+    reported as inner products (faiss's convention)."""
+    corpus = np.array(X_CORPUS - 0.5, dtype=dtype, order=order)
+    query = np.array(X_QUERY - 0.5, dtype=dtype, order=order)
+    idx = faiss.IndexIVFFlat(faiss.IndexFlatL2(D), D, 8, faiss.METRIC_INNER_PRODUCT)
+    assert idx.metric_type == faiss.METRIC_INNER_PRODUCT
+    idx.nprobe = 3
+    idx.train(corpus)
+    idx.add(corpus)
+    dist, ind = idx.search(query, K)
+    assert (np.diff(dist.astype(np.float64), axis=1) <= 0).all()
+    C = idx._ann.cluster_centroids
+    oi, od, sizes = oracle.ivf_search(C, corpus, query, K, 3, "inner product",
+                                      single=dtype == np.float32)
+    np.testing.assert_array_equal(ind, oi)
+    np.testing.assert_allclose(dist, od, rtol=_rtol(dtype) * 10, atol=1e-6)
+    np.testing.assert_array_equal(idx._ann.list_sizes, sizes)
+
+
 def _recall(ind, exact):
     return np.mean([len(set(a) & set(b)) / K for a, b in zip(ind, exact)])
 
@@ -196,7 +217,13 @@ def test_kmeans_train_assign(faiss, oracle):
     np.testing.assert_array_equal(ind, ref.labels)
 
 
-def test_kmeans_spherical_not_built(faiss):
-    km = faiss.Kmeans(8, 2, spherical=True, seed=1)
-    with pytest.raises(aoclda.DAError):
-        km.train(np.random.default_rng(0).random((20, 8)))
+def test_kmeans_spherical(faiss):
+    """spherical=True maps to the k-means option distance = "cosine": unit centroids, the
+    same as the k-means front end."""
+    X = np.random.default_rng(0).standard_normal((200, 8))
+    km = faiss.Kmeans(8, 3, spherical=True, seed=1, niter=15)
+    km.train(X)
+    np.testing.assert_allclose(np.linalg.norm(km.centroids, axis=1), 1.0, atol=1e-12)
+    ref = aoclda.clustering.kmeans(n_clusters=3, max_iter=15, n_init=1, seed=1,
+                                   distance="cosine").fit(X)
+    np.testing.assert_array_equal(km.centroids, ref.cluster_centres)

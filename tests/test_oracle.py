@@ -519,3 +519,95 @@ def test_ivf_oracle(known, oracle):
         best = members[np.lexsort((members, d2))][:5]
         got = ind[q][ind[q] >= 0]
         np.testing.assert_array_equal(got, best)
+
+
+def ann_case_arrays(c, dtype=np.float64):
+    """Training and query matrices of an ann_tests.hpp case from its storage order and
+    leading dimensions."""
+    def mat(v, rows, cols, ld):
+        v = np.asarray(v, dtype=dtype)
+        if c["order"] == "column-major":
+            return np.array([[v[i + j * ld] for j in range(cols)] for i in range(rows)],
+                            dtype=dtype)
+        return np.array([[v[i * ld + j] for j in range(cols)] for i in range(rows)],
+                        dtype=dtype)
+    return (mat(c["X_train"], c["n_samples"], c["n_features"], c["ldx_train"]),
+            mat(c["X_test"], c["n_queries"], c["n_features"], c["ldx_test"]))
+
+
+def _as_order(a, order):
+    return a.T.ravel() if order == "column-major" else a.ravel()
+
+
+@pytest.mark.parametrize("idx", range(52))
+def test_ivf_reference_cases_end_to_end(known, oracle, idx):
+    """Every fixed IVF-Flat case of ann_tests.hpp (squared euclidean, euclidean, inner product
+    and cosine) through the oracle: the k-means training of train_ivfflat (random initial
+    centres, one initialisation, tolerance sqrt(eps); spherical k-means without data
+    normalisation for inner product and cosine, on normalised rows for cosine), then add and
+    search, against the expected indices and distances (3000 eps, as the reference's test).
+    Cases that subsample the training rows draw from std::mt19937 and are left to the GPU
+    tests."""
+    cases = known["ivf_cases"]
+    if idx >= len(cases):
+        pytest.skip("fewer cases")
+    c = cases[idx]
+    if c["train_fraction"] < 1.0:
+        pytest.skip("training sample drawn with std::mt19937: checked on the GPU")
+    Xt, Xq = ann_case_arrays(c)
+    metric = c["metric"]
+    seed = max(c["seed"], 0)
+    tol = np.sqrt(np.finfo(np.float64).eps)
+    if metric in ("inner product", "cosine"):
+        Xk = oracle._unit_rows(Xt, np.float64) if metric == "cosine" else Xt
+        r = oracle.kmeans_sph(Xk, c["n_list"], "random", seed=seed, max_iter=c["kmeans_iter"],
+                              tol=tol, normalize=False)
+    else:
+        r = oracle.kmeans(Xt, c["n_list"], "random", seed=seed, max_iter=c["kmeans_iter"],
+                          tol=tol)
+    ind, dist, sizes = oracle.ivf_search(r["centres"], Xt, Xq, c["k"], c["n_probe"], metric)
+    np.testing.assert_allclose(_as_order(dist, c["order"]), c["expected_kdist"], rtol=0,
+                               atol=3e3 * np.finfo(np.float64).eps)
+    if c["expected_kind"] is not None:
+        assert _as_order(ind, c["order"]).tolist() == c["expected_kind"]
+    assert sizes.sum() == c["n_samples"]
+
+
+@pytest.mark.parametrize("metric", ["inner product", "cosine"])
+def test_ivf_oracle_ip_cosine_all_lists_is_exact(oracle, metric):
+    """With every list probed the IVF search is the exact search: the largest inner products
+    (ties to the lower index) or the smallest cosine distances of numpy's brute force."""
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((500, 7)) * rng.uniform(0.5, 2, (500, 1))
+    Q = rng.standard_normal((30, 7))
+    C = X[rng.choice(500, 9, replace=False)]
+    ind, dist, _ = oracle.ivf_search(C, X, Q, 8, 9, metric)
+    if metric == "cosine":
+        Xn = X / np.linalg.norm(X, axis=1, keepdims=True)
+        Qn = Q / np.linalg.norm(Q, axis=1, keepdims=True)
+        D = 1.0 - Qn @ Xn.T
+    else:
+        D = -(Q @ X.T)
+    for q in range(len(Q)):
+        best = np.lexsort((np.arange(500), D[q]))[:8]
+        np.testing.assert_array_equal(ind[q], best)
+        np.testing.assert_allclose(dist[q], D[q, best] if metric == "cosine" else -D[q, best],
+                                   rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", [0, 8, 78])
+@pytest.mark.parametrize("n,k", [(12, 4), (40, 7), (300, 64)])
+def test_random_init_is_the_references_selection_sampling(oracle, seed, n, k):
+    """The random initial centres are the rows da_std::sample selects (one engine draw per
+    row visited), not those of std::sample (which draws two indices from one engine call):
+    the two differ, so a mix-up fails here."""
+    import _stdrng
+    X = np.arange(n * 2, dtype=np.float64).reshape(n, 2)
+    want = _stdrng.rows(X, _stdrng.selection_sample(n, k, seed))
+    got = oracle.kmeans(X, k, "random", seed=seed, max_iter=0)["centres"]
+    np.testing.assert_array_equal(got, want)
+    gs = oracle.kmeans_sph(X + 1.0, k, "random", seed=seed, max_iter=0, normalize=False)
+    np.testing.assert_allclose(gs["centres"],
+                               want / np.linalg.norm(want + 1.0, axis=1, keepdims=True) +
+                               1.0 / np.linalg.norm(want + 1.0, axis=1, keepdims=True),
+                               rtol=1e-15)
