@@ -231,131 +231,165 @@ __device__ void block_exclusive_scan(int *v, int m, int *wsum) {
     __syncthreads();
 }
 
-// Fixed-point totals per (cluster, feature) of the points [b*ppb, (b+1)*ppb) whose label is
-// in [c0, c0+kc): the points are counting-sorted by label in LDS (two 32-bit LDS atomics per
-// point), then thread pairs (cluster, feature) sum their cluster's values in registers and
-// add the workgroup totals to the device-wide accumulators. Integer sums are exact, so
-// neither the order inside a cluster nor the order of the workgroups changes the result.
+// Fixed-point totals per (cluster, feature) pair of one pass: the clusters [c0, c0+kc) and the
+// features [l0, l0+dl) (kc * dl <= 1024 * P pairs). The grid's workgroups take the chunks of
+// ppb points in turn (chunk b, b + gridDim.x, ...): a chunk's points are counting-sorted by
+// label in LDS (two 32-bit LDS atomics per point), then thread t adds the values of its pairs
+// p = t + 1024 u (u < P) in registers, across all of its workgroup's chunks. Each workgroup
+// finally writes its totals, wpart[(b * kc * dl + p) * 2 + w] (w = 0: the value scaled by
+// 2^e[l] and rounded to int64, w = 1: the remainder scaled by 2^(e[l] + lshift)), and its
+// cluster sizes wcnt[b * kc + c]: plain stores, no device-wide atomics. Integer sums are
+// exact, so neither the order of the points nor that of the workgroups changes the result.
 // With winv, point i contributes x_i * winv[i] (the normalised samples of spherical k-means).
-template <typename T>
+template <typename T, int P>
 __global__ __launch_bounds__(1024) void kmeans_partial_kernel(
     const T *__restrict__ X, int n, int d, const int *__restrict__ labels, int c0, int kc,
-    const int *__restrict__ escale, int ppb, long long *__restrict__ part,
-    int *__restrict__ cnt, const double *__restrict__ winv) {
+    int l0, int dl, const int *__restrict__ escale, int ppb, long long *__restrict__ wpart,
+    int *__restrict__ wcnt, const double *__restrict__ winv) {
     extern __shared__ int sh[];
     int *off = sh;                 // kc + 1 offsets
     int *fill = off + kc + 1;      // kc fill counters
     int *list = fill + kc;         // ppb point offsets, sorted by label
     __shared__ int wsum[16];
-    const int b = blockIdx.x;
-    const int64_t start = (int64_t)b * ppb;
-    const int m = (int)min((int64_t)ppb, (int64_t)n - start);
-    for (int t = threadIdx.x; t <= kc; t += 1024)
-        off[t] = 0;
-    for (int t = threadIdx.x; t < kc; t += 1024)
-        fill[t] = 0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < m; i += 1024) {
-        const int lab = labels[start + i] - c0;
-        if (lab >= 0 && lab < kc)
-            atomicAdd(&off[lab], 1);
-    }
-    __syncthreads();
-    block_exclusive_scan(off, kc + 1, wsum);
-    for (int i = threadIdx.x; i < m; i += 1024) {
-        const int lab = labels[start + i] - c0;
-        if (lab >= 0 && lab < kc)
-            list[off[lab] + atomicAdd(&fill[lab], 1)] = i;
-    }
-    __syncthreads();
+    const int pairs = kc * dl;
     const int lshift = escale[d];
-    const int pairs = kc * d;
-    for (int p = threadIdx.x; p < pairs; p += 1024) {
-        const int c = p / d, l = p - c * d;
-        const int j0 = off[c], j1 = off[c + 1];
-        if (j0 == j1)
-            continue;
-        const int el = escale[l];
-        long long H = 0, L = 0;
-        const double *wv = winv ? winv + start : nullptr;
-        auto add = [&](T x, int j) {
-            const double v = wv ? (double)x * wv[list[j]] : (double)x;
-            if (v == v && v - v == 0.0) {
-                const double sv = ldexp(v, el);
-                const double h = rint(sv);
-                H += (long long)h;
-                L += (long long)rint(ldexp(sv - h, lshift));
-            }
-        };
-        const T *xs = X + start * d + l;
-        int j = j0;
-        // Eight independent gathers in flight per thread.
-        for (; j + 8 <= j1; j += 8) {
-            T v[8];
+    long long H[P], L[P];
+    int cnt[P];
 #pragma unroll
-            for (int u = 0; u < 8; u++)
-                v[u] = xs[(int64_t)list[j + u] * d];
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                add(v[u], j + u);
+    for (int u = 0; u < P; u++) {
+        H[u] = 0;
+        L[u] = 0;
+        cnt[u] = 0;
+    }
+    const int nchunks = (int)(((int64_t)n + ppb - 1) / ppb);
+    for (int ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const int64_t start = (int64_t)ch * ppb;
+        const int m = (int)min((int64_t)ppb, (int64_t)n - start);
+        for (int t = threadIdx.x; t <= kc; t += 1024)
+            off[t] = 0;
+        for (int t = threadIdx.x; t < kc; t += 1024)
+            fill[t] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < m; i += 1024) {
+            const int lab = labels[start + i] - c0;
+            if (lab >= 0 && lab < kc)
+                atomicAdd(&off[lab], 1);
         }
-        for (; j < j1; j++)
-            add(xs[(int64_t)list[j] * d], j);
-        if (H != 0)
-            atomicAdd(reinterpret_cast<unsigned long long *>(&part[2 * (int64_t)p]),
-                      (unsigned long long)H);
-        if (L != 0)
-            atomicAdd(reinterpret_cast<unsigned long long *>(&part[2 * (int64_t)p + 1]),
-                      (unsigned long long)L);
-        if (l == 0)
-            atomicAdd(&cnt[c], j1 - j0);
+        __syncthreads();
+        block_exclusive_scan(off, kc + 1, wsum);
+        for (int i = threadIdx.x; i < m; i += 1024) {
+            const int lab = labels[start + i] - c0;
+            if (lab >= 0 && lab < kc)
+                list[off[lab] + atomicAdd(&fill[lab], 1)] = i;
+        }
+        __syncthreads();
+        const double *wv = winv ? winv + start : nullptr;
+#pragma unroll
+        for (int u = 0; u < P; u++) {
+            const int p = (int)threadIdx.x + 1024 * u;
+            if (p >= pairs)
+                continue;
+            const int c = p / dl, l = l0 + (p - c * dl);
+            const int j0 = off[c], j1 = off[c + 1];
+            cnt[u] += j1 - j0;
+            const int el = escale[l];
+            long long h = 0, lo = 0;
+            auto add = [&](T x, int j) {
+                const double v = wv ? (double)x * wv[list[j]] : (double)x;
+                if (v == v && v - v == 0.0) {
+                    const double sv = ldexp(v, el);
+                    const double hv = rint(sv);
+                    h += (long long)hv;
+                    lo += (long long)rint(ldexp(sv - hv, lshift));
+                }
+            };
+            const T *xs = X + start * d + l;
+            int j = j0;
+            // Eight independent gathers in flight per thread.
+            for (; j + 8 <= j1; j += 8) {
+                T v[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    v[e] = xs[(int64_t)list[j + e] * d];
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    add(v[e], j + e);
+            }
+            for (; j < j1; j++)
+                add(xs[(int64_t)list[j] * d], j);
+            H[u] += h;
+            L[u] += lo;
+        }
+        __syncthreads(); // the next chunk reuses the LDS lists
+    }
+    long long *wp = wpart + (int64_t)blockIdx.x * pairs * 2;
+#pragma unroll
+    for (int u = 0; u < P; u++) {
+        const int p = (int)threadIdx.x + 1024 * u;
+        if (p >= pairs)
+            continue;
+        wp[2 * (int64_t)p] = H[u];
+        wp[2 * (int64_t)p + 1] = L[u];
+        if (p % dl == 0)
+            wcnt[(int64_t)blockIdx.x * kc + p / dl] = cnt[u];
     }
 }
 
-// One workgroup per cluster: the centre from the device-wide totals, which are then cleared
-// (the count once every thread of the workgroup has read it). sph: the sum normalised to unit
-// length instead of divided by the count, a zero sum left at the origin (normalize_rows_inplace,
-// da_utils.cpp:807-855).
-__global__ __launch_bounds__(256) void kmeans_reduce_kernel(long long *__restrict__ part,
-                                                            int *__restrict__ cnt, int c0, int d,
-                                                            const int *__restrict__ escale,
-                                                            double *__restrict__ C,
-                                                            int *__restrict__ counts, int sph) {
-    __shared__ double red[256];
-    const int c = blockIdx.x;
-    const int count = cnt[c];
+// One workgroup per cluster of the pass: the totals of the nb workgroups' partials (summed as
+// integers, so in any order), then the centre: the sum divided by the cluster size (an empty
+// cluster keeps count 1 and a zero sum, i.e. the origin: scale_current_cluster_centres,
+// kmeans_lloyd.hpp:363-373), or for spherical k-means (sph) the sum itself, normalised
+// afterwards.
+__global__ __launch_bounds__(256) void kmeans_reduce_kernel(
+    const long long *__restrict__ wpart, const int *__restrict__ wcnt, int nb, int c0, int kc,
+    int l0, int dl, int d, const int *__restrict__ escale, double *__restrict__ C,
+    int *__restrict__ counts, int sph) {
+    __shared__ long long sh_h[256], sh_l[256];
+    __shared__ int sh_cnt;
+    const int c = blockIdx.x, t = threadIdx.x;
+    const int pairs = kc * dl;
+    if (t < 64) {
+        int s = 0;
+        for (int b = t; b < nb; b += 64)
+            s += wcnt[(int64_t)b * kc + c];
+        for (int o = 32; o > 0; o >>= 1)
+            s += __shfl_xor(s, o, 64);
+        if (t == 0)
+            sh_cnt = s;
+    }
+    // Threads are (feature, group) pairs: the groups split the workgroups' partials.
+    const int width = dl < 256 ? dl : 256;
+    const int groups = 256 / width;
+    const int lw = t % width, grp = t / width;
     const int lshift = escale[d];
-    double ss = 0.0;
-    for (int l = threadIdx.x; l < d; l += blockDim.x) {
-        const int64_t e = (int64_t)c * d + l;
-        const long long H = part[2 * e], L = part[2 * e + 1];
-        const int ex = escale[l];
-        const double sum = ldexp((double)H, -ex) + ldexp((double)L, -(ex + lshift));
-        // An empty cluster keeps count 1 and a zero sum, i.e. the origin
-        // (scale_current_cluster_centres, kmeans_lloyd.hpp:363-373).
-        C[(int64_t)(c0 + c) * d + l] = sph ? sum : sum / (double)(count > 0 ? count : 1);
-        ss = fma(sum, sum, ss);
-        part[2 * e] = 0;
-        part[2 * e + 1] = 0;
-    }
-    if (sph) {
-        red[threadIdx.x] = ss;
-        __syncthreads();
-        for (int m = 128; m > 0; m >>= 1) {
-            if ((int)threadIdx.x < m && threadIdx.x + m < blockDim.x)
-                red[threadIdx.x] += red[threadIdx.x + m];
-            __syncthreads();
+    for (int lb = 0; lb < dl; lb += width) {
+        const int li = lb + lw;
+        long long h = 0, lo = 0;
+        if (grp < groups && li < dl) {
+            const int64_t p = (int64_t)c * dl + li;
+            for (int b = grp; b < nb; b += groups) {
+                h += wpart[((int64_t)b * pairs + p) * 2];
+                lo += wpart[((int64_t)b * pairs + p) * 2 + 1];
+            }
         }
-        const double nrm = sqrt(red[0]);
-        const double inv = nrm == 0.0 ? 1.0 : 1.0 / nrm;
-        for (int l = threadIdx.x; l < d; l += blockDim.x)
-            C[(int64_t)(c0 + c) * d + l] *= inv;
+        sh_h[t] = h;
+        sh_l[t] = lo;
+        __syncthreads();
+        if (grp == 0 && li < dl) {
+            for (int g = 1; g < groups; g++) {
+                h += sh_h[t + g * width];
+                lo += sh_l[t + g * width];
+            }
+            const int l = l0 + li;
+            const int ex = escale[l];
+            const double sum = ldexp((double)h, -ex) + ldexp((double)lo, -(ex + lshift));
+            const int count = sh_cnt;
+            C[(int64_t)(c0 + c) * d + l] = sph ? sum : sum / (double)(count > 0 ? count : 1);
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        counts[c0 + c] = count;
-        cnt[c] = 0;
-    }
+    if (t == 0)
+        counts[c0 + c] = sh_cnt;
 }
 
 __global__ void zero_words_kernel(long long *__restrict__ p, int64_t n) {
@@ -797,31 +831,54 @@ hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k
     return hipGetLastError();
 }
 
+template <typename T, int P>
+static hipError_t partial_launch(const T *X, int n, int d, const int *labels, int c0, int kc,
+                                 int l0, int dl, const int *escale, int ppb, int grid,
+                                 long long *wpart, int *wcnt, hipStream_t s, const double *winv,
+                                 size_t lds) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)kmeans_partial_kernel<T, P>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((kmeans_partial_kernel<T, P>), dim3(grid), dim3(1024), lds, s, X, n, d,
+                       labels, c0, kc, l0, dl, escale, ppb, wpart, wcnt, winv);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *labels, int c0,
-                                      int kc, const int *escale, int ppb, int nblocks,
-                                      long long *part, int *cnt, hipStream_t s,
+                                      int kc, int l0, int dl, const int *escale, int ppb,
+                                      int grid, long long *wpart, int *wcnt, hipStream_t s,
                                       const double *winv) {
     const size_t lds = kmeans_partial_lds(kc, ppb);
-    if (lds > 160 * 1024)
+    const int64_t pairs = (int64_t)kc * dl;
+    if (lds > 160 * 1024 || pairs > KMEANS_PASS_PAIRS || pairs < 1 || grid < 1)
         return hipErrorInvalidValue;
-    if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)kmeans_partial_kernel<T>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(kmeans_partial_kernel<T>, dim3(nblocks), dim3(1024), lds, s, X, n, d, labels,
-                       c0, kc, escale, ppb, part, cnt, winv);
-    return hipGetLastError();
+    const int per = (int)((pairs + 1023) / 1024);
+    if (per <= 1)
+        return partial_launch<T, 1>(X, n, d, labels, c0, kc, l0, dl, escale, ppb, grid, wpart,
+                                    wcnt, s, winv, lds);
+    if (per <= 2)
+        return partial_launch<T, 2>(X, n, d, labels, c0, kc, l0, dl, escale, ppb, grid, wpart,
+                                    wcnt, s, winv, lds);
+    if (per <= 4)
+        return partial_launch<T, 4>(X, n, d, labels, c0, kc, l0, dl, escale, ppb, grid, wpart,
+                                    wcnt, s, winv, lds);
+    if (per <= 8)
+        return partial_launch<T, 8>(X, n, d, labels, c0, kc, l0, dl, escale, ppb, grid, wpart,
+                                    wcnt, s, winv, lds);
+    return partial_launch<T, 16>(X, n, d, labels, c0, kc, l0, dl, escale, ppb, grid, wpart,
+                                 wcnt, s, winv, lds);
 }
 
 size_t kmeans_partial_lds(int kc, int ppb) {
     return (size_t)(2 * kc + 1 + ppb) * sizeof(int);
 }
 
-hipError_t launch_kmeans_reduce(long long *part, int *cnt, int c0, int kc, int d,
-                                const int *escale, double *C, int *counts, hipStream_t s,
-                                int sph) {
-    hipLaunchKernelGGL(kmeans_reduce_kernel, dim3(kc), dim3(d < 256 ? (d + 63) / 64 * 64 : 256),
-                       0, s, part, cnt, c0, d, escale, C, counts, sph);
+hipError_t launch_kmeans_reduce(const long long *wpart, const int *wcnt, int nb, int c0, int kc,
+                                int l0, int dl, int d, const int *escale, double *C,
+                                int *counts, hipStream_t s, int sph) {
+    hipLaunchKernelGGL(kmeans_reduce_kernel, dim3(kc), dim3(256), 0, s, wpart, wcnt, nb, c0, kc,
+                       l0, dl, d, escale, C, counts, sph);
     return hipGetLastError();
 }
 
@@ -969,8 +1026,9 @@ hipError_t launch_row_inv_norms(const T *X, int n, int d, double *inv, hipStream
     template hipError_t launch_kmeans_assign<T>(const T *, int, int, const double *, int, int *, \
                                                 const int *, int *, double *, hipStream_t, int); \
     template hipError_t launch_kmeans_partial_sums<T>(const T *, int, int, const int *, int, int, \
-                                                      const int *, int, int, long long *, int *, \
-                                                      hipStream_t, const double *);              \
+                                                      int, int, const int *, int, int,          \
+                                                      long long *, int *, hipStream_t,          \
+                                                      const double *);                           \
     template hipError_t launch_convert<T>(const T *, int64_t, double *, hipStream_t);           \
     template hipError_t launch_convert_back<T>(const double *, int64_t, T *, hipStream_t);      \
     template hipError_t launch_gather_rows<T>(const T *, int, const int *, int, double *,       \

@@ -10,6 +10,7 @@
 // arithmetic is listed for an exact double-precision re-check, so the labels equal the exact
 // argmin (ties to the lowest centre index).
 #include "launch.hpp"
+#include <algorithm>
 #include <climits>
 #include <hip/hip_runtime.h>
 
@@ -41,115 +42,120 @@ __global__ __launch_bounds__(256) void kmeans_assign_mfma_kernel(
         Hs[t] = t < k ? chalf[t] : -__builtin_huge_valf();
     __syncthreads();
 
-    const int s0 = blockIdx.x * SB + w * PT * 16;
-    float b[PT][KCH];
-    float xn[PT];
-#pragma unroll
-    for (int pt = 0; pt < PT; pt++) {
-        const int i = s0 + pt * 16 + r;
-        float s = 0.f;
-#pragma unroll
-        for (int f = 0; f < KCH; f += 4) {
-            v4f v = {0.f, 0.f, 0.f, 0.f};
-            if (i < n)
-                v = *reinterpret_cast<const v4f *>(X + (int64_t)i * Kp + g * KCH + f);
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                b[pt][f + e] = v[e];
-                s = fmaf(v[e], v[e], s);
-            }
-        }
-        xn[pt] = s;
-    }
-    float best1[PT], best2[PT];
-    int bidx[PT];
-#pragma unroll
-    for (int pt = 0; pt < PT; pt++) {
-        best1[pt] = -__builtin_huge_valf();
-        best2[pt] = -__builtin_huge_valf();
-        bidx[pt] = INT_MAX;
-    }
-    for (int c0 = 0; c0 < kpad; c0 += 16) {
-        float a[KCH];
-        const float *src = Cs + (c0 + r) * Kp + g * KCH;
-#pragma unroll
-        for (int f = 0; f < KCH; f += 4) {
-            const v4f v = *reinterpret_cast<const v4f *>(src + f);
-            a[f] = v[0];
-            a[f + 1] = v[1];
-            a[f + 2] = v[2];
-            a[f + 3] = v[3];
-        }
-        v4f h;
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            h[i] = Hs[c0 + 4 * g + i];
+    // The centres are staged once per workgroup, which then takes every gridDim.x-th tile of
+    // SB samples.
+    const int ntiles = (n + SB - 1) / SB;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int s0 = tile * SB + w * PT * 16;
+        float b[PT][KCH];
+        float xn[PT];
 #pragma unroll
         for (int pt = 0; pt < PT; pt++) {
-            v4f acc = h;
+            const int i = s0 + pt * 16 + r;
+            float s = 0.f;
 #pragma unroll
-            for (int f = 0; f < KCH; f++)
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[f], b[pt][f], acc, 0, 0, 0);
+            for (int f = 0; f < KCH; f += 4) {
+                v4f v = {0.f, 0.f, 0.f, 0.f};
+                if (i < n)
+                    v = *reinterpret_cast<const v4f *>(X + (int64_t)i * Kp + g * KCH + f);
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float v = acc[i];
-                const int c = c0 + 4 * g + i;
-                if (v > best1[pt]) {
-                    best2[pt] = best1[pt];
-                    best1[pt] = v;
-                    bidx[pt] = c;
-                } else if (v > best2[pt]) {
-                    best2[pt] = v;
+                for (int e = 0; e < 4; e++) {
+                    b[pt][f + e] = v[e];
+                    s = fmaf(v[e], v[e], s);
+                }
+            }
+            xn[pt] = s;
+        }
+        float best1[PT], best2[PT];
+        int bidx[PT];
+#pragma unroll
+        for (int pt = 0; pt < PT; pt++) {
+            best1[pt] = -__builtin_huge_valf();
+            best2[pt] = -__builtin_huge_valf();
+            bidx[pt] = INT_MAX;
+        }
+        for (int c0 = 0; c0 < kpad; c0 += 16) {
+            float a[KCH];
+            const float *src = Cs + (c0 + r) * Kp + g * KCH;
+#pragma unroll
+            for (int f = 0; f < KCH; f += 4) {
+                const v4f v = *reinterpret_cast<const v4f *>(src + f);
+                a[f] = v[0];
+                a[f + 1] = v[1];
+                a[f + 2] = v[2];
+                a[f + 3] = v[3];
+            }
+            v4f h;
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                h[i] = Hs[c0 + 4 * g + i];
+#pragma unroll
+            for (int pt = 0; pt < PT; pt++) {
+                v4f acc = h;
+#pragma unroll
+                for (int f = 0; f < KCH; f++)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[f], b[pt][f], acc, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float v = acc[i];
+                    const int c = c0 + 4 * g + i;
+                    if (v > best1[pt]) {
+                        best2[pt] = best1[pt];
+                        best1[pt] = v;
+                        bidx[pt] = c;
+                    } else if (v > best2[pt]) {
+                        best2[pt] = v;
+                    }
                 }
             }
         }
-    }
-    // Combine the four lane groups that hold the same sample (lanes r, r+16, r+32, r+48).
+        // Combine the four lane groups that hold the same sample (lanes r, r+16, r+32, r+48).
 #pragma unroll
-    for (int pt = 0; pt < PT; pt++) {
-        float xs = xn[pt];
+        for (int pt = 0; pt < PT; pt++) {
+            float xs = xn[pt];
 #pragma unroll
-        for (int off = 16; off <= 32; off <<= 1) {
-            const float o1 = __shfl_xor(best1[pt], off, 64);
-            const float o2 = __shfl_xor(best2[pt], off, 64);
-            const int oi = __shfl_xor(bidx[pt], off, 64);
-            xs += __shfl_xor(xs, off, 64);
-            const bool other = (o1 > best1[pt]) || (o1 == best1[pt] && oi < bidx[pt]);
-            const float lo = other ? best1[pt] : o1; // the loser's best
-            if (other) {
-                best1[pt] = o1;
-                bidx[pt] = oi;
+            for (int off = 16; off <= 32; off <<= 1) {
+                const float o1 = __shfl_xor(best1[pt], off, 64);
+                const float o2 = __shfl_xor(best2[pt], off, 64);
+                const int oi = __shfl_xor(bidx[pt], off, 64);
+                xs += __shfl_xor(xs, off, 64);
+                const bool other = (o1 > best1[pt]) || (o1 == best1[pt] && oi < bidx[pt]);
+                const float lo = other ? best1[pt] : o1; // the loser's best
+                if (other) {
+                    best1[pt] = o1;
+                    bidx[pt] = oi;
+                }
+                best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
             }
-            best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
+            const int i = s0 + pt * 16 + r;
+            const bool valid = g == 0 && i < n;
+            bool listed = false, moved = false;
+            if (valid) {
+                // |acc - exact| <= gamma_{d+2} (|x| cmax + cmax^2/2) + unit (|x| cmax + cmax^2/2)
+                const double xnorm = sqrt((double)xs);
+                const double gam = (Kp + 2) * unit / (1.0 - (Kp + 2) * unit);
+                const double e = (gam + 2.0 * unit) * (xnorm * cmax + 0.5 * cmax * cmax);
+                const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
+                const int lab = bidx[pt] < k ? bidx[pt] : 0;
+                labels[i] = lab;
+                listed = near_tie || !(best1[pt] - best1[pt] == 0.0f);
+                moved = !listed && prev && prev[i] != lab;
+            }
+            // One atomic per wave for the re-check list and one for the change count.
+            const unsigned long long lm = __ballot(listed);
+            const unsigned long long cm = __ballot(moved);
+            if (lm) {
+                const int first = __ffsll((long long)lm) - 1;
+                int base = 0;
+                if (lane == first)
+                    base = atomicAdd(list_count, __popcll(lm));
+                base = __builtin_amdgcn_readlane(base, first);
+                if (listed)
+                    list[base + __popcll(lm & ((1ull << lane) - 1))] = i;
+            }
+            if (cm && lane == 0)
+                atomicAdd(changed, __popcll(cm));
         }
-        const int i = s0 + pt * 16 + r;
-        const bool valid = g == 0 && i < n;
-        bool listed = false, moved = false;
-        if (valid) {
-            // |acc - exact| <= gamma_{d+2} (|x| cmax + cmax^2/2) + unit (|x| cmax + cmax^2/2)
-            const double xnorm = sqrt((double)xs);
-            const double gam = (Kp + 2) * unit / (1.0 - (Kp + 2) * unit);
-            const double e = (gam + 2.0 * unit) * (xnorm * cmax + 0.5 * cmax * cmax);
-            const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
-            const int lab = bidx[pt] < k ? bidx[pt] : 0;
-            labels[i] = lab;
-            listed = near_tie || !(best1[pt] - best1[pt] == 0.0f);
-            moved = !listed && prev && prev[i] != lab;
-        }
-        // One atomic per wave for the re-check list and one for the change count.
-        const unsigned long long lm = __ballot(listed);
-        const unsigned long long cm = __ballot(moved);
-        if (lm) {
-            const int first = __ffsll((long long)lm) - 1;
-            int base = 0;
-            if (lane == first)
-                base = atomicAdd(list_count, __popcll(lm));
-            base = __builtin_amdgcn_readlane(base, first);
-            if (listed)
-                list[base + __popcll(lm & ((1ull << lane) - 1))] = i;
-        }
-        if (cm && lane == 0)
-            atomicAdd(changed, __popcll(cm));
     }
 }
 
@@ -184,115 +190,120 @@ __global__ __launch_bounds__(256) void kmeans_assign_bf16_kernel(
         Hs[t] = t < k ? chalf[t] : -__builtin_huge_valf();
     __syncthreads();
 
-    const int s0 = blockIdx.x * SB + w * PT * 16;
-    v8bf b[PT][NF];
-    float xn[PT];
-#pragma unroll
-    for (int pt = 0; pt < PT; pt++) {
-        const int i = s0 + pt * 16 + r;
-        float s = 0.f;
-#pragma unroll
-        for (int f = 0; f < NF; f++) {
-            v8bf v = {};
-            if (i < n)
-                v = *reinterpret_cast<const v8bf *>(X + (int64_t)i * Kp + g * KCH + f * 8);
-            b[pt][f] = v;
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const float x = (float)v[e];
-                s = fmaf(x, x, s);
-            }
-        }
-        xn[pt] = s;
-    }
-    float best1[PT], best2[PT];
-    int bidx[PT];
-#pragma unroll
-    for (int pt = 0; pt < PT; pt++) {
-        best1[pt] = -__builtin_huge_valf();
-        best2[pt] = -__builtin_huge_valf();
-        bidx[pt] = INT_MAX;
-    }
-    for (int c0 = 0; c0 < kpad; c0 += 16) {
-        v8bf a[NF];
-        const __bf16 *src = Cs + (c0 + r) * Kp + g * KCH;
-#pragma unroll
-        for (int f = 0; f < NF; f++)
-            a[f] = *reinterpret_cast<const v8bf *>(src + f * 8);
-        v4f h;
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            h[i] = Hs[c0 + 4 * g + i];
+    // The centres are staged once per workgroup, which then takes every gridDim.x-th tile of
+    // SB samples.
+    const int ntiles = (n + SB - 1) / SB;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int s0 = tile * SB + w * PT * 16;
+        v8bf b[PT][NF];
+        float xn[PT];
 #pragma unroll
         for (int pt = 0; pt < PT; pt++) {
-            v4f acc = h;
+            const int i = s0 + pt * 16 + r;
+            float s = 0.f;
+#pragma unroll
+            for (int f = 0; f < NF; f++) {
+                v8bf v = {};
+                if (i < n)
+                    v = *reinterpret_cast<const v8bf *>(X + (int64_t)i * Kp + g * KCH + f * 8);
+                b[pt][f] = v;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const float x = (float)v[e];
+                    s = fmaf(x, x, s);
+                }
+            }
+            xn[pt] = s;
+        }
+        float best1[PT], best2[PT];
+        int bidx[PT];
+#pragma unroll
+        for (int pt = 0; pt < PT; pt++) {
+            best1[pt] = -__builtin_huge_valf();
+            best2[pt] = -__builtin_huge_valf();
+            bidx[pt] = INT_MAX;
+        }
+        for (int c0 = 0; c0 < kpad; c0 += 16) {
+            v8bf a[NF];
+            const __bf16 *src = Cs + (c0 + r) * Kp + g * KCH;
 #pragma unroll
             for (int f = 0; f < NF; f++)
-                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[f], b[pt][f], acc, 0, 0, 0);
+                a[f] = *reinterpret_cast<const v8bf *>(src + f * 8);
+            v4f h;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float v = acc[i];
-                const int c = c0 + 4 * g + i;
-                if (v > best1[pt]) {
-                    best2[pt] = best1[pt];
-                    best1[pt] = v;
-                    bidx[pt] = c;
-                } else if (v > best2[pt]) {
-                    best2[pt] = v;
+            for (int i = 0; i < 4; i++)
+                h[i] = Hs[c0 + 4 * g + i];
+#pragma unroll
+            for (int pt = 0; pt < PT; pt++) {
+                v4f acc = h;
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[f], b[pt][f], acc, 0, 0, 0);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float v = acc[i];
+                    const int c = c0 + 4 * g + i;
+                    if (v > best1[pt]) {
+                        best2[pt] = best1[pt];
+                        best1[pt] = v;
+                        bidx[pt] = c;
+                    } else if (v > best2[pt]) {
+                        best2[pt] = v;
+                    }
                 }
             }
         }
-    }
-    const double u = 1.0 / 256.0, u32 = 1.0 / 16777216.0;
+        const double u = 1.0 / 256.0, u32 = 1.0 / 16777216.0;
 #pragma unroll
-    for (int pt = 0; pt < PT; pt++) {
-        float xs = xn[pt];
+        for (int pt = 0; pt < PT; pt++) {
+            float xs = xn[pt];
 #pragma unroll
-        for (int off = 16; off <= 32; off <<= 1) {
-            const float o1 = __shfl_xor(best1[pt], off, 64);
-            const float o2 = __shfl_xor(best2[pt], off, 64);
-            const int oi = __shfl_xor(bidx[pt], off, 64);
-            xs += __shfl_xor(xs, off, 64);
-            const bool other = (o1 > best1[pt]) || (o1 == best1[pt] && oi < bidx[pt]);
-            const float lo = other ? best1[pt] : o1;
-            if (other) {
-                best1[pt] = o1;
-                bidx[pt] = oi;
+            for (int off = 16; off <= 32; off <<= 1) {
+                const float o1 = __shfl_xor(best1[pt], off, 64);
+                const float o2 = __shfl_xor(best2[pt], off, 64);
+                const int oi = __shfl_xor(bidx[pt], off, 64);
+                xs += __shfl_xor(xs, off, 64);
+                const bool other = (o1 > best1[pt]) || (o1 == best1[pt] && oi < bidx[pt]);
+                const float lo = other ? best1[pt] : o1;
+                if (other) {
+                    best1[pt] = o1;
+                    bidx[pt] = oi;
+                }
+                best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
             }
-            best2[pt] = fmaxf(fmaxf(best2[pt], o2), lo);
+            const int i = s0 + pt * 16 + r;
+            const bool valid = g == 0 && i < n;
+            bool listed = false, moved = false;
+            if (valid) {
+                // |x| of the data from the norm of its bf16 copy (relative error <= u), with a
+                // margin for the fp32 sum. Score error: operand rounding (2u + u^2)|x| cmax, the
+                // fp32 accumulation gamma_{Kp+2} over |x~| |c~| + cmax^2/2, the rounded -|c|^2/2.
+                const double xnorm = sqrt((double)xs) / (1.0 - u) * (1.0 + 1e-6);
+                const double gam = (Kp + 2) * u32 / (1.0 - (Kp + 2) * u32);
+                const double S = xnorm * cmax;
+                const double e = (2.0 * u + u * u) * S + gam * (S * (1.0 + u) * (1.0 + u) + 0.5 * cmax * cmax) +
+                                 u32 * 0.5 * cmax * cmax;
+                const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
+                const int lab = bidx[pt] < k ? bidx[pt] : 0;
+                labels[i] = lab;
+                listed = near_tie || !(best1[pt] - best1[pt] == 0.0f);
+                moved = !listed && prev && prev[i] != lab;
+            }
+            // One atomic per wave for the re-check list and one for the change count.
+            const unsigned long long lm = __ballot(listed);
+            const unsigned long long cm = __ballot(moved);
+            if (lm) {
+                const int first = __ffsll((long long)lm) - 1;
+                int base = 0;
+                if (lane == first)
+                    base = atomicAdd(list_count, __popcll(lm));
+                base = __builtin_amdgcn_readlane(base, first);
+                if (listed)
+                    list[base + __popcll(lm & ((1ull << lane) - 1))] = i;
+            }
+            if (cm && lane == 0)
+                atomicAdd(changed, __popcll(cm));
         }
-        const int i = s0 + pt * 16 + r;
-        const bool valid = g == 0 && i < n;
-        bool listed = false, moved = false;
-        if (valid) {
-            // |x| of the data from the norm of its bf16 copy (relative error <= u), with a
-            // margin for the fp32 sum. Score error: operand rounding (2u + u^2)|x| cmax, the
-            // fp32 accumulation gamma_{Kp+2} over |x~| |c~| + cmax^2/2, the rounded -|c|^2/2.
-            const double xnorm = sqrt((double)xs) / (1.0 - u) * (1.0 + 1e-6);
-            const double gam = (Kp + 2) * u32 / (1.0 - (Kp + 2) * u32);
-            const double S = xnorm * cmax;
-            const double e = (2.0 * u + u * u) * S + gam * (S * (1.0 + u) * (1.0 + u) + 0.5 * cmax * cmax) +
-                             u32 * 0.5 * cmax * cmax;
-            const bool near_tie = (double)best1[pt] - (double)best2[pt] <= 2.0 * e;
-            const int lab = bidx[pt] < k ? bidx[pt] : 0;
-            labels[i] = lab;
-            listed = near_tie || !(best1[pt] - best1[pt] == 0.0f);
-            moved = !listed && prev && prev[i] != lab;
-        }
-        // One atomic per wave for the re-check list and one for the change count.
-        const unsigned long long lm = __ballot(listed);
-        const unsigned long long cm = __ballot(moved);
-        if (lm) {
-            const int first = __ffsll((long long)lm) - 1;
-            int base = 0;
-            if (lane == first)
-                base = atomicAdd(list_count, __popcll(lm));
-            base = __builtin_amdgcn_readlane(base, first);
-            if (listed)
-                list[base + __popcll(lm & ((1ull << lane) - 1))] = i;
-        }
-        if (cm && lane == 0)
-            atomicAdd(changed, __popcll(cm));
     }
 }
 
@@ -395,6 +406,20 @@ __global__ void to_bf16_rows_kernel(const float *__restrict__ X, int n, int d, i
     Y[t] = l < d ? (__bf16)X[i * ldx + l] : (__bf16)0.0f;
 }
 
+// Workgroups of the assignments: one per tile of SB samples (wg_per_cu = 0), or at most
+// wg_per_cu per CU, each taking every gridDim.x-th tile.
+unsigned assign_grid(int n, int wg_per_cu) {
+    const int ntiles = (n + SB - 1) / SB;
+    if (wg_per_cu <= 0)
+        return (unsigned)std::max(1, ntiles);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus < 1)
+        cus = 256;
+    return (unsigned)std::max(1, std::min(ntiles, wg_per_cu * cus));
+}
+
 } // namespace
 
 hipError_t launch_centre_prep_bf16(const double *C, int k, int d, int Kp, void *C16,
@@ -423,11 +448,11 @@ size_t kmeans_assign_bf16_lds(int k, int Kp) {
 hipError_t launch_kmeans_assign_bf16(const void *X, int n, int Kp, const void *C16,
                                      const float *chalf, int k, double cmax, int *labels,
                                      const int *prev, int *changed, int *list_count, int *list,
-                                     hipStream_t s) {
+                                     hipStream_t s, int wg_per_cu) {
     const size_t lds = kmeans_assign_bf16_lds(k, Kp);
     if (lds > 160 * 1024 || Kp % 32 != 0)
         return hipErrorInvalidValue;
-    dim3 grid((n + SB - 1) / SB), block(256);
+    dim3 grid(assign_grid(n, wg_per_cu)), block(256);
     const __bf16 *x = static_cast<const __bf16 *>(X);
     const __bf16 *c = static_cast<const __bf16 *>(C16);
 #define DA_ASSIGN(KCHV)                                                                          \
@@ -466,11 +491,11 @@ size_t kmeans_assign_mfma_lds(int k, int Kp) {
 hipError_t launch_kmeans_assign_mfma(const float *X, int n, int Kp, const float *C32,
                                      const float *chalf, int k, double cmax, int *labels,
                                      const int *prev, int *changed, int *list_count, int *list,
-                                     hipStream_t s) {
+                                     hipStream_t s, int wg_per_cu) {
     const size_t lds = kmeans_assign_mfma_lds(k, Kp);
     if (lds > 160 * 1024)
         return hipErrorInvalidValue;
-    dim3 grid((n + SB - 1) / SB), block(256);
+    dim3 grid(assign_grid(n, wg_per_cu)), block(256);
     const double unit = 1.0 / 16777216.0;
 #define DA_ASSIGN(KCHV)                                                                          \
     {                                                                                            \

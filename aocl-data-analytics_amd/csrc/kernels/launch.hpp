@@ -86,25 +86,26 @@ hipError_t launch_kmeans_assign(const T *X, int n, int d, const double *C, int k
                                 const int *prev, int *changed, double *mind, hipStream_t s,
                                 int sph = 0);
 
-// Fixed-point sums of the points of each cluster in [c0, c0+kc): each workgroup b sorts the
-// points [b*ppb, (b+1)*ppb) by label in LDS, sums each cluster's values in registers and
-// adds its totals into part[(c*d + l)*2 + w] (w = 0:
-// the value scaled by 2^e[l] and rounded to int64, w = 1: the remainder scaled by
-// 2^(e[l]+lshift)) and cnt[c] with 64-bit integer atomics. Integer addition is exact and
-// associative, so the totals do not depend on the order of the workgroups. part (kc*d*2
-// words) and cnt (kc) must be zero on entry; launch_kmeans_reduce leaves them zero. With
-// winv (n doubles) sample i contributes X_i * winv[i].
+// Fixed-point centroid sums, one pass over the clusters [c0, c0+kc) and the features
+// [l0, l0+dl), kc * dl <= KMEANS_PASS_PAIRS: `grid` workgroups take the chunks of ppb points in
+// turn, sort each chunk by label in LDS and sum every (cluster, feature) pair in registers,
+// then write their totals to wpart[(b * kc * dl + p) * 2 + w] (w = 0: the value scaled by
+// 2^e[l] and rounded to int64, w = 1: the remainder scaled by 2^(e[l]+lshift)) and their
+// cluster sizes to wcnt[b * kc + c]. wpart needs grid * kc * dl * 2 words, wcnt grid * kc
+// integers; both are overwritten, never accumulated into. With winv (n doubles) sample i
+// contributes X_i * winv[i].
+constexpr int KMEANS_PASS_PAIRS = 16384;
 template <typename T>
 hipError_t launch_kmeans_partial_sums(const T *X, int n, int d, const int *labels, int c0,
-                                      int kc, const int *escale, int ppb, int nblocks,
-                                      long long *part, int *cnt, hipStream_t s,
+                                      int kc, int l0, int dl, const int *escale, int ppb,
+                                      int grid, long long *wpart, int *wcnt, hipStream_t s,
                                       const double *winv = nullptr);
-// C[(c0+c)*d + l] = sum / max(count, 1) (sph: the sum scaled to unit length, a zero sum left
-// at zero), counts[c0+c] = count from the totals of launch_kmeans_partial_sums, which it then
-// clears for the next pass.
-hipError_t launch_kmeans_reduce(long long *part, int *cnt, int c0, int kc, int d,
-                                const int *escale, double *C, int *counts, hipStream_t s,
-                                int sph = 0);
+// C[(c0+c)*d + l] for l in [l0, l0+dl) = the sum of the nb workgroup partials of a pass / the
+// cluster size (sph: the sum, to be normalised by the caller once every feature is summed),
+// counts[c0+c] = the cluster size.
+hipError_t launch_kmeans_reduce(const long long *wpart, const int *wcnt, int nb, int c0, int kc,
+                                int l0, int dl, int d, const int *escale, double *C,
+                                int *counts, hipStream_t s, int sph = 0);
 // Fills n 64-bit words with zero (a kernel, so that it can be captured in a graph).
 hipError_t launch_zero_words(long long *p, int64_t n, hipStream_t s);
 // LDS bytes of one partial-sum workgroup for kc clusters and ppb points.
@@ -367,17 +368,19 @@ size_t kmeans_assign_mfma_lds(int k, int Kp);
 // X: n x Kp zero-padded fp32 rows, Kp in {16, 32, 48, 64, 96, 128}. Writes labels; samples
 // whose best two centres are within the fp32 error bound are appended to list (count in
 // *list_count, zeroed before) for launch_kmeans_recheck; the others update *changed.
+// wg_per_cu: 0 launches one workgroup per 256 samples; n > 0 at most n workgroups per CU,
+// each staging the centres once and taking every gridDim.x-th block of samples.
 hipError_t launch_kmeans_assign_mfma(const float *X, int n, int Kp, const float *C32,
                                      const float *chalf, int k, double cmax, int *labels,
                                      const int *prev, int *changed, int *list_count, int *list,
-                                     hipStream_t s);
+                                     hipStream_t s, int wg_per_cu = 0);
 // bf16 cross-term form of the same assignment: X and C16 are bf16 rows padded to Kp (a
 // multiple of 32, at most 128) features; C16 has (k+15)/16*16 rows.
 size_t kmeans_assign_bf16_lds(int k, int Kp);
 hipError_t launch_kmeans_assign_bf16(const void *X, int n, int Kp, const void *C16,
                                      const float *chalf, int k, double cmax, int *labels,
                                      const int *prev, int *changed, int *list_count, int *list,
-                                     hipStream_t s);
+                                     hipStream_t s, int wg_per_cu = 0);
 hipError_t launch_centre_prep_bf16(const double *C, int k, int d, int Kp, void *C16,
                                    float *chalf, double *cmax2, hipStream_t s, int sph = 0);
 hipError_t launch_rows_to_bf16(const float *X, int n, int d, int ldx, void *Y, int Kp,

@@ -71,6 +71,11 @@ template <typename T> KMeansHandle<T>::KMeansHandle(ErrorStack &e, GpuContext &g
                     "Operand precision of the MFMA cross term in the assignment step (fp32 data): "
                     "working or bf16.",
                     {{"working", 0}, {"bf16", 1}}, "working");
+    // MI355X extension: workgroups per CU of the MFMA assignment (0: one per 256 samples).
+    opts.add_int("assignment workgroups per cu",
+                 "Workgroups per CU of the MFMA assignment step, each taking every n-th block "
+                 "of 256 samples (0: one workgroup per block).",
+                 0, LowerBound::greater_equal, 64, UpperBound::less_equal, 0);
     opts.add_real("convergence tolerance", "Convergence tolerance.", 0.0,
                   LowerBound::greater_equal, 0.0, UpperBound::p_inf, (double)(T)1.0e-4);
     opts.add_real("low precision convergence tolerance",
@@ -308,7 +313,7 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
             DA_HIP(err, launch_kmeans_assign_bf16(xb_.ptr(), n, kp16_, c16_.ptr(),
                                                   chalf_.as<float>(), k, std::sqrt(hc), lab_new,
                                                   lab_old, dchanged, dlist_count, list_.as<int>(),
-                                                  st));
+                                                  st, (int)opts.ival("assignment workgroups per cu")));
         }
         {
             KernelScope ks(*gpu, "kmeans_recheck");
@@ -331,7 +336,8 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
             KernelScope ks(*gpu, "kmeans_assign");
             DA_HIP(err, launch_kmeans_assign_mfma(xpad_.as<float>(), n, kp_, c32_.as<float>(),
                                                   chalf_.as<float>(), k, std::sqrt(hc), lab_new,
-                                                  lab_old, dchanged, dlist_count, list_.as<int>(), st));
+                                                  lab_old, dchanged, dlist_count, list_.as<int>(), st,
+                                                  (int)opts.ival("assignment workgroups per cu")));
         }
         {
             KernelScope ks(*gpu, "kmeans_recheck");
@@ -358,19 +364,14 @@ template <typename U>
 da_status KMeansHandle<T>::update_centres(const U *dX, const int *lab_new, const double *Cin,
                                           double *Cout) {
     hipStream_t st = gpu->stream();
-    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    const int d = n_features_, k = n_clusters_;
     KernelScope ks(*gpu, "kmeans_update");
-    for (int c0 = 0; c0 < k; c0 += kc_) {
-        const int kcc = std::min<int>(kc_, k - c0);
-        // Spherical k-means: sums of x (or of x/|x|), normalised (kmeans_lloyd.hpp:279-300,
-        // 363-405).
-        const int *esc = winv() ? escale_unit_.as<int>() : escale_.as<int>();
-        DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n, d, lab_new, c0, kcc, esc, ppb_, nblocks_,
-                                                  part_.as<long long>(), pcnt_.as<int>(), st,
-                                                  winv()));
-        DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), c0, kcc, d, esc,
-                                         Cout, counts_.as<int>(), st, sph_));
-    }
+    // Spherical k-means: sums of x (or of x/|x|), normalised (kmeans_lloyd.hpp:279-300,
+    // 363-405).
+    da_status s = centre_sums<U>(dX, lab_new, Cout, winv() ? escale_unit_.as<int>() : escale_.as<int>(),
+                                 winv(), sph_);
+    if (s != da_status_success)
+        return s;
     // The low-precision phase stores its centres in the lower precision (kmeans<float> or
     // kmeans<_Float16> in kmeans.cpp:1824-1876); the move is measured after the rounding.
     if (lp_mode_ != 0)
@@ -423,19 +424,35 @@ da_status KMeansHandle<T>::elkan_step(const U *dX, const double *Cin, double *Co
     return read_step(changed, normc2, shift2);
 }
 
+// Centres from labels: the exact fixed-point sums in passes of at most KMEANS_PASS_PAIRS
+// (cluster, feature) pairs, each summed by nblocks_ workgroups and reduced per cluster;
+// divided by the cluster sizes, or (sph) normalised once every feature is summed.
+template <typename T>
+template <typename U>
+da_status KMeansHandle<T>::centre_sums(const U *dX, const int *labels, double *C, const int *esc,
+                                       const double *w, int sph) {
+    hipStream_t st = gpu->stream();
+    const int n = n_samples_, d = n_features_, k = n_clusters_;
+    for (int c0 = 0; c0 < k; c0 += kc_) {
+        const int kc = std::min<int>(kc_, k - c0);
+        for (int l0 = 0; l0 < d; l0 += dl_) {
+            const int dl = std::min<int>(dl_, d - l0);
+            DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n, d, labels, c0, kc, l0, dl, esc, ppb_,
+                                                      nblocks_, part_.as<long long>(),
+                                                      pcnt_.as<int>(), st, w));
+            DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), nblocks_, c0,
+                                             kc, l0, dl, d, esc, C, counts_.as<int>(), st, sph));
+        }
+    }
+    if (sph)
+        DA_HIP(err, launch_normalize_rows(C, k, d, st));
+    return da_status_success;
+}
+
 template <typename T>
 template <typename U>
 da_status KMeansHandle<T>::centres_from_labels(const U *dX, const int *labels, double *C) {
-    hipStream_t st = gpu->stream();
-    for (int c0 = 0; c0 < n_clusters_; c0 += kc_) {
-        const int kcc = std::min<int>(kc_, n_clusters_ - c0);
-        DA_HIP(err, launch_kmeans_partial_sums<U>(dX, n_samples_, n_features_, labels, c0, kcc,
-                                                  escale_.as<int>(), ppb_, nblocks_,
-                                                  part_.as<long long>(), pcnt_.as<int>(), st));
-        DA_HIP(err, launch_kmeans_reduce(part_.as<long long>(), pcnt_.as<int>(), c0, kcc,
-                                         n_features_, escale_.as<int>(), C, counts_.as<int>(), st));
-    }
-    return da_status_success;
+    return centre_sums<U>(dX, labels, C, escale_.as<int>(), nullptr, 0);
 }
 
 // Empty-cluster handling after an iteration (kmeans.cpp:1127-1330): "error" flags the run,
@@ -902,33 +919,30 @@ template <typename T> da_status KMeansHandle<T>::compute() {
     DA_HIP(err, idx_.reserve((size_t)std::max(k, 64) * sizeof(int)));
     DA_HIP(err, w_.reserve((size_t)n * sizeof(double)));
     DA_HIP(err, cp_.reserve((size_t)std::max(n, 256 * 64) * sizeof(double)));
-    // Clusters per partial-sum pass: as many as fit the 160 KB of LDS of a CU (all of them
-    // for k = 256 at d = 32).
-    // Partial-sum workgroups sort up to 8,192 points by label in LDS: at least one round of
-    // two workgroups per CU, more for large n. Clusters per pass: all of them up to 12,288
-    // (the label counters share the 160 KB of LDS with the point list).
+    // Centroid sums: chunks of up to 8,192 points (sorted by label in LDS), about two per
+    // workgroup, one workgroup per CU; passes of at most KMEANS_PASS_PAIRS (cluster, feature)
+    // pairs summed in registers (all of them for k = 256 at d = 32), at most 12,288 clusters
+    // per pass (the label counters share the 160 KB of LDS with the point list).
     {
         int dev = 0, cus = 256;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
             cus < 1)
             cus = 256;
-        nblocks_ = std::max<da_int>(1, std::min<da_int>(2 * cus, (n + 255) / 256));
-        ppb_ = (n + nblocks_ - 1) / nblocks_;
-        if (ppb_ > 8192)
-            ppb_ = 8192;
-        nblocks_ = (n + ppb_ - 1) / ppb_;
+        ppb_ = std::max<da_int>(256, std::min<da_int>(8192, (n + 2 * cus - 1) / (2 * cus)));
+        const da_int nchunks = (n + ppb_ - 1) / ppb_;
+        nblocks_ = std::max<da_int>(1, std::min<da_int>(nchunks, cus));
     }
-    kc_ = std::max<da_int>(1, std::min<da_int>(k, 12288));
-    DA_HIP(err, part_.reserve((size_t)kc_ * d * 2 * sizeof(long long)));
-    DA_HIP(err, launch_zero_words(part_.as<long long>(), (int64_t)kc_ * d * 2, gpu->stream()));
+    dl_ = std::min<da_int>(d, KMEANS_PASS_PAIRS);
+    kc_ = std::max<da_int>(1, std::min<da_int>({(da_int)k, (da_int)12288,
+                                                (da_int)(KMEANS_PASS_PAIRS / dl_)}));
+    DA_HIP(err, part_.reserve((size_t)nblocks_ * kc_ * dl_ * 2 * sizeof(long long)));
     if (mfma_ok_) {
         DA_HIP(err, c32_.reserve((size_t)(k + 16) * kp_ * sizeof(float)));
         DA_HIP(err, chalf_.reserve((size_t)(k + 16) * sizeof(float)));
         DA_HIP(err, list_.reserve((size_t)n * sizeof(int)));
     }
-    DA_HIP(err, pcnt_.reserve((size_t)(kc_ + 1) / 2 * sizeof(long long)));
-    DA_HIP(err, launch_zero_words(pcnt_.as<long long>(), (kc_ + 1) / 2, gpu->stream()));
+    DA_HIP(err, pcnt_.reserve((size_t)nblocks_ * kc_ * sizeof(int)));
     Ccur_ = Ca_.as<double>();
     Cprev_ = Cb_.as<double>();
     lab_cur_ = lab_a_.as<int>();

@@ -60,6 +60,9 @@ template <typename T> class KMeansHandle : public AlgHandle {
     da_status read_step(int &changed, double &normc2, double &shift2);
     template <typename U> da_status centres_from_labels(const U *dX, const int *labels, double *C);
     template <typename U>
+    da_status centre_sums(const U *dX, const int *labels, double *C, const int *esc,
+                          const double *w, int sph);
+    template <typename U>
     da_status handle_empty_clusters(const U *dX, double *C, int *labels, bool &split,
                                     bool &empty_error);
     template <typename U> da_status prepare_scales_for(const U *dX);
@@ -88,7 +91,7 @@ template <typename T> class KMeansHandle : public AlgHandle {
     // sums of unit rows; qinv_ the inverse norms of transformed samples.
     bool sph_ = false, sph_norm_ = false;
     DeviceBuffer inv_, escale_unit_, qinv_;
-    da_int ppb_ = 0, nblocks_ = 0, kc_ = 0;
+    da_int ppb_ = 0, nblocks_ = 0, kc_ = 0, dl_ = 0;
     std::vector<int> escale_host_;
     // fp32 MFMA assignment: zero-padded copy of the data (n x kp_), fp32 centres, -|c|^2/2,
     // the re-check list, the centres transposed for the re-check.

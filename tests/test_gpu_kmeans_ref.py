@@ -221,3 +221,21 @@ def test_kmeanspp_wide_candidates(oracle):
     r = oracle.kmeans(X, 256, "k-means++", n_init=1, seed=9, max_iter=3, tol=0.0)
     np.testing.assert_array_equal(km.labels, r["labels"])
     np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("distance", ["euclidean", "cosine"])
+def test_kmeans_features_beyond_one_sum_pass(oracle, distance):
+    """d = 20000: the centroid sums take two feature passes per cluster (at most 16,384
+    (cluster, feature) pairs per pass), the spherical normalisation follows the last."""
+    X, rng = _blobs(51, 60, 20000, 2, spread=0.5)
+    if distance == "cosine":
+        km = aoclda.clustering.kmeans(n_clusters=2, n_init=1, seed=3, max_iter=10, tol=1e-8,
+                                      distance="cosine", normalize_data=True).fit(X)
+        r = oracle.kmeans_sph(X, 2, "k-means++", n_init=1, seed=3, max_iter=10, tol=1e-8,
+                              normalize=True)
+    else:
+        km = _fit(X, 2, np.float64, "row-major", n_init=1, seed=3, max_iter=10, tol=1e-8)
+        r = oracle.kmeans(X, 2, "k-means++", n_init=1, seed=3, max_iter=10, tol=1e-8)
+    np.testing.assert_array_equal(km.labels, r["labels"])
+    assert km.n_iter == r["n_iter"]
+    np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-12, atol=1e-12)
