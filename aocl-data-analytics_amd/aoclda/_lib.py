@@ -113,6 +113,9 @@ _proto("da_amd_device_name", S, C.c_char_p, da_int)
 _proto("da_amd_synchronize", S, da_handle)
 _proto("da_amd_profile_enable", S, da_handle, da_int)
 _proto("da_amd_profile_reset", S, da_handle)
+_proto("da_amd_pairwise_profile_enable", S, da_int)
+_proto("da_amd_pairwise_profile_reset", S)
+_proto("da_amd_pairwise_profile_get", S, C.c_char_p, P(C.c_double), P(da_int))
 _proto("da_amd_profile_get", S, da_handle, C.c_char_p, P(C.c_double), P(da_int))
 _proto("da_nn_last_query_stats", S, da_handle, P(da_int), P(da_int))
 _proto("da_kmeans_last_compute_stats", S, da_handle, P(da_int))

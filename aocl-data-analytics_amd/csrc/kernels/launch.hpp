@@ -52,6 +52,17 @@ hipError_t launch_pairwise_direct(int metric, const T *X, const T *Y, int m, int
                                   U *D, int64_t ldd, const double *nx, const double *ny,
                                   double p, int zero_diag, int diag_offset, hipStream_t s);
 
+// GEMM-form distance tile on the fp64 matrix cores for the metrics that are functions of x.y
+// (pairwise_mfma_metric): nx/ny are the squared row norms (cosine: the cosine norms, sqrt
+// replaced by 1 when <= eps). Squared euclidean and euclidean elements whose GEMM-form
+// rounding bound could exceed 5e-13 (double output) or 5e-7 (float output) of the value are
+// recomputed with direct differences. Same layout and diagonal rule as launch_pairwise_direct.
+bool pairwise_mfma_metric(int metric);
+template <typename T, typename U>
+hipError_t launch_pairwise_mfma(int metric, const T *X, const T *Y, int m, int n, int k, U *D,
+                                int64_t ldd, const double *nx, const double *ny, int zero_diag,
+                                int diag_offset, hipStream_t s);
+
 // Per-row top-k of a row-major key matrix (rows x n, leading dimension ld) by
 // (key, column index), ascending (descending when `largest`). Writes the k best keys and
 // indices per row, sorted, to out_key[row*k + j] / out_idx[row*k + j]. k <= 3840.

@@ -154,9 +154,9 @@ da_status pairwise_noalloc(da_order order, da_int m, da_int n, da_int k, const T
     const da_int chunk = 65535 * 64;
     for (da_int r0 = 0; r0 < ma; r0 += chunk) {
         const da_int rows = ma - r0 < chunk ? ma - r0 : chunk;
-        NA_HIP((launch_pairwise_direct<T, T>(km, A + (size_t)r0 * k, B, rows, ncols, k,
-                                             D + (size_t)r0 * ldd, ldd, na ? na + r0 : nullptr,
-                                             nb, (double)p, zero_diag, (int)r0, s)));
+        NA_HIP(pairwise_tile<T>(km, A + (size_t)r0 * k, B, rows, ncols, k, D + (size_t)r0 * ldd,
+                                ldd, na ? na + r0 : nullptr, nb, (double)p, zero_diag, (int)r0,
+                                s));
     }
     NA_HIP(hipStreamSynchronize(s));
     return da_status_success;

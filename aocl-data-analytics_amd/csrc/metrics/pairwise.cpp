@@ -120,10 +120,12 @@ da_status pairwise(da_order order, da_int m, da_int n, da_int k, const T *X, da_
     DA_HIP(err, S.dbuf.reserve((size_t)chunk * row_bytes));
     for (da_int r0 = 0; r0 < ma; r0 += chunk) {
         const da_int rows = std::min(chunk, ma - r0);
-        DA_HIP(err, (launch_pairwise_direct<T, T>(km, A + (size_t)r0 * k, B, rows, nb_cols, k,
-                                                  S.dbuf.as<T>(), nb_cols,
-                                                  na ? na + r0 : nullptr, nb, (double)p,
-                                                  zero_diag, (int)r0, s)));
+        {
+            KernelScope ks(S.gpu, pairwise_mfma_metric(km) ? "pairwise_mfma" : "pairwise_direct");
+            DA_HIP(err, pairwise_tile<T>(km, A + (size_t)r0 * k, B, rows, nb_cols, k,
+                                         S.dbuf.as<T>(), nb_cols, na ? na + r0 : nullptr, nb,
+                                         (double)p, zero_diag, (int)r0, s));
+        }
         DA_HIP(err, copy_2d(D + (size_t)r0 * ldd, (size_t)ldd * sizeof(T),
                                      S.dbuf.ptr(), row_bytes, row_bytes, rows,
                                      hipMemcpyDeviceToHost, s));
@@ -135,6 +137,33 @@ da_status pairwise(da_order order, da_int m, da_int n, da_int k, const T *X, da_
 } // namespace
 
 extern "C" {
+
+// Per-kernel timing of the da_pairwise_distances_? calls (their own stream), as
+// da_amd_profile_* does for a handle: kernels "pairwise_mfma" and "pairwise_direct".
+da_status da_amd_pairwise_profile_enable(da_int on) {
+    PairwiseState &S = state();
+    std::lock_guard<std::mutex> lock(S.mu);
+    S.gpu.enable_profile(on != 0);
+    return da_status_success;
+}
+da_status da_amd_pairwise_profile_reset(void) {
+    PairwiseState &S = state();
+    std::lock_guard<std::mutex> lock(S.mu);
+    S.gpu.reset_profile();
+    return da_status_success;
+}
+da_status da_amd_pairwise_profile_get(const char *kernel, double *total_ms, da_int *count) {
+    if (!kernel || !total_ms || !count)
+        return da_status_invalid_pointer;
+    PairwiseState &S = state();
+    std::lock_guard<std::mutex> lock(S.mu);
+    double ms = 0;
+    int c = 0;
+    S.gpu.profile_get(kernel, ms, c);
+    *total_ms = ms;
+    *count = c;
+    return da_status_success;
+}
 
 da_status da_pairwise_distances_d(da_order order, da_int m, da_int n, da_int k,
                                   const double *X, da_int ldx, const double *Y, da_int ldy,

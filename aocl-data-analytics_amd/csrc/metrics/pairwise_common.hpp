@@ -69,8 +69,21 @@ inline bool pairwise_zero_diag(int km, bool x_is_y) {
     return x_is_y && (km == KM_COSINE || km == KM_EUCLIDEAN_GEMM || km == KM_SQEUCLIDEAN_GEMM);
 }
 
-inline bool pairwise_needs_norms(int km) {
-    return km == KM_COSINE || km == KM_EUCLIDEAN_GEMM || km == KM_SQEUCLIDEAN_GEMM;
+// Row norms: squared norms for the GEMM-form tile (every metric that is a function of x.y),
+// turned into cosine norms for cosine.
+inline bool pairwise_needs_norms(int km) { return pairwise_mfma_metric(km); }
+
+// The distance tile of a metric: the fp64 matrix-core GEMM form where the metric is a function
+// of x.y, direct differences (manhattan, minkowski) otherwise.
+template <typename T>
+inline hipError_t pairwise_tile(int km, const T *A, const T *B, int m, int n, int k, T *D,
+                                int64_t ldd, const double *na, const double *nb, double p,
+                                int zero_diag, int diag_offset, hipStream_t s) {
+    if (pairwise_mfma_metric(km))
+        return launch_pairwise_mfma<T, T>(km, A, B, m, n, k, D, ldd, na, nb, zero_diag,
+                                          diag_offset, s);
+    return launch_pairwise_direct<T, T>(km, A, B, m, n, k, D, ldd, na, nb, p, zero_diag,
+                                        diag_offset, s);
 }
 
 } // namespace dagpu

@@ -75,6 +75,15 @@ da_status da_amd_profile_get(da_handle handle, const char *kernel, double *total
 da_status da_amd_profile_names(da_handle handle, char *names, da_int *lnames);
 
 /*
+ * Per-kernel timing of the stateless da_pairwise_distances_? calls (no counterpart in the
+ * reference): HIP events around the distance tiles on their stream, kernels
+ * "pairwise_mfma" (the fp64 matrix-core GEMM form) and "pairwise_direct".
+ */
+da_status da_amd_pairwise_profile_enable(da_int on);
+da_status da_amd_pairwise_profile_reset(void);
+da_status da_amd_pairwise_profile_get(const char *kernel, double *total_ms, da_int *count);
+
+/*
  * Diagnostics of the last k-NN query of the handle: number of queries whose candidate
  * set could not be certified exact and were recomputed by the exact fallback pass.
  */

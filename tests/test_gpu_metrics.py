@@ -115,3 +115,42 @@ def test_minkowski_equivalents():
     pd = aoclda.metrics.pairwise_distances
     np.testing.assert_array_equal(pd(X, Y, "minkowski", 2.0), pd(X, Y, "euclidean"))
     np.testing.assert_array_equal(pd(X, Y, "minkowski", 1.0), pd(X, Y, "manhattan"))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("metric", ["euclidean", "sqeuclidean"])
+def test_gemm_form_recheck_near_duplicates(oracle, dtype, metric):
+    """Rows of Y are X's rows moved by 1e-7 .. 1e-2 of their norm, far from the origin: the
+    GEMM form |x|^2 + |y|^2 - 2 x.y loses those distances to cancellation, so the tile must
+    recompute them with direct differences and keep the direct form's relative accuracy."""
+    rng = np.random.default_rng(9)
+    X = rng.uniform(50, 60, (300, 48))
+    scale = 10.0 ** rng.uniform(-7, -2, (300, 1))
+    Y = X + scale * rng.standard_normal((300, 48))
+    X, Y = X.astype(dtype), Y.astype(dtype)
+    D = aoclda.metrics.pairwise_distances(X, Y, metric=metric)
+    E = oracle.pairwise(X.astype(np.float64), Y.astype(np.float64), metric)
+    np.testing.assert_allclose(D, E, rtol=RTOL[dtype], atol=0)
+    # X against itself: an exactly zero diagonal, a symmetric matrix
+    Dx = aoclda.metrics.pairwise_distances(X, metric=metric)
+    assert np.all(np.diag(Dx) == 0)
+    np.testing.assert_array_equal(Dx, Dx.T)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("metric", ["euclidean", "sqeuclidean", "cosine", "inner_product",
+                                    "euclidean_gemm", "sqeuclidean_gemm"])
+def test_gemm_form_tile_edges(oracle, dtype, metric):
+    """Shapes that are not multiples of the 64 x 64 tile or of the 16-deep k slice, k from 1
+    to 133, against the oracle."""
+    rng = np.random.default_rng(10)
+    for m, n, k in ((1, 1, 1), (65, 63, 3), (130, 71, 17), (64, 128, 133), (200, 9, 64)):
+        X = rng.standard_normal((m, k)).astype(dtype)
+        Y = rng.standard_normal((n, k)).astype(dtype)
+        D = aoclda.metrics.pairwise_distances(X, Y, metric=metric)
+        E = oracle.pairwise(X.astype(np.float64), Y.astype(np.float64), metric,
+                            eps=np.finfo(dtype).eps)
+        if metric in ("euclidean_gemm", "sqeuclidean_gemm"):
+            E = np.nan_to_num(E, nan=0.0)
+        scale = 1.0 if metric in ("euclidean", "sqeuclidean", "cosine") else max(1.0, np.abs(E).max())
+        close(D, E, dtype, scale)
