@@ -525,21 +525,26 @@ __device__ __forceinline__ double dot_xx(const T *__restrict__ X, int d, int64_t
 // The distance of samples i and j used by the initialisations: the squared Euclidean
 // distance, or for spherical k-means (sph) max(0, 1 - x_i.x_j inv_i inv_j) with the inverse
 // norms inv (compute_distances_to_point, kmeans.cpp:1488-1522; without inv the samples are
-// taken as unit rows). cs: row j staged in LDS, or null to read it from global memory.
+// taken as unit rows). The _lds form reads row j staged in LDS (cs), the _global form from X.
 struct KmGeom {
     int sph;
     const double *inv;
 };
-template <typename T>
-__device__ __forceinline__ double sample_dist(const T *__restrict__ X, int d, int64_t i,
-                                              int64_t j, const double *cs, KmGeom g) {
-    if (!g.sph)
-        return cs ? sqdist_rows(X, d, i, cs) : sqdist_xx(X, d, i, j);
-    double cosv = cs ? dot_rows(X, d, i, cs) : dot_xx(X, d, i, j);
+__device__ __forceinline__ double cosine_dist(double dot, KmGeom g, int64_t i, int64_t j) {
     if (g.inv)
-        cosv = cosv * g.inv[i] * g.inv[j];
-    const double v = 1.0 - cosv;
+        dot = dot * g.inv[i] * g.inv[j];
+    const double v = 1.0 - dot;
     return v > 0.0 ? v : 0.0;
+}
+template <typename T>
+__device__ __forceinline__ double sample_dist_lds(const T *__restrict__ X, int d, int64_t i,
+                                                  int64_t j, const double *cs, KmGeom g) {
+    return g.sph ? cosine_dist(dot_rows(X, d, i, cs), g, i, j) : sqdist_rows(X, d, i, cs);
+}
+template <typename T>
+__device__ __forceinline__ double sample_dist_global(const T *__restrict__ X, int d, int64_t i,
+                                                     int64_t j, KmGeom g) {
+    return g.sph ? cosine_dist(dot_xx(X, d, i, j), g, i, j) : sqdist_xx(X, d, i, j);
 }
 
 // Rows of at most ROW_STAGE_MAX features are staged in LDS by the distance passes of the
@@ -560,7 +565,7 @@ __global__ __launch_bounds__(256) void dist_to_row_kernel(const T *__restrict__ 
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n)
         return;
-    double v = sample_dist(X, d, i, ci, staged ? cs : nullptr, g);
+    double v = staged ? sample_dist_lds(X, d, i, ci, cs, g) : sample_dist_global(X, d, i, ci, g);
     if (mode == 1) {
         // min update: max(0, min(w, v))
         const double o = w[i];
@@ -603,16 +608,21 @@ __global__ __launch_bounds__(256) void scan_blocksum_kernel(const double *__rest
         bsum[blockIdx.x] = s[0];
 }
 
-// Phase 2: exclusive scan of the block sums (sequential, fixed order).
-__global__ void scan_blockoffsets_kernel(double *bsum, int nb) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double run = 0.0;
-        for (int b = 0; b < nb; b++) {
-            const double v = bsum[b];
-            bsum[b] = run;
-            run += v;
-        }
+// Phase 2: exclusive scan of the nb <= 1024 block sums by one workgroup (Hillis-Steele in LDS,
+// a fixed order of additions).
+__global__ __launch_bounds__(1024) void scan_blockoffsets_kernel(double *bsum, int nb) {
+    __shared__ double s[1024];
+    const int t = threadIdx.x;
+    s[t] = t < nb ? bsum[t] : 0.0;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const double v = t >= off ? s[t - off] : 0.0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
     }
+    if (t < nb)
+        bsum[t] = t > 0 ? s[t - 1] : 0.0;
 }
 
 // Phase 3: each block scans its chunk with its offset.
@@ -693,7 +703,8 @@ __global__ __launch_bounds__(256) void candidate_cost_kernel(const T *__restrict
 #pragma unroll
         for (int c = 0; c < NT_MAX; c++) {
             if (c < nt) {
-                double v = sample_dist(X, d, i, (int64_t)cand[c], staged ? cs + c * d : nullptr, g);
+                double v = staged ? sample_dist_lds(X, d, i, (int64_t)cand[c], cs + c * d, g)
+                                  : sample_dist_global(X, d, i, (int64_t)cand[c], g);
                 v = wi < v ? wi : v;
                 acc[c] += v > 0.0 ? v : 0.0;
             }
@@ -947,7 +958,8 @@ hipError_t launch_normalised_scan(const double *w, int n, const double *total, d
     const int nb = SCAN_BLOCKS;
     const int chunk = (n + nb - 1) / nb;
     hipLaunchKernelGGL(scan_blocksum_kernel, dim3(nb), dim3(256), 0, s, w, n, total, tmp, chunk);
-    hipLaunchKernelGGL(scan_blockoffsets_kernel, dim3(1), dim3(64), 0, s, tmp, nb);
+    static_assert(SCAN_BLOCKS <= 1024, "one workgroup scans the block sums");
+    hipLaunchKernelGGL(scan_blockoffsets_kernel, dim3(1), dim3(1024), 0, s, tmp, nb);
     hipLaunchKernelGGL(scan_final_kernel, dim3(nb), dim3(256), 0, s, w, n, total, tmp, cp, chunk);
     return hipGetLastError();
 }

@@ -743,7 +743,7 @@ def main():
     ap.add_argument("--serving", action="store_true",
                     help="run the continuous-batching serving benchmark instead")
     ap.add_argument("--serve-rows", type=int, default=2_000_000)
-    ap.add_argument("--serve-forms", default="fp16x3")
+    ap.add_argument("--serve-forms", default="fp16,fp16x3")
     ap.add_argument("--serve-batch", type=int, default=4096)
     ap.add_argument("--serve-bytes", type=int, default=1 << 30)
     ap.add_argument("--serve-closed", type=int, default=200)

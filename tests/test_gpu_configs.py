@@ -5,7 +5,8 @@
   * k-means, 2,000,000 x 32 from 256 blobs, k = 256, k-means++ seed, 25 iterations, fp32 data
     with the bf16 cross term;
   * batched k-means, 1,024 uneven instances, 15 iterations;
-  * the serving loop over a resident 2,000,000 x 96 index of unit rows, k in {1, 10, 100}.
+  * the serving loop over a resident 2,000,000 x 96 index of unit rows, k in {1, 10, 100},
+    with the plain fp16 and the split fp16x3 cross terms.
 
 At these sizes the oracle checks a sample (the same seeded inputs), and every result is
 checked in full through properties that do not depend on the size: ascending distances that
@@ -150,7 +151,8 @@ def test_kmeans_batch_1024_instances(oracle):
         assert (got <= D.min(1) + 1e-4 * np.maximum(1.0, D.min(1))).all(), i
 
 
-def test_serving_2000000x96_alone_equals_mixed(oracle):
+@pytest.mark.parametrize("precision", ["fp16", "fp16x3"])
+def test_serving_2000000x96_alone_equals_mixed(oracle, precision):
     rng = np.random.default_rng(21)
     means = rng.standard_normal((1024, 96))
     comp = rng.integers(0, 1024, 2_000_000)
@@ -163,7 +165,7 @@ def test_serving_2000000x96_alone_equals_mixed(oracle):
         Q = (means[c] + 0.3 * rng.standard_normal((m, 96))).astype(np.float32)
         return Q / np.linalg.norm(Q, axis=1, keepdims=True)
 
-    nn = aoclda.neighbors.nearest_neighbors(precision="fp16x3").fit(Xt)
+    nn = aoclda.neighbors.nearest_neighbors(precision=precision).fit(Xt)
     sizes = [1, 37, 1000, 4096, 5, 300]
     ks = [1, 100, 10, 100, 10, 1]
     reqs = [req(s) for s in sizes]
