@@ -34,6 +34,7 @@ STATUS = {
     24: "bad_derivatives", 25: "arch_not_supported", 26: "io_error",
     700: "outlier_warning", 400: "option_not_found", 401: "option_locked", 402: "option_wrong_type",
     403: "option_invalid_bounds", 404: "option_invalid_value", 900: "empty_clusters",
+    800: "serialization_error", 801: "invalid_file_data", 802: "version_mismatch",
 }
 CODE = {v: k for k, v in STATUS.items()}
 
@@ -211,3 +212,9 @@ class Handle:
                 lib.da_handle_destroy(C.byref(self.h))
         except Exception:
             pass
+
+# Stored models (aoclda_handle.h:177-217, aoclda_utils.h:213).
+_proto("da_handle_save_model", S, da_handle, C.c_char_p)
+_proto("da_handle_load_model", S, P(da_handle), C.c_char_p)
+_proto("da_handle_print_model_versions", S, da_handle)
+_proto("da_print_model_metadata", S, C.c_char_p)

@@ -7,6 +7,7 @@
 #include "errors.hpp"
 #include "gpu.hpp"
 #include "options.hpp"
+#include "persistence.hpp"
 #include <memory>
 #include <string>
 #include <type_traits>
@@ -37,10 +38,26 @@ class AlgHandle {
                              const std::string &ld_name, da_int rows_min = 1,
                              da_int cols_min = 1);
 
+    // Stored models (basic_handle::serialize, save_model and load_model of the reference,
+    // basic_handle.hpp:82-101, basic_handle.cpp:276-325). serialize() visits the
+    // algorithm's fields in their stored order, writing them (saving) or reading them back
+    // (loading); save_check() refuses a model that cannot be saved yet; restore() rebuilds
+    // the GPU-resident state of a loaded model from the fields read back.
+    virtual da_status serialize(ModelBuffer &b) {
+        (void)b;
+        return da_status_not_implemented;
+    }
+    virtual da_status save_check() { return da_status_success; }
+    virtual da_status restore() { return da_status_success; }
+
     OptionRegistry opts;
     ErrorStack *err;
     GpuContext *gpu;
     bool model_trained = false;
+    // Serialization version of this build's models, and the library version a loaded model
+    // was saved by ("" until a model is loaded).
+    da_int serialization_version = MODEL_MIN_VERSION;
+    std::string saved_version;
 };
 
 } // namespace dagpu

@@ -1,4 +1,5 @@
 #include "options.hpp"
+#include "persistence.hpp"
 #include <cctype>
 #include <cstdio>
 #include <sstream>
@@ -227,6 +228,78 @@ void OptionRegistry::print() const {
             val << o.sval;
         std::printf("     %-40s  %s\n", o.name.c_str(), val.str().c_str());
     }
+}
+
+da_status OptionRegistry::save(ModelBuffer &b, bool single) const {
+    da_status st = b.put((da_int)opts_.size());
+    for (const auto &kv : opts_) {
+        if (st != da_status_success)
+            break;
+        const Opt &o = kv.second;
+        st = b.put(o.name);
+        if (st != da_status_success)
+            break;
+        switch (o.kind) {
+        case Kind::Int:
+            st = b.put(o.ival);
+            break;
+        case Kind::Real:
+            st = single ? b.put((float)o.rval) : b.put(o.rval);
+            break;
+        case Kind::String:
+            st = b.put(o.sval);
+            break;
+        }
+    }
+    return st;
+}
+
+da_status OptionRegistry::load(ModelBuffer &b, bool single) {
+    da_int count = 0;
+    da_status st = b.get(count);
+    if (st != da_status_success)
+        return st;
+    if (count < 0 || count > (da_int)opts_.size())
+        return da_status_invalid_file_data;
+    for (da_int i = 0; i < count; i++) {
+        std::string name;
+        st = b.get(name);
+        if (st != da_status_success)
+            return st;
+        Opt *o = find(name);
+        if (o == nullptr)
+            return da_status_invalid_file_data;
+        switch (o->kind) {
+        case Kind::Int:
+            st = b.get(o->ival);
+            break;
+        case Kind::Real:
+            if (single) {
+                float v = 0;
+                st = b.get(v);
+                o->rval = v;
+            } else {
+                st = b.get(o->rval);
+            }
+            break;
+        case Kind::String: {
+            std::string v;
+            st = b.get(v);
+            if (st != da_status_success)
+                return st;
+            bool known = false;
+            for (auto &c : o->choices)
+                known |= c.first == v;
+            if (!known)
+                return da_status_invalid_file_data;
+            o->sval = v;
+            break;
+        }
+        }
+        if (st != da_status_success)
+            return st;
+    }
+    return da_status_success;
 }
 
 } // namespace dagpu

@@ -15,6 +15,8 @@
 
 namespace dagpu {
 
+class ModelBuffer;
+
 enum class LowerBound { m_inf, greater_than, greater_equal };
 enum class UpperBound { p_inf, less_than, less_equal };
 
@@ -49,6 +51,15 @@ class OptionRegistry {
 
     void print() const;
     const std::string &errmsg() const { return errmsg_; }
+
+    // Stored-model form of the registry (the reference's save_registry / load_registry,
+    // source/core/utilities/options.hpp:956-1009): the number of options, then per option
+    // its name and value: integers as int64, reals in the handle's precision (float when
+    // `single`), strings as their selected label. Loading sets the stored values without the
+    // bounds checks of a user setting; an unknown name or label, or more options than this
+    // registry holds, is da_status_invalid_file_data.
+    da_status save(ModelBuffer &b, bool single) const;
+    da_status load(ModelBuffer &b, bool single);
 
   private:
     enum class Kind { Int, Real, String };

@@ -467,6 +467,211 @@ da_status ApproxNNHandle<T>::kneighbors(da_int n_queries, da_int n_features, con
 }
 
 // Results (approximate_neighbors.cpp:62-150).
+// Stored fields, in the order of approximate_neighbors<T>::serialize
+// (source/core/approximate_neighbors/approximate_neighbors.cpp:1346-1399): the centroids in
+// the handle's storage order, each list's vectors (row-major, normalised for cosine), global
+// indices and, for the squared euclidean search, squared norms; then the training rows.
+template <typename T> da_status ApproxNNHandle<T>::serialize(ModelBuffer &b) {
+    const bool load = b.loading();
+    bool data_set = data_set_, trained = model_trained, added = data_added_;
+    da_int n_neighbors = opts.ival("number of neighbors"), algo = algo_, internal_algo = ANN_IVFFLAT;
+    // The search metric: squared distances for "euclidean" (:269-270).
+    da_int internal_metric = metric_ == ANN_EUCLIDEAN ? (da_int)ANN_SQEUCLIDEAN : metric_;
+    da_int metric = metric_, seed = opts.ival("seed"), internal_seed = seed;
+    da_int n_train = n_samples_;
+    da_order ord = order_;
+    da_int n = n_samples_, d = n_features_;
+    T frac = (T)opts.rval("train fraction");
+    da_int n_list = n_list_, n_probe = opts.ival("n_probe"), max_iter = opts.ival("k-means_iter"),
+           n_init = 1;
+    T tol = (T)std::sqrt((double)std::numeric_limits<T>::epsilon());
+    std::vector<T> centroids, cnorms;
+    da_int ld_c = ord == column_major ? n_list : d;
+    da_int n_index = n_index_;
+    std::vector<std::vector<T>> vecs, norms;
+    std::vector<std::vector<da_int>> gidx;
+    std::vector<da_int> sizes = list_sizes_, old_sizes = list_sizes_;
+    if (!load && model_trained) {
+        centroids.resize(centroids_.size());
+        for (da_int c = 0; c < n_list; c++)
+            for (da_int j = 0; j < d; j++)
+                centroids[ord == column_major ? (size_t)j * n_list + c : (size_t)c * d + j] =
+                    centroids_[(size_t)c * d + j];
+        if (internal_metric == ANN_SQEUCLIDEAN) {
+            cnorms.assign(n_list, (T)0);
+            for (da_int c = 0; c < n_list; c++)
+                for (da_int j = 0; j < d; j++)
+                    cnorms[c] += centroids_[(size_t)c * d + j] * centroids_[(size_t)c * d + j];
+        }
+        std::vector<T> rows((size_t)n_index_ * d);
+        if (n_index_ > 0) {
+            DA_HIP(err, hipMemcpyAsync(rows.data(), X_d_.ptr(), rows.size() * sizeof(T),
+                                       hipMemcpyDeviceToHost, gpu->stream()));
+            DA_HIP(err, hipStreamSynchronize(gpu->stream()));
+        }
+        vecs.resize(n_list);
+        norms.resize(n_list);
+        gidx.resize(n_list);
+        for (da_int c = 0; c < n_list; c++)
+            for (int g : lists_[c]) {
+                gidx[c].push_back(g);
+                T nrm = 0;
+                for (da_int j = 0; j < d; j++) {
+                    const T v = rows[(size_t)g * d + j];
+                    vecs[c].push_back(v);
+                    nrm += v * v;
+                }
+                if (internal_metric == ANN_SQEUCLIDEAN)
+                    norms[c].push_back(nrm);
+            }
+    }
+    da_status s = da_status_success;
+    auto io = [&](auto &v) {
+        if (s == da_status_success)
+            s = b.io(v);
+    };
+    io(data_set);
+    io(trained);
+    io(added);
+    io(n_neighbors);
+    io(algo);
+    io(internal_algo);
+    io(internal_metric);
+    io(metric);
+    io(internal_seed);
+    io(seed);
+    io(n_train);
+    io(ord);
+    io(n);
+    io(d);
+    io(frac);
+    io(n_list);
+    io(n_probe);
+    io(max_iter);
+    io(n_init);
+    io(tol);
+    io(centroids);
+    io(ld_c);
+    io(n_index);
+    io(vecs);
+    io(gidx);
+    io(sizes);
+    io(old_sizes);
+    io(norms);
+    io(cnorms);
+    if (s != da_status_success)
+        return s;
+    std::vector<T> X;
+    if (!load && data_set_ && X_train_ != nullptr) {
+        X.resize((size_t)n * d);
+        for (da_int i = 0; i < n; i++)
+            for (da_int j = 0; j < d; j++) {
+                const size_t src = order_ == row_major ? (size_t)i * ldx_train_ + j
+                                                       : (size_t)j * ldx_train_ + i;
+                const size_t dst = order_ == row_major ? (size_t)i * d + j : (size_t)j * n + i;
+                X[dst] = X_train_[src];
+            }
+    }
+    io(X);
+    if (s != da_status_success || !load)
+        return s;
+    if (n < 0 || d <= 0 || n_list <= 0 || n_index < 0 ||
+        (trained && (int64_t)centroids.size() != (int64_t)n_list * d) ||
+        (added && ((da_int)vecs.size() != n_list || (da_int)gidx.size() != n_list)) ||
+        (!X.empty() && (int64_t)X.size() != (int64_t)n * d))
+        return da_status_invalid_file_data;
+    data_set_ = data_set && !X.empty();
+    model_trained = trained;
+    data_added_ = added;
+    order_ = ord;
+    n_samples_ = n;
+    n_features_ = d;
+    n_list_ = n_list;
+    algo_ = algo;
+    metric_ = metric;
+    kmeans_iter_ = 0;
+    stored_train_ = std::move(X);
+    X_train_ = data_set_ ? stored_train_.data() : nullptr;
+    ldx_train_ = ord == row_major ? d : n;
+    centroids_.assign((size_t)n_list * d, (T)0);
+    if (trained)
+        for (da_int c = 0; c < n_list; c++)
+            for (da_int j = 0; j < d; j++)
+                centroids_[(size_t)c * d + j] =
+                    centroids[ord == column_major ? (size_t)j * n_list + c : (size_t)c * d + j];
+    list_sizes_.assign(n_list, 0);
+    lists_.assign(n_list, {});
+    n_index_ = added ? n_index : 0;
+    stored_rows_.assign((size_t)n_index_ * d, (T)0);
+    std::vector<char> seen((size_t)n_index_, 0);
+    if (added) {
+        for (da_int c = 0; c < n_list; c++) {
+            if (vecs[c].size() != gidx[c].size() * (size_t)d)
+                return da_status_invalid_file_data;
+            for (size_t i = 0; i < gidx[c].size(); i++) {
+                const da_int g = gidx[c][i];
+                if (g < 0 || g >= n_index_ || seen[g])
+                    return da_status_invalid_file_data;
+                seen[g] = 1;
+                lists_[c].push_back((int)g);
+                std::copy(vecs[c].begin() + i * d, vecs[c].begin() + (i + 1) * d,
+                          stored_rows_.begin() + (size_t)g * d);
+            }
+            list_sizes_[c] = (da_int)gidx[c].size();
+        }
+        for (char f : seen)
+            if (!f)
+                return da_status_invalid_file_data;
+    }
+    return da_status_success;
+}
+
+template <typename T> da_status ApproxNNHandle<T>::save_check() {
+    if (!data_set_ || !model_trained)
+        return err->error(da_status_no_data,
+                          "Index has not yet been trained. Please call da_approx_nn_train_d or "
+                          "da_approx_nn_train_s before saving the model.");
+    return da_status_success;
+}
+
+// A loaded index searches again: centroids, vectors (in order of addition) and the lists go
+// back to GPU memory as train() and add() leave them.
+template <typename T> da_status ApproxNNHandle<T>::restore() {
+    if (!model_trained)
+        return da_status_success;
+    da_status s = gpu->init(err);
+    if (s != da_status_success)
+        return s;
+    hipStream_t st = gpu->stream();
+    DA_HIP(err, C_d_.reserve(centroids_.size() * sizeof(T)));
+    DA_HIP(err, cdouble_d_.reserve(centroids_.size() * sizeof(double)));
+    DA_HIP(err, hipMemcpyAsync(C_d_.ptr(), centroids_.data(), centroids_.size() * sizeof(T),
+                               hipMemcpyHostToDevice, st));
+    DA_HIP(err, launch_convert<T>(C_d_.as<T>(), (int64_t)centroids_.size(),
+                                  cdouble_d_.as<double>(), st));
+    if (data_added_ && n_index_ > 0) {
+        DA_HIP(err, X_d_.reserve(stored_rows_.size() * sizeof(T)));
+        DA_HIP(err, hipMemcpyAsync(X_d_.ptr(), stored_rows_.data(),
+                                   stored_rows_.size() * sizeof(T), hipMemcpyHostToDevice, st));
+        std::vector<int> perm, off(n_list_ + 1, 0);
+        perm.reserve(n_index_);
+        for (da_int c = 0; c < n_list_; c++) {
+            perm.insert(perm.end(), lists_[c].begin(), lists_[c].end());
+            off[c + 1] = (int)perm.size();
+        }
+        DA_HIP(err, perm_d_.reserve(perm.size() * sizeof(int)));
+        DA_HIP(err, off_d_.reserve(off.size() * sizeof(int)));
+        DA_HIP(err, hipMemcpyAsync(perm_d_.ptr(), perm.data(), perm.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, st));
+        DA_HIP(err, hipMemcpyAsync(off_d_.ptr(), off.data(), off.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, st));
+    }
+    DA_HIP(err, hipStreamSynchronize(st));
+    stored_rows_.clear();
+    stored_rows_.shrink_to_fit();
+    return da_status_success;
+}
+
 template <typename T>
 da_status ApproxNNHandle<T>::get_result(da_result query, da_int *dim, T *result) {
     if (!model_trained)

@@ -21,13 +21,20 @@ template <typename T> class ApproxNNHandle : public AlgHandle {
     using AlgHandle::get_result;
     da_status get_result(da_result query, da_int *dim, T *result) override;
     da_status get_result(da_result query, da_int *dim, da_int *result) override;
+    da_status serialize(ModelBuffer &b) override;
+    da_status save_check() override;
+    da_status restore() override;
 
   private:
     da_status check_options_update();
     da_status upload_rows(const T *X, da_int n, da_int ld, DeviceBuffer &dst, size_t offset_rows);
 
-    // Training data (the caller's pointer, as the reference keeps it).
+    // Training data (the caller's pointer, as the reference keeps it; a loaded model points
+    // at its own copy, stored_train_).
     const T *X_train_ = nullptr;
+    std::vector<T> stored_train_;
+    // Added vectors of a loaded model in order of addition (row-major), until restore().
+    std::vector<T> stored_rows_;
     da_int ldx_train_ = 0, n_samples_ = 0, n_features_ = 0;
     bool data_set_ = false, data_added_ = false;
     da_order order_ = column_major;

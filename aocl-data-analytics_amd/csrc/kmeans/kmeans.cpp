@@ -1144,6 +1144,99 @@ da_status KMeansHandle<T>::transform(da_int m_samples, da_int m_features, const 
     return da_status_success;
 }
 
+// Stored fields of a computed k-means, in the order of kmeans<T>::serialize
+// (source/core/clustering/kmeans/kmeans.cpp:1882-1912): the centres column-major k x d in
+// the working precision, and their squared norms (the reference's workc1; zero for
+// spherical k-means, as its predict step keeps them).
+template <typename T> da_status KMeansHandle<T>::serialize(ModelBuffer &b) {
+    const bool load = b.loading();
+    bool trained = model_trained;
+    da_order ord = order();
+    da_int algorithm = opts.skey("algorithm"), n_samples = n_samples_, n_features = n_features_,
+           n_iter = best_n_iter_, seed = opts.ival("seed"), k = n_clusters_,
+           max_iter = opts.ival("max_iter"), n_init = opts.ival("n_init");
+    T inertia = (T)best_inertia_, tol = (T)opts.rval("convergence tolerance");
+    std::vector<da_int> labels = best_labels_;
+    std::vector<T> centres, norms;
+    bool sph = sph_, sph_norm = sph_norm_;
+    if (!load) {
+        centres.resize((size_t)k * n_features);
+        norms.assign((size_t)k, (T)0);
+        for (da_int c = 0; c < k; c++)
+            for (da_int l = 0; l < n_features; l++) {
+                const double v = best_centres_[(size_t)c * n_features + l];
+                centres[(size_t)l * k + c] = (T)v;
+                if (!sph)
+                    norms[c] += (T)(v * v);
+            }
+    }
+    da_status s = da_status_success;
+    auto io = [&](auto &v) {
+        if (s == da_status_success)
+            s = b.io(v);
+    };
+    io(trained);
+    io(ord);
+    io(algorithm);
+    io(n_samples);
+    io(n_features);
+    io(n_iter);
+    io(inertia);
+    io(seed);
+    io(k);
+    io(tol);
+    io(max_iter);
+    io(n_init);
+    io(labels);
+    io(centres);
+    io(norms);
+    io(sph);
+    io(sph_norm);
+    if (s != da_status_success || !load)
+        return s;
+    if (n_samples < 0 || n_features <= 0 || k <= 0 || (da_int)labels.size() != n_samples ||
+        (int64_t)centres.size() != (int64_t)k * n_features)
+        return da_status_invalid_file_data;
+    model_trained = trained;
+    n_samples_ = n_samples;
+    n_features_ = n_features;
+    n_clusters_ = k;
+    best_n_iter_ = n_iter;
+    best_lp_n_iter_ = 0;
+    best_inertia_ = (double)inertia;
+    best_labels_ = labels;
+    best_centres_.assign((size_t)k * n_features, 0.0);
+    for (da_int c = 0; c < k; c++)
+        for (da_int l = 0; l < n_features; l++)
+            best_centres_[(size_t)c * n_features + l] = (double)centres[(size_t)l * k + c];
+    sph_ = sph;
+    sph_norm_ = sph && sph_norm;
+    return da_status_success;
+}
+
+template <typename T> da_status KMeansHandle<T>::save_check() {
+    if (!model_trained)
+        return err->error(da_status_no_data,
+                          "k-means clustering has not yet been computed. Please call "
+                          "da_kmeans_compute_s or da_kmeans_compute_d before saving the model.");
+    return da_status_success;
+}
+
+// A loaded k-means predicts and transforms: its centres go to GPU memory.
+template <typename T> da_status KMeansHandle<T>::restore() {
+    if (!model_trained)
+        return da_status_success;
+    da_status s = gpu->init(err);
+    if (s != da_status_success)
+        return s;
+    hipStream_t st = gpu->stream();
+    const size_t cb = best_centres_.size() * sizeof(double);
+    DA_HIP(err, Cbest_.reserve(cb));
+    DA_HIP(err, hipMemcpyAsync(Cbest_.ptr(), best_centres_.data(), cb, hipMemcpyHostToDevice, st));
+    DA_HIP(err, hipStreamSynchronize(st));
+    return da_status_success;
+}
+
 template <typename T>
 da_status KMeansHandle<T>::get_result_typed(da_result query, da_int *dim, T *result) {
     if (!model_trained)

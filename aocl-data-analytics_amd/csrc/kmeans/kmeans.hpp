@@ -28,6 +28,9 @@ template <typename T> class KMeansHandle : public AlgHandle {
     da_status get_result(da_result query, da_int *dim, float *result) override;
     da_status get_result(da_result query, da_int *dim, da_int *result) override;
     void refresh() override;
+    da_status serialize(ModelBuffer &b) override;
+    da_status save_check() override;
+    da_status restore() override;
 
   private:
     struct RunResult {

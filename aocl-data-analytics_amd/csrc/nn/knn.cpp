@@ -1557,6 +1557,174 @@ da_status NNHandle<T>::predict(da_int n_queries, da_int n_features, const T *X_t
 }
 
 // Radius results (nearest_neighbors.cpp get_result, :180-300 and :2284-2330).
+// Stored fields, in the order of neighbors<T>::serialize (source/core/nearest_neighbors/
+// nearest_neighbors.cpp:2329-2460). The search is brute force for every algorithm option, so
+// the working algorithm is stored as brute force and no tree follows the arrays; a stored
+// model that carries a tree after them loads without it. The training rows are stored
+// column-major, then the labels and targets (empty when not set). Fields the reference keeps
+// as caches of its prediction (most frequent label, mean target) are stored as not computed.
+template <typename T> da_status NNHandle<T>::serialize(ModelBuffer &b) {
+    const bool load = b.loading();
+    da_order ord = order();
+    da_int n = n_samples_, d = n_features_;
+    bool has_labels = has_labels_, has_targets = has_targets_, up_to_date = trained_,
+         classes_done = classes_ready_, sort_results = false, trained = trained_,
+         rnn_dist = rnn_dist_;
+    std::vector<da_int> classes = classes_;
+    da_int n_neighbors = n_neighbors_, algo = algo_, working_algo = NN_BRUTE, metric = metric_;
+    const bool squares =
+        metric_ == da_euclidean || metric_ == da_euclidean_gemm ||
+        (metric_ == da_minkowski && p_ == 2.0);
+    da_metric internal_metric = (da_metric)metric_;
+    if (squares)
+        internal_metric = metric_ == da_euclidean_gemm ? da_sqeuclidean_gemm : da_sqeuclidean;
+    da_int leaf_size = leaf_size_;
+    bool get_squares = squares;
+    T p = (T)p_;
+    da_int weights = opts.skey("weights"), outlier = opts.skey("outlier handling");
+    da_int n_classes = classes_ready_ ? (da_int)classes_.size() : -1;
+    T radius = (T)opts.rval("radius");
+    std::vector<da_int> rcount;
+    std::vector<std::vector<da_int>> rind;
+    std::vector<std::vector<T>> rdist;
+    if (!load && rnn_done_) {
+        rcount = rnn_count_;
+        size_t off = 0;
+        for (da_int c : rnn_count_) {
+            rind.emplace_back(rnn_ind_.begin() + off, rnn_ind_.begin() + off + c);
+            if (rnn_dist_)
+                rdist.emplace_back(rnn_d_.begin() + off, rnn_d_.begin() + off + c);
+            off += (size_t)c;
+        }
+    }
+    bool x_trained = trained_, mfl_done = false, manual_checked = false, mean_done = false;
+    da_int mfl = -1, manual_index = -1, manual_label = opts.ival("outlier label");
+    T mean_target = 0, manual_target = (T)opts.rval("outlier target");
+    da_status s = da_status_success;
+    auto io = [&](auto &v) {
+        if (s == da_status_success)
+            s = b.io(v);
+    };
+    io(ord);
+    io(n);
+    io(d);
+    io(has_labels);
+    io(has_targets);
+    io(up_to_date);
+    io(classes_done);
+    io(classes);
+    io(sort_results);
+    io(trained);
+    io(rnn_dist);
+    io(n_neighbors);
+    io(algo);
+    io(working_algo);
+    io(metric);
+    io(internal_metric);
+    io(leaf_size);
+    io(get_squares);
+    io(p);
+    io(weights);
+    io(outlier);
+    io(n_classes);
+    io(radius);
+    io(rcount);
+    io(rind);
+    io(rdist);
+    io(x_trained);
+    io(mfl_done);
+    io(mfl);
+    io(manual_checked);
+    io(manual_index);
+    io(manual_label);
+    io(mean_done);
+    io(mean_target);
+    io(manual_target);
+    if (s != da_status_success)
+        return s;
+    std::vector<T> X, yreg;
+    std::vector<da_int> ycls;
+    if (!load) {
+        if (trained_) {
+            std::vector<T> rows((size_t)n * d);
+            DA_HIP(err, hipMemcpyAsync(rows.data(), train_.ptr(), rows.size() * sizeof(T),
+                                       hipMemcpyDeviceToHost, gpu->stream()));
+            DA_HIP(err, hipStreamSynchronize(gpu->stream()));
+            X.resize(rows.size());
+            for (da_int i = 0; i < n; i++)
+                for (da_int l = 0; l < d; l++)
+                    X[(size_t)l * n + i] = rows[(size_t)i * d + l];
+        }
+        if (has_labels_)
+            ycls = y_class_;
+        if (has_targets_)
+            yreg = y_reg_;
+    }
+    io(X);
+    io(ycls);
+    io(yreg);
+    if (s != da_status_success || !load)
+        return s;
+    if (n < 0 || d < 0 || (int64_t)X.size() != (x_trained ? (int64_t)n * d : 0) ||
+        (has_labels && (da_int)ycls.size() != n) || (has_targets && (da_int)yreg.size() != n) ||
+        rind.size() != rcount.size() || (rnn_dist && rdist.size() != rcount.size()))
+        return da_status_invalid_file_data;
+    n_samples_ = n;
+    n_features_ = d;
+    trained_ = false;
+    stored_X_ = std::move(X);
+    has_labels_ = has_labels;
+    has_targets_ = has_targets;
+    y_class_ = std::move(ycls);
+    y_reg_ = std::move(yreg);
+    classes_ready_ = false;
+    rnn_done_ = !rcount.empty();
+    rnn_dist_ = rnn_dist;
+    rnn_count_ = rcount;
+    rnn_ind_.clear();
+    rnn_d_.clear();
+    for (size_t q = 0; q < rcount.size(); q++) {
+        if ((da_int)rind[q].size() != rcount[q] || (rnn_dist && (da_int)rdist[q].size() != rcount[q]))
+            return da_status_invalid_file_data;
+        rnn_ind_.insert(rnn_ind_.end(), rind[q].begin(), rind[q].end());
+        if (rnn_dist)
+            rnn_d_.insert(rnn_d_.end(), rdist[q].begin(), rdist[q].end());
+    }
+    model_trained = x_trained;
+    return da_status_success;
+}
+
+template <typename T> da_status NNHandle<T>::save_check() {
+    if (!trained_)
+        return err->error(da_status_no_data, "No training data have been set. Please call "
+                                             "da_nn_set_data_s or da_nn_set_data_d.");
+    return da_status_success;
+}
+
+// A loaded model searches again: the training rows go back to GPU memory through set_data
+// (column-major, as stored), with the labels and targets after them.
+template <typename T> da_status NNHandle<T>::restore() {
+    if (stored_X_.empty())
+        return da_status_success;
+    std::string ord;
+    (void)opts.get_string("storage order", ord);
+    (void)opts.set_string("storage order", "column-major");
+    std::vector<da_int> ycls = y_class_;
+    std::vector<T> yreg = y_reg_;
+    const bool lab = has_labels_, tgt = has_targets_;
+    da_status s = set_data(n_samples_, n_features_, stored_X_.data(), n_samples_);
+    (void)opts.set_string("storage order", ord);
+    if (s != da_status_success)
+        return s;
+    stored_X_.clear();
+    stored_X_.shrink_to_fit();
+    if (lab)
+        s = set_labels(n_samples_, ycls.data());
+    if (s == da_status_success && tgt)
+        s = set_targets(n_samples_, yreg.data());
+    return s;
+}
+
 template <typename T>
 da_status NNHandle<T>::get_result(da_result query, da_int *dim, da_int *result) {
     switch (query) {

@@ -67,6 +67,9 @@ template <typename T> class NNHandle : public AlgHandle {
     using AlgHandle::get_result;
     da_status get_result(da_result query, da_int *dim, da_int *result) override;
     da_status get_result(da_result query, da_int *dim, T *result) override;
+    da_status serialize(ModelBuffer &b) override;
+    da_status save_check() override;
+    da_status restore() override;
     void refresh() override {}
 
     // Search entry points shared with the serving loop. dQ: row-major n_features columns;
@@ -115,6 +118,8 @@ template <typename T> class NNHandle : public AlgHandle {
     da_status prepare_fused(int prec);
 
     bool trained_ = false;
+    // Training rows (column-major) read back from a stored model, until restore() uploads them.
+    std::vector<T> stored_X_;
     // Labels / targets: host copies, class list, per-row class index in GPU memory.
     bool has_labels_ = false, has_targets_ = false, classes_ready_ = false;
     std::vector<da_int> y_class_, classes_;

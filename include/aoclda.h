@@ -16,6 +16,7 @@
 #include "aoclda_options.h"
 #include "aoclda_result.h"
 #include "aoclda_types.h"
+#include "aoclda_utils.h"
 #include "aoclda_amd.h"
 
 #endif

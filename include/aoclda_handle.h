@@ -51,6 +51,18 @@ da_status da_handle_get_error_severity(da_handle handle, da_severity *severity);
 /* aoclda_handle.h:153 */
 void da_handle_refresh(da_handle handle);
 
+/*
+ * aoclda_handle.h:177, 201, 217: stored models. Saving writes the handle's options and
+ * model to file_name in the reference's stored-model format ("AOCLDA_STORED_MODEL"
+ * metadata, option registry, the algorithm's fields); loading creates *handle (which must be
+ * NULL) from such a file and rebuilds its GPU-resident state. Handles with models:
+ * da_handle_nn (training data, labels, targets, radius-search results), da_handle_kmeans
+ * (computed centres and labels) and da_handle_approx_nn (trained and filled index).
+ */
+da_status da_handle_save_model(da_handle handle, const char *file_name);
+da_status da_handle_load_model(da_handle *handle, const char *file_name);
+da_status da_handle_print_model_versions(da_handle handle);
+
 #ifdef __cplusplus
 }
 #endif
