@@ -369,22 +369,25 @@ hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const fl
 } // namespace dagpu
 
 namespace dagpu {
-// ---- MFMA k-means assignment (fp32 data) ----
-// C32[c*Kp + l] = (float)C[c*d + l] (zero padded), chalf[c] = -||C_c||^2/2, *cmax2 =
+// ---- MFMA k-means assignment (fp32 or fp64 data, bf16 cross term) ----
+// CP[c*Kp + l] = (E)C[c*d + l] (zero padded), chalf[c] = -||C_c||^2/2, *cmax2 =
 // max ||C_c||^2 (must be zeroed before).
 // sph: chalf = 0 (spherical k-means: the argmax of x.c, the reference's zero centre norms).
-hipError_t launch_centre_prep(const double *C, int k, int d, int Kp, float *C32, float *chalf,
+template <typename E>
+hipError_t launch_centre_prep(const double *C, int k, int d, int Kp, E *CP, E *chalf,
                               double *cmax2, hipStream_t s, int sph = 0);
-size_t kmeans_assign_mfma_lds(int k, int Kp);
-// X: n x Kp zero-padded fp32 rows, Kp in {16, 32, 48, 64, 96, 128}. Writes labels; samples
-// whose best two centres are within the fp32 error bound are appended to list (count in
-// *list_count, zeroed before) for launch_kmeans_recheck; the others update *changed.
-// wg_per_cu: 0 launches one workgroup per 256 samples; n > 0 at most n workgroups per CU,
-// each staging the centres once and taking every gridDim.x-th block of samples.
-hipError_t launch_kmeans_assign_mfma(const float *X, int n, int Kp, const float *C32,
-                                     const float *chalf, int k, double cmax, int *labels,
-                                     const int *prev, int *changed, int *list_count, int *list,
-                                     hipStream_t s, int wg_per_cu = 0);
+// X: n x Kp zero-padded rows of fp32 (fp32 MFMA) or fp64 (fp64 MFMA), Kp a multiple of 16
+// up to 128; CP, chalf: launch_centre_prep in the same type. Any k: centres that do not fit
+// a workgroup's LDS are staged in chunks. Writes labels; samples whose best two centres are
+// within the error bound of the MFMA arithmetic are appended to list (count in *list_count,
+// zeroed before) for launch_kmeans_recheck; the others update *changed. wg_per_cu: 0
+// launches one workgroup per 256 samples (128 for fp64 rows wider than 64); n > 0 at most n
+// workgroups per CU, each taking every gridDim.x-th block of samples.
+template <typename E>
+hipError_t launch_kmeans_assign_mfma(const E *X, int n, int Kp, const E *CP, const E *chalf,
+                                     int k, double cmax, int *labels, const int *prev,
+                                     int *changed, int *list_count, int *list, hipStream_t s,
+                                     int wg_per_cu = 0);
 // bf16 cross-term form of the same assignment: X and C16 are bf16 rows padded to Kp (a
 // multiple of 32, at most 128) features; C16 has (k+15)/16*16 rows.
 size_t kmeans_assign_bf16_lds(int k, int Kp);

@@ -200,18 +200,18 @@ template <typename T> da_status KMeansHandle<T>::finish_set_data() {
     mfma_ok_ = false;
     bf16_ok_ = false;
     xb_ready_ = false;
-    if constexpr (std::is_same_v<T, float>) {
-        // Padded copy for the MFMA assignment (feature count rounded up to 16, at most 128).
-        if (n_features <= 128) {
-            kp_ = (n_features + 15) / 16 * 16;
-            if (kp_ == 80 || kp_ == 112)
-                kp_ = kp_ == 80 ? 96 : 128;
-            DA_HIP(err, xpad_.reserve((size_t)n_samples * kp_ * sizeof(float)));
-            DA_HIP(err, (launch_pack_rows<float, float>(X_.as<float>(), n_features, n_samples,
-                                                        n_features, xpad_.as<float>(), kp_,
-                                                        n_samples, gpu->stream())));
-            DA_HIP(err, hipStreamSynchronize(gpu->stream()));
-            mfma_ok_ = true;
+    // Padded copy for the MFMA assignment in the data's precision (feature count rounded up
+    // to 16, at most 128).
+    if (n_features <= 128) {
+        kp_ = (n_features + 15) / 16 * 16;
+        if (kp_ == 80 || kp_ == 112)
+            kp_ = kp_ == 80 ? 96 : 128;
+        DA_HIP(err, xpad_.reserve((size_t)n_samples * kp_ * sizeof(T)));
+        DA_HIP(err, (launch_pack_rows<T, T>(X_.as<T>(), n_features, n_samples, n_features,
+                                            xpad_.as<T>(), kp_, n_samples, gpu->stream())));
+        DA_HIP(err, hipStreamSynchronize(gpu->stream()));
+        mfma_ok_ = true;
+        if constexpr (std::is_same_v<T, float>) {
             kp16_ = (n_features + 31) / 32 * 32;
             bf16_ok_ = true;
             xb_ready_ = false;
@@ -285,10 +285,11 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
     const int n = n_samples_, d = n_features_, k = n_clusters_;
     int *dchanged = reinterpret_cast<int *>(small_.as<double>() + 4);
     DA_HIP(err, hipMemsetAsync(dchanged, 0, 2 * sizeof(int), st));
-    const bool on_data = std::is_same_v<U, float> && (const void *)dX == (const void *)X_.ptr();
-    const bool use_bf16 = on_data && bf16_ok_ && opts.skey("cross term precision") == 1 &&
+    const bool on_data = std::is_same_v<U, T> && (const void *)dX == (const void *)X_.ptr();
+    const bool use_bf16 = on_data && std::is_same_v<U, float> && bf16_ok_ &&
+                          opts.skey("cross term precision") == 1 &&
                           kmeans_assign_bf16_lds(k, kp16_) <= 160 * 1024;
-    const bool use_mfma = on_data && mfma_ok_ && kmeans_assign_mfma_lds(k, kp_) <= 160 * 1024;
+    const bool use_mfma = on_data && mfma_ok_;
     if (use_bf16) {
         // bf16 cross term on MFMA, exact re-check of near ties (kmeans_mfma.hip).
         int *dlist_count = dchanged + 1;
@@ -323,19 +324,20 @@ da_status KMeansHandle<T>::lloyd_step(const U *dX, const double *Cin, double *Co
                                                  st, sph_));
         }
     } else if (use_mfma) {
-        // fp32 cross term on MFMA, exact re-check of near ties (kmeans_mfma.hip).
+        // Cross term on the fp32 or fp64 matrix cores, exact re-check of near ties
+        // (kmeans_mfma.hip).
         int *dlist_count = dchanged + 1;
         double *cmax2 = small_.as<double>() + 6;
         DA_HIP(err, hipMemsetAsync(cmax2, 0, sizeof(double), st));
-        DA_HIP(err, launch_centre_prep(Cin, k, d, kp_, c32_.as<float>(), chalf_.as<float>(), cmax2,
-                                       st, sph_));
+        DA_HIP(err, launch_centre_prep<T>(Cin, k, d, kp_, c32_.as<T>(), chalf_.as<T>(), cmax2,
+                                          st, sph_));
         double hc = 0;
         DA_HIP(err, hipMemcpyAsync(&hc, cmax2, sizeof(double), hipMemcpyDeviceToHost, st));
         DA_HIP(err, hipStreamSynchronize(st));
         {
             KernelScope ks(*gpu, "kmeans_assign");
-            DA_HIP(err, launch_kmeans_assign_mfma(xpad_.as<float>(), n, kp_, c32_.as<float>(),
-                                                  chalf_.as<float>(), k, std::sqrt(hc), lab_new,
+            DA_HIP(err, launch_kmeans_assign_mfma<T>(xpad_.as<T>(), n, kp_, c32_.as<T>(),
+                                                     chalf_.as<T>(), k, std::sqrt(hc), lab_new,
                                                   lab_old, dchanged, dlist_count, list_.as<int>(), st,
                                                   (int)opts.ival("assignment workgroups per cu")));
         }
@@ -938,8 +940,8 @@ template <typename T> da_status KMeansHandle<T>::compute() {
                                                 (da_int)(KMEANS_PASS_PAIRS / dl_)}));
     DA_HIP(err, part_.reserve((size_t)nblocks_ * kc_ * dl_ * 2 * sizeof(long long)));
     if (mfma_ok_) {
-        DA_HIP(err, c32_.reserve((size_t)(k + 16) * kp_ * sizeof(float)));
-        DA_HIP(err, chalf_.reserve((size_t)(k + 16) * sizeof(float)));
+        DA_HIP(err, c32_.reserve((size_t)(k + 16) * kp_ * sizeof(T)));
+        DA_HIP(err, chalf_.reserve((size_t)(k + 16) * sizeof(T)));
         DA_HIP(err, list_.reserve((size_t)n * sizeof(int)));
     }
     DA_HIP(err, pcnt_.reserve((size_t)nblocks_ * kc_ * sizeof(int)));

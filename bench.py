@@ -324,7 +324,9 @@ def kmeans_bench(args, L, aoclda):
     ref, ref_t = run(X.astype(np.float64), "working")
     ref_lab = ref.labels.copy()
     ref_inertia = float(ref.inertia)
-    log(f"kmeans: fp64 reference {ref_t:.2f}s, {ref.n_iter} iterations")
+    ref_kt = ref.kernel_times()
+    ref_step = sum(v[0] for v in ref_kt.values()) / max(1, int(ref.n_iter))
+    log(f"kmeans: fp64 reference {ref_t:.2f}s, {ref.n_iter} iterations, step {ref_step:.3f} ms")
     forms = {}
     for form in args.km_forms.split(","):
         km, el = run(X, form)
@@ -360,7 +362,9 @@ def kmeans_bench(args, L, aoclda):
         "data": "synthetic: 256 Gaussian blobs, numpy default_rng(3)",
         "config": {"workload": f"k-means Lloyd, {n}x{d} fp32, k={k}, k-means++ seed 42, "
                                f"{args.km_iter} iterations, tol 0", "precision_form": head},
-        "fp64_reference_fit_s": ref_t, "forms": forms, "roofline": roof}))
+        "fp64_reference_fit_s": ref_t, "fp64_lloyd_step_ms": ref_step,
+        "fp64_kernels_ms": {nm: {"total_ms": v[0], "launches": v[1]} for nm, v in ref_kt.items()},
+        "forms": forms, "roofline": roof}))
 
 
 BATCH_METRIC = ("batched uneven k-means: Lloyd iterations per second over 1,024 instances in "

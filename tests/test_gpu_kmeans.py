@@ -164,6 +164,29 @@ def test_larger_mfma_vs_oracle(oracle):
     np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=1e-5, atol=1e-5)
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("d,k", [(32, 40), (32, 300), (96, 260), (128, 90), (5, 700)])
+def test_mfma_assignment_any_k_vs_oracle(oracle, dtype, d, k):
+    """The MFMA assignment in the data's precision (fp64 matrix cores for double data) with
+    the centres staged in LDS chunks when they exceed a workgroup's share (k x Kp elements
+    above 80 KB: fp64 at (32, 300), (96, 260), (128, 90); fp32 at (96, 260), (5, 700)
+    needs no chunk), against the oracle: labels, iteration count, centres and inertia."""
+    rng = np.random.default_rng(d * 1000 + k)
+    centres = rng.uniform(-4, 4, (k, d))
+    X = blobs(rng, max(3, 3000 // k), centres, sigma=1.0).astype(dtype)
+    rng.shuffle(X)
+    C0 = X[:k].copy()
+    km = aoclda.clustering.kmeans(n_clusters=k, C=C0, max_iter=12, tol=0.0).fit(X)
+    r = oracle.kmeans(X.astype(np.float64), k, "supplied", C0.astype(np.float64), max_iter=12,
+                      tol=0.0)
+    np.testing.assert_array_equal(km.labels, r["labels"])
+    assert km.n_iter == r["n_iter"]
+    tol = 1e-12 if dtype == np.float64 else 1e-5
+    np.testing.assert_allclose(km.cluster_centres, r["centres"], rtol=tol, atol=tol)
+    assert abs(km.inertia - r["inertia"]) <= tol * max(1.0, abs(r["inertia"]))
+    np.testing.assert_array_equal(km.predict(X[: 3 * k]), r["labels"][: 3 * k])
+
+
 @pytest.mark.parametrize("d", [8, 32, 40, 96])
 def test_bf16_cross_term_matches_exact(oracle, d):
     """The bf16 cross term with exact re-check of near ties gives the labels, iteration count
