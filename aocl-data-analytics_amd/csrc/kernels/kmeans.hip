@@ -724,6 +724,81 @@ __global__ __launch_bounds__(256) void candidate_cost_kernel(const T *__restrict
     }
 }
 
+// The same costs for rows of exactly D features (D * sizeof(T) a multiple of 16 bytes): each
+// sample's row is read once into registers with 16-byte loads and every candidate's distance
+// is formed from there, with the operations and their order of candidate_cost_kernel, so the
+// costs are bitwise the same. The candidates are staged in LDS (nt * D <= ROW_STAGE_MAX).
+template <typename T, int D>
+__global__ __launch_bounds__(256) void candidate_cost_reg_kernel(
+    const T *__restrict__ X, int n, const double *__restrict__ w, const int *__restrict__ cand,
+    int nt, double *__restrict__ part, int nb, KmGeom g) {
+    extern __shared__ double cs[]; // nt x D
+    __shared__ double red[256];
+    for (int t = threadIdx.x; t < nt * D; t += 256) {
+        const int c = t / D, l = t - c * D;
+        cs[t] = (double)X[(int64_t)cand[c] * D + l];
+    }
+    __syncthreads();
+    constexpr int VE = 16 / (int)sizeof(T); // elements per 16-byte load
+    static_assert(D % VE == 0, "whole 16-byte loads per row");
+    double acc[NT_MAX];
+#pragma unroll
+    for (int c = 0; c < NT_MAX; c++)
+        acc[c] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        double xr[D];
+        const T *row = X + i * D;
+#pragma unroll
+        for (int l = 0; l < D; l += VE) {
+            if constexpr (sizeof(T) == 4) {
+                const float4 v = *reinterpret_cast<const float4 *>(row + l);
+                xr[l] = v.x;
+                xr[l + 1] = v.y;
+                xr[l + 2] = v.z;
+                xr[l + 3] = v.w;
+            } else {
+                const double2 v = *reinterpret_cast<const double2 *>(row + l);
+                xr[l] = v.x;
+                xr[l + 1] = v.y;
+            }
+        }
+        const double wi = w[i];
+#pragma unroll
+        for (int c = 0; c < NT_MAX; c++) {
+            if (c < nt) {
+                const double *cc = cs + c * D;
+                double v = 0.0;
+                if (g.sph) {
+#pragma unroll
+                    for (int l = 0; l < D; l++)
+                        v = fma(xr[l], cc[l], v);
+                    v = cosine_dist(v, g, i, (int64_t)cand[c]);
+                } else {
+#pragma unroll
+                    for (int l = 0; l < D; l++) {
+                        const double t = xr[l] - cc[l];
+                        v = fma(t, t, v);
+                    }
+                }
+                v = wi < v ? wi : v;
+                acc[c] += v > 0.0 ? v : 0.0;
+            }
+        }
+    }
+    for (int c = 0; c < nt; c++) {
+        red[threadIdx.x] = acc[c];
+        __syncthreads();
+        for (int m = 128; m > 0; m >>= 1) {
+            if ((int)threadIdx.x < m)
+                red[threadIdx.x] += red[threadIdx.x + m];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            part[(int64_t)c * nb + blockIdx.x] = red[0];
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(256) void reduce_rows_kernel(const double *__restrict__ part, int nt,
                                                           int nb, double *__restrict__ out) {
     __shared__ double s[256];
@@ -976,9 +1051,23 @@ hipError_t launch_candidate_cost(const T *X, int n, int d, const double *w, cons
                                  const double *inv) {
     if (nt > NT_MAX)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(candidate_cost_kernel<T>, dim3(nb), dim3(256),
-                       row_stage_bytes((int64_t)nt * d), s, X, n, d, w, cand, nt, part, nb,
-                       KmGeom{sph, inv});
+    const KmGeom geo{sph, inv};
+    const size_t lds = row_stage_bytes((int64_t)nt * d);
+    if ((int64_t)nt * d <= ROW_STAGE_MAX) {
+#define DA_CC(DV)                                                                                 \
+    if (d == DV) {                                                                                \
+        hipLaunchKernelGGL((candidate_cost_reg_kernel<T, DV>), dim3(nb), dim3(256), lds, s, X, n, \
+                           w, cand, nt, part, nb, geo);                                           \
+        return hipGetLastError();                                                                 \
+    }
+        DA_CC(8)
+        DA_CC(16)
+        DA_CC(32)
+        DA_CC(64)
+#undef DA_CC
+    }
+    hipLaunchKernelGGL(candidate_cost_kernel<T>, dim3(nb), dim3(256), lds, s, X, n, d, w, cand,
+                       nt, part, nb, geo);
     return hipGetLastError();
 }
 

@@ -675,7 +675,9 @@ template <typename T> da_status KMeansHandle<T>::kmeans_plusplus() {
         for (int c = 0; c < k; c++)
             chosen[c] = c;
     } else {
-        const int nb = 256;
+        // Partial sums of the candidate costs: one per workgroup, about 1,024 samples each, at
+        // most 2,048 (their n_trials * nb partials fit the n doubles of cp_, n_trials <= 32).
+        const int nb = (int)std::max<int64_t>(1, std::min<int64_t>(2048, (int64_t)n / 1024));
         std::vector<double> u(n_trials), cost(n_trials);
         std::vector<int> cand(n_trials);
         double *du = tmp_.as<double>() + 1024;
