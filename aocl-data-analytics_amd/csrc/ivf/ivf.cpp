@@ -268,11 +268,43 @@ da_status ApproxNNHandle<T>::add(da_int n_add, da_int n_features, const T *X, da
     DA_HIP(err, lab_d_.reserve((size_t)n_add * sizeof(int)));
     {
         // Nearest centroid: squared distance, or the largest x.c for the inner-product and
-        // cosine metrics (the GEMM with alpha = -1 of :715-731).
+        // cosine metrics (the GEMM with alpha = -1 of :715-731), ties to the lower centroid.
+        // Rows of up to 128 features: the k-means assignment on the matrix cores in the data's
+        // precision with the exact re-check of near ties (the same labels as the exact kernel).
         KernelScope ks(*gpu, "ivf_assign");
-        DA_HIP(err, launch_kmeans_assign<T>(added, n_add, n_features_, cdouble_d_.as<double>(),
-                                            n_list_, lab_d_.as<int>(), nullptr, nullptr, nullptr,
-                                            st, metric_ == ANN_INNER || metric_ == ANN_COSINE));
+        const int sph = metric_ == ANN_INNER || metric_ == ANN_COSINE;
+        const int d = (int)n_features_;
+        if (d <= 128) {
+            int kp = (d + 15) / 16 * 16;
+            if (kp == 80 || kp == 112)
+                kp = kp == 80 ? 96 : 128;
+            DA_HIP(err, as_x_.reserve((size_t)n_add * kp * sizeof(T)));
+            DA_HIP(err, as_c_.reserve((size_t)(n_list_ + 16) * kp * sizeof(T)));
+            DA_HIP(err, as_h_.reserve((size_t)(n_list_ + 16) * sizeof(T)));
+            DA_HIP(err, as_small_.reserve(4 * sizeof(double)));
+            DA_HIP(err, as_list_.reserve((size_t)n_add * sizeof(int)));
+            DA_HIP(err, as_ct_.reserve((size_t)n_list_ * d * sizeof(double)));
+            double *cmax2 = as_small_.as<double>();
+            int *cnt = reinterpret_cast<int *>(as_small_.as<double>() + 1); // changed, list count
+            DA_HIP(err, hipMemsetAsync(as_small_.ptr(), 0, 4 * sizeof(double), st));
+            DA_HIP(err, (launch_pack_rows<T, T>(added, d, n_add, d, as_x_.as<T>(), kp, n_add, st)));
+            DA_HIP(err, launch_centre_prep<T>(cdouble_d_.as<double>(), n_list_, d, kp,
+                                              as_c_.as<T>(), as_h_.as<T>(), cmax2, st, sph));
+            double hc = 0;
+            DA_HIP(err, hipMemcpyAsync(&hc, cmax2, sizeof(double), hipMemcpyDeviceToHost, st));
+            DA_HIP(err, hipStreamSynchronize(st));
+            DA_HIP(err, launch_kmeans_assign_mfma<T>(as_x_.as<T>(), n_add, kp, as_c_.as<T>(),
+                                                     as_h_.as<T>(), n_list_, std::sqrt(hc),
+                                                     lab_d_.as<int>(), nullptr, cnt, cnt + 1,
+                                                     as_list_.as<int>(), st));
+            DA_HIP(err, launch_kmeans_recheck<T>(added, d, d, cdouble_d_.as<double>(), n_list_,
+                                                 cnt + 1, as_list_.as<int>(), lab_d_.as<int>(),
+                                                 nullptr, cnt, as_ct_.as<double>(), st, sph));
+        } else {
+            DA_HIP(err, launch_kmeans_assign<T>(added, n_add, d, cdouble_d_.as<double>(), n_list_,
+                                                lab_d_.as<int>(), nullptr, nullptr, nullptr, st,
+                                                sph));
+        }
     }
     std::vector<int> lab(n_add);
     DA_HIP(err, hipMemcpyAsync(lab.data(), lab_d_.ptr(), (size_t)n_add * sizeof(int),

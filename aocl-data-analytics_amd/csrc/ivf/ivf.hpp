@@ -47,6 +47,9 @@ template <typename T> class ApproxNNHandle : public AlgHandle {
     std::vector<da_int> list_sizes_;
     std::vector<std::vector<int>> lists_;
     da_int n_index_ = 0;
+    // Buffers of the MFMA assignment of added vectors: padded rows, padded centroids and
+    // their -|c|^2/2, counters, the re-check list and the transposed centroids.
+    DeviceBuffer as_x_, as_c_, as_h_, as_small_, as_list_, as_ct_;
     DeviceBuffer C_d_, X_d_, X_spare_, perm_d_, off_d_, q_d_, q_stage_, ckeys_, probe_key_,
         probe_d_, seg_d_, cand_key_, cand_idx_, sel_key_, sel_idx_, out_ind_, out_dist_, lab_d_,
         cdouble_d_;
