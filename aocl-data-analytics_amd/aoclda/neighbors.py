@@ -18,8 +18,8 @@ class nearest_neighbors:
         # centres); results are re-ranked exactly whatever the
         # choice. candidates_per_split: candidate-list length per query and split of
         # the fused tile (0 = from k; shorter than k is allowed and stays exact).
-        # seed_thresholds: bound the tile's thresholds with a sweep of
-        # a row subset first. All three are MI355X options that change speed only.
+        # seed_thresholds: bound the tile's thresholds (k <= 16) by the group
+        # minima over a row sample first. All three are MI355X options that change speed only.
         if not isinstance(n_neighbors, int) and n_neighbors is not None:
             raise ValueError("Number of neighbors must be an integer or None. Got n_neighbors="
                              + str(n_neighbors))

@@ -348,13 +348,15 @@ __device__ __forceinline__ void take_hits(const typename Op::acc_t (&acc)[ROWT][
 }
 
 // Final compaction of a wave's queries and the per-(query, split) output: the KC best keys
-// and row indices (t_begin + buffered offset) and the split threshold tau (every key of the
-// split not kept is >= tau): the KC-th kept key, or the seeded threshold tau0 where that is
-// lower (a row at or above the seed was never buffered).
+// and row indices and the split threshold tau (every key of the split not kept is >= tau):
+// the KC-th kept key, or the seeded threshold tau0 where that is lower (a row at or above the
+// seed was never buffered). A buffered offset b is row t_begin + b of a contiguous split, or,
+// with interleaved splits (split s sweeps the 32-row steps s, s + nsplit, ...), row
+// ((b / 32) * nsplit + s) * 32 + b % 32.
 template <class Op, int KC, int CAP, int ROWT, typename IT>
 __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bkey, IT *bidx,
-                                            int t_begin, int qbase, int q0, int nq, int nsplit,
-                                            int split,
+                                            int t_begin, bool interleaved, int qbase, int q0,
+                                            int nq, int nsplit, int split,
                                             float *__restrict__ part_key,
                                             int *__restrict__ part_idx,
                                             float *__restrict__ part_tau,
@@ -377,7 +379,10 @@ __device__ __forceinline__ void write_split(const int (&cq)[ROWT][4], float *bke
                 const int valid = c < KC ? c : KC;
                 for (int u = lane; u < KC; u += 64) {
                     part_key[o + u] = u < valid ? bkey[ql * CAP + u] : __builtin_huge_valf();
-                    const int row = t_begin + (int)bidx[ql * CAP + u];
+                    const int bo = (int)bidx[ql * CAP + u];
+                    const int row = interleaved
+                                        ? ((bo / STEP) * nsplit + split) * STEP + (bo & (STEP - 1))
+                                        : t_begin + bo;
                     part_idx[o + u] = u < valid ? (perm ? perm[row] : row) : -1;
                 }
                 if (lane == 0)
@@ -543,31 +548,37 @@ __device__ __forceinline__ void knn_tile_body(
         // with immediate offsets sit between the MFMA groups: no VALU instruction writes a
         // register while the MFMAs are in flight. Each MFMA reads its A and B operands when
         // it issues, so a fragment can be reloaded once every MFMA that reads it has issued.
-        if (t_begin < t_end) {
+        // Centred rows: the splits are interleaved (split s sweeps the steps s, s + nsplit,
+        // ... of the whole set), so that the rows of one group, which are stored together
+        // and hold most of a query's nearest rows, are spread over the splits. The products
+        // of this wave's queries with the centre of a step's group (a step lies in one group)
+        // are loaded a step ahead, with the step's operands, when the group changes.
+        const int first = CEN ? split * TSTEP : t_begin;
+        const int last = CEN ? n : t_end;
+        const int stride = CEN ? nsplit * TSTEP : TSTEP;
+        if (first < last) {
             frag_t b[COLT][NS][NF];
             ae hv[COLT];
-            load_step(t_begin, b, hv);
-            // Centred rows: the products of this wave's queries with the centre of the group
-            // the current step belongs to (a step lies in one group), reloaded when the group
-            // changes.
+            load_step(first, b, hv);
             ae qcv[ROWT][4] = {};
             int cur_g = -1;
-            for (int tb = t_begin; tb < t_end; tb += TSTEP) {
-                const int nb = tb + TSTEP < t_end ? tb + TSTEP : tb;
-                const in_t *ntile = Y + (int64_t)nb * rowlen + lane * CSR;
-                if constexpr (CEN) {
-                    const int grp = cen.step_group[tb / TSTEP];
-                    if (grp != cur_g) {
-                        cur_g = grp;
+            auto load_qc = [&](int grp, ae (&dst)[ROWT][4]) {
 #pragma unroll
-                        for (int rt = 0; rt < ROWT; rt++)
+                for (int rt = 0; rt < ROWT; rt++)
 #pragma unroll
-                            for (int i = 0; i < 4; i++) {
-                                const int q = q0 + qbase + rt * 16 + 4 * g + i;
-                                qcv[rt][i] = q < nq ? cen.qc[(int64_t)q * cen.G + grp] : (ae)0;
-                            }
+                    for (int i = 0; i < 4; i++) {
+                        const int q = q0 + qbase + rt * 16 + 4 * g + i;
+                        dst[rt][i] = q < nq ? cen.qc[(int64_t)q * cen.G + grp] : (ae)0;
                     }
-                }
+            };
+            if constexpr (CEN) {
+                cur_g = cen.step_group[first / TSTEP];
+                load_qc(cur_g, qcv);
+            }
+            int off = 0; // buffered offset of the step's first row
+            for (int tb = first; tb < last; tb += stride, off += TSTEP) {
+                const int nb = tb + stride < last ? tb + stride : tb;
+                const in_t *ntile = Y + (int64_t)nb * rowlen + lane * CSR;
                 acc_t acc[ROWT][COLT];
 #pragma unroll
                 for (int ct = 0; ct < COLT; ct++)
@@ -582,6 +593,13 @@ __device__ __forceinline__ void knn_tile_body(
 #pragma unroll
                 for (int ct = 0; ct < COLT; ct++)
                     hv[ct] = nyh[nb + ct * 16 + r];
+                if constexpr (CEN) {
+                    const int ng = cen.step_group[nb / TSTEP];
+                    if (ng != cur_g) {
+                        cur_g = ng;
+                        load_qc(ng, qcv);
+                    }
+                }
                 __builtin_amdgcn_sched_barrier(0);
                 constexpr int FPC = Op::FPC;
 #pragma unroll
@@ -610,8 +628,7 @@ __device__ __forceinline__ void knn_tile_body(
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 if (any_hit(acc))
-                    take_hits<Op, KC, CAP, ROWT, COLT>(acc, ts, cq, bkey, bidx, qbase,
-                                                       tb - t_begin);
+                    take_hits<Op, KC, CAP, ROWT, COLT>(acc, ts, cq, bkey, bidx, qbase, off);
             }
         }
     } else if constexpr (PF == 1) {
@@ -650,7 +667,7 @@ __device__ __forceinline__ void knn_tile_body(
             check_step(tb, acc);
         }
     }
-    write_split<Op, KC, CAP, ROWT>(cq, bkey, bidx, t_begin, qbase, q0, nq, nsplit, split,
+    write_split<Op, KC, CAP, ROWT>(cq, bkey, bidx, t_begin, CEN, qbase, q0, nq, nsplit, split,
                                    part_key, part_idx, part_tau, tau0, CEN ? cen.perm : nullptr);
 }
 
@@ -679,8 +696,140 @@ knn_tile_kernel_k2(const typename Op::in_t *__restrict__ Q, int nq,
                                                      part_key, part_idx, part_tau, tau0, cen);
 }
 
+// Threshold seeds by group minima. The same operand layout and MFMA products as the sweep,
+// over a small sample of training rows split like the sweep, but no candidate lists: lane
+// (r, g) keeps, for each of its queries, the largest score over the sample rows whose offset
+// in the split is r mod 16, so every query has 16 disjoint row groups and the best row of
+// each. The k-th smallest of the 16 group-best keys is attained by k distinct rows, so it
+// bounds the query's k-th key over the whole training set from above (k <= 16); written to
+// part_tau[q * nsplit + split] for launch_knn_seed. Fixed shape: four waves of 32 queries,
+// two 16-row training tiles per step, loads then products.
+template <class Op, int KCH, bool SPLIT>
+__global__ __launch_bounds__(256) void knn_seed_min_kernel(
+    const typename Op::in_t *__restrict__ Q, int nq, const typename Op::in_t *__restrict__ Y,
+    const typename Op::acc_e *__restrict__ nyh, int n, int Kp, int split_len, int nsplit, int k,
+    float *__restrict__ part_tau) {
+    using frag_t = typename Op::frag_t;
+    using acc_t = typename Op::acc_t;
+    using ae = typename Op::acc_e;
+    using in_t = typename Op::in_t;
+    constexpr int NF = KCH / Op::E;
+    constexpr int NS = SPLIT ? 2 : 1;
+    constexpr int CSR = 16 / (int)sizeof(in_t);
+    constexpr int CST = 1024 / (int)sizeof(in_t);
+    constexpr int ROWT = 2, COLT = 2, QB = 4 * ROWT * 16, TSTEP = COLT * 16;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    const int qblocks = (nq + QB - 1) / QB;
+    const int work = (int)blockIdx.x;
+    if (work >= qblocks * nsplit)
+        return;
+    const int split = work / qblocks;
+    const int q0 = (work - split * qblocks) * QB;
+    const int t_begin = split * split_len;
+    const int t_end = t_begin + split_len < n ? t_begin + split_len : n;
+    const int qbase = (w * ROWT) * 16;
+    const int64_t rowlen = (int64_t)Kp * NS;
+    frag_t a[NS][ROWT][NF];
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++) {
+        const int qr = q0 + qbase + rt * 16 + r;
+#pragma unroll
+        for (int h = 0; h < NS; h++) {
+            Op::template load<NF, CSR>(Q + (int64_t)(qr < nq ? qr : 0) * rowlen + h * Kp + g * KCH,
+                                       a[h][rt]);
+            if (qr >= nq) {
+#pragma unroll
+                for (int f = 0; f < NF; f++)
+                    a[h][rt][f] = frag_t{};
+            }
+        }
+    }
+    ae best[ROWT][4];
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            best[rt][i] = (ae)-__builtin_huge_val();
+    for (int tb = t_begin; tb < t_end; tb += TSTEP) {
+        frag_t b[COLT][NS][NF];
+        ae hv[COLT];
+#pragma unroll
+        for (int ct = 0; ct < COLT; ct++) {
+            hv[ct] = nyh[tb + ct * 16 + r];
+            const in_t *tile = Y + (int64_t)(tb + ct * 16) * rowlen + lane * CSR;
+#pragma unroll
+            for (int h = 0; h < NS; h++)
+                Op::template load<NF, CST>(tile + h * 16 * Kp, b[ct][h]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        acc_t acc[ROWT][COLT];
+#pragma unroll
+        for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+            for (int rt = 0; rt < ROWT; rt++)
+                acc[rt][ct] = acc_t{hv[ct], hv[ct], hv[ct], hv[ct]};
+#pragma unroll
+        for (int f = 0; f < NF; f++)
+#pragma unroll
+            for (int ct = 0; ct < COLT; ct++)
+#pragma unroll
+                for (int rt = 0; rt < ROWT; rt++) {
+                    acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][0][f], acc[rt][ct]);
+                    if constexpr (SPLIT) {
+                        acc[rt][ct] = Op::mfma(a[0][rt][f], b[ct][NS - 1][f], acc[rt][ct]);
+                        acc[rt][ct] = Op::mfma(a[NS - 1][rt][f], b[ct][0][f], acc[rt][ct]);
+                    }
+                }
+#pragma unroll
+        for (int rt = 0; rt < ROWT; rt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int ct = 0; ct < COLT; ct++)
+                    best[rt][i] = acc[rt][ct][i] > best[rt][i] ? acc[rt][ct][i] : best[rt][i];
+    }
+    // Sort each 16-lane group's keys ascending (bitonic; the last merge ascending in every
+    // group) and take the k-th.
+#pragma unroll
+    for (int rt = 0; rt < ROWT; rt++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float key = (float)(-2.0 * (double)best[rt][i]);
+            int idx = r;
+            sort_merge<2, 1>(key, idx, lane);
+            sort_merge<4, 2>(key, idx, lane);
+            sort_merge<8, 4>(key, idx, lane);
+            sort_merge<64, 8>(key, idx, lane);
+            const float kth = __shfl(key, (lane & 48) + k - 1, 64);
+            const int q = q0 + qbase + rt * 16 + (Op::rows_strided ? g + 4 * i : 4 * g + i);
+            if (r == 0 && q < nq)
+                part_tau[(int64_t)q * nsplit + split] = kth;
+        }
+    }
+}
+
 constexpr int MMAX = 2048; // candidates merged per query (nsplit * kc)
 constexpr int RMAX = 256; // candidates re-ranked exactly per query
+
+// Error bound of a stored key |y|^2 - 2 q.y of the fused tile. The tile accumulates the
+// products of the query with the operand rows r (the rows themselves, or their residuals about
+// a group centre c) from h = -|y|^2/2 (+ q.c for a centred form). With S = sum|q_l r_l| <=
+// |q| rmax and |h| <= H = ymax^2/2 + |q| cmax: the operands rounded to the input precision
+// (2u_in + u_in^2) S; the d+1-term accumulation from h (gamma_{d+2}); the rounding of h (of
+// -|y|^2/2 alone, or of -|y|^2/2, q.c and their sum); the float storage of the key; the
+// absolute error of operands below the normal range of fp16.
+__device__ double key_error(double qnorm, int d, const KnnKeyError &e) {
+    const double S = qnorm * e.rmax;
+    const double H = 0.5 * e.ymax * e.ymax + qnorm * e.cmax;
+    const double gam = (d + 2) * e.u_acc / (1.0 - (d + 2) * e.u_acc);
+    const double u32 = 1.0 / 16777216.0;
+    const double herr = e.cmax > 0.0 ? 2.0 * u32 * H : e.u_acc * 0.5 * e.ymax * e.ymax;
+    return 2.0 * ((2.0 * e.u_in + e.u_in * e.u_in) * S +
+                  gam * (S * (1.0 + e.u_in) * (1.0 + e.u_in) + H) + herr) +
+           u32 * (2.0 * H + 2.01 * S) + 2.0 * e.abs_in * sqrt((double)d) * (qnorm + e.rmax) * 1.01;
+}
 
 // MM: candidate capacity of this instance (a power of two >= nsplit * kc).
 template <typename T, int MM>
@@ -690,20 +839,38 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
     const T *__restrict__ Yrows, int d, int k, double u_in, double abs_in, double u_acc,
     double ymax, double rmax, double cmax, int post,
     T *__restrict__ out_dist, int *__restrict__ out_idx, int *__restrict__ fb_count,
-    int *__restrict__ fb_list) {
+    int *__restrict__ fb_list, float *__restrict__ fb_bound) {
     __shared__ float skey[MM];
     __shared__ int sidx[MM];
     __shared__ double qrow[1024];
+    __shared__ int scount;
     const int q = blockIdx.x;
     const int lane = threadIdx.x;
     const int m = nsplit * kc; // <= MM
     const int64_t base = (int64_t)q * m;
-    for (int t = lane; t < MM; t += 64) {
-        skey[t] = t < m ? part_key[base + t] : __builtin_huge_valf();
-        sidx[t] = t < m ? part_idx[base + t] : INT_MAX;
-        if (skey[t] != skey[t]) {
-            skey[t] = __builtin_huge_valf();
+    // The stored candidates (finite keys; unfilled list entries are +inf) packed to the front,
+    // in any order: the sort below orders them by (key, index), and a row appears in one
+    // list at most, so the result does not depend on the packing order. Only the next power
+    // of two (at least 64 and at least k) is sorted.
+    if (lane == 0)
+        scount = 0;
+    __syncthreads();
+    for (int t = lane; t < m; t += 64) {
+        const float kv = part_key[base + t];
+        if (kv < __builtin_huge_valf()) {
+            const int p = atomicAdd(&scount, 1);
+            skey[p] = kv;
+            sidx[p] = part_idx[base + t];
         }
+    }
+    __syncthreads();
+    const int cnt = scount;
+    int ms = 64;
+    while (ms < cnt || ms < k)
+        ms <<= 1;
+    for (int t = cnt + lane; t < ms; t += 64) {
+        skey[t] = __builtin_huge_valf();
+        sidx[t] = INT_MAX;
     }
     float tau = __builtin_huge_valf();
     for (int s = lane; s < nsplit; s += 64)
@@ -720,9 +887,9 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
         qn += __shfl_xor(qn, off, 64);
     __syncthreads();
     // Bitonic sort of the (key, index) candidates, ascending.
-    for (int size = 2; size <= MM; size <<= 1) {
+    for (int size = 2; size <= ms; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int t = lane; t < MM / 2; t += 64) {
+            for (int t = lane; t < ms / 2; t += 64) {
                 const int i = 2 * t - (t & (stride - 1));
                 const int j = i + stride;
                 const bool asc = (i & size) == 0;
@@ -739,35 +906,33 @@ __global__ __launch_bounds__(64) void knn_merge_kernel(
             __syncthreads();
         }
     }
-    // Error bound of the stored key |y|^2 - 2 q.y. The tile accumulates the products of
-    // the query with the operand rows r (the rows themselves, or their residuals about a
-    // group centre c) from h = -|y|^2/2 (+ q.c for a centred form). With S = sum|q_l r_l|
-    // <= |q| rmax and |h| <= H = ymax^2/2 + |q| cmax: the operands rounded to the input
-    // precision (2u_in + u_in^2) S; the d+1-term accumulation from h (gamma_{d+2}); the
-    // rounding of h (of -|y|^2/2 alone, or of -|y|^2/2, q.c and their sum); the float
-    // storage of the key.
-    const double qnorm = sqrt(qn);
-    const double S = qnorm * rmax;
-    const double H = 0.5 * ymax * ymax + qnorm * cmax;
-    const double gam = (d + 2) * u_acc / (1.0 - (d + 2) * u_acc);
-    const double u32 = 1.0 / 16777216.0;
-    const double herr = cmax > 0.0 ? 2.0 * u32 * H : u_acc * 0.5 * ymax * ymax;
-    const double E = 2.0 * ((2.0 * u_in + u_in * u_in) * S +
-                            gam * (S * (1.0 + u_in) * (1.0 + u_in) + H) + herr) +
-                     u32 * (2.0 * H + 2.01 * S) +
-                     // operands below the normal range of fp16 carry an absolute error
-                     2.0 * abs_in * sqrt((double)d) * (qnorm + rmax) * 1.01;
+    // Error bound of the stored keys.
+    KnnKeyError ke;
+    ke.u_in = u_in;
+    ke.abs_in = abs_in;
+    ke.u_acc = u_acc;
+    ke.ymax = ymax;
+    ke.rmax = rmax;
+    ke.cmax = cmax;
+    const double E = key_error(sqrt(qn), d, ke);
     const float ak = skey[k - 1];
     const double lim = (double)ak + 2.0 * E;
     // Candidates within 2E of the k-th key, re-ranked with exact distances.
     int nr = 0;
-    while (nr < MM && (double)skey[nr] <= lim)
+    while (nr < ms && (double)skey[nr] <= lim)
         nr++;
     const bool certified = (double)tau > lim && nr <= RMAX;
     if (!certified) {
         if (lane == 0) {
             const int p = atomicAdd(fb_count, 1);
             fb_list[p] = q;
+            if (fb_bound) {
+                const double b = (double)ak + 4.0 * E;
+                float bf = (float)b;
+                if ((double)bf < b)
+                    bf = nextafterf(bf, __builtin_huge_valf());
+                fb_bound[p] = bf;
+            }
         }
         return;
     }
@@ -1041,6 +1206,47 @@ hipError_t launch_knn_tile(int prec, const void *Q, int nq, const void *Y, const
                          tau0, variant, nullptr, s, cen);
 }
 
+template <class Op, bool SPLIT>
+static hipError_t launch_seed_op(const void *Q, int nq, const void *Y, const void *nyh, int n,
+                                 int Kp, int nsplit, int k, float *part_tau, hipStream_t s) {
+    using in_t = typename Op::in_t;
+    using ae = typename Op::acc_e;
+    using kernel_t = void (*)(const in_t *, int, const in_t *, const ae *, int, int, int, int,
+                              int, float *);
+    kernel_t kern;
+    switch (Kp / 4) {
+    case 8: kern = knn_seed_min_kernel<Op, 8, SPLIT>; break;
+    case 16: kern = knn_seed_min_kernel<Op, 16, SPLIT>; break;
+    case 24: kern = knn_seed_min_kernel<Op, 24, SPLIT>; break;
+    case 32: kern = knn_seed_min_kernel<Op, 32, SPLIT>; break;
+    default: return hipErrorInvalidValue;
+    }
+    int split_len = (n + nsplit - 1) / nsplit;
+    split_len = (split_len + STEP - 1) / STEP * STEP;
+    const int qblocks = (nq + 127) / 128;
+    hipLaunchKernelGGL(kern, dim3(qblocks * nsplit), dim3(256), 0, s,
+                       static_cast<const in_t *>(Q), nq, static_cast<const in_t *>(Y),
+                       static_cast<const ae *>(nyh), n, Kp, split_len, nsplit, k, part_tau);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_seed_min(int prec, const void *Q, int nq, const void *Y, const void *nyh,
+                               int n, int Kp, int nsplit, int k, float *part_tau,
+                               hipStream_t s) {
+    if (k < 1 || k > 16 || n <= 0 || n % STEP != 0 || nsplit < 1 || Kp % 32 != 0)
+        return hipErrorInvalidValue;
+    if (nq <= 0)
+        return hipSuccess;
+    switch (prec) {
+    case KP_F32: return launch_seed_op<OpF32, false>(Q, nq, Y, nyh, n, Kp, nsplit, k, part_tau, s);
+    case KP_F16: return launch_seed_op<OpF16, false>(Q, nq, Y, nyh, n, Kp, nsplit, k, part_tau, s);
+    case KP_BF16: return launch_seed_op<OpBF16, false>(Q, nq, Y, nyh, n, Kp, nsplit, k, part_tau, s);
+    case KP_F16X3: return launch_seed_op<OpF16, true>(Q, nq, Y, nyh, n, Kp, nsplit, k, part_tau, s);
+    case KP_BF16X3: return launch_seed_op<OpBF16, true>(Q, nq, Y, nyh, n, Kp, nsplit, k, part_tau, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t knn_tile_slots(int prec, int Kp, int kc, int variant, int *slots) {
     // The centred form's kernel has the registers and LDS of the plain fp16 sweep.
     static const KnnCentring probe{};
@@ -1105,14 +1311,29 @@ __global__ __launch_bounds__(256) void knn_query_centre_kernel(const T *__restri
 
 // tau0[q] = the smallest split threshold of a sweep over a subset of the rows: the subset
 // holds kc rows with keys at most that value, so the kc-th key of the whole set is no larger.
+// With Q, raised by the error margins of the two sweeps' keys (rounded up to float).
+template <typename T>
 __global__ void knn_seed_kernel(const float *__restrict__ part_tau, int nq, int nsplit,
-                                float *__restrict__ tau0) {
+                                const T *__restrict__ Q, int d, const KnnKeyError et,
+                                const KnnKeyError es, float *__restrict__ tau0) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq)
         return;
     float t = __builtin_huge_valf();
     for (int s = 0; s < nsplit; s++)
         t = fminf(t, part_tau[(int64_t)q * nsplit + s]);
+    if (Q && t < __builtin_huge_valf()) {
+        double qn = 0.0;
+        for (int l = 0; l < d; l++) {
+            const double v = (double)Q[(int64_t)q * d + l];
+            qn = fma(v, v, qn);
+        }
+        const double qnorm = sqrt(qn);
+        const double v = (double)t + 4.0 * key_error(qnorm, d, et) + 2.0 * key_error(qnorm, d, es);
+        t = (float)v;
+        if ((double)t < v)
+            t = nextafterf(t, __builtin_huge_valf());
+    }
     tau0[q] = t;
 }
 } // namespace
@@ -1159,21 +1380,29 @@ template hipError_t launch_knn_query_centre<float>(const float *, int, int, cons
 template hipError_t launch_knn_query_centre<double>(const double *, int, int, const double *, int,
                                                     float *, hipStream_t);
 
-hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau0,
-                           hipStream_t s) {
+template <typename T>
+hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, const T *Q, int d,
+                           const KnnKeyError &tile_err, const KnnKeyError &seed_err,
+                           float *tau0, hipStream_t s) {
     if (nq <= 0)
         return hipSuccess;
-    hipLaunchKernelGGL(knn_seed_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, part_tau, nq,
-                       nsplit, tau0);
+    hipLaunchKernelGGL(knn_seed_kernel<T>, dim3((nq + 255) / 256), dim3(256), 0, s, part_tau, nq,
+                       nsplit, Q, d, tile_err, seed_err, tau0);
     return hipGetLastError();
 }
+template hipError_t launch_knn_seed<float>(const float *, int, int, const float *, int,
+                                           const KnnKeyError &, const KnnKeyError &, float *,
+                                           hipStream_t);
+template hipError_t launch_knn_seed<double>(const float *, int, int, const double *, int,
+                                            const KnnKeyError &, const KnnKeyError &, float *,
+                                            hipStream_t);
 
 template <typename T>
 hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const float *part_tau,
                             int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,
                             double u_in, double abs_in, double u_acc, double ymax, double rmax,
                             double cmax, int post, T *out_dist, int *out_idx, int *fb_count,
-                            int *fb_list, hipStream_t s) {
+                            int *fb_list, float *fb_bound, hipStream_t s) {
     const int m = nsplit * kc;
     // The lists may be shorter than k; their union must hold k candidates.
     if (m > MMAX || d > 1024 || k > 128 || k > m)
@@ -1186,17 +1415,17 @@ hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const fl
                           : knn_merge_kernel<T, 2048>;
     hipLaunchKernelGGL(kern, dim3(nq), dim3(64), 0, s, part_key, part_idx, part_tau, nsplit, kc,
                        Q, Y, d, k, u_in, abs_in, u_acc, ymax, rmax, cmax, post, out_dist, out_idx,
-                       fb_count, fb_list);
+                       fb_count, fb_list, fb_bound);
     return hipGetLastError();
 }
 
 template hipError_t launch_knn_merge<float>(const float *, const int *, const float *, int, int,
                                             const float *, const float *, int, int, int, double,
                                             double, double, double, double, double, int, float *,
-                                            int *, int *, int *, hipStream_t);
+                                            int *, int *, int *, float *, hipStream_t);
 template hipError_t launch_knn_merge<double>(const float *, const int *, const float *, int, int,
                                              const double *, const double *, int, int, int,
                                              double, double, double, double, double, double, int,
-                                             double *, int *, int *, int *, hipStream_t);
+                                             double *, int *, int *, int *, float *, hipStream_t);
 
 } // namespace dagpu

@@ -348,10 +348,30 @@ hipError_t launch_knn_centre_rows(const T *X, int d, const int *perm, const int 
 template <typename T>
 hipError_t launch_knn_query_centre(const T *Q, int nq, int d, const double *C, int G, float *qc,
                                    hipStream_t s);
+// Error model of a stored key of the fused tile: relative and absolute representation
+// error of the operands, the accumulation's unit roundoff, the largest training-row norm,
+// the largest operand-row norm (the residual norm for the centred form) and the largest
+// centre norm (0 when the rows are not centred). launch_knn_merge takes the same six values.
+struct KnnKeyError {
+    double u_in = 0, abs_in = 0, u_acc = 0, ymax = 0, rmax = 0, cmax = 0;
+};
 // tau0[q] = min over the nsplit thresholds part_tau[q*nsplit + s] of a sweep (a valid seed
-// for a sweep of any superset of the swept rows).
-hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau0,
-                           hipStream_t s);
+// for a sweep of any superset of the swept rows). With Q (nq x d rows of type T), the seed is
+// raised by 4 E_tile(|q|) + 2 E_seed(|q|), the error bounds of a key of the seeded sweep and
+// of the sweep that made the thresholds, so that a seeded sweep's certification (threshold
+// above the k-th stored key + 2 E_tile) does not fail for the seed alone.
+template <typename T>
+hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, const T *Q, int d,
+                           const KnnKeyError &tile_err, const KnnKeyError &seed_err,
+                           float *tau0, hipStream_t s);
+// Group-minimum seeds: Q and Y (n sample rows, n a multiple of 32, in the tile layout) and
+// nyh as for launch_knn_tile in the forms KP_F32, KP_F16, KP_BF16, KP_F16X3, KP_BF16X3; the
+// sample is cut into nsplit splits and each split's rows into 16 groups by offset mod 16.
+// part_tau[q*nsplit + s] = the k-th smallest (1 <= k <= 16) of split s's 16 group-best keys
+// of query q: an upper bound on the query's k-th key over any superset of the sample.
+hipError_t launch_knn_seed_min(int prec, const void *Q, int nq, const void *Y, const void *nyh,
+                               int n, int Kp, int nsplit, int k, float *part_tau,
+                               hipStream_t s);
 // Q: nq x d and Y: n x d unpadded rows (type T); u_in: relative representation error of the
 // tile operands (0 when they are the data themselves); abs_in: absolute representation error
 // per element (fp16 subnormals); u_acc: unit roundoff of the tile accumulation;
@@ -359,13 +379,16 @@ hipError_t launch_knn_seed(const float *part_tau, int nq, int nsplit, float *tau
 // largest residual of a centred form); cmax: max centre norm of a centred form (0 otherwise).
 // Requires nsplit * kc <= knn_merge_max_candidates() and k <= kc.
 // Certified queries get out_dist/out_idx (row-major nq x k); the others are appended to
-// fb_list (fb_count must be zeroed before).
+// fb_list (fb_count must be zeroed before), and, with fb_bound, fb_bound[p] for list entry p
+// receives an upper bound on that query's k-th key plus 3 E (the k-th stored key + 4 E,
+// rounded up; +inf when fewer than k candidates were stored): a threshold seed for a second
+// sweep of the same operand form.
 template <typename T>
 hipError_t launch_knn_merge(const float *part_key, const int *part_idx, const float *part_tau,
                             int nsplit, int kc, const T *Q, const T *Y, int nq, int d, int k,
                             double u_in, double abs_in, double u_acc, double ymax, double rmax,
                             double cmax, int post, T *out_dist, int *out_idx, int *fb_count,
-                            int *fb_list, hipStream_t s);
+                            int *fb_list, float *fb_bound, hipStream_t s);
 } // namespace dagpu
 
 namespace dagpu {

@@ -85,12 +85,11 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
     opts.add_int("candidates per split",
                  "Candidates kept per query and split by the fused kNN tile (0 = from k).", 0,
                  LowerBound::greater_equal, 128, UpperBound::less_equal, 0);
-    // MI355X extension: a short sweep of the first 1/64 of the rows (training sets of
-    // 131,072 rows or more) bounds each query's candidate-list threshold before the full
-    // sweep, so that the full sweep buffers far fewer rows. Speed only: the result is exact
-    // either way.
+    // MI355X extension: for k <= 16 on training sets of 131,072 rows or more, the minima of
+    // 16 row groups of an 8,192-row sample bound each query's k-th key before the sweep, so
+    // that the sweep buffers far fewer rows. Speed only: the result is exact either way.
     opts.add_string("seed thresholds",
-                    "Seed the fused kNN tile's thresholds from a sweep of a row subset.",
+                    "Seed the fused kNN tile's thresholds from group minima over a row sample.",
                     {{"yes", 1}, {"no", 0}}, "no");
 }
 
@@ -287,6 +286,9 @@ template <typename T> da_status NNHandle<T>::prepare_fused(int prec) {
                                    hipMemcpyHostToDevice, st));
         DA_HIP(err, hipStreamSynchronize(st));
     }
+    da_status ss = prepare_seed_sample(prec, sq);
+    if (ss != da_status_success)
+        return ss;
     ypad_prec_ = prec;
     fused_ready_ = true;
     return da_status_success;
@@ -403,10 +405,56 @@ template <typename T> da_status NNHandle<T>::prepare_centred() {
     DA_HIP(err, hipMemcpyAsync(nyh_.ptr(), h.data(), rows_pad * sizeof(float),
                                hipMemcpyHostToDevice, st));
     DA_HIP(err, hipStreamSynchronize(st));
+    // The queries of this form are plain fp16 rows: its seeds come from a plain fp16 sample.
+    da_status ss = prepare_seed_sample(KP_F16, sq);
+    if (ss != da_status_success)
+        return ss;
     cen_G_ = G;
     cen_rows_ = rows_pad;
     ypad_prec_ = KP_F16C;
     fused_ready_ = true;
+    return da_status_success;
+}
+
+// Sample of the group-minimum threshold seeds: 8,192 rows at an even stride through the
+// training rows, in the tile layout of the seed form. The whole training set is a superset
+// of the sample, so a bound on a query's k-th key over the sample holds for every sweep.
+template <typename T>
+da_status NNHandle<T>::prepare_seed_sample(int prec, const std::vector<double> &sq) {
+    seed_rows_ = 0;
+    seed_prec_ = prec;
+    const int n = n_samples_, d = n_features_;
+    if (n < 131072 || prec == KP_F64 || prec == KP_F16C)
+        return da_status_success;
+    constexpr int S = 8192;
+    hipStream_t st = gpu->stream();
+    std::vector<int> idx(S);
+    std::vector<float> h(S);
+    for (int j = 0; j < S; j++) {
+        idx[j] = (int)((int64_t)j * n / S);
+        h[j] = (float)(-0.5 * sq[idx[j]]);
+    }
+    const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
+    {
+        DeviceBuffer didx, rows, packed;
+        DA_HIP(err, didx.reserve(S * sizeof(int)));
+        DA_HIP(err, rows.reserve((size_t)S * d * sizeof(T)));
+        DA_HIP(err, packed.reserve((size_t)S * kp_ * esz));
+        DA_HIP(err, hipMemcpyAsync(didx.ptr(), idx.data(), S * sizeof(int), hipMemcpyHostToDevice,
+                                   st));
+        DA_HIP(err, launch_gather_scatter(train_.ptr(), rows.ptr(), didx.as<int>(), S, d,
+                                          (int)sizeof(T), 0, st));
+        DA_HIP(err, launch_pack_rows_prec<T>(rows.as<T>(), d, S, d, packed.ptr(), kp_, S, prec,
+                                             st));
+        DA_HIP(err, seed_y_.reserve((size_t)S * kp_ * esz));
+        DA_HIP(err, launch_knn_tile_operand(packed.ptr(), S, (int)(kp_ * esz),
+                                            knn_prec_row_factor(prec), seed_y_.ptr(), S, st));
+        DA_HIP(err, seed_nyh_.reserve(S * sizeof(float)));
+        DA_HIP(err, hipMemcpyAsync(seed_nyh_.ptr(), h.data(), S * sizeof(float),
+                                   hipMemcpyHostToDevice, st));
+        DA_HIP(err, hipStreamSynchronize(st));
+    }
+    seed_rows_ = S;
     return da_status_success;
 }
 
@@ -525,10 +573,50 @@ int NNHandle<T>::tile_splits(SearchContext &ctx, int prec, int kc, int variant, 
     return best;
 }
 
+// Error model of the keys the fused tile stores for operand form prec over this training set.
+// Representation error of an operand element: relative u_in, absolute abs_in (fp16
+// subnormals). Split forms keep hi + lo, so only the remainder of lo and the dropped lo*lo
+// product (folded into u_in) remain.
+template <typename T> KnnKeyError NNHandle<T>::key_error_model(int prec) const {
+    const bool same = (prec == KP_F64 && std::is_same_v<T, double>) ||
+                      (prec == KP_F32 && std::is_same_v<T, float>);
+    KnnKeyError e;
+    const double u16 = std::ldexp(1.0, -11), ub16 = std::ldexp(1.0, -8), u32 = std::ldexp(1.0, -24);
+    if (!same) {
+        switch (prec) {
+        case KP_F32:
+            e.u_in = u32;
+            break;
+        case KP_F16:
+        case KP_F16C:
+            e.u_in = u16 + u32;
+            e.abs_in = std::ldexp(1.0, -25);
+            break;
+        case KP_BF16:
+            e.u_in = ub16 + u32;
+            break;
+        case KP_F16X3:
+            e.u_in = 1.5 * u16 * u16 * (1 + u16) + 2 * u32 * u16 + u32 * u32;
+            e.abs_in = std::ldexp(1.0, -24);
+            break;
+        case KP_BF16X3:
+            e.u_in = 1.5 * ub16 * ub16 * (1 + ub16) + 2 * u32 * ub16 + u32 * u32;
+            break;
+        default:
+            break;
+        }
+    }
+    e.u_acc = prec == KP_F64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+    e.ymax = ymax_;
+    e.rmax = prec == KP_F16C ? rmax_ : ymax_;
+    e.cmax = prec == KP_F16C ? cmax_ : 0.0;
+    return e;
+}
+
 template <typename T>
 da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp, const T *dQ,
                                      da_int nq, da_int k, bool return_distance, da_int *d_ind,
-                                     T *d_dist, bool &fused) {
+                                     T *d_dist, bool &fused, const float *seed_in) {
     int kc = sp.tile_kc(k);
     // With the automatic length, a list shorter than k needs 2k candidates over the splits
     // the training set allows (at most one per 1,024 rows): on a small set the next length up
@@ -543,6 +631,7 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
             knn_tile_min_splits(sp.prec, kp_, tile_rows(sp.prec), kc, variant) <= split_cap(kc);
     ctx.last_fallback = 0;
     ctx.last_candidates = 0;
+    ctx.last_params = sp;
     if (!fused)
         return search_tiles(ctx, dQ, nq, k, return_distance, d_ind, d_dist);
     hipStream_t st = ctx.gpu->stream();
@@ -553,25 +642,24 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
     const size_t esz = (size_t)knn_prec_elem_size(prec) * knn_prec_row_factor(prec);
     DA_HIP(err, ctx.qpad.reserve((size_t)nq * kp_ * esz));
     DA_HIP(err, launch_pack_rows_prec<T>(dQ, d, nq, d, ctx.qpad.ptr(), kp_, nq, prec, st));
-    // Threshold seeds from a 16-candidate sweep of the first 1/64 of the rows in splits of at
-    // least 8,192 rows (few list restarts). Its split thresholds bound each query's 16th key
-    // over the subset, about the 1,024th over the whole set: far above the k-th (k <= 128),
-    // so the seeded sweep still certifies, while it buffers only the rows below the seed.
-    const float *seed = nullptr;
-    const int sub = sp.seed && n >= 131072 && prec != KP_F16C ? (n / 64 + 31) / 32 * 32 : 0;
-    if (sub > 0) {
+    // Threshold seeds (k <= 16): for each query the k-th smallest of 16 group-best keys over
+    // the seed sample (launch_knn_seed_min), split so that the launch fills the device with
+    // at least 1,024 sample rows per split, then the smallest over the splits. The bound is
+    // about the (n / 8,192 * 16 / k)-th key of the whole set, far above the k-th, so the
+    // seeded sweep still certifies while it buffers only the rows below the seed.
+    const float *seed = seed_in;
+    if (!seed && sp.seed && seed_rows_ > 0 && k <= 16) {
         KernelScope ks(*ctx.gpu, "knn_seed");
-        const int lo = knn_tile_min_splits(prec, kp_, sub, 16, variant);
-        const int ns = std::max(lo, std::min(knn_merge_max_candidates() / 16, sub / 8192));
-        const size_t np0 = (size_t)nq * ns;
-        DA_HIP(err, ctx.part_key.reserve(np0 * 16 * sizeof(float)));
-        DA_HIP(err, ctx.part_idx.reserve(np0 * 16 * sizeof(int)));
-        DA_HIP(err, ctx.part_tau.reserve(np0 * sizeof(float)));
+        const int qblocks = (int)((nq + 127) / 128);
+        const int ns = std::max(1, std::min(seed_rows_ / 1024, (1024 + qblocks - 1) / qblocks));
+        DA_HIP(err, ctx.part_tau.reserve((size_t)nq * ns * sizeof(float)));
         DA_HIP(err, ctx.tau0.reserve((size_t)nq * sizeof(float)));
-        DA_HIP(err, launch_knn_tile(prec, ctx.qpad.ptr(), nq, ypad_.ptr(), nyh_.ptr(), sub, kp_,
-                                    ns, 16, ctx.part_key.as<float>(), ctx.part_idx.as<int>(),
-                                    ctx.part_tau.as<float>(), nullptr, variant, st));
-        DA_HIP(err, launch_knn_seed(ctx.part_tau.as<float>(), nq, ns, ctx.tau0.as<float>(), st));
+        DA_HIP(err, launch_knn_seed_min(seed_prec_, ctx.qpad.ptr(), nq, seed_y_.ptr(),
+                                        seed_nyh_.ptr(), seed_rows_, kp_, ns, (int)k,
+                                        ctx.part_tau.as<float>(), st));
+        DA_HIP(err, launch_knn_seed<T>(ctx.part_tau.as<float>(), nq, ns, dQ, n_features_,
+                                       key_error_model(prec), key_error_model(seed_prec_),
+                                       ctx.tau0.as<float>(), st));
         seed = ctx.tau0.as<float>();
     }
     // The centred form: every query's product with every group centre.
@@ -603,8 +691,9 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
     DA_HIP(err, ctx.part_idx.reserve(np * kc * sizeof(int)));
     DA_HIP(err, ctx.part_tau.reserve(np * sizeof(float)));
     DA_HIP(err, ctx.fb.reserve((size_t)(nq + 1) * sizeof(int)));
+    DA_HIP(err, ctx.fb_bound.reserve((size_t)nq * sizeof(float)));
     {
-        KernelScope ks(*ctx.gpu, "knn_tile");
+        KernelScope ks(*ctx.gpu, ctx.second_pass ? "knn_retry_tile" : "knn_tile");
         DA_HIP(err, launch_knn_tile(prec, ctx.qpad.ptr(), nq, ypad_.ptr(), nyh_.ptr(),
                                     tile_rows(prec), kp_, nsplit, kc, ctx.part_key.as<float>(),
                                     ctx.part_idx.as<int>(), ctx.part_tau.as<float>(), seed,
@@ -613,47 +702,15 @@ da_status NNHandle<T>::search_launch(SearchContext &ctx, const SearchParams &sp,
     int *fbc = ctx.fb.as<int>();
     DA_HIP(err, hipMemsetAsync(fbc, 0, sizeof(int), st));
     const int post = (metric_ == da_sqeuclidean || metric_ == da_sqeuclidean_gemm) ? 0 : 1;
-    // Representation error of an operand element: relative u_in, absolute abs_in (fp16
-    // subnormals). Split forms keep hi + lo, so only the remainder of lo and the dropped
-    // lo*lo product (folded into u_in) remain.
-    const bool same = (prec == KP_F64 && std::is_same_v<T, double>) ||
-                      (prec == KP_F32 && std::is_same_v<T, float>);
-    double u_in = 0.0, abs_in = 0.0;
-    const double u16 = std::ldexp(1.0, -11), ub16 = std::ldexp(1.0, -8), u32 = std::ldexp(1.0, -24);
-    if (!same) {
-        switch (prec) {
-        case KP_F32:
-            u_in = u32;
-            break;
-        case KP_F16:
-        case KP_F16C:
-            u_in = u16 + u32;
-            abs_in = std::ldexp(1.0, -25);
-            break;
-        case KP_BF16:
-            u_in = ub16 + u32;
-            break;
-        case KP_F16X3:
-            u_in = 1.5 * u16 * u16 * (1 + u16) + 2 * u32 * u16 + u32 * u32;
-            abs_in = std::ldexp(1.0, -24);
-            break;
-        case KP_BF16X3:
-            u_in = 1.5 * ub16 * ub16 * (1 + ub16) + 2 * u32 * ub16 + u32 * u32;
-            break;
-        default:
-            break;
-        }
-    }
-    const double u_acc = prec == KP_F64 ? std::ldexp(1.0, -53) : std::ldexp(1.0, -24);
+    const KnnKeyError ke = key_error_model(prec);
     {
-        KernelScope ks(*ctx.gpu, "knn_merge");
+        KernelScope ks(*ctx.gpu, ctx.second_pass ? "knn_retry_merge" : "knn_merge");
         DA_HIP(err, launch_knn_merge<T>(ctx.part_key.as<float>(), ctx.part_idx.as<int>(),
                                         ctx.part_tau.as<float>(), nsplit, kc, dQ, train_.as<T>(),
-                                        nq, d, k, u_in, abs_in, u_acc, ymax_,
-                                        prec == KP_F16C ? rmax_ : ymax_,
-                                        prec == KP_F16C ? cmax_ : 0.0, post,
+                                        nq, d, k, ke.u_in, ke.abs_in, ke.u_acc, ke.ymax, ke.rmax,
+                                        ke.cmax, post,
                                         return_distance ? d_dist : nullptr, d_ind, fbc, fbc + 1,
-                                        st));
+                                        ctx.fb_bound.as<float>(), st));
     }
     DA_HIP(err, hipMemcpyAsync(ctx.h_fb, fbc, sizeof(int), hipMemcpyDeviceToHost, st));
     ctx.last_candidates = nsplit * kc;
@@ -672,16 +729,41 @@ da_status NNHandle<T>::search_finish(SearchContext &ctx, const T *dQ, da_int nq,
     const int nfb = *ctx.h_fb;
     ctx.last_fallback = nfb;
     if (nfb > 0) {
-        // Exact fallback for the uncertified queries (listed by the merge kernel): gather
-        // their rows, key tiles + selection, scatter the results back.
+        // The uncertified queries (listed by the merge kernel): gather their rows, search
+        // them again, scatter the results back. The first pass is followed by a second
+        // fused pass over them with lists of at least 32, seeded by the merge's bound on
+        // each query's k-th key: it buffers only the rows below that bound (a few dozen), so
+        // even a split that holds all of the rows near the k-th key (the centred form stores
+        // neighbouring rows together) keeps them, and its threshold lies above the bound;
+        // nearly all of them certify there. The rest take the exact fallback (key tiles +
+        // selection).
         const int d = n_features_;
         const int *list = ctx.fb.as<int>() + 1;
         DA_HIP(err, ctx.fb_q.reserve((size_t)nfb * d * sizeof(T)));
         DA_HIP(err, launch_gather_scatter(dQ, ctx.fb_q.ptr(), list, nfb, d, sizeof(T), 0, st));
         DA_HIP(err, ctx.fb_ind.reserve((size_t)nfb * k * sizeof(da_int)));
         DA_HIP(err, ctx.fb_dist.reserve((size_t)nfb * k * sizeof(T)));
-        da_status s = search_tiles(ctx, ctx.fb_q.as<T>(), nfb, k, true, ctx.fb_ind.as<da_int>(),
-                                   ctx.fb_dist.as<T>());
+        da_status s;
+        if (!ctx.second_pass) {
+            if (!ctx.retry)
+                ctx.retry = std::make_unique<SearchContext>();
+            SearchContext &rc = *ctx.retry;
+            rc.gpu = ctx.gpu;
+            rc.tuning = ctx.tuning ? ctx.tuning : &ctx;
+            rc.second_pass = true;
+            SearchParams sp2 = ctx.last_params;
+            sp2.seed = false;
+            sp2.kc_min = std::max(sp2.kc_min, 32);
+            bool fused2 = false;
+            s = search_launch(rc, sp2, ctx.fb_q.as<T>(), nfb, k, true, ctx.fb_ind.as<da_int>(),
+                              ctx.fb_dist.as<T>(), fused2, ctx.fb_bound.as<float>());
+            if (s == da_status_success)
+                s = search_finish(rc, ctx.fb_q.as<T>(), nfb, k, true, ctx.fb_ind.as<da_int>(),
+                                  ctx.fb_dist.as<T>(), fused2);
+        } else {
+            s = search_tiles(ctx, ctx.fb_q.as<T>(), nfb, k, true, ctx.fb_ind.as<da_int>(),
+                             ctx.fb_dist.as<T>());
+        }
         if (s != da_status_success)
             return s;
         DA_HIP(err, launch_gather_scatter(ctx.fb_ind.ptr(), d_ind, list, nfb, k, sizeof(da_int), 1, st));

@@ -6,9 +6,20 @@
 #include "../kernels/launch.hpp"
 #include <functional>
 #include <map>
+#include <memory>
 #include <vector>
 
 namespace dagpu {
+
+// Parameters of a search fixed by the caller (read from the options once).
+struct SearchParams {
+    int prec = 0;     // operand precision of the fused tile (KP_*)
+    int variant = -1; // fused tile kernel variant (-1: tuning table)
+    int kc_min = 0;   // minimum candidates kept per (query, split) on the fused tile
+    bool seed = false; // seed the fused tile's thresholds (group minima over a row sample)
+    // Candidate-list length of the fused tile for k neighbours.
+    int tile_kc(int k) const;
+};
 
 // Scratch state of one in-flight search: its stream (through a GpuContext) and the device
 // buffers of the fused and tile paths. A handle owns one; a serving loop owns several so
@@ -21,21 +32,16 @@ struct SearchContext {
     // that search the same handle from one thread share their tuning.
     SearchContext *tuning = nullptr;
     DeviceBuffer q_norm, keys, sel_key, sel_idx, qpad, part_key, part_idx, part_tau, fb, fb_q,
-        fb_ind, fb_dist, tau0, qc;
+        fb_ind, fb_dist, fb_bound, tau0, qc;
     // Pinned host word receiving the fallback count of the last fused launch.
     int *h_fb = nullptr;
     da_int last_fallback = 0, last_candidates = 0;
+    // Parameters of the last fused launch, and the context of the second fused pass over
+    // the queries it left uncertified (created on first use; second_pass marks it).
+    SearchParams last_params;
+    std::unique_ptr<SearchContext> retry;
+    bool second_pass = false;
     ~SearchContext();
-};
-
-// Parameters of a search fixed by the caller (read from the options once).
-struct SearchParams {
-    int prec = 0;     // operand precision of the fused tile (KP_*)
-    int variant = -1; // fused tile kernel variant (-1: tuning table)
-    int kc_min = 0;   // minimum candidates kept per (query, split) on the fused tile
-    bool seed = false; // seed the fused tile's thresholds from a sweep of a row subset
-    // Candidate-list length of the fused tile for k neighbours.
-    int tile_kc(int k) const;
 };
 
 template <typename T> class NNHandle : public AlgHandle {
@@ -74,13 +80,14 @@ template <typename T> class NNHandle : public AlgHandle {
 
     // Search entry points shared with the serving loop. dQ: row-major n_features columns;
     // outputs row-major nq x k. search_launch enqueues everything that needs no host
-    // decision; search_finish waits, runs the exact fallback if the certification listed
-    // queries, and leaves the results in d_ind/d_dist (enqueued on ctx's stream).
+    // decision; search_finish waits, searches again the queries the certification listed,
+    // and leaves the results in d_ind/d_dist (enqueued on ctx's stream). seed_in: per-query
+    // threshold seeds of the fused tile (in place of the sample seeds).
     SearchParams params() const;
     da_status prepare_search(const SearchParams &p);
     da_status search_launch(SearchContext &ctx, const SearchParams &p, const T *dQ, da_int nq,
                             da_int k, bool return_distance, da_int *d_ind, T *d_dist,
-                            bool &fused);
+                            bool &fused, const float *seed_in = nullptr);
     da_status search_finish(SearchContext &ctx, const T *dQ, da_int nq, da_int k,
                             bool return_distance, da_int *d_ind, T *d_dist, bool fused);
     bool trained() const { return trained_; }
@@ -154,6 +161,13 @@ template <typename T> class NNHandle : public AlgHandle {
     int cen_G_ = 0, cen_rows_ = 0;
     double rmax_ = 0, cmax_ = 0;
     da_status prepare_centred();
+    // Threshold-seed sample (training sets of at least 131,072 rows): seed_rows_ rows taken
+    // at an even stride through the training rows, in the tile layout of the seed form
+    // seed_prec_ (the tile's form, plain fp16 for the centred form), and their -|y|^2/2.
+    DeviceBuffer seed_y_, seed_nyh_;
+    int seed_rows_ = 0, seed_prec_ = -1;
+    da_status prepare_seed_sample(int prec, const std::vector<double> &sq);
+    KnnKeyError key_error_model(int prec) const;
     // Rows the fused tile sweeps: the padded stored rows of the centred form, else n_samples.
     int tile_rows(int prec) const { return prec == KP_F16C ? cen_rows_ : (int)n_samples_; }
     bool fused_ready_ = false;

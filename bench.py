@@ -143,7 +143,8 @@ def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1, 
         step()
     L.check(L.lib.da_amd_synchronize(h.h))
     kern = {}
-    for name in ("knn_seed", "knn_tile", "knn_merge", "pairwise_direct", "row_topk"):
+    for name in ("knn_seed", "knn_tile", "knn_merge", "knn_retry_tile", "knn_retry_merge",
+                 "pairwise_direct", "row_topk"):
         ms, cnt = C.c_double(), L.da_int()
         L.check(L.lib.da_amd_profile_get(h.h, name.encode(), C.byref(ms), C.byref(cnt)))
         if cnt.value:

@@ -255,26 +255,35 @@ def test_long_splits_16bit_offsets(oracle, variant, k):
     np.testing.assert_array_equal(ind, ei)
     np.testing.assert_allclose(dist, ed, rtol=1e-5)
 
-@pytest.mark.parametrize("precision", ["working", "fp16", "bf16x3"])
-@pytest.mark.parametrize("k", [10, 30, 100])
+@pytest.mark.parametrize("precision", ["working", "fp16", "bf16x3", "fp16x3", "fp16c"])
+@pytest.mark.parametrize("k", [1, 10, 16, 30])
 def test_seeded_thresholds(oracle, precision, k):
-    """Seeding the thresholds from a short sweep of the first 1/64 of the rows (training sets
-    of 131,072 rows or more) returns exactly the neighbours of the unseeded sweep and of the
-    oracle, also for queries far from every training row (whose seeds are loose) and for
-    plain fp16, whose wider error bound sends queries to the exact fallback."""
+    """Seeding the thresholds (k <= 16, training sets of 131,072 rows or more) from the
+    group minima of an 8,192-row sample returns exactly the neighbours of the unseeded sweep
+    and of the oracle, also for queries far from every training row (whose seeds are loose),
+    for k = 16 (the last of the 16 groups) and for plain fp16, whose wider error bound sends
+    queries to the exact fallback. A seed is an upper bound on the k-th key, so for the
+    forms that certify every query unseeded it sends none to the fallback either; k = 30
+    runs unseeded."""
     rng = np.random.default_rng(91)
     d = 40
     means = rng.uniform(-5, 5, (32, d))
     Xt = gaussian_mixture(rng, 140000, means)
-    Xq = np.concatenate([gaussian_mixture(rng, 90, means),
+    Xq = np.concatenate([gaussian_mixture(rng, 290, means),
                          rng.uniform(-40, 40, (10, d)).astype(np.float32)])
     ei, ed = oracle.knn(Xt, Xq, k)
+    fallbacks = {}
     for seed in (True, False):
         nn = aoclda.neighbors.nearest_neighbors(precision=precision,
                                                 seed_thresholds=seed).fit(Xt)
         dist, ind = nn.kneighbors(Xq, n_neighbors=k)
         np.testing.assert_array_equal(ind, ei, err_msg=f"seed={seed}")
         np.testing.assert_allclose(dist, ed, rtol=1e-5)
+        nfb, ncand = L.da_int(), L.da_int()
+        L.check(L.lib.da_nn_last_query_stats(nn.handle.h, C.byref(nfb), C.byref(ncand)))
+        fallbacks[seed] = nfb.value
+    if precision in ("working", "bf16x3", "fp16x3") and fallbacks[False] == 0:
+        assert fallbacks[True] == 0
 
 
 @pytest.mark.parametrize("precision", ["working", "bf16x3", "fp16x3"])
