@@ -12,14 +12,15 @@ class nearest_neighbors:
     def __init__(self, n_neighbors=5, weights="uniform", algorithm="auto", leaf_size=30,
                  metric="euclidean", p=2.0, radius=1.0, outlier_handling=None,
                  check_data=False, precision="working", tile_variant=-1,
-                 candidates_per_split=0, seed_thresholds=False):
+                 candidates_per_split=0, seed_thresholds=None):
         # precision: operand precision of the MFMA distance tiles ("working", "fp32",
         # "fp16", "bf16", "fp16x3", "bf16x3", "fp16c": fp16 rows stored about k-means group
         # centres); results are re-ranked exactly whatever the
         # choice. candidates_per_split: candidate-list length per query and split of
         # the fused tile (0 = from k; shorter than k is allowed and stays exact).
         # seed_thresholds: bound the tile's thresholds (k <= 16) by the group
-        # minima over a row sample first. All three are MI355X options that change speed only.
+        # minima over a row sample first (None: the library's choice, which seeds the
+        # 16-bit forms). All three are MI355X options that change speed only.
         if not isinstance(n_neighbors, int) and n_neighbors is not None:
             raise ValueError("Number of neighbors must be an integer or None. Got n_neighbors="
                              + str(n_neighbors))
@@ -49,7 +50,8 @@ class nearest_neighbors:
         h.set_string("cross term precision", self.precision)
         h.set_int("tile variant", self.tile_variant)
         h.set_int("candidates per split", self.candidates_per_split)
-        h.set_string("seed thresholds", "yes" if self.seed_thresholds else "no")
+        if self.seed_thresholds is not None:
+            h.set_string("seed thresholds", "yes" if self.seed_thresholds else "no")
         if self.radius is not None:
             h.set_real("radius", self.radius)
 

@@ -33,8 +33,8 @@ constexpr int KC = 16; // k slice staged in LDS
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <typename T>
-__device__ double direct_sqdist(const T *__restrict__ x, const T *__restrict__ y, int k) {
+template <typename T, typename TY>
+__device__ double direct_sqdist(const T *__restrict__ x, const TY *__restrict__ y, int k) {
     double acc = 0.0;
     for (int l = 0; l < k; l++) {
         const double d = (double)x[l] - (double)y[l];
@@ -43,9 +43,9 @@ __device__ double direct_sqdist(const T *__restrict__ x, const T *__restrict__ y
     return acc;
 }
 
-template <typename T, typename U, int METRIC>
+template <typename T, typename TY, typename U, int METRIC>
 __global__ __launch_bounds__(256) void pairwise_mfma_kernel(
-    const T *__restrict__ X, const T *__restrict__ Y, int m, int n, int k, U *__restrict__ D,
+    const T *__restrict__ X, const TY *__restrict__ Y, int m, int n, int k, U *__restrict__ D,
     int64_t ldd, const double *__restrict__ nx, const double *__restrict__ ny, int zero_diag,
     int diag_offset, double gam, double inv_rtol) {
     __shared__ double Xs[KC][TB + 1];
@@ -160,7 +160,7 @@ hipError_t launch_pairwise_mfma(int metric, const T *X, const T *Y, int m, int n
     const double inv_rtol = sizeof(U) == 8 ? 2e12 : 2e6;
 #define DA_PWM_CASE(M)                                                                       \
     case M:                                                                                  \
-        hipLaunchKernelGGL((pairwise_mfma_kernel<T, U, M>), grid, dim3(256), 0, s, X, Y, m, n, \
+        hipLaunchKernelGGL((pairwise_mfma_kernel<T, T, U, M>), grid, dim3(256), 0, s, X, Y, m, n, \
                            k, D, ldd, nx, ny, zero_diag, diag_offset, gam, inv_rtol);         \
         break;
     switch (metric) {
@@ -176,6 +176,30 @@ hipError_t launch_pairwise_mfma(int metric, const T *X, const T *Y, int m, int n
 #undef DA_PWM_CASE
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_inner_mfma(const T *X, const double *Y, int m, int n, int k, float *D,
+                             int64_t ldd, hipStream_t s) {
+    if (m <= 0 || n <= 0)
+        return hipSuccess;
+    // At most 65,535 row tiles per launch (the grid's y extent).
+    const int chunk = 65535 * TB;
+    for (int i0 = 0; i0 < m; i0 += chunk) {
+        const int mm = m - i0 < chunk ? m - i0 : chunk;
+        dim3 grid((n + TB - 1) / TB, (mm + TB - 1) / TB);
+        hipLaunchKernelGGL((pairwise_mfma_kernel<T, double, float, KM_INNER>), grid, dim3(256), 0,
+                           s, X + (int64_t)i0 * k, Y, mm, n, k, D + (int64_t)i0 * ldd, ldd,
+                           nullptr, nullptr, 0, 0, 0.0, 0.0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+template hipError_t launch_inner_mfma<float>(const float *, const double *, int, int, int,
+                                             float *, int64_t, hipStream_t);
+template hipError_t launch_inner_mfma<double>(const double *, const double *, int, int, int,
+                                              float *, int64_t, hipStream_t);
 
 template hipError_t launch_pairwise_mfma<float, float>(int, const float *, const float *, int,
                                                        int, int, float *, int64_t,

@@ -1283,32 +1283,6 @@ __global__ __launch_bounds__(256) void knn_centre_rows_kernel(
         rnorm2[p] = acc;
 }
 
-// qc[q*G + g] = Q_q . C_g: a 16 x 16 block of (query, centre) pairs per workgroup, the rows
-// staged in LDS in slices of 64 features, accumulated in double.
-template <typename T>
-__global__ __launch_bounds__(256) void knn_query_centre_kernel(const T *__restrict__ Q, int nq,
-                                                               int d, const double *__restrict__ C,
-                                                               int G, float *__restrict__ qc) {
-    __shared__ double qs[16][65], cs[16][65];
-    const int tq = threadIdx.x >> 4, tg = threadIdx.x & 15;
-    const int q0 = blockIdx.y * 16, g0 = blockIdx.x * 16;
-    double acc = 0.0;
-    for (int l0 = 0; l0 < d; l0 += 64) {
-        for (int t = threadIdx.x; t < 16 * 64; t += 256) {
-            const int r = t >> 6, l = t & 63;
-            qs[r][l] = (q0 + r < nq && l0 + l < d) ? (double)Q[(int64_t)(q0 + r) * d + l0 + l] : 0.0;
-            cs[r][l] = (g0 + r < G && l0 + l < d) ? C[(int64_t)(g0 + r) * d + l0 + l] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 16
-        for (int l = 0; l < 64; l++)
-            acc = fma(qs[tq][l], cs[tg][l], acc);
-        __syncthreads();
-    }
-    if (q0 + tq < nq && g0 + tg < G)
-        qc[(int64_t)(q0 + tq) * G + g0 + tg] = (float)acc;
-}
-
 // tau0[q] = the smallest split threshold of a sweep over a subset of the rows: the subset
 // holds kc rows with keys at most that value, so the kc-th key of the whole set is no larger.
 // With Q, raised by the error margins of the two sweeps' keys (rounded up to float).
@@ -1357,23 +1331,14 @@ template hipError_t launch_knn_centre_rows<double>(const double *, int, const in
                                                    const double *, int, int, void *, double *,
                                                    hipStream_t);
 
+// The products of every query with every group centre: the fp64 matrix-core inner-product
+// tile (LDS-staged 64 x 64 blocks).
 template <typename T>
 hipError_t launch_knn_query_centre(const T *Q, int nq, int d, const double *C, int G, float *qc,
                                    hipStream_t s) {
     if (nq <= 0 || G <= 0)
         return hipSuccess;
-    // At most 65,535 blocks of 16 queries per launch (the grid's y extent).
-    const int chunk = 65535 * 16;
-    for (int q0 = 0; q0 < nq; q0 += chunk) {
-        const int m = nq - q0 < chunk ? nq - q0 : chunk;
-        hipLaunchKernelGGL(knn_query_centre_kernel<T>, dim3((G + 15) / 16, (m + 15) / 16),
-                           dim3(256), 0, s, Q + (int64_t)q0 * d, m, d, C, G,
-                           qc + (int64_t)q0 * G);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess)
-            return e;
-    }
-    return hipSuccess;
+    return launch_inner_mfma<T>(Q, C, nq, G, d, qc, G, s);
 }
 template hipError_t launch_knn_query_centre<float>(const float *, int, int, const double *, int,
                                                    float *, hipStream_t);

@@ -58,6 +58,11 @@ hipError_t launch_pairwise_direct(int metric, const T *X, const T *Y, int m, int
 // rounding bound could exceed 5e-13 (double output) or 5e-7 (float output) of the value are
 // recomputed with direct differences. Same layout and diagonal rule as launch_pairwise_direct.
 bool pairwise_mfma_metric(int metric);
+// D[i*ldd + j] = (float) X_i . Y_j for X (m x k, type T) and Y (n x k, double), row-major
+// compact: the same tile with the products and sums in double, rounded once to float.
+template <typename T>
+hipError_t launch_inner_mfma(const T *X, const double *Y, int m, int n, int k, float *D,
+                             int64_t ldd, hipStream_t s);
 template <typename T, typename U>
 hipError_t launch_pairwise_mfma(int metric, const T *X, const T *Y, int m, int n, int k, U *D,
                                 int64_t ldd, const double *nx, const double *ny, int zero_diag,

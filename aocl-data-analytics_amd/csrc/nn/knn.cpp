@@ -88,9 +88,12 @@ template <typename T> NNHandle<T>::NNHandle(ErrorStack &e, GpuContext &g) : AlgH
     // MI355X extension: for k <= 16 on training sets of 131,072 rows or more, the minima of
     // 16 row groups of an 8,192-row sample bound each query's k-th key before the sweep, so
     // that the sweep buffers far fewer rows. Speed only: the result is exact either way.
+    // "auto" seeds the 16-bit operand forms; for the fp32 and fp64 forms the sample's
+    // products cost more than the sweep saves (200,000 x 128, 20,000 queries: fp32 seed
+    // 0.65 ms against 0.4 ms saved; fp16c 0.09 ms against about 1 ms).
     opts.add_string("seed thresholds",
                     "Seed the fused kNN tile's thresholds from group minima over a row sample.",
-                    {{"yes", 1}, {"no", 0}}, "no");
+                    {{"auto", 2}, {"yes", 1}, {"no", 0}}, "auto");
 }
 
 // The candidate-list length: from k, or as set by "candidates per split". A list shorter
@@ -463,7 +466,8 @@ template <typename T> SearchParams NNHandle<T>::params() const {
     p.prec = tile_prec();
     p.variant = opts.ival("tile variant");
     p.kc_min = opts.ival("candidates per split");
-    p.seed = opts.skey("seed thresholds") == 1;
+    const int seed = opts.skey("seed thresholds");
+    p.seed = seed == 1 || (seed == 2 && p.prec != KP_F32 && p.prec != KP_F64);
     return p;
 }
 

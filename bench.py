@@ -108,7 +108,7 @@ def cpu_baseline(Xt, Xq, k, ei, args):
             "vector_bits": cb.vector_bits(), "index_match_oracle": bool((ind == ei).all())}
 
 
-def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1, seed=True):
+def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1, seed=None):
     prec, kc_min = split_form(precision)
     nn = aoclda.neighbors.nearest_neighbors(n_neighbors=k, precision=prec, tile_variant=variant,
                                             candidates_per_split=kc_min,
@@ -143,8 +143,8 @@ def run_form(L, aoclda, Xt, dq, nq, d, k, precision, steps, warmup, variant=-1, 
         step()
     L.check(L.lib.da_amd_synchronize(h.h))
     kern = {}
-    for name in ("knn_seed", "knn_tile", "knn_merge", "knn_retry_tile", "knn_retry_merge",
-                 "pairwise_direct", "row_topk"):
+    for name in ("knn_seed", "knn_query_centre", "knn_tile", "knn_merge", "knn_retry_tile",
+                 "knn_retry_merge", "pairwise_direct", "row_topk"):
         ms, cnt = C.c_double(), L.da_int()
         L.check(L.lib.da_amd_profile_get(h.h, name.encode(), C.byref(ms), C.byref(cnt)))
         if cnt.value:
@@ -714,8 +714,9 @@ def main():
                     help="cross-term forms of the batched k-means mode")
     ap.add_argument("--tile-variant", type=int, default=-1,
                     help="fused tile kernel variant (-1: tuning table)")
-    ap.add_argument("--seed", action="store_true",
-                    help="seed the tile thresholds from a sweep of a row subset")
+    ap.add_argument("--seed", choices=["auto", "yes", "no"], default="auto",
+                    help="seed the tile thresholds from group minima over a row sample "
+                         "(auto: the library's choice, which seeds the 16-bit forms)")
     ap.add_argument("--cpu-sample-all", type=int, default=4000,
                     help="queries per timed run of the CPU baseline on all threads")
     ap.add_argument("--cpu-sample-one", type=int, default=300,
@@ -808,7 +809,8 @@ def main():
     results = {}
     for form in args.forms.split(","):
         results[form] = run_form(L, aoclda, Xt, dq, nq, d, k, form, args.steps, args.warmup,
-                                 args.tile_variant, args.seed)
+                                 args.tile_variant,
+                                 None if args.seed == "auto" else args.seed == "yes")
     L.lib.da_amd_free(dq)
 
     cpu = None
