@@ -177,8 +177,8 @@ hipError_t launch_pairwise_mfma(int metric, const T *X, const T *Y, int m, int n
     return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_inner_mfma(const T *X, const double *Y, int m, int n, int k, float *D,
+template <typename TX, typename TY>
+hipError_t launch_inner_mfma(const TX *X, const TY *Y, int m, int n, int k, float *D,
                              int64_t ldd, hipStream_t s) {
     if (m <= 0 || n <= 0)
         return hipSuccess;
@@ -187,7 +187,7 @@ hipError_t launch_inner_mfma(const T *X, const double *Y, int m, int n, int k, f
     for (int i0 = 0; i0 < m; i0 += chunk) {
         const int mm = m - i0 < chunk ? m - i0 : chunk;
         dim3 grid((n + TB - 1) / TB, (mm + TB - 1) / TB);
-        hipLaunchKernelGGL((pairwise_mfma_kernel<T, double, float, KM_INNER>), grid, dim3(256), 0,
+        hipLaunchKernelGGL((pairwise_mfma_kernel<TX, TY, float, KM_INNER>), grid, dim3(256), 0,
                            s, X + (int64_t)i0 * k, Y, mm, n, k, D + (int64_t)i0 * ldd, ldd,
                            nullptr, nullptr, 0, 0, 0.0, 0.0);
         const hipError_t e = hipGetLastError();
@@ -196,10 +196,10 @@ hipError_t launch_inner_mfma(const T *X, const double *Y, int m, int n, int k, f
     }
     return hipSuccess;
 }
-template hipError_t launch_inner_mfma<float>(const float *, const double *, int, int, int,
-                                             float *, int64_t, hipStream_t);
-template hipError_t launch_inner_mfma<double>(const double *, const double *, int, int, int,
-                                              float *, int64_t, hipStream_t);
+template hipError_t launch_inner_mfma<double, float>(const double *, const float *, int, int,
+                                                     int, float *, int64_t, hipStream_t);
+template hipError_t launch_inner_mfma<double, double>(const double *, const double *, int, int,
+                                                      int, float *, int64_t, hipStream_t);
 
 template hipError_t launch_pairwise_mfma<float, float>(int, const float *, const float *, int,
                                                        int, int, float *, int64_t,

@@ -568,7 +568,7 @@ __device__ __forceinline__ void knn_tile_body(
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         const int q = q0 + qbase + rt * 16 + 4 * g + i;
-                        dst[rt][i] = q < nq ? cen.qc[(int64_t)q * cen.G + grp] : (ae)0;
+                        dst[rt][i] = q < nq ? cen.qc[(int64_t)grp * nq + q] : (ae)0;
                     }
             };
             if constexpr (CEN) {
@@ -1338,7 +1338,7 @@ hipError_t launch_knn_query_centre(const T *Q, int nq, int d, const double *C, i
                                    hipStream_t s) {
     if (nq <= 0 || G <= 0)
         return hipSuccess;
-    return launch_inner_mfma<T>(Q, C, nq, G, d, qc, G, s);
+    return launch_inner_mfma<double, T>(C, Q, G, nq, d, qc, nq, s);
 }
 template hipError_t launch_knn_query_centre<float>(const float *, int, int, const double *, int,
                                                    float *, hipStream_t);

@@ -58,10 +58,10 @@ hipError_t launch_pairwise_direct(int metric, const T *X, const T *Y, int m, int
 // rounding bound could exceed 5e-13 (double output) or 5e-7 (float output) of the value are
 // recomputed with direct differences. Same layout and diagonal rule as launch_pairwise_direct.
 bool pairwise_mfma_metric(int metric);
-// D[i*ldd + j] = (float) X_i . Y_j for X (m x k, type T) and Y (n x k, double), row-major
+// D[i*ldd + j] = (float) X_i . Y_j for X (m x k, type TX) and Y (n x k, type TY), row-major
 // compact: the same tile with the products and sums in double, rounded once to float.
-template <typename T>
-hipError_t launch_inner_mfma(const T *X, const double *Y, int m, int n, int k, float *D,
+template <typename TX, typename TY>
+hipError_t launch_inner_mfma(const TX *X, const TY *Y, int m, int n, int k, float *D,
                              int64_t ldd, hipStream_t s);
 template <typename T, typename U>
 hipError_t launch_pairwise_mfma(int metric, const T *X, const T *Y, int m, int n, int k, U *D,
@@ -316,7 +316,7 @@ inline int knn_prec_row_factor(int prec) {
 }
 // Centred training rows of the KP_F16C form. The rows are stored in group order, each group
 // padded to a whole number of 32-row steps (padded rows have -|y|^2/2 = -inf); step_group[t]
-// is the group of rows 32t .. 32t+31, qc[q*G + g] the product of query q with the centre of
+// is the group of rows 32t .. 32t+31, qc[g*nq + q] the product of query q with the centre of
 // group g (float), perm[p] the original index of stored row p.
 struct KnnCentring {
     const int *step_group = nullptr;
@@ -349,7 +349,8 @@ template <typename T>
 hipError_t launch_knn_centre_rows(const T *X, int d, const int *perm, const int *step_group,
                                   const double *C, int rows_pad, int Kp, void *dst,
                                   double *rnorm2, hipStream_t s);
-// qc[q*G + g] = (float) sum_l Q[q*d + l] C[g*d + l], accumulated in double.
+// qc[g*nq + q] = (float) sum_l Q[q*d + l] C[g*d + l], accumulated in double (group-major, so
+// that the queries of one sweep read one cache line per group).
 template <typename T>
 hipError_t launch_knn_query_centre(const T *Q, int nq, int d, const double *C, int G, float *qc,
                                    hipStream_t s);
