@@ -85,7 +85,9 @@ da_status da_amd_pairwise_profile_get(const char *kernel, double *total_ms, da_i
 
 /*
  * Diagnostics of the last k-NN query of the handle: number of queries whose candidate
- * set could not be certified exact and were recomputed by the exact fallback pass.
+ * set from the first pass of the fused tile could not be certified exact (they are searched
+ * again: a second seeded pass, then the exact fallback for any still uncertified), and the
+ * candidates kept per query by that first pass.
  */
 da_status da_nn_last_query_stats(da_handle handle, da_int *n_fallback, da_int *n_candidates);
 
