@@ -85,6 +85,14 @@ def test_options_registry():
     n = L.da_int(3)
     assert lib.da_options_get_string(h.h, b"metric", buf, C.byref(n)) == st("invalid_input")
     assert n.value == 7
+    # the MI355X threshold-seed option: "auto" by default (the 16-bit forms are seeded)
+    n = L.da_int(64)
+    assert lib.da_options_get_string(h.h, b"seed thresholds", buf, C.byref(n)) == 0
+    assert buf.value == b"auto"
+    for v in (b"yes", b"no", b"auto"):
+        assert lib.da_options_set_string(h.h, b"seed thresholds", v) == 0
+    assert lib.da_options_set_string(h.h, b"seed thresholds", b"sometimes") == \
+        st("option_invalid_value")
     # real options are typed by the handle precision
     assert lib.da_options_set_real_s(h.h, b"minkowski parameter", 3.0) == st("wrong_type")
     assert lib.da_options_set_real_d(h.h, b"minkowski parameter", 3.0) == 0
